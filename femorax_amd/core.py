@@ -1,0 +1,375 @@
+"""Python host mirror of the C ABI in include/femorax_amd.h (ctypes over libfemorax_amd.so).
+
+The names follow the reference's exec/solvers interfaces: Mesh (compile::Compiled), Batch
+(InternalForce::Batch / InternalForce), apply_operator / operator_diagonal, cg
+(ConjugateGradient + Jacobi). There is no fallback: if the HIP library is not built, importing
+the bindings raises, and every compute call goes to the device.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libfemorax_amd.so")
+_LIB = None
+
+FX_OK = 0
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP"}
+FX_BATCH_FP32 = 1
+
+
+class FxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("nnode", C.c_int32), ("xyz", C.c_void_p), ("nelem", C.c_int32), ("conn", C.c_void_p),
+                ("group", C.c_void_p), ("region", C.c_void_p), ("nfix", C.c_int32), ("fix_node", C.c_void_p),
+                ("fix_dof", C.c_void_p), ("ntie", C.c_int32), ("tie_retained", C.c_void_p),
+                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p)]
+
+
+class MeshView(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nnode", "nelem", "padded", "neq", "nparam", "ncolor")] + [
+        (n, C.c_void_p) for n in ("node_order", "node_rank", "elem_order", "node_eq", "coords", "node", "eq",
+                                  "param", "param_group", "param_region", "color_start", "color_elements")]
+
+
+class Rayleigh(C.Structure):
+    _fields_ = [("alphaM", C.c_double), ("betaK", C.c_double), ("betaKinit", C.c_double), ("betaKcomm", C.c_double)]
+
+
+class ElasticParams(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double), ("b", C.c_double * 3),
+                ("mass_type", C.c_int32), ("damp", Rayleigh)]
+
+
+class Coeffs(C.Structure):
+    _fields_ = [("stiffness", C.c_double), ("damping", C.c_double), ("mass", C.c_double), ("initial", C.c_int32)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("count", C.c_int32), ("padded", C.c_int32), ("node", C.c_void_p), ("eq", C.c_void_p),
+                ("param", C.c_void_p), ("nparam", C.c_int32), ("params", C.c_void_p), ("nnode", C.c_int32),
+                ("coords", C.c_void_p), ("neq", C.c_int32)]
+
+
+class BatchInfo(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("count", "neq", "nnode", "nblocks", "ncolors", "tied", "affine_classes",
+                                         "shape_classes")] + [("device_bytes", C.c_uint64)]
+
+
+class SolveOptions(C.Structure):
+    _fields_ = [("relative_tolerance", C.c_double), ("absolute_tolerance", C.c_double),
+                ("max_iterations", C.c_int32)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
+                ("outer_iterations", C.c_int32), ("rhs_norm", C.c_double), ("initial_residual", C.c_double),
+                ("final_residual", C.c_double), ("true_residual", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# (name, restype, argtypes) of every entry point in include/femorax_amd.h
+vp, i32, i64, u32, u64, dbl = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_double
+SIGNATURES = [
+    ("fx_ctx_create", C.c_int, [C.c_int, u64, C.POINTER(vp)]),
+    ("fx_ctx_destroy", None, [vp]),
+    ("fx_last_error", C.c_char_p, [vp]),
+    ("fx_sync", C.c_int, [vp]),
+    ("fx_device_memory", C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
+    ("fx_budget_used", u64, [vp]),
+    ("fx_alloc", C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+    ("fx_free", C.c_int, [vp, vp]),
+    ("fx_upload", C.c_int, [vp, vp, vp, C.c_size_t]),
+    ("fx_download", C.c_int, [vp, vp, vp, C.c_size_t]),
+    ("fx_memset0", C.c_int, [vp, vp, C.c_size_t]),
+    ("fx_compile_model", C.c_int, [C.POINTER(ModelDesc), C.c_int, C.c_int, C.c_int, C.POINTER(vp)]),
+    ("fx_mesh_get", C.c_int, [vp, C.POINTER(MeshView)]),
+    ("fx_mesh_destroy", None, [vp]),
+    ("fx_batch_create", C.c_int, [vp, C.POINTER(BatchDesc), u32, C.POINTER(vp)]),
+    ("fx_batch_destroy", None, [vp]),
+    ("fx_batch_info_get", C.c_int, [vp, C.POINTER(BatchInfo)]),
+    ("fx_operator_apply", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
+    ("fx_operator_apply_f32", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
+    ("fx_operator_diagonal", C.c_int, [vp, C.POINTER(Coeffs), vp]),
+    ("fx_operator_time", C.c_int, [vp, C.POINTER(Coeffs), C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
+    ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
+    ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
+    ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
+]
+
+
+def lib():
+    """Load libfemorax_amd.so (raises if it is not built: there is no CPU fallback)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def _check(rc, ctx=None):
+    if rc != FX_OK:
+        msg = lib().fx_last_error(ctx.h if ctx is not None else None)
+        raise FxError(rc, msg.decode() if msg else "")
+
+
+class Context:
+    """One device, one stream, an HBM budget (bytes; 0 = 90% of free memory)."""
+
+    def __init__(self, device=0, budget=0):
+        h = C.c_void_p()
+        _check(lib().fx_ctx_create(device, budget, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        _check(lib().fx_sync(self.h), self)
+
+    def memory(self):
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(lib().fx_device_memory(self.h, C.byref(f), C.byref(t)), self)
+        return f.value, t.value
+
+
+class DeviceArray:
+    """A device buffer owned by Python (tests and the benchmark)."""
+
+    def __init__(self, ctx, n, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        p = C.c_void_p()
+        _check(lib().fx_alloc(ctx.h, max(1, self.n) * self.dtype.itemsize, C.byref(p)), ctx)
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype or a.dtype)
+        d = cls(ctx, a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(lib().fx_upload(self.ctx.h, self.ptr, a.ctypes.data, a.nbytes), self.ctx)
+
+    def get(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().fx_download(self.ctx.h, out.ctypes.data, self.ptr, out.nbytes), self.ctx)
+        return out
+
+    def zero(self):
+        _check(lib().fx_memset0(self.ctx.h, self.ptr, self.n * self.dtype.itemsize), self.ctx)
+
+    def free(self):
+        if self.ptr:
+            lib().fx_free(self.ctx.h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _arr(keep, a, dt):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dt)
+    keep.append(a)
+    return a.ctypes.data
+
+
+def model_desc(model, keep):
+    m = ModelDesc()
+    m.nnode = int(model["nnode"])
+    m.xyz = _arr(keep, model["xyz"], np.float64)
+    m.nelem = int(model["nelem"])
+    m.conn = _arr(keep, model["conn"], np.int32)
+    m.group = _arr(keep, model.get("group"), np.int32)
+    m.region = _arr(keep, model.get("region"), np.int32)
+    fix = np.asarray(model.get("fix", np.zeros((0, 2))), np.int32).reshape(-1, 2)
+    tie = np.asarray(model.get("tie", np.zeros((0, 3))), np.int32).reshape(-1, 3)
+    m.nfix = len(fix)
+    m.fix_node = _arr(keep, fix[:, 0].copy() if len(fix) else np.zeros(1), np.int32)
+    m.fix_dof = _arr(keep, fix[:, 1].copy() if len(fix) else np.zeros(1), np.int32)
+    m.ntie = len(tie)
+    m.tie_retained = _arr(keep, tie[:, 0].copy() if len(tie) else np.zeros(1), np.int32)
+    m.tie_constrained = _arr(keep, tie[:, 1].copy() if len(tie) else np.zeros(1), np.int32)
+    m.tie_dof = _arr(keep, tie[:, 2].copy() if len(tie) else np.zeros(1), np.int32)
+    return m
+
+
+class Mesh:
+    """fx_compile_model: the compile step's output as numpy arrays (host only, no GPU needed)."""
+
+    def __init__(self, model, renumber=True, simd=64, coloring=True):
+        keep = []
+        desc = model_desc(model, keep)
+        h = C.c_void_p()
+        _check(lib().fx_compile_model(C.byref(desc), int(renumber), int(simd), int(coloring), C.byref(h)))
+        v = MeshView()
+        _check(lib().fx_mesh_get(h, C.byref(v)))
+        N, P, E = v.nnode, v.padded, v.nelem
+
+        def get(p, n, dt):
+            if not p or n == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(np.ctypeslib.as_ctypes_type(dt))), shape=(n,)).copy()
+
+        self.nnode, self.nelem, self.padded, self.neq, self.nparam, self.ncolor = N, E, P, v.neq, v.nparam, v.ncolor
+        self.node_order = get(v.node_order, N, np.int32)
+        self.node_rank = get(v.node_rank, N, np.int32)
+        self.elem_order = get(v.elem_order, E, np.int32)
+        self.node_eq = get(v.node_eq, 3 * N, np.int32)
+        self.coords = get(v.coords, 3 * N, np.float64)
+        self.node = get(v.node, 8 * P, np.int32)
+        self.eq = get(v.eq, 24 * P, np.int32)
+        self.param = get(v.param, P, np.int32)
+        self.param_group = get(v.param_group, v.nparam, np.int32)
+        self.param_region = get(v.param_region, v.nparam, np.int32)
+        self.color_start = get(v.color_start, v.ncolor + 1, np.int32) if v.color_start else None
+        self.color_elements = get(v.color_elements, E, np.int32) if v.color_elements else None
+        lib().fx_mesh_destroy(h)
+
+    def params_for(self, model):
+        """Per param set (group, region) -> elastic parameters from model['params'] by group and
+        model.get('rayleigh') by region."""
+        regions = model.get("rayleigh", [])
+        out = []
+        for g, r in zip(self.param_group, self.param_region):
+            p = dict(model["params"][g])
+            if r >= 0:
+                p["rayleigh"] = tuple(regions[r])
+            out.append(p)
+        return out
+
+    def to_equations(self, nodal):
+        """Nodal values [nnode, 3] in model order -> equation space (fixed drop, tied add)."""
+        b = np.zeros(self.neq)
+        eqs = self.node_eq.reshape(-1, 3)[self.node_rank]
+        for d in range(3):
+            q = eqs[:, d]
+            m = q >= 0
+            np.add.at(b, q[m], nodal[m, d])
+        return b
+
+    def to_nodes(self, x):
+        """Equation values -> [nnode, 3] in model order (fixed DOFs 0)."""
+        eqs = self.node_eq.reshape(-1, 3)[self.node_rank]
+        out = np.zeros((self.nnode, 3))
+        m = eqs >= 0
+        out[m] = x[eqs[m]]
+        return out
+
+
+def elastic_array(params):
+    arr = (ElasticParams * max(1, len(params)))()
+    for i, p in enumerate(params):
+        b = p.get("b", (0.0, 0.0, 0.0))
+        r = p.get("rayleigh", (0.0, 0.0, 0.0, 0.0))
+        arr[i] = ElasticParams(p["E"], p["nu"], p.get("rho", 0.0), (C.c_double * 3)(*b), int(p.get("mass_type", 0)),
+                               Rayleigh(*r))
+    return arr
+
+
+def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
+    return Coeffs(stiffness, damping, mass, int(initial))
+
+
+class Batch:
+    """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
+
+    def __init__(self, ctx, mesh, params, fp32=False):
+        self.ctx = ctx
+        self.neq = mesh.neq
+        keep = []
+        self._params = elastic_array(params)
+        d = BatchDesc()
+        d.count = mesh.nelem
+        d.padded = mesh.padded
+        d.node = _arr(keep, mesh.node, np.int32)
+        d.eq = _arr(keep, mesh.eq, np.int32)
+        d.param = _arr(keep, mesh.param, np.int32)
+        d.nparam = len(params)
+        d.params = C.cast(self._params, C.c_void_p)
+        d.nnode = mesh.nnode
+        d.coords = _arr(keep, mesh.coords, np.float64)
+        d.neq = mesh.neq
+        h = C.c_void_p()
+        _check(lib().fx_batch_create(ctx.h, C.byref(d), FX_BATCH_FP32 if fp32 else 0, C.byref(h)), ctx)
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().fx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self):
+        i = BatchInfo()
+        _check(lib().fx_batch_info_get(self.h, C.byref(i)), self.ctx)
+        return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def apply(self, x, y, k=None):
+        _check(lib().fx_operator_apply(self.h, C.byref(k or coeffs()), x.ptr, y.ptr), self.ctx)
+
+    def apply_f32(self, x, y, k=None):
+        _check(lib().fx_operator_apply_f32(self.h, C.byref(k or coeffs()), x.ptr, y.ptr), self.ctx)
+
+    def diagonal(self, d, k=None):
+        _check(lib().fx_operator_diagonal(self.h, C.byref(k or coeffs()), d.ptr), self.ctx)
+
+    def time_apply(self, x, y, reps, precision=64, k=None):
+        ms = C.c_float()
+        _check(lib().fx_operator_time(self.h, C.byref(k or coeffs()), precision, x.ptr, y.ptr, reps, C.byref(ms)),
+               self.ctx)
+        return ms.value
+
+    def cg(self, b, x, rtol=1e-8, atol=0.0, max_it=10000, k=None):
+        o = SolveOptions(rtol, atol, max_it)
+        r = SolveResult()
+        _check(lib().fx_cg(self.h, C.byref(k or coeffs()), C.byref(o), b.ptr, x.ptr, C.byref(r)), self.ctx)
+        return r.as_dict()
+
+    def cg_mixed(self, b, x, rtol=1e-8, atol=0.0, max_it=100000, inner_rtol=1e-3, k=None):
+        o = SolveOptions(rtol, atol, max_it)
+        r = SolveResult()
+        _check(lib().fx_cg_mixed(self.h, C.byref(k or coeffs()), C.byref(o), inner_rtol, b.ptr, x.ptr, C.byref(r)),
+               self.ctx)
+        return r.as_dict()
+
+
+def dot(ctx, x, y):
+    out = C.c_double()
+    _check(lib().fx_dot(ctx.h, x.n, x.ptr, y.ptr, C.byref(out)), ctx)
+    return out.value

@@ -1,0 +1,379 @@
+// The extern "C" entry points of include/femorax_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+
+struct fx_mesh {
+  fx::Mesh m;
+};
+
+namespace {
+
+thread_local std::string g_error;  // errors before a context exists
+
+int fail(fx_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  else g_error = msg;
+  return code;
+}
+
+template <typename T>
+int dev_upload(fx_ctx* ctx, T*& dst, const T* src, size_t n, uint64_t& bytes) {
+  dst = nullptr;
+  if (n == 0) return FX_OK;
+  const size_t sz = n * sizeof(T);
+  if (ctx->used + sz > ctx->budget) return fail(ctx, FX_ENOMEM, "HBM budget exceeded");
+  FX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&dst), sz));
+  ctx->used += sz;
+  bytes += sz;
+  if (src) FX_HIP(ctx, hipMemcpy(dst, src, sz, hipMemcpyHostToDevice));
+  return FX_OK;
+}
+
+template <typename T>
+int dev_zeros(fx_ctx* ctx, T*& dst, size_t n, uint64_t& bytes) {
+  if (int rc = dev_upload<T>(ctx, dst, nullptr, n, bytes)) return rc;
+  if (n) FX_HIP(ctx, hipMemset(dst, 0, n * sizeof(T)));
+  return FX_OK;
+}
+
+void dev_free(fx_ctx* ctx, void* p, uint64_t& bytes, size_t sz) {
+  if (!p) return;
+  (void)hipFree(p);
+  ctx->used -= std::min<uint64_t>(ctx->used, sz);
+  (void)bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out) {
+  if (!out) return fail(nullptr, FX_EINVAL, "fx_ctx_create: null out");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, FX_EDEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(nullptr, FX_EINVAL, "fx_ctx_create: no such device");
+  if (hipSetDevice(device) != hipSuccess) return fail(nullptr, FX_EDEVICE, "hipSetDevice failed");
+  auto* c = new (std::nothrow) fx_ctx();
+  if (!c) return fail(nullptr, FX_ENOMEM, "host allocation failed");
+  c->device = device;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return fail(nullptr, FX_EDEVICE, "hipStreamCreate failed");
+  }
+  size_t fr = 0, tot = 0;
+  (void)hipMemGetInfo(&fr, &tot);
+  c->budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<uint64_t>(fr / 10 * 9);
+  if (hipMalloc(&c->scalars, 4096) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(nullptr, FX_EDEVICE, "hipMalloc failed");
+  }
+  *out = c;
+  return FX_OK;
+}
+
+void fx_ctx_destroy(fx_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->red) (void)hipFree(ctx->red);
+  if (ctx->scalars) (void)hipFree(ctx->scalars);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char* fx_last_error(const fx_ctx* ctx) { return ctx ? ctx->error.c_str() : g_error.c_str(); }
+
+int fx_sync(fx_ctx* ctx) {
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+int fx_device_memory(fx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  size_t f = 0, t = 0;
+  FX_HIP(ctx, hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return FX_OK;
+}
+
+uint64_t fx_budget_used(const fx_ctx* ctx) { return ctx ? ctx->used : 0; }
+
+int fx_alloc(fx_ctx* ctx, size_t bytes, void** dptr) {
+  if (!ctx || !dptr) return fail(ctx, FX_EINVAL, "fx_alloc: null argument");
+  *dptr = nullptr;
+  if (bytes == 0) bytes = 8;
+  FX_HIP(ctx, hipMalloc(dptr, bytes));
+  return FX_OK;
+}
+int fx_free(fx_ctx* ctx, void* dptr) {
+  if (dptr) FX_HIP(ctx, hipFree(dptr));
+  return FX_OK;
+}
+int fx_upload(fx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes) FX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+int fx_download(fx_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (bytes) FX_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+int fx_memset0(fx_ctx* ctx, void* dst, size_t bytes) {
+  if (bytes) FX_HIP(ctx, hipMemsetAsync(dst, 0, bytes, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+// ---- compile step -------------------------------------------------------------------------
+int fx_compile_model(const fx_model_desc* model, int renumber_rcm, int simd_width, int build_coloring,
+                     fx_mesh** out) {
+  if (!model || !out) return fail(nullptr, FX_EINVAL, "fx_compile_model: null argument");
+  *out = nullptr;
+  if (model->nnode < 0 || model->nelem < 0 || (model->nelem > 0 && (!model->conn || !model->xyz)))
+    return fail(nullptr, FX_EINVAL, "fx_compile_model: bad sizes");
+  auto* m = new (std::nothrow) fx_mesh();
+  if (!m) return fail(nullptr, FX_ENOMEM, "host allocation failed");
+  std::string err;
+  const int rc = fx::compile_model(*model, renumber_rcm, simd_width, build_coloring, m->m, err);
+  if (rc != FX_OK) {
+    delete m;
+    return fail(nullptr, rc, err);
+  }
+  *out = m;
+  return FX_OK;
+}
+
+int fx_mesh_get(const fx_mesh* mesh, fx_mesh_view* v) {
+  if (!mesh || !v) return fail(nullptr, FX_EINVAL, "fx_mesh_get: null argument");
+  const fx::Mesh& m = mesh->m;
+  v->nnode = m.nnode;
+  v->nelem = m.nelem;
+  v->padded = m.padded;
+  v->neq = m.neq;
+  v->nparam = m.nparam;
+  v->ncolor = m.ncolor;
+  v->node_order = m.node_order.data();
+  v->node_rank = m.node_rank.data();
+  v->elem_order = m.elem_order.data();
+  v->node_eq = m.node_eq.data();
+  v->coords = m.coords.data();
+  v->node = m.node.data();
+  v->eq = m.eq.data();
+  v->param = m.param.data();
+  v->param_group = m.param_group.data();
+  v->param_region = m.param_region.data();
+  v->color_start = m.color_start.empty() ? nullptr : m.color_start.data();
+  v->color_elements = m.color_elements.empty() ? nullptr : m.color_elements.data();
+  return FX_OK;
+}
+
+void fx_mesh_destroy(fx_mesh* mesh) { delete mesh; }
+
+// ---- batch ------------------------------------------------------------------------------------
+void fx_batch_destroy(fx_batch* b) {
+  if (!b) return;
+  fx_ctx* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
+                  b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
+                  b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
+  delete b;
+}
+
+int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batch** out) {
+  if (!ctx || !d || !out) return fail(ctx, FX_EINVAL, "fx_batch_create: null argument");
+  *out = nullptr;
+  if (d->count < 0 || d->padded < d->count || d->nparam <= 0 || d->nnode <= 0 || d->neq < 0 || !d->node ||
+      !d->param || !d->params || !d->coords || (!d->eq))
+    return fail(ctx, FX_EINVAL, "fx_batch_create: bad descriptor");
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  const int32_t P = d->padded, E = d->count, N = d->nnode;
+  // validate maps and derive the node -> equation map from Batch::eq (3 DOFs per node)
+  std::vector<int32_t> node_eq(static_cast<size_t>(3) * N, -1);
+  std::vector<char> seen(static_cast<size_t>(3) * N, 0);
+  for (int32_t e = 0; e < E; ++e) {
+    const int32_t pi = d->param[e];
+    if (pi < 0 || pi >= d->nparam) return fail(ctx, FX_EINVAL, "fx_batch_create: param index out of range");
+    for (int a = 0; a < 8; ++a) {
+      const int32_t n = d->node[static_cast<size_t>(a) * P + e];
+      if (n < 0 || n >= N) return fail(ctx, FX_EINVAL, "fx_batch_create: node index out of range");
+      for (int k = 0; k < 3; ++k) {
+        const int32_t q = d->eq[static_cast<size_t>(3 * a + k) * P + e];
+        if (q < -1 || q >= d->neq) return fail(ctx, FX_EINVAL, "fx_batch_create: equation out of range");
+        const size_t at = static_cast<size_t>(3) * n + k;
+        if (seen[at] && node_eq[at] != q)
+          return fail(ctx, FX_EINVAL, "fx_batch_create: a node DOF maps to two equations");
+        seen[at] = 1;
+        node_eq[at] = q;
+      }
+    }
+  }
+  for (int32_t i = 0; i < d->nparam; ++i) {
+    const auto& p = d->params[i];
+    if (!(p.E > 0) || !(p.nu > -1.0 && p.nu < 0.5) || p.rho < 0 || (p.mass_type != 0 && p.mass_type != 1))
+      return fail(ctx, FX_EINVAL, "fx_batch_create: invalid elastic parameters in set " + std::to_string(i));
+  }
+  auto* b = new (std::nothrow) fx_batch();
+  if (!b) return fail(ctx, FX_ENOMEM, "host allocation failed");
+  b->ctx = ctx;
+  b->count = E;
+  b->padded = P;
+  b->nnode = N;
+  b->neq = d->neq;
+  b->nparam = d->nparam;
+  b->has_f32 = (flags & FX_BATCH_FP32) != 0;
+  for (int32_t i = 0; i < d->nparam; ++i) {
+    const auto& p = d->params[i];
+    fx::DevParam q{};
+    // ElasticIsotropic (elastic_isotropic.hpp:45-48): mu2 = E/(1+nu), lam = nu mu2/(1-2nu), mu = mu2/2
+    const double mu2 = p.E / (1.0 + p.nu);
+    q.lam = p.nu * mu2 / (1.0 - 2.0 * p.nu);
+    q.mu = 0.5 * mu2;
+    q.rho = p.rho;
+    for (int k = 0; k < 3; ++k) q.b[k] = p.b[k];
+    q.mass_type = p.mass_type;
+    q.alphaM = p.damp.alphaM;
+    q.betaK = p.damp.betaK;
+    q.betaKinit = p.damp.betaKinit;
+    q.betaKcomm = p.damp.betaKcomm;
+    b->host_params.push_back(q);
+  }
+  fx::Blocks blk;
+  fx::build_blocks(E, P, d->node, N, d->coords, blk);
+  fx::Contraction con;
+  fx::build_contraction(N, d->neq, node_eq.data(), con);
+  b->tied = con.tied;
+  uint64_t& by = b->device_bytes;
+  int rc = FX_OK;
+  auto ok = [&](int r) {
+    if (r != FX_OK && rc == FX_OK) rc = r;
+    return rc == FX_OK;
+  };
+  std::vector<float> c32(d->coords, d->coords + static_cast<size_t>(3) * N);
+  ok(dev_upload(ctx, b->node, d->node, static_cast<size_t>(8) * P, by)) &&
+      ok(dev_upload(ctx, b->param, d->param, static_cast<size_t>(P), by)) &&
+      ok(dev_upload(ctx, b->node_eq, node_eq.data(), node_eq.size(), by)) &&
+      ok(dev_upload(ctx, b->coords, d->coords, static_cast<size_t>(3) * N, by)) &&
+      ok(b->has_f32 ? dev_upload(ctx, b->coords32, c32.data(), c32.size(), by) : FX_OK) &&
+      ok(dev_upload(ctx, b->params, b->host_params.data(), b->host_params.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.elems, blk.elems.data(), blk.elems.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.ln_start, blk.ln_start.data(), blk.ln_start.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.lnode, blk.lnode.data(), blk.lnode.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.corner, blk.corner.data(), blk.corner.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.ln_slots, blk.ln_slots.data(), blk.ln_slots.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.slot_list, blk.slot_list.data(), blk.slot_list.size(), by)) &&
+      ok(dev_upload(ctx, b->blk.colour_blocks, blk.colour_blocks.data(), blk.colour_blocks.size(), by)) &&
+      ok(dev_zeros(ctx, b->work, static_cast<size_t>(6) * std::max(1, d->neq), by)) &&
+      ok(b->has_f32 ? dev_zeros(ctx, b->work32, static_cast<size_t>(7) * std::max(1, d->neq), by) : FX_OK);
+  if (rc == FX_OK && b->tied) {
+    ok(dev_zeros(ctx, b->ynode, static_cast<size_t>(3) * N, by)) &&
+        ok(b->has_f32 ? dev_zeros(ctx, b->ynode32, static_cast<size_t>(3) * N, by) : FX_OK) &&
+        ok(dev_upload(ctx, b->con_start, con.start.data(), con.start.size(), by)) &&
+        ok(dev_upload(ctx, b->con_list, con.list.data(), con.list.size(), by));
+  }
+  if (rc != FX_OK) {
+    fx_batch_destroy(b);
+    return rc;
+  }
+  b->blk.nblocks = blk.nblocks;
+  b->blk.ncolors = blk.ncolors;
+  b->blk.colour_start = blk.colour_start;
+  b->info.count = E;
+  b->info.neq = d->neq;
+  b->info.nnode = N;
+  b->info.nblocks = blk.nblocks;
+  b->info.ncolors = blk.ncolors;
+  b->info.tied = b->tied ? 1 : 0;
+  b->info.device_bytes = by;
+  *out = b;
+  return FX_OK;
+}
+
+int fx_batch_info_get(const fx_batch* b, fx_batch_info* info) {
+  if (!b || !info) return fail(nullptr, FX_EINVAL, "fx_batch_info_get: null argument");
+  *info = b->info;
+  info->device_bytes = b->device_bytes;
+  return FX_OK;
+}
+
+int fx_operator_apply(fx_batch* b, const fx_coeffs* k, const double* x, double* y) {
+  if (!b || !k || !x || !y) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_apply: null argument");
+  if (x == y) return fail(b->ctx, FX_EINVAL, "fx_operator_apply: x and y alias");
+  if (int rc = fx::operator_apply64(b, *k, x, y)) return rc;
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float* y) {
+  if (!b || !k || !x || !y) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_apply_f32: null argument");
+  if (!b->has_f32) return fail(b->ctx, FX_EUNSUP, "fx_operator_apply_f32: batch created without FX_BATCH_FP32");
+  if (static_cast<const void*>(x) == static_cast<const void*>(y))
+    return fail(b->ctx, FX_EINVAL, "fx_operator_apply_f32: x and y alias");
+  if (int rc = fx::operator_apply32(b, *k, x, y)) return rc;
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_operator_diagonal(fx_batch* b, const fx_coeffs* k, double* d) {
+  if (!b || !k || !d) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_diagonal: null argument");
+  if (int rc = fx::operator_diagonal64(b, *k, d)) return rc;
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void* x, void* y, int reps, float* ms) {
+  if (!b || !k || !x || !y || !ms || reps <= 0) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_time: bad argument");
+  if (precision == 32 && !b->has_f32) return fail(b->ctx, FX_EUNSUP, "fx_operator_time: no FP32 operator");
+  fx_ctx* ctx = b->ctx;
+  hipEvent_t e0, e1;
+  FX_HIP(ctx, hipEventCreate(&e0));
+  FX_HIP(ctx, hipEventCreate(&e1));
+  FX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  for (int i = 0; i < reps; ++i) {
+    const int rc = precision == 32
+                       ? fx::operator_apply32(b, *k, static_cast<const float*>(x), static_cast<float*>(y))
+                       : fx::operator_apply64(b, *k, static_cast<const double*>(x), static_cast<double*>(y));
+    if (rc) return rc;
+  }
+  FX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+  FX_HIP(ctx, hipEventSynchronize(e1));
+  FX_HIP(ctx, hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return FX_OK;
+}
+
+int fx_dot(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out) {
+  if (!ctx || !out || (n > 0 && (!x || !y))) return fail(ctx, FX_EINVAL, "fx_dot: null argument");
+  return fx::dot64(ctx, n, x, y, out);
+}
+
+int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+          fx_solve_result* res) {
+  if (!b || !k || !opt || !rhs || !x || !res) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_cg: null argument");
+  return fx::cg64(b, *k, *opt, rhs, x, *res);
+}
+
+int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol, const double* rhs,
+                double* x, fx_solve_result* res) {
+  if (!b || !k || !opt || !rhs || !x || !res)
+    return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_cg_mixed: null argument");
+  if (!(inner_rtol > 0 && inner_rtol < 1)) return fail(b->ctx, FX_EINVAL, "fx_cg_mixed: inner_rtol outside (0, 1)");
+  return fx::cg_mixed(b, *k, *opt, inner_rtol, rhs, x, *res);
+}
+
+}  // extern "C"

@@ -1,0 +1,125 @@
+// Internal types of the femorax_amd library (host side).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/femorax_amd.h"
+
+namespace fx {
+
+// Compile-step output (compile::Compiled for one stdBrick batch).
+struct Mesh {
+  int32_t nnode = 0, nelem = 0, padded = 0, neq = 0, nparam = 0, ncolor = 0;
+  std::vector<int32_t> node_order, node_rank, elem_order, node_eq;
+  std::vector<double> coords;
+  std::vector<int32_t> node, eq, param, param_group, param_region, color_start, color_elements;
+};
+
+int compile_model(const fx_model_desc& m, int renumber_rcm, int simd_width, int build_coloring, Mesh& out,
+                  std::string& err);
+
+// Elements per block: one workgroup of this many threads, one element per thread.
+constexpr int kBlockElems = 256;
+
+// Host view of the block decomposition the kernels walk (built once per batch).
+struct Blocks {
+  int32_t nblocks = 0, ncolors = 0;
+  std::vector<int32_t> elems;        // [nblocks * kBlockElems] batch position or -1
+  std::vector<int32_t> ln_start;     // [nblocks + 1] into lnode
+  std::vector<uint32_t> lnode;       // global node id | first-toucher flag (bit 31)
+  std::vector<uint16_t> corner;      // [nblocks * 8 * kBlockElems] local node of (corner, element)
+  std::vector<uint32_t> ln_slots;    // per local node: start (low 16) | count (high 16)
+  std::vector<uint16_t> slot_list;   // [nblocks * 8 * kBlockElems] elem_local * 8 + corner
+  std::vector<int32_t> colour_start; // [ncolors + 1] into colour_blocks
+  std::vector<int32_t> colour_blocks;
+  int32_t max_ln = 0;
+};
+
+// Partition the batch's elements into compact blocks (recursive coordinate bisection of the
+// element centroids), colour the blocks so that blocks of one colour share no node, and mark
+// each node's first writer in launch order.
+void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nnode, const double* coords,
+                  Blocks& out);
+
+// Contraction of node-space values into equations when equations are shared by several
+// nodes (equalDOF ties): eq -> list of node DOFs (3 n + d), in ascending order.
+struct Contraction {
+  bool tied = false;
+  std::vector<int32_t> start, list;
+};
+void build_contraction(int32_t nnode, int32_t neq, const int32_t* node_eq, Contraction& out);
+
+}  // namespace fx
+
+// Context and batch (opaque at the C ABI).
+struct fx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t budget = 0, used = 0;
+  std::string error;
+  // device scratch for reductions and solver scalars
+  double* red = nullptr;  // partial sums
+  size_t red_cap = 0;
+  void* scalars = nullptr;  // solver state block (device)
+  void* host_pinned = nullptr;
+};
+
+namespace fx {
+struct DevBlocks {
+  int32_t nblocks = 0, ncolors = 0;
+  int32_t* elems = nullptr;
+  int32_t* ln_start = nullptr;
+  uint32_t* lnode = nullptr;
+  uint16_t* corner = nullptr;
+  uint32_t* ln_slots = nullptr;
+  uint16_t* slot_list = nullptr;
+  int32_t* colour_blocks = nullptr;
+  std::vector<int32_t> colour_start;  // host copy for launches
+};
+
+struct DevParam {  // per parameter set, device
+  double lam, mu, rho;
+  double b[3];
+  int32_t mass_type, pad;
+  double alphaM, betaK, betaKinit, betaKcomm;
+};
+}  // namespace fx
+
+struct fx_batch {
+  fx_ctx* ctx = nullptr;
+  int32_t count = 0, padded = 0, nnode = 0, neq = 0, nparam = 0;
+  bool has_f32 = false;
+  // device arrays
+  int32_t* node = nullptr;      // [8 * padded]
+  int32_t* param = nullptr;     // [padded]
+  int32_t* node_eq = nullptr;   // [3 * nnode]
+  double* coords = nullptr;     // [3 * nnode]
+  float* coords32 = nullptr;    // [3 * nnode]
+  fx::DevParam* params = nullptr;
+  fx::DevBlocks blk;
+  // tied meshes: node-space output and its contraction
+  bool tied = false;
+  double* ynode = nullptr;      // [3 * nnode]
+  float* ynode32 = nullptr;
+  int32_t* con_start = nullptr; // [neq + 1]
+  int32_t* con_list = nullptr;
+  // solver work (allocated at create: compute allocates nothing)
+  double* work = nullptr;       // 6 * neq doubles: r z p ap inv d
+  float* work32 = nullptr;      // 5 * neq floats
+  uint64_t device_bytes = 0;
+  std::vector<fx::DevParam> host_params;
+  fx_batch_info info{};
+};
+
+#define FX_HIP(ctx, call)                                                          \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      (ctx)->error = std::string(#call) + ": " + hipGetErrorString(e_);           \
+      return FX_EDEVICE;                                                           \
+    }                                                                              \
+  } while (0)

@@ -1,0 +1,359 @@
+// hex8 (stdBrick, 2x2x2 Gauss) matrix-free operator kernels for gfx950.
+//
+// One workgroup per element block (fx_blocks.cpp), one element per thread:
+//   gather   the element's 8 nodes, their coordinates and x at the node's equations
+//   compute  y_e = sum over the 8 Gauss points of B^T (D_eff B x_e) dvol + cM' (M x_e)
+//            (element_operator_apply, src/physics/operators/element_operator.hpp:82-130, with
+//            stdBrick's shape functions std_brick.hpp:45-152 and the isotropic D of
+//            elastic_isotropic.hpp:40-94)
+//   stage    the 24 values in LDS (the reference's slots, assembly.hpp:28-36, without the HBM
+//            round trip)
+//   reduce   per local node in a fixed (element, corner) order, then one store (first writer
+//            of the node in launch order) or read-add-write per node and equation
+// Colours of blocks are launched in order, so the scatter is deterministic.
+#include <hip/hip_runtime.h>
+
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+// Natural signs of the 8 nodes (OpenSees Brick order).
+__device__ __constant__ int8_t kSx[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+__device__ __constant__ int8_t kSy[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+__device__ __constant__ int8_t kSz[8] = {-1, -1, -1, -1, 1, 1, 1, 1};
+
+template <typename T>
+struct Shape {
+  T dx[8], dy[8], dz[8], N[8];
+  T dvol;
+};
+
+// shp3d at Gauss point gp = 4i + 2j + k (std_brick.hpp:45-152, 203-211).
+template <typename T>
+__device__ __forceinline__ void gauss_shape(const T (&x)[8], const T (&y)[8], const T (&z)[8], int gp, Shape<T>& s) {
+  const T r3 = T(0.57735026918962576451);  // 1/sqrt(3)
+  const T xi = (gp & 4) ? r3 : -r3, eta = (gp & 2) ? r3 : -r3, zeta = (gp & 1) ? r3 : -r3;
+  T nx[8], ny[8], nz[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const T fx = kSx[k] > 0 ? T(1) + xi : T(1) - xi;
+    const T fy = kSy[k] > 0 ? T(1) + eta : T(1) - eta;
+    const T fz = kSz[k] > 0 ? T(1) + zeta : T(1) - zeta;
+    const T cyz = T(0.125) * fy * fz, cxz = T(0.125) * fx * fz, cxy = T(0.125) * fx * fy;
+    nx[k] = kSx[k] > 0 ? cyz : -cyz;
+    ny[k] = kSy[k] > 0 ? cxz : -cxz;
+    nz[k] = kSz[k] > 0 ? cxy : -cxy;
+    s.N[k] = cxy * fz;
+  }
+  T J[3][3];
+  const T* c[3] = {x, y, z};
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const T* p = c[j];
+    J[j][0] = (p[1] - p[0]) * nx[1] + (p[2] - p[3]) * nx[2] + (p[5] - p[4]) * nx[5] + (p[6] - p[7]) * nx[6];
+    J[j][1] = (p[2] - p[1]) * ny[2] + (p[3] - p[0]) * ny[3] + (p[6] - p[5]) * ny[6] + (p[7] - p[4]) * ny[7];
+    J[j][2] = (p[4] - p[0]) * nz[4] + (p[5] - p[1]) * nz[5] + (p[6] - p[2]) * nz[6] + (p[7] - p[3]) * nz[7];
+  }
+  T ad[3][3];
+  ad[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+  ad[0][1] = J[2][1] * J[0][2] - J[2][2] * J[0][1];
+  ad[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+  ad[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+  ad[1][1] = J[2][2] * J[0][0] - J[2][0] * J[0][2];
+  ad[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+  ad[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+  ad[2][1] = J[2][0] * J[0][1] - J[2][1] * J[0][0];
+  ad[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  const T det = J[0][0] * ad[0][0] + J[0][1] * ad[1][0] + J[0][2] * ad[2][0];
+  const T rdet = T(1) / det;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const T a = nx[k], b = ny[k], cc = nz[k];
+    s.dx[k] = (a * ad[0][0] + b * ad[1][0] + cc * ad[2][0]) * rdet;
+    s.dy[k] = (a * ad[0][1] + b * ad[1][1] + cc * ad[2][1]) * rdet;
+    s.dz[k] = (a * ad[0][2] + b * ad[1][2] + cc * ad[2][2]) * rdet;
+  }
+  s.dvol = det;
+}
+
+// Effective factors of element_operator.hpp:45-56 for an elastic material (D_0 = D_c = D):
+// D_eff = (cK + c0 + cc) D; mass factor cM' = cM + cC alphaM.
+template <typename T>
+struct ElemCoef {
+  T lam, mu, rhoM;  // f*lambda, f*mu, cM'*rho
+  int consistent;
+};
+
+__host__ __device__ inline void operator_factors(const DevParam& p, const fx_coeffs& k, double& fD, double& cM) {
+  const double cK = k.initial ? k.damping * p.betaK : k.stiffness + k.damping * p.betaK;
+  const double c0 = k.initial ? k.stiffness + k.damping * p.betaKinit : k.damping * p.betaKinit;
+  const double cc = k.damping * p.betaKcomm;
+  fD = cK + c0 + cc;
+  cM = k.mass + k.damping * p.alphaM;
+}
+
+template <typename T>
+__device__ __forceinline__ void element_apply(const T (&x)[8], const T (&y)[8], const T (&z)[8], const T (&v)[24],
+                                              const ElemCoef<T>& c, T (&out)[24]) {
+#pragma unroll
+  for (int a = 0; a < 24; ++a) out[a] = T(0);
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    Shape<T> s;
+    gauss_shape(x, y, z, gp, s);
+    T e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const T u0 = v[3 * j], u1 = v[3 * j + 1], u2 = v[3 * j + 2];
+      e0 += s.dx[j] * u0;
+      e1 += s.dy[j] * u1;
+      e2 += s.dz[j] * u2;
+      e3 += s.dy[j] * u0 + s.dx[j] * u1;
+      e4 += s.dz[j] * u1 + s.dy[j] * u2;
+      e5 += s.dz[j] * u0 + s.dx[j] * u2;
+    }
+    const T lt = c.lam * (e0 + e1 + e2);
+    const T m2 = T(2) * c.mu;
+    const T s0 = (m2 * e0 + lt) * s.dvol, s1 = (m2 * e1 + lt) * s.dvol, s2 = (m2 * e2 + lt) * s.dvol;
+    const T s3 = c.mu * e3 * s.dvol, s4 = c.mu * e4 * s.dvol, s5 = c.mu * e5 * s.dvol;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      out[3 * j] += s.dx[j] * s0 + s.dy[j] * s3 + s.dz[j] * s5;
+      out[3 * j + 1] += s.dy[j] * s1 + s.dx[j] * s3 + s.dz[j] * s4;
+      out[3 * j + 2] += s.dz[j] * s2 + s.dy[j] * s4 + s.dx[j] * s5;
+    }
+    if (c.rhoM != T(0)) {
+      const T w = c.rhoM * s.dvol;
+      if (c.consistent) {
+        T g0 = 0, g1 = 0, g2 = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          g0 += s.N[k] * v[3 * k];
+          g1 += s.N[k] * v[3 * k + 1];
+          g2 += s.N[k] * v[3 * k + 2];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const T wj = w * s.N[j];
+          out[3 * j] += wj * g0;
+          out[3 * j + 1] += wj * g1;
+          out[3 * j + 2] += wj * g2;
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const T wj = w * s.N[j];
+          out[3 * j] += wj * v[3 * j];
+          out[3 * j + 1] += wj * v[3 * j + 1];
+          out[3 * j + 2] += wj * v[3 * j + 2];
+        }
+      }
+    }
+  }
+}
+
+// Diagonal of the element operator, with the cross terms of local DOFs that share an equation
+// added to the first of them (element_operator_diagonal, element_operator.hpp:138-209).
+template <typename T>
+__device__ __forceinline__ void element_diagonal(const T (&x)[8], const T (&y)[8], const T (&z)[8],
+                                                 const int (&eq)[24], const ElemCoef<T>& c, T (&out)[24]) {
+  int first[24];
+  bool merged = false;
+#pragma unroll
+  for (int a = 0; a < 24; ++a) {
+    out[a] = T(0);
+    first[a] = a;
+  }
+  for (int a = 0; a < 24; ++a) {
+    if (eq[a] < 0) continue;
+    for (int b = 0; b < a; ++b)
+      if (eq[b] == eq[a]) {
+        first[a] = first[b];
+        merged = true;
+        break;
+      }
+  }
+  const T l2 = c.lam + T(2) * c.mu;
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    Shape<T> s;
+    gauss_shape(x, y, z, gp, s);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const T dx = s.dx[j], dy = s.dy[j], dz = s.dz[j];
+      const T m = c.rhoM != T(0) ? c.rhoM * s.dvol * s.N[j] * (c.consistent ? s.N[j] : T(1)) : T(0);
+      out[3 * j] += (l2 * dx * dx + c.mu * (dy * dy + dz * dz)) * s.dvol + m;
+      out[3 * j + 1] += (l2 * dy * dy + c.mu * (dx * dx + dz * dz)) * s.dvol + m;
+      out[3 * j + 2] += (l2 * dz * dz + c.mu * (dx * dx + dy * dy)) * s.dvol + m;
+    }
+    if (!merged) continue;
+    // K_ab for distinct local DOFs a = (J, p), b = (K, q) with one equation:
+    // B_J^T D B_K entry (p, q) = lam dJp dKq + mu dJq dKp + (p == q) mu grad_J . grad_K
+    for (int a = 0; a < 24; ++a) {
+      if (eq[a] < 0) continue;
+      for (int b = 0; b < 24; ++b) {
+        if (b == a || eq[b] != eq[a]) continue;
+        const int J = a / 3, p = a % 3, K = b / 3, q = b % 3;
+        const T gJ[3] = {s.dx[J], s.dy[J], s.dz[J]}, gK[3] = {s.dx[K], s.dy[K], s.dz[K]};
+        T kab = c.lam * gJ[p] * gK[q] + c.mu * gJ[q] * gK[p];
+        if (p == q) kab += c.mu * (gJ[0] * gK[0] + gJ[1] * gK[1] + gJ[2] * gK[2]);
+        kab *= s.dvol;
+        if (p == q && J != K && c.rhoM != T(0) && c.consistent) kab += c.rhoM * s.dvol * s.N[J] * s.N[K];
+        out[first[a]] += kab;
+      }
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T load_coord(const double* c64, const float* c32, size_t i) {
+  if constexpr (sizeof(T) == 8) return c64[i];
+  else return c32[i];
+}
+
+// The block kernel: y (or the node-space ynode) += K_eff x over one colour of blocks.
+template <typename T, bool DIAG>
+__global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a, const int32_t* __restrict__ cblocks) {
+  extern __shared__ unsigned char smem_raw[];
+  T* slots = reinterpret_cast<T*>(smem_raw);  // [24][kBlockElems]
+  if (a.done && *a.done) return;
+  const int b = cblocks[blockIdx.x];
+  const int t = threadIdx.x;
+  const int e = a.elems[static_cast<size_t>(b) * kBlockElems + t];
+  if (e >= 0) {
+    T X[8], Y[8], Z[8], V[24], out[24];
+    int eqe[24];
+    const size_t P = static_cast<size_t>(a.padded);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int n = a.node[k * P + e];
+      X[k] = load_coord<T>(a.coords64, a.coords32, n);
+      Y[k] = load_coord<T>(a.coords64, a.coords32, static_cast<size_t>(a.nnode) + n);
+      Z[k] = load_coord<T>(a.coords64, a.coords32, 2 * static_cast<size_t>(a.nnode) + n);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int q = a.node_eq[3 * n + d];
+        eqe[3 * k + d] = q;
+        if constexpr (!DIAG) V[3 * k + d] = q >= 0 ? a.x[q] : T(0);
+      }
+    }
+    const DevParam& p = a.params[a.param[e]];
+    double fD, cM;
+    operator_factors(p, a.k, fD, cM);
+    ElemCoef<T> c;
+    c.lam = static_cast<T>(fD * p.lam);
+    c.mu = static_cast<T>(fD * p.mu);
+    c.rhoM = static_cast<T>(cM * p.rho);
+    c.consistent = p.mass_type == 0;
+    if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
+    else element_apply(X, Y, Z, V, c, out);
+#pragma unroll
+    for (int k = 0; k < 24; ++k) slots[k * kBlockElems + t] = out[k];
+  }
+  __syncthreads();
+  const int ln0 = a.ln_start[b], ln1 = a.ln_start[b + 1];
+  const uint16_t* list = a.slot_list + static_cast<size_t>(b) * 8 * kBlockElems;
+  for (int ln = ln0 + t; ln < ln1; ln += kBlockElems) {
+    const uint32_t sc = a.ln_slots[ln];
+    const int s0 = sc & 0xffff, ns = sc >> 16;
+    T acc0 = 0, acc1 = 0, acc2 = 0;
+    for (int j = 0; j < ns; ++j) {
+      const int s = list[s0 + j];
+      const int el = s >> 3, corner = s & 7;
+      acc0 += slots[(3 * corner) * kBlockElems + el];
+      acc1 += slots[(3 * corner + 1) * kBlockElems + el];
+      acc2 += slots[(3 * corner + 2) * kBlockElems + el];
+    }
+    const uint32_t lv = a.lnode[ln];
+    const bool first = (lv & 0x80000000u) && !a.accumulate;
+    const int n = static_cast<int>(lv & 0x7fffffffu);
+    const T acc[3] = {acc0, acc1, acc2};
+    if (a.ynode) {  // tied mesh: node space, contracted afterwards
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T* dst = a.ynode + 3 * static_cast<size_t>(n) + d;
+        *dst = first ? acc[d] : *dst + acc[d];
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int q = a.node_eq[3 * n + d];
+        if (q < 0) continue;
+        a.y[q] = first ? acc[d] : a.y[q] + acc[d];
+      }
+    }
+  }
+}
+
+// y[q] (+)= sum of the node-space values of q's node DOFs, in ascending DOF order.
+template <typename T>
+__global__ void contract_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
+                                const T* __restrict__ ynode, T* __restrict__ y, int accumulate, const int32_t* done) {
+  if (done && *done) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= neq) return;
+  T s = 0;
+  for (int j = start[q]; j < start[q + 1]; ++j) s += ynode[list[j]];
+  y[q] = accumulate ? y[q] + s : s;
+}
+
+template <typename T, bool DIAG>
+int launch_operator(fx_batch* b, const KernelArgs<T>& args) {
+  hipStream_t st = b->ctx->stream;
+  const size_t smem = sizeof(T) * 24 * kBlockElems;
+  for (int c = 0; c < b->blk.ncolors; ++c) {
+    const int n = b->blk.colour_start[c + 1] - b->blk.colour_start[c];
+    if (n == 0) continue;
+    hipLaunchKernelGGL((hex8_block_kernel<T, DIAG>), dim3(n), dim3(kBlockElems), smem, st, args,
+                       b->blk.colour_blocks + b->blk.colour_start[c]);
+  }
+  if (b->tied) {
+    const int nb = (b->neq + 255) / 256;
+    hipLaunchKernelGGL(contract_kernel<T>, dim3(nb), dim3(256), 0, st, b->neq, b->con_start, b->con_list,
+                       args.ynode, args.y, args.accumulate, args.done);
+  }
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+template <typename T>
+KernelArgs<T> operator_args(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate,
+                            const int32_t* done = nullptr) {
+  KernelArgs<T> a{};
+  a.done = done;
+  a.k = k;
+  a.padded = b->padded;
+  a.nnode = b->nnode;
+  a.node = b->node;
+  a.param = b->param;
+  a.node_eq = b->node_eq;
+  a.coords64 = b->coords;
+  a.coords32 = b->coords32;
+  a.params = b->params;
+  a.elems = b->blk.elems;
+  a.ln_start = b->blk.ln_start;
+  a.lnode = b->blk.lnode;
+  a.ln_slots = b->blk.ln_slots;
+  a.slot_list = b->blk.slot_list;
+  a.x = x;
+  a.y = y;
+  a.accumulate = accumulate;
+  if (b->tied) {
+    if constexpr (sizeof(T) == 8) a.ynode = b->ynode;
+    else a.ynode = b->ynode32;
+  }
+  return a;
+}
+
+int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
+  return launch_operator<double, false>(b, operator_args<double>(b, k, x, y, 0, done));
+}
+int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done) {
+  return launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
+}
+int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
+  return launch_operator<double, true>(b, operator_args<double>(b, k, static_cast<const double*>(nullptr), d, 0));
+}
+
+}  // namespace fx

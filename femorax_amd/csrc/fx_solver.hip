@@ -1,0 +1,425 @@
+// Device-resident Krylov solvers for the hex8 operator.
+//
+// ConjugateGradient::solve with Jacobi (src/solvers/cg.cpp:24-72, jacobi.cpp:10-26) with its
+// scalars kept on the device: each kernel reads a `done` flag and becomes a no-op once the
+// stop rule ||r|| <= max(rtol ||b||, atol) (cg.cpp:34, 60) or p.Ap <= 0 (cg.cpp:49-53) has
+// fired, so the host only polls every few iterations and the iterate is exactly the one the
+// reference stops at. Dot products follow VectorOps' order (vector_ops.cpp:19-64): chunks of
+// 2048 entries, 64 lanes, lane l summing entries l, l+64, ...; lane sums in lane order; chunk
+// sums reduced level by level. They accumulate in FP64 whatever the vector type.
+//
+// The mixed-precision solve wraps an FP32 CG (FP32 operator and vectors, FP64 dots) in FP64
+// iterative refinement: r = b - A x with the FP64 operator, d = A32^-1 r approximately,
+// x += d in FP64, until the FP64 residual meets the same stop rule.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+constexpr int kChunk = 2048;
+constexpr int kLanes = 64;
+
+struct CgState {
+  double rz, pap, alpha, beta, rr, tol, rhs_norm, init_res, final_res, scale;
+  int32_t it, done, converged, indefinite, max_it, pad;
+};
+
+__device__ __forceinline__ double as_double(double v) { return v; }
+__device__ __forceinline__ double as_double(float v) { return static_cast<double>(v); }
+
+// Partial sums per chunk of x.y (and, when y2 != null, of x2.y2) in VectorOps' lane order.
+template <typename T>
+__global__ __launch_bounds__(kLanes) void chunk_dot_kernel(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                           const T* __restrict__ x2, const T* __restrict__ y2,
+                                                           double* __restrict__ partial, const int32_t* done) {
+  if (done && *done) return;
+  __shared__ double lane_sum[2][kLanes];
+  const int64_t c = blockIdx.x;
+  const int64_t begin = c * kChunk;
+  const int64_t len = (n - begin < kChunk) ? n - begin : kChunk;
+  const int l = threadIdx.x;
+  double s = 0.0, s2 = 0.0;
+  for (int64_t k = l; k < len; k += kLanes) {
+    s += as_double(x[begin + k]) * as_double(y[begin + k]);
+    if (y2) s2 += as_double(x2[begin + k]) * as_double(y2[begin + k]);
+  }
+  lane_sum[0][l] = s;
+  lane_sum[1][l] = s2;
+  __syncthreads();
+  if (l == 0) {
+    double t = 0.0, t2 = 0.0;
+    for (int j = 0; j < kLanes; ++j) t += lane_sum[0][j];
+    for (int j = 0; j < kLanes; ++j) t2 += lane_sum[1][j];
+    if (y2) {
+      partial[2 * c] = t;
+      partial[2 * c + 1] = t2;
+    } else {
+      partial[c] = t;
+    }
+  }
+}
+
+enum ReduceOp : int { kStore = 0, kRhsNorm, kInitRes, kRz0, kPap, kIter };
+
+// Reduce `nv` interleaved sequences of chunk partials level by level in one workgroup (one
+// wave per chunk, VectorOps' order), then apply the solver step `op` to the totals.
+__global__ __launch_bounds__(kLanes) void reduce_kernel(int64_t npart, int nv, double* __restrict__ partial,
+                                                        double* __restrict__ tmp, CgState* st, int op,
+                                                        double rtol, double atol, double* out) {
+  if (op == kPap || op == kIter)
+    if (st->done) return;
+  __shared__ double lane_sum[kLanes];
+  const int l = threadIdx.x;
+  double total[2] = {0.0, 0.0};
+  for (int v = 0; v < nv; ++v) {
+    double* src = partial;
+    double* dst = tmp;
+    int64_t m = npart;
+    int stride = nv, off = v;  // first level reads interleaved
+    while (m > 1) {
+      const int64_t nch = (m + kChunk - 1) / kChunk;
+      for (int64_t c = 0; c < nch; ++c) {
+        const int64_t begin = c * kChunk;
+        const int64_t len = (m - begin < kChunk) ? m - begin : kChunk;
+        double s = 0.0;
+        for (int64_t k = l; k < len; k += kLanes) s += src[(begin + k) * stride + off];
+        lane_sum[l] = s;
+        __syncthreads();
+        if (l == 0) {
+          double t = 0.0;
+          for (int j = 0; j < kLanes; ++j) t += lane_sum[j];
+          dst[c] = t;
+        }
+        __syncthreads();
+      }
+      m = nch;
+      double* sw = src == partial ? tmp + ((npart + kChunk - 1) / kChunk + 1) : src;
+      src = dst;
+      dst = sw;
+      stride = 1;
+      off = 0;
+    }
+    total[v] = src[off];
+    __syncthreads();
+  }
+  if (l != 0) return;
+  switch (op) {
+    case kStore:
+      out[0] = total[0];
+      break;
+    case kRhsNorm:
+      st->rhs_norm = sqrt(total[0]);
+      st->tol = fmax(rtol * st->rhs_norm, atol);
+      break;
+    case kInitRes: {
+      const double r = sqrt(total[0]);
+      st->init_res = st->final_res = r;
+      st->it = 0;
+      st->indefinite = 0;
+      st->converged = r <= st->tol;
+      st->done = st->converged || st->max_it <= 0;
+      break;
+    }
+    case kRz0:
+      st->rz = total[0];
+      break;
+    case kPap:
+      st->pap = total[0];
+      if (!(total[0] > 0.0)) {
+        st->indefinite = 1;
+        st->done = 1;
+      } else {
+        st->alpha = st->rz / total[0];
+      }
+      break;
+    case kIter: {
+      const double r = sqrt(total[0]);
+      st->it += 1;
+      st->final_res = r;
+      if (r <= st->tol) {
+        st->converged = 1;
+        st->done = 1;
+      } else {
+        st->beta = total[1] / st->rz;
+        st->rz = total[1];
+        if (st->it >= st->max_it) st->done = 1;
+      }
+      break;
+    }
+  }
+}
+
+template <typename T>
+__global__ void residual_kernel(int64_t n, const T* __restrict__ b, const T* __restrict__ ap, T* __restrict__ r) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] + T(-1) * ap[i];
+}
+
+// z = inv r, p = z (first direction)
+template <typename T>
+__global__ void precond_first_kernel(int64_t n, const T* __restrict__ inv, const T* __restrict__ r, T* __restrict__ z,
+                                     T* __restrict__ p) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const T zi = inv[i] * r[i];
+    z[i] = zi;
+    p[i] = zi;
+  }
+}
+
+// x += alpha p; r -= alpha ap; z = inv r   (cg.cpp:55-56, 64), skipped once done
+template <typename T>
+__global__ void update_kernel(int64_t n, const CgState* st, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+                              const T* __restrict__ p, const T* __restrict__ ap, const T* __restrict__ inv) {
+  if (st->done) return;
+  const T alpha = static_cast<T>(st->alpha);
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    x[i] += alpha * p[i];
+    const T ri = r[i] + (-alpha) * ap[i];
+    r[i] = ri;
+    z[i] = inv[i] * ri;
+  }
+}
+
+// p = z + beta p (cg.cpp:68)
+template <typename T>
+__global__ void direction_kernel(int64_t n, const CgState* st, const T* __restrict__ z, T* __restrict__ p) {
+  if (st->done) return;
+  const T beta = static_cast<T>(st->beta);
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+template <typename T>
+__global__ void reciprocal_kernel(int64_t n, const double* __restrict__ d, T* __restrict__ inv, int32_t* zeros) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) {
+    if (d[i] == 0.0) atomicAdd(zeros, 1);
+    inv[i] = static_cast<T>(1.0 / d[i]);
+  }
+}
+
+// x64 += scale * d32; r32 = scale_in * r64 (conversions of the refinement)
+__global__ void refine_update_kernel(int64_t n, double* __restrict__ x, const float* __restrict__ d, double s) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += s * static_cast<double>(d[i]);
+}
+__global__ void to_f32_kernel(int64_t n, const double* __restrict__ a, float* __restrict__ b, double s) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = static_cast<float>(a[i] * s);
+}
+template <typename T>
+__global__ void zero_kernel(int64_t n, T* __restrict__ a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = T(0);
+}
+
+namespace {
+
+inline dim3 grid_of(int64_t n, int bs = 256) { return dim3(static_cast<unsigned>((n + bs - 1) / bs)); }
+inline int64_t chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
+
+// Reduction scratch: [2 * chunks] partials + a level buffer.
+int ensure_red(fx_ctx* ctx, int64_t n) {
+  const size_t need = static_cast<size_t>(2 * chunks(n) + 2 * (chunks(chunks(n)) + 2) + 16);
+  if (ctx->red_cap >= need) return FX_OK;
+  if (ctx->red) (void)hipFree(ctx->red);
+  ctx->red = nullptr;
+  FX_HIP(ctx, hipMalloc(&ctx->red, need * sizeof(double)));
+  ctx->red_cap = need;
+  return FX_OK;
+}
+
+}  // namespace
+
+int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out) {
+  if (n <= 0) {
+    *out = 0.0;
+    return FX_OK;
+  }
+  if (int rc = ensure_red(ctx, n)) return rc;
+  double* part = ctx->red;
+  double* tmp = ctx->red + 2 * chunks(n);
+  hipLaunchKernelGGL(chunk_dot_kernel<double>, dim3(static_cast<unsigned>(chunks(n))), dim3(kLanes), 0, ctx->stream,
+                     n, x, y, static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
+                     static_cast<const int32_t*>(nullptr));
+  double* dres = tmp + chunks(chunks(n)) + 8;
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, ctx->stream, chunks(n), 1, part, tmp,
+                     static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, dres);
+  FX_HIP(ctx, hipGetLastError());
+  FX_HIP(ctx, hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+namespace {
+
+// The CG iteration on the device for vector type T. apply(x, y) computes y = A x.
+template <typename T, typename Apply>
+int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
+              CgState* st, double rtol, double atol, int max_it, CgState& host) {
+  if (int rc = ensure_red(ctx, n)) return rc;
+  hipStream_t s = ctx->stream;
+  double* part = ctx->red;
+  double* tmp = ctx->red + 2 * chunks(n);
+  const dim3 g = grid_of(n), cg_chunks(static_cast<unsigned>(chunks(n)));
+  CgState init{};
+  init.max_it = max_it;
+  FX_HIP(ctx, hipMemcpyAsync(st, &init, sizeof(CgState), hipMemcpyHostToDevice, s));
+  const T* nt = nullptr;
+  // ||b||
+  hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, b, b, nt, nt, part,
+                     static_cast<const int32_t*>(nullptr));
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                     static_cast<int>(kRhsNorm), rtol, atol, static_cast<double*>(nullptr));
+  // r = b - A x
+  if (int rc = apply(x, ap, static_cast<const int32_t*>(nullptr))) return rc;
+  hipLaunchKernelGGL(residual_kernel<T>, g, dim3(256), 0, s, n, b, ap, r);
+  hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, nt, nt, part,
+                     static_cast<const int32_t*>(nullptr));
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                     static_cast<int>(kInitRes), rtol, atol, static_cast<double*>(nullptr));
+  // z = inv r, p = z, rz = r.z
+  hipLaunchKernelGGL(precond_first_kernel<T>, g, dim3(256), 0, s, n, inv, r, z, p);
+  hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, z, nt, nt, part,
+                     static_cast<const int32_t*>(nullptr));
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                     static_cast<int>(kRz0), rtol, atol, static_cast<double*>(nullptr));
+  FX_HIP(ctx, hipGetLastError());
+  int launched = 0, poll = 4;
+  const int32_t* done = &st->done;
+  for (;;) {
+    FX_HIP(ctx, hipMemcpyAsync(&host, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    FX_HIP(ctx, hipStreamSynchronize(s));
+    if (host.done || launched >= max_it) break;
+    const int batch = std::min(poll, max_it - launched);
+    for (int i = 0; i < batch; ++i) {
+      if (int rc = apply(p, ap, done)) return rc;
+      hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
+      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                         static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
+      hipLaunchKernelGGL(update_kernel<T>, g, dim3(256), 0, s, n, st, x, r, z, p, ap, inv);
+      hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, r, z, part, done);
+      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
+                         static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
+      hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
+    }
+    FX_HIP(ctx, hipGetLastError());
+    launched += batch;
+    poll = std::min(poll * 2, 64);
+  }
+  return FX_OK;
+}
+
+int jacobi_inverse(fx_batch* b, const fx_coeffs& k, double* d, void* inv, bool f32) {
+  fx_ctx* ctx = b->ctx;
+  if (int rc = operator_diagonal64(b, k, d)) return rc;
+  int32_t* zeros = reinterpret_cast<int32_t*>(ctx->scalars) + 64;
+  FX_HIP(ctx, hipMemsetAsync(zeros, 0, sizeof(int32_t), ctx->stream));
+  if (f32)
+    hipLaunchKernelGGL(reciprocal_kernel<float>, grid_of(b->neq), dim3(256), 0, ctx->stream, b->neq, d,
+                       static_cast<float*>(inv), zeros);
+  else
+    hipLaunchKernelGGL(reciprocal_kernel<double>, grid_of(b->neq), dim3(256), 0, ctx->stream, b->neq, d,
+                       static_cast<double*>(inv), zeros);
+  FX_HIP(ctx, hipGetLastError());
+  int32_t hz = 0;
+  FX_HIP(ctx, hipMemcpyAsync(&hz, zeros, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (hz > 0) {
+    ctx->error = "Jacobi: " + std::to_string(hz) + " zero diagonal entries";
+    return FX_EZERO;
+  }
+  return FX_OK;
+}
+
+}  // namespace
+
+int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+         fx_solve_result& res) {
+  fx_ctx* ctx = b->ctx;
+  const int64_t n = b->neq;
+  double *r = b->work, *z = r + n, *p = z + n, *ap = p + n, *inv = ap + n, *d = inv + n;
+  if (int rc = jacobi_inverse(b, k, d, inv, false)) return rc;
+  CgState* st = static_cast<CgState*>(ctx->scalars);
+  CgState h{};
+  auto apply = [&](const double* xx, double* yy, const int32_t* dn) { return operator_apply64(b, k, xx, yy, dn); };
+  if (int rc = cg_device<double>(ctx, n, apply, inv, rhs, x, r, z, p, ap, st, o.relative_tolerance,
+                                 o.absolute_tolerance, o.max_iterations, h))
+    return rc;
+  res = fx_solve_result{};
+  res.converged = h.converged;
+  res.iterations = h.it;
+  res.indefinite = h.indefinite;
+  res.rhs_norm = h.rhs_norm;
+  res.initial_residual = h.init_res;
+  res.final_residual = h.final_res;
+  // true residual ||b - A x||
+  if (int rc = operator_apply64(b, k, x, ap)) return rc;
+  hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, ctx->stream, n, rhs, ap, r);
+  double rr = 0;
+  if (int rc = dot64(ctx, n, r, r, &rr)) return rc;
+  res.true_residual = std::sqrt(rr);
+  return FX_OK;
+}
+
+int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
+             double* x, fx_solve_result& res) {
+  fx_ctx* ctx = b->ctx;
+  if (!b->has_f32) {
+    ctx->error = "fx_cg_mixed: the batch was created without FX_BATCH_FP32";
+    return FX_EUNSUP;
+  }
+  const int64_t n = b->neq;
+  double *r64 = b->work, *ap64 = r64 + 3 * n, *d64 = r64 + 5 * n;
+  float *inv = b->work32, *r = inv + n, *z = r + n, *p = z + n, *ap = p + n, *dx = ap + n, *rhs32 = dx + n;
+  if (int rc = jacobi_inverse(b, k, d64, inv, true)) return rc;
+  CgState* st = static_cast<CgState*>(ctx->scalars);
+  hipStream_t s = ctx->stream;
+  double bb = 0;
+  if (int rc = dot64(ctx, n, rhs, rhs, &bb)) return rc;
+  const double bnorm = std::sqrt(bb);
+  const double tol = std::max(o.relative_tolerance * bnorm, o.absolute_tolerance);
+  res = fx_solve_result{};
+  res.rhs_norm = bnorm;
+  int total_inner = 0;
+  auto apply32 = [&](const float* xx, float* yy, const int32_t* dn) { return operator_apply32(b, k, xx, yy, dn); };
+  for (int outer = 0;; ++outer) {
+    // r = b - A x in FP64
+    if (int rc = operator_apply64(b, k, x, ap64)) return rc;
+    hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, s, n, rhs, ap64, r64);
+    double rr = 0;
+    if (int rc = dot64(ctx, n, r64, r64, &rr)) return rc;
+    const double rn = std::sqrt(rr);
+    if (outer == 0) res.initial_residual = rn;
+    res.final_residual = res.true_residual = rn;
+    res.outer_iterations = outer;
+    if (rn <= tol) {
+      res.converged = 1;
+      break;
+    }
+    if (total_inner >= o.max_iterations || outer >= 100) break;
+    // inner FP32 solve of A d = r / ||r||
+    hipLaunchKernelGGL(to_f32_kernel, grid_of(n), dim3(256), 0, s, n, r64, rhs32, 1.0 / rn);
+    hipLaunchKernelGGL(zero_kernel<float>, grid_of(n), dim3(256), 0, s, n, dx);
+    CgState h{};
+    if (int rc = cg_device<float>(ctx, n, apply32, inv, rhs32, dx, r, z, p, ap, st, inner_rtol, 0.0,
+                                  o.max_iterations - total_inner, h))
+      return rc;
+    total_inner += h.it;
+    if (h.indefinite) res.indefinite = 1;
+    hipLaunchKernelGGL(refine_update_kernel, grid_of(n), dim3(256), 0, s, n, x, dx, rn);
+    FX_HIP(ctx, hipGetLastError());
+    if (h.it == 0) break;  // no progress possible
+  }
+  res.iterations = total_inner;
+  return FX_OK;
+}
+
+}  // namespace fx

@@ -1,0 +1,156 @@
+"""Synthetic models, built in memory the way the reference's tests build them
+(tests/common/synthetic_models.hpp:24-94): structured stdBrick grids with fixed bases and
+laminar equalDOF ties, materials per layer from the reference's 64-bit LCG
+(tests/reference/drivers/driver_common.hpp:21-32), and the load vectors the configurations
+need (consistent surface tractions, body force). There is no model file on this path: the
+reference's own tests and benchmarks build their right-hand sides in C++ the same way
+(tests/unit/test_cg.cpp:66-80, benchmarks/bench_operator.cpp:113-132).
+
+A model is a dict:
+  nnode, xyz [nnode, 3] (model order: i fastest, then j, then k; tag = index + 1)
+  nelem, conn [nelem, 8] (model node indices, OpenSees Brick order), group [nelem]
+  fix [n, 2] (node, dof 0..2), tie [n, 3] (retained, constrained, dof 0..2)
+  params: list of per-group dicts (E, nu, rho, b, mass_type, rayleigh)
+  grid: (nx, ny, nz) element counts, h: (dx, dy, dz)
+"""
+import numpy as np
+
+SEED_BASE = 20261202
+
+
+class Lcg:
+    """The reference's platform-independent LCG: uniform in [-1, 1)."""
+
+    def __init__(self, seed):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def symmetric(self):
+        self.s = (self.s * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return (self.s >> 11) * (2.0 / 9007199254740992.0) - 1.0
+
+    def uniform(self, lo, hi):
+        return lo + (hi - lo) * 0.5 * (self.symmetric() + 1.0)
+
+    def vector(self, n):
+        return np.array([self.symmetric() for _ in range(n)])
+
+
+def grid(nx, ny, nz, dx, dy, dz, z0=0.0):
+    """Nodes and hex8 connectivity of an nx x ny x nz grid (model order, i fastest)."""
+    NX, NY, NZ = nx + 1, ny + 1, nz + 1
+    k, j, i = np.meshgrid(np.arange(NZ), np.arange(NY), np.arange(NX), indexing="ij")
+    xyz = np.stack([i.ravel() * dx, j.ravel() * dy, z0 + k.ravel() * dz], axis=1).astype(np.float64)
+    ek, ej, ei = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    ei, ej, ek = ei.ravel(), ej.ravel(), ek.ravel()
+
+    def nid(a, b, c):
+        return a + NX * (b + NY * c)
+
+    conn = np.stack([nid(ei, ej, ek), nid(ei + 1, ej, ek), nid(ei + 1, ej + 1, ek), nid(ei, ej + 1, ek),
+                     nid(ei, ej, ek + 1), nid(ei + 1, ej, ek + 1), nid(ei + 1, ej + 1, ek + 1),
+                     nid(ei, ej + 1, ek + 1)], axis=1).astype(np.int32)
+    return xyz, conn, ek.astype(np.int32)
+
+
+def fixed_base(nx, ny):
+    NX, NY = nx + 1, ny + 1
+    base = np.arange(NX * NY, dtype=np.int32)
+    return np.stack([np.repeat(base, 3), np.tile(np.arange(3, dtype=np.int32), len(base))], axis=1)
+
+
+def elastic_box(n=16, L=1.0, E=30e6, nu=0.3, rho=2000.0, layers=None, body=False, seed=None):
+    """An n^3 box on [0, L]^3, base fixed. layers: None (one material) or a count of equal
+    layers whose E (log-uniform 20-200 MPa) and nu (uniform 0.25-0.45) come from the LCG."""
+    h = L / n
+    xyz, conn, ek = grid(n, n, n, h, h, h)
+    params = []
+    if layers:
+        rng = Lcg(seed if seed is not None else SEED_BASE + 1)
+        per = n // layers
+        group = np.minimum(ek // per, layers - 1).astype(np.int32)
+        for _ in range(layers):
+            Ei = float(np.exp(rng.uniform(np.log(20e6), np.log(200e6))))
+            nui = rng.uniform(0.25, 0.45)
+            b = (0.0, 0.0, -rho * 9.81) if body else (0.0, 0.0, 0.0)
+            params.append(dict(E=Ei, nu=nui, rho=rho, b=b, mass_type=0))
+    else:
+        group = np.zeros(len(conn), np.int32)
+        b = (0.0, 0.0, -rho * 9.81) if body else (0.0, 0.0, 0.0)
+        params.append(dict(E=E, nu=nu, rho=rho, b=b, mass_type=0))
+    return dict(nnode=len(xyz), xyz=xyz, nelem=len(conn), conn=conn, group=group,
+                fix=fixed_base(n, n), tie=np.zeros((0, 3), np.int32), params=params,
+                grid=(n, n, n), h=(h, h, h))
+
+
+def soil_column(nz, h=1.0, params=None, layer_of=None, nx=1, ny=1):
+    """A 1x1xN (or nx x ny x N) column: base fixed, side nodes of each level tied to the first
+    side node in DOFs 1 2 3 (the laminar boundary of synthetic_models.hpp:64-79)."""
+    xyz, conn, ek = grid(nx, ny, nz, h, h, h, z0=-nz * h)
+    NX, NY = nx + 1, ny + 1
+    ties = []
+    for k in range(1, nz + 1):
+        master = -1
+        for j in range(NY):
+            for i in range(NX):
+                if not (i == 0 or i == nx or j == 0 or j == ny):
+                    continue
+                t = i + NX * (j + NY * k)
+                if master < 0:
+                    master = t
+                    continue
+                for d in range(3):
+                    ties.append((master, t, d))
+    group = (layer_of(ek) if layer_of is not None else np.zeros(len(conn), np.int32)).astype(np.int32)
+    return dict(nnode=len(xyz), xyz=xyz, nelem=len(conn), conn=conn, group=group, fix=fixed_base(nx, ny),
+                tie=np.array(ties, np.int32).reshape(-1, 3), params=params or [], grid=(nx, ny, nz),
+                h=(h, h, h))
+
+
+def top_traction_nodal(model, traction_z):
+    """Consistent nodal loads of a -z traction on the top face (z = max): each top quad gives
+    t * A / 4 to its 4 corners. traction_z: scalar or per-top-node array (model order of the
+    top layer). Returns [nnode, 3]."""
+    nx, ny, nz = model["grid"]
+    dx, dy, _ = model["h"]
+    NX, NY = nx + 1, ny + 1
+    F = np.zeros((model["nnode"], 3))
+    top0 = NX * NY * nz
+    t = np.broadcast_to(np.asarray(traction_z, dtype=np.float64), (NX * NY,))
+    A = dx * dy
+    for j in range(ny):
+        for i in range(nx):
+            corners = [i + NX * j, i + 1 + NX * j, i + 1 + NX * (j + 1), i + NX * (j + 1)]
+            tq = np.mean(t[corners])
+            for c in corners:
+                F[top0 + c, 2] += -tq * A / 4.0
+    return F
+
+
+def nodal_to_equations(F, node_rank, node_eq, neq):
+    """Sum nodal loads [nnode, 3] (model order) into equation space; fixed DOFs drop out and
+    tied DOFs add onto their shared equation."""
+    b = np.zeros(neq)
+    eqs = node_eq.reshape(-1, 3)[node_rank]  # model node -> its equations
+    for d in range(3):
+        q = eqs[:, d]
+        m = q >= 0
+        np.add.at(b, q[m], F[m, d])
+    return b
+
+
+def config_c1():
+    """16^3 static box: E 30 MPa, nu 0.3, 1 m cube, base fixed, 10 kPa top traction."""
+    m = elastic_box(16, 1.0, 30e6, 0.3, 2000.0)
+    m["nodal_load"] = top_traction_nodal(m, 10e3)
+    return m
+
+
+def config_c2(n=128, seed=SEED_BASE + 1):
+    """n^3 layered box on [0, n]^3 m (8 layers, E log-uniform 20-200 MPa, nu 0.25-0.45 from
+    the LCG), body force, random top traction 10 kPa (1 + 0.5 u)."""
+    m = elastic_box(n, float(n), layers=8, body=True, seed=seed)
+    rng = Lcg(seed + 1000)
+    NX = n + 1
+    t = np.array([1e4 * (1.0 + 0.5 * rng.symmetric()) for _ in range(NX * NX)])
+    m["nodal_load"] = top_traction_nodal(m, t)
+    return m

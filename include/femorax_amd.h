@@ -1,0 +1,184 @@
+/*
+ * femorax_amd — C ABI of the MI355X-native hex8 matrix-free operator path.
+ *
+ * The cut follows the reference's type-erased per-pair backend InternalForce::Batch
+ * (amnp95/FemoraX src/exec/internal_force_batch.hpp:46-100) and the solver interface
+ * (src/solvers/linear_solver.hpp:57-105): a C++ Batch / LinearOperator subclass on the
+ * reference side forwards its virtual calls here, so src/exec/implicit_stage.cpp,
+ * src/exec/explicit_analysis.cpp and the solvers' callers stay unchanged.
+ *
+ * Conventions (the reference's, src/exec/internal_force.hpp:80-244):
+ *   - every entry point returns FX_OK (0) or a negative FX_E* code; fx_last_error(ctx)
+ *     gives the message (the reference throws std::runtime_error / io::ModelError, or calls
+ *     Kokkos::abort on misuse);
+ *   - construction (fx_batch_create) uploads and allocates everything; the per-call entry
+ *     points allocate nothing;
+ *   - vectors x, y, b, u, v, f are borrowed DEVICE pointers in equation numbering, FP64
+ *     unless the name says FP32; x and y never alias (linear_solver.hpp:62);
+ *   - calls are blocking at this boundary unless the name says _async (the reference's
+ *     dots and norms read back to the host: src/solvers/vector_ops.cpp:62);
+ *   - a material failure is data (NaN stress), not an error (pressure_independ_multi_yield.hpp:515-518).
+ */
+#ifndef FEMORAX_AMD_H
+#define FEMORAX_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FX_OK 0
+#define FX_EINVAL -1   /* bad argument or shape */
+#define FX_EDEVICE -2  /* HIP runtime error */
+#define FX_ENOMEM -3   /* over the HBM budget or out of device memory */
+#define FX_EZERO -4    /* Jacobi: zero diagonal entries (jacobi.cpp:19-22) */
+#define FX_EUNSUP -5   /* unsupported element/material/option */
+
+typedef struct fx_ctx fx_ctx;
+typedef struct fx_mesh fx_mesh;
+typedef struct fx_batch fx_batch;
+
+/* ---- context: one device, one stream, an HBM budget (exec/device_memory.cpp:26-38) ------- */
+/* hbm_budget_bytes = 0: 90% of free memory at creation (explicit_analysis.cpp:446-454). */
+int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out);
+void fx_ctx_destroy(fx_ctx* ctx);
+const char* fx_last_error(const fx_ctx* ctx);
+int fx_sync(fx_ctx* ctx);
+/* Free and total device memory (src/native/device_memory_hip.cpp:6-11). */
+int fx_device_memory(fx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+uint64_t fx_budget_used(const fx_ctx* ctx);
+
+/* Device buffers for callers without their own allocator (tests, the benchmark). */
+int fx_alloc(fx_ctx* ctx, size_t bytes, void** dptr);
+int fx_free(fx_ctx* ctx, void* dptr);
+int fx_upload(fx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int fx_download(fx_ctx* ctx, void* dst, const void* src, size_t bytes);
+int fx_memset0(fx_ctx* ctx, void* dst, size_t bytes);
+
+/* ---- compile step (compile::compile_model for stdBrick batches, src/compile/batches.cpp:54-227,
+ *      renumber.cpp:153-183, dof_numbering.cpp:269-463) ----------------------------------- */
+typedef struct fx_model_desc {
+  int32_t nnode;
+  const double* xyz;        /* [nnode*3], row major, model order */
+  int32_t nelem;
+  const int32_t* conn;      /* [nelem*8] model node indices, OpenSees Brick node order */
+  const int32_t* group;     /* [nelem] element group (one material each), NULL = all 0 */
+  const int32_t* region;    /* [nelem] Rayleigh region or -1, NULL = none */
+  int32_t nfix;
+  const int32_t* fix_node;  /* model node index */
+  const int32_t* fix_dof;   /* 0..2 */
+  int32_t ntie;             /* equalDOF rows (retained, constrained, dof 0..2) */
+  const int32_t* tie_retained;
+  const int32_t* tie_constrained;
+  const int32_t* tie_dof;
+} fx_model_desc;
+
+typedef struct fx_mesh_view {
+  int32_t nnode, nelem, padded, neq, nparam, ncolor;
+  const int32_t* node_order;     /* [nnode] internal -> model node (Ordering::node_order) */
+  const int32_t* node_rank;      /* [nnode] model -> internal */
+  const int32_t* elem_order;     /* [nelem] batch position -> file element */
+  const int32_t* node_eq;        /* [3*nnode] equation of internal node i, DOF d; -1 fixed */
+  const double* coords;          /* [3*nnode] SoA x[N] y[N] z[N] (Compiled::coords) */
+  const int32_t* node;           /* [8*padded] Batch::node, value[k*padded+e], -1 padding */
+  const int32_t* eq;             /* [24*padded] Batch::eq */
+  const int32_t* param;          /* [padded] Batch::param */
+  const int32_t* param_group;    /* [nparam] ParamSet::group */
+  const int32_t* param_region;   /* [nparam] ParamSet::region */
+  const int32_t* color_start;    /* [ncolor+1] Batch::color_start (NULL without colouring) */
+  const int32_t* color_elements; /* [nelem] */
+} fx_mesh_view;
+
+int fx_compile_model(const fx_model_desc* model, int renumber_rcm, int simd_width, int build_coloring,
+                     fx_mesh** out);
+int fx_mesh_get(const fx_mesh* mesh, fx_mesh_view* view);
+void fx_mesh_destroy(fx_mesh* mesh);
+
+/* ---- element parameters --------------------------------------------------------------- */
+/* Rayleigh (src/physics/contracts.hpp:69-75). */
+typedef struct fx_rayleigh { double alphaM, betaK, betaKinit, betaKcomm; } fx_rayleigh;
+
+/* stdBrick<ElasticIsotropic>::Params (std_brick.hpp:181-185, elastic_isotropic.hpp:30-34)
+ * with its ParamSet's damping (compile/batches.hpp:28-39). */
+typedef struct fx_elastic_params {
+  double E, nu, rho;
+  double b[3];        /* body force per unit volume */
+  int32_t mass_type;  /* 0 consistent, 1 lumped */
+  fx_rayleigh damp;
+} fx_elastic_params;
+
+/* OperatorCoefficients (src/physics/operators/element_operator.hpp:33-40). */
+typedef struct fx_coeffs { double stiffness, damping, mass; int32_t initial; } fx_coeffs;
+
+/* ---- batch: InternalForce::Batch behind a C ABI (internal_force_batch.hpp:46-100) -------- */
+typedef struct fx_batch_desc {
+  int32_t count, padded;         /* real elements, padded count (Batch::count / ::padded) */
+  const int32_t* node;           /* [8*padded] host, Batch::node */
+  const int32_t* eq;             /* [24*padded] host, Batch::eq */
+  const int32_t* param;          /* [padded] host, Batch::param */
+  int32_t nparam;
+  const fx_elastic_params* params; /* [nparam] host */
+  int32_t nnode;                 /* nodes of the model */
+  const double* coords;          /* [3*nnode] host SoA (Compiled::coords) */
+  int32_t neq;                   /* equations (DofNumbering::num_equations) */
+} fx_batch_desc;
+
+/* Flags for fx_batch_create. */
+#define FX_BATCH_FP32 1u /* also build the FP32 operator (mixed precision solves) */
+
+int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* desc, uint32_t flags, fx_batch** out);
+void fx_batch_destroy(fx_batch* b);
+
+typedef struct fx_batch_info {
+  int32_t count, neq, nnode, nblocks, ncolors, tied, affine_classes, shape_classes;
+  uint64_t device_bytes;
+} fx_batch_info;
+int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
+
+/* y = K_eff x, K_eff = cK K_t + cC C + cM M (InternalForce::apply_operator,
+ * internal_force.cpp:545-573; element_operator_apply, element_operator.hpp:82-130).
+ * Deterministic: the same bits for the same inputs, whatever else runs. */
+int fx_operator_apply(fx_batch* b, const fx_coeffs* k, const double* x, double* y);
+/* The same in FP32 (x, y FP32 device vectors; FP32 geometry and arithmetic). Needs FX_BATCH_FP32. */
+int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float* y);
+/* d = diag(K_eff) (InternalForce::operator_diagonal, internal_force.cpp:575-599;
+ * element_operator_diagonal, element_operator.hpp:138-209). */
+int fx_operator_diagonal(fx_batch* b, const fx_coeffs* k, double* d);
+/* Timing aid: the operator kernel alone, `reps` times back to back, device time in ms
+ * (HIP events on the batch's stream). */
+int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void* x, void* y, int reps,
+                     float* ms);
+
+/* ---- solvers (src/solvers/linear_solver.hpp:79-105, cg.cpp:24-72, jacobi.cpp:10-26) ------- */
+typedef struct fx_solve_options {
+  double relative_tolerance; /* stop when ||r|| <= max(rtol ||b||, atol) (cg.cpp:34) */
+  double absolute_tolerance;
+  int32_t max_iterations;
+} fx_solve_options;
+
+typedef struct fx_solve_result {
+  int32_t converged, iterations, indefinite;
+  int32_t outer_iterations;  /* iterative refinement sweeps (mixed precision), else 0 */
+  double rhs_norm, initial_residual, final_residual;
+  double true_residual;      /* ||b - A x|| recomputed in FP64 at the end */
+} fx_solve_result;
+
+/* VectorOps::dot (vector_ops.cpp:66-73): chunks of 2048, 64 lanes, fixed order. */
+int fx_dot(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
+
+/* Preconditioned CG with Jacobi, FP64 (ConjugateGradient::solve + Jacobi). x: initial guess in,
+ * solution out. The plain drop-in path. */
+int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+          fx_solve_result* res);
+/* FP32 inner CG (FP32 operator, FP32 vectors, FP64 dot products) inside FP64 iterative
+ * refinement (FP64 residual b - A x with the FP64 operator, FP64 update); same stop rule on the
+ * FP64 residual. inner_rtol: each inner solve's relative tolerance. */
+int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol,
+                const double* rhs, double* x, fx_solve_result* res);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

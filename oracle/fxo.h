@@ -1,0 +1,160 @@
+/*
+ * fxo — the CPU oracle for the hex8 matrix-free operator path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a plain-C restatement of the reference's algorithms
+ * (amnp95/FemoraX) for the hot path: the compile-step data the kernels read (RCM node order,
+ * element order, DOF numbering, SoA maps, greedy colouring), the stdBrick element with
+ * ElasticIsotropic and PressureIndependMultiYield, the matrix-free element operator and its
+ * diagonal, the deterministic gather assembly, the fixed-order dot product, CG + Jacobi,
+ * Newmark and CentralDifference. Only the tests, the smoke check and the CPU-baseline leg of
+ * the benchmark load it, and only as the checker. The product (femorax_amd) never links it.
+ *
+ * Every function names the reference file:line it restates. Where the restatement keeps the
+ * reference's arithmetic order (so that the goldens the reference holds match bit for bit on
+ * a compiler that does not contract multiply-adds), that is said at the function.
+ *
+ * Pinning (how each part is checked, in tests/):
+ *   brick geometry/force/tangent/mass/damping  reference goldens tests/reference/data/element
+ *                                              (Brick_*), and oracle/_ref (reference headers)
+ *   ElasticIsotropic                           goldens nd/ElasticIsotropic_*, and oracle/_ref
+ *   PIMY return map                            goldens nd/PressureIndependMultiYield_* (34)
+ *   element operator / diagonal                K·v and diag(K) from the Brick goldens, oracle/_ref
+ *   RCM, element order, colouring              closed forms on small graphs (path, grid, star),
+ *                                              invariants (bandwidth, BFS level structure)
+ *   DOF numbering                              goldens constraints/... (OpenSees equivalence classes)
+ *   gather assembly / apply                    brute-force assembled K·x on small meshes
+ *   dot                                        exact rational sums; lane order vs naive sum
+ *   CG + Jacobi                                dense direct solve on small meshes
+ * Nothing here is "parity unpinned" except where a header comment says so.
+ */
+#ifndef FXO_H
+#define FXO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- LCG (tests/reference/drivers/driver_common.hpp:21-32) ------------------------------ */
+double fxo_lcg_symmetric(uint64_t* state);
+
+/* ---- Model (the subset of io::Model the path uses, src/io/model.hpp:74-186) -------------- */
+typedef struct fxo_model {
+  int32_t nnode;
+  const double* xyz;       /* [nnode*3], row major, model (file) order */
+  int32_t nelem;
+  const int32_t* conn;     /* [nelem*8] model node indices, OpenSees Brick node order */
+  const int32_t* group;    /* [nelem] element group (file order; one material per group) */
+  const int32_t* region;   /* [nelem] Rayleigh region or -1; NULL = none */
+  int32_t nfix;
+  const int32_t* fix_node; /* model node index */
+  const int32_t* fix_dof;  /* 0..2 */
+  int32_t ntie;            /* equalDOF rows */
+  const int32_t* tie_retained;
+  const int32_t* tie_constrained;
+  const int32_t* tie_dof;  /* 0..2 */
+} fxo_model;
+
+typedef struct fxo_compiled {
+  int32_t nnode, nelem, padded, neq, nparam, ncolor;
+  int32_t* node_order;   /* [nnode] internal -> model index */
+  int32_t* node_rank;    /* [nnode] model -> internal index */
+  int32_t* elem_order;   /* [nelem] batch position -> global element index */
+  int32_t* node_eq;      /* [3*nnode] by internal node: equation or -1 */
+  double* coords;        /* [3*nnode] SoA by internal node */
+  int32_t* node;         /* [8*padded] internal node, -1 padding */
+  int32_t* eq;           /* [24*padded] equation, -1 fixed/padding */
+  int32_t* param;        /* [padded] param set, 0 padding */
+  int32_t* param_group;  /* [nparam] */
+  int32_t* param_region; /* [nparam] */
+  int32_t* color_start;  /* [ncolor+1] */
+  int32_t* color_elements; /* [nelem] */
+  /* the deterministic gather (internal_force.cpp:121-134, assembly.cpp:64-83): each
+   * equation's slots k*padded+e in (element, DOF) order, CSR by equation */
+  int64_t* gather_start; /* [neq+1] */
+  int64_t* gather_list;  /* [nnz] */
+  double* slots;         /* [24*padded] work */
+} fxo_compiled;
+
+/* src/compile/renumber.cpp:30-183, dof_numbering.cpp:21-40,269-485, batches.cpp:16-227 */
+int fxo_compile(const fxo_model* m, int renumber_rcm, int simd_width, int build_coloring,
+                fxo_compiled* out);
+void fxo_compiled_free(fxo_compiled* c);
+/* Largest |rank(a)-rank(b)| over node pairs sharing an element (renumber.cpp:185-199). */
+int32_t fxo_node_bandwidth(const fxo_model* m, const int32_t* node_rank);
+
+/* ---- Element parameters ------------------------------------------------------------------ */
+typedef struct fxo_rayleigh { double alphaM, betaK, betaKinit, betaKcomm; } fxo_rayleigh;
+
+/* stdBrick<ElasticIsotropic>::Params (std_brick.hpp:181-185, elastic_isotropic.hpp:30-34) */
+typedef struct fxo_elastic {
+  double E, nu, rho;
+  double b[3];
+  int32_t mass_type; /* 0 consistent, 1 lumped */
+  fxo_rayleigh damp;
+} fxo_elastic;
+
+typedef struct fxo_coeffs { double stiffness, damping, mass; int32_t initial; } fxo_coeffs;
+
+/* ---- stdBrick (std_brick.hpp) -------------------------------------------------------------- */
+/* shp3d at Gauss point gp (std_brick.hpp:45-152, 203-211): shp[4][8] (dN/dx, dN/dy, dN/dz, N) */
+void fxo_brick_shape(const double xl[3][8], int gp, double shp[4][8], double* dvol);
+/* Elastic D (elastic_isotropic.hpp:66-94). */
+void fxo_elastic_tangent(double E, double nu, double D[6][6]);
+/* Brick::compute<WantK> with ElasticIsotropic (std_brick.hpp:481-543): fe and optionally Ke. */
+void fxo_brick_force_elastic(const fxo_elastic* p, const double xe[8][3], const double* ue,
+                             const double* ve, double* fe, double* Ke /* 24*24 or NULL */);
+/* Brick::mass (std_brick.hpp:582-605) and lumped_mass (:609-623). */
+void fxo_brick_mass(const fxo_elastic* p, const double xe[8][3], double* Me);
+void fxo_brick_lumped_mass(double rho, const double xe[8][3], double m[8]);
+/* element_operator_apply / _diagonal with a constant D (element_operator.hpp:82-209).
+ * D: the tangent, D0: the initial tangent (may equal D), Dc: committed tangent (NULL: = D). */
+void fxo_brick_operator_apply(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3],
+                              const double* D, const double* ve, double* ye);
+void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3],
+                                 const double* D, const int32_t* eq, double* de);
+
+/* ---- Global operator (gather assembly, internal_force.cpp:121-134, 545-599) ---------------- */
+/* y = K_eff x; params[nparam]. x, y in equation space. Deterministic slot-gather order. */
+void fxo_apply(const fxo_compiled* c, const fxo_elastic* params, const fxo_coeffs* k,
+               const double* x, double* y, int nthreads);
+void fxo_diagonal(const fxo_compiled* c, const fxo_elastic* params, const fxo_coeffs* k, double* d,
+                  int nthreads);
+/* Lumped mass per equation (batches.cpp:204-224). */
+void fxo_lumped_mass(const fxo_compiled* c, const fxo_elastic* params, double* m, double* am);
+/* Body-force load vector: -f(u=0) restricted to the body term (std_brick.hpp:329-333). */
+void fxo_body_load(const fxo_compiled* c, const fxo_elastic* params, double* b);
+
+/* ---- VectorOps (solvers/vector_ops.cpp:19-124) -------------------------------------------- */
+double fxo_dot(int64_t n, const double* x, const double* y, int lanes);
+
+/* ---- CG + Jacobi (solvers/cg.cpp:24-72, jacobi.cpp:10-26) ---------------------------------- */
+typedef struct fxo_solve_result {
+  int32_t converged, iterations, indefinite;
+  double rhs_norm, initial_residual, final_residual;
+} fxo_solve_result;
+int fxo_cg(const fxo_compiled* c, const fxo_elastic* params, const fxo_coeffs* k, const double* b,
+           double* x, double rtol, double atol, int32_t max_it, int lanes, int nthreads,
+           fxo_solve_result* res);
+
+/* ---- PressureIndependMultiYield<NS> (pressure_independ_multi_yield.hpp) -------------------- */
+#define FXO_PIMY_NS 20
+typedef struct fxo_pimy_params {
+  int32_t load_stage, nsurf;
+  double rho, G, K, phi, gamma_peak, pref, cohesion, pdc, residual_press;
+  double size[FXO_PIMY_NS + 1], modulus[FXO_PIMY_NS + 1];
+} fxo_pimy_params;
+/* in: rho, refShearModul, refBulkModul, cohesi, peakShearStra, frictionAng, refPress, pressDependCoe
+ * (the constructor's order, :56-61); gredu: NULL or nsurf (gamma, G/Gmax) pairs. 0 = ok. */
+int fxo_pimy_setup(const double* in, int nsurf, const double* gredu, fxo_pimy_params* p);
+int fxo_pimy_nstate(void);
+void fxo_pimy_initial_state(const fxo_pimy_params* p, double* state);
+/* update (:228-240): sig, D at the trial strain eps from the committed buffer; trial written. */
+void fxo_pimy_update(const fxo_pimy_params* p, const double* committed, double* trial, const double eps[6],
+                     double sig[6], double D[36]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

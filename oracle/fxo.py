@@ -1,0 +1,225 @@
+"""ctypes view of the oracle (oracle/liboracle.so) — TEST INFRASTRUCTURE.
+
+Only tests/, the smoke check and the benchmark's CPU-baseline leg import this module; the
+product package never does. See fxo.h for what each function restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class Model(C.Structure):
+    _fields_ = [("nnode", C.c_int32), ("xyz", C.c_void_p), ("nelem", C.c_int32), ("conn", C.c_void_p),
+                ("group", C.c_void_p), ("region", C.c_void_p), ("nfix", C.c_int32), ("fix_node", C.c_void_p),
+                ("fix_dof", C.c_void_p), ("ntie", C.c_int32), ("tie_retained", C.c_void_p),
+                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p)]
+
+
+class Compiled(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("nnode", "nelem", "padded", "neq", "nparam", "ncolor")] + [
+        ("node_order", C.POINTER(C.c_int32)), ("node_rank", C.POINTER(C.c_int32)),
+        ("elem_order", C.POINTER(C.c_int32)), ("node_eq", C.POINTER(C.c_int32)),
+        ("coords", C.POINTER(C.c_double)), ("node", C.POINTER(C.c_int32)), ("eq", C.POINTER(C.c_int32)),
+        ("param", C.POINTER(C.c_int32)), ("param_group", C.POINTER(C.c_int32)),
+        ("param_region", C.POINTER(C.c_int32)), ("color_start", C.POINTER(C.c_int32)),
+        ("color_elements", C.POINTER(C.c_int32)), ("gather_start", C.POINTER(C.c_int64)),
+        ("gather_list", C.POINTER(C.c_int64)), ("slots", C.POINTER(C.c_double))]
+
+
+class Rayleigh(C.Structure):
+    _fields_ = [("alphaM", C.c_double), ("betaK", C.c_double), ("betaKinit", C.c_double),
+                ("betaKcomm", C.c_double)]
+
+
+class Elastic(C.Structure):
+    _fields_ = [("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double), ("b", C.c_double * 3),
+                ("mass_type", C.c_int32), ("damp", Rayleigh)]
+
+
+class Coeffs(C.Structure):
+    _fields_ = [("stiffness", C.c_double), ("damping", C.c_double), ("mass", C.c_double), ("initial", C.c_int32)]
+
+
+class SolveResult(C.Structure):
+    _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
+                ("rhs_norm", C.c_double), ("initial_residual", C.c_double), ("final_residual", C.c_double)]
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
+        L.fxo_lcg_symmetric.restype = C.c_double
+        L.fxo_lcg_symmetric.argtypes = [C.POINTER(C.c_uint64)]
+        L.fxo_compile.argtypes = [C.POINTER(Model), C.c_int, C.c_int, C.c_int, C.POINTER(Compiled)]
+        L.fxo_compiled_free.argtypes = [C.POINTER(Compiled)]
+        L.fxo_node_bandwidth.restype = C.c_int32
+        L.fxo_node_bandwidth.argtypes = [C.POINTER(Model), i32p]
+        L.fxo_brick_shape.argtypes = [f64p, C.c_int, f64p, C.POINTER(C.c_double)]
+        L.fxo_elastic_tangent.argtypes = [C.c_double, C.c_double, f64p]
+        L.fxo_brick_force_elastic.argtypes = [C.POINTER(Elastic), f64p, f64p, f64p, f64p, C.c_void_p]
+        L.fxo_brick_mass.argtypes = [C.POINTER(Elastic), f64p, f64p]
+        L.fxo_brick_lumped_mass.argtypes = [C.c_double, f64p, f64p]
+        L.fxo_brick_operator_apply.argtypes = [C.POINTER(Elastic), C.POINTER(Coeffs), f64p, f64p, f64p, f64p]
+        L.fxo_brick_operator_diagonal.argtypes = [C.POINTER(Elastic), C.POINTER(Coeffs), f64p, f64p, i32p, f64p]
+        L.fxo_apply.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_int]
+        L.fxo_diagonal.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, C.c_int]
+        L.fxo_lumped_mass.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p]
+        L.fxo_body_load.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p]
+        L.fxo_dot.restype = C.c_double
+        L.fxo_dot.argtypes = [C.c_int64, f64p, f64p, C.c_int]
+        L.fxo_cg.restype = C.c_int
+        L.fxo_cg.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_double,
+                             C.c_double, C.c_int32, C.c_int, C.c_int, C.POINTER(SolveResult)]
+        _LIB = L
+    return _LIB
+
+
+def ref():
+    """The reference's own physics compiled from its headers (oracle/_ref), or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(HERE, "_ref", "libfxref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        vp, i = C.c_void_p, C.c_int
+        R.fxref_brick_force.argtypes = [vp, vp, i, vp, vp, vp, vp, vp, vp]
+        R.fxref_brick_mass.argtypes = [vp, i, vp, vp]
+        R.fxref_operator.argtypes = [vp, i, vp, vp, i, vp, vp, vp, i, vp]
+        R.fxref_pimy_setup.argtypes = [vp, i, vp, vp]
+        R.fxref_pimy_initial_state.argtypes = [vp, vp]
+        R.fxref_pimy_update.argtypes = [vp, i, vp, vp, vp, vp, vp]
+        _REF = R
+    return _REF
+
+
+def lcg_sequence(seed, n):
+    s = C.c_uint64(seed)
+    L = lib()
+    return np.array([L.fxo_lcg_symmetric(C.byref(s)) for _ in range(n)])
+
+
+def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
+    return Coeffs(stiffness, damping, mass, int(initial))
+
+
+def elastic_params(params):
+    """params: list of dicts with E, nu, rho, b (3), mass_type, rayleigh (4) -> ctypes array."""
+    arr = (Elastic * max(1, len(params)))()
+    for i, p in enumerate(params):
+        b = p.get("b", (0.0, 0.0, 0.0))
+        r = p.get("rayleigh", (0.0, 0.0, 0.0, 0.0))
+        arr[i] = Elastic(p["E"], p["nu"], p.get("rho", 0.0), (C.c_double * 3)(*b), int(p.get("mass_type", 0)),
+                         Rayleigh(*r))
+    return arr
+
+
+class OracleMesh:
+    """A compiled model (fxo_compile) with numpy views of its arrays."""
+
+    def __init__(self, model, renumber=True, simd=64, coloring=True):
+        self._keep = []
+
+        def ptr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            self._keep.append(a)
+            return a.ctypes.data
+
+        m = Model()
+        m.nnode = model["nnode"]
+        m.xyz = ptr(model["xyz"], np.float64)
+        m.nelem = model["nelem"]
+        m.conn = ptr(model["conn"], np.int32)
+        m.group = ptr(model.get("group"), np.int32)
+        m.region = ptr(model.get("region"), np.int32)
+        fix = model.get("fix", np.zeros((0, 2), np.int32))
+        tie = model.get("tie", np.zeros((0, 3), np.int32))
+        m.nfix = len(fix)
+        m.fix_node = ptr(np.asarray(fix)[:, 0] if len(fix) else np.zeros(1), np.int32)
+        m.fix_dof = ptr(np.asarray(fix)[:, 1] if len(fix) else np.zeros(1), np.int32)
+        m.ntie = len(tie)
+        m.tie_retained = ptr(np.asarray(tie)[:, 0] if len(tie) else np.zeros(1), np.int32)
+        m.tie_constrained = ptr(np.asarray(tie)[:, 1] if len(tie) else np.zeros(1), np.int32)
+        m.tie_dof = ptr(np.asarray(tie)[:, 2] if len(tie) else np.zeros(1), np.int32)
+        self.model = m
+        self.c = Compiled()
+        rc = lib().fxo_compile(C.byref(m), int(renumber), int(simd), int(coloring), C.byref(self.c))
+        if rc != 0:
+            raise RuntimeError("fxo_compile failed")
+        c = self.c
+        N, P, E = c.nnode, c.padded, c.nelem
+        cp = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy() if n > 0 else np.zeros(0)
+        self.nnode, self.nelem, self.padded, self.neq, self.nparam = N, E, P, c.neq, c.nparam
+        self.node_order = cp(c.node_order, N)
+        self.node_rank = cp(c.node_rank, N)
+        self.elem_order = cp(c.elem_order, E)
+        self.node_eq = cp(c.node_eq, 3 * N)
+        self.coords = cp(c.coords, 3 * N)
+        self.node = cp(c.node, 8 * P)
+        self.eq = cp(c.eq, 24 * P)
+        self.param = cp(c.param, P)
+        self.param_group = cp(c.param_group, c.nparam)
+        self.param_region = cp(c.param_region, c.nparam)
+        self.ncolor = c.ncolor
+        self.color_start = cp(c.color_start, c.ncolor + 1) if coloring else None
+        self.color_elements = cp(c.color_elements, E) if coloring else None
+
+    def __del__(self):
+        try:
+            lib().fxo_compiled_free(C.byref(self.c))
+        except Exception:
+            pass
+
+    def bandwidth(self):
+        return lib().fxo_node_bandwidth(C.byref(self.model), self.node_rank.astype(np.int32))
+
+    def apply(self, params, x, k=None, nthreads=0):
+        y = np.zeros(self.neq)
+        lib().fxo_apply(C.byref(self.c), C.cast(params, C.c_void_p), C.byref(k or coeffs()),
+                        np.ascontiguousarray(x, np.float64), y, nthreads)
+        return y
+
+    def diagonal(self, params, k=None, nthreads=0):
+        d = np.zeros(self.neq)
+        lib().fxo_diagonal(C.byref(self.c), C.cast(params, C.c_void_p), C.byref(k or coeffs()), d, nthreads)
+        return d
+
+    def lumped_mass(self, params):
+        m = np.zeros(self.neq)
+        am = np.zeros(self.neq)
+        lib().fxo_lumped_mass(C.byref(self.c), C.cast(params, C.c_void_p), m, am)
+        return m, am
+
+    def body_load(self, params):
+        b = np.zeros(self.neq)
+        lib().fxo_body_load(C.byref(self.c), C.cast(params, C.c_void_p), b)
+        return b
+
+    def cg(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, lanes=1, k=None, nthreads=0):
+        x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)
+        r = SolveResult()
+        rc = lib().fxo_cg(C.byref(self.c), C.cast(params, C.c_void_p), C.byref(k or coeffs()),
+                          np.ascontiguousarray(b, np.float64), x, rtol, atol, max_it, lanes, nthreads, C.byref(r))
+        if rc != 0:
+            raise RuntimeError("Jacobi: zero diagonal entries")
+        return x, {f: getattr(r, f) for f, _ in SolveResult._fields_}
+
+
+def dot(x, y, lanes=1):
+    x = np.ascontiguousarray(x, np.float64)
+    y = np.ascontiguousarray(y, np.float64)
+    return lib().fxo_dot(len(x), x, y, lanes)
