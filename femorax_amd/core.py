@@ -17,6 +17,9 @@ _LIB = None
 FX_OK = 0
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP"}
 FX_BATCH_FP32 = 1
+FX_BATCH_GENERAL = 2
+FX_BATCH_XCD_SLABS = 8
+FX_BATCH_NO_TUNE = 16
 
 
 class FxError(RuntimeError):
@@ -58,8 +61,8 @@ class BatchDesc(C.Structure):
 
 
 class BatchInfo(C.Structure):
-    _fields_ = [(n, C.c_int32) for n in ("count", "neq", "nnode", "nblocks", "ncolors", "tied", "affine_classes",
-                                         "shape_classes")] + [("device_bytes", C.c_uint64)]
+    _fields_ = [(n, C.c_int32) for n in ("count", "neq", "nnode", "nblocks", "ncolors", "tied", "lattice",
+                                         "tiles", "variant")] + [("device_bytes", C.c_uint64)]
 
 
 class SolveOptions(C.Structure):
@@ -96,10 +99,12 @@ SIGNATURES = [
     ("fx_batch_create", C.c_int, [vp, C.POINTER(BatchDesc), u32, C.POINTER(vp)]),
     ("fx_batch_destroy", None, [vp]),
     ("fx_batch_info_get", C.c_int, [vp, C.POINTER(BatchInfo)]),
+    ("fx_batch_tune", C.c_int, [vp, C.c_int]),
     ("fx_operator_apply", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
     ("fx_operator_apply_f32", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
     ("fx_operator_diagonal", C.c_int, [vp, C.POINTER(Coeffs), vp]),
     ("fx_operator_time", C.c_int, [vp, C.POINTER(Coeffs), C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
+    ("fx_copy", C.c_int, [vp, i64, vp, vp]),
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
     ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
@@ -304,7 +309,7 @@ def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
 class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
-    def __init__(self, ctx, mesh, params, fp32=False):
+    def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -321,7 +326,9 @@ class Batch:
         d.coords = _arr(keep, mesh.coords, np.float64)
         d.neq = mesh.neq
         h = C.c_void_p()
-        _check(lib().fx_batch_create(ctx.h, C.byref(d), FX_BATCH_FP32 if fp32 else 0, C.byref(h)), ctx)
+        flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
+            (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE)
+        _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
 
     def close(self):
@@ -339,6 +346,10 @@ class Batch:
         i = BatchInfo()
         _check(lib().fx_batch_info_get(self.h, C.byref(i)), self.ctx)
         return {f: getattr(i, f) for f, _ in i._fields_}
+
+    def tune(self, variant=-1):
+        _check(lib().fx_batch_tune(self.h, variant), self.ctx)
+        return self.info()["variant"]
 
     def apply(self, x, y, k=None):
         _check(lib().fx_operator_apply(self.h, C.byref(k or coeffs()), x.ptr, y.ptr), self.ctx)
@@ -367,6 +378,10 @@ class Batch:
         _check(lib().fx_cg_mixed(self.h, C.byref(k or coeffs()), C.byref(o), inner_rtol, b.ptr, x.ptr, C.byref(r)),
                self.ctx)
         return r.as_dict()
+
+
+def copy(ctx, src, dst):
+    _check(lib().fx_copy(ctx.h, src.n, src.ptr, dst.ptr), ctx)
 
 
 def dot(ctx, x, y):

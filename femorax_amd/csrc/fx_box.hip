@@ -1,0 +1,504 @@
+// Structured-lattice hex8 operator: one workgroup per tile of 8 x 8 x 4 axis-aligned bricks.
+//
+// The element operator of stdBrick (element_operator_apply, element_operator.hpp:82-130;
+// shp3d at the 2x2x2 Gauss points, std_brick.hpp:45-152) for an axis-aligned box is a tensor
+// product: the shape function derivative along x at Gauss point (q1, q2, q3) is
+// +-1/hx * I[q2][j] * I[q3][k] with I the 2x2 matrix of the 1D linear basis at the two Gauss
+// points. Sum factorisation then evaluates B x at the 8 points with 4 differences and two
+// 2-point interpolations per (component, direction) and B^T sigma with the transposed steps,
+// about 600 FP64 operations per element instead of the ~6000 of the generic form. D_eff and
+// dvol are per parameter set and per lattice; the geometry is three numbers (hx, hy, hz).
+//
+// Memory per tile: one 32-bit word per lattice node (first equation, free-DOF mask, shared
+// flag), a 16-bit parameter index per cell, x at the tile's nodes, y written once per node.
+// The element results are summed per node inside the tile with lane shuffles (x and y within a
+// wave, one wave per z-plane of cells) and one LDS exchange between neighbouring planes, in a
+// fixed order. One launch runs every tile with no inter-workgroup dependency: a node that only
+// its tile touches gets its final y there; for a node on a tile face the tile stores its sum in
+// its own partial-sum slot, and a second kernel adds a shared node's slots in tile parity-colour
+// order (tiles of one colour share no node). The sums are deterministic: the same bits for
+// every run, tile placement and co-runner.
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+template <typename T>
+struct BoxConst {
+  T w0[3];   // 8 kappa_d, kappa_d = 1 / (4 h_d): the physical gradient of a linear mode is kappa_d (Hv)
+  T w1[3];   // 8/3 kappa_d
+  T kap[3];  // kappa_d
+  T kk[3];   // 8/9 kappa_d^2
+  T kks;     // 8/9 sum_d kappa_d^2
+};
+
+template <typename T>
+struct BoxArgs {
+  const TileHeader* tiles;
+  const uint32_t* words;
+  const uint16_t* cell_param;
+  const DevParam* params;
+  fx_coeffs k;
+  double hx, hy, hz;
+  BoxConst<T> K;
+  const T* x;
+  T* y;
+  T* partial;  // [tiles][kTileNodes][3]: this tile's sums at its shared nodes
+  int ntiles;
+  int accumulate;
+  const int32_t* done;
+};
+
+// The element operator in the modal basis of the trilinear field.
+//
+// With corner values v[q], q = i + 2j + 4k, the field is u = sum_m a_m phi_m over the modes
+// phi_m = xi^mx eta^my zeta^mz (m = mx + 2my + 4mz), a = H v / 8 with H the 8 x 8 Hadamard
+// matrix (one sum/difference butterfly per axis). On an axis-aligned box the gradient of each
+// mode is a polynomial whose terms are orthogonal under the 2 x 2 x 2 Gauss rule, which
+// integrates every term of the stdBrick stiffness and consistent mass exactly. The energy then
+// splits into seven small independent pieces: the constant-strain part (the three linear modes
+// of each component), one part per axis p with the bilinear modes that vary along p, and the
+// trilinear modes (diagonal), and the consistent mass is diagonal in this basis. K_e v is the
+// transposed butterflies of the modal gradient g = dE/d(Hv): about 250 FP64 operations per
+// element where B^T D B at the Gauss points takes about 600, with the same result up to rounding.
+
+template <typename T>
+__device__ __forceinline__ void hadamard8(T (&v)[8]) {
+#pragma unroll
+  for (int bit = 1; bit < 8; bit <<= 1)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (!(q & bit)) {
+        const T a = v[q], b = v[q | bit];
+        v[q] = a + b;
+        v[q | bit] = b - a;
+      }
+}
+
+template <typename T>
+__device__ __forceinline__ void hadamard8_transposed(T (&g)[8]) {
+#pragma unroll
+  for (int bit = 4; bit >= 1; bit >>= 1)
+#pragma unroll
+    for (int q = 0; q < 8; ++q)
+      if (!(q & bit)) {
+        const T a = g[q], b = g[q | bit];
+        g[q] = a - b;
+        g[q | bit] = a + b;
+      }
+}
+
+// y[c][q] = (K_e v)[c][q] (+ the mass term), v[c][q] from the tile's node array in LDS (xs[c][ln],
+// ln[q] the tile node of corner q). lam, mu carry dvol and the operator factor; mc = cM rho dvol
+// (MASS only; lumped: the reference's lumped mass, else consistent).
+template <typename T, bool MASS>
+__device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam, T mu, T mc, bool lumped,
+                                          const BoxConst<T>& K, T (&y)[3][8]) {
+  T b[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * kTileNodes + ln[q]];
+    hadamard8(b[c]);
+  }
+  const T m2 = mu + mu;
+  // constant strain: F[d][c] = kappa_d b[c][e_d]; g[c][e_d] = 8 kappa_d sigma_dc
+  {
+    T F[3][3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[d][c] = K.kap[d] * b[c][1 << d];
+    const T ltr = lam * (F[0][0] + F[1][1] + F[2][2]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) y[d][1 << d] = K.w0[d] * (ltr + m2 * F[d][d]);
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int c = d + 1; c < 3; ++c) {
+        const T s = mu * (F[d][c] + F[c][d]);
+        y[c][1 << d] = K.w0[d] * s;
+        y[d][1 << c] = K.w0[c] * s;
+      }
+  }
+  // the bilinear modes that vary along p: rows d1, d2 (the other axes) of the gradient's p-linear
+  // term, F[d][c] = kappa_d b[c][e_d | e_p]; g[c][e_d | e_p] gets 8/3 kappa_d sigma_dc. Mode
+  // (d, p) hears from p and from d; the smaller of the two assigns, the other adds.
+#pragma unroll
+  for (int p = 0; p < 3; ++p) {
+    const int d1 = p == 0 ? 1 : 0, d2 = p == 2 ? 1 : 2;
+    const int n1 = (1 << d1) | (1 << p), n2 = (1 << d2) | (1 << p);
+    T F1[3], F2[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      F1[c] = K.kap[d1] * b[c][n1];
+      F2[c] = K.kap[d2] * b[c][n2];
+    }
+    const T ltr = lam * (F1[d1] + F2[d2]);
+    const T s12 = mu * (F1[d2] + F2[d1]);
+    const T r1[3] = {d1 == 0 ? ltr + m2 * F1[0] : (p == 0 ? mu * F1[0] : s12),
+                     d1 == 1 ? ltr + m2 * F1[1] : (p == 1 ? mu * F1[1] : s12),
+                     p == 2 ? mu * F1[2] : s12};
+    const T r2[3] = {p == 0 ? mu * F2[0] : s12, d2 == 1 ? ltr + m2 * F2[1] : (p == 1 ? mu * F2[1] : s12),
+                     d2 == 2 ? ltr + m2 * F2[2] : (p == 2 ? mu * F2[2] : s12)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T u1 = K.w1[d1] * r1[c], u2 = K.w1[d2] * r2[c];
+      y[c][n1] = p < d1 ? u1 : y[c][n1] + u1;
+      y[c][n2] = p < d2 ? u2 : y[c][n2] + u2;
+    }
+  }
+  // trilinear modes: g[c][7] = 8/9 b[c][7] (kappa_c^2 (lam + 2 mu) + mu sum_{d != c} kappa_d^2)
+  {
+    const T lpm = lam + mu, t = mu * K.kks;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c][7] = (K.kk[c] * lpm + t) * b[c][7];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) y[c][0] = T(0);
+  if (MASS && !lumped) {
+    // consistent mass: g[c][m] += mc / (8 3^|m|) b[c][m]
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int w = (m & 1) + ((m >> 1) & 1) + ((m >> 2) & 1);
+      const T f = mc * T(w == 0 ? 1.0 / 8.0 : w == 1 ? 1.0 / 24.0 : w == 2 ? 1.0 / 72.0 : 1.0 / 216.0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) y[c][m] += f * b[c][m];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) hadamard8_transposed(y[c]);
+  if (MASS && lumped) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) y[c][q] += mc * xs[c * kTileNodes + ln[q]];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_up(T v, int d) {
+  return __shfl_up(v, static_cast<unsigned>(d), 64);
+}
+
+constexpr int kPlane = (kTileX + 1) * (kTileY + 1);  // nodes of one z-plane of a tile
+
+// What one thread fetches of a tile before its cells are computed: the words of tile nodes t
+// and t + kTileCells, x at their free DOFs, and its cell's parameter set.
+template <typename T>
+struct TileFetch {
+  uint32_t w[2];
+  T v[2][3];
+  int pidx;
+};
+
+template <typename T, bool DIAG>
+__device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, const TileHeader& h, int t, TileFetch<T>& f) {
+  const int nx = h.nx, ny = h.ny, nz = h.nz;
+  const int nnodes = (nx + 1) * (ny + 1) * (nz + 1);
+  const uint32_t* words = a.words + h.word_offset;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int ln = t + r * kTileCells;
+    f.w[r] = ln < nnodes ? words[ln] : 0u;
+  }
+  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY);
+  f.pidx = (cx < nx && cy < ny && cz < nz) ? a.cell_param[h.cell_offset + cx + nx * (cy + ny * cz)] : 0xffff;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t mask = DIAG ? 0u : (f.w[r] >> 28) & 7u;  // 0 past the tile's nodes
+    const int base = static_cast<int>(f.w[r] & 0x0fffffffu);
+    int j = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      f.v[r][d] = (mask & (1u << d)) ? a.x[base + j] : T(0);
+      j += (mask >> d) & 1u;
+    }
+  }
+}
+
+// MODE: 0 apply without mass terms, 1 apply with mass terms, 2 diagonal. The LDS holds the
+// tile's words and x; the cell's results are summed per node and committed.
+template <typename T, int MODE>
+__device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const TileHeader& h, int pidx, T* lds,
+                                         const uint32_t* wsh, int t) {
+  constexpr bool DIAG = MODE == 2;
+  const int nx = h.nx, ny = h.ny, nz = h.nz;
+  const int NX = nx + 1, NY = ny + 1;
+  const T* xs = lds;
+  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY);
+  T out[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) out[c][q] = T(0);
+  if (pidx != 0xffff) {
+    const DevParam& p = a.params[pidx];
+    double fD, cM;
+    operator_factors(p, a.k, fD, cM);
+    const double dvol = a.hx * a.hy * a.hz * 0.125;
+    if constexpr (DIAG) {
+      // diag(K_e): Gauss sums of (dN/dx_d)^2 are 8 / (9 h_d^2) at every node of a box; of N^2, 8/27
+      const double S[3] = {8.0 / (9.0 * a.hx * a.hx), 8.0 / (9.0 * a.hy * a.hy), 8.0 / (9.0 * a.hz * a.hz)};
+      const double lam = fD * p.lam, mu = fD * p.mu;
+      const double mass = cM * p.rho * dvol * (p.mass_type == 0 ? 8.0 / 27.0 : 1.0);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double dc = dvol * ((lam + 2.0 * mu) * S[c] + mu * (S[0] + S[1] + S[2] - S[c])) + mass;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[c][q] = static_cast<T>(dc);
+      }
+    } else {
+      int ln[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * ((cy + ((q >> 1) & 1)) + NY * (cz + (q >> 2)));
+      box_apply<T, MODE == 1>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
+                              static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
+    }
+  }
+  // in-plane sums: node (cx, cy) of the cell's bottom (k = 0) and top (k = 1) faces, plus the
+  // x = nx column, y = ny row and their corner for the last cells. Cells (cx-1, cy), (cx, cy-1)
+  // and (cx-1, cy-1) are lanes t-1, t-8 and t-9 of the same wave. Top-face sums go to LDS (the
+  // plane above reads them), bottom-face sums stay in registers.
+  const bool xm = cx > 0, ym = cy > 0;
+  const bool xe = cx == nx - 1, ye = cy == ny - 1;
+  const bool own = cx < nx && cy < ny && cz < nz;
+  const int pi[4] = {cx + (kTileX + 1) * cy, (cx + 1) + (kTileX + 1) * cy, cx + (kTileX + 1) * (cy + 1),
+                     (cx + 1) + (kTileX + 1) * (cy + 1)};
+  const bool has[4] = {own, own && xe, own && ye, own && xe && ye};
+  __syncthreads();  // every corner read of xs is done: the LDS takes each plane's top-face sums
+  T* top = lds;     // [kTileZ][kPlane][3]
+  T F0[3][4];       // bottom face: [component][main, x edge, y edge, corner]
+#pragma unroll
+  for (int k = 1; k >= 0; --k)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const T o00 = out[c][4 * k], o10 = out[c][1 + 4 * k], o01 = out[c][2 + 4 * k], o11 = out[c][3 + 4 * k];
+      const T s10 = shfl_up(o10, 1), s01 = shfl_up(o01, kTileX), s11 = shfl_up(o11, kTileX + 1);
+      const T sx = shfl_up(o11, kTileX), sy = shfl_up(o11, 1);
+      T m = o00;
+      if (xm) m += s10;
+      if (ym) m += s01;
+      if (xm && ym) m += s11;
+      const T f[4] = {m, ym ? o10 + sx : o10, xm ? o01 + sy : o01, o11};
+      if (k == 1) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          if (has[r]) top[(cz * kPlane + pi[r]) * 3 + c] = f[r];
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) F0[c][r] = f[r];
+      }
+    }
+  __syncthreads();
+  // commit: plane z = cz (bottom face + the plane below's top face), and for the last plane of
+  // cells also plane z = nz (its own top face)
+  T* part = a.partial + static_cast<size_t>(tile) * (kTileNodes * 3);
+  auto commit = [&](int px, int py, int pz, const T (&val)[3]) {
+    const int ln = px + NX * (py + NY * pz);
+    const uint32_t w = wsh[ln];
+    if (w & 0x80000000u) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) part[ln * 3 + d] = val[d];
+      return;
+    }
+    const uint32_t mask = (w >> 28) & 7u;
+    const int base = static_cast<int>(w & 0x0fffffffu);
+    int j = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (!(mask & (1u << d))) continue;
+      T* dst = a.y + base + j++;
+      *dst = a.accumulate ? *dst + val[d] : val[d];
+    }
+  };
+  const int qx[4] = {cx, cx + 1, cx, cx + 1}, qy[4] = {cy, cy, cy + 1, cy + 1};
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    if (!has[r]) continue;
+    T val[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[((cz - 1) * kPlane + pi[r]) * 3 + c] : F0[c][r];
+    commit(qx[r], qy[r], cz, val);
+    if (cz == nz - 1) {
+      T up[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) up[c] = top[(cz * kPlane + pi[r]) * 3 + c];
+      commit(qx[r], qy[r], cz + 1, up);
+    }
+  }
+}
+
+// One workgroup per tile (PERSIST: each workgroup walks tiles blockIdx.x, + gridDim.x, ...
+// and fetches the next tile's words and x while it computes the current one). WAVES: the
+// minimum waves per SIMD the register allocation must allow (0: the compiler's choice).
+template <typename T, int MODE, int WAVES, bool PERSIST>
+__global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_kernel(BoxArgs<T> a) {
+  constexpr bool DIAG = MODE == 2;
+  __shared__ T lds[3 * kTileNodes];  // x at the tile's nodes, then the top-face partial sums
+  __shared__ uint32_t wsh[kTileNodes];
+  const int t = threadIdx.x;
+  if (a.done && *a.done) return;
+  int tile = blockIdx.x;
+  TileHeader h = a.tiles[tile];
+  TileFetch<T> f;
+  fetch_tile<T, DIAG>(a, h, t, f);
+  for (;;) {
+    const int nnodes = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+      const int ln = t + r * kTileCells;
+      if (ln < nnodes) {
+        wsh[ln] = f.w[r];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) lds[d * kTileNodes + ln] = f.v[r][d];
+      }
+    }
+    __syncthreads();
+    const int next = PERSIST ? tile + static_cast<int>(gridDim.x) : a.ntiles;
+    TileHeader hn = h;
+    TileFetch<T> fn = f;
+    if (PERSIST && next < a.ntiles) {
+      hn = a.tiles[next];
+      fetch_tile<T, DIAG>(a, hn, t, fn);
+    }
+    box_tile<T, MODE>(a, tile, h, f.pidx, lds, wsh, t);
+    if (next >= a.ntiles) break;
+    __syncthreads();  // the LDS takes the next tile
+    tile = next;
+    h = hn;
+    f = fn;
+  }
+}
+
+// y at the nodes several tiles touch: their tiles' partial sums added in colour order. One
+// thread per shared node; the shared nodes' equations are consecutive in the internal numbering.
+template <typename T>
+__global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __restrict__ word,
+                                                              const int32_t* __restrict__ start,
+                                                              const int32_t* __restrict__ list,
+                                                              const T* __restrict__ partial, T* __restrict__ y,
+                                                              int n, int accumulate, const int32_t* done) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (done && *done) return;
+  const uint32_t w = word[i];
+  const uint32_t mask = (w >> 28) & 7u;
+  const int base = static_cast<int>(w & 0x0fffffffu);
+  const int s0 = start[i], s1 = start[i + 1];
+  T acc[3];
+  {
+    const T* p = partial + list[s0];
+    int j = 0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      acc[d] = p[d];
+      if (accumulate && (mask & (1u << d))) acc[d] = y[base + j] + acc[d];
+      j += (mask >> d) & 1u;
+    }
+  }
+  for (int s = s0 + 1; s < s1; ++s) {
+    const T* p = partial + list[s];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += p[d];
+  }
+  int j = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (mask & (1u << d)) y[base + j++] = acc[d];
+}
+
+namespace {
+// Workgroups of a tile kernel resident at once on the device (blocks per CU x CUs), cached.
+int resident_tiles(fx_ctx* ctx, const void* kernel) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void*>, int> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_pair(ctx->device, kernel);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, ncu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTileCells, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return cache[key] = per_cu * ncu;
+}
+}  // namespace
+
+template <typename T, bool DIAG>
+int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done) {
+  // the mass-free kernel unless some parameter set has a mass factor cM' = cM + cC alphaM
+  bool mass = false;
+  for (const DevParam& p : b->host_params) {
+    double fD, cM;
+    operator_factors(p, k, fD, cM);
+    mass |= cM != 0.0;
+  }
+  // variant bit 0: 5 waves per SIMD (mass-free kernel only: the mass kernel would spill heavily);
+  // bit 1: persistent workgroups with prefetch
+  using Kernel = void (*)(BoxArgs<T>);
+  static const Kernel table[3][4] = {
+      {hex8_box_kernel<T, 0, 0, false>, hex8_box_kernel<T, 0, 5, false>, hex8_box_kernel<T, 0, 0, true>,
+       hex8_box_kernel<T, 0, 5, true>},
+      {hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, true>,
+       hex8_box_kernel<T, 1, 0, true>},
+      {hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, true>,
+       hex8_box_kernel<T, 2, 0, true>}};
+  const int v = b->box_variant & 3;
+  const Kernel kernel = table[DIAG ? 2 : mass ? 1 : 0][v];
+  int grid = b->ntiles;
+  if (v & 2) {
+    const int resident = resident_tiles(b->ctx, reinterpret_cast<const void*>(kernel));
+    if (resident > 0 && resident < grid) grid = resident >= 8 ? resident / 8 * 8 : resident;
+  }
+  BoxArgs<T> a{};
+  a.tiles = b->tiles;
+  a.words = b->words;
+  a.cell_param = b->cell_param;
+  a.params = b->params;
+  a.k = k;
+  a.hx = b->hx;
+  a.hy = b->hy;
+  a.hz = b->hz;
+  const double kap[3] = {0.25 / b->hx, 0.25 / b->hy, 0.25 / b->hz};
+  for (int d = 0; d < 3; ++d) {
+    a.K.kap[d] = static_cast<T>(kap[d]);
+    a.K.w0[d] = static_cast<T>(8.0 * kap[d]);
+    a.K.w1[d] = static_cast<T>(8.0 / 3.0 * kap[d]);
+    a.K.kk[d] = static_cast<T>(8.0 / 9.0 * kap[d] * kap[d]);
+  }
+  a.K.kks = static_cast<T>(8.0 / 9.0 * (kap[0] * kap[0] + kap[1] * kap[1] + kap[2] * kap[2]));
+  a.x = x;
+  a.y = y;
+  if constexpr (sizeof(T) == 8) {
+    a.partial = b->partial64;
+  } else {
+    a.partial = b->partial32;
+  }
+  a.ntiles = b->ntiles;
+  a.accumulate = accumulate;
+  a.done = done;
+  if (!a.partial) {
+    b->ctx->error = "operator: no partial-sum buffer for this precision";
+    return FX_EINVAL;
+  }
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kTileCells), 0, b->ctx->stream, a);
+  if (b->nshared > 0)
+    hipLaunchKernelGGL(hex8_shared_sum_kernel<T>, dim3((b->nshared + 255) / 256), dim3(256), 0, b->ctx->stream,
+                       b->shared_word, b->shared_start, b->shared_list, a.partial, y, b->nshared, accumulate, done);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+template int launch_box<double, false>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*);
+template int launch_box<double, true>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*);
+template int launch_box<float, false>(fx_batch*, const fx_coeffs&, const float*, float*, int, const int32_t*);
+
+}  // namespace fx

@@ -187,7 +187,9 @@ void fx_batch_destroy(fx_batch* b) {
   (void)hipStreamSynchronize(ctx->stream);
   void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
-                  b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32};
+                  b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
+                  b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
+                  b->shared_list, b->partial64, b->partial32, b->perm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -252,38 +254,81 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     q.betaKcomm = p.damp.betaKcomm;
     b->host_params.push_back(q);
   }
+  // equations no element touches (zeroed after each apply, as the reference's gather does)
+  std::vector<char> covered(static_cast<size_t>(d->neq) + 1, 0);
+  for (int32_t e = 0; e < E; ++e)
+    for (int k = 0; k < 24; ++k) {
+      const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
+      if (q >= 0) covered[q] = 1;
+    }
+  std::vector<int32_t> orphans;
+  for (int32_t q = 0; q < d->neq; ++q)
+    if (!covered[q]) orphans.push_back(q);
+  fx::Lattice lat;
+  const bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
+                           fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
+                                             (flags & FX_BATCH_XCD_SLABS) != 0, lat);
+  if (use_lattice) {  // orphans in the internal numbering
+    std::vector<int32_t> internal_orphans;
+    for (int32_t i = 0; i < d->neq; ++i)
+      if (!covered[lat.perm[i]]) internal_orphans.push_back(i);
+    orphans.swap(internal_orphans);
+  }
   fx::Blocks blk;
-  fx::build_blocks(E, P, d->node, N, d->coords, blk);
   fx::Contraction con;
-  fx::build_contraction(N, d->neq, node_eq.data(), con);
+  if (!use_lattice) {
+    fx::build_blocks(E, P, d->node, N, d->coords, blk);
+    fx::build_contraction(N, d->neq, node_eq.data(), con);
+  }
   b->tied = con.tied;
+  b->lattice = use_lattice;
   uint64_t& by = b->device_bytes;
   int rc = FX_OK;
   auto ok = [&](int r) {
     if (r != FX_OK && rc == FX_OK) rc = r;
     return rc == FX_OK;
   };
-  std::vector<float> c32(d->coords, d->coords + static_cast<size_t>(3) * N);
-  ok(dev_upload(ctx, b->node, d->node, static_cast<size_t>(8) * P, by)) &&
-      ok(dev_upload(ctx, b->param, d->param, static_cast<size_t>(P), by)) &&
-      ok(dev_upload(ctx, b->node_eq, node_eq.data(), node_eq.size(), by)) &&
-      ok(dev_upload(ctx, b->coords, d->coords, static_cast<size_t>(3) * N, by)) &&
-      ok(b->has_f32 ? dev_upload(ctx, b->coords32, c32.data(), c32.size(), by) : FX_OK) &&
-      ok(dev_upload(ctx, b->params, b->host_params.data(), b->host_params.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.elems, blk.elems.data(), blk.elems.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.ln_start, blk.ln_start.data(), blk.ln_start.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.lnode, blk.lnode.data(), blk.lnode.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.corner, blk.corner.data(), blk.corner.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.ln_slots, blk.ln_slots.data(), blk.ln_slots.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.slot_list, blk.slot_list.data(), blk.slot_list.size(), by)) &&
-      ok(dev_upload(ctx, b->blk.colour_blocks, blk.colour_blocks.data(), blk.colour_blocks.size(), by)) &&
-      ok(dev_zeros(ctx, b->work, static_cast<size_t>(6) * std::max(1, d->neq), by)) &&
-      ok(b->has_f32 ? dev_zeros(ctx, b->work32, static_cast<size_t>(7) * std::max(1, d->neq), by) : FX_OK);
-  if (rc == FX_OK && b->tied) {
-    ok(dev_zeros(ctx, b->ynode, static_cast<size_t>(3) * N, by)) &&
-        ok(b->has_f32 ? dev_zeros(ctx, b->ynode32, static_cast<size_t>(3) * N, by) : FX_OK) &&
-        ok(dev_upload(ctx, b->con_start, con.start.data(), con.start.size(), by)) &&
-        ok(dev_upload(ctx, b->con_list, con.list.data(), con.list.size(), by));
+  ok(dev_upload(ctx, b->params, b->host_params.data(), b->host_params.size(), by)) &&
+      ok(dev_zeros(ctx, b->work, static_cast<size_t>(8) * std::max(1, d->neq), by)) &&
+      ok(b->has_f32 ? dev_zeros(ctx, b->work32, static_cast<size_t>(9) * std::max(1, d->neq), by) : FX_OK) &&
+      ok(orphans.empty() ? FX_OK : dev_upload(ctx, b->orphans, orphans.data(), orphans.size(), by));
+  b->norphans = static_cast<int32_t>(orphans.size());
+  if (use_lattice) {
+    b->hx = lat.hx;
+    b->hy = lat.hy;
+    b->hz = lat.hz;
+    b->ntiles = static_cast<int32_t>(lat.tiles.size());
+    b->internal = true;
+    b->nshared = static_cast<int32_t>(lat.shared_word.size());
+    ok(dev_upload(ctx, b->tiles, lat.tiles.data(), lat.tiles.size(), by)) &&
+        ok(dev_upload(ctx, b->words, lat.words.data(), lat.words.size(), by)) &&
+        ok(dev_upload(ctx, b->cell_param, lat.cell_param.data(), lat.cell_param.size(), by)) &&
+        ok(lat.shared_word.empty() ? FX_OK : dev_upload(ctx, b->shared_word, lat.shared_word.data(), lat.shared_word.size(), by)) &&
+        ok(dev_upload(ctx, b->shared_start, lat.shared_start.data(), lat.shared_start.size(), by)) &&
+        ok(lat.shared_list.empty() ? FX_OK : dev_upload(ctx, b->shared_list, lat.shared_list.data(), lat.shared_list.size(), by)) &&
+        ok(dev_zeros(ctx, b->partial64, static_cast<size_t>(b->ntiles) * fx::kTileNodes * 3, by)) &&
+        ok(b->has_f32 ? dev_zeros(ctx, b->partial32, static_cast<size_t>(b->ntiles) * fx::kTileNodes * 3, by) : FX_OK) &&
+        ok(dev_upload(ctx, b->perm, lat.perm.data(), lat.perm.size(), by));
+  } else {
+    std::vector<float> c32(d->coords, d->coords + static_cast<size_t>(3) * N);
+    ok(dev_upload(ctx, b->node, d->node, static_cast<size_t>(8) * P, by)) &&
+        ok(dev_upload(ctx, b->param, d->param, static_cast<size_t>(P), by)) &&
+        ok(dev_upload(ctx, b->node_eq, node_eq.data(), node_eq.size(), by)) &&
+        ok(dev_upload(ctx, b->coords, d->coords, static_cast<size_t>(3) * N, by)) &&
+        ok(b->has_f32 ? dev_upload(ctx, b->coords32, c32.data(), c32.size(), by) : FX_OK) &&
+        ok(dev_upload(ctx, b->blk.elems, blk.elems.data(), blk.elems.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.ln_start, blk.ln_start.data(), blk.ln_start.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.lnode, blk.lnode.data(), blk.lnode.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.corner, blk.corner.data(), blk.corner.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.ln_slots, blk.ln_slots.data(), blk.ln_slots.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.slot_list, blk.slot_list.data(), blk.slot_list.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.colour_blocks, blk.colour_blocks.data(), blk.colour_blocks.size(), by));
+    if (rc == FX_OK && b->tied) {
+      ok(dev_zeros(ctx, b->ynode, static_cast<size_t>(3) * N, by)) &&
+          ok(b->has_f32 ? dev_zeros(ctx, b->ynode32, static_cast<size_t>(3) * N, by) : FX_OK) &&
+          ok(dev_upload(ctx, b->con_start, con.start.data(), con.start.size(), by)) &&
+          ok(dev_upload(ctx, b->con_list, con.list.data(), con.list.size(), by));
+    }
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);
@@ -298,8 +343,63 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   b->info.nblocks = blk.nblocks;
   b->info.ncolors = blk.ncolors;
   b->info.tied = b->tied ? 1 : 0;
+  b->info.lattice = b->lattice ? 1 : 0;
+  if (b->lattice && !(flags & FX_BATCH_NO_TUNE) && E >= 32768) {
+    const int rc2 = fx_batch_tune(b, -1);
+    if (rc2 != FX_OK) {
+      fx_batch_destroy(b);
+      return rc2;
+    }
+  }
+  b->info.variant = b->box_variant;
+  b->info.tiles = static_cast<int32_t>(lat.tiles.size());
   b->info.device_bytes = by;
   *out = b;
+  return FX_OK;
+}
+
+int fx_batch_tune(fx_batch* b, int variant) {
+  if (!b) return fail(nullptr, FX_EINVAL, "fx_batch_tune: null batch");
+  if (variant >= 0) {
+    if (variant > 3) return fail(b->ctx, FX_EINVAL, "fx_batch_tune: variant out of range");
+    b->box_variant = variant;
+    b->info.variant = variant;
+    return FX_OK;
+  }
+  if (!b->lattice) return FX_OK;  // one variant only
+  // time each variant on scratch vectors: a warm-up, then the best of three (the reference's
+  // force-settings autotune, src/exec/force_settings.cpp:135-155)
+  fx_ctx* ctx = b->ctx;
+  const fx_coeffs k{1.0, 0.0, 0.0, 0};
+  double* x = b->work;
+  double* y = b->work + b->neq;
+  hipEvent_t e0, e1;
+  FX_HIP(ctx, hipEventCreate(&e0));
+  FX_HIP(ctx, hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_v = 0;
+  for (int v = 0; v < 4; ++v) {
+    b->box_variant = v;
+    if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
+    float vbest = 1e30f;
+    for (int r = 0; r < 3; ++r) {
+      FX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+      if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
+      FX_HIP(ctx, hipEventRecord(e1, ctx->stream));
+      FX_HIP(ctx, hipEventSynchronize(e1));
+      float ms = 0;
+      FX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+      vbest = std::min(vbest, ms);
+    }
+    if (vbest < best) {
+      best = vbest;
+      best_v = v;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  b->box_variant = best_v;
+  b->info.variant = best_v;
   return FX_OK;
 }
 
@@ -313,9 +413,7 @@ int fx_batch_info_get(const fx_batch* b, fx_batch_info* info) {
 int fx_operator_apply(fx_batch* b, const fx_coeffs* k, const double* x, double* y) {
   if (!b || !k || !x || !y) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_apply: null argument");
   if (x == y) return fail(b->ctx, FX_EINVAL, "fx_operator_apply: x and y alias");
-  if (int rc = fx::operator_apply64(b, *k, x, y)) return rc;
-  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
-  return FX_OK;
+  return fx::external_apply64(b, *k, x, y);
 }
 
 int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float* y) {
@@ -323,16 +421,12 @@ int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float
   if (!b->has_f32) return fail(b->ctx, FX_EUNSUP, "fx_operator_apply_f32: batch created without FX_BATCH_FP32");
   if (static_cast<const void*>(x) == static_cast<const void*>(y))
     return fail(b->ctx, FX_EINVAL, "fx_operator_apply_f32: x and y alias");
-  if (int rc = fx::operator_apply32(b, *k, x, y)) return rc;
-  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
-  return FX_OK;
+  return fx::external_apply32(b, *k, x, y);
 }
 
 int fx_operator_diagonal(fx_batch* b, const fx_coeffs* k, double* d) {
   if (!b || !k || !d) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_diagonal: null argument");
-  if (int rc = fx::operator_diagonal64(b, *k, d)) return rc;
-  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
-  return FX_OK;
+  return fx::external_diagonal64(b, *k, d);
 }
 
 int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void* x, void* y, int reps, float* ms) {
@@ -344,6 +438,7 @@ int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void*
   FX_HIP(ctx, hipEventCreate(&e1));
   FX_HIP(ctx, hipEventRecord(e0, ctx->stream));
   for (int i = 0; i < reps; ++i) {
+    // the operator as the solvers run it (internal numbering of lattice batches)
     const int rc = precision == 32
                        ? fx::operator_apply32(b, *k, static_cast<const float*>(x), static_cast<float*>(y))
                        : fx::operator_apply64(b, *k, static_cast<const double*>(x), static_cast<double*>(y));
@@ -355,6 +450,11 @@ int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void*
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   return FX_OK;
+}
+
+int fx_copy(fx_ctx* ctx, int64_t n, const double* src, double* dst) {
+  if (!ctx || (n > 0 && (!src || !dst))) return fail(ctx, FX_EINVAL, "fx_copy: null argument");
+  return fx::copy64(ctx, n, src, dst);
 }
 
 int fx_dot(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out) {

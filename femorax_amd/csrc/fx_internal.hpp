@@ -45,6 +45,35 @@ struct Blocks {
 void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nnode, const double* coords,
                   Blocks& out);
 
+// Structured fast path: tiles of kTileX x kTileY x kTileZ lattice cells (fx_lattice.cpp).
+constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
+constexpr int kTileCells = kTileX * kTileY * kTileZ;
+constexpr int kTileNodes = (kTileX + 1) * (kTileY + 1) * (kTileZ + 1);
+static_assert(kTileCells == kBlockElems, "one cell per thread");
+
+struct TileHeader {
+  int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, kTileZ)
+  int32_t word_offset;      // into the node words: (nx+1)(ny+1)(nz+1) words, x fastest
+  int32_t cell_offset;      // into the cell parameters: nx*ny*nz, x fastest
+  int32_t pad[3];
+};
+
+struct Lattice {
+  double hx = 0, hy = 0, hz = 0;
+  std::vector<TileHeader> tiles;      // in launch order (XCD-aware slabs or lattice order)
+  std::vector<uint32_t> words;        // per tile node: first equation (28 bits) | free-DOF mask << 28 | shared << 31
+  std::vector<uint16_t> cell_param;   // per tile cell: parameter set, 0xffff = no element
+  std::vector<int32_t> cell_elem;     // per tile cell: batch element, -1 = none (host)
+  std::vector<int32_t> perm;          // internal equation -> reference equation [neq]
+  // nodes touched by several tiles: word (first equation | mask) and the offsets of their tiles'
+  // partial sums (tile * kTileNodes + tile node) * 3, in tile parity-colour order
+  std::vector<uint32_t> shared_word;
+  std::vector<int32_t> shared_start, shared_list;
+};
+
+bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
+                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, Lattice& out);
+
 // Contraction of node-space values into equations when equations are shared by several
 // nodes (equalDOF ties): eq -> list of node DOFs (3 n + d), in ascending order.
 struct Contraction {
@@ -107,8 +136,28 @@ struct fx_batch {
   float* ynode32 = nullptr;
   int32_t* con_start = nullptr; // [neq + 1]
   int32_t* con_list = nullptr;
+  // structured fast path (when the batch is an axis-aligned uniform lattice)
+  bool lattice = false;
+  double hx = 0, hy = 0, hz = 0;
+  fx::TileHeader* tiles = nullptr;
+  uint32_t* words = nullptr;
+  uint16_t* cell_param = nullptr;
+  int32_t ntiles = 0;
+  int32_t nshared = 0;
+  uint32_t* shared_word = nullptr;
+  int32_t* shared_start = nullptr;
+  int32_t* shared_list = nullptr;
+  double* partial64 = nullptr;        // [ntiles * kTileNodes * 3] per-tile sums at shared nodes
+  float* partial32 = nullptr;
+  int box_variant = 0;                // register-allocation variant of the mass-free kernel (tuned)
+  // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference
+  bool internal = false;
+  int32_t* perm = nullptr;
+  // free equations no element touches: zeroed after each apply
+  int32_t* orphans = nullptr;
+  int32_t norphans = 0;
   // solver work (allocated at create: compute allocates nothing)
-  double* work = nullptr;       // 6 * neq doubles: r z p ap inv d
+  double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats
   uint64_t device_bytes = 0;
   std::vector<fx::DevParam> host_params;

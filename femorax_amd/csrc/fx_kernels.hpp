@@ -27,11 +27,36 @@ struct KernelArgs {
   const int32_t* done;  // solver stop flag: the launch is a no-op once it is set
 };
 
+// Effective factors of element_operator.hpp:45-56 for an elastic material (D_0 = D_c = D):
+// D_eff = (cK + c0 + cc) D, mass factor cM' = cM + cC alphaM.
+__host__ __device__ inline void operator_factors(const DevParam& p, const fx_coeffs& k, double& fD, double& cM) {
+  const double cK = k.initial ? k.damping * p.betaK : k.stiffness + k.damping * p.betaK;
+  const double c0 = k.initial ? k.stiffness + k.damping * p.betaKinit : k.damping * p.betaKinit;
+  const double cc = k.damping * p.betaKcomm;
+  fD = cK + c0 + cc;
+  cM = k.mass + k.damping * p.alphaM;
+}
+
+template <typename T, bool DIAG>
+int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done);
+
 int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done = nullptr);
 int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 
+// The C-ABI operator entry points (reference equation numbering; lattice batches permute to
+// and from their internal numbering around the internal apply), fx_operator.hip.
+int external_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y);
+int external_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y);
+int external_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
+// Permutations between the reference and the internal numbering (identity for general batches).
+template <typename T>
+int to_internal(fx_batch* b, const T* ext, T* in);
+template <typename T>
+int to_external(fx_batch* b, const T* in, T* ext);
+
 // Solver-side launchers (fx_solver.hip).
+int copy64(fx_ctx* ctx, int64_t n, const double* a, double* b);
 int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& r);

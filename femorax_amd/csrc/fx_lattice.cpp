@@ -1,0 +1,241 @@
+// Lattice detection for the structured fast path.
+//
+// Femora-built soil models are mostly regular grids of axis-aligned bricks. When every element
+// of a batch is an axis-aligned box of one size (hx, hy, hz) sitting on a common lattice, with
+// its nodes in the OpenSees Brick order, the kernels need neither the connectivity nor the
+// coordinates: a brick of lattice cells is a workgroup's block, its nodes are addressed by
+// lattice position, and the element geometry is three numbers. Per block the device then holds
+// one 32-bit word per lattice node (first equation | free-DOF mask | shared flag) and a 16-bit
+// parameter index per cell. Anything else (distorted or rotated elements, holes inside
+// a tile are fine, ties that merge a node's DOFs with another node's) uses the general blocks.
+#include <algorithm>
+#include <cmath>
+#include <unordered_map>
+
+#include "fx_internal.hpp"
+
+namespace fx {
+
+bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
+                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, Lattice& L) {
+  if (count <= 0) return false;
+  const double* X = coords;
+  const double* Y = coords + nnode;
+  const double* Z = coords + 2 * static_cast<size_t>(nnode);
+  auto nd = [&](int a, int32_t e) { return node[static_cast<size_t>(a) * padded + e]; };
+  // element 0 fixes the cell size
+  const int32_t n0 = nd(0, 0);
+  const double hx = X[nd(1, 0)] - X[n0], hy = Y[nd(3, 0)] - Y[n0], hz = Z[nd(4, 0)] - Z[n0];
+  if (!(hx > 0 && hy > 0 && hz > 0)) return false;
+  const double tol = 1e-9 * std::min(hx, std::min(hy, hz));
+  // corner offsets of the OpenSees Brick order
+  static const int OX[8] = {0, 1, 1, 0, 0, 1, 1, 0}, OY[8] = {0, 0, 1, 1, 0, 0, 1, 1}, OZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  double x0 = 1e300, y0 = 1e300, z0 = 1e300;
+  for (int32_t e = 0; e < count; ++e) {
+    const int32_t b = nd(0, e);
+    x0 = std::min(x0, X[b]);
+    y0 = std::min(y0, Y[b]);
+    z0 = std::min(z0, Z[b]);
+  }
+  std::vector<int32_t> ci(count), cj(count), ck(count);
+  int32_t ni = 0, nj = 0, nk = 0;
+  for (int32_t e = 0; e < count; ++e) {
+    const int32_t b = nd(0, e);
+    for (int a = 0; a < 8; ++a) {
+      const int32_t n = nd(a, e);
+      if (std::fabs(X[n] - X[b] - OX[a] * hx) > tol || std::fabs(Y[n] - Y[b] - OY[a] * hy) > tol ||
+          std::fabs(Z[n] - Z[b] - OZ[a] * hz) > tol)
+        return false;
+    }
+    const double fi = (X[b] - x0) / hx, fj = (Y[b] - y0) / hy, fk = (Z[b] - z0) / hz;
+    const double ri = std::nearbyint(fi), rj = std::nearbyint(fj), rk = std::nearbyint(fk);
+    if (std::fabs(fi - ri) > 1e-6 || std::fabs(fj - rj) > 1e-6 || std::fabs(fk - rk) > 1e-6) return false;
+    if (ri > 1e6 || rj > 1e6 || rk > 1e6) return false;
+    ci[e] = static_cast<int32_t>(ri);
+    cj[e] = static_cast<int32_t>(rj);
+    ck[e] = static_cast<int32_t>(rk);
+    ni = std::max(ni, ci[e] + 1);
+    nj = std::max(nj, cj[e] + 1);
+    nk = std::max(nk, ck[e] + 1);
+    if (param[e] < 0 || param[e] >= 0xffff) return false;
+  }
+  // free DOFs of a node must have consecutive equations (no ties across nodes)
+  {
+    std::unordered_map<int32_t, int32_t> owner;
+    for (int32_t n = 0; n < nnode; ++n)
+      for (int d = 0; d < 3; ++d) {
+        const int32_t q = node_eq[3 * n + d];
+        if (q < 0) continue;
+        auto it = owner.emplace(q, n);
+        if (!it.second && it.first->second != n) return false;  // an equation shared by two nodes
+      }
+  }
+  // lattice -> element and -> node
+  const int64_t ncell = static_cast<int64_t>(ni) * nj * nk;
+  if (ncell > 4 * static_cast<int64_t>(count) + 4096) return false;  // too sparse to be a lattice
+  std::vector<int32_t> cell(ncell, -1);
+  const int32_t NI = ni + 1, NJ = nj + 1;
+  std::vector<int32_t> lat_node(static_cast<size_t>(NI) * NJ * (nk + 1), -1);
+  for (int32_t e = 0; e < count; ++e) {
+    const int64_t c = ci[e] + static_cast<int64_t>(ni) * (cj[e] + static_cast<int64_t>(nj) * ck[e]);
+    if (cell[c] >= 0) return false;  // two elements in one cell
+    cell[c] = e;
+    for (int a = 0; a < 8; ++a) {
+      const size_t p = (ci[e] + OX[a]) + static_cast<size_t>(NI) * ((cj[e] + OY[a]) + static_cast<size_t>(NJ) * (ck[e] + OZ[a]));
+      const int32_t n = nd(a, e);
+      if (lat_node[p] >= 0 && lat_node[p] != n) return false;
+      lat_node[p] = n;
+    }
+  }
+  // tiles of kTile cells; tiles of one parity colour share no node, so the (at most 8) tiles at
+  // a node have 8 different colours and their partial sums are added in colour order
+  const int TX = kTileX, TY = kTileY, TZ = kTileZ;
+  const int32_t bi = (ni + TX - 1) / TX, bj = (nj + TY - 1) / TY, bk = (nk + TZ - 1) / TZ;
+  L = Lattice{};
+  L.hx = hx;
+  L.hy = hy;
+  L.hz = hz;
+  std::vector<int32_t> order;
+  order.reserve(static_cast<size_t>(bi) * bj * bk);
+  if (xcd_order) {
+    // Workgroups of a launch go round-robin to the 8 XCDs (each with its own L2): the tiles of
+    // one of 8 slabs along the longest tile axis sit at positions r, r + 8, r + 16, ... so that
+    // tiles sharing nodes meet in one L2. The tile order changes no sum (colour order does).
+    const int axis = (bi >= bj && bi >= bk) ? 0 : (bj >= bk ? 1 : 2);
+    const int32_t len = axis == 0 ? bi : axis == 1 ? bj : bk;
+    std::vector<std::vector<int32_t>> slab(8);
+    for (int32_t t = 0; t < bi * bj * bk; ++t) {
+      const int32_t ti = t % bi, tj = (t / bi) % bj, tk = t / (bi * bj);
+      const int32_t pos = axis == 0 ? ti : axis == 1 ? tj : tk;
+      slab[std::min<int64_t>(7, static_cast<int64_t>(pos) * 8 / std::max<int32_t>(len, 1))].push_back(t);
+    }
+    std::vector<size_t> at(8, 0);
+    for (bool more = true; more;) {
+      more = false;
+      for (int r = 0; r < 8; ++r)
+        if (at[r] < slab[r].size()) {
+          order.push_back(slab[r][at[r]++]);
+          more = true;
+        }
+    }
+  } else {
+    for (int32_t t = 0; t < bi * bj * bk; ++t) order.push_back(t);
+  }
+  std::vector<int32_t> tile_nodes;  // global node (or -1) of every tile node, in word order
+  std::vector<int8_t> tile_colour;
+  for (int32_t t : order) {
+    const int32_t ti = t % bi, tj = (t / bi) % bj, tk = t / (bi * bj);
+    const int32_t i0 = ti * TX, j0 = tj * TY, k0 = tk * TZ;
+    const int32_t nx = std::min(TX, ni - i0), ny = std::min(TY, nj - j0), nz = std::min(TZ, nk - k0);
+    TileHeader h{};
+    h.nx = nx;
+    h.ny = ny;
+    h.nz = nz;
+    h.word_offset = static_cast<int32_t>(tile_nodes.size());
+    h.cell_offset = static_cast<int32_t>(L.cell_param.size());
+    bool any = false;
+    for (int32_t z = 0; z < nz; ++z)
+      for (int32_t y = 0; y < ny; ++y)
+        for (int32_t x = 0; x < nx; ++x) {
+          const int32_t e = cell[(i0 + x) + static_cast<int64_t>(ni) * ((j0 + y) + static_cast<int64_t>(nj) * (k0 + z))];
+          L.cell_param.push_back(e >= 0 ? static_cast<uint16_t>(param[e]) : uint16_t(0xffff));
+          L.cell_elem.push_back(e);
+          any |= e >= 0;
+        }
+    if (!any) {
+      L.cell_param.resize(h.cell_offset);
+      L.cell_elem.resize(h.cell_offset);
+      continue;
+    }
+    for (int32_t z = 0; z <= nz; ++z)
+      for (int32_t y = 0; y <= ny; ++y)
+        for (int32_t x = 0; x <= nx; ++x)
+          tile_nodes.push_back(lat_node[(i0 + x) + static_cast<size_t>(NI) * ((j0 + y) + static_cast<size_t>(NJ) * (k0 + z))]);
+    L.tiles.push_back(h);
+    tile_colour.push_back(static_cast<int8_t>((ti & 1) | ((tj & 1) << 1) | ((tk & 1) << 2)));
+  }
+  // how many tiles touch each node
+  const int32_t nt = static_cast<int32_t>(L.tiles.size());
+  std::vector<int32_t> touch_count(nnode, 0);
+  for (int32_t n : tile_nodes)
+    if (n >= 0) ++touch_count[n];
+  // internal numbering: first the nodes only one tile touches (tile by tile, lattice order),
+  // then the shared nodes (by the first tile that touches them), then untouched equations.
+  // A node's free DOFs are consecutive. perm: internal -> reference equation.
+  std::vector<int32_t> internal_of(static_cast<size_t>(neq), -1);
+  L.perm.assign(static_cast<size_t>(neq), -1);
+  int32_t next = 0;
+  std::vector<char> numbered(nnode, 0);
+  auto number = [&](int32_t n) {
+    numbered[n] = 1;
+    for (int d = 0; d < 3; ++d) {
+      const int32_t q = node_eq[3 * n + d];
+      if (q < 0) continue;
+      internal_of[q] = next;
+      L.perm[next++] = q;
+    }
+  };
+  for (int pass = 0; pass < 2; ++pass)
+    for (int32_t n : tile_nodes)
+      if (n >= 0 && !numbered[n] && ((touch_count[n] > 1) == (pass == 1))) number(n);
+  for (int32_t q = 0; q < neq; ++q)
+    if (internal_of[q] < 0) {
+      internal_of[q] = next;
+      L.perm[next++] = q;
+    }
+  auto word_of = [&](int32_t n, uint32_t& w) {
+    int32_t base = -1;
+    uint32_t mask = 0;
+    for (int d = 0; d < 3; ++d) {
+      const int32_t q = node_eq[3 * n + d];
+      if (q < 0) continue;
+      if (base < 0) base = internal_of[q];
+      mask |= 1u << d;
+    }
+    if (base >= (1 << 28)) return false;
+    w = (mask << 28) | static_cast<uint32_t>(base < 0 ? 0 : base);
+    return true;
+  };
+  // tile node words: first equation | free-DOF mask | shared flag
+  L.words.resize(tile_nodes.size());
+  for (size_t i = 0; i < tile_nodes.size(); ++i) {
+    const int32_t n = tile_nodes[i];
+    uint32_t w = 0;
+    if (n >= 0) {
+      if (!word_of(n, w)) return false;
+      if (touch_count[n] > 1) w |= 0x80000000u;
+    }
+    L.words[i] = w;
+  }
+  // shared nodes in internal order, each with its tiles' partial-sum offsets in colour order
+  std::vector<std::vector<int32_t>> partials(nnode);
+  for (int c = 0; c < 8; ++c)
+    for (int32_t t = 0; t < nt; ++t) {
+      if (tile_colour[t] != c) continue;
+      const TileHeader& h = L.tiles[t];
+      const int32_t nw = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
+      for (int32_t i = 0; i < nw; ++i) {
+        const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
+        if (n >= 0 && touch_count[n] > 1) partials[n].push_back((t * kTileNodes + i) * 3);
+      }
+    }
+  if (static_cast<int64_t>(nt) * kTileNodes * 3 >= (int64_t(1) << 31)) return false;
+  std::vector<int32_t> shared_nodes;
+  std::fill(numbered.begin(), numbered.end(), 0);
+  for (int32_t n : tile_nodes)
+    if (n >= 0 && touch_count[n] > 1 && !numbered[n]) {
+      numbered[n] = 1;
+      shared_nodes.push_back(n);
+    }
+  L.shared_start.assign(1, 0);
+  for (int32_t n : shared_nodes) {
+    uint32_t w = 0;
+    word_of(n, w);
+    L.shared_word.push_back(w);
+    L.shared_list.insert(L.shared_list.end(), partials[n].begin(), partials[n].end());
+    L.shared_start.push_back(static_cast<int32_t>(L.shared_list.size()));
+  }
+  return true;
+}
+
+}  // namespace fx

@@ -85,13 +85,6 @@ struct ElemCoef {
   int consistent;
 };
 
-__host__ __device__ inline void operator_factors(const DevParam& p, const fx_coeffs& k, double& fD, double& cM) {
-  const double cK = k.initial ? k.damping * p.betaK : k.stiffness + k.damping * p.betaK;
-  const double c0 = k.initial ? k.stiffness + k.damping * p.betaKinit : k.damping * p.betaKinit;
-  const double cc = k.damping * p.betaKcomm;
-  fD = cK + c0 + cc;
-  cM = k.mass + k.damping * p.alphaM;
-}
 
 template <typename T>
 __device__ __forceinline__ void element_apply(const T (&x)[8], const T (&y)[8], const T (&z)[8], const T (&v)[24],
@@ -346,14 +339,117 @@ KernelArgs<T> operator_args(fx_batch* b, const fx_coeffs& k, const T* x, T* y, i
   return a;
 }
 
+template <typename T>
+__global__ void zero_orphans_kernel(int32_t n, const int32_t* __restrict__ list, T* __restrict__ y, const int32_t* done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[list[i]] = T(0);
+}
+
+template <typename T>
+int zero_orphans(fx_batch* b, T* y, const int32_t* done) {
+  if (b->norphans == 0) return FX_OK;
+  hipLaunchKernelGGL(zero_orphans_kernel<T>, dim3((b->norphans + 255) / 256), dim3(256), 0, b->ctx->stream, b->norphans,
+                     b->orphans, y, done);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
 int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
-  return launch_operator<double, false>(b, operator_args<double>(b, k, x, y, 0, done));
+  const int rc = b->lattice ? launch_box<double, false>(b, k, x, y, 0, done)
+                            : launch_operator<double, false>(b, operator_args<double>(b, k, x, y, 0, done));
+  return rc ? rc : zero_orphans(b, y, done);
 }
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done) {
-  return launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
+  const int rc = b->lattice ? launch_box<float, false>(b, k, x, y, 0, done)
+                            : launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
+  return rc ? rc : zero_orphans(b, y, done);
 }
 int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
-  return launch_operator<double, true>(b, operator_args<double>(b, k, static_cast<const double*>(nullptr), d, 0));
+  const int rc = b->lattice
+                     ? launch_box<double, true>(b, k, static_cast<const double*>(nullptr), d, 0, nullptr)
+                     : launch_operator<double, true>(b, operator_args<double>(b, k, static_cast<const double*>(nullptr), d, 0));
+  return rc ? rc : zero_orphans(b, d, static_cast<const int32_t*>(nullptr));
+}
+
+}  // namespace fx
+
+namespace fx {
+
+template <typename T>
+__global__ void gather_perm_kernel(int32_t n, const int32_t* __restrict__ perm, const T* __restrict__ src,
+                                   T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+template <typename T>
+__global__ void scatter_perm_kernel(int32_t n, const int32_t* __restrict__ perm, const T* __restrict__ src,
+                                    T* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+
+template <typename T>
+int to_internal(fx_batch* b, const T* ext, T* in) {
+  if (b->neq == 0) return FX_OK;
+  if (!b->internal) {
+    if (ext != in) FX_HIP(b->ctx, hipMemcpyAsync(in, ext, sizeof(T) * b->neq, hipMemcpyDeviceToDevice, b->ctx->stream));
+    return FX_OK;
+  }
+  hipLaunchKernelGGL(gather_perm_kernel<T>, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq, b->perm,
+                     ext, in);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+template <typename T>
+int to_external(fx_batch* b, const T* in, T* ext) {
+  if (b->neq == 0) return FX_OK;
+  if (!b->internal) {
+    if (ext != in) FX_HIP(b->ctx, hipMemcpyAsync(ext, in, sizeof(T) * b->neq, hipMemcpyDeviceToDevice, b->ctx->stream));
+    return FX_OK;
+  }
+  hipLaunchKernelGGL(scatter_perm_kernel<T>, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq, b->perm,
+                     in, ext);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+template int to_internal<double>(fx_batch*, const double*, double*);
+template int to_internal<float>(fx_batch*, const float*, float*);
+template int to_external<double>(fx_batch*, const double*, double*);
+template int to_external<float>(fx_batch*, const float*, float*);
+
+namespace {
+template <typename T, typename Op>
+int external_call(fx_batch* b, const T* x, T* y, T* xi, T* yi, Op op) {
+  if (b->internal) {
+    if (x) {
+      if (int rc = to_internal<T>(b, x, xi)) return rc;
+    }
+    if (int rc = op(x ? xi : nullptr, yi)) return rc;
+    if (int rc = to_external<T>(b, yi, y)) return rc;
+  } else {
+    if (int rc = op(x, y)) return rc;
+  }
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+}  // namespace
+
+int external_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y) {
+  double* xi = b->work + 6 * static_cast<size_t>(b->neq);
+  double* yi = b->work + 7 * static_cast<size_t>(b->neq);
+  return external_call<double>(b, x, y, xi, yi,
+                               [&](const double* xx, double* yy) { return operator_apply64(b, k, xx, yy); });
+}
+int external_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y) {
+  float* xi = b->work32 + 7 * static_cast<size_t>(b->neq);
+  float* yi = b->work32 + 8 * static_cast<size_t>(b->neq);
+  return external_call<float>(b, x, y, xi, yi, [&](const float* xx, float* yy) { return operator_apply32(b, k, xx, yy); });
+}
+int external_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
+  double* yi = b->work + 7 * static_cast<size_t>(b->neq);
+  return external_call<double>(b, static_cast<const double*>(nullptr), d, static_cast<double*>(nullptr), yi,
+                               [&](const double*, double* yy) { return operator_diagonal64(b, k, yy); });
 }
 
 }  // namespace fx

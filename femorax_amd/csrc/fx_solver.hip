@@ -220,6 +220,21 @@ __global__ void zero_kernel(int64_t n, T* __restrict__ a) {
   if (i < n) a[i] = T(0);
 }
 
+// Plain vector copy (also the calibration kernel of the HBM traffic counters: it moves exactly
+// n * 16 bytes).
+__global__ void copy_kernel(int64_t n, const double* __restrict__ a, double* __restrict__ b) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+int copy64(fx_ctx* ctx, int64_t n, const double* a, double* b) {
+  if (n <= 0) return FX_OK;
+  hipLaunchKernelGGL(copy_kernel, dim3(static_cast<unsigned>((n + 255) / 256)), dim3(256), 0, ctx->stream, n, a, b);
+  FX_HIP(ctx, hipGetLastError());
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
 namespace {
 
 inline dim3 grid_of(int64_t n, int bs = 256) { return dim3(static_cast<unsigned>((n + bs - 1) / bs)); }
@@ -341,16 +356,19 @@ int jacobi_inverse(fx_batch* b, const fx_coeffs& k, double* d, void* inv, bool f
 
 }  // namespace
 
-int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
-         fx_solve_result& res) {
+namespace {
+int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+              fx_solve_result& res) {
   fx_ctx* ctx = b->ctx;
   const int64_t n = b->neq;
-  double *r = b->work, *z = r + n, *p = z + n, *ap = p + n, *inv = ap + n, *d = inv + n;
+  double *r = b->work, *z = r + n, *p = z + n, *ap = p + n, *inv = ap + n, *d = inv + n, *bi = d + n, *xi = bi + n;
   if (int rc = jacobi_inverse(b, k, d, inv, false)) return rc;
+  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
+  if (int rc = to_internal<double>(b, x, xi)) return rc;
   CgState* st = static_cast<CgState*>(ctx->scalars);
   CgState h{};
   auto apply = [&](const double* xx, double* yy, const int32_t* dn) { return operator_apply64(b, k, xx, yy, dn); };
-  if (int rc = cg_device<double>(ctx, n, apply, inv, rhs, x, r, z, p, ap, st, o.relative_tolerance,
+  if (int rc = cg_device<double>(ctx, n, apply, inv, bi, xi, r, z, p, ap, st, o.relative_tolerance,
                                  o.absolute_tolerance, o.max_iterations, h))
     return rc;
   res = fx_solve_result{};
@@ -361,12 +379,18 @@ int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const doubl
   res.initial_residual = h.init_res;
   res.final_residual = h.final_res;
   // true residual ||b - A x||
-  if (int rc = operator_apply64(b, k, x, ap)) return rc;
-  hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, ctx->stream, n, rhs, ap, r);
+  if (int rc = operator_apply64(b, k, xi, ap)) return rc;
+  hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, ctx->stream, n, bi, ap, r);
   double rr = 0;
   if (int rc = dot64(ctx, n, r, r, &rr)) return rc;
   res.true_residual = std::sqrt(rr);
-  return FX_OK;
+  return to_external<double>(b, xi, x);
+}
+}  // namespace
+
+int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+         fx_solve_result& res) {
+  return cg64_once(b, k, o, rhs, x, res);
 }
 
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
@@ -377,13 +401,15 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
     return FX_EUNSUP;
   }
   const int64_t n = b->neq;
-  double *r64 = b->work, *ap64 = r64 + 3 * n, *d64 = r64 + 5 * n;
+  double *r64 = b->work, *ap64 = r64 + 3 * n, *d64 = r64 + 5 * n, *bi = r64 + 6 * n, *xi = r64 + 7 * n;
   float *inv = b->work32, *r = inv + n, *z = r + n, *p = z + n, *ap = p + n, *dx = ap + n, *rhs32 = dx + n;
   if (int rc = jacobi_inverse(b, k, d64, inv, true)) return rc;
+  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
+  if (int rc = to_internal<double>(b, x, xi)) return rc;
   CgState* st = static_cast<CgState*>(ctx->scalars);
   hipStream_t s = ctx->stream;
   double bb = 0;
-  if (int rc = dot64(ctx, n, rhs, rhs, &bb)) return rc;
+  if (int rc = dot64(ctx, n, bi, bi, &bb)) return rc;
   const double bnorm = std::sqrt(bb);
   const double tol = std::max(o.relative_tolerance * bnorm, o.absolute_tolerance);
   res = fx_solve_result{};
@@ -392,8 +418,8 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
   auto apply32 = [&](const float* xx, float* yy, const int32_t* dn) { return operator_apply32(b, k, xx, yy, dn); };
   for (int outer = 0;; ++outer) {
     // r = b - A x in FP64
-    if (int rc = operator_apply64(b, k, x, ap64)) return rc;
-    hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, s, n, rhs, ap64, r64);
+    if (int rc = operator_apply64(b, k, xi, ap64)) return rc;
+    hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, s, n, bi, ap64, r64);
     double rr = 0;
     if (int rc = dot64(ctx, n, r64, r64, &rr)) return rc;
     const double rn = std::sqrt(rr);
@@ -414,12 +440,12 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
       return rc;
     total_inner += h.it;
     if (h.indefinite) res.indefinite = 1;
-    hipLaunchKernelGGL(refine_update_kernel, grid_of(n), dim3(256), 0, s, n, x, dx, rn);
+    hipLaunchKernelGGL(refine_update_kernel, grid_of(n), dim3(256), 0, s, n, xi, dx, rn);
     FX_HIP(ctx, hipGetLastError());
     if (h.it == 0) break;  // no progress possible
   }
   res.iterations = total_inner;
-  return FX_OK;
+  return to_external<double>(b, xi, x);
 }
 
 }  // namespace fx

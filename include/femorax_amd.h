@@ -126,16 +126,29 @@ typedef struct fx_batch_desc {
 } fx_batch_desc;
 
 /* Flags for fx_batch_create. */
-#define FX_BATCH_FP32 1u /* also build the FP32 operator (mixed precision solves) */
+#define FX_BATCH_FP32 1u    /* also build the FP32 operator (mixed precision solves) */
+#define FX_BATCH_GENERAL 2u  /* do not use the structured-lattice fast path (testing) */
+#define FX_BATCH_XCD_SLABS 8u /* lattice path: tiles placed in 8 slabs, one per XCD (default: lattice order,
+                                which measured faster on MI355X; same bits either way) */
+#define FX_BATCH_NO_TUNE 16u /* skip the kernel-variant autotune at creation (variant 0) */
 
 int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* desc, uint32_t flags, fx_batch** out);
 void fx_batch_destroy(fx_batch* b);
 
 typedef struct fx_batch_info {
-  int32_t count, neq, nnode, nblocks, ncolors, tied, affine_classes, shape_classes;
+  int32_t count, neq, nnode, nblocks, ncolors, tied;
+  int32_t lattice;          /* 1: the batch runs on the structured-lattice fast path */
+  int32_t tiles;            /* lattice tiles (workgroups per apply) */
+  int32_t variant;          /* tuned kernel variant */
   uint64_t device_bytes;
 } fx_batch_info;
 int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
+/* Kernel-variant choice (the reference's force-settings autotune, src/exec/force_settings.cpp:
+ * 135-205): variant >= 0 sets it, -1 times the variants on this batch and keeps the fastest.
+ * Lattice batches have variants 0-3 (bit 0: 5 waves per SIMD register budget; bit 1: persistent
+ * workgroups that prefetch their next tile); general batches have one. Every variant gives the
+ * same bits. */
+int fx_batch_tune(fx_batch* b, int variant);
 
 /* y = K_eff x, K_eff = cK K_t + cC C + cM M (InternalForce::apply_operator,
  * internal_force.cpp:545-573; element_operator_apply, element_operator.hpp:82-130).
@@ -165,6 +178,8 @@ typedef struct fx_solve_result {
   double true_residual;      /* ||b - A x|| recomputed in FP64 at the end */
 } fx_solve_result;
 
+/* VectorOps::copy (vector_ops.cpp:79): dst = src, n doubles, as a plain device kernel. */
+int fx_copy(fx_ctx* ctx, int64_t n, const double* src, double* dst);
 /* VectorOps::dot (vector_ops.cpp:66-73): chunks of 2048, 64 lanes, fixed order. */
 int fx_dot(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 

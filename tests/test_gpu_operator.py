@@ -23,10 +23,10 @@ def ctx():
     c.close()
 
 
-def setup(ctx, model, fp32=False):
+def setup(ctx, model, fp32=False, general=False):
     pm = core.Mesh(model)
     params = pm.params_for(model)
-    batch = core.Batch(ctx, pm, params, fp32=fp32)
+    batch = core.Batch(ctx, pm, params, fp32=fp32, general=general)
     om = fxo.OracleMesh(model)
     return pm, batch, om, fxo.elastic_params(params)
 
@@ -44,14 +44,24 @@ def rel_err(got, want):
     return np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-300)
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_operator_apply_matches_oracle(ctx, name):
+STRUCTURED = {"c1_box16", "c2_layered16"}
+PATHS = [(n, g) for n in sorted(CASES) for g in ((False, True) if n in STRUCTURED else (True,))]
+
+
+def path_id(p):
+    return f"{p[0]}-{'general' if p[1] else 'lattice'}"
+
+
+@pytest.mark.parametrize("case", PATHS, ids=path_id)
+def test_operator_apply_matches_oracle(ctx, case):
+    name, general = case
     model = CASES[name]()
     if name == "distorted6":
         model["params"] = [dict(p, rayleigh=(0.6, 5.8e-4, 1e-4, 0.0)) for p in model["params"]]
         model["region"] = model["group"].copy()
         model["rayleigh"] = [(0.6, 5.8e-4, 1e-4, 0.0), (0.9, 3e-4, 0.0, 2e-5)]
-    pm, batch, om, op = setup(ctx, model)
+    pm, batch, om, op = setup(ctx, model, general=general)
+    assert batch.info()["lattice"] == (0 if general else 1)
     x = meshes.random_vector(pm.neq, 31)
     dx = core.DeviceArray.from_host(ctx, x)
     dy = core.DeviceArray(ctx, pm.neq)
@@ -63,10 +73,11 @@ def test_operator_apply_matches_oracle(ctx, name):
         assert rel_err(got, want) <= 1e-12, (kk, rel_err(got, want))
 
 
-@pytest.mark.parametrize("name", sorted(CASES))
-def test_operator_diagonal_matches_oracle(ctx, name):
+@pytest.mark.parametrize("case", PATHS, ids=path_id)
+def test_operator_diagonal_matches_oracle(ctx, case):
+    name, general = case
     model = CASES[name]()
-    pm, batch, om, op = setup(ctx, model)
+    pm, batch, om, op = setup(ctx, model, general=general)
     dd = core.DeviceArray(ctx, pm.neq)
     for kk in COEFFS:
         batch.diagonal(dd, core.coeffs(*kk))
@@ -74,23 +85,75 @@ def test_operator_diagonal_matches_oracle(ctx, name):
         assert rel_err(dd.get(), want) <= 1e-12, kk
 
 
-def test_operator_is_deterministic(ctx):
+@pytest.mark.parametrize("general", [False, True])
+def test_operator_is_deterministic(ctx, general):
     model = S.config_c2(n=24)
-    pm, batch, om, op = setup(ctx, model)
+    pm, batch, om, op = setup(ctx, model, general=general)
     x = meshes.random_vector(pm.neq, 5)
     dx = core.DeviceArray.from_host(ctx, x)
     dy = core.DeviceArray(ctx, pm.neq)
     batch.apply(dx, dy)
     a = dy.get()
+    assert rel_err(a, om.apply(op, x)) <= 1e-12
     for _ in range(3):
         batch.apply(dx, dy)
         np.testing.assert_array_equal(dy.get(), a)
 
 
-@pytest.mark.parametrize("name", ["c1_box16", "distorted6", "column24_tied"])
-def test_operator_fp32_matches_oracle(ctx, name):
+def test_kernel_variants_and_tile_placement_give_the_same_bits(ctx):
+    """The tuned register-allocation variants and the XCD-aware tile placement change no bit."""
+    model = S.config_c2(n=40)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    a = core.Batch(ctx, pm, params)
+    b = core.Batch(ctx, pm, params, xcd=True)
+    x = meshes.random_vector(pm.neq, 78)
+    dx = core.DeviceArray.from_host(ctx, x)
+    ya, yb = core.DeviceArray(ctx, pm.neq), core.DeviceArray(ctx, pm.neq)
+    outs = []
+    for v in (0, 1, 2, 3):
+        a.tune(v)
+        a.apply(dx, ya)
+        outs.append(ya.get())
+    b.apply(dx, yb)
+    for o in outs[1:]:
+        np.testing.assert_array_equal(outs[0], o)
+    # the tile order decides the internal numbering, not the sums at a node
+    np.testing.assert_array_equal(yb.get(), outs[0])
+    a.tune(-1)
+    assert a.info()["variant"] in (0, 1, 2, 3)
+
+
+def test_lattice_fp32_diagonal_and_mass_paths(ctx):
+    """Lattice path at a size with many tiles: FP32 apply, diagonal and the mass-term kernel
+    against the oracle."""
+    model = S.config_c2(n=40)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    a = core.Batch(ctx, pm, params, fp32=True)
+    assert a.info()["lattice"] == 1 and a.info()["tiles"] > 64
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    x = meshes.random_vector(pm.neq, 77)
+    dx = core.DeviceArray.from_host(ctx, x)
+    ya = core.DeviceArray(ctx, pm.neq)
+    for kk in COEFFS:
+        a.apply(dx, ya, core.coeffs(*kk))
+        assert rel_err(ya.get(), om.apply(op, x, fxo.coeffs(*kk))) <= 1e-12
+        a.diagonal(ya, core.coeffs(*kk))
+        assert rel_err(ya.get(), om.diagonal(op, fxo.coeffs(*kk))) <= 1e-12
+    dx32 = core.DeviceArray.from_host(ctx, x.astype(np.float32))
+    za = core.DeviceArray(ctx, pm.neq, np.float32)
+    a.apply_f32(dx32, za)
+    assert rel_err(za.get().astype(np.float64), om.apply(op, x.astype(np.float32).astype(np.float64))) <= 2e-6
+
+
+@pytest.mark.parametrize("case", [("c1_box16", False), ("c1_box16", True), ("distorted6", True),
+                                  ("column24_tied", True)], ids=path_id)
+def test_operator_fp32_matches_oracle(ctx, case):
+    name, general = case
     model = CASES[name]()
-    pm, batch, om, op = setup(ctx, model, fp32=True)
+    pm, batch, om, op = setup(ctx, model, fp32=True, general=general)
     x = meshes.random_vector(pm.neq, 17)
     dx = core.DeviceArray.from_host(ctx, x.astype(np.float32))
     dy = core.DeviceArray(ctx, pm.neq, np.float32)
