@@ -32,7 +32,7 @@ class ModelDesc(C.Structure):
     _fields_ = [("nnode", C.c_int32), ("xyz", C.c_void_p), ("nelem", C.c_int32), ("conn", C.c_void_p),
                 ("group", C.c_void_p), ("region", C.c_void_p), ("nfix", C.c_int32), ("fix_node", C.c_void_p),
                 ("fix_dof", C.c_void_p), ("ntie", C.c_int32), ("tie_retained", C.c_void_p),
-                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p)]
+                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p), ("tie_command", C.c_void_p)]
 
 
 class MeshView(C.Structure):
@@ -226,6 +226,8 @@ def model_desc(model, keep):
     m.tie_retained = _arr(keep, tie[:, 0].copy() if len(tie) else np.zeros(1), np.int32)
     m.tie_constrained = _arr(keep, tie[:, 1].copy() if len(tie) else np.zeros(1), np.int32)
     m.tie_dof = _arr(keep, tie[:, 2].copy() if len(tie) else np.zeros(1), np.int32)
+    cmd = model.get("tie_command")
+    m.tie_command = _arr(keep, cmd, np.int32) if cmd is not None and len(tie) else None
     return m
 
 

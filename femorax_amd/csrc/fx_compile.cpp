@@ -182,6 +182,7 @@ int compile_model(const fx_model_desc& m, int renumber_rcm, int simd_width, int 
       err = "fx_compile_model: equalDOF row " + std::to_string(r) + " is out of range";
       return FX_EINVAL;
     }
+  if (int rc = check_equal_dof_commands(m, err)) return rc;
   out.nnode = N;
   out.nelem = E;
   // --- node and element order -------------------------------------------------------------
@@ -224,6 +225,7 @@ int compile_model(const fx_model_desc& m, int renumber_rcm, int simd_width, int 
     out.node_eq[a] = eq_of_root[r];
   }
   out.neq = neq;
+  if (int rc = check_transformation_handler(m, out.node_rank, out.node_eq, err)) return rc;
   // --- coordinates SoA ---------------------------------------------------------------------
   out.coords.resize(static_cast<size_t>(3) * N);
   for (int32_t i = 0; i < N; ++i)

@@ -19,6 +19,10 @@ struct Mesh {
   std::vector<int32_t> node, eq, param, param_group, param_region, color_start, color_elements;
 };
 
+// Constraint refusals of the compile step (fx_constraints.cpp).
+int check_equal_dof_commands(const fx_model_desc& m, std::string& err);
+int check_transformation_handler(const fx_model_desc& m, const std::vector<int32_t>& node_rank,
+                                 const std::vector<int32_t>& node_eq, std::string& err);
 int compile_model(const fx_model_desc& m, int renumber_rcm, int simd_width, int build_coloring, Mesh& out,
                   std::string& err);
 

@@ -73,7 +73,14 @@ typedef struct fx_model_desc {
   const int32_t* tie_retained;
   const int32_t* tie_constrained;
   const int32_t* tie_dof;
+  const int32_t* tie_command; /* [ntie] equalDOF command of each row (EqualDof::command), NULL:
+                                 one command per (retained, constrained) pair */
 } fx_model_desc;
+/* Node i of the model has tag i + 1 (the constraint rules process nodes in ascending tag order,
+ * dof_numbering.cpp:117-129). fx_compile_model refuses, with FX_EINVAL and a message naming the
+ * pattern, the models the reference refuses (dof_numbering.cpp:162-262, 303-339): a node tied
+ * by two equalDOF commands, a fix on a tied DOF whose retained DOF is free, partial chains and
+ * chains through a node of larger tag. */
 
 typedef struct fx_mesh_view {
   int32_t nnode, nelem, padded, neq, nparam, ncolor;

@@ -54,6 +54,7 @@ typedef struct fxo_model {
   const int32_t* tie_retained;
   const int32_t* tie_constrained;
   const int32_t* tie_dof;  /* 0..2 */
+  const int32_t* tie_command; /* equalDOF command id per row, NULL: per (retained, constrained) */
 } fxo_model;
 
 typedef struct fxo_compiled {
@@ -77,7 +78,14 @@ typedef struct fxo_compiled {
   double* slots;         /* [24*padded] work */
 } fxo_compiled;
 
-/* src/compile/renumber.cpp:30-183, dof_numbering.cpp:21-40,269-485, batches.cpp:16-227 */
+/* src/compile/renumber.cpp:30-183, dof_numbering.cpp:21-40,269-485, batches.cpp:16-227.
+ * Returns 0, or the pattern of a model the reference refuses (dof_numbering.cpp:162-339);
+ * node i has tag i + 1. */
+#define FXO_REFUSE_COMMANDS 2      /* a node tied by two equalDOF commands */
+#define FXO_REFUSE_FIX_LOST 3      /* a fix on a tied DOF whose retained DOF is free */
+#define FXO_REFUSE_PARTIAL_CHAIN 4 /* C -> R -> M with different DOF lists */
+#define FXO_REFUSE_LARGER_TAG 5    /* C -> R -> M with R's tag larger than C's */
+#define FXO_REFUSE_OTHER 6
 int fxo_compile(const fxo_model* m, int renumber_rcm, int simd_width, int build_coloring,
                 fxo_compiled* out);
 void fxo_compiled_free(fxo_compiled* c);

@@ -20,7 +20,7 @@ class Model(C.Structure):
     _fields_ = [("nnode", C.c_int32), ("xyz", C.c_void_p), ("nelem", C.c_int32), ("conn", C.c_void_p),
                 ("group", C.c_void_p), ("region", C.c_void_p), ("nfix", C.c_int32), ("fix_node", C.c_void_p),
                 ("fix_dof", C.c_void_p), ("ntie", C.c_int32), ("tie_retained", C.c_void_p),
-                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p)]
+                ("tie_constrained", C.c_void_p), ("tie_dof", C.c_void_p), ("tie_command", C.c_void_p)]
 
 
 class Compiled(C.Structure):
@@ -126,6 +126,18 @@ def elastic_params(params):
     return arr
 
 
+REFUSALS = {2: "equalDOF commands", 3: "the fix is silently lost", 4: "partial chain", 5: "has a larger tag",
+            6: "other"}
+
+
+class RefusedModel(RuntimeError):
+    """fxo_compile refused the model; .code is the pattern (FXO_REFUSE_*)."""
+
+    def __init__(self, code):
+        super().__init__(f"fxo_compile refused the model: {REFUSALS.get(code, code)}")
+        self.code = code
+
+
 class OracleMesh:
     """A compiled model (fxo_compile) with numpy views of its arrays."""
 
@@ -155,11 +167,13 @@ class OracleMesh:
         m.tie_retained = ptr(np.asarray(tie)[:, 0] if len(tie) else np.zeros(1), np.int32)
         m.tie_constrained = ptr(np.asarray(tie)[:, 1] if len(tie) else np.zeros(1), np.int32)
         m.tie_dof = ptr(np.asarray(tie)[:, 2] if len(tie) else np.zeros(1), np.int32)
+        cmd = model.get("tie_command")
+        m.tie_command = ptr(cmd, np.int32) if cmd is not None and len(tie) else None
         self.model = m
         self.c = Compiled()
         rc = lib().fxo_compile(C.byref(m), int(renumber), int(simd), int(coloring), C.byref(self.c))
         if rc != 0:
-            raise RuntimeError("fxo_compile failed")
+            raise RefusedModel(rc)
         c = self.c
         N, P, E = c.nnode, c.padded, c.nelem
         cp = lambda p, n: np.ctypeslib.as_array(p, shape=(n,)).copy() if n > 0 else np.zeros(0)

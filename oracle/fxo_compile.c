@@ -235,10 +235,169 @@ static void uf_join(int32_t* parent, int32_t a, int32_t b) {
   parent[b] = a;
 }
 
+/* ---- Constraint refusals (dof_numbering.cpp:44-262, 303-339) ------------------------------
+ * Rule 1 (:303-339): one equalDOF command per constrained node; a command is the row's
+ * command id, or without ids the (retained, constrained) pair.
+ * Rule 2 (:61-150, 241-262): emulate OpenSees' TransformationConstraintHandler (nodes in
+ * ascending tag order, tag = model index + 1) and compare with the union-find numbering: the
+ * fixed sets must agree and the handler's values and the equations be one to one. On failure
+ * the pattern is classified as refusal() does (:162-237). */
+#define FXO_MAXID 8
+static int classify_refusal(const fxo_model* m) {
+  /* a fix on a tied DOF whose retained DOF is free (:189-206) */
+  for (int32_t r = 0; r < m->nfix; ++r) {
+    const int32_t C = m->fix_node[r], d = m->fix_dof[r];
+    int32_t R = -1, tied = 0;
+    for (int32_t t = 0; t < m->ntie; ++t)
+      if (m->tie_constrained[t] == C) {
+        R = m->tie_retained[t];
+        if (m->tie_dof[t] == d) tied = 1;
+      }
+    if (!tied) continue;
+    int rfixed = 0;
+    for (int32_t q = 0; q < m->nfix; ++q) rfixed |= m->fix_node[q] == R && m->fix_dof[q] == d;
+    if (!rfixed) return FXO_REFUSE_FIX_LOST;
+  }
+  /* chains C -> R -> M (:207-233): partial if the DOF lists differ, else refused if R > C */
+  for (int32_t t = 0; t < m->ntie; ++t) {
+    const int32_t C = m->tie_constrained[t];
+    int32_t R = -1;
+    for (int32_t u = 0; u < m->ntie; ++u)
+      if (m->tie_constrained[u] == C) R = m->tie_retained[u];
+    int chained = 0;
+    for (int32_t u = 0; u < m->ntie; ++u) chained |= m->tie_constrained[u] == R;
+    if (!chained) continue;
+    /* DOF lists in row order */
+    int32_t dc[FXO_MAXID], dr[FXO_MAXID], nc = 0, nr = 0;
+    for (int32_t u = 0; u < m->ntie; ++u) {
+      if (m->tie_constrained[u] == C && nc < FXO_MAXID) dc[nc++] = m->tie_dof[u];
+      if (m->tie_constrained[u] == R && nr < FXO_MAXID) dr[nr++] = m->tie_dof[u];
+    }
+    int same = nc == nr;
+    for (int32_t k = 0; same && k < nc; ++k) same = dc[k] == dr[k];
+    if (!same) return FXO_REFUSE_PARTIAL_CHAIN;
+    if (R > C) return FXO_REFUSE_LARGER_TAG;
+  }
+  return FXO_REFUSE_OTHER;
+}
+
+static int check_commands(const fxo_model* m) {
+  for (int32_t r = 0; r < m->ntie; ++r) {
+    const int64_t id = m->tie_command ? (int64_t)m->tie_command[r] : -1 - (int64_t)m->tie_retained[r];
+    for (int32_t q = 0; q < r; ++q) {
+      if (m->tie_constrained[q] != m->tie_constrained[r]) continue;
+      const int64_t iq = m->tie_command ? (int64_t)m->tie_command[q] : -1 - (int64_t)m->tie_retained[q];
+      if (iq != id) return FXO_REFUSE_COMMANDS;
+    }
+  }
+  return 0;
+}
+
+static int check_handler(const fxo_model* m, const int32_t* rank, const int32_t* node_eq) {
+  const int32_t N = m->nnode;
+  int32_t* retained = (int32_t*)malloc(((size_t)N + 1) * sizeof(int32_t));
+  int32_t* nd = (int32_t*)calloc((size_t)N + 1, sizeof(int32_t));        /* MP DOF count */
+  int32_t* dofs = (int32_t*)malloc(((size_t)N * FXO_MAXID + 1) * sizeof(int32_t));
+  int32_t* nid = (int32_t*)calloc((size_t)N + 1, sizeof(int32_t));
+  int64_t* id = (int64_t*)malloc(((size_t)N * FXO_MAXID + 1) * sizeof(int64_t));
+  int32_t* nunc = (int32_t*)calloc((size_t)N + 1, sizeof(int32_t));
+  int32_t* unc = (int32_t*)malloc(((size_t)N * 3 + 1) * sizeof(int32_t));
+  for (int32_t i = 0; i < N; ++i) retained[i] = -1;
+  for (int32_t r = 0; r < m->ntie; ++r) {
+    const int32_t c = m->tie_constrained[r];
+    retained[c] = m->tie_retained[r];
+    if (nd[c] < FXO_MAXID - 3) dofs[(size_t)c * FXO_MAXID + nd[c]++] = m->tie_dof[r];
+  }
+  for (int32_t i = 0; i < N; ++i) {
+    int64_t* I = id + (size_t)i * FXO_MAXID;
+    const int32_t* D = dofs + (size_t)i * FXO_MAXID;
+    if (retained[i] < 0) {
+      nid[i] = 3;
+      for (int d = 0; d < 3; ++d) I[d] = 3 * (int64_t)i + d;
+      for (int32_t r = 0; r < m->nfix; ++r)
+        if (m->fix_node[r] == i) I[m->fix_dof[r]] = -1;
+      continue;
+    }
+    for (int d = 0; d < 3; ++d) {
+      int in = 0;
+      for (int32_t k = 0; k < nd[i]; ++k) in |= D[k] == d;
+      if (!in) unc[3 * (size_t)i + nunc[i]++] = d;
+    }
+    nid[i] = nunc[i] + nd[i];
+    for (int32_t p = 0; p < nid[i]; ++p) I[p] = -1;
+    for (int32_t p = 0; p < nunc[i]; ++p) I[p] = 3 * (int64_t)i + unc[3 * (size_t)i + p];
+    for (int32_t r = 0; r < m->nfix; ++r) {
+      if (m->fix_node[r] != i) continue;
+      int32_t loc = 0;
+      for (int32_t d = 0; d < m->fix_dof[r]; ++d) {
+        int in = 0;
+        for (int32_t k = 0; k < nd[i]; ++k) in |= D[k] == d;
+        loc += !in;
+      }
+      if (loc < nid[i]) I[loc] = -1;
+    }
+  }
+  for (int32_t i = 0; i < N; ++i) { /* doneID in ascending tag order */
+    if (retained[i] < 0) continue;
+    const int64_t* O = id + (size_t)retained[i] * FXO_MAXID;
+    for (int32_t k = 0; k < nd[i]; ++k) {
+      const int32_t pos = dofs[(size_t)i * FXO_MAXID + k];
+      id[(size_t)i * FXO_MAXID + nunc[i] + k] = pos < nid[retained[i]] ? O[pos] : -1;
+    }
+  }
+  /* one-to-one check through dense maps: value (a DOF id or -1) <-> equation */
+  const int64_t ndof = 3 * (int64_t)N;
+  int32_t* eq_of_value = (int32_t*)malloc(((size_t)ndof + 1) * sizeof(int32_t));
+  int64_t* value_of_eq = (int64_t*)malloc(((size_t)ndof + 1) * sizeof(int64_t));
+  for (int64_t a = 0; a < ndof; ++a) {
+    eq_of_value[a] = -1;
+    value_of_eq[a] = -2;
+  }
+  int bad = 0;
+  for (int32_t i = 0; i < N && !bad; ++i)
+    for (int d = 0; d < 3 && !bad; ++d) {
+      int64_t v;
+      const int32_t* D = dofs + (size_t)i * FXO_MAXID;
+      const int64_t* I = id + (size_t)i * FXO_MAXID;
+      if (retained[i] < 0) {
+        v = I[d];
+      } else {
+        int32_t k = 0;
+        while (k < nd[i] && D[k] != d) ++k;
+        if (k == nd[i]) {
+          int32_t p = 0;
+          while (unc[3 * (size_t)i + p] != d) ++p;
+          v = I[p];
+        } else {
+          v = I[nunc[i] + k];
+        }
+      }
+      const int32_t e = node_eq[3 * (size_t)rank[i] + d];
+      if ((v < 0) != (e < 0)) {
+        bad = 1;
+      } else if (e >= 0) {
+        if (eq_of_value[v] < 0) eq_of_value[v] = e;
+        if (value_of_eq[e] == -2) value_of_eq[e] = v;
+        bad = eq_of_value[v] != e || value_of_eq[e] != v;
+      }
+    }
+  free(retained);
+  free(nd);
+  free(dofs);
+  free(nid);
+  free(id);
+  free(nunc);
+  free(unc);
+  free(eq_of_value);
+  free(value_of_eq);
+  return bad ? classify_refusal(m) : 0;
+}
+
 int fxo_compile(const fxo_model* m, int renumber_rcm, int simd_width, int build_coloring,
                 fxo_compiled* c) {
   memset(c, 0, sizeof(*c));
   const int32_t N = m->nnode, E = m->nelem;
+  if (check_commands(m)) return FXO_REFUSE_COMMANDS;
   c->nnode = N;
   c->nelem = E;
   /* --- ordering ------------------------------------------------------------------------ */
@@ -294,6 +453,13 @@ int fxo_compile(const fxo_model* m, int renumber_rcm, int simd_width, int build_
   free(parent);
   free(fixed_root);
   free(eq_of_root);
+  {
+    const int refused = check_handler(m, c->node_rank, c->node_eq);
+    if (refused) {
+      fxo_compiled_free(c);
+      return refused;
+    }
+  }
   /* --- coordinates, SoA by internal node (batches.cpp:66-70) ----------------------------- */
   c->coords = (double*)malloc(((size_t)3 * N + 1) * sizeof(double));
   for (int32_t i = 0; i < N; ++i)

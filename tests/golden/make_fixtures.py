@@ -58,6 +58,7 @@ def pack(files, out):
 
 def read_case(path):
     nodes, fixes, ties, expect = [], [], [], ""
+    commands = 0
     with open(path) as f:
         for line in f:
             t = line.split()
@@ -68,8 +69,10 @@ def read_case(path):
             elif t[0] == "fix":
                 fixes.append([int(t[1]), int(t[2])])
             elif t[0] == "equal":
+                # one equalDOF command per "equal" line (the id is the line's ordinal)
                 for d in t[3:]:
-                    ties.append([int(t[1]), int(t[2]), int(d)])
+                    ties.append([int(t[1]), int(t[2]), int(d), commands])
+                commands += 1
             elif t[0] == "expect":
                 expect = " ".join(t[1:])
     return nodes, fixes, ties, expect
@@ -83,7 +86,7 @@ def pack_constraints(ref, out):
         nodes, fixes, ties, expect = read_case(os.path.join(d, "cases", name + ".txt"))
         arrays[name + "/nodes"] = np.array(nodes, dtype=np.float64).reshape(-1, 5)
         arrays[name + "/fix"] = np.array(fixes, dtype=np.int32).reshape(-1, 2)
-        arrays[name + "/equal"] = np.array(ties, dtype=np.int32).reshape(-1, 3)
+        arrays[name + "/equal"] = np.array(ties, dtype=np.int32).reshape(-1, 4)
         arrays[name + "/expect"] = np.array(expect)
         eqf = os.path.join(d, name + ".txt")
         if os.path.exists(eqf):
