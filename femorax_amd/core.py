@@ -70,6 +70,11 @@ class SolveOptions(C.Structure):
                 ("max_iterations", C.c_int32)]
 
 
+class CdDesc(C.Structure):
+    _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("dash", C.c_void_p), ("nload", C.c_int32),
+                ("load_eq", C.c_void_p), ("load_hist", C.c_void_p)]
+
+
 class SolveResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
                 ("outer_iterations", C.c_int32), ("rhs_norm", C.c_double), ("initial_residual", C.c_double),
@@ -108,6 +113,11 @@ SIGNATURES = [
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
     ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
+    ("fx_lumped_mass", C.c_int, [vp, vp, vp]),
+    ("fx_body_load", C.c_int, [vp, vp]),
+    ("fx_force", C.c_int, [vp, vp, vp, vp]),
+    ("fx_critical_dt", C.c_int, [vp, C.c_int, C.POINTER(dbl)]),
+    ("fx_explicit_cd", C.c_int, [vp, C.POINTER(CdDesc), vp, vp, vp, vp]),
 ]
 
 
@@ -380,6 +390,28 @@ class Batch:
         _check(lib().fx_cg_mixed(self.h, C.byref(k or coeffs()), C.byref(o), inner_rtol, b.ptr, x.ptr, C.byref(r)),
                self.ctx)
         return r.as_dict()
+
+    def lumped_mass(self, m, am=None):
+        _check(lib().fx_lumped_mass(self.h, m.ptr, am.ptr if am is not None else None), self.ctx)
+
+    def body_load(self, fb):
+        _check(lib().fx_body_load(self.h, fb.ptr), self.ctx)
+
+    def force(self, u, f, v=None):
+        _check(lib().fx_force(self.h, u.ptr, v.ptr if v is not None else None, f.ptr), self.ctx)
+
+    def critical_dt(self, scheme=0):
+        dt = C.c_double()
+        _check(lib().fx_critical_dt(self.h, scheme, C.byref(dt)), self.ctx)
+        return dt.value
+
+    def explicit_cd(self, dt, nsteps, u, um, v, a, dash=None, load_eq=None, load_hist=None):
+        """nsteps CentralDifference steps on device arrays u, um, v, a (in place)."""
+        le = np.ascontiguousarray(load_eq if load_eq is not None else np.zeros(0), np.int32)
+        lh = np.ascontiguousarray(load_hist if load_hist is not None else np.zeros(0), np.float64)
+        d = CdDesc(dt, nsteps, dash.ptr if dash is not None else None, len(le), le.ctypes.data if len(le) else None,
+                   lh.ctypes.data if len(lh) else None)
+        _check(lib().fx_explicit_cd(self.h, C.byref(d), u.ptr, um.ptr, v.ptr, a.ptr), self.ctx)
 
 
 def copy(ctx, src, dst):

@@ -189,7 +189,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
-                  b->shared_list, b->partial64, b->partial32, b->perm};
+                  b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -329,6 +329,63 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
           ok(dev_upload(ctx, b->con_start, con.start.data(), con.start.size(), by)) &&
           ok(dev_upload(ctx, b->con_list, con.list.data(), con.list.size(), by));
     }
+  }
+  // lumped mass, lumped alphaM mass and body load (batches.cpp:204-224; std_brick.hpp:519-542,
+  // 609-623): rho, alphaM rho and b times the node weights sum_gp N_j dvol, in element order.
+  // The congruent boxes of a lattice batch share one set of weights.
+  if (rc == FX_OK) {
+    const int32_t n = d->neq;
+    std::vector<double> lm(n, 0.0), la(n, 0.0), fb(n, 0.0);
+    bool any_am = false, any_body = false;
+    double w[8];
+    auto element_xe = [&](int32_t e, double (&xe)[8][3]) {
+      for (int a = 0; a < 8; ++a) {
+        const int32_t nd = d->node[static_cast<size_t>(a) * P + e];
+        for (int k = 0; k < 3; ++k) xe[a][k] = d->coords[static_cast<size_t>(k) * N + nd];
+      }
+    };
+    if (use_lattice) {
+      const double box[8][3] = {{0, 0, 0}, {lat.hx, 0, 0}, {lat.hx, lat.hy, 0}, {0, lat.hy, 0},
+                                {0, 0, lat.hz}, {lat.hx, 0, lat.hz}, {lat.hx, lat.hy, lat.hz}, {0, lat.hy, lat.hz}};
+      fx::brick_node_weights(box, w);
+    }
+    for (int32_t e = 0; e < E; ++e) {
+      if (!use_lattice) {
+        double xe[8][3];
+        element_xe(e, xe);
+        fx::brick_node_weights(xe, w);
+      }
+      const fx::DevParam& p = b->host_params[d->param[e]];
+      for (int k = 0; k < 24; ++k) {
+        const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
+        if (q < 0) continue;
+        const double mj = p.rho * w[k / 3];
+        lm[q] += mj;
+        la[q] += p.alphaM * mj;
+        fb[q] += p.b[k % 3] * w[k / 3];
+        any_am |= p.alphaM != 0.0;
+        any_body |= p.b[k % 3] != 0.0;
+      }
+    }
+    if (use_lattice) {
+      b->internal_of.assign(n, -1);
+      for (int32_t i = 0; i < n; ++i) b->internal_of[lat.perm[i]] = i;
+      auto to_int = [&](std::vector<double>& v) {
+        std::vector<double> t(n);
+        for (int32_t i = 0; i < n; ++i) t[i] = v[lat.perm[i]];
+        v.swap(t);
+      };
+      to_int(lm);
+      to_int(la);
+      to_int(fb);
+    } else {
+      b->host_node.assign(d->node, d->node + static_cast<size_t>(8) * P);
+      b->host_param.assign(d->param, d->param + P);
+      b->host_coords.assign(d->coords, d->coords + static_cast<size_t>(3) * N);
+    }
+    ok(n > 0 ? dev_upload(ctx, b->lumped_m, lm.data(), lm.size(), by) : FX_OK) &&
+        ok(any_am ? dev_upload(ctx, b->lumped_am, la.data(), la.size(), by) : FX_OK) &&
+        ok(any_body ? dev_upload(ctx, b->body, fb.data(), fb.size(), by) : FX_OK);
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);
@@ -477,3 +534,83 @@ int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, do
 }
 
 }  // extern "C"
+
+int fx_lumped_mass(fx_batch* b, double* m, double* am) {
+  if (!b || !m) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_lumped_mass: null argument");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  if (int rc = fx::to_external<double>(b, b->lumped_m, m)) return rc;
+  if (am) {
+    if (b->lumped_am) {
+      if (int rc = fx::to_external<double>(b, b->lumped_am, am)) return rc;
+    } else {
+      FX_HIP(b->ctx, hipMemsetAsync(am, 0, sizeof(double) * b->neq, b->ctx->stream));
+    }
+  }
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_body_load(fx_batch* b, double* fb) {
+  if (!b || !fb) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_body_load: null argument");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  if (b->body) {
+    if (int rc = fx::to_external<double>(b, b->body, fb)) return rc;
+  } else {
+    FX_HIP(b->ctx, hipMemsetAsync(fb, 0, sizeof(double) * b->neq, b->ctx->stream));
+  }
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_force(fx_batch* b, const double* u, const double* v, double* f) {
+  if (!b || !u || !f) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_force: null argument");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  const size_t n = static_cast<size_t>(b->neq);
+  double* W = b->work;
+  if (b->internal) {
+    if (int rc = fx::to_internal<double>(b, u, W)) return rc;
+    if (v)
+      if (int rc = fx::to_internal<double>(b, v, W + n)) return rc;
+    if (int rc = fx::force64(b, W, v ? W + n : nullptr, W + 2 * n, W + 3 * n)) return rc;
+    if (int rc = fx::to_external<double>(b, W + 2 * n, f)) return rc;
+  } else {
+    if (int rc = fx::force64(b, u, v, f, W)) return rc;
+  }
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
+  if (!b || !dt || (scheme != 0 && scheme != 1)) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_critical_dt: bad argument");
+  double best = INFINITY;
+  if (b->lattice) {
+    const double hx = b->hx, hy = b->hy, hz = b->hz;
+    const double box[8][3] = {{0, 0, 0}, {hx, 0, 0}, {hx, hy, 0}, {0, hy, 0}, {0, 0, hz}, {hx, 0, hz}, {hx, hy, hz}, {0, hy, hz}};
+    for (const fx::DevParam& p : b->host_params)
+      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(box, p.lam, p.mu, p.rho)));
+  } else {
+    const int32_t P = b->padded, N = b->nnode;
+    for (int32_t e = 0; e < b->count; ++e) {
+      double xe[8][3];
+      for (int a = 0; a < 8; ++a) {
+        const int32_t nd = b->host_node[static_cast<size_t>(a) * P + e];
+        for (int k = 0; k < 3; ++k) xe[a][k] = b->host_coords[static_cast<size_t>(k) * N + nd];
+      }
+      const fx::DevParam& p = b->host_params[b->host_param[e]];
+      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(xe, p.lam, p.mu, p.rho)));
+    }
+  }
+  *dt = best;
+  return FX_OK;
+}
+
+int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, double* v, double* a) {
+  if (!b || !d || !u || !um || !v || !a) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_explicit_cd: null argument");
+  if (!(d->dt > 0) || d->nsteps < 0 || d->nload < 0 || d->nload > FX_MAX_POINT_LOADS ||
+      (d->nload > 0 && (!d->load_eq || !d->load_hist)))
+    return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: bad descriptor");
+  for (int j = 0; j < d->nload; ++j)
+    if (d->load_eq[j] < 0 || d->load_eq[j] >= b->neq) return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: load equation out of range");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  return fx::cd_run(b, *d, u, um, v, a);
+}

@@ -1,0 +1,196 @@
+// Internal force of elastic stdBrick batches and the explicit CentralDifference loop.
+//
+// For ElasticIsotropic the element force is linear: f = K u + C v - F_body (std_brick.hpp:
+// 493-543 with element_operator's Factors), so the force runs on the operator kernels (K u,
+// and C v with the operator coefficients (0, 1, 0)) plus a combine kernel; F_body is a vector
+// built once per batch. The CentralDifference step (explicit_analysis.cpp:87-117, 712-722)
+// is one device-resident loop: per step the garbage velocity v_e = -c2 u(n-1) (only when the
+// elements damp), the force, the point loads, and one fused update per equation that forms
+// R = -f - M a_g + P, solves the diagonal system (c3 M + c2 C) U = R and writes U, the recorded
+// velocity and acceleration. U goes into u(n-1)'s buffer and the two buffers swap roles, so the
+// update writes 3 vectors, not 4.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+
+namespace fx {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+inline unsigned blocks_for(int64_t n) { return static_cast<unsigned>((n + kThreads - 1) / kThreads); }
+
+__global__ void scale_kernel(int32_t n, double s, const double* __restrict__ x, double* __restrict__ y) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) y[q] = x[q] * s;
+}
+
+// f = Ku (+ Cv) - F_body
+__global__ void force_combine_kernel(int32_t n, const double* __restrict__ ku, const double* __restrict__ cv,
+                                     const double* __restrict__ body, double* __restrict__ f) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  double v = ku[q];
+  if (cv) v += cv[q];
+  if (body) v -= body[q];
+  f[q] = v;
+}
+
+struct PointLoads {
+  int32_t n;
+  int32_t eq[kMaxPointLoads];
+  double val[kMaxPointLoads];
+};
+
+// f(q) -= P(q): the point loads join the residual R = -f + P
+__global__ void point_load_kernel(PointLoads p, double* f) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    for (int j = 0; j < p.n; ++j) f[p.eq[j]] -= p.val[j];
+}
+
+struct CdArgs {
+  int32_t n;
+  double c2, c3, dt;
+  const double* f;     // K u (+ loads folded in)
+  const double* fd;    // C v_e of the elements, or null
+  const double* body;  // F_body, or null
+  const double* m;
+  const double* am;    // lumped alphaM mass, or null
+  const double* dash;  // dampers outside the elements, or null
+  const double* u;     // U(n)
+  double* um;          // in: U(n-1); out: U(n+1)
+  double* v;
+  double* a;
+};
+
+__global__ __launch_bounds__(kThreads) void cd_update_kernel(CdArgs g) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= g.n) return;
+  const double uq = g.u[q], umq = g.um[q], mq = g.m[q];
+  double fq = g.f[q];
+  if (g.fd) fq += g.fd[q];
+  if (g.body) fq -= g.body[q];
+  double cq = 0.0;
+  if (g.am) cq += g.am[q];
+  if (g.dash) {
+    const double dq = g.dash[q];
+    fq += dq * (umq * -g.c2);  // the damper's force at the garbage velocity
+    cq += dq;
+  }
+  const double ag = uq * (-2.0 * g.c3) + umq * g.c3;
+  const double R = -fq - mq * ag;
+  const double A = mq * g.c3 + cq * g.c2;
+  const double U = R / A;
+  double Udot = U * 3.0;
+  Udot += uq * -4.0;
+  Udot += umq;
+  Udot *= g.c2;
+  const double Udotdot = (Udot - g.v[q]) / g.dt;
+  g.um[q] = U;
+  g.v[q] = Udot;
+  g.a[q] = Udotdot;
+}
+
+bool elements_damp(const fx_batch* b) {
+  for (const DevParam& p : b->host_params)
+    if (p.alphaM != 0.0 || p.betaK != 0.0 || p.betaKinit != 0.0 || p.betaKcomm != 0.0) return true;
+  return false;
+}
+
+}  // namespace
+
+int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch) {
+  const int32_t n = b->neq;
+  if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, u, f)) return rc;
+  const double* cv = nullptr;
+  if (v && elements_damp(b)) {
+    if (int rc = operator_apply64(b, fx_coeffs{0.0, 1.0, 0.0, 0}, v, scratch)) return rc;
+    cv = scratch;
+  }
+  if (cv || b->body)
+    hipLaunchKernelGGL(force_combine_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, b->ctx->stream, n, f, cv,
+                       b->body, f);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a) {
+  fx_ctx* ctx = b->ctx;
+  const int32_t n = b->neq;
+  if (n == 0 || d.nsteps <= 0) return FX_OK;
+  double* W = b->work;
+  const size_t N = static_cast<size_t>(n);
+  // state in the solver numbering
+  double *U = u, *Um = um, *V = v, *Acc = a, *dash = const_cast<double*>(d.dash);
+  if (b->internal) {
+    U = W;
+    Um = W + N;
+    V = W + 2 * N;
+    Acc = W + 3 * N;
+    if (int rc = to_internal<double>(b, u, U)) return rc;
+    if (int rc = to_internal<double>(b, um, Um)) return rc;
+    if (int rc = to_internal<double>(b, v, V)) return rc;
+    if (int rc = to_internal<double>(b, a, Acc)) return rc;
+    if (d.dash) {
+      dash = W + 6 * N;
+      if (int rc = to_internal<double>(b, d.dash, dash)) return rc;
+    }
+  }
+  double* f = W + 4 * N;
+  double* ve = W + 5 * N;
+  double* fd = b->internal ? W + 7 * N : W + 6 * N;
+  const bool damp = elements_damp(b);
+  const double c2 = 0.5 / d.dt, c3 = 1.0 / (d.dt * d.dt);
+  PointLoads pl{};
+  pl.n = d.nload;
+  for (int j = 0; j < d.nload; ++j) pl.eq[j] = b->internal ? b->internal_of[d.load_eq[j]] : d.load_eq[j];
+  CdArgs g{};
+  g.n = n;
+  g.c2 = c2;
+  g.c3 = c3;
+  g.dt = d.dt;
+  g.f = f;
+  g.fd = damp ? fd : nullptr;
+  g.body = b->body;
+  g.m = b->lumped_m;
+  g.am = b->lumped_am;
+  g.dash = dash;
+  for (int32_t k = 0; k < d.nsteps; ++k) {
+    if (damp) {
+      hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, n, -c2, Um, ve);
+      if (int rc = operator_apply64(b, fx_coeffs{0.0, 1.0, 0.0, 0}, ve, fd)) return rc;
+    }
+    if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, U, f)) return rc;
+    if (pl.n > 0) {
+      for (int j = 0; j < pl.n; ++j) pl.val[j] = d.load_hist[static_cast<size_t>(k) * pl.n + j];
+      hipLaunchKernelGGL(point_load_kernel, dim3(1), dim3(64), 0, ctx->stream, pl, f);
+    }
+    g.u = U;
+    g.um = Um;
+    g.v = V;
+    g.a = Acc;
+    hipLaunchKernelGGL(cd_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
+    std::swap(U, Um);  // U(n+1) sits in the old U(n-1) buffer
+  }
+  FX_HIP(ctx, hipGetLastError());
+  // back to the caller's buffers: u = U(n+1), um = U(n)
+  if (b->internal) {
+    if (int rc = to_external<double>(b, U, u)) return rc;
+    if (int rc = to_external<double>(b, Um, um)) return rc;
+    if (int rc = to_external<double>(b, V, v)) return rc;
+    if (int rc = to_external<double>(b, Acc, a)) return rc;
+  } else if (U != u) {  // an odd number of steps: the roles are swapped
+    if (int rc = copy64(ctx, n, U, ve)) return rc;
+    if (int rc = copy64(ctx, n, Um, U)) return rc;
+    if (int rc = copy64(ctx, n, ve, Um)) return rc;
+  }
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+}  // namespace fx

@@ -122,6 +122,16 @@ struct DevParam {  // per parameter set, device
 };
 }  // namespace fx
 
+namespace fx {
+constexpr int kMaxPointLoads = 32;
+// Host stdBrick helpers (fx_brick_host.cpp).
+void brick_node_weights(const double (&xe)[8][3], double (&w)[8]);
+void brick_stiffness(const double (&xe)[8][3], double lam, double mu, double (&K)[24][24]);
+double max_eigenvalue_symmetric(double (&A)[24][24]);
+double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho);
+double stable_dt(int scheme, const DevParam& p, double omega_sq);
+}  // namespace fx
+
 struct fx_batch {
   fx_ctx* ctx = nullptr;
   int32_t count = 0, padded = 0, nnode = 0, neq = 0, nparam = 0;
@@ -160,6 +170,14 @@ struct fx_batch {
   // free equations no element touches: zeroed after each apply
   int32_t* orphans = nullptr;
   int32_t norphans = 0;
+  // lumped mass, lumped alphaM mass and body load in the solver numbering (null when all zero)
+  double* lumped_m = nullptr;
+  double* lumped_am = nullptr;
+  double* body = nullptr;
+  std::vector<int32_t> internal_of;   // host: reference equation -> internal (lattice batches)
+  // host copies for the stable time step of general batches (lattice batches need none)
+  std::vector<int32_t> host_node, host_param;
+  std::vector<double> host_coords;
   // solver work (allocated at create: compute allocates nothing)
   double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats

@@ -63,4 +63,8 @@ int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const doubl
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
              double* x, fx_solve_result& r);
 
+// Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.
+int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch);
+int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a);
+
 }  // namespace fx

@@ -154,3 +154,68 @@ def config_c2(n=128, seed=SEED_BASE + 1):
     t = np.array([1e4 * (1.0 + 0.5 * rng.symmetric()) for _ in range(NX * NX)])
     m["nodal_load"] = top_traction_nodal(m, t)
     return m
+
+
+def lysmer_dashpots(model, vs_of_group, nu=0.3):
+    """Lysmer-Kuhlemeyer boundary dashpots on the four sides and the bottom of a grid model:
+    each boundary face of an element gives its 4 corners rho Vp A/4 in the face's normal
+    direction and rho Vs A/4 in the two tangential ones (Vp = Vs sqrt(2(1-nu)/(1-2nu))).
+    Returns nodal dashpot coefficients [nnode, 3] (model order)."""
+    nx, ny, nz = model["grid"]
+    dx, dy, dz = model["h"]
+    NX, NY = nx + 1, ny + 1
+    C = np.zeros((model["nnode"], 3))
+    ratio = np.sqrt(2.0 * (1.0 - nu) / (1.0 - 2.0 * nu))
+    conn = model["conn"]
+    group = model["group"]
+    params = model["params"]
+    ek = np.arange(model["nelem"]) // (nx * ny)
+    ej = (np.arange(model["nelem"]) // nx) % ny
+    ei = np.arange(model["nelem"]) % nx
+    # (selector, face corners in Brick order, normal axis, face area)
+    faces = [(ei == 0, [0, 3, 7, 4], 0, dy * dz), (ei == nx - 1, [1, 2, 6, 5], 0, dy * dz),
+             (ej == 0, [0, 1, 5, 4], 1, dx * dz), (ej == ny - 1, [3, 2, 6, 7], 1, dx * dz),
+             (ek == 0, [0, 1, 2, 3], 2, dx * dy)]
+    for sel, corners, axis, area in faces:
+        for e in np.nonzero(sel)[0]:
+            g = group[e]
+            rho = params[g]["rho"]
+            vs = vs_of_group[g]
+            for c in corners:
+                n = conn[e, c]
+                for d in range(3):
+                    C[n, d] += rho * (vs * ratio if d == axis else vs) * area / 4.0
+    return C
+
+
+def ricker(t, f=5.0, amplitude=1.0e6):
+    """Ricker wavelet f0 (1 - 2 pi^2 f^2 tau^2) exp(-pi^2 f^2 tau^2), tau = t - 1.2 / f."""
+    tau = np.asarray(t) - 1.2 / f
+    a = (np.pi * f * tau) ** 2
+    return amplitude * (1.0 - 2.0 * a) * np.exp(-a)
+
+
+def config_c3(nx=200, ny=200, nz=100, h=5.0, layers=10, seed=SEED_BASE + 2):
+    """Explicit wave propagation box: nx x ny x nz bricks of h, `layers` equal layers with
+    Vs ~ U(150, 800) m/s, rho ~ U(1800, 2200) kg/m^3, nu 0.3 (G = rho Vs^2, E = 2G(1+nu)),
+    lumped mass, no fixed DOFs, Lysmer dashpots on the sides and bottom, a 5 Hz Ricker point
+    force (z) at the centre node of the surface."""
+    xyz, conn, ek = grid(nx, ny, nz, h, h, h, z0=-nz * h)
+    rng = Lcg(seed)
+    per = max(1, nz // layers)
+    # the top layer is group layers-1 counted from the base, as the grid counts k from the base
+    group = np.minimum(ek // per, layers - 1).astype(np.int32)
+    params, vs = [], []
+    for _ in range(layers):
+        v = rng.uniform(150.0, 800.0)
+        rho = rng.uniform(1800.0, 2200.0)
+        G = rho * v * v
+        params.append(dict(E=2.0 * G * 1.3, nu=0.3, rho=rho, b=(0.0, 0.0, 0.0), mass_type=1))
+        vs.append(v)
+    m = dict(nnode=len(xyz), xyz=xyz, nelem=len(conn), conn=conn, group=group, fix=np.zeros((0, 2), np.int32),
+             tie=np.zeros((0, 3), np.int32), params=params, grid=(nx, ny, nz), h=(h, h, h))
+    m["vs"] = vs
+    m["dashpots"] = lysmer_dashpots(m, vs)
+    NX, NY = nx + 1, ny + 1
+    m["source_node"] = (nx // 2) + NX * ((ny // 2) + NY * nz)
+    return m

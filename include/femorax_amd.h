@@ -200,6 +200,36 @@ int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const do
 int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol,
                 const double* rhs, double* x, fx_solve_result* res);
 
+/* ---- internal force and the explicit CentralDifference loop -------------------------------- */
+/* Lumped mass m and lumped alphaM mass am per equation (batches.cpp:204-224); am may be NULL. */
+int fx_lumped_mass(fx_batch* b, double* m, double* am);
+/* F_body: the body force's nodal loads b * sum_gp N_j dvol (std_brick.hpp:519-542). */
+int fx_body_load(fx_batch* b, double* fb);
+/* f = internal force at (u, v) (InternalForce::compute, internal_force.cpp:354-377): for
+ * ElasticIsotropic f = K u + C v - F_body. v may be NULL (no damping force). */
+int fx_force(fx_batch* b, const double* u, const double* v, double* f);
+/* Smallest element stable step (InternalForce::min_critical_dt; std_brick.hpp:658-694,
+ * explicit_stability.hpp:15-32): scheme 0 CentralDifference, 1 ExplicitDifference. Host work;
+ * lattice batches solve one eigenproblem per parameter set. */
+int fx_critical_dt(fx_batch* b, int scheme, double* dt);
+
+#define FX_MAX_POINT_LOADS 32
+typedef struct fx_cd_desc {
+  double dt;
+  int32_t nsteps;
+  const double* dash;        /* [neq] device: diagonal dampers outside the elements (e.g. Lysmer-
+                                Kuhlemeyer boundaries), NULL = none: their force dash * v_e joins
+                                the element forces at the garbage velocity and dash joins the
+                                tangent's C */
+  int32_t nload;             /* point loads, <= FX_MAX_POINT_LOADS */
+  const int32_t* load_eq;    /* [nload] host: equations */
+  const double* load_hist;   /* [nsteps * nload] host: P(t_k) of load j at [k * nload + j] */
+} fx_cd_desc;
+/* nsteps CentralDifference steps (explicit_analysis.cpp:87-117, 661-662, 712-722) with the
+ * batch's lumped mass and lumped alphaM mass: from u = U(n), um = U(n-1), v, a (device, in/out)
+ * to u = U(n+nsteps), um = U(n+nsteps-1) and the recorded v, a of the last step. */
+int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, double* v, double* a);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,28 @@ void fxo_pimy_initial_state(const fxo_pimy_params* p, double* state);
 void fxo_pimy_update(const fxo_pimy_params* p, const double* committed, double* trial, const double eps[6],
                      double sig[6], double D[36]);
 
+/* ---- Internal force, stable step, CentralDifference (fxo_time.c) ------------------------- */
+/* f = gather of the elements' stdBrick forces at u (damping force from v; body force), the
+ * flat force path (internal_force.cpp:354-377, element_kernels.hpp:198-235). */
+void fxo_force(const fxo_compiled* c, const fxo_elastic* params, const double* u, const double* v, double* f,
+               int nthreads);
+/* sym_eigen.hpp:11-49 (overwrites A) */
+double fxo_max_eigenvalue_symmetric24(double A[24][24]);
+/* largest omega^2 of M_l^-1/2 K M_l^-1/2 (std_brick.hpp:658-687) */
+double fxo_brick_max_omega_sq(const fxo_elastic* p, const double xe[8][3]);
+/* scheme 0 CentralDifference, 1 ExplicitDifference (explicit_stability.hpp:15-32) */
+double fxo_stable_dt(int scheme, const fxo_rayleigh* damp, double omega_sq);
+double fxo_critical_dt(const fxo_compiled* c, const fxo_elastic* params, int scheme);
+/* nsteps CentralDifference steps (explicit_analysis.cpp:87-117, 712-722) from (u = U_t,
+ * um = U_t-1, v, a); m, cm: the lumped mass and the damping diagonal of the tangent
+ * c3 M + c2 C (the reference's lumped alphaM mass; NULL = 0); dash: a diagonal damper outside the
+ * elements (NULL = none), whose force dash * v_elem joins the element forces at the garbage
+ * velocity v_elem = -c2 um, as element damping does; point loads load_hist[k*nload+j] on
+ * equation load_eq[j] at t_k join the residual. */
+void fxo_cd_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
+                const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt, int32_t nsteps,
+                double* u, double* um, double* v, double* a, int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

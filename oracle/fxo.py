@@ -79,6 +79,15 @@ def lib():
         L.fxo_body_load.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p]
         L.fxo_dot.restype = C.c_double
         L.fxo_dot.argtypes = [C.c_int64, f64p, f64p, C.c_int]
+        L.fxo_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, f64p, C.c_int]
+        L.fxo_brick_max_omega_sq.restype = C.c_double
+        L.fxo_brick_max_omega_sq.argtypes = [C.POINTER(Elastic), f64p]
+        L.fxo_stable_dt.restype = C.c_double
+        L.fxo_stable_dt.argtypes = [C.c_int, C.POINTER(Rayleigh), C.c_double]
+        L.fxo_critical_dt.restype = C.c_double
+        L.fxo_critical_dt.argtypes = [C.POINTER(Compiled), C.c_void_p, C.c_int]
+        L.fxo_cd_run.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
+                                 C.c_void_p, C.c_double, C.c_int32, f64p, f64p, f64p, f64p, C.c_int]
         L.fxo_cg.restype = C.c_int
         L.fxo_cg.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_double,
                              C.c_double, C.c_int32, C.c_int, C.c_int, C.POINTER(SolveResult)]
@@ -222,6 +231,26 @@ class OracleMesh:
         b = np.zeros(self.neq)
         lib().fxo_body_load(C.byref(self.c), C.cast(params, C.c_void_p), b)
         return b
+
+    def force(self, params, u, v=None, nthreads=0):
+        f = np.zeros(self.neq)
+        vv = None if v is None else np.ascontiguousarray(v, np.float64)
+        lib().fxo_force(C.byref(self.c), C.cast(params, C.c_void_p), np.ascontiguousarray(u, np.float64),
+                        None if vv is None else vv.ctypes.data, f, nthreads)
+        return f
+
+    def critical_dt(self, params, scheme=0):
+        return lib().fxo_critical_dt(C.byref(self.c), C.cast(params, C.c_void_p), scheme)
+
+    def cd_run(self, params, m, cm, dt, nsteps, u, um, v, a, load_eq=None, load_hist=None, dash=None, nthreads=0):
+        """nsteps CentralDifference steps; u, um, v, a (numpy) are updated in place."""
+        cmv = None if cm is None else np.ascontiguousarray(cm, np.float64)
+        dsv = None if dash is None else np.ascontiguousarray(dash, np.float64)
+        le = np.ascontiguousarray(load_eq if load_eq is not None else np.zeros(0), np.int32)
+        lh = np.ascontiguousarray(load_hist if load_hist is not None else np.zeros(0), np.float64)
+        lib().fxo_cd_run(C.byref(self.c), C.cast(params, C.c_void_p), np.ascontiguousarray(m, np.float64),
+                         None if cmv is None else cmv.ctypes.data, None if dsv is None else dsv.ctypes.data, len(le), le.ctypes.data, lh.ctypes.data, dt,
+                         nsteps, u, um, v, a, nthreads)
 
     def cg(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, lanes=1, k=None, nthreads=0):
         x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)

@@ -122,4 +122,18 @@ void fxref_pimy_update(const double* params, int load_stage, const double* commi
   std::memcpy(D, DD, sizeof DD);
 }
 
+// StdBrick::critical_dt (std_brick.hpp:629-694) with ElasticIsotropic (no state):
+// scheme 0 CentralDifference, 1 ExplicitDifference. Returns omega^2 in *omega_sq.
+double fxref_brick_critical_dt(const double* mat, const double* ray, int scheme, const double* xe,
+                               double* omega_sq) {
+  const double b[3] = {0, 0, 0};
+  const Brick::Params p = brick_params(mat, b, 0);
+  double x[8][3];
+  std::memcpy(x, xe, sizeof x);
+  const PointArray<const double> none{};
+  *omega_sq = Brick::max_omega_sq(p, x, none);
+  return Brick::critical_dt(p, rayleigh(ray), scheme == 0 ? ExplicitScheme::CentralDifference
+                                                         : ExplicitScheme::ExplicitDifference, x, none);
+}
+
 }  // extern "C"
