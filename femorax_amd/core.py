@@ -75,6 +75,50 @@ class CdDesc(C.Structure):
                 ("load_eq", C.c_void_p), ("load_hist", C.c_void_p)]
 
 
+class PimyBatchDesc(C.Structure):
+    _fields_ = [("count", C.c_int32), ("padded", C.c_int32), ("node", C.c_void_p), ("eq", C.c_void_p),
+                ("param", C.c_void_p), ("nparam", C.c_int32), ("params", C.c_void_p), ("nnode", C.c_int32),
+                ("coords", C.c_void_p), ("neq", C.c_int32)]
+
+
+class PimyMaterial(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rho", "G", "K", "cohesion", "gamma_peak", "phi", "pref", "pdc")] + [
+        ("nsurf", C.c_int32), ("gredu", C.c_void_p), ("b", C.c_double * 3), ("mass_type", C.c_int32)]
+
+
+PIMY_NSTATE = 190
+
+
+def pimy_material(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf=20, gredu=None, b=(0.0, 0.0, 0.0),
+                  mass_type=0, keep=None):
+    """fx_pimy_material; gredu: optional (gamma, G/Gmax) pairs (kept alive in `keep`)."""
+    g = None
+    if gredu is not None and len(gredu):
+        arr = np.ascontiguousarray(gredu, np.float64)
+        if keep is not None:
+            keep.append(arr)
+        g = arr.ctypes.data
+    return PimyMaterial(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf, g, (C.c_double * 3)(*b), mass_type)
+
+
+def pimy_surfaces(m):
+    size, modulus = np.zeros(21), np.zeros(21)
+    _check(lib().fx_pimy_surfaces(C.byref(m), size.ctypes.data, modulus.ctypes.data))
+    return size, modulus
+
+
+def pimy_init_state(ctx, mats, mat, state):
+    arr = (PimyMaterial * len(mats))(*mats)
+    _check(lib().fx_pimy_init_state(ctx.h, len(mats), arr, mat.n, mat.ptr, state.ptr), ctx)
+
+
+def pimy_points(ctx, mats, load_stage, mat, committed, trial, eps, sig, D=None):
+    arr = (PimyMaterial * len(mats))(*mats)
+    st = np.ascontiguousarray(load_stage, np.int32)
+    _check(lib().fx_pimy_points(ctx.h, len(mats), arr, st.ctypes.data, mat.n, mat.ptr, committed.ptr, trial.ptr,
+                                eps.ptr, sig.ptr, D.ptr if D is not None else None), ctx)
+
+
 class SolveResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
                 ("outer_iterations", C.c_int32), ("rhs_norm", C.c_double), ("initial_residual", C.c_double),
@@ -118,6 +162,14 @@ SIGNATURES = [
     ("fx_force", C.c_int, [vp, vp, vp, vp]),
     ("fx_critical_dt", C.c_int, [vp, C.c_int, C.POINTER(dbl)]),
     ("fx_explicit_cd", C.c_int, [vp, C.POINTER(CdDesc), vp, vp, vp, vp]),
+    ("fx_pimy_surfaces", C.c_int, [C.POINTER(PimyMaterial), vp, vp]),
+    ("fx_pimy_batch_create", C.c_int, [vp, C.POINTER(PimyBatchDesc), u32, C.POINTER(vp)]),
+    ("fx_set_material_stage", C.c_int, [vp, C.c_int32]),
+    ("fx_store_tangents", C.c_int, [vp, vp, vp]),
+    ("fx_commit_state", C.c_int, [vp]),
+    ("fx_pimy_state", C.c_int, [vp, C.c_int, vp]),
+    ("fx_pimy_init_state", C.c_int, [vp, C.c_int32, C.POINTER(PimyMaterial), i64, vp, vp]),
+    ("fx_pimy_points", C.c_int, [vp, C.c_int32, C.POINTER(PimyMaterial), vp, i64, vp, vp, vp, vp, vp, vp]),
 ]
 
 
@@ -342,6 +394,46 @@ class Batch:
             (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE)
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
+
+    @classmethod
+    def pimy(cls, ctx, mesh, materials):
+        """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
+        fx_pimy_material per compiled parameter set."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.neq = mesh.neq
+        self.npts = 8 * mesh.nelem
+        keep = []
+        self._params = (PimyMaterial * len(materials))(*materials)
+        d = PimyBatchDesc()
+        d.count = mesh.nelem
+        d.padded = mesh.padded
+        d.node = _arr(keep, mesh.node, np.int32)
+        d.eq = _arr(keep, mesh.eq, np.int32)
+        d.param = _arr(keep, mesh.param, np.int32)
+        d.nparam = len(materials)
+        d.params = C.cast(self._params, C.c_void_p)
+        d.nnode = mesh.nnode
+        d.coords = _arr(keep, mesh.coords, np.float64)
+        d.neq = mesh.neq
+        h = C.c_void_p()
+        _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), 0, C.byref(h)), ctx)
+        self.h = h
+        return self
+
+    def set_stage(self, stage):
+        _check(lib().fx_set_material_stage(self.h, stage), self.ctx)
+
+    def store_tangents(self, u, f=None):
+        _check(lib().fx_store_tangents(self.h, u.ptr, f.ptr if f is not None else None), self.ctx)
+
+    def commit(self):
+        _check(lib().fx_commit_state(self.h), self.ctx)
+
+    def pimy_state(self, committed=True):
+        d = DeviceArray(self.ctx, PIMY_NSTATE * self.npts)
+        _check(lib().fx_pimy_state(self.h, int(committed), d.ptr), self.ctx)
+        return d.get().reshape(PIMY_NSTATE, self.npts).T
 
     def close(self):
         if self.h:

@@ -8,6 +8,7 @@
 
 #include "fx_internal.hpp"
 #include "fx_kernels.hpp"
+#include "fx_pimy.hpp"
 
 struct fx_mesh {
   fx::Mesh m;
@@ -71,7 +72,9 @@ int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out) {
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<uint64_t>(fr / 10 * 9);
-  if (hipMalloc(&c->scalars, 4096) != hipSuccess) {
+  if (hipMalloc(&c->scalars, 4096) != hipSuccess ||
+      hipMalloc(&c->pimy_params, sizeof(fx::PimyParams) * fx::kMaxPointMaterials) != hipSuccess) {
+    if (c->scalars) (void)hipFree(c->scalars);
     (void)hipStreamDestroy(c->stream);
     delete c;
     return fail(nullptr, FX_EDEVICE, "hipMalloc failed");
@@ -86,6 +89,7 @@ void fx_ctx_destroy(fx_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->red) (void)hipFree(ctx->red);
   if (ctx->scalars) (void)hipFree(ctx->scalars);
+  if (ctx->pimy_params) (void)hipFree(ctx->pimy_params);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -189,7 +193,8 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
-                  b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body};
+                  b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -613,4 +618,168 @@ int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, doub
     if (d->load_eq[j] < 0 || d->load_eq[j] >= b->neq) return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: load equation out of range");
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
   return fx::cd_run(b, *d, u, um, v, a);
+}
+
+int fx_pimy_surfaces(const fx_pimy_material* m, double* size, double* modulus) {
+  if (!m || !size || !modulus) return fail(nullptr, FX_EINVAL, "fx_pimy_surfaces: null argument");
+  fx::PimyParams p;
+  std::string err;
+  if (int rc = fx::pimy_setup(*m, p, err)) return fail(nullptr, rc, err);
+  for (int i = 0; i <= fx::kPimyNS; ++i) {
+    size[i] = p.size[i];
+    modulus[i] = p.modulus[i];
+  }
+  return FX_OK;
+}
+
+namespace {
+int upload_point_materials(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, const int32_t* load_stage) {
+  if (nmat <= 0 || nmat > fx::kMaxPointMaterials || !mats)
+    return fail(ctx, FX_EINVAL, "fx_pimy: 1 to 8 materials");
+  fx::PimyParams p[fx::kMaxPointMaterials];
+  for (int32_t i = 0; i < nmat; ++i) {
+    std::string err;
+    if (int rc = fx::pimy_setup(mats[i], p[i], err)) return fail(ctx, rc, err);
+    p[i].load_stage = load_stage ? load_stage[i] : 0;
+  }
+  FX_HIP(ctx, hipMemcpyAsync(ctx->pimy_params, p, sizeof(fx::PimyParams) * nmat, hipMemcpyHostToDevice, ctx->stream));
+  return FX_OK;
+}
+}  // namespace
+
+int fx_pimy_init_state(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, int64_t npts, const int32_t* mat,
+                       double* state) {
+  if (!ctx || (npts > 0 && (!mat || !state))) return fail(ctx, FX_EINVAL, "fx_pimy_init_state: null argument");
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = upload_point_materials(ctx, nmat, mats, nullptr)) return rc;
+  if (int rc = fx::pimy_init(ctx, npts, static_cast<const fx::PimyParams*>(ctx->pimy_params), mat, state)) return rc;
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+int fx_pimy_points(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, const int32_t* load_stage, int64_t npts,
+                   const int32_t* mat, const double* committed, double* trial, const double* eps, double* sig,
+                   double* D) {
+  if (!ctx || (npts > 0 && (!mat || !committed || !trial || !eps || !sig)))
+    return fail(ctx, FX_EINVAL, "fx_pimy_points: null argument");
+  if (committed == trial) return fail(ctx, FX_EINVAL, "fx_pimy_points: committed and trial history alias");
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  if (int rc = upload_point_materials(ctx, nmat, mats, load_stage)) return rc;
+  if (int rc = fx::pimy_points(ctx, npts, static_cast<const fx::PimyParams*>(ctx->pimy_params), mat, committed, trial,
+                               eps, sig, D))
+    return rc;
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flags, fx_batch** out) {
+  if (!ctx || !d || !out || !d->params || d->nparam <= 0) return fail(ctx, FX_EINVAL, "fx_pimy_batch_create: null argument");
+  *out = nullptr;
+  std::vector<fx::PimyParams> pp(d->nparam);
+  std::vector<fx_elastic_params> ep(d->nparam);
+  for (int32_t i = 0; i < d->nparam; ++i) {
+    std::string err;
+    if (int rc = fx::pimy_setup(d->params[i], pp[i], err)) return fail(ctx, rc, err + " (set " + std::to_string(i) + ")");
+    // the elastic moduli of the reference G, K carry the lumped mass, body load and the
+    // general-path geometry the batch shares with elastic batches
+    const double G = d->params[i].G, K = d->params[i].K;
+    fx_elastic_params& e = ep[i];
+    std::memset(&e, 0, sizeof e);
+    e.E = 9.0 * K * G / (3.0 * K + G);
+    e.nu = (3.0 * K - 2.0 * G) / (2.0 * (3.0 * K + G));
+    e.rho = pp[i].rho;
+    for (int k = 0; k < 3; ++k) e.b[k] = d->params[i].b[k];
+    e.mass_type = d->params[i].mass_type;
+  }
+  fx_batch_desc ed{};
+  ed.count = d->count;
+  ed.padded = d->padded;
+  ed.node = d->node;
+  ed.eq = d->eq;
+  ed.param = d->param;
+  ed.nparam = d->nparam;
+  ed.params = ep.data();
+  ed.nnode = d->nnode;
+  ed.coords = d->coords;
+  ed.neq = d->neq;
+  fx_batch* b = nullptr;
+  if (int rc = fx_batch_create(ctx, &ed, (flags & ~FX_BATCH_FP32) | FX_BATCH_GENERAL | FX_BATCH_NO_TUNE, &b)) return rc;
+  b->pimy = true;
+  b->host_pimy = pp;
+  b->npts = static_cast<int64_t>(d->count) * 8;
+  // the gather: each equation's slots k * padded + e in (element, DOF) order
+  const int32_t P = d->padded, n = d->neq;
+  std::vector<int32_t> start(static_cast<size_t>(n) + 1, 0), list;
+  for (int32_t e = 0; e < d->count; ++e)
+    for (int k = 0; k < 24; ++k) {
+      const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
+      if (q >= 0) ++start[q + 1];
+    }
+  for (int32_t q = 0; q < n; ++q) start[q + 1] += start[q];
+  list.resize(start[n]);
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int32_t e = 0; e < d->count; ++e)
+    for (int k = 0; k < 24; ++k) {
+      const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
+      if (q >= 0) list[fill[q]++] = k * P + e;
+    }
+  uint64_t& by = b->device_bytes;
+  int rc = FX_OK;
+  auto ok = [&](int r) {
+    if (r != FX_OK && rc == FX_OK) rc = r;
+    return rc == FX_OK;
+  };
+  const size_t np = static_cast<size_t>(std::max<int64_t>(b->npts, 1));
+  ok(dev_upload(ctx, b->pimy_params, pp.data(), pp.size(), by)) &&
+      ok(dev_zeros(ctx, b->state_c, np * fx::kPimyNState, by)) && ok(dev_zeros(ctx, b->state_t, np * fx::kPimyNState, by)) &&
+      ok(dev_zeros(ctx, b->tangent, np * 21, by)) &&
+      ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
+      ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&
+      ok(list.empty() ? FX_OK : dev_upload(ctx, b->gather_list, list.data(), list.size(), by)) &&
+      ok(fx::pimy_batch_init(b));
+  if (rc == FX_OK) {
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, FX_EDEVICE, "fx_pimy_batch_create: init failed");
+  }
+  if (rc != FX_OK) {
+    fx_batch_destroy(b);
+    return rc;
+  }
+  b->info.device_bytes = by;
+  *out = b;
+  return FX_OK;
+}
+
+int fx_set_material_stage(fx_batch* b, int32_t stage) {
+  if (!b || (stage != 0 && stage != 1)) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_set_material_stage: bad argument");
+  if (!b->pimy) return FX_OK;  // elastic materials have one stage
+  for (auto& p : b->host_pimy) p.load_stage = stage;
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  FX_HIP(b->ctx, hipMemcpyAsync(b->pimy_params, b->host_pimy.data(), sizeof(fx::PimyParams) * b->host_pimy.size(),
+                                hipMemcpyHostToDevice, b->ctx->stream));
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_store_tangents(fx_batch* b, const double* u, double* f) {
+  if (!b || !u) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_store_tangents: null argument");
+  if (!b->pimy) return FX_OK;  // elastic tangents are constant
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  if (int rc = fx::pimy_force64(b, u, f ? f : b->work, true)) return rc;
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
+int fx_commit_state(fx_batch* b) {
+  if (!b) return fail(nullptr, FX_EINVAL, "fx_commit_state: null batch");
+  if (b->pimy) std::swap(b->state_c, b->state_t);
+  return FX_OK;
+}
+
+int fx_pimy_state(fx_batch* b, int committed, double* dst) {
+  if (!b || !dst || !b->pimy) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_pimy_state: not a PIMY batch");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  FX_HIP(b->ctx, hipMemcpyAsync(dst, committed ? b->state_c : b->state_t, sizeof(double) * fx::kPimyNState * b->npts,
+                                hipMemcpyDeviceToDevice, b->ctx->stream));
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
 }

@@ -105,6 +105,7 @@ bool elements_damp(const fx_batch* b) {
 }  // namespace
 
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch) {
+  if (b->pimy) return pimy_force64(b, u, f, false);  // the return map; no element damping
   const int32_t n = b->neq;
   if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, u, f)) return rc;
   const double* cv = nullptr;
@@ -144,7 +145,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   double* f = W + 4 * N;
   double* ve = W + 5 * N;
   double* fd = b->internal ? W + 7 * N : W + 6 * N;
-  const bool damp = elements_damp(b);
+  const bool damp = !b->pimy && elements_damp(b);
   const double c2 = 0.5 / d.dt, c3 = 1.0 / (d.dt * d.dt);
   PointLoads pl{};
   pl.n = d.nload;
@@ -156,7 +157,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   g.dt = d.dt;
   g.f = f;
   g.fd = damp ? fd : nullptr;
-  g.body = b->body;
+  g.body = b->pimy ? nullptr : b->body;  // the return-map force carries the body force
   g.m = b->lumped_m;
   g.am = b->lumped_am;
   g.dash = dash;
@@ -165,7 +166,11 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
       hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, n, -c2, Um, ve);
       if (int rc = operator_apply64(b, fx_coeffs{0.0, 1.0, 0.0, 0}, ve, fd)) return rc;
     }
-    if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, U, f)) return rc;
+    if (b->pimy) {
+      if (int rc = pimy_force64(b, U, f, false)) return rc;
+    } else if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, U, f)) {
+      return rc;
+    }
     if (pl.n > 0) {
       for (int j = 0; j < pl.n; ++j) pl.val[j] = d.load_hist[static_cast<size_t>(k) * pl.n + j];
       hipLaunchKernelGGL(point_load_kernel, dim3(1), dim3(64), 0, ctx->stream, pl, f);
@@ -176,6 +181,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
     g.a = Acc;
     hipLaunchKernelGGL(cd_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
     std::swap(U, Um);  // U(n+1) sits in the old U(n-1) buffer
+    if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state: the trial history becomes committed
   }
   FX_HIP(ctx, hipGetLastError());
   // back to the caller's buffers: u = U(n+1), um = U(n)

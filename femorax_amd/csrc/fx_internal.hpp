@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "../../include/femorax_amd.h"
+#include "fx_pimy.hpp"
 
 namespace fx {
 
@@ -99,6 +100,7 @@ struct fx_ctx {
   size_t red_cap = 0;
   void* scalars = nullptr;  // solver state block (device)
   void* host_pinned = nullptr;
+  void* pimy_params = nullptr;  // material-point driver: up to kMaxPointMaterials parameter sets
 };
 
 namespace fx {
@@ -124,6 +126,13 @@ struct DevParam {  // per parameter set, device
 
 namespace fx {
 constexpr int kMaxPointLoads = 32;
+constexpr int kMaxPointMaterials = 8;
+struct PimyParams;
+// PressureIndependMultiYield (fx_pimy_host.cpp, fx_pimy.hip)
+int pimy_setup(const fx_pimy_material& m, PimyParams& p, std::string& err);
+int pimy_points(fx_ctx* ctx, int64_t npts, const PimyParams* params, const int32_t* param, const double* C, double* T,
+                const double* eps, double* sig, double* D);
+int pimy_init(fx_ctx* ctx, int64_t npts, const PimyParams* params, const int32_t* param, double* S);
 // Host stdBrick helpers (fx_brick_host.cpp).
 void brick_node_weights(const double (&xe)[8][3], double (&w)[8]);
 void brick_stiffness(const double (&xe)[8][3], double lam, double mu, double (&K)[24][24]);
@@ -178,6 +187,18 @@ struct fx_batch {
   // host copies for the stable time step of general batches (lattice batches need none)
   std::vector<int32_t> host_node, host_param;
   std::vector<double> host_coords;
+  // stdBrick<PressureIndependMultiYield> batches: history, stored tangents, slots + gather
+  bool pimy = false;
+  int64_t npts = 0;                   // count * 8 Gauss points
+  fx::PimyParams* pimy_params = nullptr;
+  std::vector<fx::PimyParams> host_pimy;
+  double* state_c = nullptr;          // committed history [kPimyNState * npts]
+  double* state_t = nullptr;          // trial history
+  double* tangent = nullptr;          // [21 * npts] upper triangles
+  bool tangents_stored = false;
+  double* slots = nullptr;            // [24 * padded]
+  int32_t* gather_start = nullptr;    // [neq + 1]
+  int32_t* gather_list = nullptr;     // slots in (element, DOF) order
   // solver work (allocated at create: compute allocates nothing)
   double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats

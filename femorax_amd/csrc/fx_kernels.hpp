@@ -63,6 +63,12 @@ int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const doubl
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
              double* x, fx_solve_result& r);
 
+// stdBrick<PressureIndependMultiYield> batches (fx_pimy.hip).
+int pimy_batch_init(fx_batch* b);
+int pimy_force64(fx_batch* b, const double* u, double* f, bool store_tangents);
+int pimy_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
+int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
+
 // Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch);
 int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a);

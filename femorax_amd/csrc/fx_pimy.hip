@@ -1,0 +1,899 @@
+// PressureIndependMultiYield device return map (the algorithm and its sources are listed in
+// fx_pimy.hpp), the material-point driver, and the stdBrick<PressureIndependMultiYield> batch
+// kernels: internal force with the return map at every Gauss point, the operator and its
+// diagonal on the stored tangents.
+//
+// The batch kernels run one thread per Gauss point (8 consecutive lanes per element), so the
+// return maps of an element run side by side and the SoA history is read coalesced. Each point
+// leaves its node contributions in LDS; the element's first lane adds them in Gauss-point
+// order and writes the element's 24 values to its slots; a gather adds the slots of each
+// equation in (element, DOF) order (assembly.cpp:64-95). Deterministic.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "fx_brick_device.hpp"
+#include "fx_internal.hpp"
+#include "fx_kernels.hpp"
+#include "fx_pimy.hpp"
+
+namespace fx {
+
+namespace pimy {
+
+constexpr double kLow = 20.0 * DBL_EPSILON;
+constexpr double kBig = 1.0e200;
+
+struct V6 {
+  double v[6];
+};
+
+__device__ inline V6 zero6() { return V6{{0, 0, 0, 0, 0, 0}}; }
+// tensor double contraction in vector form (shear terms count twice)
+__device__ inline double dd(const V6& a, const V6& b) {
+  double r = 0.;
+  for (int i = 0; i < 3; ++i) r += a.v[i] * b.v[i] + 2 * a.v[i + 3] * b.v[i + 3];
+  return r;
+}
+// a = s * a + f * o with OpenSees Vector::addVector's special cases (t2_vector.hpp:40-75)
+__device__ inline void axpby(V6& a, double s, const V6& o, double f) {
+  if (f == 0.0) {
+    if (s == 1.0) return;
+    if (s == 0.0) {
+      a = zero6();
+      return;
+    }
+    for (int i = 0; i < 6; ++i) a.v[i] *= s;
+    return;
+  }
+  for (int i = 0; i < 6; ++i) {
+    if (s == 1.0) {
+      if (f == 1.0) a.v[i] += o.v[i];
+      else if (f == -1.0) a.v[i] -= o.v[i];
+      else a.v[i] += o.v[i] * f;
+    } else if (s == 0.0) {
+      if (f == 1.0) a.v[i] = o.v[i];
+      else if (f == -1.0) a.v[i] = -(o.v[i]);
+      else a.v[i] = o.v[i] * f;
+    } else {
+      if (f == 1.0) a.v[i] = a.v[i] * s + o.v[i];
+      else if (f == -1.0) a.v[i] = a.v[i] * s - o.v[i];
+      else a.v[i] = a.v[i] * s + o.v[i] * f;
+    }
+  }
+}
+// a /= f, with the 1e200 fill of a division by zero (t2_vector.hpp:86-92)
+__device__ inline void divide(V6& a, double f) {
+  for (int i = 0; i < 6; ++i) a.v[i] = f == 0.0 ? kBig : a.v[i] / f;
+}
+
+// a symmetric tensor with its deviator and mean (soil::T2)
+struct T2 {
+  V6 t2, dev;
+  double vol;
+  __device__ void set(const V6& x, bool engineering) {
+    t2 = x;
+    vol = (t2.v[0] + t2.v[1] + t2.v[2]) / 3.0;
+    for (int i = 0; i < 3; ++i) {
+      dev.v[i] = t2.v[i] - vol;
+      dev.v[i + 3] = t2.v[i + 3];
+      if (engineering) {
+        dev.v[i + 3] /= 2.;
+        t2.v[i + 3] /= 2.;
+      }
+    }
+  }
+  __device__ void set_dev(const V6& d, double volume) {
+    vol = volume;
+    const double dv = (d.v[0] + d.v[1] + d.v[2]) / 3.;
+    for (int i = 0; i < 3; ++i) {
+      dev.v[i] = d.v[i] - dv;
+      dev.v[i + 3] = d.v[i + 3];
+      t2.v[i] = dev.v[i] + vol;
+      t2.v[i + 3] = dev.v[i + 3];
+    }
+  }
+  __device__ V6 engineering() const {
+    V6 o = t2;
+    for (int i = 0; i < 3; ++i) o.v[i + 3] *= 2.;
+    return o;
+  }
+  __device__ double dev_len() const { return sqrt(dd(dev, dev)); }
+  __device__ bool is_zero() const {
+    for (int i = 0; i < 6; ++i)
+      if (t2.v[i] != 0.0) return false;
+    return true;
+  }
+};
+
+// the smaller non-negative root of A x^2 + B x + C (MultiYieldSurface secondOrderEqn)
+__device__ inline double quad_root(double A, double B, double C, bool& ok) {
+  if (A == 0) {
+    ok = false;
+    return NAN;
+  }
+  if (C == 0) return 0;
+  if (B == 0) {
+    if (C / A > 0) {
+      ok = false;
+      return NAN;
+    }
+    return sqrt(-C / A);
+  }
+  const double det = B * B - 4. * A * C;
+  if (det < 0) {
+    ok = false;
+    return NAN;
+  }
+  double r1 = B > 0 ? (-B - sqrt(det)) / (2. * A) : (-B + sqrt(det)) / (2. * A);
+  double r2 = C / (A * r1);
+  if (r1 < 0 && r2 < 0) {
+    if (fabs(r1) < kLow) r1 = 0.;
+    else if (fabs(r2) < kLow) r2 = 0.;
+  }
+  if (r1 < 0 && r2 < 0) {
+    ok = false;
+    return NAN;
+  }
+  if (r1 < 0) return r2;
+  if (r2 < 0) return r1;
+  return r1 > r2 ? r2 : r1;
+}
+
+__device__ inline int to_int(double x) {
+  if (!(x == x)) return 0;
+  if (x >= 2147483647.0) return 2147483647;
+  if (x <= -2147483648.0) return -2147483647 - 1;
+  return static_cast<int>(x);
+}
+
+// One material point: committed history C, trial history T (strided views into the SoA state).
+struct Point {
+  const PimyParams& p;
+  const double* C;
+  double* T;
+  int64_t s;  // stride between fields
+  bool ok = true;
+  T2 cur_stress, cur_strain, trial, rate, sub_rate;
+  int committed_active = 0, active = 0, e2p = 0;
+  double G = 0, K = 0;
+
+  __device__ Point(const PimyParams& pp, const double* c, double* t, int64_t stride) : p(pp), C(c), T(t), s(stride) {}
+
+  __device__ double c(int k) const { return C[k * s]; }
+  __device__ double& t(int k) const { return T[k * s]; }
+  __device__ double size(int i) const { return i == 0 ? 0. : t(kPSizes + i - 1); }
+  __device__ double modulus(int i) const { return i == 0 ? 0. : t(kPModuli + i - 1); }
+  __device__ V6 center(int i) const {
+    V6 o = zero6();
+    if (i == 0) return o;
+    for (int k = 0; k < 6; ++k) o.v[k] = t(kPCenters + 6 * (i - 1) + k);
+    return o;
+  }
+  __device__ void set_center(int i, const V6& o) const {
+    for (int k = 0; k < 6; ++k) t(kPCenters + 6 * (i - 1) + k) = o.v[k];
+  }
+  __device__ T2 load_t2(int k) const {
+    T2 x;
+    for (int i = 0; i < 6; ++i) x.t2.v[i] = c(k + i);
+    for (int i = 0; i < 6; ++i) x.dev.v[i] = c(k + 6 + i);
+    x.vol = c(k + 12);
+    return x;
+  }
+  __device__ void store_t2(const T2& x, int k) const {
+    for (int i = 0; i < 6; ++i) t(k + i) = x.t2.v[i];
+    for (int i = 0; i < 6; ++i) t(k + 6 + i) = x.dev.v[i];
+    t(k + 12) = x.vol;
+  }
+
+  __device__ double yield(const T2& x, int i) const {
+    V6 d = x.dev;
+    const V6 cc = center(i);
+    for (int k = 0; k < 6; ++k) d.v[k] -= cc.v[k];
+    const double sz = size(i);
+    return 3. / 2. * dd(d, d) - sz * sz;
+  }
+
+  __device__ void deviator_scaling(T2& x, int i) {
+    const int n = p.nsurf;
+    for (int iter = 0; iter < 100000; ++iter) {
+      const double diff = yield(x, i);
+      if (i < n && diff < 0.) {
+        const double sz = size(i);
+        const double dsz = sqrt(sz * sz + diff);
+        V6 devia = x.dev, tmp;
+        const V6 cc = center(i);
+        for (int k = 0; k < 6; ++k) tmp.v[k] = devia.v[k] - cc.v[k];
+        double coeff = (sz - dsz) / dsz;
+        if (coeff < 1.e-13) coeff = 1.e-13;
+        axpby(devia, 1.0, tmp, coeff);
+        x.set_dev(devia, x.vol);
+        continue;
+      }
+      if (i == n && fabs(diff) > kLow) {
+        const double sz = size(i);
+        V6 nd = zero6();
+        axpby(nd, 0.0, x.dev, sz / sqrt(diff + sz * sz));
+        x.set_dev(nd, x.vol);
+      }
+      return;
+    }
+    ok = false;
+  }
+
+  __device__ void init_surface_update() {
+    if (committed_active == 0) return;
+    const int n = p.nsurf;
+    const V6 devia = cur_stress.dev;
+    const double Ms = sqrt(3. / 2. * dd(devia, devia));
+    if (committed_active < n) {
+      V6 nc = zero6();
+      axpby(nc, 0.0, devia, 1.0 - size(committed_active) / Ms);
+      set_center(committed_active, nc);
+    }
+    for (int i = 1; i < committed_active; ++i) {
+      V6 nc;
+      const double f = 1. - size(i) / Ms;
+      for (int k = 0; k < 6; ++k) nc.v[k] = devia.v[k] * f;
+      set_center(i, nc);
+    }
+  }
+
+  __device__ void param_scaling() {
+    if (p.phi == 0.) return;
+    const double con = -(cur_stress.vol - p.residual_press);
+    double scale = -con / (p.pref - p.residual_press);
+    scale = pow(scale, p.pdc);
+    G *= scale;
+    K *= scale;
+    for (int i = 1; i <= p.nsurf; ++i) {
+      const double pm = modulus(i) * scale;
+      const double sz = size(i) * con;
+      t(kPSizes + i - 1) = sz;
+      t(kPModuli + i - 1) = pm;
+      set_center(i, zero6());
+    }
+  }
+
+  __device__ void elastic_to_plastic() {
+    if (p.load_stage != 1 || e2p == 1) return;
+    e2p = 1;
+    if (cur_stress.vol > 0. && p.phi > 0.) {
+      const V6 d = cur_stress.dev;
+      cur_stress.set(d, false);
+    }
+    param_scaling();
+    if (cur_stress.dev_len() == 0.) return;
+    while (yield(cur_stress, ++committed_active) > 0) {
+      if (committed_active == p.nsurf) {
+        deviator_scaling(cur_stress, p.nsurf);
+        init_surface_update();
+        return;
+      }
+    }
+    committed_active--;
+    init_surface_update();
+  }
+
+  __device__ void set_trial_stress(const T2& x) {
+    V6 devia = x.dev;
+    axpby(devia, 1.0, sub_rate.dev, 2. * G);
+    trial.set_dev(devia, x.vol);
+  }
+
+  __device__ int set_sub_strain_rate() {
+    const int n = p.nsurf;
+    if (rate.is_zero()) return 0;
+    double epm;
+    if (active == 0) {
+      epm = 2 * G;
+    } else {
+      const double pm = modulus(active);
+      epm = 2 * G * pm / (2 * G + pm);
+    }
+    V6 inc = zero6();
+    axpby(inc, 0.0, rate.dev, epm);
+    T2 x;
+    x.set(inc, false);
+    const double single = size(n) / n;
+    const double total = 3. * (sqrt(1. / 3.) * x.dev_len()) / sqrt(2.);
+    int m = to_int(total / single + 1);
+    if (m > n) m = n;
+    inc = rate.t2;
+    divide(inc, m);
+    sub_rate.set(inc, false);
+    return m;
+  }
+
+  __device__ V6 normal(const T2& x) const {
+    V6 nrm = x.dev;
+    const V6 cc = center(active);
+    for (int k = 0; k < 6; ++k) nrm.v[k] -= cc.v[k];
+    divide(nrm, sqrt(dd(nrm, nrm)));
+    return nrm;
+  }
+
+  __device__ bool load_reversal() const {
+    if (active == 0) return false;
+    const V6 nrm = normal(cur_stress);
+    V6 a = trial.dev;
+    for (int k = 0; k < 6; ++k) a.v[k] -= cur_stress.dev.v[k];
+    return dd(a, nrm) < 0;
+  }
+
+  __device__ T2 contact_stress() const {
+    const V6 cc = center(active);
+    V6 devia = trial.dev;
+    for (int k = 0; k < 6; ++k) devia.v[k] -= cc.v[k];
+    const double Ms = sqrt(3. / 2. * dd(devia, devia));
+    const double f = size(active) / Ms;
+    for (int k = 0; k < 6; ++k) devia.v[k] *= f;
+    for (int k = 0; k < 6; ++k) devia.v[k] += cc.v[k];
+    T2 x;
+    x.set_dev(devia, trial.vol);
+    return x;
+  }
+
+  __device__ double loading(const T2& cs, const V6& nrm, bool crossed) const {
+    const double t1 = 2. * G, t2 = modulus(active);
+    const double tt = t1 + t2;
+    V6 tmp = trial.dev;
+    for (int k = 0; k < 6; ++k) tmp.v[k] -= cs.dev.v[k];
+    double lf = dd(nrm, tmp) / tt;
+    if (crossed) {
+      const double t3 = modulus(active - 1);
+      lf *= (t3 - t2) / t3;
+    }
+    return lf;
+  }
+
+  __device__ bool crossing_next() const {
+    if (active == p.nsurf) return false;
+    return yield(trial, active + 1) > 0;
+  }
+
+  __device__ void stress_correction(bool crossed) {
+    for (;;) {
+      const T2 cs = contact_stress();
+      const V6 nrm = normal(cs);
+      const double lf = loading(cs, nrm, crossed);
+      V6 devia = zero6();
+      axpby(devia, 0.0, nrm, -2 * G * lf);
+      for (int k = 0; k < 6; ++k) devia.v[k] += trial.dev.v[k];
+      trial.set_dev(devia, trial.vol);
+      deviator_scaling(trial, active);
+      if (crossing_next()) {
+        active++;
+        crossed = true;
+        continue;
+      }
+      return;
+    }
+  }
+
+  __device__ void update_active_surface() {
+    const int n = p.nsurf;
+    if (active == n) return;
+    V6 cen = center(active);
+    const double sz = size(active);
+    const V6 outc = center(active + 1);
+    const double outsz = size(active + 1);
+    V6 t1, t2, tmp;
+    for (int k = 0; k < 6; ++k) {
+      t1.v[k] = trial.dev.v[k] - cen.v[k];
+      t2.v[k] = cen.v[k] - outc.v[k];
+    }
+    double A = dd(t1, t1);
+    double B = 2. * dd(t1, t2);
+    double Cc = dd(t2, t2) - 2. / 3. * outsz * outsz;
+    double X = quad_root(A, B, Cc, ok);
+    if (fabs(X - 1.) < kLow) X = 1.;
+    if (X < 1.) {
+      ok = false;
+      return;
+    }
+    tmp = cen;
+    axpby(tmp, 1.0, t1, X);
+    const double f = 1.0 - sz / outsz;
+    for (int k = 0; k < 6; ++k) tmp.v[k] *= f;
+    t2 = cen;
+    axpby(t2, 1.0, outc, -sz / outsz);
+    for (int k = 0; k < 6; ++k) tmp.v[k] -= t2.v[k];
+    T2 dir;
+    dir.set(tmp, false);
+    if (dir.dev_len() < kLow) return;
+    tmp = dir.dev;
+    A = dd(tmp, tmp);
+    B = -2 * dd(t1, tmp);
+    if (fabs(B) < kLow) B = 0.;
+    Cc = dd(t1, t1) - 2. / 3. * sz * sz;
+    if (fabs(Cc) < kLow || fabs(Cc) / dd(t1, t1) < kLow) return;
+    if ((fabs(B) < 1.0e-10) && (fabs(Cc) < 1.0e-10)) return;
+    if (B > 0. || Cc < 0.) {
+      ok = false;
+      return;
+    }
+    X = quad_root(A, B, Cc, ok);
+    axpby(cen, 1.0, tmp, X);
+    set_center(active, cen);
+  }
+
+  __device__ void update_inner_surface() {
+    if (active <= 1) return;
+    const V6 devia = cur_stress.dev;
+    const V6 cen = center(active);
+    const double sz = size(active);
+    for (int i = 1; i < active; ++i) {
+      V6 nc;
+      const double f = size(i) / sz;
+      for (int k = 0; k < 6; ++k) nc.v[k] = (cen.v[k] - devia.v[k]) * f + devia.v[k];
+      set_center(i, nc);
+    }
+  }
+
+  // the 6 x 6 tangent, row-major
+  __device__ void tangent(double (&D)[36]) const {
+    double coeff = 0.;
+    V6 devia = zero6();
+    if (p.load_stage == 1 && active > 0) {
+      const V6 cc = center(active);
+      devia = trial.dev;
+      for (int k = 0; k < 6; ++k) devia.v[k] -= cc.v[k];
+      const double sz = size(active), pm = modulus(active);
+      coeff = 6. * G * G / (2. * G + pm) / sz / sz;
+    }
+    for (int i = 0; i < 6; ++i)
+      for (int j = 0; j < 6; ++j) {
+        double v = p.load_stage == 1 ? -coeff * devia.v[i] * devia.v[j] : 0.0;
+        if (i == j) v += G;
+        if (i < 3 && j < 3 && i == j) v += G;
+        if (i < 3 && j < 3) v += (K - 2. * G / 3.);
+        D[6 * i + j] = v;
+      }
+  }
+
+  // the update at strain eps (engineering shear): stress, tangent, trial history written
+  __device__ void update(const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
+    cur_stress = load_t2(kPStress);
+    cur_strain = load_t2(kPStrain);
+    committed_active = static_cast<int>(c(kPActive));
+    active = committed_active;
+    e2p = static_cast<int>(c(kPE2P));
+    G = c(kPG);
+    K = c(kPK);
+    {
+      V6 d, cs = cur_strain.engineering();
+      for (int k = 0; k < 6; ++k) d.v[k] = eps[k] - cs.v[k];
+      rate.set(d, true);
+    }
+    for (int k = kPSizes; k < kPimyNState; ++k) t(k) = c(k);  // committed surfaces into the trial buffer
+    if (p.load_stage == 1 && e2p == 0) elastic_to_plastic();
+    if (p.load_stage != 1) {
+      double De[36];
+      tangent(De);
+      V6 a = cur_stress.t2;
+      const V6 r = rate.engineering();
+      for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) a.v[j] += De[6 * j + i] * r.v[i];
+      trial.set(a, false);
+    } else {
+      active = committed_active;
+      sub_rate = rate;
+      set_trial_stress(cur_stress);
+      if (load_reversal()) {
+        update_inner_surface();
+        active = 0;
+      }
+      const int nsub = set_sub_strain_rate();
+      for (int i = 0; i < nsub; ++i) {
+        if (i == 0) set_trial_stress(cur_stress);
+        else set_trial_stress(trial);
+        if (active == 0 && !crossing_next()) continue;
+        if (active == 0) active++;
+        stress_correction(false);
+        update_active_surface();
+      }
+      double vol = K * (rate.vol * 3.);
+      vol += cur_stress.vol;
+      const V6 d = trial.dev;
+      trial.set_dev(d, vol);
+    }
+    tangent(D);
+    for (int i = 0; i < 6; ++i) sig[i] = ok ? trial.t2.v[i] : NAN;
+    V6 sc = cur_strain.t2;
+    for (int k = 0; k < 6; ++k) sc.v[k] += rate.t2.v[k];
+    T2 strain_commit;
+    strain_commit.set(sc, false);
+    store_t2(trial, kPStress);
+    store_t2(strain_commit, kPStrain);
+    t(kPE2P) = e2p;
+    t(kPG) = G;
+    t(kPK) = K;
+    t(kPActive) = p.load_stage == 1 ? active : committed_active;
+  }
+};
+
+}  // namespace pimy
+
+__device__ void pimy_update(const PimyParams& p, const double* C, double* T, int64_t stride, const double (&eps)[6],
+                            double (&sig)[6], double (&D)[36]) {
+  pimy::Point pt(p, C, T, stride);
+  pt.update(eps, sig, D);
+}
+
+// Material-point driver: point i uses parameter set param[i], committed history C, trial
+// history T (SoA, stride npts), strain eps[k * npts + i]; writes sig[k * npts + i] and the
+// row-major tangent D[k * npts + i] (k < 36).
+__global__ __launch_bounds__(128) void pimy_point_kernel(int64_t npts, const PimyParams* __restrict__ params,
+                                                         const int32_t* __restrict__ param, const double* C, double* T,
+                                                         const double* __restrict__ eps, double* __restrict__ sig,
+                                                         double* __restrict__ D) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  double e[6], s[6], d[36];
+  for (int k = 0; k < 6; ++k) e[k] = eps[k * npts + i];
+  pimy_update(params[param[i]], C + i, T + i, npts, e, s, d);
+  for (int k = 0; k < 6; ++k) sig[k * npts + i] = s[k];
+  if (D)
+    for (int k = 0; k < 36; ++k) D[k * npts + i] = d[k];
+}
+
+// State at rest: zeros, the elastic moduli and the backbone surfaces (initial_state :142-150).
+__global__ void pimy_init_kernel(int64_t npts, const PimyParams* __restrict__ params, const int32_t* __restrict__ param,
+                                 double* S) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const PimyParams& p = params[param[i]];
+  for (int k = 0; k < kPimyNState; ++k) S[k * npts + i] = 0.0;
+  S[kPG * npts + i] = p.G;
+  S[kPK * npts + i] = p.K;
+  for (int j = 1; j <= kPimyNS; ++j) {
+    S[(kPSizes + j - 1) * npts + i] = p.size[j];
+    S[(kPModuli + j - 1) * npts + i] = p.modulus[j];
+  }
+}
+
+// ---- stdBrick<PressureIndependMultiYield> batches ------------------------------------------------
+
+namespace {
+
+constexpr int kPointsPerBlock = 128;  // 16 elements
+constexpr int kElemsPerBlock = kPointsPerBlock / 8;
+
+struct PimyBatchArgs {
+  int32_t count, padded, nnode;
+  int64_t npts;
+  const int32_t* node;
+  const int32_t* param;
+  const int32_t* node_eq;
+  const double* coords;
+  const PimyParams* params;
+  const double* C;
+  double* T;
+  double* tangent;  // [21 * npts] upper triangle row-major, or null
+  const double* x;  // u (force) or v (operator)
+  double* slots;    // [24 * padded]
+  fx_coeffs k;
+};
+
+__device__ inline void element_geometry(const PimyBatchArgs& a, int e, double (&X)[8], double (&Y)[8], double (&Z)[8],
+                                        int (&eq)[24]) {
+  const size_t P = static_cast<size_t>(a.padded);
+  for (int k = 0; k < 8; ++k) {
+    const int n = a.node[k * P + e];
+    X[k] = a.coords[n];
+    Y[k] = a.coords[static_cast<size_t>(a.nnode) + n];
+    Z[k] = a.coords[2 * static_cast<size_t>(a.nnode) + n];
+    for (int d = 0; d < 3; ++d) eq[3 * k + d] = a.node_eq[3 * n + d];
+  }
+}
+
+// strain 11 22 33 12 23 31 with engineering shear (std_brick.hpp:276-301)
+__device__ inline void point_strain(const Shape<double>& s, const double (&v)[24], double (&eps)[6]) {
+  for (int c = 0; c < 6; ++c) eps[c] = 0.0;
+  for (int j = 0; j < 8; ++j) {
+    const double dx = s.dx[j], dy = s.dy[j], dz = s.dz[j];
+    const double u0 = v[3 * j], u1 = v[3 * j + 1], u2 = v[3 * j + 2];
+    eps[0] += dx * u0;
+    eps[1] += dy * u1;
+    eps[2] += dz * u2;
+    eps[3] += dy * u0 + dx * u1;
+    eps[4] += dz * u1 + dy * u2;
+    eps[5] += dz * u0 + dx * u2;
+  }
+}
+
+// B_j^T (sigma dvol) (std_brick.hpp:305-327)
+__device__ inline void node_force(const Shape<double>& s, const double (&sig)[6], int j, double (&out)[3]) {
+  double t[6];
+  for (int c = 0; c < 6; ++c) t[c] = sig[c] * s.dvol;
+  const double dx = s.dx[j], dy = s.dy[j], dz = s.dz[j];
+  out[0] = dx * t[0] + dy * t[3] + dz * t[5];
+  out[1] = dy * t[1] + dx * t[3] + dz * t[4];
+  out[2] = dz * t[2] + dy * t[4] + dx * t[5];
+}
+
+__device__ inline void initial_D(const PimyParams& p, double (&D)[36]) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double v = 0.;
+      if (i == j) v += p.G;
+      if (i < 3 && j < 3 && i == j) v += p.G;
+      if (i < 3 && j < 3) v += (p.K - 2. * p.G / 3.);
+      D[6 * i + j] = v;
+    }
+}
+
+// D_eff = cK D_t + c0 D_0 (Rayleigh off; element_operator.hpp:45-75)
+__device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, int64_t pt, double (&D)[36]) {
+  const double cK = a.k.initial ? 0.0 : a.k.stiffness, c0 = a.k.initial ? a.k.stiffness : 0.0;
+  double D0[36];
+  initial_D(p, D0);
+  int u = 0;
+  double Dt[36];
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j, ++u) Dt[6 * i + j] = Dt[6 * j + i] = a.tangent[u * a.npts + pt];
+  for (int i = 0; i < 36; ++i) {
+    double v = cK * Dt[i];
+    if (c0 != 0.0) v += c0 * D0[i];
+    D[i] = v;
+  }
+}
+
+// MODE 0: force (update at u; STORE also keeps the tangents). MODE 1: operator on the stored
+// tangents. MODE 2: its diagonal.
+template <int MODE, bool STORE>
+__global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchArgs a) {
+  __shared__ double part[kPointsPerBlock][48];  // per point: 8 nodes x (3 first terms, 3 second terms)
+  const int t = threadIdx.x;
+  const int e = blockIdx.x * kElemsPerBlock + t / 8, gp = t % 8;
+  const bool live = e < a.count;
+  if (live) {
+    double X[8], Y[8], Z[8];
+    int eq[24];
+    element_geometry(a, e, X, Y, Z, eq);
+    const PimyParams& p = a.params[a.param[e]];
+    Shape<double> s;
+    gauss_shape(X, Y, Z, gp, s);
+    const int64_t pt = static_cast<int64_t>(e) * 8 + gp;
+    double* mine = part[t];
+    if constexpr (MODE == 0) {
+      double u[24], eps[6], sig[6], D[36];
+      for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+      point_strain(s, u, eps);
+      pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+      if constexpr (STORE) {
+        int w = 0;
+        for (int i = 0; i < 6; ++i)
+          for (int j = i; j < 6; ++j, ++w) a.tangent[w * a.npts + pt] = D[6 * i + j];
+      }
+      for (int j = 0; j < 8; ++j) {
+        double rj[3];
+        node_force(s, sig, j, rj);
+        for (int q = 0; q < 3; ++q) {
+          mine[6 * j + q] = rj[q];
+          mine[6 * j + 3 + q] = s.dvol * p.b[q] * s.N[j];  // body
+        }
+      }
+    } else if constexpr (MODE == 1) {
+      double v[24], eps[6], sig[6], D[36];
+      for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+      effective_D(a, p, pt, D);
+      point_strain(s, v, eps);
+      for (int r = 0; r < 6; ++r) {
+        double sum = 0.0;
+        for (int c = 0; c < 6; ++c) sum += D[6 * r + c] * eps[c];
+        sig[r] = sum;
+      }
+      const double cM = a.k.mass;
+      double g[3] = {0, 0, 0};
+      if (cM != 0.0 && p.mass_type == 0)
+        for (int kk = 0; kk < 8; ++kk)
+          for (int q = 0; q < 3; ++q) g[q] += s.N[kk] * v[3 * kk + q];
+      for (int j = 0; j < 8; ++j) {
+        double fj[3];
+        node_force(s, sig, j, fj);
+        const double temp = s.N[j] * s.dvol * p.rho;
+        for (int q = 0; q < 3; ++q) {
+          mine[6 * j + q] = fj[q];
+          mine[6 * j + 3 + q] = cM != 0.0 ? cM * (temp * (p.mass_type == 0 ? g[q] : v[3 * j + q])) : 0.0;
+        }
+      }
+    } else {
+      double D[36];
+      effective_D(a, p, pt, D);
+      const double cM = a.k.mass;
+      double de[24];
+      for (int k = 0; k < 24; ++k) de[k] = 0.0;
+      // B_J^T (D dvol) B_K entry (r, c): with g the gradients, the isotropic-free general form
+      auto block = [&](int J, int K, int r, int c) {
+        double BJ[6][3] = {}, BK[6][3] = {};
+        const double gJ[3] = {s.dx[J], s.dy[J], s.dz[J]}, gK[3] = {s.dx[K], s.dy[K], s.dz[K]};
+        BJ[0][0] = gJ[0]; BJ[1][1] = gJ[1]; BJ[2][2] = gJ[2]; BJ[3][0] = gJ[1]; BJ[3][1] = gJ[0];
+        BJ[4][1] = gJ[2]; BJ[4][2] = gJ[1]; BJ[5][0] = gJ[2]; BJ[5][2] = gJ[0];
+        BK[0][0] = gK[0]; BK[1][1] = gK[1]; BK[2][2] = gK[2]; BK[3][0] = gK[1]; BK[3][1] = gK[0];
+        BK[4][1] = gK[2]; BK[4][2] = gK[1]; BK[5][0] = gK[2]; BK[5][2] = gK[0];
+        double out = 0.0;
+        for (int m = 0; m < 6; ++m) {
+          double bd = 0.0;
+          for (int l = 0; l < 6; ++l) bd += BJ[l][r] * (D[6 * l + m] * s.dvol);
+          out += bd * BK[m][c];
+        }
+        return out;
+      };
+      auto mass_entry = [&](int J, int K) {
+        const double temp = s.N[J] * s.dvol * p.rho;
+        if (p.mass_type == 0) return temp * s.N[K];
+        return J == K ? temp : 0.0;
+      };
+      int first[24];
+      bool merged = false;
+      for (int k = 0; k < 24; ++k) {
+        first[k] = k;
+        if (eq[k] < 0) continue;
+        for (int b = 0; b < k; ++b)
+          if (eq[b] == eq[k]) {
+            first[k] = first[b];
+            merged = true;
+            break;
+          }
+      }
+      for (int j = 0; j < 8; ++j) {
+        const double m = cM != 0.0 ? cM * mass_entry(j, j) : 0.0;
+        for (int q = 0; q < 3; ++q) de[3 * j + q] += block(j, j, q, q) + m;
+      }
+      if (merged)
+        for (int J = 0; J < 8; ++J)
+          for (int K = 0; K < 8; ++K)
+            for (int r = 0; r < 3; ++r)
+              for (int c = 0; c < 3; ++c) {
+                const int ia = 3 * J + r, ic = 3 * K + c;
+                if (ia == ic || eq[ia] < 0 || eq[ia] != eq[ic]) continue;
+                const double m = (cM != 0.0 && J != K && r == c) ? cM * mass_entry(J, K) : 0.0;
+                de[first[ia]] += block(J, K, r, c) + m;
+              }
+      for (int k = 0; k < 24; ++k) {
+        mine[2 * k] = de[k];
+        mine[2 * k + 1] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live || gp != 0) return;
+  double out[24];
+  for (int k = 0; k < 24; ++k) out[k] = 0.0;
+  for (int g = 0; g < 8; ++g) {
+    const double* c = part[t + g];
+    if constexpr (MODE == 0) {
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 3; ++q) {
+          out[3 * j + q] += c[6 * j + q];
+          out[3 * j + q] -= c[6 * j + 3 + q];
+        }
+    } else if constexpr (MODE == 1) {
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 3; ++q) {
+          out[3 * j + q] += c[6 * j + q];
+          if (a.k.mass != 0.0) out[3 * j + q] += c[6 * j + 3 + q];
+        }
+    } else {
+      for (int k = 0; k < 24; ++k) out[k] += c[2 * k];
+    }
+  }
+  const size_t P = static_cast<size_t>(a.padded);
+  for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
+}
+
+__global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
+                                   const double* __restrict__ slots, double* __restrict__ y, const int32_t* done) {
+  if (done && *done) return;
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= neq) return;
+  double s = 0.0;
+  for (int32_t j = start[q]; j < start[q + 1]; ++j) s += slots[list[j]];
+  y[q] = s;
+}
+
+__global__ void pimy_batch_init_kernel(int64_t npts, const PimyParams* __restrict__ params,
+                                       const int32_t* __restrict__ param, double* S) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= npts) return;
+  const PimyParams& p = params[param[i / 8]];
+  for (int k = 0; k < kPimyNState; ++k) S[k * npts + i] = 0.0;
+  S[kPG * npts + i] = p.G;
+  S[kPK * npts + i] = p.K;
+  for (int j = 1; j <= kPimyNS; ++j) {
+    S[(kPSizes + j - 1) * npts + i] = p.size[j];
+    S[(kPModuli + j - 1) * npts + i] = p.modulus[j];
+  }
+}
+
+PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k) {
+  PimyBatchArgs a{};
+  a.count = b->count;
+  a.padded = b->padded;
+  a.nnode = b->nnode;
+  a.npts = b->npts;
+  a.node = b->node;
+  a.param = b->param;
+  a.node_eq = b->node_eq;
+  a.coords = b->coords;
+  a.params = b->pimy_params;
+  a.C = b->state_c;
+  a.T = b->state_t;
+  a.tangent = b->tangent;
+  a.x = x;
+  a.slots = b->slots;
+  a.k = k;
+  return a;
+}
+
+template <int MODE, bool STORE>
+int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done) {
+  if (b->count > 0)
+    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE>), dim3((b->count + kElemsPerBlock - 1) / kElemsPerBlock),
+                       dim3(kPointsPerBlock), 0, b->ctx->stream, a);
+  if (b->neq > 0)
+    hipLaunchKernelGGL(slot_gather_kernel, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
+                       b->gather_start, b->gather_list, b->slots, y, done);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+}  // namespace
+
+int pimy_batch_init(fx_batch* b) {
+  for (double* S : {b->state_c, b->state_t})
+    if (b->npts > 0)
+      hipLaunchKernelGGL(pimy_batch_init_kernel, dim3(static_cast<unsigned>((b->npts + 255) / 256)), dim3(256), 0,
+                         b->ctx->stream, b->npts, b->pimy_params, b->param, S);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+int pimy_force64(fx_batch* b, const double* u, double* f, bool store) {
+  const PimyBatchArgs a = batch_args(b, u, fx_coeffs{1.0, 0.0, 0.0, 0});
+  if (store) {
+    if (int rc = run_pimy<0, true>(b, a, f, nullptr)) return rc;
+    b->tangents_stored = true;
+    return FX_OK;
+  }
+  return run_pimy<0, false>(b, a, f, nullptr);
+}
+
+int pimy_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
+  if (!b->tangents_stored && !k.initial && k.stiffness != 0.0) {
+    b->ctx->error = "operator: no stored tangents (call fx_store_tangents first)";
+    return FX_EINVAL;
+  }
+  return run_pimy<1, false>(b, batch_args(b, x, k), y, done);
+}
+
+int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
+  if (!b->tangents_stored && !k.initial && k.stiffness != 0.0) {
+    b->ctx->error = "operator diagonal: no stored tangents (call fx_store_tangents first)";
+    return FX_EINVAL;
+  }
+  return run_pimy<2, false>(b, batch_args(b, nullptr, k), d, nullptr);
+}
+
+int pimy_points(fx_ctx* ctx, int64_t npts, const PimyParams* params, const int32_t* param, const double* C, double* T,
+                const double* eps, double* sig, double* D) {
+  if (npts <= 0) return FX_OK;
+  hipLaunchKernelGGL(pimy_point_kernel, dim3(static_cast<unsigned>((npts + 127) / 128)), dim3(128), 0, ctx->stream,
+                     npts, params, param, C, T, eps, sig, D);
+  FX_HIP(ctx, hipGetLastError());
+  return FX_OK;
+}
+
+int pimy_init(fx_ctx* ctx, int64_t npts, const PimyParams* params, const int32_t* param, double* S) {
+  if (npts <= 0) return FX_OK;
+  hipLaunchKernelGGL(pimy_init_kernel, dim3(static_cast<unsigned>((npts + 255) / 256)), dim3(256), 0, ctx->stream, npts,
+                     params, param, S);
+  FX_HIP(ctx, hipGetLastError());
+  return FX_OK;
+}
+
+}  // namespace fx

@@ -230,6 +230,63 @@ typedef struct fx_cd_desc {
  * to u = U(n+nsteps), um = U(n+nsteps-1) and the recorded v, a of the last step. */
 int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, double* v, double* a);
 
+/* ---- PressureIndependMultiYield (J2 multi-surface soil) ------------------------------------ */
+/* PressureIndependMultiYield<NS = 20>::Params from the constructor's arguments
+ * (pressure_independ_multi_yield.hpp:56-61, 110-139) plus the element's body force and mass type. */
+typedef struct fx_pimy_material {
+  double rho, G, K, cohesion, gamma_peak, phi, pref, pdc; /* refShearModul, refBulkModul, cohesi,
+                                                             peakShearStra, frictionAng, refPress,
+                                                             pressDependCoe */
+  int32_t nsurf;          /* yield surfaces, <= 20 (0: 10) */
+  const double* gredu;    /* host [2 * nsurf] (gamma, G/Gmax) pairs, or NULL: automatic surfaces */
+  double b[3];            /* body force per unit volume */
+  int32_t mass_type;      /* 0 consistent, 1 lumped */
+} fx_pimy_material;
+#define FX_PIMY_NSTATE 190  /* doubles of history per material point (30 + 8 NS) */
+/* The backbone surfaces (setUpSurfaces, :275-381), host only: size[21], modulus[21] (index 0
+ * unused). FX_EINVAL with the reason for parameters the reference rejects. */
+int fx_pimy_surfaces(const fx_pimy_material* m, double* size, double* modulus);
+/* Material-point driver on device arrays, structure of arrays (field k of point i at
+ * [k * npts + i]). mat[npts] (device) picks one of nmat <= 8 materials; load_stage[nmat] (host)
+ * is each material's stage (updateMaterialStage: 0 elastic, 1 plastic).
+ *   fx_pimy_init_state: history at rest (initial_state, :142-150), state[FX_PIMY_NSTATE * npts].
+ *   fx_pimy_points: update at strain eps[6 * npts] (engineering shear, order 11 22 33 12 23 31)
+ *     from the committed history into the trial history (update :228-240); sig[6 * npts],
+ *     D[36 * npts] (row-major tangent, may be NULL). NaN stress marks a failed point. */
+int fx_pimy_init_state(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, int64_t npts, const int32_t* mat,
+                       double* state);
+int fx_pimy_points(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, const int32_t* load_stage, int64_t npts,
+                   const int32_t* mat, const double* committed, double* trial, const double* eps, double* sig,
+                   double* D);
+
+/* ---- stdBrick<PressureIndependMultiYield> batches --------------------------------------------- */
+typedef struct fx_pimy_batch_desc {
+  int32_t count, padded;
+  const int32_t* node;             /* [8 * padded] as fx_batch_desc */
+  const int32_t* eq;               /* [24 * padded] */
+  const int32_t* param;            /* [padded] */
+  int32_t nparam;
+  const fx_pimy_material* params;  /* [nparam] */
+  int32_t nnode;
+  const double* coords;            /* [3 * nnode] SoA */
+  int32_t neq;
+} fx_pimy_batch_desc;
+/* A batch whose material points (8 per element, point e * 8 + gp) carry PIMY history (committed
+ * and trial, at rest at creation). fx_force runs the return map at every point (the trial
+ * history; InternalForce::compute), fx_operator_apply / _diagonal / fx_cg use the tangents kept
+ * by the last fx_store_tangents (store_tangents, internal_force.cpp:532-543; D_eff = cK D_t +
+ * c0 D_0, no Rayleigh damping), fx_explicit_cd commits after every step. No FP32 operator. */
+int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* desc, uint32_t flags, fx_batch** out);
+/* updateMaterialStage for every parameter set (0 elastic, 1 plastic). No-op on elastic batches. */
+int fx_set_material_stage(fx_batch* b, int32_t stage);
+/* The return map at u (trial history written), the tangents kept for the operator; f (may be
+ * NULL) receives the internal force. No-op on elastic batches. */
+int fx_store_tangents(fx_batch* b, const double* u, double* f);
+/* commit_state: the trial history becomes the committed one (Domain::commit). */
+int fx_commit_state(fx_batch* b);
+/* Device copy of the history, [FX_PIMY_NSTATE * npts] structure of arrays (committed or trial). */
+int fx_pimy_state(fx_batch* b, int committed, double* dst);
+
 #ifdef __cplusplus
 }
 #endif

@@ -162,6 +162,27 @@ void fxo_pimy_initial_state(const fxo_pimy_params* p, double* state);
 void fxo_pimy_update(const fxo_pimy_params* p, const double* committed, double* trial, const double eps[6],
                      double sig[6], double D[36]);
 
+/* ---- stdBrick<PressureIndependMultiYield> (fxo_brick.c, fxo_ops.c) ------------------------ */
+typedef struct fxo_pimy_elem {
+  fxo_pimy_params mat;
+  double b[3];       /* body force per unit volume */
+  int32_t mass_type; /* 0 consistent, 1 lumped */
+} fxo_pimy_elem;
+void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* const* C,
+                          double* const* T, double* fe, double* Dgp);
+void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                 const double* Dgp, const double* ve, double* ye);
+void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                    const double* Dgp, const int32_t* eq, double* de);
+/* global: material points e * 8 + gp (batch order), history point-major (FXO_PIMY_NSTATE each),
+ * tangents D[point][36] */
+void fxo_pimy_force(const fxo_compiled* c, const fxo_pimy_elem* params, const double* C, double* T,
+                    const double* u, double* f, double* D, int nthreads);
+void fxo_apply_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
+                  const double* x, double* y, int nthreads);
+void fxo_diagonal_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
+                     double* d, int nthreads);
+
 /* ---- Internal force, stable step, CentralDifference (fxo_time.c) ------------------------- */
 /* f = gather of the elements' stdBrick forces at u (damping force from v; body force), the
  * flat force path (internal_force.cpp:354-377, element_kernels.hpp:198-235). */

@@ -48,6 +48,37 @@ class Coeffs(C.Structure):
     _fields_ = [("stiffness", C.c_double), ("damping", C.c_double), ("mass", C.c_double), ("initial", C.c_int32)]
 
 
+class PimyParams(C.Structure):
+    _fields_ = [("load_stage", C.c_int32), ("nsurf", C.c_int32)] + [
+        (n, C.c_double) for n in ("rho", "G", "K", "phi", "gamma_peak", "pref", "cohesion", "pdc", "residual_press")
+    ] + [("size", C.c_double * 21), ("modulus", C.c_double * 21)]
+
+
+class PimyElem(C.Structure):
+    _fields_ = [("mat", PimyParams), ("b", C.c_double * 3), ("mass_type", C.c_int32)]
+
+
+PIMY_NSTATE = 190
+
+
+def pimy_elems(materials, stage=0):
+    """materials: dicts with rho G K cohesion gamma_peak phi pref pdc [nsurf gredu b mass_type]
+    -> ctypes array of fxo_pimy_elem (setUpSurfaces by the oracle)."""
+    arr = (PimyElem * max(1, len(materials)))()
+    for i, m in enumerate(materials):
+        inp = np.array([m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"]])
+        g = m.get("gredu")
+        g = None if g is None else np.ascontiguousarray(g, np.float64)
+        rc = lib().fxo_pimy_setup(inp.ctypes.data, int(m.get("nsurf", 20)), None if g is None else g.ctypes.data,
+                                  C.byref(arr[i].mat))
+        if rc != 0:
+            raise ValueError(f"fxo_pimy_setup: {rc}")
+        arr[i].mat.load_stage = stage
+        arr[i].b = (C.c_double * 3)(*m.get("b", (0.0, 0.0, 0.0)))
+        arr[i].mass_type = int(m.get("mass_type", 0))
+    return arr
+
+
 class SolveResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
                 ("rhs_norm", C.c_double), ("initial_residual", C.c_double), ("final_residual", C.c_double)]
@@ -88,6 +119,12 @@ def lib():
         L.fxo_critical_dt.argtypes = [C.POINTER(Compiled), C.c_void_p, C.c_int]
         L.fxo_cd_run.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                  C.c_void_p, C.c_double, C.c_int32, f64p, f64p, f64p, f64p, C.c_int]
+        L.fxo_pimy_setup.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(PimyParams)]
+        L.fxo_pimy_nstate.restype = C.c_int
+        L.fxo_pimy_initial_state.argtypes = [C.POINTER(PimyParams), C.c_void_p]
+        L.fxo_pimy_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p, f64p, f64p, C.c_void_p, C.c_int]
+        L.fxo_apply_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, f64p, C.c_int]
+        L.fxo_diagonal_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_int]
         L.fxo_cg.restype = C.c_int
         L.fxo_cg.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_double,
                              C.c_double, C.c_int32, C.c_int, C.c_int, C.POINTER(SolveResult)]
@@ -251,6 +288,34 @@ class OracleMesh:
         lib().fxo_cd_run(C.byref(self.c), C.cast(params, C.c_void_p), np.ascontiguousarray(m, np.float64),
                          None if cmv is None else cmv.ctypes.data, None if dsv is None else dsv.ctypes.data, len(le), le.ctypes.data, lh.ctypes.data, dt,
                          nsteps, u, um, v, a, nthreads)
+
+    # ---- PressureIndependMultiYield batches: points e * 8 + gp, history [point, PIMY_NSTATE]
+    def pimy_initial_state(self, elems):
+        S = np.zeros((self.nelem * 8, PIMY_NSTATE))
+        one = np.zeros(PIMY_NSTATE)
+        for e in range(self.nelem):
+            lib().fxo_pimy_initial_state(C.byref(elems[self.param[e]].mat), one.ctypes.data)
+            S[8 * e:8 * e + 8] = one
+        return S
+
+    def pimy_force(self, elems, Cs, Ts, u, want_D=False, nthreads=0):
+        f = np.zeros(self.neq)
+        D = np.zeros((self.nelem * 8, 36)) if want_D else None
+        lib().fxo_pimy_force(C.byref(self.c), C.cast(elems, C.c_void_p), Cs, Ts, np.ascontiguousarray(u, np.float64), f,
+                             None if D is None else D.ctypes.data, nthreads)
+        return (f, D) if want_D else f
+
+    def apply_gp(self, elems, D, x, k=None, nthreads=0):
+        y = np.zeros(self.neq)
+        lib().fxo_apply_gp(C.byref(self.c), C.cast(elems, C.c_void_p), C.byref(k or coeffs()), np.ascontiguousarray(D),
+                           np.ascontiguousarray(x, np.float64), y, nthreads)
+        return y
+
+    def diagonal_gp(self, elems, D, k=None, nthreads=0):
+        d = np.zeros(self.neq)
+        lib().fxo_diagonal_gp(C.byref(self.c), C.cast(elems, C.c_void_p), C.byref(k or coeffs()), np.ascontiguousarray(D),
+                              d, nthreads)
+        return d
 
     def cg(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, lanes=1, k=None, nthreads=0):
         x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)

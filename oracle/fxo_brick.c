@@ -386,3 +386,146 @@ void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, cons
       }
   }
 }
+
+/* ---- stdBrick with PressureIndependMultiYield ------------------------------------------------ */
+/* Brick::compute_with (std_brick.hpp:493-543) with the material update per Gauss point:
+ * fe = sum_gp [B^T sigma dvol - b N dvol] in OpenSees' per-node order (resid += residJ;
+ * resid -= body). C[gp], T[gp]: the point's committed and trial history (FXO_PIMY_NSTATE
+ * doubles each); Dgp: the points' tangents (8 x 36) or NULL. No Rayleigh damping. */
+void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* const* C,
+                          double* const* T, double* fe, double* Dgp) {
+  double xl[3][8];
+  local_coords(xe, xl);
+  double resid[24];
+  for (int a = 0; a < 24; ++a) resid[a] = 0.0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double shp[4][8], dvol, eps[6], sig[6], D[36];
+    fxo_brick_shape(xl, gp, shp, &dvol);
+    point_strain(shp, ue, eps);
+    fxo_pimy_update(&p->mat, C[gp], T[gp], eps, sig, D);
+    if (Dgp) memcpy(Dgp + 36 * gp, D, sizeof D);
+    for (int j = 0; j < 8; ++j) {
+      double rj[3], body[3];
+      point_node_force(shp, dvol, sig, j, rj);
+      for (int q = 0; q < 3; ++q) body[q] = dvol * p->b[q] * shp[3][j];
+      for (int q = 0; q < 3; ++q) {
+        resid[3 * j + q] += rj[q];
+        resid[3 * j + q] -= body[q];
+      }
+    }
+  }
+  for (int a = 0; a < 24; ++a) fe[a] = resid[a];
+}
+
+/* the elastic initial tangent of the material's reference moduli (initial_tangent) */
+static void pimy_initial_D(const fxo_pimy_elem* p, double* D) {
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) {
+      double v = 0.;
+      if (i == j) v += p->mat.G;
+      if (i < 3 && j < 3 && i == j) v += p->mat.G;
+      if (i < 3 && j < 3) v += (p->mat.K - 2. * p->mat.G / 3.);
+      D[6 * i + j] = v;
+    }
+}
+
+/* element_operator_apply with stored tangents (Source::Stored, element_operator.hpp:82-130):
+ * D_eff(gp) = cK D_t(gp) + c0 D_0 (Rayleigh off: cK = stiffness, c0 = 0, or c0 = stiffness for
+ * the initial operator); cM' = mass. */
+void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                 const double* Dgp, const double* ve, double* ye) {
+  const double cK = k->initial ? 0.0 : k->stiffness, c0 = k->initial ? k->stiffness : 0.0, cM = k->mass;
+  double xl[3][8], D0[36];
+  local_coords(xe, xl);
+  pimy_initial_D(p, D0);
+  fxo_elastic me;
+  memset(&me, 0, sizeof me);
+  me.rho = p->mat.rho;
+  me.mass_type = p->mass_type;
+  for (int a = 0; a < 24; ++a) ye[a] = 0.0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double shp[4][8], dvol, eps[6], sig[6], Deff[36];
+    fxo_brick_shape(xl, gp, shp, &dvol);
+    for (int a = 0; a < 36; ++a) {
+      double v = cK * Dgp[36 * gp + a];
+      if (c0 != 0.0) v += c0 * D0[a];
+      Deff[a] = v;
+    }
+    point_strain(shp, ve, eps);
+    for (int a = 0; a < 6; ++a) {
+      double sum = 0.0;
+      for (int c = 0; c < 6; ++c) sum += Deff[6 * a + c] * eps[c];
+      sig[a] = sum;
+    }
+    for (int j = 0; j < 8; ++j) {
+      double fj[3];
+      point_node_force(shp, dvol, sig, j, fj);
+      for (int q = 0; q < 3; ++q) ye[3 * j + q] += fj[q];
+      if (cM != 0.0) {
+        double mj[3];
+        point_mass_apply(&me, shp, dvol, ve, j, mj);
+        for (int q = 0; q < 3; ++q) ye[3 * j + q] += cM * mj[q];
+      }
+    }
+  }
+}
+
+/* element_operator_diagonal with stored tangents, merged DOFs' cross terms on the first */
+void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                    const double* Dgp, const int32_t* eq, double* de) {
+  const double cK = k->initial ? 0.0 : k->stiffness, c0 = k->initial ? k->stiffness : 0.0, cM = k->mass;
+  double xl[3][8], D0[36];
+  local_coords(xe, xl);
+  pimy_initial_D(p, D0);
+  fxo_elastic me;
+  memset(&me, 0, sizeof me);
+  me.rho = p->mat.rho;
+  me.mass_type = p->mass_type;
+  for (int a = 0; a < 24; ++a) de[a] = 0.0;
+  int first[24], merged = 0;
+  for (int a = 0; a < 24; ++a) {
+    first[a] = a;
+    if (eq[a] < 0) continue;
+    for (int b = 0; b < a; ++b)
+      if (eq[b] == eq[a]) {
+        first[a] = first[b];
+        merged = 1;
+        break;
+      }
+  }
+  for (int gp = 0; gp < 8; ++gp) {
+    double shp[4][8], dvol, Deff[36];
+    fxo_brick_shape(xl, gp, shp, &dvol);
+    for (int a = 0; a < 36; ++a) {
+      double v = cK * Dgp[36 * gp + a];
+      if (c0 != 0.0) v += c0 * D0[a];
+      Deff[a] = v;
+    }
+    for (int j = 0; j < 8; ++j) {
+      double blk[3][3];
+      point_block_tangent(shp, dvol, Deff, j, j, blk);
+      const double m = cM != 0.0 ? cM * point_mass_entry(&me, shp, dvol, j, j) : 0.0;
+      for (int q = 0; q < 3; ++q) de[3 * j + q] += blk[q][q] + m;
+    }
+    if (!merged) continue;
+    for (int J = 0; J < 8; ++J)
+      for (int K = 0; K < 8; ++K) {
+        int any = 0;
+        for (int a = 0; a < 3 && !any; ++a)
+          for (int c = 0; c < 3; ++c) {
+            const int ia = 3 * J + a, ic = 3 * K + c;
+            if (ia != ic && eq[ia] >= 0 && eq[ia] == eq[ic]) any = 1;
+          }
+        if (!any) continue;
+        double blk[3][3];
+        point_block_tangent(shp, dvol, Deff, J, K, blk);
+        const double m = (cM != 0.0 && J != K) ? cM * point_mass_entry(&me, shp, dvol, J, K) : 0.0;
+        for (int a = 0; a < 3; ++a)
+          for (int c = 0; c < 3; ++c) {
+            const int ia = 3 * J + a, ic = 3 * K + c;
+            if (ia == ic || eq[ia] < 0 || eq[ia] != eq[ic]) continue;
+            de[first[ia]] += blk[a][c] + (a == c ? m : 0.0);
+          }
+      }
+  }
+}

@@ -1,0 +1,167 @@
+"""GPU parity of stdBrick<PressureIndependMultiYield> batches against the oracle: the internal
+force with the return map at every Gauss point (trial history), the operator and its diagonal
+on the stored tangents, commits, stage switches, CG on the tangent operator, and the explicit
+loop with the history committed every step.
+
+Tolerances: forces, operator, diagonal 1e-10 of the largest entry; histories 1e-10 of each
+field's largest value; CG 1e-8 (the reference's time-history tolerance).
+"""
+import numpy as np
+import pytest
+
+from femorax_amd import core
+from femorax_amd import synthetic as S
+from oracle import fxo
+from tests import meshes
+
+pytestmark = pytest.mark.gpu
+NS = core.PIMY_NSTATE
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = core.Context(0)
+    yield c
+    c.close()
+
+
+def rel(got, want):
+    return np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-300)
+
+
+def soil(seed, nlayer):
+    rng = S.Lcg(seed)
+    out = []
+    for _ in range(nlayer):
+        G = rng.uniform(10e6, 300e6)
+        out.append(dict(rho=2000.0, G=G, K=2.17 * G, cohesion=rng.uniform(20e3, 150e3), gamma_peak=0.1, phi=0.0,
+                        pref=100e3, pdc=0.0, nsurf=20, b=(0.0, 0.0, -2000.0 * 9.81), mass_type=1))
+    return out
+
+
+def pimy_model(name):
+    if name == "column12_tied":
+        m = meshes.column(12)
+    else:
+        m = meshes.distorted_box(3, layers=3)
+    mats = soil(20261206, 3)
+    return m, mats
+
+
+def setup(ctx, name):
+    model, mats = pimy_model(name)
+    pm = core.Mesh(model)
+    per_set = [mats[g] for g in pm.param_group]
+    keep = []
+    dev_mats = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep)
+                for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev_mats)
+    om = fxo.OracleMesh(model)
+    return model, pm, batch, om, per_set
+
+
+@pytest.mark.parametrize("name", ["column12_tied", "distorted3"])
+def test_force_tangents_operator_and_commits(ctx, name):
+    model, pm, batch, om, per_set = setup(ctx, name)
+    n = pm.neq
+    pe = fxo.pimy_elems(per_set, stage=0)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    np.testing.assert_array_equal(batch.pimy_state(True), Co)
+    dir_ = meshes.random_vector(n, 3)
+    du, df = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    x = meshes.random_vector(n, 4)
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, n)
+    for step in range(8):
+        stage = 0 if step < 2 else 1
+        if step == 2:
+            batch.set_stage(1)
+            pe = fxo.pimy_elems(per_set, stage=1)
+        u = dir_ * (2e-3 * (step + 1) * (1 if step % 3 else -0.5))
+        du.upload(u)
+        # a discarded trial first, then the one that is committed
+        batch.force(du, df)
+        f_o, D_o = om.pimy_force(pe, Co, To, u, want_D=True)
+        assert rel(df.get(), f_o) <= 1e-10, (step, rel(df.get(), f_o))
+        batch.store_tangents(du, df)
+        assert rel(df.get(), f_o) <= 1e-10
+        T_dev = batch.pimy_state(False)
+        scale = np.maximum(np.max(np.abs(To), axis=0), 1e-300)
+        assert np.max(np.abs(T_dev - To) / scale) <= 1e-10, step
+        for kk in ((1.0, 0.0, 0.0, False), (1.0, 0.0, 400.0, False), (1.0, 0.0, 0.0, True)):
+            batch.apply(dx, dy, core.coeffs(*kk))
+            want = om.apply_gp(pe, D_o, x, fxo.coeffs(*kk))
+            assert rel(dy.get(), want) <= 1e-10, (step, kk)
+            batch.diagonal(dy, core.coeffs(*kk))
+            want = om.diagonal_gp(pe, D_o, fxo.coeffs(*kk))
+            assert rel(dy.get(), want) <= 1e-10, (step, kk)
+        batch.commit()
+        Co, To = To, Co
+    active = T_dev[:, 26]
+    assert np.any(active > 0), "the walk reached the plastic range"
+
+
+def test_cg_on_the_tangent_operator(ctx):
+    model, pm, batch, om, per_set = setup(ctx, "column12_tied")
+    n = pm.neq
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    u = meshes.random_vector(n, 9) * 5e-3
+    du = core.DeviceArray.from_host(ctx, u)
+    batch.store_tangents(du)
+    _, D_o = om.pimy_force(pe, Co, To, u, want_D=True)
+    k = (1.0, 0.0, 1.6e3, False)  # Newmark-like c3 M term keeps it definite
+    b = meshes.random_vector(n, 10)
+    db, dx = core.DeviceArray.from_host(ctx, b), core.DeviceArray(ctx, n)
+    dx.zero()
+    r = batch.cg(db, dx, rtol=1e-12, k=core.coeffs(*k))
+    assert r["converged"]
+    A = np.column_stack([om.apply_gp(pe, D_o, e, fxo.coeffs(*k)) for e in np.eye(n)])
+    xd = np.linalg.solve(A, b)
+    assert rel(dx.get(), xd) <= 1e-8
+
+
+def test_explicit_loop_commits_the_history(ctx):
+    model, pm, batch, om, per_set = setup(ctx, "column12_tied")
+    n = pm.neq
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    m, _ = om.lumped_mass(pe_as_elastic(per_set))
+    dt = 2e-4
+    nsteps = 40
+    u0 = np.zeros(n)
+    v0 = meshes.random_vector(n, 12) * 0.2
+    um0 = u0 - v0 * dt
+    load_eq = np.array([n - 1], np.int32)
+    hist = (2e5 * np.sin(np.arange(nsteps) * dt * 60.0))[:, None]
+    # oracle loop with the PIMY force and a commit per step
+    u, um, v, a = u0.copy(), um0.copy(), v0.copy(), np.zeros(n)
+    c2, c3 = 0.5 / dt, 1.0 / dt**2
+    for k in range(nsteps):
+        f = om.pimy_force(pe, Co, To, u)
+        f[load_eq] -= hist[k]
+        ag = u * (-2.0 * c3) + um * c3
+        R = -f - m * ag
+        U = R / (m * c3)
+        Ud = (U * 3.0 + u * -4.0 + um) * c2
+        a = (Ud - v) / dt
+        um, u, v = u, U, Ud
+        Co, To = To, Co
+    du, dum = core.DeviceArray.from_host(ctx, u0), core.DeviceArray.from_host(ctx, um0)
+    dv, da = core.DeviceArray.from_host(ctx, v0), core.DeviceArray(ctx, n)
+    da.zero()
+    batch.explicit_cd(dt, nsteps, du, dum, dv, da, load_eq=load_eq, load_hist=hist.ravel())
+    assert rel(du.get(), u) <= 1e-9
+    assert rel(dv.get(), v) <= 1e-8
+    scale = np.maximum(np.max(np.abs(Co), axis=0), 1e-300)
+    assert np.max(np.abs(batch.pimy_state(True) - Co) / scale) <= 1e-9
+
+
+def pe_as_elastic(per_set):
+    """The lumped mass only needs rho (elastic parameter records with the PIMY sets' rho)."""
+    return fxo.elastic_params([dict(E=1.0, nu=0.3, rho=m["rho"], mass_type=m["mass_type"]) for m in per_set])
