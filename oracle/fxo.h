@@ -183,6 +183,40 @@ void fxo_apply_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_
 void fxo_diagonal_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
                      double* d, int nthreads);
 
+/* ---- implicit Newmark + Newton (fxo_newmark.c) -------------------------------------------- */
+typedef struct fxo_system {
+  const fxo_compiled* c;
+  const fxo_elastic* elastic; /* elastic parameter sets, or NULL */
+  const fxo_pimy_elem* pimy;  /* PIMY parameter sets, or NULL */
+  double* C;                  /* PIMY history (point-major), committed and trial */
+  double* T;
+  double* D;                  /* PIMY stored tangents [points * 36] */
+  int nthreads;
+} fxo_system;
+typedef struct fxo_newmark_opts {
+  double gamma, beta, dt;
+  int32_t nsteps;
+  int32_t test;          /* 0 NormUnbalance, 1 NormDispIncr */
+  double tol;
+  int32_t max_iter;
+  double cg_rtol;
+  int32_t cg_max_it;
+  int32_t lanes;         /* dot order: 64 as the device */
+  const double* ground_accel; /* [nsteps] a_g at the end of each step, or NULL */
+} fxo_newmark_opts;
+typedef struct fxo_newmark_stats {
+  int32_t steps, newton_iterations, max_newton_iterations, krylov_iterations;
+  double last_test;
+} fxo_newmark_stats;
+/* PCG + Jacobi on a system (cg.cpp:24-72) */
+int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
+                  int32_t max_it, int lanes, fxo_solve_result* res);
+/* nsteps Newmark steps with Newton (implicit_stage.cpp); ground_dir: the influence vector of the
+ * excitation's direction (1 at its equations), or NULL. Returns 0, or 1 at the first failed step
+ * (its state restored). du: the Krylov initial guess in, the last increment out. */
+int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
+                double* du, fxo_newmark_stats* st);
+
 /* ---- Internal force, stable step, CentralDifference (fxo_time.c) ------------------------- */
 /* f = gather of the elements' stdBrick forces at u (damping force from v; body force), the
  * flat force path (internal_force.cpp:354-377, element_kernels.hpp:198-235). */

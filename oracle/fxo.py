@@ -79,6 +79,22 @@ def pimy_elems(materials, stage=0):
     return arr
 
 
+class System(C.Structure):
+    _fields_ = [("c", C.c_void_p), ("elastic", C.c_void_p), ("pimy", C.c_void_p), ("C", C.c_void_p), ("T", C.c_void_p),
+                ("D", C.c_void_p), ("nthreads", C.c_int)]
+
+
+class NewmarkOpts(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double), ("nsteps", C.c_int32),
+                ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
+                ("cg_max_it", C.c_int32), ("lanes", C.c_int32), ("ground_accel", C.c_void_p)]
+
+
+class NewmarkStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("newton_iterations", C.c_int32), ("max_newton_iterations", C.c_int32),
+                ("krylov_iterations", C.c_int32), ("last_test", C.c_double)]
+
+
 class SolveResult(C.Structure):
     _fields_ = [("converged", C.c_int32), ("iterations", C.c_int32), ("indefinite", C.c_int32),
                 ("rhs_norm", C.c_double), ("initial_residual", C.c_double), ("final_residual", C.c_double)]
@@ -125,6 +141,9 @@ def lib():
         L.fxo_pimy_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p, f64p, f64p, C.c_void_p, C.c_int]
         L.fxo_apply_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, f64p, C.c_int]
         L.fxo_diagonal_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_int]
+        L.fxo_newmark.restype = C.c_int
+        L.fxo_newmark.argtypes = [C.POINTER(System), C.POINTER(NewmarkOpts), C.c_void_p, f64p, f64p, f64p, f64p,
+                                  C.POINTER(NewmarkStats)]
         L.fxo_cg.restype = C.c_int
         L.fxo_cg.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_double,
                              C.c_double, C.c_int32, C.c_int, C.c_int, C.POINTER(SolveResult)]
@@ -316,6 +335,35 @@ class OracleMesh:
         lib().fxo_diagonal_gp(C.byref(self.c), C.cast(elems, C.c_void_p), C.byref(k or coeffs()), np.ascontiguousarray(D),
                               d, nthreads)
         return d
+
+    def newmark(self, u, v, a, du, dt, nsteps, elastic=None, pimy=None, pimy_state=None, gamma=0.5, beta=0.25,
+                test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, ground_dir=None, ground_accel=None,
+                lanes=64, nthreads=0):
+        """nsteps Newmark + Newton steps (implicit_stage.cpp) on u, v, a, du (numpy, in place).
+        pimy_state: [C, T] point-major histories (swapped in place on commits). Returns
+        (status, stats dict)."""
+        s = System()
+        s.c = C.cast(C.pointer(self.c), C.c_void_p)
+        s.elastic = C.cast(elastic, C.c_void_p) if elastic is not None else None
+        s.pimy = C.cast(pimy, C.c_void_p) if pimy is not None else None
+        keep = []
+        if pimy is not None:
+            Cs, Ts = pimy_state
+            D = np.zeros((self.nelem * 8, 36))
+            keep += [Cs, Ts, D]
+            s.C, s.T, s.D = Cs.ctypes.data, Ts.ctypes.data, D.ctypes.data
+        s.nthreads = nthreads
+        ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        o = NewmarkOpts(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, lanes,
+                        None if ga is None else ga.ctypes.data)
+        gd = None if ground_dir is None else np.ascontiguousarray(ground_dir, np.float64)
+        st = NewmarkStats()
+        rc = lib().fxo_newmark(C.byref(s), C.byref(o), None if gd is None else gd.ctypes.data, u, v, a, du, C.byref(st))
+        if pimy is not None and s.C != pimy_state[0].ctypes.data:  # an odd number of commits
+            tmp = pimy_state[0].copy()
+            pimy_state[0][...] = pimy_state[1]
+            pimy_state[1][...] = tmp
+        return rc, {f: getattr(st, f) for f, _ in NewmarkStats._fields_}
 
     def cg(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, lanes=1, k=None, nthreads=0):
         x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)

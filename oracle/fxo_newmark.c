@@ -1,0 +1,189 @@
+/*
+ * fxo implicit Newmark stage with Newton iterations — TEST INFRASTRUCTURE (the oracle; see fxo.h).
+ *
+ * Restates src/exec/implicit_stage.cpp: newmark_predict :29-40, newmark_update :43-52,
+ * form_residual :55-68, the integrator constants and operator coefficients :150-159, 283-286,
+ * update_tangent :180-193, form_unbalance :212-219, solve :223-238 (Newton accepts an increment
+ * whose solve reduced the residual by 1e-6), test_value :239-246, the Newton attempt :274-360
+ * (without retries and substeps), commit :365-372; the PCG of src/solvers/cg.cpp:24-72 with
+ * Jacobi (jacobi.cpp:10-26) on a generic system (elastic operator, or the stored tangents of
+ * PressureIndependMultiYield batches).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "fxo.h"
+
+static void sys_force(fxo_system* s, const double* u, const double* v, double* f) {
+  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, f, NULL, s->nthreads);
+  else fxo_force(s->c, s->elastic, u, v, f, s->nthreads);
+}
+static void sys_store(fxo_system* s, const double* u, double* scratch) {
+  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, scratch, s->D, s->nthreads);
+}
+static void sys_apply(fxo_system* s, const fxo_coeffs* k, const double* x, double* y) {
+  if (s->pimy) fxo_apply_gp(s->c, s->pimy, k, s->D, x, y, s->nthreads);
+  else fxo_apply(s->c, s->elastic, k, x, y, s->nthreads);
+}
+static void sys_diagonal(fxo_system* s, const fxo_coeffs* k, double* d) {
+  if (s->pimy) fxo_diagonal_gp(s->c, s->pimy, k, s->D, d, s->nthreads);
+  else fxo_diagonal(s->c, s->elastic, k, d, s->nthreads);
+}
+
+int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
+                  int32_t max_it, int lanes, fxo_solve_result* res) {
+  const int64_t n = s->c->neq;
+  memset(res, 0, sizeof(*res));
+  double* r = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* z = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* p = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* ap = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* inv = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  int status = 0;
+  sys_diagonal(s, k, inv);
+  for (int64_t i = 0; i < n; ++i) {
+    if (inv[i] == 0.0) status = -1;
+    inv[i] = 1.0 / inv[i];
+  }
+  if (status == 0) {
+    res->rhs_norm = sqrt(fxo_dot(n, b, b, lanes));
+    sys_apply(s, k, x, ap);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i];
+    for (int64_t i = 0; i < n; ++i) r[i] += -1.0 * ap[i];
+    double rnorm = sqrt(fxo_dot(n, r, r, lanes));
+    res->initial_residual = res->final_residual = rnorm;
+    const double tol = fmax(rtol * res->rhs_norm, atol);
+    if (rnorm <= tol) {
+      res->converged = 1;
+    } else {
+      for (int64_t i = 0; i < n; ++i) z[i] = inv[i] * r[i];
+      for (int64_t i = 0; i < n; ++i) p[i] = z[i];
+      double rz = fxo_dot(n, r, z, lanes);
+      for (int32_t it = 1; it <= max_it; ++it) {
+        sys_apply(s, k, p, ap);
+        const double pap = fxo_dot(n, p, ap, lanes);
+        if (!(pap > 0.0)) {
+          res->indefinite = 1;
+          res->iterations = it - 1;
+          break;
+        }
+        const double alpha = rz / pap;
+        for (int64_t i = 0; i < n; ++i) x[i] += alpha * p[i];
+        for (int64_t i = 0; i < n; ++i) r[i] += -alpha * ap[i];
+        rnorm = sqrt(fxo_dot(n, r, r, lanes));
+        res->iterations = it;
+        res->final_residual = rnorm;
+        if (rnorm <= tol) {
+          res->converged = 1;
+          break;
+        }
+        for (int64_t i = 0; i < n; ++i) z[i] = inv[i] * r[i];
+        const double rz_new = fxo_dot(n, r, z, lanes);
+        const double beta = rz_new / rz;
+        rz = rz_new;
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+      }
+    }
+  }
+  free(r);
+  free(z);
+  free(p);
+  free(ap);
+  free(inv);
+  return status;
+}
+
+int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
+                double* du, fxo_newmark_stats* st) {
+  const int64_t n = s->c->neq;
+  memset(st, 0, sizeof(*st));
+  double* ut = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* vt = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* at = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* f = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* tmp = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* b = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* mr = (double*)calloc((size_t)n + 1, sizeof(double));
+  double* scratch = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  const double gamma = o->gamma, beta = o->beta, h = o->dt;
+  const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
+  if (ground_dir) sys_apply(s, &mass, ground_dir, mr); /* mass_times_ground */
+  const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
+  const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
+  const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+  int status = 0;
+  for (int32_t step = 0; step < o->nsteps && status == 0; ++step) {
+    for (int64_t q = 0; q < n; ++q) {
+      ut[q] = u[q];
+      vt[q] = v[q];
+      at[q] = a[q];
+      v[q] = v[q] * a1 + at[q] * a2;
+      a[q] = a[q] * a4 + vt[q] * a3;
+    }
+    const double ag = o->ground_accel ? o->ground_accel[step] : 0.0;
+    /* form_unbalance: f(u) at the velocities v, inertia M a, the ground load */
+#define FORM_UNBALANCE()                                        \
+  do {                                                          \
+    sys_force(s, u, v, f);                                      \
+    sys_apply(s, &mass, a, tmp);                                \
+    for (int64_t q = 0; q < n; ++q) {                           \
+      double r = -f[q];                                         \
+      r -= tmp[q];                                              \
+      if (ground_dir) r -= ag * mr[q];                          \
+      b[q] = r;                                                 \
+    }                                                           \
+  } while (0)
+    FORM_UNBALANCE();
+    int iter = 1, ok = 1;
+    for (;;) {
+      sys_store(s, u, scratch); /* update_tangent: the tangents at u, Jacobi set up on them */
+      fxo_solve_result r;
+      fxo_cg_system(s, &k, b, du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, &r);
+      st->krylov_iterations += r.iterations;
+      if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
+        ok = 0;
+        break;
+      }
+      for (int64_t q = 0; q < n; ++q) {
+        u[q] += du[q];
+        v[q] += du[q] * k.damping;
+        a[q] += du[q] * k.mass;
+      }
+      FORM_UNBALANCE();
+      const double value = o->test == 0 ? sqrt(fxo_dot(n, b, b, o->lanes)) : sqrt(fxo_dot(n, du, du, o->lanes));
+      st->last_test = value;
+      if (value <= o->tol) break;
+      if (iter >= o->max_iter) {
+        ok = 0;
+        break;
+      }
+      ++iter;
+    }
+#undef FORM_UNBALANCE
+    st->newton_iterations += iter;
+    if (iter > st->max_newton_iterations) st->max_newton_iterations = iter;
+    if (!ok) {
+      memcpy(u, ut, (size_t)n * sizeof(double));
+      memcpy(v, vt, (size_t)n * sizeof(double));
+      memcpy(a, at, (size_t)n * sizeof(double));
+      status = 1;
+      break;
+    }
+    if (s->pimy) { /* commit_state */
+      double* t = s->C;
+      s->C = s->T;
+      s->T = t;
+    }
+    st->steps = step + 1;
+  }
+  free(ut);
+  free(vt);
+  free(at);
+  free(f);
+  free(tmp);
+  free(b);
+  free(mr);
+  free(scratch);
+  return status;
+}

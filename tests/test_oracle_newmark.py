@@ -1,0 +1,85 @@
+"""Pins of the oracle's implicit Newmark stage with Newton (implicit_stage.cpp) on the CPU.
+
+- Elastic: the textbook Newmark recurrence (average acceleration, gamma 1/2, beta 1/4) with the
+  dense K, M and C = alphaM M + betaK K assembled from the oracle's operator (which a densely
+  assembled K pins), body force and a ground acceleration: the same displacements,
+  velocities and accelerations.
+- PressureIndependMultiYield: every converged step is in equilibrium, -f(u) - M a - a_g M r = 0
+  to the Newton tolerance, checked with the oracle's force (pinned by the PIMY goldens and the
+  element goldens), and the histories commit once per step.
+"""
+import numpy as np
+
+from oracle import fxo
+from tests import meshes
+
+
+def ground_dir(om, d):
+    r = np.zeros(om.neq)
+    q = om.node_eq.reshape(-1, 3)[:, d]
+    r[q[q >= 0]] = 1.0
+    return r
+
+
+def test_elastic_newmark_matches_the_textbook_recurrence():
+    model = meshes.column(8)
+    model["params"] = [dict(p, rayleigh=(0.4, 2e-4, 0.0, 0.0)) for p in model["params"]]
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    E = np.eye(n)
+    K = np.column_stack([om.apply(op, e) for e in E])
+    M = np.column_stack([om.apply(op, e, fxo.coeffs(0.0, 0.0, 1.0)) for e in E])
+    Cd = np.column_stack([om.apply(op, e, fxo.coeffs(0.0, 1.0, 0.0)) for e in E])
+    Fb = om.body_load(op)
+    r = ground_dir(om, 0)
+    h, nsteps, g, b = 0.004, 40, 0.5, 0.25
+    ag = 2.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 3)
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, elastic=op, ground_dir=r, ground_accel=ag, tol=1e-12, test=1)
+    assert rc == 0 and st["steps"] == nsteps
+    # textbook: M a1 + C v1 + K u1 = Fb - ag M r, with the Newmark kinematics
+    U, V, A = np.zeros(n), np.zeros(n), np.zeros(n)
+    Keff = K + g / (b * h) * Cd + M / (b * h * h)
+    for k in range(nsteps):
+        P = Fb - ag[k] * (M @ r)
+        rhs = P + M @ (U / (b * h * h) + V / (b * h) + (0.5 / b - 1) * A) + \
+            Cd @ (g / (b * h) * U + (g / b - 1) * V + h * (0.5 * g / b - 1) * A)
+        U1 = np.linalg.solve(Keff, rhs)
+        A1 = (U1 - U) / (b * h * h) - V / (b * h) - (0.5 / b - 1) * A
+        V1 = V + h * ((1 - g) * A + g * A1)
+        U, V, A = U1, V1, A1
+    assert np.max(np.abs(u - U)) <= 1e-9 * np.max(np.abs(U))
+    assert np.max(np.abs(v - V)) <= 1e-8 * np.max(np.abs(V))
+    assert np.max(np.abs(a - A)) <= 1e-7 * np.max(np.abs(A))
+
+
+def test_pimy_newmark_steps_end_in_equilibrium():
+    model = meshes.column(10)
+    om = fxo.OracleMesh(model)
+    mats = [dict(rho=2000.0, G=G, K=2.17 * G, cohesion=c, gamma_peak=0.1, phi=0.0, pref=100e3, pdc=0.0,
+                 b=(0.0, 0.0, 0.0), mass_type=1) for G, c in ((40e6, 30e3), (80e6, 60e3), (150e6, 90e3))]
+    sets = [mats[g] for g in om.param_group]
+    pe = fxo.pimy_elems(sets, stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    n = om.neq
+    r = ground_dir(om, 0)
+    h, nsteps = 0.005, 30
+    ag = 4.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 2)
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    op_mass = fxo.elastic_params([dict(E=1.0, nu=0.3, rho=m["rho"], mass_type=1) for m in sets])
+    for k in range(nsteps):
+        rc, st = om.newmark(u, v, a, du, h, 1, pimy=pe, pimy_state=[Cs, Ts], ground_dir=r, ground_accel=ag[k:k + 1],
+                            test=0, tol=1e-4, max_iter=25)
+        assert rc == 0 and st["steps"] == 1
+        # equilibrium at the committed state: the trial buffer is free, so evaluate on a copy
+        T2 = Cs.copy()
+        C_prev = Ts  # after the commit, Ts holds the previous committed history
+        f = om.pimy_force(pe, C_prev, T2, u)
+        Ma = om.apply(op_mass, a, fxo.coeffs(0.0, 0.0, 1.0))
+        Mr = om.apply(op_mass, r, fxo.coeffs(0.0, 0.0, 1.0))
+        res = -f - Ma - ag[k] * Mr
+        assert np.linalg.norm(res) <= 1e-4 * 1.0001, (k, np.linalg.norm(res))
+        np.testing.assert_array_equal(T2, Cs)  # the committed history is the state at u
+    assert np.max(Cs[:, 26]) > 0, "the excitation reached the plastic range"
