@@ -19,6 +19,7 @@ ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP"}
 FX_BATCH_FP32 = 1
 FX_BATCH_GENERAL = 2
 FX_BATCH_XCD_SLABS = 8
+FX_BATCH_DYNAMIC = 32
 FX_BATCH_NO_TUNE = 16
 
 
@@ -73,6 +74,17 @@ class SolveOptions(C.Structure):
 class CdDesc(C.Structure):
     _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("dash", C.c_void_p), ("nload", C.c_int32),
                 ("load_eq", C.c_void_p), ("load_hist", C.c_void_p)]
+
+
+class NewmarkDesc(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double), ("nsteps", C.c_int32),
+                ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
+                ("cg_max_it", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p)]
+
+
+class NewmarkResult(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("newton_iterations", C.c_int32), ("max_newton_iterations", C.c_int32),
+                ("krylov_iterations", C.c_int32), ("failed", C.c_int32), ("last_test", C.c_double)]
 
 
 class PimyBatchDesc(C.Structure):
@@ -165,6 +177,7 @@ SIGNATURES = [
     ("fx_pimy_surfaces", C.c_int, [C.POINTER(PimyMaterial), vp, vp]),
     ("fx_pimy_batch_create", C.c_int, [vp, C.POINTER(PimyBatchDesc), u32, C.POINTER(vp)]),
     ("fx_set_material_stage", C.c_int, [vp, C.c_int32]),
+    ("fx_newmark", C.c_int, [vp, C.POINTER(NewmarkDesc), vp, vp, vp, vp, C.POINTER(NewmarkResult)]),
     ("fx_store_tangents", C.c_int, [vp, vp, vp]),
     ("fx_commit_state", C.c_int, [vp]),
     ("fx_pimy_state", C.c_int, [vp, C.c_int, vp]),
@@ -373,7 +386,7 @@ def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
 class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
-    def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True):
+    def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -391,7 +404,7 @@ class Batch:
         d.neq = mesh.neq
         h = C.c_void_p()
         flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
-            (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE)
+            (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0)
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
 
@@ -420,6 +433,17 @@ class Batch:
         _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), 0, C.byref(h)), ctx)
         self.h = h
         return self
+
+    def newmark(self, u, v, a, du, dt, nsteps, gamma=0.5, beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12,
+                cg_max_it=10000, ground_dir=-1, ground_accel=None):
+        """nsteps Newmark + Newton steps on device arrays u, v, a, du (in place); returns the
+        fx_newmark_result as a dict."""
+        ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        d = NewmarkDesc(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, ground_dir,
+                        None if ga is None else ga.ctypes.data)
+        r = NewmarkResult()
+        _check(lib().fx_newmark(self.h, C.byref(d), u.ptr, v.ptr, a.ptr, du.ptr, C.byref(r)), self.ctx)
+        return {f: getattr(r, f) for f, _ in NewmarkResult._fields_}
 
     def set_stage(self, stage):
         _check(lib().fx_set_material_stage(self.h, stage), self.ctx)

@@ -194,7 +194,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list};
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -391,6 +391,15 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     ok(n > 0 ? dev_upload(ctx, b->lumped_m, lm.data(), lm.size(), by) : FX_OK) &&
         ok(any_am ? dev_upload(ctx, b->lumped_am, la.data(), la.size(), by) : FX_OK) &&
         ok(any_body ? dev_upload(ctx, b->body, fb.data(), fb.size(), by) : FX_OK);
+  }
+  if (rc == FX_OK) {
+    b->eq_dir.assign(static_cast<size_t>(d->neq), -1);
+    for (int32_t e = 0; e < E; ++e)
+      for (int k = 0; k < 24; ++k) {
+        const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
+        if (q >= 0) b->eq_dir[q] = static_cast<int8_t>(k % 3);
+      }
+    if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(12) * std::max(1, d->neq), by));
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);
@@ -703,7 +712,9 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
   ed.coords = d->coords;
   ed.neq = d->neq;
   fx_batch* b = nullptr;
-  if (int rc = fx_batch_create(ctx, &ed, (flags & ~FX_BATCH_FP32) | FX_BATCH_GENERAL | FX_BATCH_NO_TUNE, &b)) return rc;
+  if (int rc = fx_batch_create(ctx, &ed, (flags & ~FX_BATCH_FP32) | FX_BATCH_GENERAL | FX_BATCH_NO_TUNE | FX_BATCH_DYNAMIC,
+                               &b))
+    return rc;
   b->pimy = true;
   b->host_pimy = pp;
   b->npts = static_cast<int64_t>(d->count) * 8;
@@ -782,4 +793,15 @@ int fx_pimy_state(fx_batch* b, int committed, double* dst) {
                                 hipMemcpyDeviceToDevice, b->ctx->stream));
   FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return FX_OK;
+}
+
+int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, double* a, double* du,
+               fx_newmark_result* result) {
+  if (!b || !d || !u || !v || !a || !du || !result) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_newmark: null argument");
+  if (!b->dyn) return fail(b->ctx, FX_EINVAL, "fx_newmark: the batch was created without FX_BATCH_DYNAMIC");
+  if (!(d->dt > 0) || !(d->beta > 0) || d->nsteps < 0 || d->max_iter <= 0 || d->ground_dir > 2 ||
+      (d->ground_dir >= 0 && !d->ground_accel))
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: bad descriptor");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  return fx::newmark_run(b, *d, u, v, a, du, *result);
 }

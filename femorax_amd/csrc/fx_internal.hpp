@@ -199,6 +199,10 @@ struct fx_batch {
   double* slots = nullptr;            // [24 * padded]
   int32_t* gather_start = nullptr;    // [neq + 1]
   int32_t* gather_list = nullptr;     // slots in (element, DOF) order
+  // time-integration workspace (FX_BATCH_DYNAMIC): 12 * neq doubles; the DOF direction of each
+  // equation (reference numbering, host)
+  double* dyn = nullptr;
+  std::vector<int8_t> eq_dir;
   // solver work (allocated at create: compute allocates nothing)
   double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats

@@ -60,6 +60,9 @@ int copy64(fx_ctx* ctx, int64_t n, const double* a, double* b);
 int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& r);
+// CG + Jacobi in the solver numbering; uses the batch's work vectors 0-5.
+int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
+                fx_solve_result& r);
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
              double* x, fx_solve_result& r);
 
@@ -72,5 +75,9 @@ int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 // Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch);
 int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a);
+
+// The implicit Newmark stage (fx_implicit.hip).
+int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, double* a, double* du,
+                fx_newmark_result& out);
 
 }  // namespace fx

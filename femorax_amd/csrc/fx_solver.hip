@@ -356,15 +356,13 @@ int jacobi_inverse(fx_batch* b, const fx_coeffs& k, double* d, void* inv, bool f
 
 }  // namespace
 
-namespace {
-int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
-              fx_solve_result& res) {
+// CG + Jacobi on vectors already in the solver numbering (no permutation, no true residual).
+int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
+                fx_solve_result& res) {
   fx_ctx* ctx = b->ctx;
   const int64_t n = b->neq;
-  double *r = b->work, *z = r + n, *p = z + n, *ap = p + n, *inv = ap + n, *d = inv + n, *bi = d + n, *xi = bi + n;
+  double *r = b->work, *z = r + n, *p = z + n, *ap = p + n, *inv = ap + n, *d = inv + n;
   if (int rc = jacobi_inverse(b, k, d, inv, false)) return rc;
-  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
-  if (int rc = to_internal<double>(b, x, xi)) return rc;
   CgState* st = static_cast<CgState*>(ctx->scalars);
   CgState h{};
   auto apply = [&](const double* xx, double* yy, const int32_t* dn) { return operator_apply64(b, k, xx, yy, dn); };
@@ -378,6 +376,18 @@ int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const 
   res.rhs_norm = h.rhs_norm;
   res.initial_residual = h.init_res;
   res.final_residual = h.final_res;
+  return FX_OK;
+}
+
+namespace {
+int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+              fx_solve_result& res) {
+  fx_ctx* ctx = b->ctx;
+  const int64_t n = b->neq;
+  double *r = b->work, *ap = r + 3 * n, *bi = r + 6 * n, *xi = r + 7 * n;
+  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
+  if (int rc = to_internal<double>(b, x, xi)) return rc;
+  if (int rc = cg64_solver(b, k, o, bi, xi, res)) return rc;
   // true residual ||b - A x||
   if (int rc = operator_apply64(b, k, xi, ap)) return rc;
   hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, ctx->stream, n, bi, ap, r);

@@ -135,6 +135,8 @@ typedef struct fx_batch_desc {
 /* Flags for fx_batch_create. */
 #define FX_BATCH_FP32 1u    /* also build the FP32 operator (mixed precision solves) */
 #define FX_BATCH_GENERAL 2u  /* do not use the structured-lattice fast path (testing) */
+#define FX_BATCH_DYNAMIC 32u  /* allocate the time-integration workspace (12 * neq doubles) that
+                                 fx_newmark needs (PIMY batches always have it) */
 #define FX_BATCH_XCD_SLABS 8u /* lattice path: tiles placed in 8 slabs, one per XCD (default: lattice order,
                                 which measured faster on MI355X; same bits either way) */
 #define FX_BATCH_NO_TUNE 16u /* skip the kernel-variant autotune at creation (variant 0) */
@@ -286,6 +288,33 @@ int fx_store_tangents(fx_batch* b, const double* u, double* f);
 int fx_commit_state(fx_batch* b);
 /* Device copy of the history, [FX_PIMY_NSTATE * npts] structure of arrays (committed or trial). */
 int fx_pimy_state(fx_batch* b, int committed, double* dst);
+
+/* ---- implicit Newmark stage with Newton (implicit_stage.cpp) ---------------------------------- */
+typedef struct fx_newmark_desc {
+  double gamma, beta, dt;     /* Newmark constants and step */
+  int32_t nsteps;
+  int32_t test;               /* 0 NormUnbalance, 1 NormDispIncr */
+  double tol;
+  int32_t max_iter;           /* Newton iterations per step */
+  double cg_rtol;             /* the solver's relative tolerance (the reference's default 1e-12) */
+  int32_t cg_max_it;
+  int32_t ground_dir;         /* UniformExcitation direction 0..2, or -1 */
+  const double* ground_accel; /* host [nsteps]: the ground acceleration at the end of each step */
+} fx_newmark_desc;
+typedef struct fx_newmark_result {
+  int32_t steps;              /* converged (committed) steps */
+  int32_t newton_iterations, max_newton_iterations, krylov_iterations;
+  int32_t failed;             /* 1: step `steps` did not converge; its start state is restored */
+  double last_test;
+} fx_newmark_result;
+/* nsteps Newmark steps, each solved by Newton with PCG + Jacobi on K + gamma/(beta dt) C +
+ * 1/(beta dt^2) M (stored tangents for history materials), committing the material history after
+ * each converged step. u, v, a, du: device vectors in/out (du: the Krylov initial guess, the last
+ * increment out). Needs FX_BATCH_DYNAMIC (or a PIMY batch). Retries and substeps are the
+ * caller's (the reference's recover loop, :374-402): a failed step returns FX_OK with
+ * result.failed = 1. */
+int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, double* a, double* du,
+               fx_newmark_result* result);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,107 @@
+"""GPU parity of the implicit Newmark stage with Newton (implicit_stage.cpp) against the oracle's
+restatement: elastic tied column with Rayleigh damping and a ground acceleration (general
+path), elastic box with body force (lattice path), and a PressureIndependMultiYield column
+shaken into the plastic range (stored tangents, commits).
+
+Tolerances: displacements 1e-9 of the largest, velocities and accelerations 1e-8 (the solves
+stop at rtol 1e-12, so the two runs' iterates differ by rounding only); histories 1e-8.
+"""
+import numpy as np
+import pytest
+
+from femorax_amd import core
+from femorax_amd import synthetic as S
+from oracle import fxo
+from tests import meshes
+from tests.test_gpu_pimy_batch import soil
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = core.Context(0)
+    yield c
+    c.close()
+
+
+def rel(got, want):
+    return np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-300)
+
+
+def ground_dir(om, d):
+    r = np.zeros(om.neq)
+    q = om.node_eq.reshape(-1, 3)[:, d]
+    r[q[q >= 0]] = 1.0
+    return r
+
+
+def run_both(ctx, om, batch, n, h, nsteps, oracle_kw, ag, test=1, tol=1e-10):
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, ground_dir=ground_dir(om, 0), ground_accel=ag, test=test, tol=tol,
+                        **oracle_kw)
+    assert rc == 0 and st["steps"] == nsteps
+    d = [core.DeviceArray(ctx, n) for _ in range(4)]
+    for x in d:
+        x.zero()
+    r = batch.newmark(*d, h, nsteps, ground_dir=0, ground_accel=ag, test=test, tol=tol)
+    assert r["failed"] == 0 and r["steps"] == nsteps, r
+    assert abs(r["newton_iterations"] - st["newton_iterations"]) <= nsteps // 10 + 1, (r, st)
+    return (u, v, a), tuple(x.get() for x in d[:3])
+
+
+def test_elastic_column_with_damping_and_ground_motion(ctx):
+    model = meshes.column(16)
+    model["params"] = [dict(p, rayleigh=(0.4, 2e-4, 0.0, 0.0)) for p in model["params"]]
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    batch = core.Batch(ctx, pm, params, dynamic=True)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    h, nsteps = 0.004, 40
+    ag = 2.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 3)
+    want, got = run_both(ctx, om, batch, pm.neq, h, nsteps, dict(elastic=op), ag)
+    for g, w, tol in zip(got, want, (1e-9, 1e-8, 1e-8)):
+        assert rel(g, w) <= tol
+
+
+def test_elastic_box_lattice_path(ctx):
+    model = S.elastic_box(8, 8.0, layers=2, body=True, seed=5)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    batch = core.Batch(ctx, pm, params, dynamic=True)
+    assert batch.info()["lattice"] == 1
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    h, nsteps = 0.002, 25
+    ag = 1.5 * np.cos(np.arange(1, nsteps + 1) * h * 2 * np.pi * 5)
+    want, got = run_both(ctx, om, batch, pm.neq, h, nsteps, dict(elastic=op), ag)
+    for g, w, tol in zip(got, want, (1e-9, 1e-8, 1e-8)):
+        assert rel(g, w) <= tol
+
+
+def test_pimy_column_shaken_into_plasticity(ctx):
+    model = meshes.column(12)
+    pm = core.Mesh(model)
+    mats = soil(20261206, 3)
+    per_set = [mats[g] for g in pm.param_group]
+    for m in per_set:
+        m["b"] = (0.0, 0.0, 0.0)
+    keep = []
+    dev = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"],
+                              nsurf=20, b=m["b"], mass_type=m["mass_type"], keep=keep) for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev)
+    batch.set_stage(1)
+    om = fxo.OracleMesh(model)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    h, nsteps = 0.005, 20
+    ag = 6.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 2)
+    want, got = run_both(ctx, om, batch, pm.neq, h, nsteps, dict(pimy=pe, pimy_state=[Cs, Ts]), ag, test=0, tol=1e-3)
+    for g, w, tol in zip(got, want, (1e-8, 1e-7, 1e-7)):
+        assert rel(g, w) <= tol
+    Cd = batch.pimy_state(True)
+    scale = np.maximum(np.max(np.abs(Cs), axis=0), 1e-300)
+    assert np.max(np.abs(Cd - Cs) / scale) <= 1e-7
+    assert np.max(Cs[:, 26]) > 0, "the shaking reached the plastic range"
