@@ -20,6 +20,7 @@ FX_BATCH_FP32 = 1
 FX_BATCH_GENERAL = 2
 FX_BATCH_XCD_SLABS = 8
 FX_BATCH_DYNAMIC = 32
+FX_BATCH_GMRES = 64
 FX_BATCH_NO_TUNE = 16
 
 
@@ -169,6 +170,8 @@ SIGNATURES = [
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
     ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
+    ("fx_gmres", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
+    ("fx_krylov", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_lumped_mass", C.c_int, [vp, vp, vp]),
     ("fx_body_load", C.c_int, [vp, vp]),
     ("fx_force", C.c_int, [vp, vp, vp, vp]),
@@ -386,7 +389,8 @@ def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
 class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
-    def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False):
+    def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False,
+                 gmres=False):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -404,7 +408,8 @@ class Batch:
         d.neq = mesh.neq
         h = C.c_void_p()
         flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
-            (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0)
+            (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0) | \
+            (FX_BATCH_GMRES if gmres else 0)
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
 
@@ -498,6 +503,18 @@ class Batch:
         o = SolveOptions(rtol, atol, max_it)
         r = SolveResult()
         _check(lib().fx_cg(self.h, C.byref(k or coeffs()), C.byref(o), b.ptr, x.ptr, C.byref(r)), self.ctx)
+        return r.as_dict()
+
+    def gmres(self, b, x, rtol=1e-8, atol=0.0, max_it=10000, k=None):
+        o = SolveOptions(rtol, atol, max_it)
+        r = SolveResult()
+        _check(lib().fx_gmres(self.h, C.byref(k or coeffs()), C.byref(o), b.ptr, x.ptr, C.byref(r)), self.ctx)
+        return r.as_dict()
+
+    def krylov(self, b, x, rtol=1e-8, atol=0.0, max_it=10000, k=None):
+        o = SolveOptions(rtol, atol, max_it)
+        r = SolveResult()
+        _check(lib().fx_krylov(self.h, C.byref(k or coeffs()), C.byref(o), b.ptr, x.ptr, C.byref(r)), self.ctx)
         return r.as_dict()
 
     def cg_mixed(self, b, x, rtol=1e-8, atol=0.0, max_it=100000, inner_rtol=1e-3, k=None):

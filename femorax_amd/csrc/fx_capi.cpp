@@ -194,7 +194,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn};
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -400,6 +400,9 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         if (q >= 0) b->eq_dir[q] = static_cast<int8_t>(k % 3);
       }
     if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(12) * std::max(1, d->neq), by));
+    if (flags & FX_BATCH_GMRES)
+      ok(dev_zeros(ctx, b->gm,
+                   static_cast<size_t>(2 * fx::kGmresRestart + 3) * std::max(1, d->neq) + fx::kGmresRestart + 1, by));
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);
@@ -804,4 +807,26 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
     return fail(b->ctx, FX_EINVAL, "fx_newmark: bad descriptor");
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
   return fx::newmark_run(b, *d, u, v, a, du, *result);
+}
+
+namespace {
+int krylov_entry(fx_batch* b, int method, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs,
+                 double* x, fx_solve_result* res, const char* name) {
+  if (!b || !k || !opt || !rhs || !x || !res) return fail(b ? b->ctx : nullptr, FX_EINVAL, std::string(name) + ": null argument");
+  if (method == 1 && !b->gm) return fail(b->ctx, FX_EINVAL, std::string(name) + ": the batch was created without FX_BATCH_GMRES");
+  FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+  const int rc = fx::krylov64(b, method, *k, *opt, rhs, x, *res);
+  if (rc == FX_OK) FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return rc;
+}
+}  // namespace
+
+int fx_gmres(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+             fx_solve_result* res) {
+  return krylov_entry(b, 1, k, opt, rhs, x, res, "fx_gmres");
+}
+
+int fx_krylov(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+              fx_solve_result* res) {
+  return krylov_entry(b, 2, k, opt, rhs, x, res, "fx_krylov");
 }

@@ -127,6 +127,7 @@ struct DevParam {  // per parameter set, device
 namespace fx {
 constexpr int kMaxPointLoads = 32;
 constexpr int kMaxPointMaterials = 8;
+constexpr int kGmresRestart = 30;  // GMRES(30) (gmres.hpp default restart)
 struct PimyParams;
 // PressureIndependMultiYield (fx_pimy_host.cpp, fx_pimy.hip)
 int pimy_setup(const fx_pimy_material& m, PimyParams& p, std::string& err);
@@ -203,6 +204,8 @@ struct fx_batch {
   // equation (reference numbering, host)
   double* dyn = nullptr;
   std::vector<int8_t> eq_dir;
+  // GMRES workspace (FX_BATCH_GMRES): V [(m + 1) n], Z [m n], coefficients [m + 1], CG start [n]
+  double* gm = nullptr;
   // solver work (allocated at create: compute allocates nothing)
   double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats

@@ -60,6 +60,15 @@ int copy64(fx_ctx* ctx, int64_t n, const double* a, double* b);
 int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& r);
+int jacobi_inverse_public(fx_batch* b, const fx_coeffs& k, double* d, double* inv);
+// GMRES(30) + Jacobi, and CG with the GMRES fallback, in the solver numbering (fx_gmres.hip);
+// the C-ABI wrapper (method 1 GMRES, 2 CG then GMRES).
+int gmres64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
+                   fx_solve_result& r);
+int krylov64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
+                    fx_solve_result& r);
+int krylov64(fx_batch* b, int method, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+             fx_solve_result& r);
 // CG + Jacobi in the solver numbering; uses the batch's work vectors 0-5.
 int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
                 fx_solve_result& r);

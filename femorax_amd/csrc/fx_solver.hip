@@ -356,6 +356,10 @@ int jacobi_inverse(fx_batch* b, const fx_coeffs& k, double* d, void* inv, bool f
 
 }  // namespace
 
+int jacobi_inverse_public(fx_batch* b, const fx_coeffs& k, double* d, double* inv) {
+  return jacobi_inverse(b, k, d, inv, false);
+}
+
 // CG + Jacobi on vectors already in the solver numbering (no permutation, no true residual).
 int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
                 fx_solve_result& res) {
@@ -401,6 +405,24 @@ int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const 
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& res) {
   return cg64_once(b, k, o, rhs, x, res);
+}
+
+// GMRES (method 1) or CG with the GMRES fallback (method 2) at the C ABI: permutations around
+// the solver-numbering solve, the true residual reported.
+int krylov64(fx_batch* b, int method, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
+             fx_solve_result& res) {
+  fx_ctx* ctx = b->ctx;
+  const int64_t n = b->neq;
+  double *r = b->work, *ap = r + 3 * n, *bi = r + 6 * n, *xi = r + 7 * n;
+  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
+  if (int rc = to_internal<double>(b, x, xi)) return rc;
+  if (int rc = method == 1 ? gmres64_solver(b, k, o, bi, xi, res) : krylov64_solver(b, k, o, bi, xi, res)) return rc;
+  if (int rc = operator_apply64(b, k, xi, ap)) return rc;
+  hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, ctx->stream, n, bi, ap, r);
+  double rr = 0;
+  if (int rc = dot64(ctx, n, r, r, &rr)) return rc;
+  res.true_residual = std::sqrt(rr);
+  return to_external<double>(b, xi, x);
 }
 
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,

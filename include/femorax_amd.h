@@ -137,6 +137,8 @@ typedef struct fx_batch_desc {
 #define FX_BATCH_GENERAL 2u  /* do not use the structured-lattice fast path (testing) */
 #define FX_BATCH_DYNAMIC 32u  /* allocate the time-integration workspace (12 * neq doubles) that
                                  fx_newmark needs (PIMY batches always have it) */
+#define FX_BATCH_GMRES 64u    /* allocate the GMRES(30) workspace ((2 * 30 + 3) * neq doubles):
+                                 fx_gmres, fx_krylov, and fx_newmark's CG -> GMRES fallback */
 #define FX_BATCH_XCD_SLABS 8u /* lattice path: tiles placed in 8 slabs, one per XCD (default: lattice order,
                                 which measured faster on MI355X; same bits either way) */
 #define FX_BATCH_NO_TUNE 16u /* skip the kernel-variant autotune at creation (variant 0) */
@@ -199,6 +201,13 @@ int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const do
 /* FP32 inner CG (FP32 operator, FP32 vectors, FP64 dot products) inside FP64 iterative
  * refinement (FP64 residual b - A x with the FP64 operator, FP64 update); same stop rule on the
  * FP64 residual. inner_rtol: each inner solve's relative tolerance. */
+/* FGMRES(30), CGS2, right Jacobi (Gmres::solve, gmres.cpp:32-123). Needs FX_BATCH_GMRES. */
+int fx_gmres(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+             fx_solve_result* res);
+/* KrylovSolver::solve (krylov.cpp:25-42): CG; if it meets p^T A p <= 0, GMRES from the same
+ * initial guess (iterations add up, indefinite set). Without FX_BATCH_GMRES: CG only. */
+int fx_krylov(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
+              fx_solve_result* res);
 int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol,
                 const double* rhs, double* x, fx_solve_result* res);
 

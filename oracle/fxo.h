@@ -211,6 +211,9 @@ typedef struct fxo_newmark_stats {
 /* PCG + Jacobi on a system (cg.cpp:24-72) */
 int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
                   int32_t max_it, int lanes, fxo_solve_result* res);
+/* FGMRES(m), CGS2, right Jacobi (gmres.cpp:32-123, block_ops.cpp:59-90) on a system */
+int fxo_gmres_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
+                     int32_t max_it, int32_t m, int lanes, fxo_solve_result* res);
 /* nsteps Newmark steps with Newton (implicit_stage.cpp); ground_dir: the influence vector of the
  * excitation's direction (1 at its equations), or NULL. Returns 0, or 1 at the first failed step
  * (its state restored). du: the Krylov initial guess in, the last increment out. */

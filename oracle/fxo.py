@@ -141,6 +141,9 @@ def lib():
         L.fxo_pimy_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p, f64p, f64p, C.c_void_p, C.c_int]
         L.fxo_apply_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, f64p, C.c_int]
         L.fxo_diagonal_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_int]
+        L.fxo_gmres_system.restype = C.c_int
+        L.fxo_gmres_system.argtypes = [C.POINTER(System), C.POINTER(Coeffs), f64p, f64p, C.c_double, C.c_double,
+                                       C.c_int32, C.c_int32, C.c_int, C.POINTER(SolveResult)]
         L.fxo_newmark.restype = C.c_int
         L.fxo_newmark.argtypes = [C.POINTER(System), C.POINTER(NewmarkOpts), C.c_void_p, f64p, f64p, f64p, f64p,
                                   C.POINTER(NewmarkStats)]
@@ -364,6 +367,21 @@ class OracleMesh:
             pimy_state[0][...] = pimy_state[1]
             pimy_state[1][...] = tmp
         return rc, {f: getattr(st, f) for f, _ in NewmarkStats._fields_}
+
+    def gmres(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, restart=30, lanes=64, k=None,
+              nthreads=0):
+        """FGMRES(restart) with Jacobi on the elastic operator (gmres.cpp:32-123)."""
+        s = System()
+        s.c = C.cast(C.pointer(self.c), C.c_void_p)
+        s.elastic = C.cast(params, C.c_void_p)
+        s.nthreads = nthreads
+        x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)
+        r = SolveResult()
+        rc = lib().fxo_gmres_system(C.byref(s), C.byref(k or coeffs()), np.ascontiguousarray(b, np.float64), x, rtol,
+                                    atol, max_it, restart, lanes, C.byref(r))
+        if rc != 0:
+            raise RuntimeError("Jacobi: zero diagonal entries")
+        return x, {f: getattr(r, f) for f, _ in SolveResult._fields_}
 
     def cg(self, params, b, x0=None, rtol=1e-8, atol=0.0, max_it=10000, lanes=1, k=None, nthreads=0):
         x = np.zeros(self.neq) if x0 is None else np.array(x0, dtype=np.float64)

@@ -94,6 +94,128 @@ int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x
   return status;
 }
 
+/* FGMRES(m) with CGS2 and right Jacobi preconditioning (gmres.cpp:32-123; BlockOps::dots is a
+ * VectorOps dot per column, combine adds sum_c coef_c V(:, c) per row, block_ops.cpp:59-90) */
+int fxo_gmres_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
+                     int32_t max_it, int32_t m, int lanes, fxo_solve_result* res) {
+  const int64_t n = s->c->neq;
+  memset(res, 0, sizeof(*res));
+  double* V = (double*)malloc((size_t)(n * (m + 1) + 1) * sizeof(double));
+  double* Z = (double*)malloc((size_t)(n * m + 1) * sizeof(double));
+  double* w = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* r = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* inv = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  double* H = (double*)calloc((size_t)((m + 1) * m), sizeof(double));
+  double *cs = (double*)calloc((size_t)m + 1, sizeof(double)), *sn = (double*)calloc((size_t)m + 1, sizeof(double));
+  double *g = (double*)calloc((size_t)m + 2, sizeof(double)), *h = (double*)calloc((size_t)m + 2, sizeof(double));
+  double *h2 = (double*)calloc((size_t)m + 2, sizeof(double)), *y = (double*)calloc((size_t)m + 2, sizeof(double));
+#define HIJ(i, j) H[(size_t)(j) * (m + 1) + (i)]
+  int status = 0;
+  sys_diagonal(s, k, inv);
+  for (int64_t i = 0; i < n; ++i) {
+    if (inv[i] == 0.0) status = -1;
+    inv[i] = 1.0 / inv[i];
+  }
+  res->rhs_norm = sqrt(fxo_dot(n, b, b, lanes));
+  const double tol = fmax(rtol * res->rhs_norm, atol);
+  int first = 1;
+  while (status == 0) {
+    sys_apply(s, k, x, w);
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i];
+    for (int64_t i = 0; i < n; ++i) r[i] += -1.0 * w[i];
+    const double beta = sqrt(fxo_dot(n, r, r, lanes));
+    if (first) res->initial_residual = beta;
+    first = 0;
+    res->final_residual = beta;
+    if (beta <= tol) {
+      res->converged = 1;
+      break;
+    }
+    if (res->iterations >= max_it) break;
+    for (int64_t i = 0; i < n; ++i) V[i] = 0.0;
+    for (int64_t i = 0; i < n; ++i) V[i] += (1.0 / beta) * r[i];
+    for (int i = 0; i <= m; ++i) g[i] = 0.0;
+    g[0] = beta;
+    int j = 0;
+    for (; j < m && res->iterations < max_it; ++j) {
+      double *vj = V + (size_t)j * n, *zj = Z + (size_t)j * n;
+      for (int64_t i = 0; i < n; ++i) zj[i] = inv[i] * vj[i];
+      sys_apply(s, k, zj, w);
+      for (int pass = 0; pass < 2; ++pass) {
+        double* hh = pass == 0 ? h : h2;
+        for (int c = 0; c <= j; ++c) hh[c] = fxo_dot(n, V + (size_t)c * n, w, lanes);
+        for (int64_t i = 0; i < n; ++i) {
+          double acc = 0.0;
+          for (int c = 0; c <= j; ++c) acc += (-1.0 * hh[c]) * V[(size_t)c * n + i];
+          w[i] += acc;
+        }
+      }
+      for (int i = 0; i <= j; ++i) HIJ(i, j) = h[i] + h2[i];
+      const double hn = sqrt(fxo_dot(n, w, w, lanes));
+      HIJ(j + 1, j) = hn;
+      if (hn > 0.0) {
+        double* vn = V + (size_t)(j + 1) * n;
+        for (int64_t i = 0; i < n; ++i) vn[i] = 0.0;
+        for (int64_t i = 0; i < n; ++i) vn[i] += (1.0 / hn) * w[i];
+      }
+      for (int i = 0; i < j; ++i) {
+        const double a = HIJ(i, j), c = HIJ(i + 1, j);
+        HIJ(i, j) = cs[i] * a + sn[i] * c;
+        HIJ(i + 1, j) = -sn[i] * a + cs[i] * c;
+      }
+      const double a = HIJ(j, j), c = HIJ(j + 1, j);
+      const double d = hypot(a, c);
+      cs[j] = d > 0.0 ? a / d : 1.0;
+      sn[j] = d > 0.0 ? c / d : 0.0;
+      HIJ(j, j) = d;
+      HIJ(j + 1, j) = 0.0;
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = cs[j] * g[j];
+      ++res->iterations;
+      res->final_residual = fabs(g[j + 1]);
+      if (res->final_residual <= tol || hn == 0.0) {
+        ++j;
+        break;
+      }
+    }
+    for (int i = j - 1; i >= 0; --i) {
+      double sum = g[i];
+      for (int kk = i + 1; kk < j; ++kk) sum -= HIJ(i, kk) * y[kk];
+      y[i] = sum / HIJ(i, i);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+      double acc = 0.0;
+      for (int c = 0; c < j; ++c) acc += (1.0 * y[c]) * Z[(size_t)c * n + i];
+      x[i] += acc;
+    }
+    if (res->final_residual <= tol) {
+      sys_apply(s, k, x, w);
+      for (int64_t i = 0; i < n; ++i) r[i] = b[i];
+      for (int64_t i = 0; i < n; ++i) r[i] += -1.0 * w[i];
+      res->final_residual = sqrt(fxo_dot(n, r, r, lanes));
+      if (res->final_residual <= tol) {
+        res->converged = 1;
+        break;
+      }
+    }
+    if (res->iterations >= max_it) break;
+  }
+#undef HIJ
+  free(V);
+  free(Z);
+  free(w);
+  free(r);
+  free(inv);
+  free(H);
+  free(cs);
+  free(sn);
+  free(g);
+  free(h);
+  free(h2);
+  free(y);
+  return status;
+}
+
 int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
                 double* du, fxo_newmark_stats* st) {
   const int64_t n = s->c->neq;
