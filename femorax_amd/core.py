@@ -6,6 +6,7 @@ The names follow the reference's exec/solvers interfaces: Mesh (compile::Compile
 the bindings raises, and every compute call goes to the device.
 """
 import ctypes as C
+import weakref
 import os
 
 import numpy as np
@@ -217,9 +218,15 @@ class Context:
         h = C.c_void_p()
         _check(lib().fx_ctx_create(device, budget, C.byref(h)))
         self.h = h
+        self._children = weakref.WeakSet()  # batches and buffers: released before the context
+
+    def adopt(self, child):
+        self._children.add(child)
 
     def close(self):
         if self.h:
+            for child in list(self._children):
+                child.release()
             lib().fx_ctx_destroy(self.h)
             self.h = None
 
@@ -246,6 +253,7 @@ class DeviceArray:
         p = C.c_void_p()
         _check(lib().fx_alloc(ctx.h, max(1, self.n) * self.dtype.itemsize, C.byref(p)), ctx)
         self.ptr = p
+        ctx.adopt(self)
 
     @classmethod
     def from_host(cls, ctx, a, dtype=None):
@@ -268,9 +276,12 @@ class DeviceArray:
         _check(lib().fx_memset0(self.ctx.h, self.ptr, self.n * self.dtype.itemsize), self.ctx)
 
     def free(self):
-        if self.ptr:
+        if self.ptr and self.ctx.h:
             lib().fx_free(self.ctx.h, self.ptr)
-            self.ptr = None
+        self.ptr = None
+
+    def release(self):
+        self.free()
 
     def __del__(self):
         try:
@@ -412,6 +423,7 @@ class Batch:
             (FX_BATCH_GMRES if gmres else 0)
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
+        ctx.adopt(self)
 
     @classmethod
     def pimy(cls, ctx, mesh, materials):
@@ -437,6 +449,7 @@ class Batch:
         h = C.c_void_p()
         _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), 0, C.byref(h)), ctx)
         self.h = h
+        ctx.adopt(self)
         return self
 
     def newmark(self, u, v, a, du, dt, nsteps, gamma=0.5, beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12,
@@ -465,9 +478,12 @@ class Batch:
         return d.get().reshape(PIMY_NSTATE, self.npts).T
 
     def close(self):
-        if self.h:
+        if self.h and self.ctx.h:
             lib().fx_batch_destroy(self.h)
-            self.h = None
+        self.h = None
+
+    def release(self):
+        self.close()
 
     def __del__(self):
         try:

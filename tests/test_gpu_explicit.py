@@ -32,6 +32,7 @@ def damped_distorted():
     m = meshes.distorted_box(4)
     m["params"] = [dict(p, rayleigh=(0.6, 5.8e-4, 1e-4, 2e-5)) for p in m["params"]]
     m["region"] = m["group"].copy()
+    m["rayleigh"] = [(0.6, 5.8e-4, 1e-4, 2e-5), (0.9, 3e-4, 0.0, 2e-5)][:len(m["params"])]
     return m
 
 

@@ -29,6 +29,19 @@ def rel(got, want):
     return np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-300)
 
 
+# fields of the history grouped as the material uses them: stress tensor, strain tensor, the four
+# scalars, surface sizes, moduli, centres; differences are measured against each group's magnitude
+HISTORY_GROUPS = [(0, 13), (13, 26), (26, 27), (27, 28), (28, 29), (29, 30), (30, 50), (50, 70), (70, 190)]
+
+
+def history_error(got, want):
+    worst = 0.0
+    for a, b in HISTORY_GROUPS:
+        scale = max(np.max(np.abs(want[:, a:b])), 1e-300)
+        worst = max(worst, np.max(np.abs(got[:, a:b] - want[:, a:b])) / scale)
+    return worst
+
+
 def soil(seed, nlayer):
     rng = S.Lcg(seed)
     out = []
@@ -87,8 +100,7 @@ def test_force_tangents_operator_and_commits(ctx, name):
         batch.store_tangents(du, df)
         assert rel(df.get(), f_o) <= 1e-10
         T_dev = batch.pimy_state(False)
-        scale = np.maximum(np.max(np.abs(To), axis=0), 1e-300)
-        assert np.max(np.abs(T_dev - To) / scale) <= 1e-10, step
+        assert history_error(T_dev, To) <= 1e-10, step
         for kk in ((1.0, 0.0, 0.0, False), (1.0, 0.0, 400.0, False), (1.0, 0.0, 0.0, True)):
             batch.apply(dx, dy, core.coeffs(*kk))
             want = om.apply_gp(pe, D_o, x, fxo.coeffs(*kk))
@@ -158,8 +170,7 @@ def test_explicit_loop_commits_the_history(ctx):
     batch.explicit_cd(dt, nsteps, du, dum, dv, da, load_eq=load_eq, load_hist=hist.ravel())
     assert rel(du.get(), u) <= 1e-9
     assert rel(dv.get(), v) <= 1e-8
-    scale = np.maximum(np.max(np.abs(Co), axis=0), 1e-300)
-    assert np.max(np.abs(batch.pimy_state(True) - Co) / scale) <= 1e-9
+    assert history_error(batch.pimy_state(True), Co) <= 1e-9
 
 
 def pe_as_elastic(per_set):

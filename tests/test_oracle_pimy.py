@@ -15,17 +15,11 @@ from tests.golden_data import files
 REL = 1e-14
 
 
-class PimyParams(C.Structure):
-    _fields_ = [("load_stage", C.c_int32), ("nsurf", C.c_int32)] + [
-        (n, C.c_double) for n in ("rho", "G", "K", "phi", "gamma_peak", "pref", "cohesion", "pdc", "residual_press")
-    ] + [("size", C.c_double * 21), ("modulus", C.c_double * 21)]
+PimyParams = fxo.PimyParams  # one ctypes class for the library's signatures
 
 
 def lib():
     L = fxo.lib()
-    L.fxo_pimy_setup.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(PimyParams)]
-    L.fxo_pimy_nstate.restype = C.c_int
-    L.fxo_pimy_initial_state.argtypes = [C.POINTER(PimyParams), C.c_void_p]
     L.fxo_pimy_update.argtypes = [C.POINTER(PimyParams)] + [C.c_void_p] * 5
     return L
 
