@@ -29,6 +29,21 @@ def algorithmic_bytes(neq, nel, nnode, w):
     return 2 * w * neq + 4 * 8 * nel + 4 * nel + 3 * w * nnode + 12 * nnode
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "hex8_operator_c2_128_hbm_traffic.md")
+
+
+def measured_traffic(key):
+    """HBM bytes per apply of the C2 128^3 operator from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (calibrated against a plain copy), or None."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.startswith(key + " = "):
+                return int(line.split("=")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_threads():
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
@@ -129,7 +144,10 @@ def main():
                    "elements": mesh.nelem, "dof": mesh.neq, "nodes": mesh.nnode, "lattice_path": info["lattice"],
                    "tiles": info["tiles"], "general_blocks": info["nblocks"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs64, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs64 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(gbs64 / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic("traffic_bytes_per_apply_f64") if args.n == 128 else None,
+                     "traffic_source": "profiles/hex8_operator_c2_128_hbm_traffic.md (rocprofv3 FETCH_SIZE, "
+                                       "WRITE_SIZE per apply, copy-calibrated)",
                      "kernel": "hex8_box_kernel<double, 0, *> + hex8_shared_sum_kernel<double> (one apply)",
                      "algorithmic_bytes_per_apply": B64, "kernel_ms_per_apply": round(ms64, 5)},
         "dropin_apply": {"ms": round(ms_dropin, 4), "gdof_s": round(mesh.neq / (ms_dropin * 1e-3) / 1e9, 3),
