@@ -41,6 +41,7 @@ template <typename T>
 struct BoxArgs {
   const TileHeader* tiles;
   const uint32_t* words;
+  const int32_t* slots;  // per tile node: partial-sum slot of a shared node
   const uint16_t* cell_param;
   const DevParam* params;
   fx_coeffs k;
@@ -193,6 +194,7 @@ constexpr int kPlane = (kTileX + 1) * (kTileY + 1);  // nodes of one z-plane of 
 template <typename T>
 struct TileFetch {
   uint32_t w[2];
+  int32_t s[2];
   T v[2][3];
   int pidx;
 };
@@ -202,10 +204,12 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, const TileHeader
   const int nx = h.nx, ny = h.ny, nz = h.nz;
   const int nnodes = (nx + 1) * (ny + 1) * (nz + 1);
   const uint32_t* words = a.words + h.word_offset;
+  const int32_t* slots = a.slots + h.word_offset;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int ln = t + r * kTileCells;
     f.w[r] = ln < nnodes ? words[ln] : 0u;
+    f.s[r] = ln < nnodes ? slots[ln] : -1;
   }
   const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY);
   f.pidx = (cx < nx && cy < ny && cz < nz) ? a.cell_param[h.cell_offset + cx + nx * (cy + ny * cz)] : 0xffff;
@@ -226,7 +230,7 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, const TileHeader
 // tile's words and x; the cell's results are summed per node and committed.
 template <typename T, int MODE>
 __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const TileHeader& h, int pidx, T* lds,
-                                         const uint32_t* wsh, int t) {
+                                         const uint32_t* wsh, const int32_t* ssh, int t) {
   constexpr bool DIAG = MODE == 2;
   const int nx = h.nx, ny = h.ny, nz = h.nz;
   const int NX = nx + 1, NY = ny + 1;
@@ -298,13 +302,13 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
   __syncthreads();
   // commit: plane z = cz (bottom face + the plane below's top face), and for the last plane of
   // cells also plane z = nz (its own top face)
-  T* part = a.partial + static_cast<size_t>(tile) * (kTileNodes * 3);
   auto commit = [&](int px, int py, int pz, const T (&val)[3]) {
     const int ln = px + NX * (py + NY * pz);
     const uint32_t w = wsh[ln];
-    if (w & 0x80000000u) {
+    if (w & 0x80000000u) {  // a shared node: this tile's partial-sum slot
+      T* part = a.partial + static_cast<size_t>(ssh[ln]) * 3;
 #pragma unroll
-      for (int d = 0; d < 3; ++d) part[ln * 3 + d] = val[d];
+      for (int d = 0; d < 3; ++d) part[d] = val[d];
       return;
     }
     const uint32_t mask = (w >> 28) & 7u;
@@ -342,6 +346,7 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
   constexpr bool DIAG = MODE == 2;
   __shared__ T lds[3 * kTileNodes];  // x at the tile's nodes, then the top-face partial sums
   __shared__ uint32_t wsh[kTileNodes];
+  __shared__ int32_t ssh[kTileNodes];
   const int t = threadIdx.x;
   if (a.done && *a.done) return;
   int tile = blockIdx.x;
@@ -355,6 +360,7 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
       const int ln = t + r * kTileCells;
       if (ln < nnodes) {
         wsh[ln] = f.w[r];
+        ssh[ln] = f.s[r];
 #pragma unroll
         for (int d = 0; d < 3; ++d) lds[d * kTileNodes + ln] = f.v[r][d];
       }
@@ -367,7 +373,7 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
       hn = a.tiles[next];
       fetch_tile<T, DIAG>(a, hn, t, fn);
     }
-    box_tile<T, MODE>(a, tile, h, f.pidx, lds, wsh, t);
+    box_tile<T, MODE>(a, tile, h, f.pidx, lds, wsh, ssh, t);
     if (next >= a.ntiles) break;
     __syncthreads();  // the LDS takes the next tile
     tile = next;
@@ -377,11 +383,13 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
 }
 
 // y at the nodes several tiles touch: their tiles' partial sums added in colour order. One
-// thread per shared node; the shared nodes' equations are consecutive in the internal numbering.
+// thread per shared node; a group of nodes with the same tiles keeps one block of slots per
+// tile, so the threads of a wave read each block contiguously. The shared nodes' equations are
+// consecutive in the internal numbering, in the same order.
 template <typename T>
 __global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __restrict__ word,
                                                               const int32_t* __restrict__ start,
-                                                              const int32_t* __restrict__ list,
+                                                              const uint32_t* __restrict__ meta,
                                                               const T* __restrict__ partial, T* __restrict__ y,
                                                               int n, int accumulate, const int32_t* done) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -390,10 +398,12 @@ __global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __
   const uint32_t w = word[i];
   const uint32_t mask = (w >> 28) & 7u;
   const int base = static_cast<int>(w & 0x0fffffffu);
-  const int s0 = start[i], s1 = start[i + 1];
+  const int s0 = start[i];
+  const uint32_t mt = meta[i];
+  const int stride = static_cast<int>(mt & 0xffffffu), k = static_cast<int>(mt >> 24);
   T acc[3];
   {
-    const T* p = partial + list[s0];
+    const T* p = partial + static_cast<size_t>(s0) * 3;
     int j = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -402,8 +412,8 @@ __global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __
       j += (mask >> d) & 1u;
     }
   }
-  for (int s = s0 + 1; s < s1; ++s) {
-    const T* p = partial + list[s];
+  for (int t = 1; t < k; ++t) {
+    const T* p = partial + (static_cast<size_t>(s0) + static_cast<size_t>(t) * stride) * 3;
 #pragma unroll
     for (int d = 0; d < 3; ++d) acc[d] += p[d];
   }
@@ -461,6 +471,7 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   BoxArgs<T> a{};
   a.tiles = b->tiles;
   a.words = b->words;
+  a.slots = b->tile_slots;
   a.cell_param = b->cell_param;
   a.params = b->params;
   a.k = k;
@@ -492,7 +503,7 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kTileCells), 0, b->ctx->stream, a);
   if (b->nshared > 0)
     hipLaunchKernelGGL(hex8_shared_sum_kernel<T>, dim3((b->nshared + 255) / 256), dim3(256), 0, b->ctx->stream,
-                       b->shared_word, b->shared_start, b->shared_list, a.partial, y, b->nshared, accumulate, done);
+                       b->shared_word, b->shared_start, b->shared_meta, a.partial, y, b->nshared, accumulate, done);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }

@@ -192,8 +192,8 @@ void fx_batch_destroy(fx_batch* b) {
   void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
-                  b->tiles, b->words, b->cell_param, b->orphans, b->shared_word, b->shared_start,
-                  b->shared_list, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
+                  b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
+                  b->shared_meta, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -307,12 +307,13 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     b->nshared = static_cast<int32_t>(lat.shared_word.size());
     ok(dev_upload(ctx, b->tiles, lat.tiles.data(), lat.tiles.size(), by)) &&
         ok(dev_upload(ctx, b->words, lat.words.data(), lat.words.size(), by)) &&
+        ok(dev_upload(ctx, b->tile_slots, lat.slots.data(), lat.slots.size(), by)) &&
         ok(dev_upload(ctx, b->cell_param, lat.cell_param.data(), lat.cell_param.size(), by)) &&
         ok(lat.shared_word.empty() ? FX_OK : dev_upload(ctx, b->shared_word, lat.shared_word.data(), lat.shared_word.size(), by)) &&
-        ok(dev_upload(ctx, b->shared_start, lat.shared_start.data(), lat.shared_start.size(), by)) &&
-        ok(lat.shared_list.empty() ? FX_OK : dev_upload(ctx, b->shared_list, lat.shared_list.data(), lat.shared_list.size(), by)) &&
-        ok(dev_zeros(ctx, b->partial64, static_cast<size_t>(b->ntiles) * fx::kTileNodes * 3, by)) &&
-        ok(b->has_f32 ? dev_zeros(ctx, b->partial32, static_cast<size_t>(b->ntiles) * fx::kTileNodes * 3, by) : FX_OK) &&
+        ok(lat.shared_start.empty() ? FX_OK : dev_upload(ctx, b->shared_start, lat.shared_start.data(), lat.shared_start.size(), by)) &&
+        ok(lat.shared_meta.empty() ? FX_OK : dev_upload(ctx, b->shared_meta, lat.shared_meta.data(), lat.shared_meta.size(), by)) &&
+        ok(dev_zeros(ctx, b->partial64, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by)) &&
+        ok(b->has_f32 ? dev_zeros(ctx, b->partial32, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by) : FX_OK) &&
         ok(dev_upload(ctx, b->perm, lat.perm.data(), lat.perm.size(), by));
   } else {
     std::vector<float> c32(d->coords, d->coords + static_cast<size_t>(3) * N);

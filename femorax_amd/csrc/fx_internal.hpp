@@ -67,13 +67,17 @@ struct Lattice {
   double hx = 0, hy = 0, hz = 0;
   std::vector<TileHeader> tiles;      // in launch order (XCD-aware slabs or lattice order)
   std::vector<uint32_t> words;        // per tile node: first equation (28 bits) | free-DOF mask << 28 | shared << 31
+  std::vector<int32_t> slots;         // per tile node: the tile's partial-sum slot of a shared node, else -1
   std::vector<uint16_t> cell_param;   // per tile cell: parameter set, 0xffff = no element
   std::vector<int32_t> cell_elem;     // per tile cell: batch element, -1 = none (host)
   std::vector<int32_t> perm;          // internal equation -> reference equation [neq]
-  // nodes touched by several tiles: word (first equation | mask) and the offsets of their tiles'
-  // partial sums (tile * kTileNodes + tile node) * 3, in tile parity-colour order
+  // nodes touched by several tiles, in group order: word (first equation | mask), first slot,
+  // and (slot stride | tile count << 24); tile j's partial sum sits at slot first + j * stride,
+  // tiles in parity-colour order
   std::vector<uint32_t> shared_word;
-  std::vector<int32_t> shared_start, shared_list;
+  std::vector<int32_t> shared_start;
+  std::vector<uint32_t> shared_meta;
+  int64_t nslot = 0;
 };
 
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
@@ -165,13 +169,14 @@ struct fx_batch {
   double hx = 0, hy = 0, hz = 0;
   fx::TileHeader* tiles = nullptr;
   uint32_t* words = nullptr;
+  int32_t* tile_slots = nullptr;
   uint16_t* cell_param = nullptr;
   int32_t ntiles = 0;
   int32_t nshared = 0;
   uint32_t* shared_word = nullptr;
   int32_t* shared_start = nullptr;
-  int32_t* shared_list = nullptr;
-  double* partial64 = nullptr;        // [ntiles * kTileNodes * 3] per-tile sums at shared nodes
+  uint32_t* shared_meta = nullptr;
+  double* partial64 = nullptr;        // [nslot * 3] per-tile sums at shared nodes
   float* partial32 = nullptr;
   int box_variant = 0;                // register-allocation variant of the mass-free kernel (tuned)
   // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference

@@ -10,6 +10,7 @@
 // a tile are fine, ties that merge a node's DOFs with another node's) uses the general blocks.
 #include <algorithm>
 #include <cmath>
+#include <map>
 #include <unordered_map>
 
 #include "fx_internal.hpp"
@@ -154,14 +155,54 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     L.tiles.push_back(h);
     tile_colour.push_back(static_cast<int8_t>((ti & 1) | ((tj & 1) << 1) | ((tk & 1) << 2)));
   }
-  // how many tiles touch each node
+  // how many tiles touch each node, and which (in parity-colour order: a node's tiles all differ
+  // in colour)
   const int32_t nt = static_cast<int32_t>(L.tiles.size());
   std::vector<int32_t> touch_count(nnode, 0);
   for (int32_t n : tile_nodes)
     if (n >= 0) ++touch_count[n];
+  std::vector<int32_t> touch_start(static_cast<size_t>(nnode) + 1, 0);
+  for (int32_t n = 0; n < nnode; ++n) touch_start[n + 1] = touch_start[n] + (touch_count[n] > 1 ? touch_count[n] : 0);
+  std::vector<int32_t> touch_tile(touch_start[nnode]), touch_ln(touch_start[nnode]);
+  {
+    std::vector<int32_t> fill(touch_start.begin(), touch_start.end() - 1);
+    for (int c = 0; c < 8; ++c)
+      for (int32_t t = 0; t < nt; ++t) {
+        if (tile_colour[t] != c) continue;
+        const TileHeader& h = L.tiles[t];
+        const int32_t nw = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
+        for (int32_t i = 0; i < nw; ++i) {
+          const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
+          if (n < 0 || touch_count[n] < 2) continue;
+          touch_tile[fill[n]] = t;
+          touch_ln[fill[n]++] = i;
+        }
+      }
+  }
+  // shared nodes grouped by their set of tiles (a face, an edge or a corner between tiles), groups
+  // and their nodes in first-touch order. A group of k tiles and m nodes keeps one block of m
+  // partial-sum slots per tile: slot (base + j m + i) holds tile j's sum at node i.
+  std::vector<int32_t> shared_nodes;
+  {
+    std::vector<char> seen(nnode, 0);
+    std::map<std::vector<int32_t>, int32_t> group_of;
+    std::vector<std::vector<int32_t>> groups;
+    for (int32_t n : tile_nodes) {
+      if (n < 0 || touch_count[n] < 2 || seen[n]) continue;
+      seen[n] = 1;
+      std::vector<int32_t> key(touch_tile.begin() + touch_start[n], touch_tile.begin() + touch_start[n + 1]);
+      auto it = group_of.find(key);
+      if (it == group_of.end()) {
+        it = group_of.emplace(std::move(key), static_cast<int32_t>(groups.size())).first;
+        groups.emplace_back();
+      }
+      groups[it->second].push_back(n);
+    }
+    for (const auto& g : groups) shared_nodes.insert(shared_nodes.end(), g.begin(), g.end());
+  }
   // internal numbering: first the nodes only one tile touches (tile by tile, lattice order),
-  // then the shared nodes (by the first tile that touches them), then untouched equations.
-  // A node's free DOFs are consecutive. perm: internal -> reference equation.
+  // then the shared nodes in group order, then untouched equations. A node's free DOFs are
+  // consecutive. perm: internal -> reference equation.
   std::vector<int32_t> internal_of(static_cast<size_t>(neq), -1);
   L.perm.assign(static_cast<size_t>(neq), -1);
   int32_t next = 0;
@@ -175,9 +216,9 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       L.perm[next++] = q;
     }
   };
-  for (int pass = 0; pass < 2; ++pass)
-    for (int32_t n : tile_nodes)
-      if (n >= 0 && !numbered[n] && ((touch_count[n] > 1) == (pass == 1))) number(n);
+  for (int32_t n : tile_nodes)
+    if (n >= 0 && !numbered[n] && touch_count[n] == 1) number(n);
+  for (int32_t n : shared_nodes) number(n);
   for (int32_t q = 0; q < neq; ++q)
     if (internal_of[q] < 0) {
       internal_of[q] = next;
@@ -196,7 +237,45 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     w = (mask << 28) | static_cast<uint32_t>(base < 0 ? 0 : base);
     return true;
   };
-  // tile node words: first equation | free-DOF mask | shared flag
+  // slots: group by group (nodes of a group are consecutive in shared_nodes)
+  std::vector<int32_t> slot_of_touch(touch_tile.size(), -1);
+  L.shared_start.clear();
+  L.shared_meta.clear();
+  L.shared_word.clear();
+  int64_t nslot = 0;
+  for (size_t g0 = 0; g0 < shared_nodes.size();) {
+    const int32_t first = shared_nodes[g0];
+    const int32_t k = touch_count[first];
+    size_t g1 = g0 + 1;
+    while (g1 < shared_nodes.size()) {
+      const int32_t n = shared_nodes[g1];
+      if (touch_count[n] != k ||
+          !std::equal(touch_tile.begin() + touch_start[n], touch_tile.begin() + touch_start[n + 1],
+                      touch_tile.begin() + touch_start[first]))
+        break;
+      ++g1;
+    }
+    const int32_t m = static_cast<int32_t>(g1 - g0);
+    for (int32_t i = 0; i < m; ++i) {
+      const int32_t n = shared_nodes[g0 + i];
+      for (int32_t j = 0; j < k; ++j) slot_of_touch[touch_start[n] + j] = static_cast<int32_t>(nslot + static_cast<int64_t>(j) * m + i);
+      uint32_t w = 0;
+      if (!word_of(n, w)) return false;
+      L.shared_word.push_back(w);
+      L.shared_start.push_back(static_cast<int32_t>(nslot + i));
+      L.shared_meta.push_back(static_cast<uint32_t>(m) | (static_cast<uint32_t>(k) << 24));
+    }
+    nslot += static_cast<int64_t>(k) * m;
+    if (m >= (1 << 24) || nslot >= (int64_t(1) << 28)) return false;
+    g0 = g1;
+  }
+  L.nslot = nslot;
+  // tile node words: first equation | free-DOF mask | shared flag; and for the shared nodes the
+  // tile's partial-sum slot
+  L.slots.assign(tile_nodes.size(), -1);
+  for (int32_t n = 0; n < nnode; ++n)
+    for (int32_t j = touch_start[n]; j < touch_start[n + 1]; ++j)
+      L.slots[static_cast<size_t>(L.tiles[touch_tile[j]].word_offset) + touch_ln[j]] = slot_of_touch[j];
   L.words.resize(tile_nodes.size());
   for (size_t i = 0; i < tile_nodes.size(); ++i) {
     const int32_t n = tile_nodes[i];
@@ -206,34 +285,6 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       if (touch_count[n] > 1) w |= 0x80000000u;
     }
     L.words[i] = w;
-  }
-  // shared nodes in internal order, each with its tiles' partial-sum offsets in colour order
-  std::vector<std::vector<int32_t>> partials(nnode);
-  for (int c = 0; c < 8; ++c)
-    for (int32_t t = 0; t < nt; ++t) {
-      if (tile_colour[t] != c) continue;
-      const TileHeader& h = L.tiles[t];
-      const int32_t nw = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
-      for (int32_t i = 0; i < nw; ++i) {
-        const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
-        if (n >= 0 && touch_count[n] > 1) partials[n].push_back((t * kTileNodes + i) * 3);
-      }
-    }
-  if (static_cast<int64_t>(nt) * kTileNodes * 3 >= (int64_t(1) << 31)) return false;
-  std::vector<int32_t> shared_nodes;
-  std::fill(numbered.begin(), numbered.end(), 0);
-  for (int32_t n : tile_nodes)
-    if (n >= 0 && touch_count[n] > 1 && !numbered[n]) {
-      numbered[n] = 1;
-      shared_nodes.push_back(n);
-    }
-  L.shared_start.assign(1, 0);
-  for (int32_t n : shared_nodes) {
-    uint32_t w = 0;
-    word_of(n, w);
-    L.shared_word.push_back(w);
-    L.shared_list.insert(L.shared_list.end(), partials[n].begin(), partials[n].end());
-    L.shared_start.push_back(static_cast<int32_t>(L.shared_list.size()));
   }
   return true;
 }
