@@ -76,6 +76,57 @@ def cpu_baseline(model, params, x, budget_s=20.0):
                       f"{per * 1e3:.1f} ms per apply"}
 
 
+def bench_c1(core, S, ctx, reps=30):
+    """C1: 16^3 box, E 30 MPa, 10 kPa top traction, CG + Jacobi to rtol 1e-8 from x0 = 0; each
+    solve is one blocking fx_cg call (the latency a caller sees)."""
+    model = S.config_c1()
+    mesh = core.Mesh(model)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model))
+    b = core.DeviceArray.from_host(ctx, mesh.to_equations(model["nodal_load"]))
+    x = core.DeviceArray(ctx, mesh.neq)
+    lat = []
+    for i in range(reps + 3):
+        x.zero()
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = batch.cg(b, x, rtol=1e-8)
+        lat.append(time.perf_counter() - t0)
+    lat = np.array(lat[3:])
+    return {"dof": mesh.neq, "iterations": r["iterations"], "solves_per_s": round(1.0 / lat.mean(), 2),
+            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "note": "one blocking fx_cg per solve, rtol 1e-8, x0 = 0"}
+
+
+def bench_c3(core, S, ctx, nsteps):
+    """C3: 200 x 200 x 100 bricks (h 5 m), 10 layers, lumped mass, Lysmer dashpots, Ricker 5 Hz
+    point force at the surface centre; CentralDifference at 0.5 of the stable step."""
+    t0 = time.perf_counter()
+    model = S.config_c3()
+    mesh = core.Mesh(model)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model))
+    print(f"[bench] C3 model {mesh.nelem} elements, {mesh.neq} DoF ready in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    n = mesh.neq
+    dt = 0.5 * batch.critical_dt(0)
+    dash = core.DeviceArray.from_host(ctx, mesh.to_equations(model["dashpots"]))
+    src = model["source_node"]
+    load_eq = np.array([mesh.node_eq[3 * mesh.node_rank[src] + 2]], np.int32)
+    u, um, v, a = (core.DeviceArray(ctx, n) for _ in range(4))
+    for d in (u, um, v, a):
+        d.zero()
+    warm = 20
+    hist = S.ricker(np.arange(warm + nsteps) * dt, f=5.0, amplitude=1e6)[:, None]
+    batch.explicit_cd(dt, warm, u, um, v, a, dash=dash, load_eq=load_eq, load_hist=hist[:warm].ravel())
+    ctx.sync()
+    t0 = time.perf_counter()
+    batch.explicit_cd(dt, nsteps, u, um, v, a, dash=dash, load_eq=load_eq, load_hist=hist[warm:].ravel())
+    ctx.sync()
+    sec = time.perf_counter() - t0
+    return {"elements": mesh.nelem, "dof": n, "dt": dt, "steps": nsteps, "ms_per_step": round(sec / nsteps * 1e3, 4),
+            "gdof_steps_per_s": round(n * nsteps / sec / 1e9, 3),
+            "note": "force on the lattice operator kernels + fused cd_solve update per step; dashpots, Ricker source"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +135,9 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time full plain-FP64 and mixed-precision solves")
+    ap.add_argument("--c1", action="store_true", help="also time C1 (16^3 static box) solves: solves/s, latency")
+    ap.add_argument("--c3", type=int, default=0, metavar="STEPS",
+                    help="also time STEPS explicit CentralDifference steps of C3 (200x200x100, h 5 m)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard by "
@@ -172,6 +226,10 @@ def main():
             res[name] = r
             print(f"[bench] {name}: {r}", file=sys.stderr, flush=True)
         out["c2_static_solve"] = res
+    if args.c1:
+        out["c1_static_solves"] = bench_c1(core, S, ctx)
+    if args.c3:
+        out["c3_explicit"] = bench_c3(core, S, ctx, args.c3)
     if not args.no_cpu:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(model, params, x)
