@@ -101,6 +101,28 @@ class PimyMaterial(C.Structure):
 
 
 PIMY_NSTATE = 190
+FX_EFULL = -6
+FX_COLUMN_MAX_ELEMS = 512
+FX_COLUMN_MIXED = 1
+FX_COLUMN_NO_GRAPH = 2
+
+
+class ColumnPoolDesc(C.Structure):
+    _fields_ = [("max_elements", C.c_int64), ("max_columns", C.c_int32), ("max_materials", C.c_int32),
+                ("max_record", C.c_int64), ("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double),
+                ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
+                ("cg_max_it", C.c_int32), ("flags", C.c_uint32), ("inner_rtol", C.c_double)]
+
+
+class ColumnDesc(C.Structure):
+    _fields_ = [("nelem", C.c_int32), ("hx", C.c_double), ("hy", C.c_double), ("hz", C.c_double),
+                ("nmat", C.c_int32), ("mats", C.c_void_p), ("mat_of", C.c_void_p), ("stage", C.c_int32),
+                ("nsteps", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p), ("record", C.c_int32)]
+
+
+class ColumnResult(C.Structure):
+    _fields_ = [("status", C.c_int32), ("steps", C.c_int32), ("newton_iterations", C.c_int32),
+                ("max_newton_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("last_test", C.c_double)]
 
 
 def pimy_material(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf=20, gredu=None, b=(0.0, 0.0, 0.0),
@@ -187,6 +209,14 @@ SIGNATURES = [
     ("fx_pimy_state", C.c_int, [vp, C.c_int, vp]),
     ("fx_pimy_init_state", C.c_int, [vp, C.c_int32, C.POINTER(PimyMaterial), i64, vp, vp]),
     ("fx_pimy_points", C.c_int, [vp, C.c_int32, C.POINTER(PimyMaterial), vp, i64, vp, vp, vp, vp, vp, vp]),
+    ("fx_column_pool_create", C.c_int, [vp, C.POINTER(ColumnPoolDesc), C.POINTER(vp)]),
+    ("fx_column_pool_destroy", None, [vp]),
+    ("fx_column_add", C.c_int, [vp, C.POINTER(ColumnDesc), C.POINTER(i32)]),
+    ("fx_column_run", C.c_int, [vp, i32, C.POINTER(i32)]),
+    ("fx_column_status", C.c_int, [vp, i32, C.POINTER(ColumnResult)]),
+    ("fx_column_fetch", C.c_int, [vp, i32, vp, vp, vp, vp]),
+    ("fx_column_state", C.c_int, [vp, i32, vp]),
+    ("fx_column_remove", C.c_int, [vp, i32]),
 ]
 
 
@@ -571,3 +601,84 @@ def dot(ctx, x, y):
     out = C.c_double()
     _check(lib().fx_dot(ctx.h, x.n, x.ptr, y.ptr, C.byref(out)), ctx)
     return out.value
+
+
+class ColumnPool:
+    """fx_column_pool: independent 1 x 1 x N PressureIndependMultiYield columns (laminar ties,
+    base fixed), each with its own Newmark + Newton + PCG analysis, advanced together."""
+
+    def __init__(self, ctx, max_elements, max_columns, dt, max_materials=None, max_record=None, gamma=0.5,
+                 beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, graph=True, mixed=False,
+                 inner_rtol=1e-5):
+        self.ctx = ctx
+        flags = (0 if graph else FX_COLUMN_NO_GRAPH) | (FX_COLUMN_MIXED if mixed else 0)
+        d = ColumnPoolDesc(int(max_elements), int(max_columns), int(max_materials or 16 * max_columns),
+                           int(max_record or 1 << 20), gamma, beta, dt, test, tol, max_iter, cg_rtol, cg_max_it, flags,
+                           inner_rtol)
+        h = C.c_void_p()
+        _check(lib().fx_column_pool_create(ctx.h, C.byref(d), C.byref(h)), ctx)
+        self.h = h
+        self.columns = {}  # id -> (nelem, nsteps, record)
+        ctx.adopt(self)
+
+    def add(self, mats, mat_of, nsteps, ground_accel, h=(1.0, 1.0, 1.0), stage=1, ground_dir=0, record=True):
+        """Admit a column (mats: PimyMaterial list, mat_of: parameter set per element bottom to
+        top). Returns its id, or None when the pool has no room now (FX_EFULL)."""
+        mo = np.ascontiguousarray(mat_of, np.int32)
+        ga = np.ascontiguousarray(ground_accel, np.float64)
+        arr = (PimyMaterial * len(mats))(*mats)
+        d = ColumnDesc(len(mo), h[0], h[1], h[2], len(mats), C.cast(arr, C.c_void_p), mo.ctypes.data, stage,
+                       int(nsteps), ground_dir, ga.ctypes.data, int(bool(record)))
+        cid = C.c_int32()
+        rc = lib().fx_column_add(self.h, C.byref(d), C.byref(cid))
+        if rc == FX_EFULL:
+            return None
+        _check(rc, self.ctx)
+        self.columns[cid.value] = (len(mo), int(nsteps), bool(record))
+        return cid.value
+
+    def run(self, iterations=1):
+        """Newton iterations of every running column; returns the number still running."""
+        n = C.c_int32()
+        _check(lib().fx_column_run(self.h, int(iterations), C.byref(n)), self.ctx)
+        return n.value
+
+    def status(self, cid):
+        r = ColumnResult()
+        _check(lib().fx_column_status(self.h, cid, C.byref(r)), self.ctx)
+        return {f: getattr(r, f) for f, _ in ColumnResult._fields_}
+
+    def fetch(self, cid, history=True):
+        n, nsteps, rec = self.columns[cid]
+        u, v, a = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        hist = np.zeros(9 * nsteps) if (history and rec) else None
+        _check(lib().fx_column_fetch(self.h, cid, u.ctypes.data, v.ctypes.data, a.ctypes.data,
+                                     hist.ctypes.data if hist is not None else None), self.ctx)
+        out = dict(u=u.reshape(n, 3), v=v.reshape(n, 3), a=a.reshape(n, 3))
+        if hist is not None:
+            out["history"] = hist.reshape(nsteps, 9)
+        return out
+
+    def state(self, cid):
+        n = self.columns[cid][0]
+        s = np.zeros(PIMY_NSTATE * n)
+        _check(lib().fx_column_state(self.h, cid, s.ctypes.data), self.ctx)
+        return s.reshape(PIMY_NSTATE, n).T
+
+    def remove(self, cid):
+        _check(lib().fx_column_remove(self.h, cid), self.ctx)
+        del self.columns[cid]
+
+    def close(self):
+        if self.h and self.ctx.h:
+            lib().fx_column_pool_destroy(self.h)
+        self.h = None
+
+    def release(self):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

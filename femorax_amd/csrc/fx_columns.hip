@@ -1,0 +1,801 @@
+// Batched soil columns: many independent 1 x 1 x N stdBrick<PressureIndependMultiYield> columns
+// (the layered column of synthetic_models.hpp:14-91: base fixed, side nodes of each level tied
+// in DOFs 1 2 3), each running its own implicit Newmark stage with Newton and PCG + Jacobi
+// (implicit_stage.cpp:29-68, 150-372; cg.cpp:24-72; jacobi.cpp:10-26), advanced together.
+//
+// Tied column algebra. Level L (0 = the fixed base, 1..N free) has three equations shared by
+// its four nodes. On an axis-aligned box element (size hx, hy, hz) the sums over a face's four
+// shape functions have gradient (0, 0, -1/hz) (lower face) and (0, 0, 1/hz) (upper face), so
+// the strain is uniform over the element:
+//   eps33 = du_z / hz, gamma23 = du_y / hz, gamma31 = du_x / hz (du: upper - lower level),
+// the other three components zero. Every Gauss point of an element sees the same strain
+// history and so holds the same PIMY state: the pool keeps one material point per element and
+// integrates it with weight 8 (sum of the eight dvol = V). The element's upper-level force is
+// F = A (sigma31, sigma23, sigma33) (A = hx hy), the lower-level force -F; its tangent in the
+// tied space is [[S, -S], [-S, S]] with S = (A / hz) D restricted to rows and columns (31, 23,
+// 33). Mass per element and direction: lumped rho V / 2 at each level; consistent rho V [[1/3,
+// 1/6], [1/6, 1/3]] (the exact integral of the face sums (1 -+ zeta) / 2, which the 2-point
+// Gauss rule reproduces). Body load per element: V b / 2 at each level.
+//
+// Kernels, one pair per Newton iteration of every running column:
+//  - column_force_kernel: one thread per element slot of the pool: strain from the level
+//    displacements, the return map (committed history of the column's current buffer, trial
+//    history into the other), F and S written per element;
+//  - column_solve_kernel<WAVES>: one workgroup of 64 WAVES threads per column of at most
+//    64 WAVES elements, one level per thread. The column's own Newmark/Newton state machine:
+//    unbalance B = -f - M a - a_g M r; the test (NormUnbalance ||B||, or NormDispIncr ||dU||
+//    of the last solve); on convergence the commit (buffer swap), the recorded top-level values
+//    and the next step's predictor; otherwise PCG + Jacobi on K_eff = K_t + c3 M (block
+//    tridiagonal, 3 x 3 blocks held in registers; neighbour values through LDS), warm-started
+//    from the previous increment, and the update. Workgroup sums are wave butterflies then a
+//    fixed-order sum over waves: a column's bits do not depend on the other columns.
+// The pair is captured once as a hipGraph and replayed.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "fx_internal.hpp"
+#include "fx_pimy.hpp"
+#include "fx_pimy_point.hpp"
+
+namespace fx {
+namespace {
+
+enum : int32_t { kIdle = 0, kRunning = 1, kFinished = 2, kFailed = 3 };
+
+struct ColumnDev {
+  int32_t status, step, nsteps, iter;
+  int32_t cur, nelem, ground_dir, record;
+  int32_t newton, max_newton, krylov, pad;
+  int64_t e0;    // first element slot; level j (1-based j + 1) has equations 3 (e0 + j) + d
+  int64_t rec;   // ground record offset in the record arena
+  int64_t hist;  // history offset (9 doubles per step), or -1
+  double hx, hy, hz;
+  double dnorm;      // ||dU|| of the last solve
+  double last_test;  // the last test value
+};
+
+struct PoolArgs {
+  int64_t cap;                  // element slots
+  int32_t max_columns;
+  int32_t test, max_iter, cg_max_it;
+  double tol, cg_rtol;
+  double a1, a2, a3, a4;        // predictor constants
+  double c2, c3;                // update constants gamma / (beta h), 1 / (beta h^2)
+  ColumnDev* cols;
+  const int32_t* elem_col;      // [cap] column slot or -1
+  const int32_t* elem_param;    // [cap] parameter set
+  const PimyParams* params;
+  double* state;                // 2 buffers of [kPimyNState * cap]
+  double* eout;                 // [9 * cap]: F (3), S upper triangle (6)
+  double* u;                    // [3 cap] each
+  double* v;
+  double* a;
+  double* ut;
+  double* vt;
+  double* at;
+  double* du;
+  const double* record;         // record arena
+  double* rec_out;              // same arena, written (histories)
+};
+
+// ---- the return map, one thread per element slot ------------------------------------------------
+
+__global__ __launch_bounds__(256) void column_force_kernel(PoolArgs a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= a.cap) return;
+  const int32_t c = a.elem_col[i];
+  if (c < 0) return;
+  const ColumnDev& col = a.cols[c];
+  if (col.status != kRunning) return;
+  const int64_t j = i - col.e0;
+  const double hz = col.hz, A = col.hx * col.hy;
+  double lo[3] = {0.0, 0.0, 0.0}, up[3];
+  for (int d = 0; d < 3; ++d) {
+    up[d] = a.u[3 * i + d];
+    if (j > 0) lo[d] = a.u[3 * (i - 1) + d];
+  }
+  double eps[6] = {0.0, 0.0, (up[2] - lo[2]) / hz, 0.0, (up[1] - lo[1]) / hz, (up[0] - lo[0]) / hz};
+  const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(a.cap);
+  const double* C = a.state + (col.cur ? NS : 0) + i;
+  double* T = a.state + (col.cur ? 0 : NS) + i;
+  double sig[6], D[36];
+  pimy_update(a.params[a.elem_param[i]], C, T, a.cap, eps, sig, D);
+  const int64_t P = a.cap;
+  a.eout[0 * P + i] = A * sig[5];
+  a.eout[1 * P + i] = A * sig[4];
+  a.eout[2 * P + i] = A * sig[2];
+  const double s = A / hz;
+  constexpr int idx[3] = {5, 4, 2};  // x, y, z <- gamma31, gamma23, eps33
+  int w = 3;
+  for (int r = 0; r < 3; ++r)
+    for (int q = r; q < 3; ++q, ++w) a.eout[w * P + i] = s * D[6 * idx[r] + idx[q]];
+}
+
+// ---- per-column solve ---------------------------------------------------------------------------
+
+struct Sym3 {  // xx xy xz yy yz zz
+  double m[6];
+  __device__ void mul_add(const double (&x)[3], double (&y)[3]) const {
+    y[0] += m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
+    y[1] += m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
+    y[2] += m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
+  }
+};
+
+__device__ inline double wave_sum(double x) {
+  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
+  return x;
+}
+
+template <int WAVES>
+struct Reducer {
+  double* red;  // LDS [2 * WAVES]
+  __device__ void sum2(double& x, double& y) const {
+    x = wave_sum(x);
+    y = wave_sum(y);
+    if constexpr (WAVES > 1) {
+      const int w = threadIdx.x / 64;
+      if ((threadIdx.x & 63) == 0) {
+        red[2 * w] = x;
+        red[2 * w + 1] = y;
+      }
+      __syncthreads();
+      x = red[0];
+      y = red[1];
+      for (int k = 1; k < WAVES; ++k) {
+        x += red[2 * k];
+        y += red[2 * k + 1];
+      }
+    }
+  }
+};
+
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
+  constexpr int NT = 64 * WAVES;
+  __shared__ double xs[3][NT];
+  __shared__ double redA[2 * WAVES], redB[2 * WAVES], redC[2 * WAVES];
+  const int32_t c = blockIdx.x;
+  ColumnDev& col = a.cols[c];
+  const int32_t status = col.status, N = col.nelem;
+  if (status != kRunning || N > NT || (WAVES > 1 && N <= NT / 2)) return;
+  const int t = threadIdx.x;
+  const bool live = t < N;
+  const int64_t e = col.e0 + t;  // the element below this level, whose top the level is
+  const int64_t P = a.cap;
+  const double V = col.hx * col.hy * col.hz;
+  const Reducer<WAVES> RA{redA}, RB{redB}, RC{redC};
+  auto neighbours = [&](const double (&x)[3], double (&xm)[3], double (&xp)[3]) {
+    __syncthreads();  // the previous exchange's reads are done
+    for (int d = 0; d < 3; ++d) xs[d][t] = x[d];
+    __syncthreads();
+    for (int d = 0; d < 3; ++d) {
+      xm[d] = (live && t > 0) ? xs[d][t - 1] : 0.0;
+      xp[d] = (live && t + 1 < N) ? xs[d][t + 1] : 0.0;
+    }
+  };
+  // mass of this level: diagonal and couplings (lumped: no coupling)
+  double m = 0.0, mlo = 0.0, mhi = 0.0, body[3] = {0, 0, 0};
+  const bool above = live && t + 1 < N;
+  if (live) {
+    const PimyParams& pe = a.params[a.elem_param[e]];
+    const bool lumped = pe.mass_type != 0;
+    m += lumped ? pe.rho * V / 2.0 : pe.rho * V / 3.0;
+    if (!lumped && t > 0) mlo = pe.rho * V / 6.0;
+    for (int d = 0; d < 3; ++d) body[d] += V * pe.b[d] / 2.0;
+    if (above) {
+      const PimyParams& pa = a.params[a.elem_param[e + 1]];
+      const bool lumped_a = pa.mass_type != 0;
+      m += lumped_a ? pa.rho * V / 2.0 : pa.rho * V / 3.0;
+      if (!lumped_a) mhi = pa.rho * V / 6.0;
+      for (int d = 0; d < 3; ++d) body[d] += V * pa.b[d] / 2.0;
+    }
+  }
+  // the unbalance B = -f - M a - a_g M r at the current u (forces from column_force_kernel)
+  double B[3] = {0, 0, 0}, av[3] = {0, 0, 0};
+  if (live)
+    for (int d = 0; d < 3; ++d) av[d] = a.a[3 * e + d];
+  double am[3], ap[3];
+  neighbours(av, am, ap);
+  const int step = col.step;
+  const double ag = a.record[col.rec + step];
+  if (live)
+    for (int d = 0; d < 3; ++d) {
+      double f = a.eout[d * P + e];
+      if (above) f -= a.eout[d * P + e + 1];
+      f -= body[d];
+      double ma = m * av[d];
+      ma += mlo * am[d];
+      ma += mhi * ap[d];
+      double r = -f;
+      r -= ma;
+      if (d == col.ground_dir) r -= ag * (m + mlo + mhi);
+      B[d] = r;
+    }
+  double bb = 0.0, unused = 0.0;
+  for (int d = 0; d < 3; ++d) bb += B[d] * B[d];
+  RA.sum2(bb, unused);
+  const double bnorm = sqrt(bb);
+  const int iter = col.iter;
+  if (iter > 0) {
+    const double value = a.test == 0 ? bnorm : col.dnorm;
+    bool done = value <= a.tol;
+    if (done) {  // commit, record, next step's predictor
+      const int next = step + 1;
+      if (live && t == N - 1 && col.hist >= 0) {
+        double* h = a.rec_out + col.hist + 9 * static_cast<int64_t>(step);
+        for (int d = 0; d < 3; ++d) {
+          h[d] = a.u[3 * e + d];
+          h[3 + d] = a.v[3 * e + d];
+          h[6 + d] = a.a[3 * e + d];
+        }
+      }
+      if (live && next < col.nsteps)
+        for (int d = 0; d < 3; ++d) {
+          const int64_t q = 3 * e + d;
+          const double u0 = a.u[q], v0 = a.v[q], a0 = a.a[q];
+          a.ut[q] = u0;
+          a.vt[q] = v0;
+          a.at[q] = a0;
+          a.v[q] = v0 * a.a1 + a0 * a.a2;
+          a.a[q] = a0 * a.a4 + v0 * a.a3;
+        }
+      __syncthreads();
+      if (t == 0) {
+        col.last_test = value;
+        col.newton += iter;
+        col.max_newton = max(col.max_newton, iter);
+        col.cur ^= 1;
+        col.step = next;
+        col.iter = 0;
+        if (next >= col.nsteps) col.status = kFinished;
+      }
+      return;
+    }
+    if (iter >= a.max_iter) {  // restore the step's start
+      if (live)
+        for (int d = 0; d < 3; ++d) {
+          const int64_t q = 3 * e + d;
+          a.u[q] = a.ut[q];
+          a.v[q] = a.vt[q];
+          a.a[q] = a.at[q];
+        }
+      __syncthreads();
+      if (t == 0) {
+        col.last_test = value;
+        col.newton += iter;
+        col.max_newton = max(col.max_newton, iter);
+        col.status = kFailed;
+      }
+      return;
+    }
+  }
+  // K_eff blocks: diagonal Kd = S_e + S_{e+1} + c3 m I; couplings Klo (to the level below)
+  // = -S_e + c3 mlo I, Khi (to the level above) = -S_{e+1} + c3 mhi I
+  Sym3 Kd{}, Klo{}, Khi{};
+  double inv[3] = {0, 0, 0};
+  if (live) {
+    for (int w = 0; w < 6; ++w) {
+      const double se = a.eout[(3 + w) * P + e];
+      const double sa = above ? a.eout[(3 + w) * P + e + 1] : 0.0;
+      Kd.m[w] = se + sa;
+      Klo.m[w] = -se;
+      Khi.m[w] = -sa;
+    }
+    constexpr int diag[3] = {0, 3, 5};
+    for (int d = 0; d < 3; ++d) {
+      Kd.m[diag[d]] += a.c3 * m;
+      Klo.m[diag[d]] += a.c3 * mlo;
+      Khi.m[diag[d]] += a.c3 * mhi;
+    }
+    for (int d = 0; d < 3; ++d) inv[d] = 1.0 / Kd.m[diag[d]];
+  }
+  auto apply = [&](const double (&x)[3], double (&y)[3]) {
+    double xm[3], xp[3];
+    neighbours(x, xm, xp);
+    for (int d = 0; d < 3; ++d) y[d] = 0.0;
+    if (!live) return;
+    Klo.mul_add(xm, y);
+    Kd.mul_add(x, y);
+    Khi.mul_add(xp, y);
+  };
+  // PCG (cg.cpp:24-72): warm start from the previous increment, stop on the recursive residual
+  double x[3] = {0, 0, 0}, r[3], z[3], p[3], Ap[3];
+  if (live)
+    for (int d = 0; d < 3; ++d) x[d] = a.du[3 * e + d];
+  apply(x, Ap);
+  double rr = 0.0, rz = 0.0;
+  for (int d = 0; d < 3; ++d) {
+    r[d] = B[d] - Ap[d];
+    z[d] = inv[d] * r[d];
+    p[d] = z[d];
+    rr += r[d] * r[d];
+    rz += r[d] * z[d];
+  }
+  RB.sum2(rr, rz);
+  const double tol = a.cg_rtol * bnorm;
+  double rnorm = sqrt(rr);
+  int its = 0;
+  bool converged = rnorm <= tol, indefinite = false;
+  while (!converged && its < a.cg_max_it) {
+    apply(p, Ap);
+    double pAp = 0.0, dummy = 0.0;
+    for (int d = 0; d < 3; ++d) pAp += p[d] * Ap[d];
+    RC.sum2(pAp, dummy);
+    if (!(pAp > 0.0)) {
+      indefinite = true;
+      break;
+    }
+    const double alpha = rz / pAp;
+    double rr2 = 0.0, rz2 = 0.0;
+    for (int d = 0; d < 3; ++d) {
+      x[d] += alpha * p[d];
+      r[d] -= alpha * Ap[d];
+      z[d] = inv[d] * r[d];
+      rr2 += r[d] * r[d];
+      rz2 += r[d] * z[d];
+    }
+    RB.sum2(rr2, rz2);
+    ++its;
+    rnorm = sqrt(rr2);
+    if (rnorm <= tol) {
+      converged = true;
+      break;
+    }
+    const double beta = rz2 / rz;
+    rz = rz2;
+    for (int d = 0; d < 3; ++d) p[d] = z[d] + beta * p[d];
+  }
+  (void)indefinite;
+  const bool accept = converged || rnorm <= 1e-6 * bnorm;
+  if (!accept) {  // the solve failed: restore the step's start (implicit_stage.cpp:226-238)
+    if (live)
+      for (int d = 0; d < 3; ++d) {
+        const int64_t q = 3 * e + d;
+        a.u[q] = a.ut[q];
+        a.v[q] = a.vt[q];
+        a.a[q] = a.at[q];
+      }
+    __syncthreads();
+    if (t == 0) {
+      col.krylov += its;
+      col.newton += iter + 1;
+      col.max_newton = max(col.max_newton, iter + 1);
+      col.status = kFailed;
+    }
+    return;
+  }
+  double dd = 0.0, unused2 = 0.0;
+  if (live)
+    for (int d = 0; d < 3; ++d) {
+      const int64_t q = 3 * e + d;
+      a.du[q] = x[d];
+      a.u[q] += x[d];
+      a.v[q] += x[d] * a.c2;
+      a.a[q] += x[d] * a.c3;
+      dd += x[d] * x[d];
+    }
+  RA.sum2(dd, unused2);
+  __syncthreads();
+  if (t == 0) {
+    col.dnorm = sqrt(dd);
+    col.krylov += its;
+    col.iter = iter + 1;
+  }
+}
+
+__global__ void column_init_kernel(int64_t cap, int64_t e0, int32_t n, const PimyParams* __restrict__ params,
+                                   const int32_t* __restrict__ elem_param, double* state) {
+  const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = e0 + k;
+  const PimyParams& p = params[elem_param[i]];
+  const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(cap);
+  for (double* S : {state, state + NS}) {
+    for (int f = 0; f < kPimyNState; ++f) S[f * cap + i] = 0.0;
+    S[kPG * cap + i] = p.G;
+    S[kPK * cap + i] = p.K;
+    for (int s = 1; s <= kPimyNS; ++s) {
+      S[(kPSizes + s - 1) * cap + i] = p.size[s];
+      S[(kPModuli + s - 1) * cap + i] = p.modulus[s];
+    }
+  }
+}
+
+__global__ void column_count_kernel(const ColumnDev* cols, int32_t n, int32_t* out) {
+  __shared__ int32_t part[256];
+  int32_t k = 0;
+  for (int32_t c = threadIdx.x; c < n; c += blockDim.x) k += cols[c].status == kRunning ? 1 : 0;
+  part[threadIdx.x] = k;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (static_cast<int>(threadIdx.x) < s) part[threadIdx.x] += part[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// First-fit allocator of [offset, offset + length) ranges with coalescing.
+class Ranges {
+ public:
+  void reset(int64_t cap) {
+    free_.clear();
+    if (cap > 0) free_[0] = cap;
+  }
+  int64_t take(int64_t n) {
+    if (n <= 0) return 0;
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= n) {
+        const int64_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > n) free_[off + n] = len - n;
+        return off;
+      }
+    return -1;
+  }
+  void give(int64_t off, int64_t n) {
+    if (n <= 0) return;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+
+ private:
+  std::map<int64_t, int64_t> free_;
+};
+
+struct HostColumn {
+  bool used = false;
+  int64_t e0 = 0, rec = 0, hist = -1, rec_len = 0;
+  int32_t nelem = 0, m0 = 0, nmat = 0, nsteps = 0;
+};
+
+}  // namespace
+}  // namespace fx
+
+struct fx_column_pool {
+  fx_ctx* ctx = nullptr;
+  fx_column_pool_desc d{};
+  fx::PoolArgs args{};
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  fx::Ranges elems, mats, recs;
+  std::vector<fx::HostColumn> host;
+  int32_t* count = nullptr;       // device
+  int32_t* count_host = nullptr;  // pinned
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace fx {
+namespace {
+
+int fail(fx_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->error = msg;
+  return code;
+}
+
+template <typename T>
+int pool_alloc(fx_column_pool* p, T*& dst, size_t n) {
+  dst = nullptr;
+  const size_t sz = std::max<size_t>(n, 1) * sizeof(T);
+  fx_ctx* ctx = p->ctx;
+  if (ctx->used + sz > ctx->budget) return fail(ctx, FX_ENOMEM, "column pool: HBM budget exceeded");
+  FX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&dst), sz));
+  p->allocs.push_back(dst);
+  ctx->used += sz;
+  p->bytes += sz;
+  FX_HIP(ctx, hipMemset(dst, 0, sz));
+  return FX_OK;
+}
+
+// One Newton iteration of every running column: the return maps, then the solves in four
+// workgroup sizes (1, 2, 4, 8 waves: columns of up to 64, 128, 256, 512 elements).
+int launch_iteration(fx_column_pool* p, hipStream_t st) {
+  const PoolArgs& a = p->args;
+  hipLaunchKernelGGL(column_force_kernel, dim3(static_cast<unsigned>((a.cap + 255) / 256)), dim3(256), 0, st, a);
+  const unsigned nc = static_cast<unsigned>(a.max_columns);
+  hipLaunchKernelGGL(column_solve_kernel<1>, dim3(nc), dim3(64), 0, st, a);
+  hipLaunchKernelGGL(column_solve_kernel<2>, dim3(nc), dim3(128), 0, st, a);
+  hipLaunchKernelGGL(column_solve_kernel<4>, dim3(nc), dim3(256), 0, st, a);
+  hipLaunchKernelGGL(column_solve_kernel<8>, dim3(nc), dim3(512), 0, st, a);
+  FX_HIP(p->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+}  // namespace
+}  // namespace fx
+
+extern "C" {
+
+void fx_column_pool_destroy(fx_column_pool* p) {
+  if (!p) return;
+  fx_ctx* ctx = p->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (p->exec) (void)hipGraphExecDestroy(p->exec);
+  if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (void* q : p->allocs) (void)hipFree(q);
+  if (p->count_host) (void)hipHostFree(p->count_host);
+  ctx->used -= std::min<uint64_t>(ctx->used, p->bytes);
+  delete p;
+}
+
+int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_pool** out) {
+  using namespace fx;
+  if (!ctx || !d || !out) return fail(ctx, FX_EINVAL, "fx_column_pool_create: null argument");
+  *out = nullptr;
+  if (d->max_elements <= 0 || d->max_columns <= 0 || d->max_materials <= 0 || d->max_record <= 0 ||
+      !(d->dt > 0.0) || !(d->beta > 0.0) || d->max_iter <= 0 || d->cg_max_it <= 0 || (d->test != 0 && d->test != 1))
+    return fail(ctx, FX_EINVAL, "fx_column_pool_create: bad descriptor");
+  if (d->flags & FX_COLUMN_MIXED) return fail(ctx, FX_EUNSUP, "fx_column_pool_create: FX_COLUMN_MIXED is not built");
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  fx_column_pool* p = new (std::nothrow) fx_column_pool;
+  if (!p) return fail(ctx, FX_ENOMEM, "host allocation failed");
+  p->ctx = ctx;
+  p->d = *d;
+  PoolArgs& a = p->args;
+  a.cap = d->max_elements;
+  a.max_columns = d->max_columns;
+  a.test = d->test;
+  a.max_iter = d->max_iter;
+  a.cg_max_it = d->cg_max_it;
+  a.tol = d->tol;
+  a.cg_rtol = d->cg_rtol;
+  const double gamma = d->gamma, beta = d->beta, h = d->dt;
+  a.a1 = 1.0 - gamma / beta;
+  a.a2 = h * (1.0 - 0.5 * gamma / beta);
+  a.a3 = -1.0 / (beta * h);
+  a.a4 = 1.0 - 0.5 / beta;
+  a.c2 = gamma / (beta * h);
+  a.c3 = 1.0 / (beta * h * h);
+  const size_t cap = static_cast<size_t>(a.cap);
+  int rc = FX_OK;
+  ColumnDev* cols = nullptr;
+  int32_t *elem_col = nullptr, *elem_param = nullptr;
+  PimyParams* params = nullptr;
+  double* rec = nullptr;
+  if ((rc = pool_alloc(p, cols, d->max_columns)) || (rc = pool_alloc(p, elem_col, cap)) ||
+      (rc = pool_alloc(p, elem_param, cap)) || (rc = pool_alloc(p, params, d->max_materials)) ||
+      (rc = pool_alloc(p, a.state, 2 * kPimyNState * cap)) || (rc = pool_alloc(p, a.eout, 9 * cap)) ||
+      (rc = pool_alloc(p, a.u, 3 * cap)) || (rc = pool_alloc(p, a.v, 3 * cap)) || (rc = pool_alloc(p, a.a, 3 * cap)) ||
+      (rc = pool_alloc(p, a.ut, 3 * cap)) || (rc = pool_alloc(p, a.vt, 3 * cap)) ||
+      (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) ||
+      (rc = pool_alloc(p, rec, static_cast<size_t>(d->max_record))) || (rc = pool_alloc(p, p->count, 1))) {
+    fx_column_pool_destroy(p);
+    return rc;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&p->count_host), sizeof(int32_t)) != hipSuccess) {
+    fx_column_pool_destroy(p);
+    return fail(ctx, FX_EDEVICE, "hipHostMalloc failed");
+  }
+  a.cols = cols;
+  a.elem_col = elem_col;
+  a.elem_param = elem_param;
+  a.params = params;
+  a.record = rec;
+  a.rec_out = rec;
+  // no element slot belongs to a column yet
+  std::vector<int32_t> none(cap, -1);
+  if (hipMemcpy(elem_col, none.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    fx_column_pool_destroy(p);
+    return fail(ctx, FX_EDEVICE, "hipMemcpy failed");
+  }
+  p->elems.reset(a.cap);
+  p->mats.reset(d->max_materials);
+  p->recs.reset(d->max_record);
+  p->host.assign(d->max_columns, HostColumn{});
+  if (!(d->flags & FX_COLUMN_NO_GRAPH)) {
+    hipStream_t st = ctx->stream;
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      fx_column_pool_destroy(p);
+      return fail(ctx, FX_EDEVICE, "hipStreamBeginCapture failed");
+    }
+    const int lrc = launch_iteration(p, st);
+    const hipError_t ec = hipStreamEndCapture(st, &p->graph);
+    if (lrc || ec != hipSuccess || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
+      fx_column_pool_destroy(p);
+      return fail(ctx, FX_EDEVICE, "column pool: graph capture failed");
+    }
+  }
+  *out = p;
+  return FX_OK;
+}
+
+int fx_column_add(fx_column_pool* p, const fx_column_desc* d, int32_t* id) {
+  using namespace fx;
+  if (!p || !d || !id) return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_add: null argument");
+  fx_ctx* ctx = p->ctx;
+  *id = -1;
+  const int32_t N = d->nelem;
+  if (N < 1 || N > FX_COLUMN_MAX_ELEMS || d->nmat < 1 || !d->mats || !d->mat_of || d->nsteps < 1 ||
+      !d->ground_accel || d->ground_dir < 0 || d->ground_dir > 2 || !(d->hx > 0) || !(d->hy > 0) || !(d->hz > 0) ||
+      (d->stage != 0 && d->stage != 1))
+    return fail(ctx, FX_EINVAL, "fx_column_add: bad descriptor");
+  for (int32_t k = 0; k < N; ++k)
+    if (d->mat_of[k] < 0 || d->mat_of[k] >= d->nmat) return fail(ctx, FX_EINVAL, "fx_column_add: mat_of out of range");
+  std::vector<PimyParams> pp(d->nmat);
+  for (int32_t i = 0; i < d->nmat; ++i) {
+    std::string err;
+    if (int rc = pimy_setup(d->mats[i], pp[i], err)) return fail(ctx, rc, err + " (set " + std::to_string(i) + ")");
+    pp[i].load_stage = d->stage;
+  }
+  int32_t c = -1;
+  for (int32_t k = 0; k < p->d.max_columns; ++k)
+    if (!p->host[k].used) {
+      c = k;
+      break;
+    }
+  const int64_t rec_len = d->nsteps + (d->record ? 9 * static_cast<int64_t>(d->nsteps) : 0);
+  if (c < 0) return fail(ctx, FX_EFULL, "fx_column_add: no free column slot");
+  const int64_t e0 = p->elems.take(N);
+  if (e0 < 0) return fail(ctx, FX_EFULL, "fx_column_add: no room for the elements");
+  const int64_t m0 = p->mats.take(d->nmat);
+  if (m0 < 0) {
+    p->elems.give(e0, N);
+    return fail(ctx, FX_EFULL, "fx_column_add: no room for the parameter sets");
+  }
+  const int64_t r0 = p->recs.take(rec_len);
+  if (r0 < 0) {
+    p->elems.give(e0, N);
+    p->mats.give(m0, d->nmat);
+    return fail(ctx, FX_EFULL, "fx_column_add: no room for the record");
+  }
+  HostColumn& hc = p->host[c];
+  hc.used = true;
+  hc.e0 = e0;
+  hc.nelem = N;
+  hc.m0 = static_cast<int32_t>(m0);
+  hc.nmat = d->nmat;
+  hc.rec = r0;
+  hc.rec_len = rec_len;
+  hc.hist = d->record ? r0 + d->nsteps : -1;
+  hc.nsteps = d->nsteps;
+  PoolArgs& a = p->args;
+  hipStream_t st = ctx->stream;
+  std::vector<int32_t> ecol(N, c), epar(N);
+  for (int32_t k = 0; k < N; ++k) epar[k] = static_cast<int32_t>(m0) + d->mat_of[k];
+  ColumnDev cd{};
+  cd.status = kRunning;
+  cd.nsteps = d->nsteps;
+  cd.nelem = N;
+  cd.ground_dir = d->ground_dir;
+  cd.record = d->record;
+  cd.e0 = e0;
+  cd.rec = r0;
+  cd.hist = hc.hist;
+  cd.hx = d->hx;
+  cd.hy = d->hy;
+  cd.hz = d->hz;
+  const size_t n3 = 3 * static_cast<size_t>(N) * sizeof(double);
+  FX_HIP(ctx, hipMemcpyAsync(const_cast<PimyParams*>(a.params) + m0, pp.data(), sizeof(PimyParams) * d->nmat,
+                             hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipMemcpyAsync(const_cast<int32_t*>(a.elem_param) + e0, epar.data(), sizeof(int32_t) * N,
+                             hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipMemcpyAsync(const_cast<double*>(a.record) + r0, d->ground_accel, sizeof(double) * d->nsteps,
+                             hipMemcpyHostToDevice, st));
+  for (double* q : {a.u, a.v, a.a, a.ut, a.vt, a.at, a.du}) FX_HIP(ctx, hipMemsetAsync(q + 3 * e0, 0, n3, st));
+  hipLaunchKernelGGL(column_init_kernel, dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, st, a.cap, e0, N,
+                     a.params, a.elem_param, a.state);
+  FX_HIP(ctx, hipGetLastError());
+  // the column joins the running set last: slot ownership, then its status
+  FX_HIP(ctx, hipMemcpyAsync(const_cast<int32_t*>(a.elem_col) + e0, ecol.data(), sizeof(int32_t) * N,
+                             hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipMemcpyAsync(a.cols + c, &cd, sizeof cd, hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  *id = c;
+  return FX_OK;
+}
+
+int fx_column_run(fx_column_pool* p, int32_t iterations, int32_t* running) {
+  using namespace fx;
+  if (!p || iterations < 0) return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_run: bad argument");
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  for (int32_t k = 0; k < iterations; ++k) {
+    if (p->exec) FX_HIP(ctx, hipGraphLaunch(p->exec, st));
+    else if (int rc = launch_iteration(p, st)) return rc;
+  }
+  hipLaunchKernelGGL(column_count_kernel, dim3(1), dim3(256), 0, st, p->args.cols, p->args.max_columns, p->count);
+  FX_HIP(ctx, hipGetLastError());
+  FX_HIP(ctx, hipMemcpyAsync(p->count_host, p->count, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  if (running) *running = *p->count_host;
+  return FX_OK;
+}
+
+int fx_column_status(fx_column_pool* p, int32_t id, fx_column_result* r) {
+  using namespace fx;
+  if (!p || !r || id < 0 || id >= p->d.max_columns || !p->host[id].used)
+    return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_status: bad column");
+  fx_ctx* ctx = p->ctx;
+  ColumnDev cd{};
+  FX_HIP(ctx, hipMemcpyAsync(&cd, p->args.cols + id, sizeof cd, hipMemcpyDeviceToHost, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  r->status = cd.status == kRunning ? 0 : cd.status == kFinished ? 1 : 2;
+  r->steps = cd.step;
+  r->newton_iterations = cd.newton;
+  r->max_newton_iterations = cd.max_newton;
+  r->krylov_iterations = cd.krylov;
+  r->last_test = cd.last_test;
+  return FX_OK;
+}
+
+int fx_column_fetch(fx_column_pool* p, int32_t id, double* u, double* v, double* a, double* history) {
+  using namespace fx;
+  if (!p || id < 0 || id >= p->d.max_columns || !p->host[id].used)
+    return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_fetch: bad column");
+  const HostColumn& hc = p->host[id];
+  if (history && hc.hist < 0) return fail(p->ctx, FX_EINVAL, "fx_column_fetch: the column keeps no history");
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  const size_t n3 = 3 * static_cast<size_t>(hc.nelem) * sizeof(double);
+  const double* src[3] = {p->args.u, p->args.v, p->args.a};
+  double* dst[3] = {u, v, a};
+  for (int k = 0; k < 3; ++k)
+    if (dst[k]) FX_HIP(ctx, hipMemcpyAsync(dst[k], src[k] + 3 * hc.e0, n3, hipMemcpyDeviceToHost, st));
+  if (history)
+    FX_HIP(ctx, hipMemcpyAsync(history, p->args.record + hc.hist, sizeof(double) * 9 * hc.nsteps,
+                               hipMemcpyDeviceToHost, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  return FX_OK;
+}
+
+int fx_column_state(fx_column_pool* p, int32_t id, double* state) {
+  using namespace fx;
+  if (!p || !state || id < 0 || id >= p->d.max_columns || !p->host[id].used)
+    return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_state: bad column");
+  const HostColumn& hc = p->host[id];
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  ColumnDev cd{};
+  FX_HIP(ctx, hipMemcpyAsync(&cd, p->args.cols + id, sizeof cd, hipMemcpyDeviceToHost, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  const size_t cap = static_cast<size_t>(p->args.cap);
+  const double* S = p->args.state + (cd.cur ? static_cast<size_t>(kPimyNState) * cap : 0) + hc.e0;
+  FX_HIP(ctx, hipMemcpy2DAsync(state, sizeof(double) * hc.nelem, S, sizeof(double) * cap, sizeof(double) * hc.nelem,
+                               kPimyNState, hipMemcpyDeviceToHost, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  return FX_OK;
+}
+
+int fx_column_remove(fx_column_pool* p, int32_t id) {
+  using namespace fx;
+  if (!p || id < 0 || id >= p->d.max_columns || !p->host[id].used)
+    return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_remove: bad column");
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  HostColumn& hc = p->host[id];
+  ColumnDev cd{};
+  cd.status = kIdle;
+  std::vector<int32_t> none(hc.nelem, -1);
+  FX_HIP(ctx, hipMemcpyAsync(p->args.cols + id, &cd, sizeof cd, hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipMemcpyAsync(const_cast<int32_t*>(p->args.elem_col) + hc.e0, none.data(), sizeof(int32_t) * hc.nelem,
+                             hipMemcpyHostToDevice, st));
+  FX_HIP(ctx, hipStreamSynchronize(st));
+  p->elems.give(hc.e0, hc.nelem);
+  p->mats.give(hc.m0, hc.nmat);
+  p->recs.give(hc.rec, hc.rec_len);
+  hc = HostColumn{};
+  return FX_OK;
+}
+
+}  // extern "C"

@@ -219,3 +219,47 @@ def config_c3(nx=200, ny=200, nz=100, h=5.0, layers=10, seed=SEED_BASE + 2):
     NX, NY = nx + 1, ny + 1
     m["source_node"] = (nx // 2) + NX * ((ny // 2) + NY * nz)
     return m
+
+
+def ground_motion(nsteps, dt, seed, pga=0.3 * 9.81, band=(0.1, 20.0)):
+    """Base acceleration: LCG white noise, 4th-order Butterworth band-pass (zero-phase: forward
+    and backward), scaled to the peak ground acceleration `pga`. Values at the end of each
+    step (t = (k + 1) dt)."""
+    from scipy.signal import butter, sosfiltfilt
+
+    rng = Lcg(seed)
+    length = max(nsteps, 1024)  # short records: the first nsteps values of a 1024-step record
+    w = np.array([rng.symmetric() for _ in range(length)])
+    sos = butter(4, [band[0], band[1]], btype="bandpass", fs=1.0 / dt, output="sos")
+    g = sosfiltfilt(sos, w)[:nsteps]
+    return g * (pga / np.max(np.abs(g)))
+
+
+def soil_layers(rng, nlayer=10):
+    """PressureIndependMultiYield layers: Gmax ~ U(10, 300) MPa, K = 2.17 Gmax, cohesion
+    ~ U(20, 150) kPa, rho 2000, phi 0, gamma_peak 0.1, 20 automatic surfaces, lumped mass, no
+    body force."""
+    out = []
+    for _ in range(nlayer):
+        G = rng.uniform(10e6, 300e6)
+        out.append(dict(rho=2000.0, G=G, K=2.17 * G, cohesion=rng.uniform(20e3, 150e3), gamma_peak=0.1, phi=0.0,
+                        pref=100e3, pdc=0.0, nsurf=20, b=(0.0, 0.0, 0.0), mass_type=1))
+    return out
+
+
+def column_layer_of(nelem, nlayer=10):
+    """Layer of each element of a column, bottom to top: `nlayer` nearly equal layers."""
+    return np.minimum(np.arange(nelem) * nlayer // nelem, nlayer - 1).astype(np.int32)
+
+
+def config_c4(ncol=4096, nsteps=4000, dt=0.005, seed=SEED_BASE + 3):
+    """Batched site response: `ncol` columns 1 x 1 x N (h = 1 m), N ~ U{20..400}, 10 layers per
+    column (soil_layers), stage 1 from t = 0, one base record (ground_motion, 0.3 g) in x.
+    Returns (columns, record): columns = list of dict(nelem, mats, mat_of)."""
+    rng = Lcg(seed)
+    cols = []
+    for _ in range(ncol):
+        n = 20 + int((rng.symmetric() + 1.0) * 0.5 * 381) % 381
+        mats = soil_layers(rng, 10)
+        cols.append(dict(nelem=n, mats=mats, mat_of=column_layer_of(n, 10)))
+    return cols, ground_motion(nsteps, dt, seed + 1000)

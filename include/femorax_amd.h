@@ -35,6 +35,7 @@ extern "C" {
 #define FX_ENOMEM -3   /* over the HBM budget or out of device memory */
 #define FX_EZERO -4    /* Jacobi: zero diagonal entries (jacobi.cpp:19-22) */
 #define FX_EUNSUP -5   /* unsupported element/material/option */
+#define FX_EFULL -6    /* a column pool has no room for the column now (retry after columns finish) */
 
 typedef struct fx_ctx fx_ctx;
 typedef struct fx_mesh fx_mesh;
@@ -324,6 +325,78 @@ typedef struct fx_newmark_result {
  * result.failed = 1. */
 int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, double* a, double* du,
                fx_newmark_result* result);
+
+/* ---- batched soil columns ------------------------------------------------------------------
+ * Many independent 1 x 1 x N stdBrick<PressureIndependMultiYield> columns (the layered soil
+ * column of synthetic_models.hpp:14-91: base fixed, the side nodes of each level tied by
+ * equalDOF in DOFs 1 2 3), each running its own implicit Newmark stage with Newton and PCG +
+ * Jacobi (implicit_stage.cpp:150-372, cg.cpp:24-72), advanced together on one GPU. No
+ * counterpart entry point in the reference (it runs one model per analysis); each column's
+ * result is that analysis' result within the stated tolerance, and does not depend on which
+ * other columns share the pool.
+ *
+ * The four nodes of a level share their three equations, so on the column's box elements the
+ * strain is uniform: eps33, gamma23, gamma31 from the level displacement differences, the rest
+ * zero. All eight Gauss points of an element see the same strain history and hold the same
+ * state, so the pool keeps one material point per element (one history, one return map) and
+ * integrates it with weight 8. Columns advance asynchronously: each has its own step, Newton
+ * iteration and CG scalars, so columns can be added while others run. */
+#define FX_COLUMN_MAX_ELEMS 512
+#define FX_COLUMN_MIXED 1u    /* inner CG in FP32 on FP32 blocks, inside FP64 iterative refinement */
+#define FX_COLUMN_NO_GRAPH 2u /* launch the iteration kernels one by one instead of replaying a hipGraph */
+typedef struct fx_column_pool fx_column_pool;
+typedef struct fx_column_pool_desc {
+  int64_t max_elements;       /* element slots of all resident columns together */
+  int32_t max_columns;        /* columns resident at once */
+  int32_t max_materials;      /* parameter sets resident at once */
+  int64_t max_record;         /* doubles for excitation records and per-step top-level histories */
+  double gamma, beta, dt;     /* Newmark constants and step (shared by the pool) */
+  int32_t test;               /* 0 NormUnbalance, 1 NormDispIncr */
+  double tol;
+  int32_t max_iter;           /* Newton iterations per step */
+  double cg_rtol;             /* PCG relative tolerance (the reference's default 1e-12) */
+  int32_t cg_max_it;
+  uint32_t flags;             /* FX_COLUMN_* */
+  double inner_rtol;          /* FX_COLUMN_MIXED: relative reduction of each FP32 inner solve */
+} fx_column_pool_desc;
+typedef struct fx_column_desc {
+  int32_t nelem;              /* elements, bottom to top (1..FX_COLUMN_MAX_ELEMS) */
+  double hx, hy, hz;          /* element size */
+  int32_t nmat;               /* parameter sets of this column */
+  const fx_pimy_material* mats;
+  const int32_t* mat_of;      /* host [nelem]: parameter set of each element, bottom to top */
+  int32_t stage;              /* material stage: 0 elastic, 1 elastoplastic */
+  int32_t nsteps;
+  int32_t ground_dir;         /* UniformExcitation direction 0..2 */
+  const double* ground_accel; /* host [nsteps]: the ground acceleration at the end of each step */
+  int32_t record;             /* 1: keep u, v, a of the top level after each step */
+} fx_column_desc;
+typedef struct fx_column_result {
+  int32_t status;             /* 0 running, 1 finished, 2 failed (the failed step's start is kept) */
+  int32_t steps;              /* converged (committed) steps */
+  int32_t newton_iterations, max_newton_iterations, krylov_iterations;
+  double last_test;
+} fx_column_result;
+/* Allocates all device memory of the pool (nothing is allocated per column); FX_ENOMEM over the
+ * context's budget. */
+int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_pool** out);
+void fx_column_pool_destroy(fx_column_pool* p);
+/* Admits a column at rest (u = v = a = 0, histories at their initial state) and starts its
+ * first step; *id names it until fx_column_remove. FX_EFULL when the pool lacks element slots,
+ * materials, record space or a column slot now. */
+int fx_column_add(fx_column_pool* p, const fx_column_desc* d, int32_t* id);
+/* Runs `iterations` Newton iterations of every running column (asynchronous columns: a column
+ * that converges commits, records and predicts its next step within the same iteration);
+ * *running: columns still running afterwards. Blocking. */
+int fx_column_run(fx_column_pool* p, int32_t iterations, int32_t* running);
+int fx_column_status(fx_column_pool* p, int32_t id, fx_column_result* r);
+/* Host copies of a column: u, v, a [3 nelem] (level 1..N, x y z; null to skip) and the
+ * recorded history [9 nsteps] (u, v, a of the top level per committed step; null to skip). */
+int fx_column_fetch(fx_column_pool* p, int32_t id, double* u, double* v, double* a, double* history);
+/* The column's committed histories [FX_PIMY_NSTATE * nelem], field-major (host). */
+int fx_column_state(fx_column_pool* p, int32_t id, double* state);
+/* Frees the column's slots (a running column is stopped). */
+int fx_column_remove(fx_column_pool* p, int32_t id);
 
 #ifdef __cplusplus
 }

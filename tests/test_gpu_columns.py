@@ -1,0 +1,210 @@
+"""GPU parity of the batched soil-column pool (fx_column_*): each column's Newmark + Newton +
+PCG analysis against (a) the oracle's implicit stage on the same column model (eight Gauss
+points per element, the reference's numbering and ties), (b) the per-model GPU path
+(fx_newmark on a stdBrick<PressureIndependMultiYield> batch), for columns of every solve
+workgroup size; the independence of a column's result from the other columns in the pool
+(bit for bit, admitted at a different time and place); admission limits and the failure path.
+
+Tolerances: the pool evaluates one material point per element (the strain of a tied box
+element is uniform) and sums in its own order, so it agrees with the eight-point models to
+rounding: displacements, velocities, accelerations and histories within 1e-10 of the largest
+value of the history (measured: about 1e-15 for u, 1e-13 for a; the differences are printed).
+"""
+import numpy as np
+import pytest
+
+from femorax_amd import core
+from femorax_amd import synthetic as S
+from oracle import fxo
+from tests.test_gpu_pimy_batch import history_error
+
+pytestmark = pytest.mark.gpu
+
+DT = 0.005
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = core.Context(0)
+    yield c
+    c.close()
+
+
+def dev_mats(mats, keep):
+    return [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"],
+                               nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep) for m in mats]
+
+
+def column_case(n, seed, nlayer=3, mass_type=1, body=(0.0, 0.0, 0.0)):
+    rng = S.Lcg(seed)
+    mats = S.soil_layers(rng, nlayer)
+    for m in mats:
+        m["mass_type"] = mass_type
+        m["b"] = body
+    return mats, S.column_layer_of(n, nlayer)
+
+
+def column_model(n, nlayer, mat_of):
+    dummy = [dict(E=1e8, nu=0.3, rho=2000.0) for _ in range(nlayer)]
+    return S.soil_column(n, 1.0, params=dummy, layer_of=lambda k: mat_of[k])
+
+
+def level_values(pm, x, n):
+    """Equation vector of a column model -> [n, 3] values of levels 1..n (the side node tied
+    to the others at each level is node 4 k)."""
+    return pm.to_nodes(x)[4 * np.arange(1, n + 1)]
+
+
+def run_pool(ctx, cases, nsteps, ag, max_iter=25, tol=1e-10, iters_per_call=4, pool=None):
+    own = pool is None
+    if own:
+        pool = core.ColumnPool(ctx, max_elements=sum(len(mo) for _, mo in cases) + 64, max_columns=len(cases) + 2,
+                               dt=DT, max_record=10 * nsteps * (len(cases) + 1) + 16, max_iter=max_iter, tol=tol)
+    keep = []
+    ids = [pool.add(dev_mats(m, keep), mo, nsteps, ag) for m, mo in cases]
+    assert all(i is not None for i in ids)
+    for _ in range(100000):
+        if pool.run(iters_per_call) == 0:
+            break
+    out = [dict(pool.fetch(i), status=pool.status(i), state=pool.state(i)) for i in ids]
+    if own:
+        pool.close()
+    return out
+
+
+def rel(got, want):
+    return np.max(np.abs(got - want)) / max(np.max(np.abs(want)), 1e-300)
+
+
+def compare(res, hist_u, hist_v, hist_a, u, v, a, label):
+    h = res["history"]
+    errs = (rel(h[:, 0:3], hist_u), rel(h[:, 3:6], hist_v), rel(h[:, 6:9], hist_a), rel(res["u"], u),
+            rel(res["v"], v), rel(res["a"], a))
+    print(label, "errors u v a (history), u v a (final):", ["%.2e" % e for e in errs])
+    assert max(errs) <= 1e-10, errs
+
+
+@pytest.mark.parametrize("n,mass_type", [(6, 1), (9, 0)])
+def test_columns_match_the_oracle(ctx, n, mass_type):
+    nsteps = 16
+    mats, mat_of = column_case(n, 101 + n, mass_type=mass_type, body=(0.0, 0.0, -1e3) if mass_type == 0 else (0, 0, 0))
+    ag = S.ground_motion(nsteps, DT, 7, pga=4.0)
+    res = run_pool(ctx, [(mats, mat_of)], nsteps, ag)[0]
+    assert res["status"]["status"] == 1 and res["status"]["steps"] == nsteps, res["status"]
+    model = column_model(n, len(mats), mat_of)
+    pm = core.Mesh(model)
+    om = fxo.OracleMesh(model)
+    per_set = [mats[g] for g in pm.param_group]
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    N = om.neq
+    u, v, a, du = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+    gd = np.zeros(N)
+    q = om.node_eq.reshape(-1, 3)[:, 0]
+    gd[q[q >= 0]] = 1.0
+    hu, hv, ha = [], [], []
+    for k in range(nsteps):
+        rc, st = om.newmark(u, v, a, du, DT, 1, pimy=pe, pimy_state=[Cs, Ts], ground_dir=gd, ground_accel=ag[k:k + 1])
+        assert rc == 0 and st["steps"] == 1
+        hu.append(level_values(pm, u, n)[-1])
+        hv.append(level_values(pm, v, n)[-1])
+        ha.append(level_values(pm, a, n)[-1])
+    compare(res, np.array(hu), np.array(hv), np.array(ha), level_values(pm, u, n), level_values(pm, v, n),
+            level_values(pm, a, n), f"oracle n={n}")
+    # the committed histories of the column's points against Gauss point 0 of each element
+    elem_of_level = np.argsort(pm.elem_order)  # model element k -> batch element
+    want = Cs[8 * elem_of_level]
+    err = history_error(res["state"], want)
+    print("history error", err)
+    assert err <= 1e-10
+    assert np.max(res["state"][:, 26]) > 0, "the record drives the column into the plastic range"
+
+
+@pytest.mark.parametrize("n", [40, 100, 200, 330])
+def test_columns_match_the_per_model_gpu_path(ctx, n):
+    nsteps = 12
+    mats, mat_of = column_case(n, 500 + n, nlayer=4)
+    ag = S.ground_motion(nsteps, DT, 11, pga=3.0)
+    res = run_pool(ctx, [(mats, mat_of)], nsteps, ag)[0]
+    assert res["status"]["status"] == 1, res["status"]
+    model = column_model(n, len(mats), mat_of)
+    pm = core.Mesh(model)
+    per_set = [mats[g] for g in pm.param_group]
+    keep = []
+    batch = core.Batch.pimy(ctx, pm, dev_mats(per_set, keep))
+    batch.set_stage(1)
+    d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+    for x in d:
+        x.zero()
+    hu, hv, ha = [], [], []
+    newton = 0
+    for k in range(nsteps):
+        r = batch.newmark(*d, DT, 1, ground_dir=0, ground_accel=ag[k:k + 1])
+        assert r["failed"] == 0 and r["steps"] == 1
+        newton += r["newton_iterations"]
+        u, v, a = (level_values(pm, x.get(), n) for x in d[:3])
+        hu.append(u[-1])
+        hv.append(v[-1])
+        ha.append(a[-1])
+    compare(res, np.array(hu), np.array(hv), np.array(ha), u, v, a, f"gpu n={n}")
+    assert abs(res["status"]["newton_iterations"] - newton) <= nsteps // 4 + 1, (res["status"], newton)
+
+
+def test_a_column_does_not_depend_on_the_others(ctx):
+    nsteps = 20
+    ag = S.ground_motion(nsteps, DT, 13, pga=3.5)
+    target = column_case(75, 901)
+    alone = run_pool(ctx, [target], nsteps, ag)[0]
+    others = [column_case(n, 1000 + n) for n in (30, 150, 260, 8, 64)]
+    pool = core.ColumnPool(ctx, max_elements=2000, max_columns=12, dt=DT, max_record=20000)
+    keep = []
+    ids = [pool.add(dev_mats(m, keep), mo, nsteps, ag) for m, mo in others[:3]]
+    pool.run(7)  # the target joins while the others are mid-analysis, in another slot
+    pool.remove(ids[1])
+    ids.append(pool.add(dev_mats(others[3][0], keep), others[3][1], nsteps, ag))
+    tid = pool.add(dev_mats(target[0], keep), target[1], nsteps, ag)
+    ids.append(pool.add(dev_mats(others[4][0], keep), others[4][1], nsteps, ag))
+    while pool.run(3):
+        pass
+    mixed = dict(pool.fetch(tid), status=pool.status(tid), state=pool.state(tid))
+    pool.close()
+    for key in ("u", "v", "a", "history", "state"):
+        np.testing.assert_array_equal(mixed[key], alone[key], err_msg=key)
+    assert mixed["status"] == alone["status"]
+
+
+def test_admission_limits_and_reuse(ctx):
+    pool = core.ColumnPool(ctx, max_elements=100, max_columns=3, dt=DT, max_record=1000)
+    keep = []
+    ag = np.zeros(5)
+    m, mo = column_case(80, 3)
+    a = pool.add(dev_mats(m, keep), mo, 5, ag)
+    assert a is not None
+    m2, mo2 = column_case(30, 4)
+    assert pool.add(dev_mats(m2, keep), mo2, 5, ag) is None  # 80 + 30 > 100 element slots
+    pool.remove(a)
+    b = pool.add(dev_mats(m2, keep), mo2, 5, ag)
+    assert b is not None
+    with pytest.raises(core.FxError):
+        pool.add(dev_mats(m2, keep), np.full(30, 7), 5, ag)  # parameter set out of range
+    with pytest.raises(core.FxError):
+        big, mob = column_case(core.FX_COLUMN_MAX_ELEMS + 1, 5)
+        pool.add(dev_mats(big, keep), mob, 5, ag)
+    while pool.run(4):
+        pass
+    st = pool.status(b)
+    assert st["status"] == 1 and st["steps"] == 5
+    r = pool.fetch(b)
+    assert np.all(r["u"] == 0.0)  # no load, no motion
+    pool.close()
+
+
+def test_a_step_that_does_not_converge_fails_and_keeps_its_start(ctx):
+    nsteps = 4
+    ag = S.ground_motion(nsteps, DT, 17, pga=3.0)
+    case = column_case(20, 77)
+    res = run_pool(ctx, [case], nsteps, ag, max_iter=1, tol=1e-300)[0]
+    st = res["status"]
+    assert st["status"] == 2 and st["steps"] == 0 and st["newton_iterations"] == 1, st
+    assert np.all(res["u"] == 0.0) and np.all(res["v"] == 0.0) and np.all(res["a"] == 0.0)
