@@ -127,6 +127,166 @@ def bench_c3(core, S, ctx, nsteps):
             "note": "force on the lattice operator kernels + fused cd_solve update per step; dashpots, Ricker source"}
 
 
+def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cpu=True, plain=True):
+    """C4: ncol PressureIndependMultiYield columns 1 x 1 x N (N ~ U{20..400}, 10 layers each),
+    Newmark (gamma 0.5, beta 0.25, dt 0.005 s) + Newton (NormDispIncr 1e-10) + PCG (1e-12),
+    one band-limited base record at 0.3 g. All columns admitted at once into one pool and run to
+    `nsteps` steps; timed from the first iteration to the last column's end."""
+    cols, rec = S.config_c4(ncol=ncol, nsteps=max(nsteps, 150))
+    total = sum(c["nelem"] for c in cols)
+    pool = core.ColumnPool(ctx, max_elements=total + 1024, max_columns=ncol, dt=0.005, max_materials=10 * ncol,
+                           max_record=ncol * 10 * nsteps + 1024)
+    keep = []
+    t0 = time.perf_counter()
+    ids = [pool.add(core.pimy_materials(c["mats"], keep), c["mat_of"], nsteps, rec, record=True) for c in cols]
+    print(f"[bench] C4 {ncol} columns, {total} elements admitted in {time.perf_counter() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    ctx.sync()
+    per_call = 8
+    t0 = time.perf_counter()
+    calls = 0
+    while pool.run(per_call):
+        calls += 1
+    calls += 1
+    sec = time.perf_counter() - t0
+    st = [pool.status(i) for i in ids]
+    steps = sum(s["steps"] for s in st)
+    newton = sum(s["newton_iterations"] for s in st)
+    krylov = sum(s["krylov_iterations"] for s in st)
+    out = {"columns": ncol, "elements": total, "equations": 3 * total, "steps_per_column": nsteps,
+           "failed_columns": sum(s["status"] == 2 for s in st), "seconds": round(sec, 3),
+           "column_steps_per_s": round(steps / sec, 1), "pool_iterations": calls * per_call,
+           "ms_per_pool_iteration": round(sec / (calls * per_call) * 1e3, 4),
+           "newton_per_step": round(newton / max(steps, 1), 3), "cg_per_newton": round(krylov / max(newton, 1), 2),
+           "state_bytes_per_iteration": total * 2 * 190 * 8,
+           "note": "one material point per tied element (uniform strain), return map + per-column PCG per "
+                   "Newton iteration, hipGraph replay"}
+    pool.close()
+    if not plain:
+        return out
+    # the plain per-model FP64 path (fx_newmark on one stdBrick<PIMY> batch per column), sampled
+    t_plain, n_plain = 0.0, 0
+    for c in cols[:plain_columns]:
+        n = c["nelem"]
+        model = S.soil_column(n, 1.0, params=[dict(E=1e8, nu=0.3, rho=2000.0)] * 10, layer_of=lambda k: c["mat_of"][k])
+        pm = core.Mesh(model)
+        batch = core.Batch.pimy(ctx, pm, core.pimy_materials([c["mats"][g] for g in pm.param_group], keep))
+        batch.set_stage(1)
+        d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+        for x in d:
+            x.zero()
+        ctx.sync()
+        t1 = time.perf_counter()
+        r = batch.newmark(*d, 0.005, plain_steps, ground_dir=0, ground_accel=rec[:plain_steps])
+        t_plain += time.perf_counter() - t1
+        n_plain += r["steps"]
+        batch.close()
+    out["plain_fp64_per_model"] = {"columns": plain_columns, "steps": plain_steps,
+                                   "column_steps_per_s": round(n_plain / t_plain, 2)}
+    out["speedup_vs_plain"] = round(out["column_steps_per_s"] / (n_plain / t_plain), 1)
+    if cpu:
+        out["cpu_baseline"] = c4_cpu_baseline(core, S, cols[:8], rec)
+    return out
+
+
+def bench_c5(core, S, duration=60.0, rate=200.0, burst=2000, plain_columns=2, plain_blocks=4, plain_steps=50,
+             log=None):
+    """C5: Poisson arrivals (rate per s, `duration` s): 70% soil columns (as C4, 500 steps), 30%
+    static blocks n^3 (n ~ U{8..48}, C1 loads, rtol 1e-8); HBM budget 200 GB; served with
+    continuous batching in the mixed-precision form (femorax_amd/serve.py). Also: the same
+    server's capacity on a burst of `burst` requests arriving at once, and the plain FP64
+    per-solve path (one request at a time: fx_newmark per column model, fx_cg per block) on a
+    sample of the same requests, as requests/s at its mean service time."""
+    from femorax_amd import serve
+
+    reqs = serve.mixed_requests(duration=duration, rate=rate)
+    srv = serve.Server(hbm_budget=200e9, mixed=True)
+    try:
+        recs = srv.run(reqs, log=log)
+        served = serve.summary(recs)
+        burst_reqs = serve.mixed_requests(duration=1.0, rate=float(burst), seed=S.SEED_BASE + 5)
+        for r in burst_reqs:
+            r.arrival = 0.0
+        cap = serve.summary(srv.run(burst_reqs, log=log))
+    finally:
+        srv.close()
+    # the plain FP64 per-solve path on a sample of the requests
+    ctx = core.Context(0)
+    cols = [r for r in reqs if r.kind == "column"][:plain_columns]
+    blks = [r for r in reqs if r.kind == "block"][:plain_blocks]
+    t_col = []
+    for r in cols:
+        model = S.soil_column(r.n, 1.0, params=[dict(E=1e8, nu=0.3, rho=2000.0)] * 10,
+                              layer_of=lambda k, mo=r.mat_of: mo[k])
+        pm = core.Mesh(model)
+        batch = core.Batch.pimy(ctx, pm, core.pimy_materials([r.mats[g] for g in pm.param_group]))
+        batch.set_stage(1)
+        d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+        for x in d:
+            x.zero()
+        ctx.sync()
+        t0 = time.perf_counter()
+        batch.newmark(*d, 0.005, plain_steps, ground_dir=0, ground_accel=r.record[:plain_steps])
+        t_col.append((time.perf_counter() - t0) * r.nsteps / plain_steps)
+        batch.close()
+    t_blk = []
+    problems = serve.BlockProblems()
+    for r in blks:
+        model, mesh, rhs = problems.get(r.n)
+        batch = core.Batch(ctx, mesh, mesh.params_for(model))
+        b = core.DeviceArray.from_host(ctx, rhs)
+        x = core.DeviceArray(ctx, mesh.neq)
+        x.zero()
+        ctx.sync()
+        t0 = time.perf_counter()
+        batch.cg(b, x, rtol=1e-8)
+        t_blk.append(time.perf_counter() - t0)
+        batch.close()
+    ctx.close()
+    frac = sum(r.kind == "column" for r in reqs) / len(reqs)
+    mean_service = frac * float(np.mean(t_col)) + (1.0 - frac) * float(np.mean(t_blk))
+    plain = {"solves_per_s": round(1.0 / mean_service, 3), "column_s": round(float(np.mean(t_col)), 3),
+             "block_ms": round(float(np.mean(t_blk)) * 1e3, 2),
+             "sample": f"{len(cols)} columns ({plain_steps} of {cols[0].nsteps} steps, scaled), {len(blks)} blocks"}
+    return {"served": served, "capacity_burst": cap, "plain_fp64_per_solve": plain,
+            "speedup_served_vs_plain": round(served["solves_per_s"] / plain["solves_per_s"], 1),
+            "capacity_vs_plain": round(cap["solves_per_s"] / plain["solves_per_s"], 1),
+            "config": {"rate_per_s": rate, "duration_s": duration, "columns": "70%, N~U{20..400}, 500 steps",
+                       "blocks": "30%, n~U{8..48}, rtol 1e-8", "hbm_budget_GB": 200, "block_workers": 3,
+                       "precision": "blocks: FP32 inner CG + FP64 refinement; columns: FP64"}}
+
+
+def c4_cpu_baseline(core, S, cols, rec, steps=150):
+    """The oracle's implicit stage (eight Gauss points per element, the reference's algorithm)
+    on a sample of C4 columns, one after another, on the host cores."""
+    from oracle import fxo
+
+    cores = os.cpu_count() or 1
+    sec, done, elems = 0.0, 0, 0
+    for col in cols:
+        n = col["nelem"]
+        model = S.soil_column(n, 1.0, params=[dict(E=1e8, nu=0.3, rho=2000.0)] * 10,
+                              layer_of=lambda k: col["mat_of"][k])
+        om = fxo.OracleMesh(model)
+        pm = core.Mesh(model)
+        pe = fxo.pimy_elems([col["mats"][g] for g in pm.param_group], stage=1)
+        Cs = om.pimy_initial_state(pe)
+        Ts = Cs.copy()
+        N = om.neq
+        u, v, a, du = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+        gd = np.zeros(N)
+        q = om.node_eq.reshape(-1, 3)[:, 0]
+        gd[q[q >= 0]] = 1.0
+        t0 = time.perf_counter()
+        _, st = om.newmark(u, v, a, du, 0.005, steps, pimy=pe, pimy_state=[Cs, Ts], ground_dir=gd,
+                           ground_accel=rec[:steps], nthreads=cores)
+        sec += time.perf_counter() - t0
+        done += st["steps"]
+        elems += n
+    return {"value": round(done / sec, 2), "unit": "column_steps/s", "cores": cores, "kind": "port",
+            "sample": f"{len(cols)} columns ({elems} elements) x {steps} steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -136,6 +296,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time full plain-FP64 and mixed-precision solves")
     ap.add_argument("--c1", action="store_true", help="also time C1 (16^3 static box) solves: solves/s, latency")
+    ap.add_argument("--c4", type=int, default=0, metavar="STEPS",
+                    help="also time STEPS Newmark steps of C4 (4096 PIMY soil columns) and the plain per-model path")
+    ap.add_argument("--c4-columns", type=int, default=4096)
+    ap.add_argument("--c5", type=float, default=0.0, metavar="SECONDS",
+                    help="also serve C5 (Poisson 200/s mixed columns and blocks) for SECONDS of arrivals")
     ap.add_argument("--c3", type=int, default=0, metavar="STEPS",
                     help="also time STEPS explicit CentralDifference steps of C3 (200x200x100, h 5 m)")
     args = ap.parse_args()
@@ -230,6 +395,10 @@ def main():
         out["c1_static_solves"] = bench_c1(core, S, ctx)
     if args.c3:
         out["c3_explicit"] = bench_c3(core, S, ctx, args.c3)
+    if args.c5:
+        out["c5_serving"] = bench_c5(core, S, duration=args.c5, log=lambda s: print(s, file=sys.stderr, flush=True))
+    if args.c4:
+        out["c4_columns"] = bench_c4(core, S, ctx, args.c4, ncol=args.c4_columns, cpu=not args.no_cpu)
     if not args.no_cpu:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
         out["cpu_baseline"] = cpu_baseline(model, params, x)

@@ -137,6 +137,14 @@ def pimy_material(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf=20, gre
     return PimyMaterial(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf, g, (C.c_double * 3)(*b), mass_type)
 
 
+def pimy_materials(dicts, keep=None):
+    """fx_pimy_material list from dicts (rho G K cohesion gamma_peak phi pref pdc [nsurf gredu b
+    mass_type])."""
+    return [pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"],
+                          nsurf=m.get("nsurf", 20), gredu=m.get("gredu"), b=m.get("b", (0.0, 0.0, 0.0)),
+                          mass_type=m.get("mass_type", 0), keep=keep) for m in dicts]
+
+
 def pimy_surfaces(m):
     size, modulus = np.zeros(21), np.zeros(21)
     _check(lib().fx_pimy_surfaces(C.byref(m), size.ctypes.data, modulus.ctypes.data))
@@ -214,6 +222,7 @@ SIGNATURES = [
     ("fx_column_add", C.c_int, [vp, C.POINTER(ColumnDesc), C.POINTER(i32)]),
     ("fx_column_run", C.c_int, [vp, i32, C.POINTER(i32)]),
     ("fx_column_status", C.c_int, [vp, i32, C.POINTER(ColumnResult)]),
+    ("fx_column_finished", C.c_int, [vp, vp, i32, C.POINTER(i32)]),
     ("fx_column_fetch", C.c_int, [vp, i32, vp, vp, vp, vp]),
     ("fx_column_state", C.c_int, [vp, i32, vp]),
     ("fx_column_remove", C.c_int, [vp, i32]),
@@ -642,6 +651,14 @@ class ColumnPool:
         n = C.c_int32()
         _check(lib().fx_column_run(self.h, int(iterations), C.byref(n)), self.ctx)
         return n.value
+
+    def finished(self):
+        """Ids of the columns finished or failed at the end of the last run (not removed)."""
+        cap = max(1, len(self.columns))
+        ids = np.zeros(cap, np.int32)
+        n = C.c_int32()
+        _check(lib().fx_column_finished(self.h, ids.ctypes.data, cap, C.byref(n)), self.ctx)
+        return [int(i) for i in ids[:min(n.value, cap)]]
 
     def status(self, cid):
         r = ColumnResult()

@@ -21,14 +21,16 @@
 //  - column_force_kernel: one thread per element slot of the pool: strain from the level
 //    displacements, the return map (committed history of the column's current buffer, trial
 //    history into the other), F and S written per element;
-//  - column_solve_kernel<WAVES>: one workgroup of 64 WAVES threads per column of at most
+//  - column_solve_kernel<WAVES, MIXED>: one workgroup of 64 WAVES threads per column of at most
 //    64 WAVES elements, one level per thread. The column's own Newmark/Newton state machine:
 //    unbalance B = -f - M a - a_g M r; the test (NormUnbalance ||B||, or NormDispIncr ||dU||
 //    of the last solve); on convergence the commit (buffer swap), the recorded top-level values
 //    and the next step's predictor; otherwise PCG + Jacobi on K_eff = K_t + c3 M (block
 //    tridiagonal, 3 x 3 blocks held in registers; neighbour values through LDS), warm-started
-//    from the previous increment, and the update. Workgroup sums are wave butterflies then a
-//    fixed-order sum over waves: a column's bits do not depend on the other columns.
+//    from the previous increment, and the update. MIXED: the same solve as FP32 PCG (FP32
+//    blocks, FP32 sums) inside FP64 iterative refinement, stopped on the FP64 true residual.
+//    Workgroup sums are wave butterflies then a fixed-order sum over waves: a column's bits
+//    do not depend on the other columns.
 // The pair is captured once as a hipGraph and replayed.
 #pragma clang fp contract(off)
 
@@ -66,7 +68,7 @@ struct PoolArgs {
   int64_t cap;                  // element slots
   int32_t max_columns;
   int32_t test, max_iter, cg_max_it;
-  double tol, cg_rtol;
+  double tol, cg_rtol, inner_rtol;
   double a1, a2, a3, a4;        // predictor constants
   double c2, c3;                // update constants gamma / (beta h), 1 / (beta h^2)
   ColumnDev* cols;
@@ -121,24 +123,27 @@ __global__ __launch_bounds__(256) void column_force_kernel(PoolArgs a) {
 
 // ---- per-column solve ---------------------------------------------------------------------------
 
+template <typename T>
 struct Sym3 {  // xx xy xz yy yz zz
-  double m[6];
-  __device__ void mul_add(const double (&x)[3], double (&y)[3]) const {
+  T m[6];
+  __device__ void mul_add(const T (&x)[3], T (&y)[3]) const {
     y[0] += m[0] * x[0] + m[1] * x[1] + m[2] * x[2];
     y[1] += m[1] * x[0] + m[3] * x[1] + m[4] * x[2];
     y[2] += m[2] * x[0] + m[4] * x[1] + m[5] * x[2];
   }
 };
 
-__device__ inline double wave_sum(double x) {
+template <typename T>
+__device__ inline T wave_sum(T x) {
   for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
   return x;
 }
 
-template <int WAVES>
+// Workgroup sums of two values: butterflies within each wave, then the wave sums in wave order.
+template <int WAVES, typename T>
 struct Reducer {
-  double* red;  // LDS [2 * WAVES]
-  __device__ void sum2(double& x, double& y) const {
+  T* red;  // LDS [2 * WAVES]
+  __device__ void sum2(T& x, T& y) const {
     x = wave_sum(x);
     y = wave_sum(y);
     if constexpr (WAVES > 1) {
@@ -158,11 +163,45 @@ struct Reducer {
   }
 };
 
-template <int WAVES>
+// Neighbouring levels' values of a level vector (one level per thread) through LDS.
+template <typename T, int NT>
+struct Exchange {
+  T (*xs)[NT];
+  int t, N;
+  bool live;
+  __device__ void operator()(const T (&x)[3], T (&xm)[3], T (&xp)[3]) const {
+    __syncthreads();  // the previous exchange's reads are done
+    for (int d = 0; d < 3; ++d) xs[d][t] = x[d];
+    __syncthreads();
+    for (int d = 0; d < 3; ++d) {
+      xm[d] = (live && t > 0) ? xs[d][t - 1] : T(0);
+      xp[d] = (live && t + 1 < N) ? xs[d][t + 1] : T(0);
+    }
+  }
+};
+
+// y = K x on the column's block-tridiagonal operator (blocks of this thread's level).
+template <typename T, int NT>
+__device__ inline void column_apply(const Exchange<T, NT>& ex, const Sym3<T>& Klo, const Sym3<T>& Kd,
+                                    const Sym3<T>& Khi, const T (&x)[3], T (&y)[3]) {
+  T xm[3], xp[3];
+  ex(x, xm, xp);
+  for (int d = 0; d < 3; ++d) y[d] = T(0);
+  if (!ex.live) return;
+  Klo.mul_add(xm, y);
+  Kd.mul_add(x, y);
+  Khi.mul_add(xp, y);
+}
+
+constexpr int kMaxRefinements = 50;
+
+template <int WAVES, bool MIXED>
 __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   constexpr int NT = 64 * WAVES;
   __shared__ double xs[3][NT];
+  __shared__ float xs32[3][NT];
   __shared__ double redA[2 * WAVES], redB[2 * WAVES], redC[2 * WAVES];
+  __shared__ float redD[2 * WAVES], redE[2 * WAVES];
   const int32_t c = blockIdx.x;
   ColumnDev& col = a.cols[c];
   const int32_t status = col.status, N = col.nelem;
@@ -172,16 +211,8 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   const int64_t e = col.e0 + t;  // the element below this level, whose top the level is
   const int64_t P = a.cap;
   const double V = col.hx * col.hy * col.hz;
-  const Reducer<WAVES> RA{redA}, RB{redB}, RC{redC};
-  auto neighbours = [&](const double (&x)[3], double (&xm)[3], double (&xp)[3]) {
-    __syncthreads();  // the previous exchange's reads are done
-    for (int d = 0; d < 3; ++d) xs[d][t] = x[d];
-    __syncthreads();
-    for (int d = 0; d < 3; ++d) {
-      xm[d] = (live && t > 0) ? xs[d][t - 1] : 0.0;
-      xp[d] = (live && t + 1 < N) ? xs[d][t + 1] : 0.0;
-    }
-  };
+  const Reducer<WAVES, double> RA{redA}, RB{redB}, RC{redC};
+  const Exchange<double, NT> neighbours{xs, t, N, live};
   // mass of this level: diagonal and couplings (lumped: no coupling)
   double m = 0.0, mlo = 0.0, mhi = 0.0, body[3] = {0, 0, 0};
   const bool above = live && t + 1 < N;
@@ -280,7 +311,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   }
   // K_eff blocks: diagonal Kd = S_e + S_{e+1} + c3 m I; couplings Klo (to the level below)
   // = -S_e + c3 mlo I, Khi (to the level above) = -S_{e+1} + c3 mhi I
-  Sym3 Kd{}, Klo{}, Khi{};
+  Sym3<double> Kd{}, Klo{}, Khi{};
   double inv[3] = {0, 0, 0};
   if (live) {
     for (int w = 0; w < 6; ++w) {
@@ -298,63 +329,116 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     }
     for (int d = 0; d < 3; ++d) inv[d] = 1.0 / Kd.m[diag[d]];
   }
-  auto apply = [&](const double (&x)[3], double (&y)[3]) {
-    double xm[3], xp[3];
-    neighbours(x, xm, xp);
-    for (int d = 0; d < 3; ++d) y[d] = 0.0;
-    if (!live) return;
-    Klo.mul_add(xm, y);
-    Kd.mul_add(x, y);
-    Khi.mul_add(xp, y);
-  };
-  // PCG (cg.cpp:24-72): warm start from the previous increment, stop on the recursive residual
-  double x[3] = {0, 0, 0}, r[3], z[3], p[3], Ap[3];
+  auto apply = [&](const double (&x)[3], double (&y)[3]) { column_apply(neighbours, Klo, Kd, Khi, x, y); };
+  double x[3] = {0, 0, 0}, r[3], Ap[3];
   if (live)
     for (int d = 0; d < 3; ++d) x[d] = a.du[3 * e + d];
-  apply(x, Ap);
-  double rr = 0.0, rz = 0.0;
-  for (int d = 0; d < 3; ++d) {
-    r[d] = B[d] - Ap[d];
-    z[d] = inv[d] * r[d];
-    p[d] = z[d];
-    rr += r[d] * r[d];
-    rz += r[d] * z[d];
-  }
-  RB.sum2(rr, rz);
   const double tol = a.cg_rtol * bnorm;
-  double rnorm = sqrt(rr);
+  double rnorm = 0.0;
   int its = 0;
-  bool converged = rnorm <= tol, indefinite = false;
-  while (!converged && its < a.cg_max_it) {
-    apply(p, Ap);
-    double pAp = 0.0, dummy = 0.0;
-    for (int d = 0; d < 3; ++d) pAp += p[d] * Ap[d];
-    RC.sum2(pAp, dummy);
-    if (!(pAp > 0.0)) {
-      indefinite = true;
-      break;
-    }
-    const double alpha = rz / pAp;
-    double rr2 = 0.0, rz2 = 0.0;
+  bool converged = false;
+  if constexpr (!MIXED) {
+    // PCG (cg.cpp:24-72): warm start from the previous increment, stop on the recursive residual
+    double z[3], p[3];
+    apply(x, Ap);
+    double rr = 0.0, rz = 0.0;
     for (int d = 0; d < 3; ++d) {
-      x[d] += alpha * p[d];
-      r[d] -= alpha * Ap[d];
+      r[d] = B[d] - Ap[d];
       z[d] = inv[d] * r[d];
-      rr2 += r[d] * r[d];
-      rz2 += r[d] * z[d];
+      p[d] = z[d];
+      rr += r[d] * r[d];
+      rz += r[d] * z[d];
     }
-    RB.sum2(rr2, rz2);
-    ++its;
-    rnorm = sqrt(rr2);
-    if (rnorm <= tol) {
-      converged = true;
-      break;
+    RB.sum2(rr, rz);
+    rnorm = sqrt(rr);
+    converged = rnorm <= tol;
+    while (!converged && its < a.cg_max_it) {
+      apply(p, Ap);
+      double pAp = 0.0, dummy = 0.0;
+      for (int d = 0; d < 3; ++d) pAp += p[d] * Ap[d];
+      RC.sum2(pAp, dummy);
+      if (!(pAp > 0.0)) break;  // indefinite
+      const double alpha = rz / pAp;
+      double rr2 = 0.0, rz2 = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        x[d] += alpha * p[d];
+        r[d] -= alpha * Ap[d];
+        z[d] = inv[d] * r[d];
+        rr2 += r[d] * r[d];
+        rz2 += r[d] * z[d];
+      }
+      RB.sum2(rr2, rz2);
+      ++its;
+      rnorm = sqrt(rr2);
+      if (rnorm <= tol) {
+        converged = true;
+        break;
+      }
+      const double beta = rz2 / rz;
+      rz = rz2;
+      for (int d = 0; d < 3; ++d) p[d] = z[d] + beta * p[d];
     }
-    const double beta = rz2 / rz;
-    rz = rz2;
-    for (int d = 0; d < 3; ++d) p[d] = z[d] + beta * p[d];
+  } else {
+    // FP64 iterative refinement around FP32 PCG: r = B - K x in FP64 (the true residual, the
+    // stop test), the correction K e = r solved on FP32 blocks with FP32 sums to inner_rtol
+    const Reducer<WAVES, float> RD{redD}, RE{redE};
+    const Exchange<float, NT> neighbours32{xs32, t, N, live};
+    Sym3<float> Klo32, Kd32, Khi32;
+    for (int w = 0; w < 6; ++w) {
+      Klo32.m[w] = static_cast<float>(Klo.m[w]);
+      Kd32.m[w] = static_cast<float>(Kd.m[w]);
+      Khi32.m[w] = static_cast<float>(Khi.m[w]);
+    }
+    float inv32[3];
+    for (int d = 0; d < 3; ++d) inv32[d] = live ? 1.0f / Kd32.m[d == 0 ? 0 : d == 1 ? 3 : 5] : 0.0f;
+    const float inner_rtol = static_cast<float>(a.inner_rtol);
+    for (int cycle = 0;; ++cycle) {
+      apply(x, Ap);
+      double rr = 0.0, unused_r = 0.0;
+      for (int d = 0; d < 3; ++d) {
+        r[d] = B[d] - Ap[d];
+        rr += r[d] * r[d];
+      }
+      RB.sum2(rr, unused_r);
+      rnorm = sqrt(rr);
+      converged = rnorm <= tol;
+      if (converged || its >= a.cg_max_it || cycle >= kMaxRefinements) break;
+      float rf[3], zf[3], pf[3], Apf[3], ef[3] = {0.f, 0.f, 0.f};
+      float rrf = 0.f, rzf = 0.f;
+      for (int d = 0; d < 3; ++d) {
+        rf[d] = static_cast<float>(r[d]);
+        zf[d] = inv32[d] * rf[d];
+        pf[d] = zf[d];
+        rrf += rf[d] * rf[d];
+        rzf += rf[d] * zf[d];
+      }
+      RD.sum2(rrf, rzf);
+      const float itol = inner_rtol * sqrtf(rrf);
+      while (its < a.cg_max_it) {
+        column_apply(neighbours32, Klo32, Kd32, Khi32, pf, Apf);
+        float pAp = 0.f, dummy = 0.f;
+        for (int d = 0; d < 3; ++d) pAp += pf[d] * Apf[d];
+        RE.sum2(pAp, dummy);
+        if (!(pAp > 0.f)) break;
+        const float alpha = rzf / pAp;
+        float rr2 = 0.f, rz2 = 0.f;
+        for (int d = 0; d < 3; ++d) {
+          ef[d] += alpha * pf[d];
+          rf[d] -= alpha * Apf[d];
+          zf[d] = inv32[d] * rf[d];
+          rr2 += rf[d] * rf[d];
+          rz2 += rf[d] * zf[d];
+        }
+        RD.sum2(rr2, rz2);
+        ++its;
+        if (sqrtf(rr2) <= itol) break;
+        const float beta = rz2 / rzf;
+        rzf = rz2;
+        for (int d = 0; d < 3; ++d) pf[d] = zf[d] + beta * pf[d];
+      }
+      for (int d = 0; d < 3; ++d) x[d] += static_cast<double>(ef[d]);
+    }
   }
-  (void)indefinite;
   const bool accept = converged || rnorm <= 1e-6 * bnorm;
   if (!accept) {  // the solve failed: restore the step's start (implicit_stage.cpp:226-238)
     if (live)
@@ -410,10 +494,14 @@ __global__ void column_init_kernel(int64_t cap, int64_t e0, int32_t n, const Pim
   }
 }
 
-__global__ void column_count_kernel(const ColumnDev* cols, int32_t n, int32_t* out) {
+__global__ void column_count_kernel(const ColumnDev* cols, int32_t n, int32_t* out, int8_t* status) {
   __shared__ int32_t part[256];
   int32_t k = 0;
-  for (int32_t c = threadIdx.x; c < n; c += blockDim.x) k += cols[c].status == kRunning ? 1 : 0;
+  for (int32_t c = threadIdx.x; c < n; c += blockDim.x) {
+    const int32_t s = cols[c].status;
+    status[c] = static_cast<int8_t>(s);
+    k += s == kRunning ? 1 : 0;
+  }
   part[threadIdx.x] = k;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
@@ -479,8 +567,8 @@ struct fx_column_pool {
   uint64_t bytes = 0;
   fx::Ranges elems, mats, recs;
   std::vector<fx::HostColumn> host;
-  int32_t* count = nullptr;       // device
-  int32_t* count_host = nullptr;  // pinned
+  int32_t* count = nullptr;       // device: running columns, then the status byte of each column
+  int32_t* count_host = nullptr;  // pinned copy of the same
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -513,10 +601,17 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
   const PoolArgs& a = p->args;
   hipLaunchKernelGGL(column_force_kernel, dim3(static_cast<unsigned>((a.cap + 255) / 256)), dim3(256), 0, st, a);
   const unsigned nc = static_cast<unsigned>(a.max_columns);
-  hipLaunchKernelGGL(column_solve_kernel<1>, dim3(nc), dim3(64), 0, st, a);
-  hipLaunchKernelGGL(column_solve_kernel<2>, dim3(nc), dim3(128), 0, st, a);
-  hipLaunchKernelGGL(column_solve_kernel<4>, dim3(nc), dim3(256), 0, st, a);
-  hipLaunchKernelGGL(column_solve_kernel<8>, dim3(nc), dim3(512), 0, st, a);
+  if (p->d.flags & FX_COLUMN_MIXED) {
+    hipLaunchKernelGGL((column_solve_kernel<1, true>), dim3(nc), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<2, true>), dim3(nc), dim3(128), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<4, true>), dim3(nc), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<8, true>), dim3(nc), dim3(512), 0, st, a);
+  } else {
+    hipLaunchKernelGGL((column_solve_kernel<1, false>), dim3(nc), dim3(64), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<2, false>), dim3(nc), dim3(128), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<4, false>), dim3(nc), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((column_solve_kernel<8, false>), dim3(nc), dim3(512), 0, st, a);
+  }
   FX_HIP(p->ctx, hipGetLastError());
   return FX_OK;
 }
@@ -546,7 +641,6 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   if (d->max_elements <= 0 || d->max_columns <= 0 || d->max_materials <= 0 || d->max_record <= 0 ||
       !(d->dt > 0.0) || !(d->beta > 0.0) || d->max_iter <= 0 || d->cg_max_it <= 0 || (d->test != 0 && d->test != 1))
     return fail(ctx, FX_EINVAL, "fx_column_pool_create: bad descriptor");
-  if (d->flags & FX_COLUMN_MIXED) return fail(ctx, FX_EUNSUP, "fx_column_pool_create: FX_COLUMN_MIXED is not built");
   FX_HIP(ctx, hipSetDevice(ctx->device));
   fx_column_pool* p = new (std::nothrow) fx_column_pool;
   if (!p) return fail(ctx, FX_ENOMEM, "host allocation failed");
@@ -560,6 +654,7 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   a.cg_max_it = d->cg_max_it;
   a.tol = d->tol;
   a.cg_rtol = d->cg_rtol;
+  a.inner_rtol = d->inner_rtol > 0.0 ? d->inner_rtol : 1e-5;
   const double gamma = d->gamma, beta = d->beta, h = d->dt;
   a.a1 = 1.0 - gamma / beta;
   a.a2 = h * (1.0 - 0.5 * gamma / beta);
@@ -579,11 +674,13 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
       (rc = pool_alloc(p, a.u, 3 * cap)) || (rc = pool_alloc(p, a.v, 3 * cap)) || (rc = pool_alloc(p, a.a, 3 * cap)) ||
       (rc = pool_alloc(p, a.ut, 3 * cap)) || (rc = pool_alloc(p, a.vt, 3 * cap)) ||
       (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) ||
-      (rc = pool_alloc(p, rec, static_cast<size_t>(d->max_record))) || (rc = pool_alloc(p, p->count, 1))) {
+      (rc = pool_alloc(p, rec, static_cast<size_t>(d->max_record))) ||
+      (rc = pool_alloc(p, p->count, 1 + (static_cast<size_t>(d->max_columns) + 3) / 4))) {
     fx_column_pool_destroy(p);
     return rc;
   }
-  if (hipHostMalloc(reinterpret_cast<void**>(&p->count_host), sizeof(int32_t)) != hipSuccess) {
+  if (hipHostMalloc(reinterpret_cast<void**>(&p->count_host),
+                    sizeof(int32_t) * (1 + (static_cast<size_t>(d->max_columns) + 3) / 4)) != hipSuccess) {
     fx_column_pool_destroy(p);
     return fail(ctx, FX_EDEVICE, "hipHostMalloc failed");
   }
@@ -714,11 +811,27 @@ int fx_column_run(fx_column_pool* p, int32_t iterations, int32_t* running) {
     if (p->exec) FX_HIP(ctx, hipGraphLaunch(p->exec, st));
     else if (int rc = launch_iteration(p, st)) return rc;
   }
-  hipLaunchKernelGGL(column_count_kernel, dim3(1), dim3(256), 0, st, p->args.cols, p->args.max_columns, p->count);
+  hipLaunchKernelGGL(column_count_kernel, dim3(1), dim3(256), 0, st, p->args.cols, p->args.max_columns, p->count,
+                     reinterpret_cast<int8_t*>(p->count + 1));
   FX_HIP(ctx, hipGetLastError());
-  FX_HIP(ctx, hipMemcpyAsync(p->count_host, p->count, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+  FX_HIP(ctx, hipMemcpyAsync(p->count_host, p->count, sizeof(int32_t) + static_cast<size_t>(p->args.max_columns),
+                             hipMemcpyDeviceToHost, st));
   FX_HIP(ctx, hipStreamSynchronize(st));
   if (running) *running = *p->count_host;
+  return FX_OK;
+}
+
+int fx_column_finished(fx_column_pool* p, int32_t* ids, int32_t cap, int32_t* count) {
+  using namespace fx;
+  if (!p || !count || (cap > 0 && !ids)) return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_finished: bad argument");
+  const int8_t* s = reinterpret_cast<const int8_t*>(p->count_host + 1);
+  int32_t k = 0;
+  for (int32_t c = 0; c < p->d.max_columns; ++c)
+    if (p->host[c].used && (s[c] == kFinished || s[c] == kFailed)) {
+      if (k < cap) ids[k] = c;
+      ++k;
+    }
+  *count = k;
   return FX_OK;
 }
 

@@ -156,8 +156,16 @@ struct Point {
 
   __device__ double c(int k) const { return C[k * s]; }
   __device__ double& t(int k) const { return T[k * s]; }
-  __device__ double size(int i) const { return i == 0 ? 0. : t(kPSizes + i - 1); }
-  __device__ double modulus(int i) const { return i == 0 ? 0. : t(kPModuli + i - 1); }
+  // With phi = 0 the surfaces' sizes and moduli never change (paramScaling is a no-op,
+  // :633-639): they are read from the parameter set and their history fields are left as
+  // initialised (same values), so an update moves 40 doubles less per point.
+  __device__ bool fixed_surfaces() const { return p.phi == 0.; }
+  __device__ double size(int i) const {
+    return i == 0 ? 0. : fixed_surfaces() ? p.size[i] : t(kPSizes + i - 1);
+  }
+  __device__ double modulus(int i) const {
+    return i == 0 ? 0. : fixed_surfaces() ? p.modulus[i] : t(kPModuli + i - 1);
+  }
   __device__ V6 center(int i) const {
     V6 o = zero6();
     if (i == 0) return o;
@@ -460,7 +468,10 @@ struct Point {
       for (int k = 0; k < 6; ++k) d.v[k] = eps[k] - cs.v[k];
       rate.set(d, true);
     }
-    for (int k = kPSizes; k < kPimyNState; ++k) t(k) = c(k);  // committed surfaces into the trial buffer
+    // committed surfaces into the trial buffer (centres of the nsurf surfaces in use)
+    if (!fixed_surfaces())
+      for (int k = kPSizes; k < kPCenters; ++k) t(k) = c(k);
+    for (int k = kPCenters; k < kPCenters + 6 * p.nsurf; ++k) t(k) = c(k);
     if (p.load_stage == 1 && e2p == 0) elastic_to_plastic();
     if (p.load_stage != 1) {
       double De[36];

@@ -138,11 +138,16 @@ def nodal_to_equations(F, node_rank, node_eq, neq):
     return b
 
 
-def config_c1():
-    """16^3 static box: E 30 MPa, nu 0.3, 1 m cube, base fixed, 10 kPa top traction."""
-    m = elastic_box(16, 1.0, 30e6, 0.3, 2000.0)
+def config_block(n):
+    """n^3 static box: E 30 MPa, nu 0.3, 1 m cube, base fixed, 10 kPa top traction (the C1 loads)."""
+    m = elastic_box(n, 1.0, 30e6, 0.3, 2000.0)
     m["nodal_load"] = top_traction_nodal(m, 10e3)
     return m
+
+
+def config_c1():
+    """16^3 static box: E 30 MPa, nu 0.3, 1 m cube, base fixed, 10 kPa top traction."""
+    return config_block(16)
 
 
 def config_c2(n=128, seed=SEED_BASE + 1):

@@ -390,6 +390,9 @@ int fx_column_add(fx_column_pool* p, const fx_column_desc* d, int32_t* id);
  * *running: columns still running afterwards. Blocking. */
 int fx_column_run(fx_column_pool* p, int32_t iterations, int32_t* running);
 int fx_column_status(fx_column_pool* p, int32_t id, fx_column_result* r);
+/* Host only (no device call): the columns that had finished or failed at the end of the last
+ * fx_column_run and are not removed yet; up to cap ids, *count = how many there are. */
+int fx_column_finished(fx_column_pool* p, int32_t* ids, int32_t cap, int32_t* count);
 /* Host copies of a column: u, v, a [3 nelem] (level 1..N, x y z; null to skip) and the
  * recorded history [9 nsteps] (u, v, a of the top level per committed step; null to skip). */
 int fx_column_fetch(fx_column_pool* p, int32_t id, double* u, double* v, double* a, double* history);

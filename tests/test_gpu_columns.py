@@ -2,8 +2,9 @@
 PCG analysis against (a) the oracle's implicit stage on the same column model (eight Gauss
 points per element, the reference's numbering and ties), (b) the per-model GPU path
 (fx_newmark on a stdBrick<PressureIndependMultiYield> batch), for columns of every solve
-workgroup size; the independence of a column's result from the other columns in the pool
-(bit for bit, admitted at a different time and place); admission limits and the failure path.
+workgroup size; the mixed-precision solve (FP32 inner PCG in FP64 refinement) against the
+FP64 one; the independence of a column's result from the other columns in the pool (bit for
+bit, admitted at a different time and place); admission limits and the failure path.
 
 Tolerances: the pool evaluates one material point per element (the strain of a tied box
 element is uniform) and sums in its own order, so it agrees with the eight-point models to
@@ -55,11 +56,12 @@ def level_values(pm, x, n):
     return pm.to_nodes(x)[4 * np.arange(1, n + 1)]
 
 
-def run_pool(ctx, cases, nsteps, ag, max_iter=25, tol=1e-10, iters_per_call=4, pool=None):
+def run_pool(ctx, cases, nsteps, ag, max_iter=25, tol=1e-10, iters_per_call=4, pool=None, mixed=False):
     own = pool is None
     if own:
         pool = core.ColumnPool(ctx, max_elements=sum(len(mo) for _, mo in cases) + 64, max_columns=len(cases) + 2,
-                               dt=DT, max_record=10 * nsteps * (len(cases) + 1) + 16, max_iter=max_iter, tol=tol)
+                               dt=DT, max_record=10 * nsteps * (len(cases) + 1) + 16, max_iter=max_iter, tol=tol,
+                               mixed=mixed)
     keep = []
     ids = [pool.add(dev_mats(m, keep), mo, nsteps, ag) for m, mo in cases]
     assert all(i is not None for i in ids)
@@ -193,6 +195,7 @@ def test_admission_limits_and_reuse(ctx):
         pool.add(dev_mats(big, keep), mob, 5, ag)
     while pool.run(4):
         pass
+    assert pool.finished() == [b]
     st = pool.status(b)
     assert st["status"] == 1 and st["steps"] == 5
     r = pool.fetch(b)
@@ -208,3 +211,20 @@ def test_a_step_that_does_not_converge_fails_and_keeps_its_start(ctx):
     st = res["status"]
     assert st["status"] == 2 and st["steps"] == 0 and st["newton_iterations"] == 1, st
     assert np.all(res["u"] == 0.0) and np.all(res["v"] == 0.0) and np.all(res["a"] == 0.0)
+
+
+def test_mixed_precision_columns_match_fp64(ctx):
+    """FP32 inner PCG on FP32 blocks inside FP64 refinement to the same relative residual: the
+    error against the all-FP64 pool is stated per result (printed) and bounded by 1e-10."""
+    nsteps = 16
+    ag = S.ground_motion(nsteps, DT, 19, pga=3.5)
+    cases = [column_case(n, 300 + n) for n in (12, 90, 180, 400)]
+    f64 = run_pool(ctx, cases, nsteps, ag)
+    f32 = run_pool(ctx, cases, nsteps, ag, mixed=True)
+    for (m, mo), x, y in zip(cases, f64, f32):
+        assert y["status"]["status"] == 1 and y["status"]["steps"] == nsteps
+        errs = [rel(y[k], x[k]) for k in ("u", "v", "a", "history")] + [history_error(y["state"], x["state"])]
+        print(f"n={len(mo)} mixed vs FP64 (u v a history state):", ["%.2e" % e for e in errs],
+              "newton", y["status"]["newton_iterations"], x["status"]["newton_iterations"],
+              "cg", y["status"]["krylov_iterations"], x["status"]["krylov_iterations"])
+        assert max(errs) <= 1e-10, errs
