@@ -17,7 +17,8 @@
 // 1/6], [1/6, 1/3]] (the exact integral of the face sums (1 -+ zeta) / 2, which the 2-point
 // Gauss rule reproduces). Body load per element: V b / 2 at each level.
 //
-// Kernels, one pair per Newton iteration of every running column:
+// Kernels, one pair per Newton iteration of every running column (FX_COLUMN_COMPACT splits the
+// first into the reference's predict / compact / correct passes, below):
 //  - column_force_kernel: one thread per element slot of the pool: strain from the level
 //    displacements, the return map (committed history of the column's current buffer, trial
 //    history into the other), F and S written per element;
@@ -86,7 +87,14 @@ struct PoolArgs {
   double* du;
   const double* record;         // record arena
   double* rec_out;              // same arena, written (histories)
+  // compacted return map (FX_COLUMN_COMPACT): the plastic points' strains and list
+  double* ceps;                 // [3 * cap]: eps33, gamma23, gamma31
+  int32_t* plist;               // [cap] element slots
+  int32_t* pcount;              // [1]
 };
+
+__device__ inline void column_outputs(const PoolArgs& a, int64_t i, double A, double hz, const double (&sig)[6],
+                                      const double (&D)[36]);
 
 // ---- the return map, one thread per element slot ------------------------------------------------
 
@@ -110,6 +118,18 @@ __global__ __launch_bounds__(256) void column_force_kernel(PoolArgs a) {
   double* T = a.state + (col.cur ? 0 : NS) + i;
   double sig[6], D[36];
   pimy_update(a.params[a.elem_param[i]], C, T, a.cap, eps, sig, D);
+  column_outputs(a, i, A, hz, sig, D);
+}
+
+// ---- the compacted return map (compacted_kernels.hpp:85-202) -------------------------------------
+// 1. predict: one thread per element slot; a point that needs no stress correction is finished
+//    (F, S written); a plastic one keeps its strain and joins the list (one atomic per wave);
+// 2. correct: a fixed grid walks the list, one full update per plastic point (the same
+//    function as the fused kernel, so the same bits), F and S written;
+// 3. the list count is cleared for the next iteration (a kernel, not a memset: graph replays).
+
+__device__ inline void column_outputs(const PoolArgs& a, int64_t i, double A, double hz, const double (&sig)[6],
+                                      const double (&D)[36]) {
   const int64_t P = a.cap;
   a.eout[0 * P + i] = A * sig[5];
   a.eout[1 * P + i] = A * sig[4];
@@ -119,6 +139,62 @@ __global__ __launch_bounds__(256) void column_force_kernel(PoolArgs a) {
   int w = 3;
   for (int r = 0; r < 3; ++r)
     for (int q = r; q < 3; ++q, ++w) a.eout[w * P + i] = s * D[6 * idx[r] + idx[q]];
+}
+
+__global__ __launch_bounds__(256) void column_predict_kernel(PoolArgs a) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  bool plastic = false;
+  if (i < a.cap) {
+    const int32_t c = a.elem_col[i];
+    if (c >= 0 && a.cols[c].status == kRunning) {
+      const ColumnDev& col = a.cols[c];
+      const int64_t j = i - col.e0;
+      const double hz = col.hz, A = col.hx * col.hy;
+      double lo[3] = {0.0, 0.0, 0.0}, up[3];
+      for (int d = 0; d < 3; ++d) {
+        up[d] = a.u[3 * i + d];
+        if (j > 0) lo[d] = a.u[3 * (i - 1) + d];
+      }
+      double eps[6] = {0.0, 0.0, (up[2] - lo[2]) / hz, 0.0, (up[1] - lo[1]) / hz, (up[0] - lo[0]) / hz};
+      const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(a.cap);
+      double sig[6], D[36];
+      plastic = pimy_predict(a.params[a.elem_param[i]], a.state + (col.cur ? NS : 0) + i,
+                             a.state + (col.cur ? 0 : NS) + i, a.cap, eps, sig, D);
+      if (plastic) {
+        a.ceps[i] = eps[2];
+        a.ceps[a.cap + i] = eps[4];
+        a.ceps[2 * a.cap + i] = eps[5];
+      } else {
+        column_outputs(a, i, A, hz, sig, D);
+      }
+    }
+  }
+  const uint64_t mask = __ballot(plastic);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(a.pcount, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (plastic) a.plist[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = static_cast<int32_t>(i);
+}
+
+__global__ __launch_bounds__(256) void column_correct_kernel(PoolArgs a) {
+  const int32_t n = *a.pcount;
+  const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(a.cap);
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int64_t i = a.plist[k];
+    const ColumnDev& col = a.cols[a.elem_col[i]];
+    double eps[6] = {0.0, 0.0, a.ceps[i], 0.0, a.ceps[a.cap + i], a.ceps[2 * a.cap + i]};
+    double sig[6], D[36];
+    pimy_update(a.params[a.elem_param[i]], a.state + (col.cur ? NS : 0) + i, a.state + (col.cur ? 0 : NS) + i, a.cap,
+                eps, sig, D);
+    column_outputs(a, i, col.hx * col.hy, col.hz, sig, D);
+  }
+}
+
+__global__ void column_clear_count_kernel(int32_t* count) {
+  if (threadIdx.x == 0) count[0] = 0;
 }
 
 // ---- per-column solve ---------------------------------------------------------------------------
@@ -133,10 +209,40 @@ struct Sym3 {  // xx xy xz yy yz zz
   }
 };
 
+// DPP lane moves within rows of 16 lanes (quad_perm [1,0,3,2] and [2,3,0,1], row_half_mirror,
+// row_mirror): every source lane is in the same row, so all lanes are written.
+template <int CTRL>
+__device__ inline float dpp_move(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ inline double dpp_move(double x) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float lane_value(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+__device__ inline double lane_value(double x, int lane) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), lane),
+                          __builtin_amdgcn_readlane(__double2loint(x), lane));
+}
+
+// Sum over a full wave in a fixed order: pairs, quads, halves and rows of 16 by DPP (each step
+// adds two equal-shape partial sums, so every lane of a row ends with the same bits), then the
+// four row sums ((r0 + r1) + r2) + r3. All 64 lanes must be active.
 template <typename T>
 __device__ inline T wave_sum(T x) {
-  for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o, 64);
-  return x;
+  x += dpp_move<0xB1>(x);   // quad_perm [1,0,3,2]
+  x += dpp_move<0x4E>(x);   // quad_perm [2,3,0,1]
+  x += dpp_move<0x141>(x);  // row_half_mirror
+  x += dpp_move<0x140>(x);  // row_mirror
+  T s = lane_value(x, 0);
+  s += lane_value(x, 16);
+  s += lane_value(x, 32);
+  s += lane_value(x, 48);
+  return s;
 }
 
 // Workgroup sums of two values: butterflies within each wave, then the wave sums in wave order.
@@ -163,26 +269,30 @@ struct Reducer {
   }
 };
 
-// Neighbouring levels' values of a level vector (one level per thread) through LDS.
+// Neighbouring levels' values of a level vector (one level per thread) through LDS. Two
+// buffers in turn: exchange k + 2 writes the buffer exchange k read, and every thread has left
+// exchange k's reads before it passes exchange k + 1's barrier, so one barrier per exchange.
 template <typename T, int NT>
 struct Exchange {
-  T (*xs)[NT];
+  T (*xs)[3][NT];
   int t, N;
   bool live;
-  __device__ void operator()(const T (&x)[3], T (&xm)[3], T (&xp)[3]) const {
-    __syncthreads();  // the previous exchange's reads are done
-    for (int d = 0; d < 3; ++d) xs[d][t] = x[d];
+  int phase = 0;
+  __device__ void operator()(const T (&x)[3], T (&xm)[3], T (&xp)[3]) {
+    T (*b)[NT] = xs[phase];
+    phase ^= 1;
+    for (int d = 0; d < 3; ++d) b[d][t] = x[d];
     __syncthreads();
     for (int d = 0; d < 3; ++d) {
-      xm[d] = (live && t > 0) ? xs[d][t - 1] : T(0);
-      xp[d] = (live && t + 1 < N) ? xs[d][t + 1] : T(0);
+      xm[d] = (live && t > 0) ? b[d][t - 1] : T(0);
+      xp[d] = (live && t + 1 < N) ? b[d][t + 1] : T(0);
     }
   }
 };
 
 // y = K x on the column's block-tridiagonal operator (blocks of this thread's level).
 template <typename T, int NT>
-__device__ inline void column_apply(const Exchange<T, NT>& ex, const Sym3<T>& Klo, const Sym3<T>& Kd,
+__device__ inline void column_apply(Exchange<T, NT>& ex, const Sym3<T>& Klo, const Sym3<T>& Kd,
                                     const Sym3<T>& Khi, const T (&x)[3], T (&y)[3]) {
   T xm[3], xp[3];
   ex(x, xm, xp);
@@ -198,8 +308,8 @@ constexpr int kMaxRefinements = 50;
 template <int WAVES, bool MIXED>
 __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   constexpr int NT = 64 * WAVES;
-  __shared__ double xs[3][NT];
-  __shared__ float xs32[3][NT];
+  __shared__ double xs[2][3][NT];
+  __shared__ float xs32[2][3][NT];
   __shared__ double redA[2 * WAVES], redB[2 * WAVES], redC[2 * WAVES];
   __shared__ float redD[2 * WAVES], redE[2 * WAVES];
   const int32_t c = blockIdx.x;
@@ -212,7 +322,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   const int64_t P = a.cap;
   const double V = col.hx * col.hy * col.hz;
   const Reducer<WAVES, double> RA{redA}, RB{redB}, RC{redC};
-  const Exchange<double, NT> neighbours{xs, t, N, live};
+  Exchange<double, NT> neighbours{xs, t, N, live};
   // mass of this level: diagonal and couplings (lumped: no coupling)
   double m = 0.0, mlo = 0.0, mhi = 0.0, body[3] = {0, 0, 0};
   const bool above = live && t + 1 < N;
@@ -382,7 +492,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     // FP64 iterative refinement around FP32 PCG: r = B - K x in FP64 (the true residual, the
     // stop test), the correction K e = r solved on FP32 blocks with FP32 sums to inner_rtol
     const Reducer<WAVES, float> RD{redD}, RE{redE};
-    const Exchange<float, NT> neighbours32{xs32, t, N, live};
+    Exchange<float, NT> neighbours32{xs32, t, N, live};
     Sym3<float> Klo32, Kd32, Khi32;
     for (int w = 0; w < 6; ++w) {
       Klo32.m[w] = static_cast<float>(Klo.m[w]);
@@ -403,10 +513,13 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       rnorm = sqrt(rr);
       converged = rnorm <= tol;
       if (converged || its >= a.cg_max_it || cycle >= kMaxRefinements) break;
+      // the correction solves K e = r / ||r|| (unit right-hand side: no FP32 underflow of the
+      // inner dots as the Newton residual shrinks) and is scaled back in FP64
+      const double scale = rnorm > 0.0 ? 1.0 / rnorm : 0.0;
       float rf[3], zf[3], pf[3], Apf[3], ef[3] = {0.f, 0.f, 0.f};
       float rrf = 0.f, rzf = 0.f;
       for (int d = 0; d < 3; ++d) {
-        rf[d] = static_cast<float>(r[d]);
+        rf[d] = static_cast<float>(r[d] * scale);
         zf[d] = inv32[d] * rf[d];
         pf[d] = zf[d];
         rrf += rf[d] * rf[d];
@@ -436,7 +549,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
         rzf = rz2;
         for (int d = 0; d < 3; ++d) pf[d] = zf[d] + beta * pf[d];
       }
-      for (int d = 0; d < 3; ++d) x[d] += static_cast<double>(ef[d]);
+      for (int d = 0; d < 3; ++d) x[d] += rnorm * static_cast<double>(ef[d]);
     }
   }
   const bool accept = converged || rnorm <= 1e-6 * bnorm;
@@ -599,7 +712,14 @@ int pool_alloc(fx_column_pool* p, T*& dst, size_t n) {
 // workgroup sizes (1, 2, 4, 8 waves: columns of up to 64, 128, 256, 512 elements).
 int launch_iteration(fx_column_pool* p, hipStream_t st) {
   const PoolArgs& a = p->args;
-  hipLaunchKernelGGL(column_force_kernel, dim3(static_cast<unsigned>((a.cap + 255) / 256)), dim3(256), 0, st, a);
+  const unsigned slots = static_cast<unsigned>((a.cap + 255) / 256);
+  if (p->d.flags & FX_COLUMN_COMPACT) {
+    hipLaunchKernelGGL(column_predict_kernel, dim3(slots), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(column_correct_kernel, dim3(std::min(slots, 2048u)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL(column_clear_count_kernel, dim3(1), dim3(64), 0, st, a.pcount);
+  } else {
+    hipLaunchKernelGGL(column_force_kernel, dim3(slots), dim3(256), 0, st, a);
+  }
   const unsigned nc = static_cast<unsigned>(a.max_columns);
   if (p->d.flags & FX_COLUMN_MIXED) {
     hipLaunchKernelGGL((column_solve_kernel<1, true>), dim3(nc), dim3(64), 0, st, a);
@@ -675,6 +795,9 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
       (rc = pool_alloc(p, a.ut, 3 * cap)) || (rc = pool_alloc(p, a.vt, 3 * cap)) ||
       (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) ||
       (rc = pool_alloc(p, rec, static_cast<size_t>(d->max_record))) ||
+      ((d->flags & FX_COLUMN_COMPACT) &&
+       ((rc = pool_alloc(p, a.ceps, 3 * cap)) || (rc = pool_alloc(p, a.plist, cap)) ||
+        (rc = pool_alloc(p, a.pcount, 1)))) ||
       (rc = pool_alloc(p, p->count, 1 + (static_cast<size_t>(d->max_columns) + 3) / 4))) {
     fx_column_pool_destroy(p);
     return rc;

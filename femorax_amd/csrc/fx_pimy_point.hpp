@@ -454,8 +454,10 @@ struct Point {
       }
   }
 
-  // the update at strain eps (engineering shear): stress, tangent, trial history written
-  __device__ void update(const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
+  // getStress (:455-509). With stop_at_correction (the compacted path's predictor) it returns
+  // true, and stops, where the first stress correction would run; otherwise it sets the trial
+  // stress and returns false.
+  __device__ bool get_stress(const double (&eps)[6], bool stop_at_correction) {
     cur_stress = load_t2(kPStress);
     cur_strain = load_t2(kPStrain);
     committed_active = static_cast<int>(c(kPActive));
@@ -494,6 +496,7 @@ struct Point {
         if (i == 0) set_trial_stress(cur_stress);
         else set_trial_stress(trial);
         if (active == 0 && !crossing_next()) continue;
+        if (stop_at_correction) return true;
         if (active == 0) active++;
         stress_correction(false);
         update_active_surface();
@@ -503,6 +506,11 @@ struct Point {
       const V6 d = trial.dev;
       trial.set_dev(d, vol);
     }
+    return false;
+  }
+
+  // finish (:512-539): the tangent, the outputs and the trial history.
+  __device__ void finish(double (&sig)[6], double (&D)[36]) {
     tangent(D);
     for (int i = 0; i < 6; ++i) sig[i] = ok ? trial.t2.v[i] : NAN;
     V6 sc = cur_strain.t2;
@@ -516,6 +524,21 @@ struct Point {
     t(kPK) = K;
     t(kPActive) = p.load_stage == 1 ? active : committed_active;
   }
+
+  // the update at strain eps (engineering shear): stress, tangent, trial history written
+  __device__ void update(const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
+    get_stress(eps, false);
+    finish(sig, D);
+  }
+
+  // The compacted path's predictor (pressure_independ_multi_yield.hpp:247-255): a point that
+  // needs no stress correction is finished (false); otherwise true, and the corrector redoes
+  // the whole update (the same function, so the bits are those of update).
+  __device__ bool predict(const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
+    const bool plastic = get_stress(eps, true);
+    if (!plastic) finish(sig, D);
+    return plastic;
+  }
 };
 
 }  // namespace pimy
@@ -524,6 +547,12 @@ __device__ inline void pimy_update(const PimyParams& p, const double* C, double*
                                    const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
   pimy::Point pt(p, C, T, stride);
   pt.update(eps, sig, D);
+}
+
+__device__ inline bool pimy_predict(const PimyParams& p, const double* C, double* T, int64_t stride,
+                                    const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
+  pimy::Point pt(p, C, T, stride);
+  return pt.predict(eps, sig, D);
 }
 
 }  // namespace fx

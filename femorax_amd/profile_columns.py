@@ -1,7 +1,7 @@
 """Profiling driver for the batched soil-column pool: the C4 workload (4096 PressureIndependMultiYield
 columns, N ~ U{20..400}, Newmark + Newton + PCG, 0.3 g base record) for a few steps, pool only.
 
-    python femorax_amd/profile_columns.py STEPS [NCOL]
+    python femorax_amd/profile_columns.py STEPS [NCOL] [compact] [mixed]
 """
 import json
 import os
@@ -17,5 +17,6 @@ if __name__ == "__main__":
     steps = int(sys.argv[1]) if len(sys.argv) > 1 else 10
     ncol = int(sys.argv[2]) if len(sys.argv) > 2 else 4096
     ctx = core.Context(0)
-    print(json.dumps(bench.bench_c4(core, S, ctx, steps, ncol=ncol, cpu=False, plain=False)), flush=True)
+    print(json.dumps(bench.bench_c4(core, S, ctx, steps, ncol=ncol, cpu=False, plain=False,
+                                    compact="compact" in sys.argv[3:], mixed="mixed" in sys.argv[3:])), flush=True)
     ctx.close()

@@ -99,59 +99,60 @@ class BlockWorker(threading.Thread):
         self.error = None
 
     def run(self):
+        ctx = None
         try:
-            self._serve()
-        except Exception as e:  # reported by the server after the run
+            ctx = core.Context(self.device)
+        except Exception as e:  # noqa: BLE001 - every request then reports the failure
             self.error = e
-            while True:  # keep draining so the server is not left waiting
-                r = self.inbox.get()
-                if r is None:
-                    return
-                r.end = time.perf_counter()
-                r.result = {"error": repr(e)}
-                self.done.put(r)
-
-    def _serve(self):
-        ctx = core.Context(self.device)
         resident = collections.OrderedDict()  # n -> (batch, b, x, bytes)
-        used = 0
+        self.used = 0
         while True:
             r = self.inbox.get()
             if r is None:
                 break
             r.start = time.perf_counter()
-            if r.n in resident:
-                resident.move_to_end(r.n)
-            else:
-                model, mesh, rhs = self.problems.get(r.n)
-                need = _block_bytes(mesh.neq, self.mixed)
-                while resident and used + need > self.budget:  # admission within the block share
-                    _, (batch, b, x, nb) = resident.popitem(last=False)
-                    batch.close()
-                    b.close()
-                    x.close()
-                    used -= nb
-                batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=self.mixed)
-                b = core.DeviceArray.from_host(ctx, rhs)
-                x = core.DeviceArray(ctx, mesh.neq)
-                resident[r.n] = (batch, b, x, need)
-                used += need
-            batch, b, x, _ = resident[r.n]
-            x.zero()
-            if self.mixed:
-                res = batch.cg_mixed(b, x, rtol=self.rtol, inner_rtol=1e-3)
-            else:
-                res = batch.cg(b, x, rtol=self.rtol)
-            r.result = dict(res)
-            if self.keep_x:
-                r.result["x"] = x.get()
+            try:
+                if ctx is None:
+                    raise RuntimeError(f"no device context: {self.error!r}")
+                self._solve(ctx, resident, r)
+            except Exception as e:  # noqa: BLE001 - the request fails, the worker goes on
+                self.error = e
+                r.result = {"error": repr(e)}
             r.end = time.perf_counter()
             self.done.put(r)
         for batch, b, x, _ in resident.values():
             batch.close()
-            b.close()
-            x.close()
-        ctx.close()
+            b.free()
+            x.free()
+        if ctx is not None:
+            ctx.close()
+
+    def _solve(self, ctx, resident, r):
+        if r.n in resident:
+            resident.move_to_end(r.n)
+        else:
+            model, mesh, rhs = self.problems.get(r.n)
+            need = _block_bytes(mesh.neq, self.mixed)
+            while resident and self.used + need > self.budget:  # admission within the block share
+                _, (batch, b, x, nb) = resident.popitem(last=False)
+                batch.close()
+                b.free()
+                x.free()
+                self.used -= nb
+            batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=self.mixed, tune=False)
+            b = core.DeviceArray.from_host(ctx, rhs)
+            x = core.DeviceArray(ctx, mesh.neq)
+            resident[r.n] = (batch, b, x, need)
+            self.used += need
+        batch, b, x, _ = resident[r.n]
+        x.zero()
+        if self.mixed:
+            res = batch.cg_mixed(b, x, rtol=self.rtol, inner_rtol=1e-3)
+        else:
+            res = batch.cg(b, x, rtol=self.rtol)
+        r.result = dict(res)
+        if self.keep_x:
+            r.result["x"] = x.get()
 
 
 def _block_bytes(neq, mixed):
@@ -246,9 +247,6 @@ class Server:
                 next_log += 5.0
                 log(f"[serve] t={time.perf_counter() - t0:.1f}s done {len(finished)} resident columns {len(resident)} "
                     f"waiting {len(waiting)} blocks out {blocks_out}")
-        for w in self.workers:
-            if w.error is not None:
-                raise RuntimeError(f"block worker failed: {w.error!r}")
         out = []
         for r in finished:
             out.append(dict(rid=r.rid, kind=r.kind, n=r.n, arrival=r.arrival, start=r.start - t0, end=r.end - t0,
