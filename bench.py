@@ -128,7 +128,7 @@ def bench_c3(core, S, ctx, nsteps):
 
 
 def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cpu=True, plain=True, compact=False,
-             mixed=False):
+             mixed=False, waves=0):
     """C4: ncol PressureIndependMultiYield columns 1 x 1 x N (N ~ U{20..400}, 10 layers each),
     Newmark (gamma 0.5, beta 0.25, dt 0.005 s) + Newton (NormDispIncr 1e-10) + PCG (1e-12),
     one band-limited base record at 0.3 g. All columns admitted at once into one pool and run to
@@ -136,7 +136,7 @@ def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cp
     cols, rec = S.config_c4(ncol=ncol, nsteps=max(nsteps, 150))
     total = sum(c["nelem"] for c in cols)
     pool = core.ColumnPool(ctx, max_elements=total + 1024, max_columns=ncol, dt=0.005, max_materials=10 * ncol,
-                           max_record=ncol * 10 * nsteps + 1024, compact=compact, mixed=mixed)
+                           max_record=ncol * 10 * nsteps + 1024, compact=compact, mixed=mixed, waves=waves)
     keep = []
     t0 = time.perf_counter()
     ids = [pool.add(core.pimy_materials(c["mats"], keep), c["mat_of"], nsteps, rec, record=True) for c in cols]
@@ -161,6 +161,7 @@ def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cp
            "newton_per_step": round(newton / max(steps, 1), 3), "cg_per_newton": round(krylov / max(newton, 1), 2),
            "state_bytes_per_iteration": total * 2 * 150 * 8,
            "return_map": "compacted (predict, compact, correct)" if compact else "fused",
+           "waves_per_simd": {0: 3, 1: "compiler"}.get(waves, waves),
            "solve": "FP32 inner PCG in FP64 refinement" if mixed else "FP64 PCG",
            "note": "one material point per tied element (uniform strain), return map + per-column PCG per "
                    "Newton iteration, hipGraph replay"}

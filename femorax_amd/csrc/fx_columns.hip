@@ -32,7 +32,8 @@
 //    blocks, FP32 sums) inside FP64 iterative refinement, stopped on the FP64 true residual.
 //    Workgroup sums are wave butterflies then a fixed-order sum over waves: a column's bits
 //    do not depend on the other columns.
-// The pair is captured once as a hipGraph and replayed.
+// The four solve sizes run on side streams forked from the pool's stream (they touch disjoint
+// columns); the iteration is captured once as a hipGraph (branches for the forks) and replayed.
 #pragma clang fp contract(off)
 
 #include <hip/hip_runtime.h>
@@ -42,6 +43,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fx_internal.hpp"
@@ -98,7 +100,8 @@ __device__ inline void column_outputs(const PoolArgs& a, int64_t i, double A, do
 
 // ---- the return map, one thread per element slot ------------------------------------------------
 
-__global__ __launch_bounds__(256) void column_force_kernel(PoolArgs a) {
+template <int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void column_force_kernel(PoolArgs a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= a.cap) return;
   const int32_t c = a.elem_col[i];
@@ -141,7 +144,8 @@ __device__ inline void column_outputs(const PoolArgs& a, int64_t i, double A, do
     for (int q = r; q < 3; ++q, ++w) a.eout[w * P + i] = s * D[6 * idx[r] + idx[q]];
 }
 
-__global__ __launch_bounds__(256) void column_predict_kernel(PoolArgs a) {
+template <int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void column_predict_kernel(PoolArgs a) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   bool plastic = false;
   if (i < a.cap) {
@@ -179,7 +183,8 @@ __global__ __launch_bounds__(256) void column_predict_kernel(PoolArgs a) {
   if (plastic) a.plist[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = static_cast<int32_t>(i);
 }
 
-__global__ __launch_bounds__(256) void column_correct_kernel(PoolArgs a) {
+template <int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void column_correct_kernel(PoolArgs a) {
   const int32_t n = *a.pcount;
   const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(a.cap);
   for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
@@ -684,6 +689,8 @@ struct fx_column_pool {
   int32_t* count_host = nullptr;  // pinned copy of the same
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};  // default priority
+  hipEvent_t fork = nullptr, join[4] = {nullptr, nullptr, nullptr, nullptr};
 };
 
 namespace fx {
@@ -713,24 +720,43 @@ int pool_alloc(fx_column_pool* p, T*& dst, size_t n) {
 int launch_iteration(fx_column_pool* p, hipStream_t st) {
   const PoolArgs& a = p->args;
   const unsigned slots = static_cast<unsigned>((a.cap + 255) / 256);
-  if (p->d.flags & FX_COLUMN_COMPACT) {
-    hipLaunchKernelGGL(column_predict_kernel, dim3(slots), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(column_correct_kernel, dim3(std::min(slots, 2048u)), dim3(256), 0, st, a);
-    hipLaunchKernelGGL(column_clear_count_kernel, dim3(1), dim3(64), 0, st, a.pcount);
-  } else {
-    hipLaunchKernelGGL(column_force_kernel, dim3(slots), dim3(256), 0, st, a);
-  }
+  // register budget of the return-map kernels: waves per SIMD the compiler must allow
+  const uint32_t bits = (p->d.flags >> 8) & 0xfu;
+  const int wpe = bits ? static_cast<int>(bits) : 3;  // 3: measured best on C4
+  auto return_map = [&](auto w) {
+    constexpr int W = decltype(w)::value;
+    if (p->d.flags & FX_COLUMN_COMPACT) {
+      hipLaunchKernelGGL(column_predict_kernel<W>, dim3(slots), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(column_correct_kernel<W>, dim3(std::min(slots, 2048u)), dim3(256), 0, st, a);
+      hipLaunchKernelGGL(column_clear_count_kernel, dim3(1), dim3(64), 0, st, a.pcount);
+    } else {
+      hipLaunchKernelGGL(column_force_kernel<W>, dim3(slots), dim3(256), 0, st, a);
+    }
+  };
+  if (wpe >= 4) return_map(std::integral_constant<int, 4>{});
+  else if (wpe == 3) return_map(std::integral_constant<int, 3>{});
+  else return_map(std::integral_constant<int, 1>{});
+  // the four solve sizes touch disjoint columns: forked onto side streams so they overlap
+  // (graph branches when captured), joined back before the next iteration
   const unsigned nc = static_cast<unsigned>(a.max_columns);
-  if (p->d.flags & FX_COLUMN_MIXED) {
-    hipLaunchKernelGGL((column_solve_kernel<1, true>), dim3(nc), dim3(64), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<2, true>), dim3(nc), dim3(128), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<4, true>), dim3(nc), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<8, true>), dim3(nc), dim3(512), 0, st, a);
-  } else {
-    hipLaunchKernelGGL((column_solve_kernel<1, false>), dim3(nc), dim3(64), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<2, false>), dim3(nc), dim3(128), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<4, false>), dim3(nc), dim3(256), 0, st, a);
-    hipLaunchKernelGGL((column_solve_kernel<8, false>), dim3(nc), dim3(512), 0, st, a);
+  const bool mixed = (p->d.flags & FX_COLUMN_MIXED) != 0;
+  FX_HIP(p->ctx, hipEventRecord(p->fork, st));
+  for (int k = 0; k < 4; ++k) {
+    hipStream_t s = p->side[k];
+    FX_HIP(p->ctx, hipStreamWaitEvent(s, p->fork, 0));
+    const dim3 g(nc), b(64u << k);
+    switch (k + (mixed ? 4 : 0)) {
+      case 0: hipLaunchKernelGGL((column_solve_kernel<1, false>), g, b, 0, s, a); break;
+      case 1: hipLaunchKernelGGL((column_solve_kernel<2, false>), g, b, 0, s, a); break;
+      case 2: hipLaunchKernelGGL((column_solve_kernel<4, false>), g, b, 0, s, a); break;
+      case 3: hipLaunchKernelGGL((column_solve_kernel<8, false>), g, b, 0, s, a); break;
+      case 4: hipLaunchKernelGGL((column_solve_kernel<1, true>), g, b, 0, s, a); break;
+      case 5: hipLaunchKernelGGL((column_solve_kernel<2, true>), g, b, 0, s, a); break;
+      case 6: hipLaunchKernelGGL((column_solve_kernel<4, true>), g, b, 0, s, a); break;
+      default: hipLaunchKernelGGL((column_solve_kernel<8, true>), g, b, 0, s, a); break;
+    }
+    FX_HIP(p->ctx, hipEventRecord(p->join[k], s));
+    FX_HIP(p->ctx, hipStreamWaitEvent(st, p->join[k], 0));
   }
   FX_HIP(p->ctx, hipGetLastError());
   return FX_OK;
@@ -748,6 +774,11 @@ void fx_column_pool_destroy(fx_column_pool* p) {
   (void)hipStreamSynchronize(ctx->stream);
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
+  for (int k = 0; k < 4; ++k) {
+    if (p->side[k]) (void)hipStreamDestroy(p->side[k]);
+    if (p->join[k]) (void)hipEventDestroy(p->join[k]);
+  }
+  if (p->fork) (void)hipEventDestroy(p->fork);
   for (void* q : p->allocs) (void)hipFree(q);
   if (p->count_host) (void)hipHostFree(p->count_host);
   ctx->used -= std::min<uint64_t>(ctx->used, p->bytes);
@@ -818,6 +849,14 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   if (hipMemcpy(elem_col, none.data(), cap * sizeof(int32_t), hipMemcpyHostToDevice) != hipSuccess) {
     fx_column_pool_destroy(p);
     return fail(ctx, FX_EDEVICE, "hipMemcpy failed");
+  }
+  bool streams_ok = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming) == hipSuccess;
+  for (int k = 0; k < 4 && streams_ok; ++k)
+    streams_ok = hipStreamCreateWithFlags(&p->side[k], hipStreamNonBlocking) == hipSuccess &&
+                 hipEventCreateWithFlags(&p->join[k], hipEventDisableTiming) == hipSuccess;
+  if (!streams_ok) {
+    fx_column_pool_destroy(p);
+    return fail(ctx, FX_EDEVICE, "column pool: stream or event creation failed");
   }
   p->elems.reset(a.cap);
   p->mats.reset(d->max_materials);
