@@ -199,20 +199,20 @@ struct TileFetch {
   int pidx;
 };
 
+// Every tile owns full blocks of kTileNodes words and slots (padding: word 0, slot -1) and
+// kTileCells cell parameters (full-tile order, 0xffff outside the tile): the fetch needs only
+// the tile index, so the tile header's load runs beside it.
 template <typename T, bool DIAG>
-__device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, const TileHeader& h, int t, TileFetch<T>& f) {
-  const int nx = h.nx, ny = h.ny, nz = h.nz;
-  const int nnodes = (nx + 1) * (ny + 1) * (nz + 1);
-  const uint32_t* words = a.words + h.word_offset;
-  const int32_t* slots = a.slots + h.word_offset;
+__device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t, TileFetch<T>& f) {
+  const uint32_t* words = a.words + static_cast<size_t>(tile) * kTileNodes;
+  const int32_t* slots = a.slots + static_cast<size_t>(tile) * kTileNodes;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int ln = t + r * kTileCells;
-    f.w[r] = ln < nnodes ? words[ln] : 0u;
-    f.s[r] = ln < nnodes ? slots[ln] : -1;
+    f.w[r] = ln < kTileNodes ? words[ln] : 0u;
+    f.s[r] = ln < kTileNodes ? slots[ln] : -1;
   }
-  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY);
-  f.pidx = (cx < nx && cy < ny && cz < nz) ? a.cell_param[h.cell_offset + cx + nx * (cy + ny * cz)] : 0xffff;
+  f.pidx = a.cell_param[static_cast<size_t>(tile) * kTileCells + t];
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const uint32_t mask = DIAG ? 0u : (f.w[r] >> 28) & 7u;  // 0 past the tile's nodes
@@ -352,13 +352,12 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
   int tile = blockIdx.x;
   TileHeader h = a.tiles[tile];
   TileFetch<T> f;
-  fetch_tile<T, DIAG>(a, h, t, f);
+  fetch_tile<T, DIAG>(a, tile, t, f);
   for (;;) {
-    const int nnodes = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
       const int ln = t + r * kTileCells;
-      if (ln < nnodes) {
+      if (ln < kTileNodes) {
         wsh[ln] = f.w[r];
         ssh[ln] = f.s[r];
 #pragma unroll
@@ -371,7 +370,7 @@ __global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_ke
     TileFetch<T> fn = f;
     if (PERSIST && next < a.ntiles) {
       hn = a.tiles[next];
-      fetch_tile<T, DIAG>(a, hn, t, fn);
+      fetch_tile<T, DIAG>(a, next, t, fn);
     }
     box_tile<T, MODE>(a, tile, h, f.pidx, lds, wsh, ssh, t);
     if (next >= a.ntiles) break;
@@ -451,20 +450,25 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
     operator_factors(p, k, fD, cM);
     mass |= cM != 0.0;
   }
-  // variant bit 0: 5 waves per SIMD (mass-free kernel only: the mass kernel would spill heavily);
-  // bit 1: persistent workgroups with prefetch
+  // variant bits 0-1: the register budget of the mass-free kernel (the compiler's choice, or at
+  // least 5, 7, 8 waves per SIMD: 84, 72, 64 VGPRs in FP64, the last two with a few spills; the
+  // mass and diagonal kernels keep the compiler's choice); bit 2: persistent workgroups with
+  // prefetch
   using Kernel = void (*)(BoxArgs<T>);
-  static const Kernel table[3][4] = {
-      {hex8_box_kernel<T, 0, 0, false>, hex8_box_kernel<T, 0, 5, false>, hex8_box_kernel<T, 0, 0, true>,
-       hex8_box_kernel<T, 0, 5, true>},
-      {hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, true>,
-       hex8_box_kernel<T, 1, 0, true>},
-      {hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, true>,
-       hex8_box_kernel<T, 2, 0, true>}};
-  const int v = b->box_variant & 3;
+  static const Kernel table[3][8] = {
+      {hex8_box_kernel<T, 0, 0, false>, hex8_box_kernel<T, 0, 5, false>, hex8_box_kernel<T, 0, 7, false>,
+       hex8_box_kernel<T, 0, 8, false>, hex8_box_kernel<T, 0, 0, true>, hex8_box_kernel<T, 0, 5, true>,
+       hex8_box_kernel<T, 0, 7, true>, hex8_box_kernel<T, 0, 8, true>},
+      {hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>,
+       hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, true>, hex8_box_kernel<T, 1, 0, true>,
+       hex8_box_kernel<T, 1, 0, true>, hex8_box_kernel<T, 1, 0, true>},
+      {hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>,
+       hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, true>, hex8_box_kernel<T, 2, 0, true>,
+       hex8_box_kernel<T, 2, 0, true>, hex8_box_kernel<T, 2, 0, true>}};
+  const int v = b->box_variant & 7;
   const Kernel kernel = table[DIAG ? 2 : mass ? 1 : 0][v];
   int grid = b->ntiles;
-  if (v & 2) {
+  if (v & 4) {
     const int resident = resident_tiles(b->ctx, reinterpret_cast<const void*>(kernel));
     if (resident > 0 && resident < grid) grid = resident >= 8 ? resident / 8 * 8 : resident;
   }

@@ -436,7 +436,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
 int fx_batch_tune(fx_batch* b, int variant) {
   if (!b) return fail(nullptr, FX_EINVAL, "fx_batch_tune: null batch");
   if (variant >= 0) {
-    if (variant > 3) return fail(b->ctx, FX_EINVAL, "fx_batch_tune: variant out of range");
+    if (variant > 7) return fail(b->ctx, FX_EINVAL, "fx_batch_tune: variant out of range");
     b->box_variant = variant;
     b->info.variant = variant;
     return FX_OK;
@@ -453,7 +453,7 @@ int fx_batch_tune(fx_batch* b, int variant) {
   FX_HIP(ctx, hipEventCreate(&e1));
   float best = 1e30f;
   int best_v = 0;
-  for (int v = 0; v < 4; ++v) {
+  for (int v = 0; v < 8; ++v) {
     b->box_variant = v;
     if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
     float vbest = 1e30f;

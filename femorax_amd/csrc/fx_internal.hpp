@@ -58,8 +58,8 @@ static_assert(kTileCells == kBlockElems, "one cell per thread");
 
 struct TileHeader {
   int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, kTileZ)
-  int32_t word_offset;      // into the node words: (nx+1)(ny+1)(nz+1) words, x fastest
-  int32_t cell_offset;      // into the cell parameters: nx*ny*nz, x fastest
+  int32_t word_offset;      // into the node words: tile * kTileNodes ((nx+1)(ny+1)(nz+1) used, x fastest)
+  int32_t cell_offset;      // into the cell parameters: tile * kTileCells (full-tile order, x fastest)
   int32_t pad[3];
 };
 

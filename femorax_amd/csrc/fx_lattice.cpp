@@ -132,13 +132,18 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     h.nx = nx;
     h.ny = ny;
     h.nz = nz;
+    // every tile owns a full block of kTileNodes words (its (nx+1)(ny+1)(nz+1) nodes, x fastest,
+    // then padding) and kTileCells cell parameters in full-tile order (x + 8 (y + 8 z)), so a
+    // workgroup finds its words and cells from the tile index alone
     h.word_offset = static_cast<int32_t>(tile_nodes.size());
     h.cell_offset = static_cast<int32_t>(L.cell_param.size());
     bool any = false;
-    for (int32_t z = 0; z < nz; ++z)
-      for (int32_t y = 0; y < ny; ++y)
-        for (int32_t x = 0; x < nx; ++x) {
-          const int32_t e = cell[(i0 + x) + static_cast<int64_t>(ni) * ((j0 + y) + static_cast<int64_t>(nj) * (k0 + z))];
+    for (int32_t z = 0; z < TZ; ++z)
+      for (int32_t y = 0; y < TY; ++y)
+        for (int32_t x = 0; x < TX; ++x) {
+          int32_t e = -1;
+          if (x < nx && y < ny && z < nz)
+            e = cell[(i0 + x) + static_cast<int64_t>(ni) * ((j0 + y) + static_cast<int64_t>(nj) * (k0 + z))];
           L.cell_param.push_back(e >= 0 ? static_cast<uint16_t>(param[e]) : uint16_t(0xffff));
           L.cell_elem.push_back(e);
           any |= e >= 0;
@@ -152,6 +157,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       for (int32_t y = 0; y <= ny; ++y)
         for (int32_t x = 0; x <= nx; ++x)
           tile_nodes.push_back(lat_node[(i0 + x) + static_cast<size_t>(NI) * ((j0 + y) + static_cast<size_t>(NJ) * (k0 + z))]);
+    tile_nodes.resize(static_cast<size_t>(h.word_offset) + kTileNodes, -1);
     L.tiles.push_back(h);
     tile_colour.push_back(static_cast<int8_t>((ti & 1) | ((tj & 1) << 1) | ((tk & 1) << 2)));
   }

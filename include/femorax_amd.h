@@ -157,8 +157,9 @@ typedef struct fx_batch_info {
 int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
 /* Kernel-variant choice (the reference's force-settings autotune, src/exec/force_settings.cpp:
  * 135-205): variant >= 0 sets it, -1 times the variants on this batch and keeps the fastest.
- * Lattice batches have variants 0-3 (bit 0: 5 waves per SIMD register budget; bit 1: persistent
- * workgroups that prefetch their next tile); general batches have one. Every variant gives the
+ * Lattice batches have variants 0-7 (bits 0-1: the tile kernel's register budget, the compiler's
+ * choice or at least 5, 7, 8 waves per SIMD; bit 2: persistent workgroups that prefetch their
+ * next tile); general batches have one. Every variant gives the
  * same bits. */
 int fx_batch_tune(fx_batch* b, int variant);
 
