@@ -106,6 +106,7 @@ FX_COLUMN_MAX_ELEMS = 512
 FX_COLUMN_MIXED = 1
 FX_COLUMN_NO_GRAPH = 2
 FX_COLUMN_COMPACT = 4
+FX_BATCH_COMPACT = 128
 
 
 class ColumnPoolDesc(C.Structure):
@@ -466,9 +467,9 @@ class Batch:
         ctx.adopt(self)
 
     @classmethod
-    def pimy(cls, ctx, mesh, materials):
+    def pimy(cls, ctx, mesh, materials, compact=False):
         """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
-        fx_pimy_material per compiled parameter set."""
+        fx_pimy_material per compiled parameter set; compact: the compacted return map."""
         self = cls.__new__(cls)
         self.ctx = ctx
         self.neq = mesh.neq
@@ -487,7 +488,7 @@ class Batch:
         d.coords = _arr(keep, mesh.coords, np.float64)
         d.neq = mesh.neq
         h = C.c_void_p()
-        _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), 0, C.byref(h)), ctx)
+        _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), FX_BATCH_COMPACT if compact else 0, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)
         return self

@@ -194,7 +194,8 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm};
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm,
+                  b->cres, b->clist, b->ccount};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -751,6 +752,9 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
       ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
       ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&
       ok(list.empty() ? FX_OK : dev_upload(ctx, b->gather_list, list.data(), list.size(), by)) &&
+      (!(flags & FX_BATCH_COMPACT) ||
+       (ok(dev_zeros(ctx, b->cres, np * 33, by)) && ok(dev_zeros(ctx, b->clist, np, by)) &&
+        ok(dev_zeros(ctx, b->ccount, static_cast<size_t>(1), by)))) &&
       ok(fx::pimy_batch_init(b));
   if (rc == FX_OK) {
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, FX_EDEVICE, "fx_pimy_batch_create: init failed");

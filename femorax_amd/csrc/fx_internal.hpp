@@ -205,6 +205,11 @@ struct fx_batch {
   double* slots = nullptr;            // [24 * padded]
   int32_t* gather_start = nullptr;    // [neq + 1]
   int32_t* gather_list = nullptr;     // slots in (element, DOF) order
+  // compacted return map (FX_BATCH_COMPACT): per point stress (6), tangent upper triangle (21)
+  // and strain (6); the plastic points' list and its length
+  double* cres = nullptr;             // [33 * npts]
+  int32_t* clist = nullptr;           // [npts]
+  int32_t* ccount = nullptr;          // [1]
   // time-integration workspace (FX_BATCH_DYNAMIC): 12 * neq doubles; the DOF direction of each
   // equation (reference numbering, host)
   double* dyn = nullptr;

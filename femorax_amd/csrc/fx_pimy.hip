@@ -12,6 +12,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "fx_brick_device.hpp"
@@ -75,6 +76,9 @@ struct PimyBatchArgs {
   const double* x;  // u (force) or v (operator)
   double* slots;    // [24 * padded]
   fx_coeffs k;
+  double* cres;     // compacted return map: [33 * npts] stress, tangent (upper), strain
+  int32_t* clist;
+  int32_t* ccount;
 };
 
 __device__ inline void element_geometry(const PimyBatchArgs& a, int e, double (&X)[8], double (&Y)[8], double (&Z)[8],
@@ -141,9 +145,10 @@ __device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, 
   }
 }
 
-// MODE 0: force (update at u; STORE also keeps the tangents). MODE 1: operator on the stored
+// MODE 0: force (update at u; STORE also keeps the tangents; REPLAY takes each point's stress
+// and tangent from the compacted pass instead of updating). MODE 1: operator on the stored
 // tangents. MODE 2: its diagonal.
-template <int MODE, bool STORE>
+template <int MODE, bool STORE, bool REPLAY = false>
 __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchArgs a) {
   __shared__ double part[kPointsPerBlock][48];  // per point: 8 nodes x (3 first terms, 3 second terms)
   const int t = threadIdx.x;
@@ -159,14 +164,21 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
     const int64_t pt = static_cast<int64_t>(e) * 8 + gp;
     double* mine = part[t];
     if constexpr (MODE == 0) {
-      double u[24], eps[6], sig[6], D[36];
-      for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
-      point_strain(s, u, eps);
-      pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
-      if constexpr (STORE) {
-        int w = 0;
-        for (int i = 0; i < 6; ++i)
-          for (int j = i; j < 6; ++j, ++w) a.tangent[w * a.npts + pt] = D[6 * i + j];
+      double sig[6];
+      if constexpr (REPLAY) {
+        for (int c = 0; c < 6; ++c) sig[c] = a.cres[c * a.npts + pt];
+        if constexpr (STORE)
+          for (int w = 0; w < 21; ++w) a.tangent[w * a.npts + pt] = a.cres[(6 + w) * a.npts + pt];
+      } else {
+        double u[24], eps[6], D[36];
+        for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+        point_strain(s, u, eps);
+        pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+        if constexpr (STORE) {
+          int w = 0;
+          for (int i = 0; i < 6; ++i)
+            for (int j = i; j < 6; ++j, ++w) a.tangent[w * a.npts + pt] = D[6 * i + j];
+        }
       }
       for (int j = 0; j < 8; ++j) {
         double rj[3];
@@ -285,6 +297,66 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
   for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
 }
 
+// The compacted return map (compacted_kernels.hpp:85-202). Predict: one thread per Gauss point,
+// the strain as the fused kernel forms it, the elastic predictor; an elastic point is finished
+// (stress and tangent stored), a plastic one keeps its strain and joins the list (one atomic per
+// wave). Correct: a fixed grid walks the list, one full update per listed point (the same
+// function as the fused kernel, so the same bits). The force kernel then replays the stored
+// results.
+__global__ __launch_bounds__(kPointsPerBlock) void pimy_predict_kernel(PimyBatchArgs a) {
+  const int t = threadIdx.x;
+  const int e = blockIdx.x * kElemsPerBlock + t / 8, gp = t % 8;
+  bool plastic = false;
+  const int64_t pt = static_cast<int64_t>(e) * 8 + gp;
+  if (e < a.count) {
+    double X[8], Y[8], Z[8];
+    int eq[24];
+    element_geometry(a, e, X, Y, Z, eq);
+    const PimyParams& p = a.params[a.param[e]];
+    Shape<double> s;
+    gauss_shape(X, Y, Z, gp, s);
+    double u[24], eps[6], sig[6], D[36];
+    for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+    point_strain(s, u, eps);
+    plastic = pimy_predict(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+    if (plastic) {
+      for (int c = 0; c < 6; ++c) a.cres[(27 + c) * a.npts + pt] = eps[c];
+    } else {
+      for (int c = 0; c < 6; ++c) a.cres[c * a.npts + pt] = sig[c];
+      int w = 0;
+      for (int i = 0; i < 6; ++i)
+        for (int j = i; j < 6; ++j, ++w) a.cres[(6 + w) * a.npts + pt] = D[6 * i + j];
+    }
+  }
+  const uint64_t mask = __ballot(plastic);
+  if (mask == 0) return;
+  const int lane = __lane_id();
+  const int leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(a.ccount, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (plastic) a.clist[base + __popcll(mask & ((uint64_t(1) << lane) - 1))] = static_cast<int32_t>(pt);
+}
+
+__global__ __launch_bounds__(128) void pimy_correct_kernel(PimyBatchArgs a) {
+  const int32_t n = *a.ccount;
+  for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+    const int64_t pt = a.clist[k];
+    const PimyParams& p = a.params[a.param[pt / 8]];
+    double eps[6], sig[6], D[36];
+    for (int c = 0; c < 6; ++c) eps[c] = a.cres[(27 + c) * a.npts + pt];
+    pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+    for (int c = 0; c < 6; ++c) a.cres[c * a.npts + pt] = sig[c];
+    int w = 0;
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j, ++w) a.cres[(6 + w) * a.npts + pt] = D[6 * i + j];
+  }
+}
+
+__global__ void pimy_clear_count_kernel(int32_t* count) {
+  if (threadIdx.x == 0) count[0] = 0;
+}
+
 __global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
                                    const double* __restrict__ slots, double* __restrict__ y, const int32_t* done) {
   if (done && *done) return;
@@ -326,14 +398,25 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k) {
   a.x = x;
   a.slots = b->slots;
   a.k = k;
+  a.cres = b->cres;
+  a.clist = b->clist;
+  a.ccount = b->ccount;
   return a;
 }
 
 template <int MODE, bool STORE>
 int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done) {
-  if (b->count > 0)
-    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE>), dim3((b->count + kElemsPerBlock - 1) / kElemsPerBlock),
-                       dim3(kPointsPerBlock), 0, b->ctx->stream, a);
+  const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
+  hipStream_t st = b->ctx->stream;
+  if (b->count > 0 && MODE == 0 && b->cres) {  // compacted: predict, correct, replay
+    hipLaunchKernelGGL(pimy_predict_kernel, grid, dim3(kPointsPerBlock), 0, st, a);
+    const unsigned cgrid = static_cast<unsigned>(std::min<int64_t>((b->npts + 127) / 128, 2048));
+    hipLaunchKernelGGL(pimy_correct_kernel, dim3(cgrid), dim3(128), 0, st, a);
+    hipLaunchKernelGGL(pimy_clear_count_kernel, dim3(1), dim3(64), 0, st, a.ccount);
+    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE, true>), grid, dim3(kPointsPerBlock), 0, st, a);
+  } else if (b->count > 0) {
+    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE>), grid, dim3(kPointsPerBlock), 0, st, a);
+  }
   if (b->neq > 0)
     hipLaunchKernelGGL(slot_gather_kernel, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
                        b->gather_start, b->gather_list, b->slots, y, done);

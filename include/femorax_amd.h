@@ -143,6 +143,10 @@ typedef struct fx_batch_desc {
 #define FX_BATCH_XCD_SLABS 8u /* lattice path: tiles placed in 8 slabs, one per XCD (default: lattice order,
                                 which measured faster on MI355X; same bits either way) */
 #define FX_BATCH_NO_TUNE 16u /* skip the kernel-variant autotune at creation (variant 0) */
+#define FX_BATCH_COMPACT 128u /* stdBrick<PressureIndependMultiYield> batches: the compacted return map
+                                 (compacted_kernels.hpp:85-202): an elastic predictor per Gauss point, the
+                                 plastic points listed, one full update per listed point, the element
+                                 forces replayed from the stored results (same bits as the fused kernel) */
 
 int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* desc, uint32_t flags, fx_batch** out);
 void fx_batch_destroy(fx_batch* b);

@@ -1,7 +1,8 @@
 """GPU parity of stdBrick<PressureIndependMultiYield> batches against the oracle: the internal
 force with the return map at every Gauss point (trial history), the operator and its diagonal
-on the stored tangents, commits, stage switches, CG on the tangent operator, and the explicit
-loop with the history committed every step.
+on the stored tangents, commits, stage switches, CG on the tangent operator, the explicit
+loop with the history committed every step, and the compacted return map (predict / compact /
+correct / replay) against the fused one, bit for bit.
 
 Tolerances: forces, operator, diagonal 1e-10 of the largest entry; histories 1e-10 of each
 field's largest value; CG 1e-8 (the reference's time-history tolerance).
@@ -176,3 +177,46 @@ def test_explicit_loop_commits_the_history(ctx):
 def pe_as_elastic(per_set):
     """The lumped mass only needs rho (elastic parameter records with the PIMY sets' rho)."""
     return fxo.elastic_params([dict(E=1.0, nu=0.3, rho=m["rho"], mass_type=m["mass_type"]) for m in per_set])
+
+
+def test_compacted_return_map_gives_the_same_bits(ctx):
+    """FX_BATCH_COMPACT (compacted_kernels.hpp:85-202): forces, stored tangents (through the
+    operator) and trial histories equal to the fused kernel's bit for bit, along a walk with
+    elastic and plastic points, commits between steps."""
+    model, mats = pimy_model("distorted3")
+    pm = core.Mesh(model)
+    per_set = [mats[g] for g in pm.param_group]
+    keep = []
+
+    def dev():
+        return [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep)
+                for m in per_set]
+
+    fused, comp = core.Batch.pimy(ctx, pm, dev()), core.Batch.pimy(ctx, pm, dev(), compact=True)
+    n = pm.neq
+    dir_ = meshes.random_vector(n, 31)
+    du = core.DeviceArray(ctx, n)
+    f1, f2 = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    x = core.DeviceArray.from_host(ctx, meshes.random_vector(n, 32))
+    y1, y2 = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    plastic = 0
+    for step in range(7):
+        if step == 1:
+            fused.set_stage(1)
+            comp.set_stage(1)
+        du.upload(dir_ * (3e-3 * (step + 1) * (1 if step % 3 else -0.6)))
+        fused.store_tangents(du, f1)
+        comp.store_tangents(du, f2)
+        np.testing.assert_array_equal(f2.get(), f1.get())
+        np.testing.assert_array_equal(comp.pimy_state(False), fused.pimy_state(False))
+        fused.apply(x, y1)
+        comp.apply(x, y2)
+        np.testing.assert_array_equal(y2.get(), y1.get())
+        fused.force(du, f1)
+        comp.force(du, f2)
+        np.testing.assert_array_equal(f2.get(), f1.get())
+        plastic = max(plastic, int(np.sum(fused.pimy_state(False)[:, 26] > 0)))
+        fused.commit()
+        comp.commit()
+    assert plastic > 0
