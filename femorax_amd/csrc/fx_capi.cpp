@@ -87,11 +87,19 @@ void fx_ctx_destroy(fx_ctx* ctx) {
   if (!ctx) return;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  fx::release_cg_graphs(ctx, nullptr);
   if (ctx->red) (void)hipFree(ctx->red);
   if (ctx->scalars) (void)hipFree(ctx->scalars);
   if (ctx->pimy_params) (void)hipFree(ctx->pimy_params);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+int fx_ctx_set_graphs(fx_ctx* ctx, int enable) {
+  if (!ctx) return fail(nullptr, FX_EINVAL, "fx_ctx_set_graphs: null context");
+  ctx->use_graphs = enable != 0;
+  if (!ctx->use_graphs) fx::release_cg_graphs(ctx, nullptr);
+  return FX_OK;
 }
 
 const char* fx_last_error(const fx_ctx* ctx) { return ctx ? ctx->error.c_str() : g_error.c_str(); }
@@ -189,6 +197,7 @@ void fx_batch_destroy(fx_batch* b) {
   fx_ctx* ctx = b->ctx;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  fx::release_cg_graphs(ctx, b);  // they read the batch's arrays
   void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,

@@ -93,6 +93,15 @@ void build_contraction(int32_t nnode, int32_t neq, const int32_t* node_eq, Contr
 
 }  // namespace fx
 
+// A captured run of CG iterations (cg_device): the key holds everything the kernels' arguments
+// depend on, owner the batch whose arrays they read (its graphs go when the batch goes).
+struct fx_cg_graph {
+  std::vector<unsigned char> key;
+  const void* owner = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 // Context and batch (opaque at the C ABI).
 struct fx_ctx {
   int device = 0;
@@ -105,7 +114,14 @@ struct fx_ctx {
   void* scalars = nullptr;  // solver state block (device)
   void* host_pinned = nullptr;
   void* pimy_params = nullptr;  // material-point driver: up to kMaxPointMaterials parameter sets
+  bool use_graphs = false;      // CG iterations replayed from captured graphs (fx_ctx_set_graphs)
+  std::vector<fx_cg_graph> cg_graphs;
 };
+
+namespace fx {
+// Drop the captured CG graphs of one owner (nullptr: all).
+void release_cg_graphs(fx_ctx* ctx, const void* owner);
+}  // namespace fx
 
 namespace fx {
 struct DevBlocks {

@@ -244,6 +244,7 @@ inline int64_t chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
 int ensure_red(fx_ctx* ctx, int64_t n) {
   const size_t need = static_cast<size_t>(2 * chunks(n) + 2 * (chunks(chunks(n)) + 2) + 16);
   if (ctx->red_cap >= need) return FX_OK;
+  release_cg_graphs(ctx, nullptr);  // captured graphs read the old buffer
   if (ctx->red) (void)hipFree(ctx->red);
   ctx->red = nullptr;
   FX_HIP(ctx, hipMalloc(&ctx->red, need * sizeof(double)));
@@ -273,12 +274,70 @@ int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out)
   return FX_OK;
 }
 
+void release_cg_graphs(fx_ctx* ctx, const void* owner) {
+  auto& g = ctx->cg_graphs;
+  for (auto it = g.begin(); it != g.end();) {
+    if (owner && it->owner != owner) {
+      ++it;
+      continue;
+    }
+    if (it->exec) (void)hipGraphExecDestroy(it->exec);
+    if (it->graph) (void)hipGraphDestroy(it->graph);
+    it = g.erase(it);
+  }
+}
+
 namespace {
 
-// The CG iteration on the device for vector type T. apply(x, y) computes y = A x.
+constexpr int kGraphIters = 8;     // CG iterations per captured graph
+constexpr size_t kMaxGraphs = 16;  // per context
+
+struct CgGraphKey {
+  const void* owner;
+  int64_t n;
+  int32_t tsize, initial;
+  double ks, kd, km, rtol, atol;
+  const void* ptr[10];
+};
+
+// The captured graph of kGraphIters iterations for this key, capturing it on first use.
+template <typename Body>
+hipGraphExec_t cg_graph(fx_ctx* ctx, const CgGraphKey& key, Body body) {
+  std::vector<unsigned char> k(sizeof key);
+  std::memcpy(k.data(), &key, sizeof key);
+  for (auto& g : ctx->cg_graphs)
+    if (g.key == k) return g.exec;
+  if (ctx->cg_graphs.size() >= kMaxGraphs) {  // drop the oldest
+    fx_cg_graph& old = ctx->cg_graphs.front();
+    if (old.exec) (void)hipGraphExecDestroy(old.exec);
+    if (old.graph) (void)hipGraphDestroy(old.graph);
+    ctx->cg_graphs.erase(ctx->cg_graphs.begin());
+  }
+  hipStream_t s = ctx->stream;
+  if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) return nullptr;
+  int rc = FX_OK;
+  for (int i = 0; i < kGraphIters && rc == FX_OK; ++i) rc = body();
+  hipGraph_t graph = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &graph);
+  hipGraphExec_t exec = nullptr;
+  if (rc != FX_OK || ec != hipSuccess || !graph || hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    return nullptr;  // the caller launches the iterations itself
+  }
+  ctx->cg_graphs.push_back(fx_cg_graph{std::move(k), key.owner, graph, exec});
+  return exec;
+}
+
+// The CG iteration on the device for vector type T. apply(x, y) computes y = A x for the
+// operator of `owner` with coefficients k. The iterations run in groups between host polls of
+// the device stop flag (4, 8, ..., 64 iterations); iterations after the stop are no-ops. With
+// graphs on, each group replays a captured graph of kGraphIters iterations (the same kernels
+// and arguments, so the same bits as launching them).
 template <typename T, typename Apply>
 int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
-              CgState* st, double rtol, double atol, int max_it, CgState& host) {
+              CgState* st, double rtol, double atol, int max_it, CgState& host, const void* owner = nullptr,
+              const fx_coeffs* kc = nullptr) {
   if (int rc = ensure_red(ctx, n)) return rc;
   hipStream_t s = ctx->stream;
   double* part = ctx->red;
@@ -309,24 +368,49 @@ int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* 
   FX_HIP(ctx, hipGetLastError());
   int launched = 0, poll = 4;
   const int32_t* done = &st->done;
+  auto body = [&]() -> int {
+    if (int rc = apply(p, ap, done)) return rc;
+    hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                       static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
+    hipLaunchKernelGGL(update_kernel<T>, g, dim3(256), 0, s, n, st, x, r, z, p, ap, inv);
+    hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, r, z, part, done);
+    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
+                       static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
+    hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
+    return FX_OK;
+  };
+  hipGraphExec_t exec = nullptr;
+  if (ctx->use_graphs && owner && kc && max_it > 0) {
+    CgGraphKey key{};
+    key.owner = owner;
+    key.n = n;
+    key.tsize = static_cast<int32_t>(sizeof(T));
+    key.initial = kc->initial;
+    key.ks = kc->stiffness;
+    key.kd = kc->damping;
+    key.km = kc->mass;
+    key.rtol = rtol;
+    key.atol = atol;
+    const void* ptrs[10] = {inv, b, x, r, z, p, ap, st, part, tmp};
+    for (int i = 0; i < 10; ++i) key.ptr[i] = ptrs[i];
+    exec = cg_graph(ctx, key, body);
+  }
   for (;;) {
     FX_HIP(ctx, hipMemcpyAsync(&host, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FX_HIP(ctx, hipStreamSynchronize(s));
     if (host.done || launched >= max_it) break;
     const int batch = std::min(poll, max_it - launched);
-    for (int i = 0; i < batch; ++i) {
-      if (int rc = apply(p, ap, done)) return rc;
-      hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
-      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
-                         static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
-      hipLaunchKernelGGL(update_kernel<T>, g, dim3(256), 0, s, n, st, x, r, z, p, ap, inv);
-      hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, r, z, part, done);
-      hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
-                         static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
-      hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
+    if (exec) {
+      const int graphs = (batch + kGraphIters - 1) / kGraphIters;
+      for (int i = 0; i < graphs; ++i) FX_HIP(ctx, hipGraphLaunch(exec, s));
+      launched += graphs * kGraphIters;
+    } else {
+      for (int i = 0; i < batch; ++i)
+        if (int rc = body()) return rc;
+      launched += batch;
     }
     FX_HIP(ctx, hipGetLastError());
-    launched += batch;
     poll = std::min(poll * 2, 64);
   }
   return FX_OK;
@@ -371,7 +455,7 @@ int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, cons
   CgState h{};
   auto apply = [&](const double* xx, double* yy, const int32_t* dn) { return operator_apply64(b, k, xx, yy, dn); };
   if (int rc = cg_device<double>(ctx, n, apply, inv, bi, xi, r, z, p, ap, st, o.relative_tolerance,
-                                 o.absolute_tolerance, o.max_iterations, h))
+                                 o.absolute_tolerance, o.max_iterations, h, b, &k))
     return rc;
   res = fx_solve_result{};
   res.converged = h.converged;
@@ -468,7 +552,7 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
     hipLaunchKernelGGL(zero_kernel<float>, grid_of(n), dim3(256), 0, s, n, dx);
     CgState h{};
     if (int rc = cg_device<float>(ctx, n, apply32, inv, rhs32, dx, r, z, p, ap, st, inner_rtol, 0.0,
-                                  o.max_iterations - total_inner, h))
+                                  o.max_iterations - total_inner, h, b, &k))
       return rc;
     total_inner += h.it;
     if (h.indefinite) res.indefinite = 1;

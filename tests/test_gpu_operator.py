@@ -223,3 +223,30 @@ def test_cg_mixed_precision_meets_rtol(ctx, name):
     # error against the all-FP64 oracle: bounded by the residual tolerance times the conditioning
     err = np.max(np.abs(x - xo)) / np.max(np.abs(xo))
     assert err <= 1e-6, err
+
+
+def test_graph_replayed_cg_gives_the_launched_bits(ctx):
+    """CG iterations replayed from captured hipGraphs against the same solves launched kernel by
+    kernel: every replay (several solves in a row, with other work and synchronisations between
+    them, a cached graph reused) gives the same bits, FP64 and mixed precision."""
+    model = S.config_block(12)
+    pm = core.Mesh(model)
+    b = core.DeviceArray.from_host(ctx, pm.to_equations(model["nodal_load"]))
+    x = core.DeviceArray(ctx, pm.neq)
+    batch = core.Batch(ctx, pm, pm.params_for(model), fp32=True)
+    ref = {}
+    ctx.set_graphs(False)
+    for kind in ("cg", "mixed"):
+        x.zero()
+        r = batch.cg(b, x, rtol=1e-10) if kind == "cg" else batch.cg_mixed(b, x, rtol=1e-10, inner_rtol=1e-3)
+        ref[kind] = (x.get(), r["iterations"])
+    ctx.set_graphs(True)
+    other = core.DeviceArray(ctx, pm.neq)
+    for rep in range(4):
+        for kind in ("cg", "mixed"):
+            x.zero()
+            r = batch.cg(b, x, rtol=1e-10) if kind == "cg" else batch.cg_mixed(b, x, rtol=1e-10, inner_rtol=1e-3)
+            np.testing.assert_array_equal(x.get(), ref[kind][0], err_msg=f"{kind} replay {rep}")
+            assert r["iterations"] == ref[kind][1]
+            batch.apply(x, other)  # eager work between the replays
+            ctx.sync()
