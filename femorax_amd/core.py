@@ -181,7 +181,7 @@ SIGNATURES = [
     ("fx_ctx_destroy", None, [vp]),
     ("fx_last_error", C.c_char_p, [vp]),
     ("fx_sync", C.c_int, [vp]),
-    ("fx_ctx_set_graphs", C.c_int, [vp, C.c_int]),
+    ("fx_ctx_set_option", C.c_int, [vp, C.c_int, i64]),
     ("fx_device_memory", C.c_int, [vp, C.POINTER(u64), C.POINTER(u64)]),
     ("fx_budget_used", u64, [vp]),
     ("fx_alloc", C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
@@ -283,7 +283,7 @@ class Context:
 
     def set_graphs(self, enable):
         """CG iterations from captured hipGraphs, or launched one by one (default)."""
-        _check(lib().fx_ctx_set_graphs(self.h, int(bool(enable))), self)
+        _check(lib().fx_ctx_set_option(self.h, 1, int(bool(enable))), self)
 
     def memory(self):
         f, t = C.c_uint64(), C.c_uint64()

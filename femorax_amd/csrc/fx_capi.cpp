@@ -95,11 +95,16 @@ void fx_ctx_destroy(fx_ctx* ctx) {
   delete ctx;
 }
 
-int fx_ctx_set_graphs(fx_ctx* ctx, int enable) {
-  if (!ctx) return fail(nullptr, FX_EINVAL, "fx_ctx_set_graphs: null context");
-  ctx->use_graphs = enable != 0;
-  if (!ctx->use_graphs) fx::release_cg_graphs(ctx, nullptr);
-  return FX_OK;
+int fx_ctx_set_option(fx_ctx* ctx, int option, int64_t value) {
+  if (!ctx) return fail(nullptr, FX_EINVAL, "fx_ctx_set_option: null context");
+  switch (option) {
+    case FX_OPT_GRAPHS:
+      ctx->use_graphs = value != 0;
+      if (!ctx->use_graphs) fx::release_cg_graphs(ctx, nullptr);
+      return FX_OK;
+    default:
+      return fail(ctx, FX_EINVAL, "fx_ctx_set_option: unknown option");
+  }
 }
 
 const char* fx_last_error(const fx_ctx* ctx) { return ctx ? ctx->error.c_str() : g_error.c_str(); }

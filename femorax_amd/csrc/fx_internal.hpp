@@ -114,7 +114,7 @@ struct fx_ctx {
   void* scalars = nullptr;  // solver state block (device)
   void* host_pinned = nullptr;
   void* pimy_params = nullptr;  // material-point driver: up to kMaxPointMaterials parameter sets
-  bool use_graphs = false;      // CG iterations replayed from captured graphs (fx_ctx_set_graphs)
+  bool use_graphs = false;      // CG iterations replayed from captured graphs (FX_OPT_GRAPHS)
   std::vector<fx_cg_graph> cg_graphs;
 };
 

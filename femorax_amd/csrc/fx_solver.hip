@@ -196,6 +196,7 @@ __global__ void direction_kernel(int64_t n, const CgState* st, const T* __restri
   if (i < n) p[i] = z[i] + beta * p[i];
 }
 
+
 template <typename T>
 __global__ void reciprocal_kernel(int64_t n, const double* __restrict__ d, T* __restrict__ inv, int32_t* zeros) {
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;

@@ -47,13 +47,14 @@ int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out);
 void fx_ctx_destroy(fx_ctx* ctx);
 const char* fx_last_error(const fx_ctx* ctx);
 int fx_sync(fx_ctx* ctx);
-/* enable = 1: CG iterations of fx_cg / fx_cg_mixed (and the solves inside fx_newmark) are
- * replayed from captured hipGraphs, 8 iterations each, cached per batch, coefficients and
- * vectors (the same kernels and arguments, so the same bits). Default 0: launched one by one.
- * Capture needs the context's stream to be the only one the process touches from other
- * threads through the null stream meanwhile (serving loops with upload threads keep it off).
- * No counterpart in the reference (its Kokkos launches are per kernel). */
-int fx_ctx_set_graphs(fx_ctx* ctx, int enable);
+/* Context options (no counterpart in the reference, whose Kokkos launches are per kernel):
+ * FX_OPT_GRAPHS (default 0): 1 replays the CG iterations of fx_cg / fx_cg_mixed (and the solves
+ *   inside fx_newmark) from captured hipGraphs, 8 iterations each, cached per batch,
+ *   coefficients and vectors (the same kernels and arguments, so the same bits). Capture needs
+ *   no other thread to use the null stream meanwhile (serving loops with upload threads keep it
+ *   off). */
+#define FX_OPT_GRAPHS 1
+int fx_ctx_set_option(fx_ctx* ctx, int option, int64_t value);
 /* Free and total device memory (src/native/device_memory_hip.cpp:6-11). */
 int fx_device_memory(fx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);
 uint64_t fx_budget_used(const fx_ctx* ctx);

@@ -235,13 +235,13 @@ def test_graph_replayed_cg_gives_the_launched_bits(ctx):
     x = core.DeviceArray(ctx, pm.neq)
     batch = core.Batch(ctx, pm, pm.params_for(model), fp32=True)
     ref = {}
-    ctx.set_graphs(False)
+    ctx.set_graphs(False)  # the reference bits: the iterations launched
     for kind in ("cg", "mixed"):
         x.zero()
         r = batch.cg(b, x, rtol=1e-10) if kind == "cg" else batch.cg_mixed(b, x, rtol=1e-10, inner_rtol=1e-3)
         ref[kind] = (x.get(), r["iterations"])
-    ctx.set_graphs(True)
     other = core.DeviceArray(ctx, pm.neq)
+    ctx.set_graphs(True)
     for rep in range(4):
         for kind in ("cg", "mixed"):
             x.zero()
@@ -250,3 +250,4 @@ def test_graph_replayed_cg_gives_the_launched_bits(ctx):
             assert r["iterations"] == ref[kind][1]
             batch.apply(x, other)  # eager work between the replays
             ctx.sync()
+    ctx.set_graphs(False)
