@@ -208,6 +208,7 @@ def bench_c5(core, S, duration=60.0, rate=200.0, burst=2000, plain_columns=2, pl
     try:
         recs = srv.run(reqs, log=log)
         served = serve.summary(recs)
+        served["loop"] = srv.last_stats
         burst_reqs = serve.mixed_requests(duration=1.0, rate=float(burst), seed=S.SEED_BASE + 5)
         for r in burst_reqs:
             r.arrival = 0.0
@@ -256,7 +257,8 @@ def bench_c5(core, S, duration=60.0, rate=200.0, burst=2000, plain_columns=2, pl
             "speedup_served_vs_plain": round(served["solves_per_s"] / plain["solves_per_s"], 1),
             "capacity_vs_plain": round(cap["solves_per_s"] / plain["solves_per_s"], 1),
             "config": {"rate_per_s": rate, "duration_s": duration, "columns": "70%, N~U{20..400}, 500 steps",
-                       "blocks": "30%, n~U{8..48}, rtol 1e-8", "hbm_budget_GB": 200, "block_workers": 3,
+                       "blocks": "30%, n~U{8..48}, rtol 1e-8", "hbm_budget_GB": 200, "block_workers": 2,
+                       "pool": "one stream, 4 iterations per admission round",
                        "precision": "blocks and columns: FP32 inner CG in FP64 refinement (FP64 return map and state)"}}
 
 

@@ -625,10 +625,10 @@ class ColumnPool:
 
     def __init__(self, ctx, max_elements, max_columns, dt, max_materials=None, max_record=None, gamma=0.5,
                  beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, graph=True, mixed=False,
-                 inner_rtol=1e-5, compact=False, waves=0):
+                 inner_rtol=1e-5, compact=False, waves=0, one_stream=False):
         self.ctx = ctx
         flags = ((0 if graph else FX_COLUMN_NO_GRAPH) | (FX_COLUMN_MIXED if mixed else 0) |
-                 (FX_COLUMN_COMPACT if compact else 0) | (int(waves) << 8))
+                 (FX_COLUMN_COMPACT if compact else 0) | (int(waves) << 8) | (8 if one_stream else 0))
         d = ColumnPoolDesc(int(max_elements), int(max_columns), int(max_materials or 16 * max_columns),
                            int(max_record or 1 << 20), gamma, beta, dt, test, tol, max_iter, cg_rtol, cg_max_it, flags,
                            inner_rtol)

@@ -740,10 +740,11 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
   // (graph branches when captured), joined back before the next iteration
   const unsigned nc = static_cast<unsigned>(a.max_columns);
   const bool mixed = (p->d.flags & FX_COLUMN_MIXED) != 0;
-  FX_HIP(p->ctx, hipEventRecord(p->fork, st));
+  const bool fork = !(p->d.flags & FX_COLUMN_ONE_STREAM);
+  if (fork) FX_HIP(p->ctx, hipEventRecord(p->fork, st));
   for (int k = 0; k < 4; ++k) {
-    hipStream_t s = p->side[k];
-    FX_HIP(p->ctx, hipStreamWaitEvent(s, p->fork, 0));
+    hipStream_t s = fork ? p->side[k] : st;
+    if (fork) FX_HIP(p->ctx, hipStreamWaitEvent(s, p->fork, 0));
     const dim3 g(nc), b(64u << k);
     switch (k + (mixed ? 4 : 0)) {
       case 0: hipLaunchKernelGGL((column_solve_kernel<1, false>), g, b, 0, s, a); break;
@@ -755,8 +756,10 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
       case 6: hipLaunchKernelGGL((column_solve_kernel<4, true>), g, b, 0, s, a); break;
       default: hipLaunchKernelGGL((column_solve_kernel<8, true>), g, b, 0, s, a); break;
     }
-    FX_HIP(p->ctx, hipEventRecord(p->join[k], s));
-    FX_HIP(p->ctx, hipStreamWaitEvent(st, p->join[k], 0));
+    if (fork) {
+      FX_HIP(p->ctx, hipEventRecord(p->join[k], s));
+      FX_HIP(p->ctx, hipStreamWaitEvent(st, p->join[k], 0));
+    }
   }
   FX_HIP(p->ctx, hipGetLastError());
   return FX_OK;

@@ -167,17 +167,21 @@ COLUMN_BYTES_PER_ELEMENT = 2 * core.PIMY_NSTATE * 8 + 9 * 8 + 7 * 3 * 8 + 8
 class Server:
     """One GPU serving soil columns (a resident column pool) and static blocks (worker
     threads, each its own stream). hbm_budget: bytes for all resident state; column_share of
-    it sizes the column pool, the rest is split between the block workers."""
+    it sizes the column pool, the rest is split between the block workers. Defaults measured on
+    C5 (Poisson 200/s): the pool's solve sizes on its one stream (forked side streams share the
+    process's hardware queues with the block solves and slow both), two block workers, four
+    pool iterations between admissions."""
 
-    def __init__(self, device=0, hbm_budget=200e9, column_share=0.5, block_workers=3, mixed=True, dt=0.005,
-                 max_columns=4096, iterations_per_call=2, keep_results=False, max_record_steps=500):
+    def __init__(self, device=0, hbm_budget=200e9, column_share=0.5, block_workers=2, mixed=True, dt=0.005,
+                 max_columns=4096, iterations_per_call=4, keep_results=False, max_record_steps=500,
+                 one_stream=True):
         self.ctx = core.Context(device)
         self.mixed, self.iters, self.keep = mixed, iterations_per_call, keep_results
         pool_elems = int(column_share * hbm_budget / COLUMN_BYTES_PER_ELEMENT)
         pool_elems = min(pool_elems, max_columns * core.FX_COLUMN_MAX_ELEMS)
         self.pool = core.ColumnPool(self.ctx, max_elements=pool_elems, max_columns=max_columns, dt=dt,
                                     max_materials=10 * max_columns, max_record=max_columns * 10 * max_record_steps,
-                                    mixed=mixed)
+                                    mixed=mixed, one_stream=one_stream)
         self.problems = BlockProblems()
         self.inbox, self.done = queue.Queue(), queue.Queue()
         per_worker = (1.0 - column_share) * hbm_budget / max(1, block_workers)
@@ -190,6 +194,7 @@ class Server:
         """Serve `requests` (sorted by arrival, seconds from the start) in real time; returns
         per-request records (kind, n, arrival, start, end, latency, result)."""
         t0 = time.perf_counter()
+        stats = collections.Counter()
         pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
         waiting = collections.deque()
         resident = {}
@@ -205,6 +210,7 @@ class Server:
                 else:
                     self.inbox.put(r)
                     blocks_out += 1
+            ta = time.perf_counter()
             while waiting:  # admission: the pool's slots and element room are the column share of HBM
                 r = waiting[0]
                 cid = self.pool.add(core.pimy_materials(r.mats), r.mat_of, r.nsteps, r.record, record=self.keep)
@@ -213,8 +219,15 @@ class Server:
                 waiting.popleft()
                 r.start = time.perf_counter()
                 resident[cid] = r
+                stats["admitted"] += 1
+            tb = time.perf_counter()
+            stats["admit_s"] += tb - ta
             if resident:
                 self.pool.run(self.iters)
+                tc = time.perf_counter()
+                stats["pool_s"] += tc - tb
+                stats["pool_calls"] += 1
+                stats["resident_sum"] += len(resident)
                 for cid in self.pool.finished():
                     r = resident.pop(cid)
                     st = self.pool.status(cid)
@@ -225,6 +238,7 @@ class Server:
                     r.end = time.perf_counter()
                     self.pool.remove(cid)
                     finished.append(r)
+                stats["finish_s"] += time.perf_counter() - tc
             while True:
                 try:
                     r = self.done.get_nowait()
@@ -247,6 +261,11 @@ class Server:
                 next_log += 5.0
                 log(f"[serve] t={time.perf_counter() - t0:.1f}s done {len(finished)} resident columns {len(resident)} "
                     f"waiting {len(waiting)} blocks out {blocks_out}")
+        self.last_stats = {k: (round(v, 3) if isinstance(v, float) else v) for k, v in stats.items()}
+        self.last_stats["span_s"] = round(time.perf_counter() - t0, 3)
+        if stats["pool_calls"]:
+            self.last_stats["ms_per_pool_call"] = round(stats["pool_s"] / stats["pool_calls"] * 1e3, 3)
+            self.last_stats["mean_resident"] = round(stats["resident_sum"] / stats["pool_calls"], 1)
         out = []
         for r in finished:
             out.append(dict(rid=r.rid, kind=r.kind, n=r.n, arrival=r.arrival, start=r.start - t0, end=r.end - t0,

@@ -359,6 +359,9 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
 #define FX_COLUMN_NO_GRAPH 2u /* launch the iteration kernels one by one instead of replaying a hipGraph */
 #define FX_COLUMN_WAVES(n) ((uint32_t)(n) << 8) /* register budget of the return-map kernels: at least n
                                  waves per SIMD (3 or 4; 1: the compiler's choice); default 3 */
+#define FX_COLUMN_ONE_STREAM 8u /* the four solve sizes one after another on the pool's stream (default:
+                                 forked onto four side streams so they overlap; one stream leaves the
+                                 other hardware queues to co-running work) */
 #define FX_COLUMN_COMPACT 4u  /* compacted return map (compacted_kernels.hpp:85-202): an elastic predictor
                                  per point, the plastic points listed, one full update per listed point
                                  (same bits as the fused kernel) */
