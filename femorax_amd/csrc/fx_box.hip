@@ -187,7 +187,20 @@ __device__ __forceinline__ T shfl_up(T v, int d) {
   return __shfl_up(v, static_cast<unsigned>(d), 64);
 }
 
+// The value of lane t - 1 by a DPP row shift (row_shr:1). Lane 0 of each 16-lane row receives
+// 0; the callers use the result only where the cell has an x-neighbour (cx > 0), and cells
+// with cx = 0 sit at lanes 0 and 8 of every row.
+__device__ __forceinline__ double lane_up1(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xf, 0xf, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ float lane_up1(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+}
+
 constexpr int kPlane = (kTileX + 1) * (kTileY + 1);  // nodes of one z-plane of a tile
+static_assert(3 * kTileZ * kPlane <= 3 * kTileNodes, "the top-face sums fit in the x buffer");
 
 // What one thread fetches of a tile before its cells are computed: the words of tile nodes t
 // and t + kTileCells, x at their free DOFs, and its cell's parameter set.
@@ -276,15 +289,15 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
                      (cx + 1) + (kTileX + 1) * (cy + 1)};
   const bool has[4] = {own, own && xe, own && ye, own && xe && ye};
   __syncthreads();  // every corner read of xs is done: the LDS takes each plane's top-face sums
-  T* top = lds;     // [kTileZ][kPlane][3]
+  T* top = lds;     // [3][kTileZ][kPlane]: consecutive cells, consecutive entries
   T F0[3][4];       // bottom face: [component][main, x edge, y edge, corner]
 #pragma unroll
   for (int k = 1; k >= 0; --k)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const T o00 = out[c][4 * k], o10 = out[c][1 + 4 * k], o01 = out[c][2 + 4 * k], o11 = out[c][3 + 4 * k];
-      const T s10 = shfl_up(o10, 1), s01 = shfl_up(o01, kTileX), s11 = shfl_up(o11, kTileX + 1);
-      const T sx = shfl_up(o11, kTileX), sy = shfl_up(o11, 1);
+      const T s10 = lane_up1(o10), s01 = shfl_up(o01, kTileX), s11 = shfl_up(o11, kTileX + 1);
+      const T sx = shfl_up(o11, kTileX), sy = lane_up1(o11);
       T m = o00;
       if (xm) m += s10;
       if (ym) m += s01;
@@ -293,7 +306,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
       if (k == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (has[r]) top[(cz * kPlane + pi[r]) * 3 + c] = f[r];
+          if (has[r]) top[(c * kTileZ + cz) * kPlane + pi[r]] = f[r];
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) F0[c][r] = f[r];
@@ -327,12 +340,12 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
     if (!has[r]) continue;
     T val[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[((cz - 1) * kPlane + pi[r]) * 3 + c] : F0[c][r];
+    for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[(c * kTileZ + cz - 1) * kPlane + pi[r]] : F0[c][r];
     commit(qx[r], qy[r], cz, val);
     if (cz == nz - 1) {
       T up[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) up[c] = top[(cz * kPlane + pi[r]) * 3 + c];
+      for (int c = 0; c < 3; ++c) up[c] = top[(c * kTileZ + cz) * kPlane + pi[r]];
       commit(qx[r], qy[r], cz + 1, up);
     }
   }
