@@ -76,7 +76,7 @@ def cpu_baseline(model, params, x, budget_s=20.0):
                       f"{per * 1e3:.1f} ms per apply"}
 
 
-def bench_c1(core, S, ctx, reps=30):
+def bench_c1(core, S, ctx, reps=30, cpu=True):
     """C1: 16^3 box, E 30 MPa, 10 kPa top traction, CG + Jacobi to rtol 1e-8 from x0 = 0; each
     solve is one blocking fx_cg call (the latency a caller sees)."""
     model = S.config_c1()
@@ -92,12 +92,30 @@ def bench_c1(core, S, ctx, reps=30):
         r = batch.cg(b, x, rtol=1e-8)
         lat.append(time.perf_counter() - t0)
     lat = np.array(lat[3:])
-    return {"dof": mesh.neq, "iterations": r["iterations"], "solves_per_s": round(1.0 / lat.mean(), 2),
-            "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
-            "note": "one blocking fx_cg per solve, rtol 1e-8, x0 = 0"}
+    out = {"dof": mesh.neq, "iterations": r["iterations"], "solves_per_s": round(1.0 / lat.mean(), 2),
+           "p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3), "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "note": "one blocking fx_cg per solve, rtol 1e-8, x0 = 0"}
+    if cpu:
+        from oracle import fxo
+
+        threads = cpu_threads()
+        om = fxo.OracleMesh(model)
+        op = fxo.elastic_params(mesh.params_for(model))
+        rhs = mesh.to_equations(model["nodal_load"])
+        om.cg(op, rhs, rtol=1e-8, lanes=1, nthreads=threads)  # warm-up
+        reps, t_total = 0, 0.0
+        while t_total < 10.0 and reps < 200:
+            t0 = time.perf_counter()
+            om.cg(op, rhs, rtol=1e-8, lanes=1, nthreads=threads)
+            t_total += time.perf_counter() - t0
+            reps += 1
+        out["cpu_baseline"] = {"value": round(reps / t_total, 2), "unit": "solves/s", "cores": threads, "kind": "port",
+                               "sample": f"{reps} oracle CG + Jacobi solves of C1 to rtol 1e-8 (VectorOps order, "
+                                         f"lanes 1), {threads} threads"}
+    return out
 
 
-def bench_c3(core, S, ctx, nsteps):
+def bench_c3(core, S, ctx, nsteps, cpu=True):
     """C3: 200 x 200 x 100 bricks (h 5 m), 10 layers, lumped mass, Lysmer dashpots, Ricker 5 Hz
     point force at the surface centre; CentralDifference at 0.5 of the stable step."""
     t0 = time.perf_counter()
@@ -122,9 +140,27 @@ def bench_c3(core, S, ctx, nsteps):
     batch.explicit_cd(dt, nsteps, u, um, v, a, dash=dash, load_eq=load_eq, load_hist=hist[warm:].ravel())
     ctx.sync()
     sec = time.perf_counter() - t0
-    return {"elements": mesh.nelem, "dof": n, "dt": dt, "steps": nsteps, "ms_per_step": round(sec / nsteps * 1e3, 4),
-            "gdof_steps_per_s": round(n * nsteps / sec / 1e9, 3),
-            "note": "force on the lattice operator kernels + fused cd_solve update per step; dashpots, Ricker source"}
+    out = {"elements": mesh.nelem, "dof": n, "dt": dt, "steps": nsteps, "ms_per_step": round(sec / nsteps * 1e3, 4),
+           "gdof_steps_per_s": round(n * nsteps / sec / 1e9, 3),
+           "note": "force on the lattice operator kernels + fused cd_solve update per step; dashpots, Ricker source"}
+    if cpu:
+        from oracle import fxo
+
+        threads = cpu_threads()
+        om = fxo.OracleMesh(model, coloring=False)
+        op = fxo.elastic_params(mesh.params_for(model))
+        m, _ = om.lumped_mass(op)
+        dashd = mesh.to_equations(model["dashpots"])
+        uh, umh, vh, ah = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        steps = 3
+        t0 = time.perf_counter()
+        om.cd_run(op, m, None, dt, steps, uh, umh, vh, ah, load_eq=load_eq, load_hist=hist[:steps].ravel(), dash=dashd,
+                  nthreads=threads)
+        per = (time.perf_counter() - t0) / steps
+        out["cpu_baseline"] = {"value": round(n / per / 1e9, 4), "unit": "GDoF*step/s", "cores": threads,
+                               "kind": "port", "sample": f"{steps} oracle CentralDifference steps of the same model, "
+                                                         f"{threads} threads, {per * 1e3:.0f} ms per step"}
+    return out
 
 
 def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cpu=True, plain=True, compact=False,
@@ -301,7 +337,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time full plain-FP64 and mixed-precision solves")
-    ap.add_argument("--c1", action="store_true", help="also time C1 (16^3 static box) solves: solves/s, latency")
+    ap.add_argument("--c1", type=int, default=1, help="time C1 (16^3 static box) solves: solves/s, latency (0: skip)")
     ap.add_argument("--c4", type=int, default=40, metavar="STEPS",
                     help="time STEPS Newmark steps of C4 (4096 PIMY soil columns) and the plain per-model path "
                          "(0: skip)")
@@ -388,7 +424,7 @@ def main():
         xs = core.DeviceArray(ctx, mesh.neq)
         res = {}
         for name, fn in (("plain_fp64", lambda: batch.cg(db, xs, rtol=1e-8, max_it=100000)),
-                         ("mixed_fp32_ir", lambda: batch.cg_mixed(db, xs, rtol=1e-8, inner_rtol=1e-4))):
+                         ("mixed_fp32_ir", lambda: batch.cg_mixed(db, xs, rtol=1e-8, inner_rtol=1e-6))):
             xs.zero()
             ctx.sync()
             t0 = time.perf_counter()
@@ -397,11 +433,26 @@ def main():
             r["seconds"] = round(time.perf_counter() - t0, 3)
             res[name] = r
             print(f"[bench] {name}: {r}", file=sys.stderr, flush=True)
+        if not args.no_cpu:  # the oracle's CG for a few iterations: time per iteration, whole solve estimated
+            from oracle import fxo
+
+            threads = cpu_threads()
+            om = fxo.OracleMesh(model, coloring=False)
+            op = fxo.elastic_params(params)
+            its = 5
+            t0 = time.perf_counter()
+            om.cg(op, b, rtol=1e-8, max_it=its, lanes=1, nthreads=threads)
+            per = (time.perf_counter() - t0) / its
+            n_it = res["plain_fp64"]["iterations"]
+            res["cpu_baseline"] = {"value": round(1.0 / (per * n_it), 6), "unit": "solves/s", "cores": threads,
+                                   "kind": "port", "sample": f"{its} oracle CG iterations ({per * 1e3:.0f} ms each, "
+                                                             f"{threads} threads), scaled to the {n_it} iterations "
+                                                             f"of the GPU solve"}
         out["c2_static_solve"] = res
     if args.c1:
-        out["c1_static_solves"] = bench_c1(core, S, ctx)
+        out["c1_static_solves"] = bench_c1(core, S, ctx, cpu=not args.no_cpu)
     if args.c3:
-        out["c3_explicit"] = bench_c3(core, S, ctx, args.c3)
+        out["c3_explicit"] = bench_c3(core, S, ctx, args.c3, cpu=not args.no_cpu)
     # the batched and served workloads: reported beside the operator metric; a failure there is
     # recorded in the line instead of losing the line
     if args.c4:

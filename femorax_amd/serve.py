@@ -13,8 +13,9 @@ while earlier requests are still on the device.
   has its own context, so its own HIP stream: block solves overlap with the column iterations
   and with each other on the device. A worker keeps the compiled batch of each block size it
   has served resident, within the block share of the budget, evicting the least recently
-  used. Mixed mode solves with FP32 inner CG inside FP64 refinement (fx_cg_mixed); plain mode
-  uses FP64 CG (fx_cg).
+  used. Mixed mode solves with FP32 inner CG inside FP64 refinement (fx_cg_mixed, each inner
+  solve to 1e-6: on C2 that is 0.53 s against 0.58 s for plain FP64 CG and 0.69 s at 1e-4); plain
+  mode uses FP64 CG (fx_cg).
 Each request's result is computed by the same kernels with the same per-request reductions
 alone or in the mix: a column's bits do not depend on its co-runners (the pool's solve is per
 column), and a block solve always starts from x = 0 on its own batch.
@@ -92,10 +93,11 @@ class BlockProblems:
 class BlockWorker(threading.Thread):
     """Solves block requests on its own context (stream), keeping batches resident by size."""
 
-    def __init__(self, device, problems, inbox, done, budget, mixed, rtol=1e-8, keep_x=False):
+    def __init__(self, device, problems, inbox, done, budget, mixed, rtol=1e-8, keep_x=False, inner_rtol=1e-6):
         super().__init__(daemon=True)
         self.device, self.problems, self.inbox, self.done = device, problems, inbox, done
         self.budget, self.mixed, self.rtol, self.keep_x = budget, mixed, rtol, keep_x
+        self.inner_rtol = inner_rtol
         self.error = None
 
     def run(self):
@@ -147,7 +149,7 @@ class BlockWorker(threading.Thread):
         batch, b, x, _ = resident[r.n]
         x.zero()
         if self.mixed:
-            res = batch.cg_mixed(b, x, rtol=self.rtol, inner_rtol=1e-3)
+            res = batch.cg_mixed(b, x, rtol=self.rtol, inner_rtol=self.inner_rtol)
         else:
             res = batch.cg(b, x, rtol=self.rtol)
         r.result = dict(res)
