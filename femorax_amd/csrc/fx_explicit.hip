@@ -68,6 +68,10 @@ struct CdArgs {
   double* a;
 };
 
+// RECORD: also the recorded velocity and acceleration. They feed no later displacement (the
+// dampers act at the garbage velocity), so the loop writes them only in its last two steps: the
+// last step's acceleration needs the velocity of the step before.
+template <bool RECORD>
 __global__ __launch_bounds__(kThreads) void cd_update_kernel(CdArgs g) {
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= g.n) return;
@@ -86,14 +90,16 @@ __global__ __launch_bounds__(kThreads) void cd_update_kernel(CdArgs g) {
   const double R = -fq - mq * ag;
   const double A = mq * g.c3 + cq * g.c2;
   const double U = R / A;
-  double Udot = U * 3.0;
-  Udot += uq * -4.0;
-  Udot += umq;
-  Udot *= g.c2;
-  const double Udotdot = (Udot - g.v[q]) / g.dt;
   g.um[q] = U;
-  g.v[q] = Udot;
-  g.a[q] = Udotdot;
+  if constexpr (RECORD) {
+    double Udot = U * 3.0;
+    Udot += uq * -4.0;
+    Udot += umq;
+    Udot *= g.c2;
+    const double Udotdot = (Udot - g.v[q]) / g.dt;
+    g.v[q] = Udot;
+    g.a[q] = Udotdot;
+  }
 }
 
 bool elements_damp(const fx_batch* b) {
@@ -179,7 +185,10 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
     g.um = Um;
     g.v = V;
     g.a = Acc;
-    hipLaunchKernelGGL(cd_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
+    if (k >= d.nsteps - 2)
+      hipLaunchKernelGGL(cd_update_kernel<true>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
+    else
+      hipLaunchKernelGGL(cd_update_kernel<false>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
     std::swap(U, Um);  // U(n+1) sits in the old U(n-1) buffer
     if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state: the trial history becomes committed
   }
