@@ -107,6 +107,8 @@ FX_COLUMN_MIXED = 1
 FX_COLUMN_NO_GRAPH = 2
 FX_COLUMN_COMPACT = 4
 FX_BATCH_COMPACT = 128
+FX_BATCH_TILE4 = 256
+FX_BATCH_TILE8 = 512
 
 
 class ColumnPoolDesc(C.Structure):
@@ -447,7 +449,7 @@ class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
     def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False,
-                 gmres=False):
+                 gmres=False, tile8=None):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -466,7 +468,7 @@ class Batch:
         h = C.c_void_p()
         flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
             (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0) | \
-            (FX_BATCH_GMRES if gmres else 0)
+            (FX_BATCH_GMRES if gmres else 0) | (0 if tile8 is None else FX_BATCH_TILE8 if tile8 else FX_BATCH_TILE4)
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)

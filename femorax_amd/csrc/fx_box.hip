@@ -49,7 +49,7 @@ struct BoxArgs {
   BoxConst<T> K;
   const T* x;
   T* y;
-  T* partial;  // [tiles][kTileNodes][3]: this tile's sums at its shared nodes
+  T* partial;  // [slots][3]: the tiles' sums at their shared nodes
   int ntiles;
   int accumulate;
   const int32_t* done;
@@ -97,14 +97,14 @@ __device__ __forceinline__ void hadamard8_transposed(T (&g)[8]) {
 // y[c][q] = (K_e v)[c][q] (+ the mass term), v[c][q] from the tile's node array in LDS (xs[c][ln],
 // ln[q] the tile node of corner q). lam, mu carry dvol and the operator factor; mc = cM rho dvol
 // (MASS only; lumped: the reference's lumped mass, else consistent).
-template <typename T, bool MASS>
+template <typename T, bool MASS, int NODES>
 __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam, T mu, T mc, bool lumped,
                                           const BoxConst<T>& K, T (&y)[3][8]) {
   T b[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * kTileNodes + ln[q]];
+    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * NODES + ln[q]];
     hadamard8(b[c]);
   }
   const T m2 = mu + mu;
@@ -178,7 +178,7 @@ __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
-      for (int q = 0; q < 8; ++q) y[c][q] += mc * xs[c * kTileNodes + ln[q]];
+      for (int q = 0; q < 8; ++q) y[c][q] += mc * xs[c * NODES + ln[q]];
   }
 }
 
@@ -200,55 +200,72 @@ __device__ __forceinline__ float lane_up1(float v) {
 }
 
 constexpr int kPlane = (kTileX + 1) * (kTileY + 1);  // nodes of one z-plane of a tile
-static_assert(3 * kTileZ * kPlane <= 3 * kTileNodes, "the top-face sums fit in the x buffer");
+constexpr int kTileThreads = kTileX * kTileY * 4;     // one wave per z-plane of cells, four planes a pass
 
-// What one thread fetches of a tile before its cells are computed: the words of tile nodes t
-// and t + kTileCells, x at their free DOFs, and its cell's parameter set.
-template <typename T>
-struct TileFetch {
-  uint32_t w[2];
-  int32_t s[2];
-  T v[2][3];
-  int pidx;
+// Tiles of 8 x 8 x TZ cells (TZ = 4 or 8): 256 threads, TZ / 4 passes of four cell planes (a
+// thread takes cell (cx, cy, wave + 4 pass)). Every tile owns full blocks of kNodes words and
+// slots and kCells cell parameters (full-tile order, x + 8 (y + 8 z)).
+template <int TZ>
+struct TileGeom {
+  static constexpr int kNodes = kPlane * (TZ + 1);
+  static constexpr int kCells = kTileX * kTileY * TZ;
+  static constexpr int kPasses = TZ / 4;
+  static constexpr int kRounds = (kNodes + kTileThreads - 1) / kTileThreads;
+  // top-face sums of the cell planes kept in LDS: a ring of kRing planes. With two passes, pass
+  // 1 writes the slots of planes 0-2 only where pass 0 read them in the same wave, and plane
+  // 3's slot (read by pass 1) is not written again.
+  static constexpr int kRing = TZ == 4 ? 4 : 5;
 };
 
-// Every tile owns full blocks of kTileNodes words and slots (padding: word 0, slot -1) and
-// kTileCells cell parameters (full-tile order, 0xffff outside the tile): the fetch needs only
-// the tile index, so the tile header's load runs beside it.
-template <typename T, bool DIAG>
-__device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t, TileFetch<T>& f) {
-  const uint32_t* words = a.words + static_cast<size_t>(tile) * kTileNodes;
-  const int32_t* slots = a.slots + static_cast<size_t>(tile) * kTileNodes;
+// What one thread fetches of a tile before its cells are computed: the words of tile nodes
+// t + 256 r (the partial-sum slot in place of the first equation for a shared node), x at their
+// free DOFs, and its cells' parameter sets.
+template <typename T, int TZ>
+struct TileFetch {
+  uint32_t w[TileGeom<TZ>::kRounds];
+  T v[TileGeom<TZ>::kRounds][3];
+  int pidx[TileGeom<TZ>::kPasses];
+};
+
+template <typename T, bool DIAG, int TZ>
+__device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t, TileFetch<T, TZ>& f) {
+  using G = TileGeom<TZ>;
+  const uint32_t* words = a.words + static_cast<size_t>(tile) * G::kNodes;
+  const int32_t* slots = a.slots + static_cast<size_t>(tile) * G::kNodes;
+  uint32_t w[G::kRounds];
+  int32_t s[G::kRounds];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int ln = t + r * kTileCells;
-    f.w[r] = ln < kTileNodes ? words[ln] : 0u;
-    f.s[r] = ln < kTileNodes ? slots[ln] : -1;
+  for (int r = 0; r < G::kRounds; ++r) {
+    const int ln = t + r * kTileThreads;
+    w[r] = ln < G::kNodes ? words[ln] : 0u;
+    s[r] = ln < G::kNodes ? slots[ln] : 0;
   }
-  f.pidx = a.cell_param[static_cast<size_t>(tile) * kTileCells + t];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const uint32_t mask = DIAG ? 0u : (f.w[r] >> 28) & 7u;  // 0 past the tile's nodes
-    const int base = static_cast<int>(f.w[r] & 0x0fffffffu);
+  for (int p = 0; p < G::kPasses; ++p) f.pidx[p] = a.cell_param[static_cast<size_t>(tile) * G::kCells + t + kTileThreads * p];
+#pragma unroll
+  for (int r = 0; r < G::kRounds; ++r) {
+    const uint32_t mask = DIAG ? 0u : (w[r] >> 28) & 7u;  // 0 past the tile's nodes
+    const int base = static_cast<int>(w[r] & 0x0fffffffu);
     int j = 0;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       f.v[r][d] = (mask & (1u << d)) ? a.x[base + j] : T(0);
       j += (mask >> d) & 1u;
     }
+    f.w[r] = (w[r] & 0x80000000u) ? ((w[r] & 0xf0000000u) | static_cast<uint32_t>(s[r])) : w[r];
   }
 }
 
-// MODE: 0 apply without mass terms, 1 apply with mass terms, 2 diagonal. The LDS holds the
-// tile's words and x; the cell's results are summed per node and committed.
-template <typename T, int MODE>
-__device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const TileHeader& h, int pidx, T* lds,
-                                         const uint32_t* wsh, const int32_t* ssh, int t) {
+// One pass of a tile: cell (cx, cy, cz = wave + 4 pass). MODE: 0 apply without mass terms, 1
+// apply with mass terms, 2 diagonal. The cell's results are summed per node and committed.
+template <typename T, int MODE, int TZ>
+__device__ __forceinline__ void box_tile(const BoxArgs<T>& a, const TileHeader& h, int pidx, const T* xs,
+                                         const uint32_t* wsh, T* top, int t, int pass) {
+  using G = TileGeom<TZ>;
   constexpr bool DIAG = MODE == 2;
   const int nx = h.nx, ny = h.ny, nz = h.nz;
   const int NX = nx + 1, NY = ny + 1;
-  const T* xs = lds;
-  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY);
+  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY) + 4 * pass;
   T out[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -274,23 +291,22 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
       int ln[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * ((cy + ((q >> 1) & 1)) + NY * (cz + (q >> 2)));
-      box_apply<T, MODE == 1>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
-                              static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
+      box_apply<T, MODE == 1, G::kNodes>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
+                                         static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
     }
   }
   // in-plane sums: node (cx, cy) of the cell's bottom (k = 0) and top (k = 1) faces, plus the
   // x = nx column, y = ny row and their corner for the last cells. Cells (cx-1, cy), (cx, cy-1)
-  // and (cx-1, cy-1) are lanes t-1, t-8 and t-9 of the same wave. Top-face sums go to LDS (the
-  // plane above reads them), bottom-face sums stay in registers.
+  // and (cx-1, cy-1) are lanes t-1, t-8 and t-9 of the same wave. Top-face sums go to the LDS
+  // ring (the plane above reads them), bottom-face sums stay in registers.
   const bool xm = cx > 0, ym = cy > 0;
   const bool xe = cx == nx - 1, ye = cy == ny - 1;
   const bool own = cx < nx && cy < ny && cz < nz;
   const int pi[4] = {cx + (kTileX + 1) * cy, (cx + 1) + (kTileX + 1) * cy, cx + (kTileX + 1) * (cy + 1),
                      (cx + 1) + (kTileX + 1) * (cy + 1)};
   const bool has[4] = {own, own && xe, own && ye, own && xe && ye};
-  __syncthreads();  // every corner read of xs is done: the LDS takes each plane's top-face sums
-  T* top = lds;     // [3][kTileZ][kPlane]: consecutive cells, consecutive entries
-  T F0[3][4];       // bottom face: [component][main, x edge, y edge, corner]
+  auto slot = [&](int c, int plane) { return (c * G::kRing + plane % G::kRing) * kPlane; };
+  T F0[3][4];  // bottom face: [component][main, x edge, y edge, corner]
 #pragma unroll
   for (int k = 1; k >= 0; --k)
 #pragma unroll
@@ -306,7 +322,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
       if (k == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
-          if (has[r]) top[(c * kTileZ + cz) * kPlane + pi[r]] = f[r];
+          if (has[r]) top[slot(c, cz) + pi[r]] = f[r];
       } else {
 #pragma unroll
         for (int r = 0; r < 4; ++r) F0[c][r] = f[r];
@@ -319,7 +335,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
     const int ln = px + NX * (py + NY * pz);
     const uint32_t w = wsh[ln];
     if (w & 0x80000000u) {  // a shared node: this tile's partial-sum slot
-      T* part = a.partial + static_cast<size_t>(ssh[ln]) * 3;
+      T* part = a.partial + static_cast<size_t>(w & 0x0fffffffu) * 3;
 #pragma unroll
       for (int d = 0; d < 3; ++d) part[d] = val[d];
       return;
@@ -340,12 +356,12 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
     if (!has[r]) continue;
     T val[3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[(c * kTileZ + cz - 1) * kPlane + pi[r]] : F0[c][r];
+    for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[slot(c, cz - 1) + pi[r]] : F0[c][r];
     commit(qx[r], qy[r], cz, val);
     if (cz == nz - 1) {
       T up[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) up[c] = top[(c * kTileZ + cz) * kPlane + pi[r]];
+      for (int c = 0; c < 3; ++c) up[c] = top[slot(c, cz) + pi[r]];
       commit(qx[r], qy[r], cz + 1, up);
     }
   }
@@ -354,38 +370,39 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int tile, const Ti
 // One workgroup per tile (PERSIST: each workgroup walks tiles blockIdx.x, + gridDim.x, ...
 // and fetches the next tile's words and x while it computes the current one). WAVES: the
 // minimum waves per SIMD the register allocation must allow (0: the compiler's choice).
-template <typename T, int MODE, int WAVES, bool PERSIST>
-__global__ __launch_bounds__(kTileCells, WAVES > 0 ? WAVES : 1) void hex8_box_kernel(BoxArgs<T> a) {
+template <typename T, int MODE, int WAVES, bool PERSIST, int TZ>
+__global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_kernel(BoxArgs<T> a) {
+  using G = TileGeom<TZ>;
   constexpr bool DIAG = MODE == 2;
-  __shared__ T lds[3 * kTileNodes];  // x at the tile's nodes, then the top-face partial sums
-  __shared__ uint32_t wsh[kTileNodes];
-  __shared__ int32_t ssh[kTileNodes];
+  __shared__ T xs[3 * G::kNodes];               // x at the tile's nodes
+  __shared__ uint32_t wsh[G::kNodes];           // node words (slot for shared nodes)
+  __shared__ T top[3 * G::kRing * kPlane];      // top-face sums of the cell planes
   const int t = threadIdx.x;
   if (a.done && *a.done) return;
   int tile = blockIdx.x;
   TileHeader h = a.tiles[tile];
-  TileFetch<T> f;
-  fetch_tile<T, DIAG>(a, tile, t, f);
+  TileFetch<T, TZ> f;
+  fetch_tile<T, DIAG, TZ>(a, tile, t, f);
   for (;;) {
 #pragma unroll
-    for (int r = 0; r < 2; ++r) {
-      const int ln = t + r * kTileCells;
-      if (ln < kTileNodes) {
+    for (int r = 0; r < G::kRounds; ++r) {
+      const int ln = t + r * kTileThreads;
+      if (ln < G::kNodes) {
         wsh[ln] = f.w[r];
-        ssh[ln] = f.s[r];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) lds[d * kTileNodes + ln] = f.v[r][d];
+        for (int d = 0; d < 3; ++d) xs[d * G::kNodes + ln] = f.v[r][d];
       }
     }
     __syncthreads();
     const int next = PERSIST ? tile + static_cast<int>(gridDim.x) : a.ntiles;
     TileHeader hn = h;
-    TileFetch<T> fn = f;
+    TileFetch<T, TZ> fn = f;
     if (PERSIST && next < a.ntiles) {
       hn = a.tiles[next];
-      fetch_tile<T, DIAG>(a, next, t, fn);
+      fetch_tile<T, DIAG, TZ>(a, next, t, fn);
     }
-    box_tile<T, MODE>(a, tile, h, f.pidx, lds, wsh, ssh, t);
+#pragma unroll
+    for (int pass = 0; pass < G::kPasses; ++pass) box_tile<T, MODE, TZ>(a, h, f.pidx[pass], xs, wsh, top, t, pass);
     if (next >= a.ntiles) break;
     __syncthreads();  // the LDS takes the next tile
     tile = next;
@@ -445,7 +462,7 @@ int resident_tiles(fx_ctx* ctx, const void* kernel) {
   auto it = cache.find(key);
   if (it != cache.end()) return it->second;
   int per_cu = 0, ncu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTileCells, 0) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTileThreads, 0) != hipSuccess ||
       hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess) {
     (void)hipGetLastError();
     return 0;
@@ -468,18 +485,21 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   // mass and diagonal kernels keep the compiler's choice); bit 2: persistent workgroups with
   // prefetch
   using Kernel = void (*)(BoxArgs<T>);
-  static const Kernel table[3][8] = {
-      {hex8_box_kernel<T, 0, 0, false>, hex8_box_kernel<T, 0, 5, false>, hex8_box_kernel<T, 0, 7, false>,
-       hex8_box_kernel<T, 0, 8, false>, hex8_box_kernel<T, 0, 0, true>, hex8_box_kernel<T, 0, 5, true>,
-       hex8_box_kernel<T, 0, 7, true>, hex8_box_kernel<T, 0, 8, true>},
-      {hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, false>,
-       hex8_box_kernel<T, 1, 0, false>, hex8_box_kernel<T, 1, 0, true>, hex8_box_kernel<T, 1, 0, true>,
-       hex8_box_kernel<T, 1, 0, true>, hex8_box_kernel<T, 1, 0, true>},
-      {hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, false>,
-       hex8_box_kernel<T, 2, 0, false>, hex8_box_kernel<T, 2, 0, true>, hex8_box_kernel<T, 2, 0, true>,
-       hex8_box_kernel<T, 2, 0, true>, hex8_box_kernel<T, 2, 0, true>}};
+#define FX_BOX_TABLE(TZ)                                                                                         \
+  {{hex8_box_kernel<T, 0, 0, false, TZ>, hex8_box_kernel<T, 0, 5, false, TZ>, hex8_box_kernel<T, 0, 7, false, TZ>, \
+    hex8_box_kernel<T, 0, 8, false, TZ>, hex8_box_kernel<T, 0, 0, true, TZ>, hex8_box_kernel<T, 0, 5, true, TZ>,    \
+    hex8_box_kernel<T, 0, 7, true, TZ>, hex8_box_kernel<T, 0, 8, true, TZ>},                                       \
+   {hex8_box_kernel<T, 1, 0, false, TZ>, hex8_box_kernel<T, 1, 0, false, TZ>, hex8_box_kernel<T, 1, 0, false, TZ>, \
+    hex8_box_kernel<T, 1, 0, false, TZ>, hex8_box_kernel<T, 1, 0, true, TZ>, hex8_box_kernel<T, 1, 0, true, TZ>,    \
+    hex8_box_kernel<T, 1, 0, true, TZ>, hex8_box_kernel<T, 1, 0, true, TZ>},                                       \
+   {hex8_box_kernel<T, 2, 0, false, TZ>, hex8_box_kernel<T, 2, 0, false, TZ>, hex8_box_kernel<T, 2, 0, false, TZ>, \
+    hex8_box_kernel<T, 2, 0, false, TZ>, hex8_box_kernel<T, 2, 0, true, TZ>, hex8_box_kernel<T, 2, 0, true, TZ>,    \
+    hex8_box_kernel<T, 2, 0, true, TZ>, hex8_box_kernel<T, 2, 0, true, TZ>}}
+  static const Kernel table4[3][8] = FX_BOX_TABLE(4);
+  static const Kernel table8[3][8] = FX_BOX_TABLE(8);
+#undef FX_BOX_TABLE
   const int v = b->box_variant & 7;
-  const Kernel kernel = table[DIAG ? 2 : mass ? 1 : 0][v];
+  const Kernel kernel = (b->tile_z == 8 ? table8 : table4)[DIAG ? 2 : mass ? 1 : 0][v];
   int grid = b->ntiles;
   if (v & 4) {
     const int resident = resident_tiles(b->ctx, reinterpret_cast<const void*>(kernel));
@@ -517,7 +537,7 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
     b->ctx->error = "operator: no partial-sum buffer for this precision";
     return FX_EINVAL;
   }
-  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kTileCells), 0, b->ctx->stream, a);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kTileThreads), 0, b->ctx->stream, a);
   if (b->nshared > 0)
     hipLaunchKernelGGL(hex8_shared_sum_kernel<T>, dim3((b->nshared + 255) / 256), dim3(256), 0, b->ctx->stream,
                        b->shared_word, b->shared_start, b->shared_meta, a.partial, y, b->nshared, accumulate, done);

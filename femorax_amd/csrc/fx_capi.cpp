@@ -287,7 +287,12 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   fx::Lattice lat;
   const bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
                            fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
-                                             (flags & FX_BATCH_XCD_SLABS) != 0, lat);
+                                             (flags & FX_BATCH_XCD_SLABS) != 0,
+                                             (flags & FX_BATCH_TILE4)   ? 4
+                                             : (flags & FX_BATCH_TILE8) ? 8
+                                             : E >= (1 << 20)           ? 8
+                                                                        : 4,
+                                             lat);
   if (use_lattice) {  // orphans in the internal numbering
     std::vector<int32_t> internal_orphans;
     for (int32_t i = 0; i < d->neq; ++i)
@@ -314,6 +319,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
       ok(orphans.empty() ? FX_OK : dev_upload(ctx, b->orphans, orphans.data(), orphans.size(), by));
   b->norphans = static_cast<int32_t>(orphans.size());
   if (use_lattice) {
+    b->tile_z = lat.tz;
     b->hx = lat.hx;
     b->hy = lat.hy;
     b->hz = lat.hz;

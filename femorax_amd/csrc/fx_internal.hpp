@@ -50,21 +50,24 @@ struct Blocks {
 void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nnode, const double* coords,
                   Blocks& out);
 
-// Structured fast path: tiles of kTileX x kTileY x kTileZ lattice cells (fx_lattice.cpp).
+// Structured fast path: tiles of kTileX x kTileY x (4 or 8) lattice cells (fx_lattice.cpp); a
+// workgroup of kTileCells threads takes kTileZ = 4 cell planes per pass (one pass for 4-deep
+// tiles, two for 8-deep). kTileNodes: the nodes of a 4-deep tile.
 constexpr int kTileX = 8, kTileY = 8, kTileZ = 4;
 constexpr int kTileCells = kTileX * kTileY * kTileZ;
 constexpr int kTileNodes = (kTileX + 1) * (kTileY + 1) * (kTileZ + 1);
 static_assert(kTileCells == kBlockElems, "one cell per thread");
 
 struct TileHeader {
-  int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, kTileZ)
-  int32_t word_offset;      // into the node words: tile * kTileNodes ((nx+1)(ny+1)(nz+1) used, x fastest)
-  int32_t cell_offset;      // into the cell parameters: tile * kTileCells (full-tile order, x fastest)
+  int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, tile depth)
+  int32_t word_offset;      // into the node words: tile * 81 (tz + 1) ((nx+1)(ny+1)(nz+1) used, x fastest)
+  int32_t cell_offset;      // into the cell parameters: tile * 64 tz (full-tile order, x fastest)
   int32_t pad[3];
 };
 
 struct Lattice {
   double hx = 0, hy = 0, hz = 0;
+  int tz = 4;                         // cells per tile in z
   std::vector<TileHeader> tiles;      // in launch order (XCD-aware slabs or lattice order)
   std::vector<uint32_t> words;        // per tile node: first equation (28 bits) | free-DOF mask << 28 | shared << 31
   std::vector<int32_t> slots;         // per tile node: the tile's partial-sum slot of a shared node, else -1
@@ -80,8 +83,10 @@ struct Lattice {
   int64_t nslot = 0;
 };
 
+// tile_z: cells per tile in z (4 or 8; tiles are 8 x 8 x tile_z cells)
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
-                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, Lattice& out);
+                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, int tile_z,
+                   Lattice& out);
 
 // Contraction of node-space values into equations when equations are shared by several
 // nodes (equalDOF ties): eq -> list of node DOFs (3 n + d), in ascending order.
@@ -195,6 +200,7 @@ struct fx_batch {
   double* partial64 = nullptr;        // [nslot * 3] per-tile sums at shared nodes
   float* partial32 = nullptr;
   int box_variant = 0;                // register-allocation variant of the mass-free kernel (tuned)
+  int tile_z = 8;                     // lattice tiles of 8 x 8 x tile_z cells
   // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference
   bool internal = false;
   int32_t* perm = nullptr;

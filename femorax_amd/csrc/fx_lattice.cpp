@@ -18,8 +18,9 @@
 namespace fx {
 
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
-                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, Lattice& L) {
-  if (count <= 0) return false;
+                   const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, int tile_z,
+                   Lattice& L) {
+  if (count <= 0 || (tile_z != 4 && tile_z != 8)) return false;
   const double* X = coords;
   const double* Y = coords + nnode;
   const double* Z = coords + 2 * static_cast<size_t>(nnode);
@@ -90,9 +91,11 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
   }
   // tiles of kTile cells; tiles of one parity colour share no node, so the (at most 8) tiles at
   // a node have 8 different colours and their partial sums are added in colour order
-  const int TX = kTileX, TY = kTileY, TZ = kTileZ;
+  const int TX = kTileX, TY = kTileY, TZ = tile_z;
+  const int tile_nodes_full = (TX + 1) * (TY + 1) * (TZ + 1);
   const int32_t bi = (ni + TX - 1) / TX, bj = (nj + TY - 1) / TY, bk = (nk + TZ - 1) / TZ;
   L = Lattice{};
+  L.tz = tile_z;
   L.hx = hx;
   L.hy = hy;
   L.hz = hz;
@@ -132,8 +135,8 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     h.nx = nx;
     h.ny = ny;
     h.nz = nz;
-    // every tile owns a full block of kTileNodes words (its (nx+1)(ny+1)(nz+1) nodes, x fastest,
-    // then padding) and kTileCells cell parameters in full-tile order (x + 8 (y + 8 z)), so a
+    // every tile owns a full block of 81 (TZ + 1) words (its (nx+1)(ny+1)(nz+1) nodes, x fastest,
+    // then padding) and 64 TZ cell parameters in full-tile order (x + 8 (y + 8 z)), so a
     // workgroup finds its words and cells from the tile index alone
     h.word_offset = static_cast<int32_t>(tile_nodes.size());
     h.cell_offset = static_cast<int32_t>(L.cell_param.size());
@@ -157,7 +160,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       for (int32_t y = 0; y <= ny; ++y)
         for (int32_t x = 0; x <= nx; ++x)
           tile_nodes.push_back(lat_node[(i0 + x) + static_cast<size_t>(NI) * ((j0 + y) + static_cast<size_t>(NJ) * (k0 + z))]);
-    tile_nodes.resize(static_cast<size_t>(h.word_offset) + kTileNodes, -1);
+    tile_nodes.resize(static_cast<size_t>(h.word_offset) + tile_nodes_full, -1);
     L.tiles.push_back(h);
     tile_colour.push_back(static_cast<int8_t>((ti & 1) | ((tj & 1) << 1) | ((tk & 1) << 2)));
   }
@@ -276,6 +279,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     g0 = g1;
   }
   L.nslot = nslot;
+  if (nslot >= (int64_t(1) << 28)) return false;  // a shared node's slot takes the word's 28 equation bits
   // tile node words: first equation | free-DOF mask | shared flag; and for the shared nodes the
   // tile's partial-sum slot
   L.slots.assign(tile_nodes.size(), -1);

@@ -13,9 +13,11 @@ while earlier requests are still on the device.
   has its own context, so its own HIP stream: block solves overlap with the column iterations
   and with each other on the device. A worker keeps the compiled batch of each block size it
   has served resident, within the block share of the budget, evicting the least recently
-  used. Mixed mode solves with FP32 inner CG inside FP64 refinement (fx_cg_mixed, each inner
-  solve to 1e-6: on C2 that is 0.53 s against 0.58 s for plain FP64 CG and 0.69 s at 1e-4); plain
-  mode uses FP64 CG (fx_cg).
+  used. Blocks solve with FP64 CG (fx_cg) by default: up to 48^3 (346k DoF) a solve is
+  latency-bound, and FP32 inner CG inside FP64 refinement (fx_cg_mixed, block_mixed=True) needs
+  1.5-2.6x the iterations at about the same time per iteration (measured 1.4-2.6x slower); the
+  mixed form pays from C2's size up (6.4M DoF: 0.53 s against 0.58 s). Columns use the mixed
+  PCG (mixed=True), which measures slightly faster on the pool.
 Each request's result is computed by the same kernels with the same per-request reductions
 alone or in the mix: a column's bits do not depend on its co-runners (the pool's solve is per
 column), and a block solve always starts from x = 0 on its own batch.
@@ -176,7 +178,7 @@ class Server:
 
     def __init__(self, device=0, hbm_budget=200e9, column_share=0.5, block_workers=2, mixed=True, dt=0.005,
                  max_columns=4096, iterations_per_call=4, keep_results=False, max_record_steps=500,
-                 one_stream=True):
+                 one_stream=True, block_mixed=False):
         self.ctx = core.Context(device)
         self.mixed, self.iters, self.keep = mixed, iterations_per_call, keep_results
         pool_elems = int(column_share * hbm_budget / COLUMN_BYTES_PER_ELEMENT)
@@ -187,7 +189,7 @@ class Server:
         self.problems = BlockProblems()
         self.inbox, self.done = queue.Queue(), queue.Queue()
         per_worker = (1.0 - column_share) * hbm_budget / max(1, block_workers)
-        self.workers = [BlockWorker(device, self.problems, self.inbox, self.done, per_worker, mixed,
+        self.workers = [BlockWorker(device, self.problems, self.inbox, self.done, per_worker, block_mixed,
                                     keep_x=keep_results) for _ in range(block_workers)]
         for w in self.workers:
             w.start()

@@ -125,13 +125,14 @@ def test_kernel_variants_and_tile_placement_give_the_same_bits(ctx):
     assert a.info()["variant"] in range(8)
 
 
-def test_lattice_fp32_diagonal_and_mass_paths(ctx):
-    """Lattice path at a size with many tiles: FP32 apply, diagonal and the mass-term kernel
-    against the oracle."""
+@pytest.mark.parametrize("tile8", [False, True], ids=["tile8x8x4", "tile8x8x8"])
+def test_lattice_fp32_diagonal_and_mass_paths(ctx, tile8):
+    """Lattice path at a size with many tiles (both tile shapes): FP32 apply, diagonal and the
+    mass-term kernel against the oracle."""
     model = S.config_c2(n=40)
     pm = core.Mesh(model)
     params = pm.params_for(model)
-    a = core.Batch(ctx, pm, params, fp32=True)
+    a = core.Batch(ctx, pm, params, fp32=True, tile8=tile8)
     assert a.info()["lattice"] == 1 and a.info()["tiles"] > 64
     om = fxo.OracleMesh(model)
     op = fxo.elastic_params(params)
@@ -251,3 +252,41 @@ def test_graph_replayed_cg_gives_the_launched_bits(ctx):
             batch.apply(x, other)  # eager work between the replays
             ctx.sync()
     ctx.set_graphs(False)
+
+
+def test_tile8_shape_variants_and_partial_tiles(ctx):
+    """8 x 8 x 8 tiles (two passes of four cell planes): every kernel variant gives the same bits,
+    the apply matches the oracle at 42^3 (partial tiles in x, y and z) and on a box with
+    nz = 5 (one pass with cells, the second all empty)."""
+    for model in (S.config_c2(n=42), S.elastic_box(12, 3.0, layers=2, body=True, seed=9)):
+        pm = core.Mesh(model)
+        params = pm.params_for(model)
+        a = core.Batch(ctx, pm, params, tile8=True, tune=False)
+        assert a.info()["lattice"] == 1
+        om = fxo.OracleMesh(model)
+        op = fxo.elastic_params(params)
+        x = meshes.random_vector(pm.neq, 91)
+        dx = core.DeviceArray.from_host(ctx, x)
+        ya = core.DeviceArray(ctx, pm.neq)
+        outs = []
+        for v in range(8):
+            a.tune(v)
+            a.apply(dx, ya)
+            outs.append(ya.get())
+        for o in outs[1:]:
+            np.testing.assert_array_equal(outs[0], o)
+        assert rel_err(outs[0], om.apply(op, x)) <= 1e-12
+        for kk in COEFFS:
+            a.apply(dx, ya, core.coeffs(*kk))
+            assert rel_err(ya.get(), om.apply(op, x, fxo.coeffs(*kk))) <= 1e-12
+    # a 5-deep box: the second pass of every tile has no cells
+    model = S.elastic_box(5, 1.0)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    a = core.Batch(ctx, pm, params, tile8=True, tune=False)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    x = meshes.random_vector(pm.neq, 92)
+    dx, ya = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, pm.neq)
+    a.apply(dx, ya)
+    assert rel_err(ya.get(), om.apply(op, x)) <= 1e-12
