@@ -173,20 +173,6 @@ __global__ void precond_first_kernel(int64_t n, const T* __restrict__ inv, const
 }
 
 // x += alpha p; r -= alpha ap; z = inv r   (cg.cpp:55-56, 64), skipped once done
-template <typename T>
-__global__ void update_kernel(int64_t n, const CgState* st, T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
-                              const T* __restrict__ p, const T* __restrict__ ap, const T* __restrict__ inv) {
-  if (st->done) return;
-  const T alpha = static_cast<T>(st->alpha);
-  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] + (-alpha) * ap[i];
-    r[i] = ri;
-    z[i] = inv[i] * ri;
-  }
-}
-
 // p = z + beta p (cg.cpp:68)
 template <typename T>
 __global__ void direction_kernel(int64_t n, const CgState* st, const T* __restrict__ z, T* __restrict__ p) {
@@ -194,6 +180,45 @@ __global__ void direction_kernel(int64_t n, const CgState* st, const T* __restri
   const T beta = static_cast<T>(st->beta);
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) p[i] = z[i] + beta * p[i];
+}
+
+// update_kernel followed by chunk_dot_kernel(r, r, r, z) in one pass: workgroup c updates x, r, z
+// on chunk c (lane l on entries l, l + 64, ...) and forms the chunk's lane sums of r.r and r.z
+// from the updated values in VectorOps' order, so x, r, z and the partials have the bits of the
+// two kernels.
+template <typename T>
+__global__ __launch_bounds__(kLanes) void update_dot_kernel(int64_t n, const CgState* st, T* __restrict__ x,
+                                                            T* __restrict__ r, T* __restrict__ z, const T* __restrict__ p,
+                                                            const T* __restrict__ ap, const T* __restrict__ inv,
+                                                            double* __restrict__ partial) {
+  if (st->done) return;
+  __shared__ double lane_sum[2][kLanes];
+  const T alpha = static_cast<T>(st->alpha);
+  const int64_t c = blockIdx.x;
+  const int64_t begin = c * kChunk;
+  const int64_t len = (n - begin < kChunk) ? n - begin : kChunk;
+  const int l = threadIdx.x;
+  double s = 0.0, s2 = 0.0;
+  for (int64_t k = l; k < len; k += kLanes) {
+    const int64_t i = begin + k;
+    x[i] += alpha * p[i];
+    const T ri = r[i] + (-alpha) * ap[i];
+    r[i] = ri;
+    const T zi = inv[i] * ri;
+    z[i] = zi;
+    s += as_double(ri) * as_double(ri);
+    s2 += as_double(ri) * as_double(zi);
+  }
+  lane_sum[0][l] = s;
+  lane_sum[1][l] = s2;
+  __syncthreads();
+  if (l == 0) {
+    double t = 0.0, t2 = 0.0;
+    for (int j = 0; j < kLanes; ++j) t += lane_sum[0][j];
+    for (int j = 0; j < kLanes; ++j) t2 += lane_sum[1][j];
+    partial[2 * c] = t;
+    partial[2 * c + 1] = t2;
+  }
 }
 
 
@@ -374,8 +399,7 @@ int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* 
     hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
                        static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
-    hipLaunchKernelGGL(update_kernel<T>, g, dim3(256), 0, s, n, st, x, r, z, p, ap, inv);
-    hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, r, z, part, done);
+    hipLaunchKernelGGL(update_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, st, x, r, z, p, ap, inv, part);
     hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
                        static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
     hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);

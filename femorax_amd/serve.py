@@ -15,9 +15,8 @@ while earlier requests are still on the device.
   has served resident, within the block share of the budget, evicting the least recently
   used. Blocks solve with FP64 CG (fx_cg) by default: up to 48^3 (346k DoF) a solve is
   latency-bound, and FP32 inner CG inside FP64 refinement (fx_cg_mixed, block_mixed=True) needs
-  1.5-2.6x the iterations at about the same time per iteration (measured 1.4-2.6x slower); the
-  mixed form pays from C2's size up (6.4M DoF: 0.53 s against 0.58 s). Columns use the mixed
-  PCG (mixed=True), which measures slightly faster on the pool.
+  1.5-2.6x the iterations at about the same time per iteration (measured 1.4-2.6x slower). Columns
+  use the mixed PCG (mixed=True), which measures slightly faster on the pool.
 Each request's result is computed by the same kernels with the same per-request reductions
 alone or in the mix: a column's bits do not depend on its co-runners (the pool's solve is per
 column), and a block solve always starts from x = 0 on its own batch.
