@@ -182,21 +182,24 @@ __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam
   }
 }
 
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <typename T>
 __device__ __forceinline__ T shfl_up(T v, int d) {
   return __shfl_up(v, static_cast<unsigned>(d), 64);
 }
 
 // The value of lane t - 1 by a DPP row shift (row_shr:1). Lane 0 of each 16-lane row receives
-// 0; the callers use the result only where the cell has an x-neighbour (cx > 0), and cells
-// with cx = 0 sit at lanes 0 and 8 of every row.
+// 0 (bound_ctrl); the callers weigh the result by 0 where the cell has no x-neighbour (cx = 0:
+// lanes 0 and 8 of every row).
 __device__ __forceinline__ double lane_up1(double v) {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xf, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xf, 0xf, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x111, 0xf, 0xf, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x111, 0xf, 0xf, true);
   return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ float lane_up1(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x111, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x111, 0xf, 0xf, true));
 }
 
 constexpr int kPlane = (kTileX + 1) * (kTileY + 1);  // nodes of one z-plane of a tile
@@ -245,12 +248,17 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t,
 #pragma unroll
   for (int r = 0; r < G::kRounds; ++r) {
     const uint32_t mask = DIAG ? 0u : (w[r] >> 28) & 7u;  // 0 past the tile's nodes
-    const int base = static_cast<int>(w[r] & 0x0fffffffu);
-    int j = 0;
+    const T* src = a.x + (w[r] & 0x0fffffffu);
+    if (mask == 7u) {  // a node with three free DOFs (almost all): three consecutive equations
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      f.v[r][d] = (mask & (1u << d)) ? a.x[base + j] : T(0);
-      j += (mask >> d) & 1u;
+      for (int d = 0; d < 3; ++d) f.v[r][d] = src[d];
+    } else {
+      int j = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        f.v[r][d] = (mask & (1u << d)) ? src[j] : T(0);
+        j += (mask >> d) & 1u;
+      }
     }
     f.w[r] = (w[r] & 0x80000000u) ? ((w[r] & 0xf0000000u) | static_cast<uint32_t>(s[r])) : w[r];
   }
@@ -258,14 +266,18 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t,
 
 // One pass of a tile: cell (cx, cy, cz = wave + 4 pass). MODE: 0 apply without mass terms, 1
 // apply with mass terms, 2 diagonal. The cell's results are summed per node and committed.
+//
+// Every lane runs the full 8 x 8 plane: a cell past the end of a cut tile has no parameter set
+// (zero results) and a node past it has no word (nothing fetched, nothing committed), so the
+// node sums need no bounds. Node (x, y, z) of the tile sits at x + 9 (y + 9 z).
 template <typename T, int MODE, int TZ>
-__device__ __forceinline__ void box_tile(const BoxArgs<T>& a, const TileHeader& h, int pidx, const T* xs,
-                                         const uint32_t* wsh, T* top, int t, int pass) {
+__device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T* xs, const uint32_t* wsh, T* top,
+                                         int t, int pass) {
   using G = TileGeom<TZ>;
   constexpr bool DIAG = MODE == 2;
-  const int nx = h.nx, ny = h.ny, nz = h.nz;
-  const int NX = nx + 1, NY = ny + 1;
-  const int cx = t % kTileX, cy = (t / kTileX) % kTileY, cz = t / (kTileX * kTileY) + 4 * pass;
+  constexpr int NX = kTileX + 1;
+  const int cx = t % kTileX, cy = (t / kTileX) % kTileY;
+  const int cz = __builtin_amdgcn_readfirstlane(t / (kTileX * kTileY)) + 4 * pass;  // one plane per wave
   T out[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c)
@@ -290,35 +302,34 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, const TileHeader& 
     } else {
       int ln[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * ((cy + ((q >> 1) & 1)) + NY * (cz + (q >> 2)));
+      for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * ((cy + ((q >> 1) & 1)) + (kTileY + 1) * (cz + (q >> 2)));
       box_apply<T, MODE == 1, G::kNodes>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
                                          static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
     }
   }
-  // in-plane sums: node (cx, cy) of the cell's bottom (k = 0) and top (k = 1) faces, plus the
-  // x = nx column, y = ny row and their corner for the last cells. Cells (cx-1, cy), (cx, cy-1)
-  // and (cx-1, cy-1) are lanes t-1, t-8 and t-9 of the same wave. Top-face sums go to the LDS
-  // ring (the plane above reads them), bottom-face sums stay in registers.
-  const bool xm = cx > 0, ym = cy > 0;
-  const bool xe = cx == nx - 1, ye = cy == ny - 1;
-  const bool own = cx < nx && cy < ny && cz < nz;
-  const int pi[4] = {cx + (kTileX + 1) * cy, (cx + 1) + (kTileX + 1) * cy, cx + (kTileX + 1) * (cy + 1),
-                     (cx + 1) + (kTileX + 1) * (cy + 1)};
-  const bool has[4] = {own, own && xe, own && ye, own && xe && ye};
+  // in-plane sums, x first then y: the cell's corners a (cx, cy), b (cx+1, cy), c (cx, cy+1),
+  // d (cx+1, cy+1) of its bottom (k = 0) or top (k = 1) face. Row sums p = a + b(t-1) at node
+  // (cx, cy) and q = c + d(t-1) at (cx, cy+1); then node (cx, cy) gets m = p + q(t-8), node
+  // (cx+1, cy) e = b + d(t-8). A missing neighbour (cx = 0 or cy = 0) enters with factor 0 (a
+  // fused multiply-add by 1 is the plain sum). The last column and row of cells also keep e, q
+  // and d for the x = 8 column, y = 8 row and their corner. Top-face sums go to the LDS ring
+  // (the plane above reads them), bottom-face sums stay in registers.
+  const T fx = cx > 0 ? T(1) : T(0), fy = cy > 0 ? T(1) : T(0);
+  const bool xe = cx == kTileX - 1, ye = cy == kTileY - 1;
+  const int pi[4] = {cx + NX * cy, (cx + 1) + NX * cy, cx + NX * (cy + 1), (cx + 1) + NX * (cy + 1)};
+  const bool has[4] = {true, xe, ye, xe && ye};
   auto slot = [&](int c, int plane) { return (c * G::kRing + plane % G::kRing) * kPlane; };
   T F0[3][4];  // bottom face: [component][main, x edge, y edge, corner]
 #pragma unroll
   for (int k = 1; k >= 0; --k)
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      const T o00 = out[c][4 * k], o10 = out[c][1 + 4 * k], o01 = out[c][2 + 4 * k], o11 = out[c][3 + 4 * k];
-      const T s10 = lane_up1(o10), s01 = shfl_up(o01, kTileX), s11 = shfl_up(o11, kTileX + 1);
-      const T sx = shfl_up(o11, kTileX), sy = lane_up1(o11);
-      T m = o00;
-      if (xm) m += s10;
-      if (ym) m += s01;
-      if (xm && ym) m += s11;
-      const T f[4] = {m, ym ? o10 + sx : o10, xm ? o01 + sy : o01, o11};
+      const T oa = out[c][4 * k], ob = out[c][1 + 4 * k], oc = out[c][2 + 4 * k], od = out[c][3 + 4 * k];
+      const T p = fma_t(lane_up1(ob), fx, oa);
+      const T q = fma_t(lane_up1(od), fx, oc);
+      const T m = fma_t(shfl_up(q, kTileX), fy, p);
+      const T e = fma_t(shfl_up(od, kTileX), fy, ob);
+      const T f[4] = {m, e, q, od};
       if (k == 1) {
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -330,39 +341,43 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, const TileHeader& 
     }
   __syncthreads();
   // commit: plane z = cz (bottom face + the plane below's top face), and for the last plane of
-  // cells also plane z = nz (its own top face)
-  auto commit = [&](int px, int py, int pz, const T (&val)[3]) {
-    const int ln = px + NX * (py + NY * pz);
+  // cells also plane z = TZ (its own top face)
+  auto commit = [&](int ln, const T (&val)[3]) {
     const uint32_t w = wsh[ln];
-    if (w & 0x80000000u) {  // a shared node: this tile's partial-sum slot
-      T* part = a.partial + static_cast<size_t>(w & 0x0fffffffu) * 3;
+    const uint32_t idx = w & 0x0fffffffu;
+    const bool shared = (w & 0x80000000u) != 0u;
+    const uint32_t mask = shared ? 7u : (w >> 28) & 7u;
+    // a shared node: this tile's partial-sum slot (always three values, never accumulated)
+    T* dst = shared ? a.partial + static_cast<size_t>(idx) * 3 : a.y + idx;
+    if (mask == 7u) {
+      T v[3] = {val[0], val[1], val[2]};
+      if (a.accumulate && !shared)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) part[d] = val[d];
-      return;
-    }
-    const uint32_t mask = (w >> 28) & 7u;
-    const int base = static_cast<int>(w & 0x0fffffffu);
-    int j = 0;
+        for (int d = 0; d < 3; ++d) v[d] = dst[d] + v[d];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      if (!(mask & (1u << d))) continue;
-      T* dst = a.y + base + j++;
-      *dst = a.accumulate ? *dst + val[d] : val[d];
+      for (int d = 0; d < 3; ++d) dst[d] = v[d];
+    } else if (mask != 0u) {
+      int j = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (!(mask & (1u << d))) continue;
+        T* o = dst + j++;
+        *o = a.accumulate ? *o + val[d] : val[d];
+      }
     }
   };
-  const int qx[4] = {cx, cx + 1, cx, cx + 1}, qy[4] = {cy, cy, cy + 1, cy + 1};
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     if (!has[r]) continue;
     T val[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[slot(c, cz - 1) + pi[r]] : F0[c][r];
-    commit(qx[r], qy[r], cz, val);
-    if (cz == nz - 1) {
+    commit(pi[r] + kPlane * cz, val);
+    if (cz == TZ - 1) {
       T up[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) up[c] = top[slot(c, cz) + pi[r]];
-      commit(qx[r], qy[r], cz + 1, up);
+      commit(pi[r] + kPlane * (cz + 1), up);
     }
   }
 }
@@ -380,7 +395,6 @@ __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_
   const int t = threadIdx.x;
   if (a.done && *a.done) return;
   int tile = blockIdx.x;
-  TileHeader h = a.tiles[tile];
   TileFetch<T, TZ> f;
   fetch_tile<T, DIAG, TZ>(a, tile, t, f);
   for (;;) {
@@ -395,18 +409,13 @@ __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_
     }
     __syncthreads();
     const int next = PERSIST ? tile + static_cast<int>(gridDim.x) : a.ntiles;
-    TileHeader hn = h;
     TileFetch<T, TZ> fn = f;
-    if (PERSIST && next < a.ntiles) {
-      hn = a.tiles[next];
-      fetch_tile<T, DIAG, TZ>(a, next, t, fn);
-    }
+    if (PERSIST && next < a.ntiles) fetch_tile<T, DIAG, TZ>(a, next, t, fn);
 #pragma unroll
-    for (int pass = 0; pass < G::kPasses; ++pass) box_tile<T, MODE, TZ>(a, h, f.pidx[pass], xs, wsh, top, t, pass);
+    for (int pass = 0; pass < G::kPasses; ++pass) box_tile<T, MODE, TZ>(a, f.pidx[pass], xs, wsh, top, t, pass);
     if (next >= a.ntiles) break;
     __syncthreads();  // the LDS takes the next tile
     tile = next;
-    h = hn;
     f = fn;
   }
 }

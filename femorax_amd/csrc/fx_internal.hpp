@@ -60,7 +60,7 @@ static_assert(kTileCells == kBlockElems, "one cell per thread");
 
 struct TileHeader {
   int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, tile depth)
-  int32_t word_offset;      // into the node words: tile * 81 (tz + 1) ((nx+1)(ny+1)(nz+1) used, x fastest)
+  int32_t word_offset;      // into the node words: tile * 81 (tz + 1), node (x, y, z) at x + 9 (y + 9 z)
   int32_t cell_offset;      // into the cell parameters: tile * 64 tz (full-tile order, x fastest)
   int32_t pad[3];
 };

@@ -135,9 +135,10 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     h.nx = nx;
     h.ny = ny;
     h.nz = nz;
-    // every tile owns a full block of 81 (TZ + 1) words (its (nx+1)(ny+1)(nz+1) nodes, x fastest,
-    // then padding) and 64 TZ cell parameters in full-tile order (x + 8 (y + 8 z)), so a
-    // workgroup finds its words and cells from the tile index alone
+    // every tile owns a full block of 81 (TZ + 1) words (node (x, y, z) at x + 9 (y + 9 z), also
+    // in a tile cut by the lattice's end: its missing nodes have no word) and 64 TZ cell
+    // parameters in full-tile order (x + 8 (y + 8 z)), so a workgroup finds its words and cells
+    // from the tile index alone and addresses its nodes with constant strides
     h.word_offset = static_cast<int32_t>(tile_nodes.size());
     h.cell_offset = static_cast<int32_t>(L.cell_param.size());
     bool any = false;
@@ -156,11 +157,12 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       L.cell_elem.resize(h.cell_offset);
       continue;
     }
+    tile_nodes.resize(static_cast<size_t>(h.word_offset) + tile_nodes_full, -1);
     for (int32_t z = 0; z <= nz; ++z)
       for (int32_t y = 0; y <= ny; ++y)
         for (int32_t x = 0; x <= nx; ++x)
-          tile_nodes.push_back(lat_node[(i0 + x) + static_cast<size_t>(NI) * ((j0 + y) + static_cast<size_t>(NJ) * (k0 + z))]);
-    tile_nodes.resize(static_cast<size_t>(h.word_offset) + tile_nodes_full, -1);
+          tile_nodes[static_cast<size_t>(h.word_offset) + x + (TX + 1) * (y + (TY + 1) * z)] =
+              lat_node[(i0 + x) + static_cast<size_t>(NI) * ((j0 + y) + static_cast<size_t>(NJ) * (k0 + z))];
     L.tiles.push_back(h);
     tile_colour.push_back(static_cast<int8_t>((ti & 1) | ((tj & 1) << 1) | ((tk & 1) << 2)));
   }
@@ -179,8 +181,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       for (int32_t t = 0; t < nt; ++t) {
         if (tile_colour[t] != c) continue;
         const TileHeader& h = L.tiles[t];
-        const int32_t nw = (h.nx + 1) * (h.ny + 1) * (h.nz + 1);
-        for (int32_t i = 0; i < nw; ++i) {
+        for (int32_t i = 0; i < tile_nodes_full; ++i) {
           const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
           if (n < 0 || touch_count[n] < 2) continue;
           touch_tile[fill[n]] = t;
