@@ -78,15 +78,24 @@ class CdDesc(C.Structure):
                 ("load_eq", C.c_void_p), ("load_hist", C.c_void_p)]
 
 
+class TimeSeries(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("count", C.c_int32), ("dt", C.c_double), ("factor", C.c_double),
+                ("start", C.c_double), ("use_last", C.c_int32)]
+
+
 class NewmarkDesc(C.Structure):
     _fields_ = [("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double), ("nsteps", C.c_int32),
                 ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
-                ("cg_max_it", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p)]
+                ("cg_max_it", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p),
+                ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32),
+                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double)]
 
 
 class NewmarkResult(C.Structure):
     _fields_ = [("steps", C.c_int32), ("newton_iterations", C.c_int32), ("max_newton_iterations", C.c_int32),
-                ("krylov_iterations", C.c_int32), ("failed", C.c_int32), ("last_test", C.c_double)]
+                ("krylov_iterations", C.c_int32), ("failed", C.c_int32), ("last_test", C.c_double),
+                ("commits", C.c_int32), ("failed_attempts", C.c_int32), ("retries", C.c_int32),
+                ("substepped", C.c_int32), ("time", C.c_double)]
 
 
 class PimyBatchDesc(C.Structure):
@@ -501,12 +510,21 @@ class Batch:
         return self
 
     def newmark(self, u, v, a, du, dt, nsteps, gamma=0.5, beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12,
-                cg_max_it=10000, ground_dir=-1, ground_accel=None):
+                cg_max_it=10000, ground_dir=-1, ground_accel=None, max_retries=0, num_sublevels=0, num_substeps=2,
+                series=None, ground_factor=1.0, time=0.0):
         """nsteps Newmark + Newton steps on device arrays u, v, a, du (in place); returns the
-        fx_newmark_result as a dict."""
+        fx_newmark_result as a dict. test: 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr.
+        series: the excitation as a uniform path series, dict(values, dt, factor=1, start=0,
+        use_last=False), evaluated at every attempt's end time (needed to substep)."""
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        sv = ts = None
+        if series is not None:
+            sv = np.ascontiguousarray(series["values"], np.float64)
+            ts = TimeSeries(sv.ctypes.data, len(sv), series["dt"], series.get("factor", 1.0), series.get("start", 0.0),
+                            int(bool(series.get("use_last", False))))
         d = NewmarkDesc(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, ground_dir,
-                        None if ga is None else ga.ctypes.data)
+                        None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
+                        None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time)
         r = NewmarkResult()
         _check(lib().fx_newmark(self.h, C.byref(d), u.ptr, v.ptr, a.ptr, du.ptr, C.byref(r)), self.ctx)
         return {f: getattr(r, f) for f, _ in NewmarkResult._fields_}

@@ -421,7 +421,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
         if (q >= 0) b->eq_dir[q] = static_cast<int8_t>(k % 3);
       }
-    if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(12) * std::max(1, d->neq), by));
+    if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(13) * std::max(1, d->neq), by));
     if (flags & FX_BATCH_GMRES)
       ok(dev_zeros(ctx, b->gm,
                    static_cast<size_t>(2 * fx::kGmresRestart + 3) * std::max(1, d->neq) + fx::kGmresRestart + 1, by));
@@ -803,7 +803,7 @@ int fx_store_tangents(fx_batch* b, const double* u, double* f) {
   if (!b || !u) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_store_tangents: null argument");
   if (!b->pimy) return FX_OK;  // elastic tangents are constant
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
-  if (int rc = fx::pimy_force64(b, u, f ? f : b->work, true)) return rc;
+  if (int rc = fx::pimy_force64(b, u, nullptr, f ? f : b->work, true)) return rc;
   FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return FX_OK;
 }
@@ -827,9 +827,15 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
                fx_newmark_result* result) {
   if (!b || !d || !u || !v || !a || !du || !result) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_newmark: null argument");
   if (!b->dyn) return fail(b->ctx, FX_EINVAL, "fx_newmark: the batch was created without FX_BATCH_DYNAMIC");
-  if (!(d->dt > 0) || !(d->beta > 0) || d->nsteps < 0 || d->max_iter <= 0 || d->ground_dir > 2 ||
-      (d->ground_dir >= 0 && !d->ground_accel))
+  if (!(d->dt > 0) || !(d->beta > 0) || d->nsteps < 0 || d->max_iter <= 0 || d->ground_dir > 2 || d->test < 0 ||
+      d->test > 2 || (d->ground_dir >= 0 && !d->ground_accel && !d->ground_series))
     return fail(b->ctx, FX_EINVAL, "fx_newmark: bad descriptor");
+  if (d->max_retries < 0 || d->num_sublevels < 0 || (d->num_sublevels > 0 && d->num_substeps < 1))
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: max_retries and num_sublevels must be non-negative, num_substeps positive");
+  if (d->ground_dir >= 0 && d->num_sublevels > 0 && !d->ground_series)
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: substeps under an excitation need its time series (ground_series)");
+  if (d->ground_series && (!d->ground_series->values || d->ground_series->count < 1 || !(d->ground_series->dt > 0)))
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: bad ground_series");
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
   return fx::newmark_run(b, *d, u, v, a, du, *result);
 }

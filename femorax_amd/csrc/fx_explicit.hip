@@ -111,7 +111,7 @@ bool elements_damp(const fx_batch* b) {
 }  // namespace
 
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch) {
-  if (b->pimy) return pimy_force64(b, u, f, false);  // the return map; no element damping
+  if (b->pimy) return pimy_force64(b, u, v, f, false);  // the return map (and the damping force at v)
   const int32_t n = b->neq;
   if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, u, f)) return rc;
   const double* cv = nullptr;
@@ -173,7 +173,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
       if (int rc = operator_apply64(b, fx_coeffs{0.0, 1.0, 0.0, 0}, ve, fd)) return rc;
     }
     if (b->pimy) {
-      if (int rc = pimy_force64(b, U, f, false)) return rc;
+      if (int rc = pimy_force64(b, U, damp ? ve : nullptr, f, false)) return rc;
     } else if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, U, f)) {
       return rc;
     }

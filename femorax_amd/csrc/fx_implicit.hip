@@ -8,11 +8,13 @@
 // up on the current operator), PCG on K_eff = K + gamma/(beta h) C + 1/(beta h^2) M with the
 // previous increment as initial guess (an increment whose solve reduced the residual by 1e-6 is
 // accepted, :223-238), the update (newmark_update :43-52), a new unbalance and the test value
-// (NormUnbalance ||B|| or NormDispIncr ||dU||, :239-246). A converged step commits the material
-// history (:365-372); a failed one restores the step's start and ends the call.
+// (NormUnbalance ||B||, NormDispIncr ||dU|| or EnergyIncr 0.5 |dU . B|, :239-246). A converged
+// attempt commits the material history (:365-372); a failed one restores its start, and the
+// step is retried and substepped as the reference's recover does (:385-402).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <functional>
 
 #include "fx_internal.hpp"
 #include "fx_kernels.hpp"
@@ -56,18 +58,38 @@ __global__ void newmark_residual_kernel(int32_t n, const double* f, const double
 
 }  // namespace
 
+// PathSeries::getFactor for a uniform series (ground_motion.cpp:17-33), times the pattern factor;
+// 0 before time 0 (GroundMotions::evaluate, :121-145).
+double series_value(const fx_time_series& s, double fact, double t) {
+  if (t < 0.0) return 0.0;
+  double a = 0.0;
+  if (!(t < s.start)) {
+    const double incr = (t - s.start) / s.dt;
+    const long long incr1 = static_cast<long long>(std::floor(incr));
+    const long long incr2 = incr1 + 1;
+    if (incr2 >= s.count) {
+      a = s.use_last ? s.factor * s.values[s.count - 1] : 0.0;
+    } else {
+      const double value1 = s.values[incr1], value2 = s.values[incr2];
+      a = s.factor * (value1 + (value2 - value1) * (incr - static_cast<double>(incr1)));
+    }
+  }
+  return fact * a;
+}
+
 int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, double* a, double* du,
                 fx_newmark_result& out) {
   fx_ctx* ctx = b->ctx;
   hipStream_t st = ctx->stream;
   const int32_t n = b->neq;
   out = fx_newmark_result{};
+  out.time = d.time;
   if (n == 0 || d.nsteps <= 0) return FX_OK;
   const size_t N = static_cast<size_t>(n);
   double* W = b->dyn;
   double *U = u, *V = v, *A = a, *dU = du;
   double *Ut = W + 4 * N, *Vt = W + 5 * N, *At = W + 6 * N, *F = W + 7 * N, *MA = W + 8 * N, *B = W + 9 * N,
-         *MR = W + 10 * N, *SCR = W + 11 * N;
+         *MR = W + 10 * N, *SCR = W + 11 * N, *dU0 = W + 12 * N;
   if (b->internal) {
     U = W;
     V = W + N;
@@ -88,12 +110,10 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     FX_HIP(ctx, hipStreamSynchronize(st));
     if (int rc = operator_apply64(b, mass, SCR, MR)) return rc;
   }
-  const double gamma = d.gamma, beta = d.beta, h = d.dt;
-  const fx_coeffs k{1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
-  const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
-  const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+  const double gamma = d.gamma, beta = d.beta;
   fx_solve_options so{d.cg_rtol, 0.0, d.cg_max_it};
-  auto unbalance = [&](double ag) -> int {
+  double ag = 0.0;
+  auto unbalance = [&]() -> int {
     if (int rc = force64(b, U, V, F, SCR)) return rc;
     if (int rc = operator_apply64(b, mass, A, MA)) return rc;
     hipLaunchKernelGGL(newmark_residual_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, F, MA,
@@ -101,22 +121,38 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     FX_HIP(ctx, hipGetLastError());
     return FX_OK;
   };
-  auto norm = [&](const double* x, double& val) -> int {
+  // test_value (:239-246): ||B||, ||dU|| or 0.5 |dU . B|, dots in the VectorOps order
+  auto test_value = [&](double& val) -> int {
     double s = 0.0;
+    if (d.test == 2) {
+      if (int rc = dot64(ctx, n, dU, B, &s)) return rc;
+      val = 0.5 * std::fabs(s);
+      return FX_OK;
+    }
+    const double* x = d.test == 0 ? B : dU;
     if (int rc = dot64(ctx, n, x, x, &s)) return rc;
     val = std::sqrt(s);
     return FX_OK;
   };
-  for (int32_t step = 0; step < d.nsteps; ++step) {
+  int32_t step = 0;  // the step of size dt being recovered (its ground_accel entry)
+  // One attempt of size h (:274-372): predictor, unbalance, Newton; a failure restores u, v, a,
+  // the initial guess and the time.
+  auto attempt = [&](double h, bool& ok) -> int {
+    const double time0 = out.time;
+    const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
+    const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
     hipLaunchKernelGGL(newmark_predict_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, U, V, A, Ut, Vt, At, a1,
                        a2, a3, a4);
-    const double ag = ground && d.ground_accel ? d.ground_accel[step] : 0.0;
-    if (int rc = unbalance(ag)) return rc;
-    int iter = 1;
-    bool ok = true;
+    out.time += h;
+    if (int rc = copy64(ctx, n, dU, dU0)) return rc;  // the solver's initial guess, restored on failure
+    ag = !ground ? 0.0 : d.ground_series ? series_value(*d.ground_series, d.ground_factor, out.time) : d.ground_accel[step];
+    const fx_coeffs k{1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
+    if (int rc = unbalance()) return rc;
+    int iter = 1, iterations = 0;
+    ok = true;
     for (;;) {
       if (b->pimy)  // update_tangent: the tangents at u
-        if (int rc = pimy_force64(b, U, SCR, true)) return rc;
+        if (int rc = pimy_force64(b, U, V, SCR, true)) return rc;
       fx_solve_result r;
       if (int rc = b->gm ? krylov64_solver(b, k, so, B, dU, r) : cg64_solver(b, k, so, B, dU, r)) return rc;
       out.krylov_iterations += r.iterations;
@@ -126,9 +162,10 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
       }
       hipLaunchKernelGGL(newmark_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, dU, U, V, A, k.damping,
                          k.mass);
-      if (int rc = unbalance(ag)) return rc;
+      if (int rc = unbalance()) return rc;
       double value = 0.0;
-      if (int rc = norm(d.test == 0 ? B : dU, value)) return rc;
+      if (int rc = test_value(value)) return rc;
+      iterations = iter;
       out.last_test = value;
       if (value <= d.tol) break;
       if (iter >= d.max_iter) {
@@ -137,16 +174,46 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
       }
       ++iter;
     }
-    out.newton_iterations += iter;
-    out.max_newton_iterations = std::max(out.max_newton_iterations, iter);
+    out.newton_iterations += iterations;
     if (!ok) {
       if (int rc = copy64(ctx, n, Ut, U)) return rc;
       if (int rc = copy64(ctx, n, Vt, V)) return rc;
       if (int rc = copy64(ctx, n, At, A)) return rc;
+      if (int rc = copy64(ctx, n, dU0, dU)) return rc;
+      out.time = time0;
+      ++out.failed_attempts;
+      return FX_OK;
+    }
+    out.max_newton_iterations = std::max(out.max_newton_iterations, iterations);
+    ++out.commits;
+    if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state
+    return FX_OK;
+  };
+  // recover (:385-402): retries at the same size, then substeps one level down
+  std::function<int(double, int, bool&)> recover = [&](double h, int level, bool& ok) -> int {
+    if (int rc = attempt(h, ok)) return rc;
+    if (ok) return FX_OK;
+    for (int r = 1; r <= d.max_retries; ++r) {
+      ++out.retries;
+      if (int rc = attempt(h, ok)) return rc;
+      if (ok) return FX_OK;
+    }
+    if (level >= d.num_sublevels) return FX_OK;
+    ++out.substepped;
+    const double hs = h / static_cast<double>(d.num_substeps);
+    for (int i = 0; i < d.num_substeps; ++i) {
+      if (int rc = recover(hs, level + 1, ok)) return rc;
+      if (!ok) return FX_OK;
+    }
+    return FX_OK;
+  };
+  for (step = 0; step < d.nsteps; ++step) {
+    bool ok = false;
+    if (int rc = recover(d.dt, 0, ok)) return rc;
+    if (!ok) {
       out.failed = 1;
       break;
     }
-    if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state
     out.steps = step + 1;
   }
   if (b->internal) {

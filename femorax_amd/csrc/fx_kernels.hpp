@@ -77,7 +77,7 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
 
 // stdBrick<PressureIndependMultiYield> batches (fx_pimy.hip).
 int pimy_batch_init(fx_batch* b);
-int pimy_force64(fx_batch* b, const double* u, double* f, bool store_tangents);
+int pimy_force64(fx_batch* b, const double* u, const double* v, double* f, bool store_tangents);
 int pimy_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
 int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 

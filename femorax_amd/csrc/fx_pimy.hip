@@ -435,7 +435,7 @@ int pimy_batch_init(fx_batch* b) {
   return FX_OK;
 }
 
-int pimy_force64(fx_batch* b, const double* u, double* f, bool store) {
+int pimy_force64(fx_batch* b, const double* u, const double* v, double* f, bool store) {
   const PimyBatchArgs a = batch_args(b, u, fx_coeffs{1.0, 0.0, 0.0, 0});
   if (store) {
     if (int rc = run_pimy<0, true>(b, a, f, nullptr)) return rc;

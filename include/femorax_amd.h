@@ -318,29 +318,52 @@ int fx_commit_state(fx_batch* b);
 int fx_pimy_state(fx_batch* b, int committed, double* dst);
 
 /* ---- implicit Newmark stage with Newton (implicit_stage.cpp) ---------------------------------- */
+/* A uniform path time series (PathSeries::getFactor, ground_motion.cpp:17-33): the value at time
+ * t is factor * (linear interpolation of values[] at (t - start) / dt); 0 before start and, past
+ * the last value, 0 or (use_last) factor * values[count - 1]. A series the reference reads with
+ * a leading zero (prepend_zero) is passed with that zero in values[]. */
+typedef struct fx_time_series {
+  const double* values;       /* host [count] */
+  int32_t count;
+  double dt, factor, start;
+  int32_t use_last;
+} fx_time_series;
 typedef struct fx_newmark_desc {
   double gamma, beta, dt;     /* Newmark constants and step */
   int32_t nsteps;
-  int32_t test;               /* 0 NormUnbalance, 1 NormDispIncr */
+  int32_t test;               /* 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr (0.5 |dU . B|) */
   double tol;
   int32_t max_iter;           /* Newton iterations per step */
   double cg_rtol;             /* the solver's relative tolerance (the reference's default 1e-12) */
   int32_t cg_max_it;
   int32_t ground_dir;         /* UniformExcitation direction 0..2, or -1 */
-  const double* ground_accel; /* host [nsteps]: the ground acceleration at the end of each step */
+  const double* ground_accel; /* host [nsteps]: the ground acceleration at the end of each step (used
+                                 when ground_series is NULL) */
+  /* recovery of a failed step (implicit_stage.cpp:262-267, 385-402): max_retries more attempts at
+   * the same size, then, while the level is below num_sublevels, num_substeps steps of 1/num_substeps
+   * the size, each recovered the same way one level down. 0, 0: none. */
+  int32_t max_retries, num_sublevels, num_substeps;
+  const fx_time_series* ground_series; /* the excitation's acceleration series, evaluated at the end time
+                                 of every attempt (GroundMotions::evaluate), or NULL; needed to substep
+                                 under an excitation */
+  double ground_factor;       /* UniformExcitation factor applied to ground_series (OpenSees -fact) */
+  double time;                /* the stage's start time (the domain time before the first step) */
 } fx_newmark_desc;
 typedef struct fx_newmark_result {
-  int32_t steps;              /* converged (committed) steps */
+  int32_t steps;              /* converged steps of size dt (each may have been substepped) */
   int32_t newton_iterations, max_newton_iterations, krylov_iterations;
-  int32_t failed;             /* 1: step `steps` did not converge; its start state is restored */
+  int32_t failed;             /* 1: step `steps` could not be recovered; its start state is restored */
   double last_test;
+  int32_t commits;            /* converged attempts (substeps included) */
+  int32_t failed_attempts, retries, substepped;
+  double time;                /* the domain time after the last converged attempt */
 } fx_newmark_result;
 /* nsteps Newmark steps, each solved by Newton with PCG + Jacobi on K + gamma/(beta dt) C +
  * 1/(beta dt^2) M (stored tangents for history materials), committing the material history after
- * each converged step. u, v, a, du: device vectors in/out (du: the Krylov initial guess, the last
- * increment out). Needs FX_BATCH_DYNAMIC (or a PIMY batch). Retries and substeps are the
- * caller's (the reference's recover loop, :374-402): a failed step returns FX_OK with
- * result.failed = 1. */
+ * each converged attempt. u, v, a, du: device vectors in/out (du: the Krylov initial guess, the
+ * last increment out). Needs FX_BATCH_DYNAMIC (or a PIMY batch). A step that fails is retried
+ * and substepped as desc asks (the reference's recover, :385-402); one that cannot be recovered
+ * returns FX_OK with result.failed = 1 (the reference stops the stage with a ModelError). */
 int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, double* a, double* du,
                fx_newmark_result* result);
 

@@ -193,20 +193,34 @@ typedef struct fxo_system {
   double* D;                  /* PIMY stored tangents [points * 36] */
   int nthreads;
 } fxo_system;
+/* PathSeries with a uniform step (ground_motion.cpp:17-33) */
+typedef struct fxo_series {
+  const double* values;
+  int32_t count;
+  double dt, factor, start;
+  int32_t use_last;
+} fxo_series;
+double fxo_series_value(const fxo_series* s, double fact, double t);
 typedef struct fxo_newmark_opts {
   double gamma, beta, dt;
   int32_t nsteps;
-  int32_t test;          /* 0 NormUnbalance, 1 NormDispIncr */
+  int32_t test;          /* 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr */
   double tol;
   int32_t max_iter;
   double cg_rtol;
   int32_t cg_max_it;
   int32_t lanes;         /* dot order: 64 as the device */
   const double* ground_accel; /* [nsteps] a_g at the end of each step, or NULL */
+  int32_t max_retries, num_sublevels, num_substeps; /* recover (implicit_stage.cpp:385-402) */
+  const fxo_series* ground_series; /* a_g = ground_factor * series(t) at each attempt's end time, or NULL */
+  double ground_factor;
+  double time;           /* the stage's start time */
 } fxo_newmark_opts;
 typedef struct fxo_newmark_stats {
   int32_t steps, newton_iterations, max_newton_iterations, krylov_iterations;
   double last_test;
+  int32_t commits, failed_attempts, retries, substepped;
+  double time;
 } fxo_newmark_stats;
 /* PCG + Jacobi on a system (cg.cpp:24-72) */
 int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,

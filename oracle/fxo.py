@@ -84,15 +84,31 @@ class System(C.Structure):
                 ("D", C.c_void_p), ("nthreads", C.c_int)]
 
 
+class Series(C.Structure):
+    _fields_ = [("values", C.c_void_p), ("count", C.c_int32), ("dt", C.c_double), ("factor", C.c_double),
+                ("start", C.c_double), ("use_last", C.c_int32)]
+
+
+def series_value(values, dt, t, factor=1.0, start=0.0, use_last=False, fact=1.0):
+    """PathSeries value at time t (uniform step), times the pattern factor."""
+    sv = np.ascontiguousarray(values, np.float64)
+    s = Series(sv.ctypes.data, len(sv), dt, factor, start, int(bool(use_last)))
+    return lib().fxo_series_value(C.cast(C.pointer(s), C.c_void_p), fact, t)
+
+
 class NewmarkOpts(C.Structure):
     _fields_ = [("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double), ("nsteps", C.c_int32),
                 ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
-                ("cg_max_it", C.c_int32), ("lanes", C.c_int32), ("ground_accel", C.c_void_p)]
+                ("cg_max_it", C.c_int32), ("lanes", C.c_int32), ("ground_accel", C.c_void_p),
+                ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32),
+                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double)]
 
 
 class NewmarkStats(C.Structure):
     _fields_ = [("steps", C.c_int32), ("newton_iterations", C.c_int32), ("max_newton_iterations", C.c_int32),
-                ("krylov_iterations", C.c_int32), ("last_test", C.c_double)]
+                ("krylov_iterations", C.c_int32), ("last_test", C.c_double), ("commits", C.c_int32),
+                ("failed_attempts", C.c_int32), ("retries", C.c_int32), ("substepped", C.c_int32),
+                ("time", C.c_double)]
 
 
 class SolveResult(C.Structure):
@@ -126,6 +142,8 @@ def lib():
         L.fxo_body_load.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p]
         L.fxo_dot.restype = C.c_double
         L.fxo_dot.argtypes = [C.c_int64, f64p, f64p, C.c_int]
+        L.fxo_series_value.restype = C.c_double
+        L.fxo_series_value.argtypes = [C.c_void_p, C.c_double, C.c_double]
         L.fxo_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, f64p, C.c_int]
         L.fxo_brick_max_omega_sq.restype = C.c_double
         L.fxo_brick_max_omega_sq.argtypes = [C.POINTER(Elastic), f64p]
@@ -341,7 +359,8 @@ class OracleMesh:
 
     def newmark(self, u, v, a, du, dt, nsteps, elastic=None, pimy=None, pimy_state=None, gamma=0.5, beta=0.25,
                 test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, ground_dir=None, ground_accel=None,
-                lanes=64, nthreads=0):
+                lanes=64, nthreads=0, max_retries=0, num_sublevels=0, num_substeps=2, series=None, ground_factor=1.0,
+                time=0.0):
         """nsteps Newmark + Newton steps (implicit_stage.cpp) on u, v, a, du (numpy, in place).
         pimy_state: [C, T] point-major histories (swapped in place on commits). Returns
         (status, stats dict)."""
@@ -357,8 +376,15 @@ class OracleMesh:
             s.C, s.T, s.D = Cs.ctypes.data, Ts.ctypes.data, D.ctypes.data
         s.nthreads = nthreads
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        ser = None
+        if series is not None:
+            sv = np.ascontiguousarray(series["values"], np.float64)
+            ser = Series(sv.ctypes.data, len(sv), series["dt"], series.get("factor", 1.0), series.get("start", 0.0),
+                         int(bool(series.get("use_last", False))))
+            keep += [sv, ser]
         o = NewmarkOpts(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, lanes,
-                        None if ga is None else ga.ctypes.data)
+                        None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
+                        None if ser is None else C.cast(C.pointer(ser), C.c_void_p), ground_factor, time)
         gd = None if ground_dir is None else np.ascontiguousarray(ground_dir, np.float64)
         st = NewmarkStats()
         rc = lib().fxo_newmark(C.byref(s), C.byref(o), None if gd is None else gd.ctypes.data, u, v, a, du, C.byref(st))

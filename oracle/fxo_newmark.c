@@ -4,8 +4,10 @@
  * Restates src/exec/implicit_stage.cpp: newmark_predict :29-40, newmark_update :43-52,
  * form_residual :55-68, the integrator constants and operator coefficients :150-159, 283-286,
  * update_tangent :180-193, form_unbalance :212-219, solve :223-238 (Newton accepts an increment
- * whose solve reduced the residual by 1e-6), test_value :239-246, the Newton attempt :274-360
- * (without retries and substeps), commit :365-372; the PCG of src/solvers/cg.cpp:24-72 with
+ * whose solve reduced the residual by 1e-6), test_value :239-246, the Newton attempt :274-372,
+ * recover :385-402 (retries, substeps) and the stage loop :404-418, the ground acceleration at
+ * each attempt's end time (GroundMotions::evaluate, ground_motion.cpp:17-33, 121-145), commit
+ * :365-372; the PCG of src/solvers/cg.cpp:24-72 with
  * Jacobi (jacobi.cpp:10-26) on a generic system (elastic operator, or the stored tangents of
  * PressureIndependMultiYield batches).
  */
@@ -19,7 +21,8 @@ static void sys_force(fxo_system* s, const double* u, const double* v, double* f
   if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, f, NULL, s->nthreads);
   else fxo_force(s->c, s->elastic, u, v, f, s->nthreads);
 }
-static void sys_store(fxo_system* s, const double* u, double* scratch) {
+static void sys_store(fxo_system* s, const double* u, const double* v, double* scratch) {
+  (void)v;
   if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, scratch, s->D, s->nthreads);
 }
 static void sys_apply(fxo_system* s, const fxo_coeffs* k, const double* x, double* y) {
@@ -216,96 +219,163 @@ int fxo_gmres_system(fxo_system* s, const fxo_coeffs* k, const double* b, double
   return status;
 }
 
+double fxo_series_value(const fxo_series* s, double fact, double t) {
+  if (t < 0.0) return 0.0; /* GroundMotion::getAccel: 0 before time 0 */
+  double a = 0.0;
+  if (!(t < s->start)) {   /* PathSeries::getFactor */
+    const double incr = (t - s->start) / s->dt;
+    const long long incr1 = (long long)floor(incr);
+    const long long incr2 = incr1 + 1;
+    if (incr2 >= s->count) {
+      a = s->use_last ? s->factor * s->values[s->count - 1] : 0.0;
+    } else {
+      const double value1 = s->values[incr1], value2 = s->values[incr2];
+      a = s->factor * (value1 + (value2 - value1) * (incr - (double)incr1));
+    }
+  }
+  return fact * a;
+}
+
+typedef struct nm_state {
+  fxo_system* s;
+  const fxo_newmark_opts* o;
+  const double* ground_dir;
+  fxo_newmark_stats* st;
+  int64_t n;
+  int32_t step;
+  double *u, *v, *a, *du, *ut, *vt, *at, *du0, *f, *tmp, *b, *mr, *scratch;
+} nm_state;
+
+/* form_unbalance (:212-219) and form_residual (:55-68): B = -f(u, v) - M a - a_g M r */
+static void nm_unbalance(nm_state* w, double ag) {
+  const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
+  sys_force(w->s, w->u, w->v, w->f);
+  sys_apply(w->s, &mass, w->a, w->tmp);
+  for (int64_t q = 0; q < w->n; ++q) {
+    double r = -w->f[q];
+    r -= w->tmp[q];
+    if (w->ground_dir) r -= ag * w->mr[q];
+    w->b[q] = r;
+  }
+}
+
+/* one attempt of size h (:274-372); 1 = converged and committed */
+static int nm_attempt(nm_state* w, double h) {
+  const fxo_newmark_opts* o = w->o;
+  const int64_t n = w->n;
+  const double gamma = o->gamma, beta = o->beta;
+  const double time0 = w->st->time;
+  const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
+  const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+  for (int64_t q = 0; q < n; ++q) { /* newmark_predict (:29-40) */
+    w->ut[q] = w->u[q];
+    w->vt[q] = w->v[q];
+    w->at[q] = w->a[q];
+    w->v[q] = w->v[q] * a1 + w->at[q] * a2;
+    w->a[q] = w->a[q] * a4 + w->vt[q] * a3;
+  }
+  w->st->time += h;
+  memcpy(w->du0, w->du, (size_t)n * sizeof(double));
+  double ag = 0.0;
+  if (w->ground_dir)
+    ag = o->ground_series ? fxo_series_value(o->ground_series, o->ground_factor, w->st->time)
+                          : (o->ground_accel ? o->ground_accel[w->step] : 0.0);
+  const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
+  nm_unbalance(w, ag);
+  int iter = 1, iterations = 0, ok = 1;
+  for (;;) {
+    sys_store(w->s, w->u, w->v, w->scratch); /* update_tangent: the tangents at u */
+    fxo_solve_result r;
+    fxo_cg_system(w->s, &k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, &r);
+    w->st->krylov_iterations += r.iterations;
+    if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
+      ok = 0;
+      break;
+    }
+    for (int64_t q = 0; q < n; ++q) { /* newmark_update (:43-52) */
+      w->u[q] += w->du[q];
+      w->v[q] += w->du[q] * k.damping;
+      w->a[q] += w->du[q] * k.mass;
+    }
+    nm_unbalance(w, ag);
+    double value;
+    if (o->test == 2) value = 0.5 * fabs(fxo_dot(n, w->du, w->b, o->lanes));
+    else if (o->test == 0) value = sqrt(fxo_dot(n, w->b, w->b, o->lanes));
+    else value = sqrt(fxo_dot(n, w->du, w->du, o->lanes));
+    iterations = iter;
+    w->st->last_test = value;
+    if (value <= o->tol) break;
+    if (iter >= o->max_iter) {
+      ok = 0;
+      break;
+    }
+    ++iter;
+  }
+  w->st->newton_iterations += iterations;
+  if (!ok) {
+    memcpy(w->u, w->ut, (size_t)n * sizeof(double));
+    memcpy(w->v, w->vt, (size_t)n * sizeof(double));
+    memcpy(w->a, w->at, (size_t)n * sizeof(double));
+    memcpy(w->du, w->du0, (size_t)n * sizeof(double));
+    w->st->time = time0;
+    ++w->st->failed_attempts;
+    return 0;
+  }
+  if (iterations > w->st->max_newton_iterations) w->st->max_newton_iterations = iterations;
+  ++w->st->commits;
+  if (w->s->pimy) { /* commit_state */
+    double* t = w->s->C;
+    w->s->C = w->s->T;
+    w->s->T = t;
+  }
+  return 1;
+}
+
+/* recover (:385-402) */
+static int nm_recover(nm_state* w, double h, int level) {
+  if (nm_attempt(w, h)) return 1;
+  for (int r = 1; r <= w->o->max_retries; ++r) {
+    ++w->st->retries;
+    if (nm_attempt(w, h)) return 1;
+  }
+  if (level >= w->o->num_sublevels) return 0;
+  ++w->st->substepped;
+  const double hs = h / (double)w->o->num_substeps;
+  for (int i = 0; i < w->o->num_substeps; ++i)
+    if (!nm_recover(w, hs, level + 1)) return 0;
+  return 1;
+}
+
 int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
                 double* du, fxo_newmark_stats* st) {
   const int64_t n = s->c->neq;
   memset(st, 0, sizeof(*st));
-  double* ut = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* vt = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* at = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* f = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* tmp = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* b = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  double* mr = (double*)calloc((size_t)n + 1, sizeof(double));
-  double* scratch = (double*)malloc((size_t)(n + 1) * sizeof(double));
-  const double gamma = o->gamma, beta = o->beta, h = o->dt;
+  st->time = o->time;
+  nm_state w;
+  w.s = s;
+  w.o = o;
+  w.ground_dir = ground_dir;
+  w.st = st;
+  w.n = n;
+  w.step = 0;
+  w.u = u;
+  w.v = v;
+  w.a = a;
+  w.du = du;
+  double** bufs[] = {&w.ut, &w.vt, &w.at, &w.du0, &w.f, &w.tmp, &w.b, &w.scratch};
+  for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) *bufs[i] = (double*)malloc((size_t)(n + 1) * sizeof(double));
+  w.mr = (double*)calloc((size_t)n + 1, sizeof(double));
   const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
-  if (ground_dir) sys_apply(s, &mass, ground_dir, mr); /* mass_times_ground */
-  const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
-  const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
-  const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+  if (ground_dir) sys_apply(s, &mass, ground_dir, w.mr); /* mass_times_ground */
   int status = 0;
-  for (int32_t step = 0; step < o->nsteps && status == 0; ++step) {
-    for (int64_t q = 0; q < n; ++q) {
-      ut[q] = u[q];
-      vt[q] = v[q];
-      at[q] = a[q];
-      v[q] = v[q] * a1 + at[q] * a2;
-      a[q] = a[q] * a4 + vt[q] * a3;
-    }
-    const double ag = o->ground_accel ? o->ground_accel[step] : 0.0;
-    /* form_unbalance: f(u) at the velocities v, inertia M a, the ground load */
-#define FORM_UNBALANCE()                                        \
-  do {                                                          \
-    sys_force(s, u, v, f);                                      \
-    sys_apply(s, &mass, a, tmp);                                \
-    for (int64_t q = 0; q < n; ++q) {                           \
-      double r = -f[q];                                         \
-      r -= tmp[q];                                              \
-      if (ground_dir) r -= ag * mr[q];                          \
-      b[q] = r;                                                 \
-    }                                                           \
-  } while (0)
-    FORM_UNBALANCE();
-    int iter = 1, ok = 1;
-    for (;;) {
-      sys_store(s, u, scratch); /* update_tangent: the tangents at u, Jacobi set up on them */
-      fxo_solve_result r;
-      fxo_cg_system(s, &k, b, du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, &r);
-      st->krylov_iterations += r.iterations;
-      if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
-        ok = 0;
-        break;
-      }
-      for (int64_t q = 0; q < n; ++q) {
-        u[q] += du[q];
-        v[q] += du[q] * k.damping;
-        a[q] += du[q] * k.mass;
-      }
-      FORM_UNBALANCE();
-      const double value = o->test == 0 ? sqrt(fxo_dot(n, b, b, o->lanes)) : sqrt(fxo_dot(n, du, du, o->lanes));
-      st->last_test = value;
-      if (value <= o->tol) break;
-      if (iter >= o->max_iter) {
-        ok = 0;
-        break;
-      }
-      ++iter;
-    }
-#undef FORM_UNBALANCE
-    st->newton_iterations += iter;
-    if (iter > st->max_newton_iterations) st->max_newton_iterations = iter;
-    if (!ok) {
-      memcpy(u, ut, (size_t)n * sizeof(double));
-      memcpy(v, vt, (size_t)n * sizeof(double));
-      memcpy(a, at, (size_t)n * sizeof(double));
+  for (w.step = 0; w.step < o->nsteps; ++w.step) {
+    if (!nm_recover(&w, o->dt, 0)) {
       status = 1;
       break;
     }
-    if (s->pimy) { /* commit_state */
-      double* t = s->C;
-      s->C = s->T;
-      s->T = t;
-    }
-    st->steps = step + 1;
+    st->steps = w.step + 1;
   }
-  free(ut);
-  free(vt);
-  free(at);
-  free(f);
-  free(tmp);
-  free(b);
-  free(mr);
-  free(scratch);
+  for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(*bufs[i]);
+  free(w.mr);
   return status;
 }

@@ -103,3 +103,68 @@ def test_pimy_column_shaken_into_plasticity(ctx):
         assert rel(g, w) <= tol
     assert history_error(batch.pimy_state(True), Cs) <= 1e-7
     assert np.max(Cs[:, 26]) > 0, "the shaking reached the plastic range"
+
+
+def test_energy_incr_test(ctx):
+    """EnergyIncr (0.5 |dU . B|, implicit_stage.cpp:239-246) on the elastic damped column: the
+    same steps, iterations and state as the oracle."""
+    model = meshes.column(16)
+    model["params"] = [dict(p, rayleigh=(0.4, 2e-4, 0.0, 0.0)) for p in model["params"]]
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    batch = core.Batch(ctx, pm, params, dynamic=True)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    h, nsteps = 0.004, 20
+    ag = 2.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 3)
+    want, got = run_both(ctx, om, batch, pm.neq, h, nsteps, dict(elastic=op), ag, test=2, tol=1e-12)
+    for g, w, tol in zip(got, want, (1e-9, 1e-8, 1e-8)):
+        assert rel(g, w) <= tol
+
+
+def test_pimy_steps_retried_and_substepped(ctx):
+    """Recovery (implicit_stage.cpp:385-402) on a PIMY column with Newton capped at 2 iterations:
+    steps that need 3 at full size fail, are retried once and split into halves (quarters where
+    a half fails), the excitation taken from its time series at every attempt's end time. The
+    same commits, retries, substeps, failed attempts and time as the oracle; the state to the
+    plastic-column tolerance."""
+    from tests.test_oracle_newmark import _pimy_column_case
+    model, om, per_set, pe, h, nsteps, series = _pimy_column_case()
+    pm = core.Mesh(model)
+    keep = []
+    # per_set follows the oracle mesh's parameter groups, which the product mesh shares
+    assert list(pm.param_group) == list(om.param_group)
+    dev = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"],
+                              nsurf=20, b=m["b"], mass_type=m["mass_type"], keep=keep) for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev)
+    batch.set_stage(1)
+    n = pm.neq
+    kw = dict(test=0, tol=1e-5, max_iter=2, max_retries=1, num_sublevels=2, num_substeps=2)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, pimy=pe, pimy_state=[Cs, Ts], ground_dir=ground_dir(om, 0),
+                        series=series, **kw)
+    assert rc == 0 and st["substepped"] > 0
+    d = [core.DeviceArray(ctx, n) for _ in range(4)]
+    for x in d:
+        x.zero()
+    r = batch.newmark(*d, h, nsteps, ground_dir=0, series=series, **kw)
+    assert r["failed"] == 0 and r["steps"] == nsteps, r
+    for key in ("commits", "retries", "substepped", "failed_attempts", "newton_iterations", "max_newton_iterations"):
+        assert r[key] == st[key], (key, r, st)
+    assert r["time"] == st["time"]
+    for g, w, tol in zip((x.get() for x in d[:3]), (u, v, a), (1e-8, 1e-7, 1e-7)):
+        assert rel(g, w) <= tol
+    assert history_error(batch.pimy_state(True), Cs) <= 1e-7
+
+
+def test_substeps_under_excitation_need_a_series(ctx):
+    model = meshes.column(4)
+    pm = core.Mesh(model)
+    batch = core.Batch(ctx, pm, pm.params_for(model), dynamic=True)
+    d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+    for x in d:
+        x.zero()
+    with pytest.raises(core.FxError):
+        batch.newmark(*d, 0.01, 2, ground_dir=0, ground_accel=np.zeros(2), num_sublevels=1)

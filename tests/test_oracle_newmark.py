@@ -83,3 +83,111 @@ def test_pimy_newmark_steps_end_in_equilibrium():
         assert np.linalg.norm(res) <= 1e-4 * 1.0001, (k, np.linalg.norm(res))
         np.testing.assert_array_equal(T2, Cs)  # the committed history is the state at u
     assert np.max(Cs[:, 26]) > 0, "the excitation reached the plastic range"
+
+
+def test_path_series_values():
+    """PathSeries::getFactor with a uniform step (ground_motion.cpp:17-33), hand-derived: linear
+    interpolation between samples, 0 before the start and at or past the last sample unless
+    use_last, 0 before time 0, the pattern factor on top."""
+    v = [0.0, 1.0, 3.0]
+    assert fxo.series_value(v, 0.5, 0.25, factor=2.0) == 1.0
+    assert fxo.series_value(v, 0.5, 0.75, factor=2.0) == 4.0
+    assert fxo.series_value(v, 0.5, 1.0, factor=2.0) == 0.0
+    assert fxo.series_value(v, 0.5, 1.0, factor=2.0, use_last=True) == 6.0
+    assert fxo.series_value(v, 0.5, 7.0, use_last=True) == 3.0
+    assert fxo.series_value(v, 0.5, -0.1, use_last=True) == 0.0
+    assert fxo.series_value(v, 0.5, 0.2, start=0.25) == 0.0
+    assert fxo.series_value(v, 0.5, 0.5, start=0.25) == 0.5
+    assert fxo.series_value(v, 0.5, 0.25, fact=3.0) == 1.5
+    # samples at the step ends reproduce the per-step accelerations
+    ag = [0.3, -0.7, 1.1]
+    assert [fxo.series_value([0.0] + ag, 0.25, 0.25 * (k + 1)) for k in range(2)] == ag[:2]
+
+
+def test_energy_incr_is_half_the_last_increments_work():
+    """EnergyIncr (implicit_stage.cpp:239-246): 0.5 |dU . B| with B the unbalance after the
+    update; B recomputed from the oracle's force, mass and ground load at the final state."""
+    model = meshes.column(8)
+    model["params"] = [dict(p, rayleigh=(0.4, 2e-4, 0.0, 0.0)) for p in model["params"]]
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    r = ground_dir(om, 0)
+    h, nsteps = 0.004, 6
+    ag = 2.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 3)
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, elastic=op, ground_dir=r, ground_accel=ag, test=2, tol=1e-12,
+                        max_iter=10)
+    assert rc == 0 and st["steps"] == nsteps
+    B = -om.force(op, u, v) - om.apply(op, a, fxo.coeffs(0.0, 0.0, 1.0)) - ag[-1] * om.apply(op, r, fxo.coeffs(0.0, 0.0, 1.0))
+    assert st["last_test"] == 0.5 * abs(fxo.dot(du, B, 64))
+    assert st["last_test"] <= 1e-12
+
+
+def _pimy_column_case():
+    from tests.test_gpu_pimy_batch import soil
+    model = meshes.column(12)
+    om = fxo.OracleMesh(model)
+    mats = soil(20261206, 3)
+    per_set = [dict(mats[g], b=(0.0, 0.0, 0.0)) for g in om.param_group]
+    pe = fxo.pimy_elems(per_set, stage=1)
+    h, nsteps = 0.01, 12
+    series = dict(values=np.concatenate([[0.0], 6.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 2)]), dt=h)
+    return model, om, per_set, pe, h, nsteps, series
+
+
+def test_retries_and_substeps_compose_single_attempts():
+    """recover (implicit_stage.cpp:385-402) inside the oracle's stage loop equals the same
+    recursion driven from here with single attempts (a one-step call without recovery is one
+    attempt that restores u, v, a, the initial guess and the time when it fails): the same bits,
+    commits, retries, substeps and times. The case (a PIMY column, Newton capped at 2
+    iterations) has steps that need 3 iterations at full size."""
+    model, om, per_set, pe, h, nsteps, series = _pimy_column_case()
+    n = om.neq
+    r = ground_dir(om, 0)
+    kw = dict(pimy=pe, ground_dir=r, series=series, test=0, tol=1e-5, max_iter=2)
+    R, L, S = 1, 2, 2
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    A = [np.zeros(n) for _ in range(4)]
+    rc, st = om.newmark(*A, h, nsteps, pimy_state=[Cs, Ts], max_retries=R, num_sublevels=L, num_substeps=S, **kw)
+    assert rc == 0 and st["steps"] == nsteps
+    assert st["substepped"] > 0 and st["retries"] > 0 and st["commits"] > nsteps
+    # the same recursion, one attempt per call
+    Cb = om.pimy_initial_state(pe)
+    Tb = Cb.copy()
+    B = [np.zeros(n) for _ in range(4)]
+    count = dict(commits=0, retries=0, substepped=0, failed_attempts=0, newton_iterations=0)
+    clock = [0.0]
+
+    def attempt(dt):
+        rc1, s1 = om.newmark(*B, dt, 1, pimy_state=[Cb, Tb], time=clock[0], **kw)
+        count["newton_iterations"] += s1["newton_iterations"]
+        if rc1 == 0:
+            count["commits"] += 1
+            clock[0] = s1["time"]
+            return True
+        assert s1["time"] == clock[0]
+        count["failed_attempts"] += 1
+        return False
+
+    def recover(dt, level):
+        if attempt(dt):
+            return True
+        for _ in range(R):
+            count["retries"] += 1
+            if attempt(dt):
+                return True
+        if level >= L:
+            return False
+        count["substepped"] += 1
+        return all(recover(dt / S, level + 1) for _ in range(S))
+
+    for _ in range(nsteps):
+        assert recover(h, 0)
+    for x, y in zip(A, B):
+        np.testing.assert_array_equal(x, y)
+    np.testing.assert_array_equal(Cs, Cb)
+    for key, val in count.items():
+        assert st[key] == val, key
+    assert st["time"] == clock[0]
