@@ -101,7 +101,7 @@ class NewmarkResult(C.Structure):
 class PimyBatchDesc(C.Structure):
     _fields_ = [("count", C.c_int32), ("padded", C.c_int32), ("node", C.c_void_p), ("eq", C.c_void_p),
                 ("param", C.c_void_p), ("nparam", C.c_int32), ("params", C.c_void_p), ("nnode", C.c_int32),
-                ("coords", C.c_void_p), ("neq", C.c_int32)]
+                ("coords", C.c_void_p), ("neq", C.c_int32), ("damp", C.c_void_p)]
 
 
 class PimyMaterial(C.Structure):
@@ -483,9 +483,10 @@ class Batch:
         ctx.adopt(self)
 
     @classmethod
-    def pimy(cls, ctx, mesh, materials, compact=False):
+    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None):
         """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
-        fx_pimy_material per compiled parameter set; compact: the compacted return map."""
+        fx_pimy_material per compiled parameter set; compact: the compacted return map;
+        rayleigh: per parameter set (alphaM, betaK, betaKinit, betaKcomm), or None."""
         self = cls.__new__(cls)
         self.ctx = ctx
         self.neq = mesh.neq
@@ -503,6 +504,9 @@ class Batch:
         d.nnode = mesh.nnode
         d.coords = _arr(keep, mesh.coords, np.float64)
         d.neq = mesh.neq
+        if rayleigh is not None:
+            self._damp = (Rayleigh * len(materials))(*[Rayleigh(*r) for r in rayleigh])
+            d.damp = C.cast(self._damp, C.c_void_p)
         h = C.c_void_p()
         _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), FX_BATCH_COMPACT if compact else 0, C.byref(h)), ctx)
         self.h = h

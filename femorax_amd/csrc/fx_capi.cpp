@@ -724,6 +724,13 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
     e.rho = pp[i].rho;
     for (int k = 0; k < 3; ++k) e.b[k] = d->params[i].b[k];
     e.mass_type = d->params[i].mass_type;
+    if (d->damp) {  // the lumped alphaM mass and the stable step see the damping too
+      e.damp = d->damp[i];
+      pp[i].alphaM = d->damp[i].alphaM;
+      pp[i].betaK = d->damp[i].betaK;
+      pp[i].betaKinit = d->damp[i].betaKinit;
+      pp[i].betaKcomm = d->damp[i].betaKcomm;
+    }
   }
   fx_batch_desc ed{};
   ed.count = d->count;
@@ -742,6 +749,8 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
     return rc;
   b->pimy = true;
   b->host_pimy = pp;
+  for (const auto& q : pp)
+    b->pimy_damped |= q.alphaM != 0.0 || q.betaK != 0.0 || q.betaKinit != 0.0 || q.betaKcomm != 0.0;
   b->npts = static_cast<int64_t>(d->count) * 8;
   // the gather: each equation's slots k * padded + e in (element, DOF) order
   const int32_t P = d->padded, n = d->neq;

@@ -151,7 +151,9 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   double* f = W + 4 * N;
   double* ve = W + 5 * N;
   double* fd = b->internal ? W + 7 * N : W + 6 * N;
-  const bool damp = !b->pimy && elements_damp(b);
+  // the elements' damping force at the garbage velocity: by the operator (C v_e) for elastic
+  // batches, inside the return-map force for PIMY batches
+  const bool edamp = elements_damp(b), damp = !b->pimy && edamp;
   const double c2 = 0.5 / d.dt, c3 = 1.0 / (d.dt * d.dt);
   PointLoads pl{};
   pl.n = d.nload;
@@ -168,12 +170,12 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   g.am = b->lumped_am;
   g.dash = dash;
   for (int32_t k = 0; k < d.nsteps; ++k) {
-    if (damp) {
+    if (edamp)
       hipLaunchKernelGGL(scale_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, n, -c2, Um, ve);
+    if (damp)
       if (int rc = operator_apply64(b, fx_coeffs{0.0, 1.0, 0.0, 0}, ve, fd)) return rc;
-    }
     if (b->pimy) {
-      if (int rc = pimy_force64(b, U, damp ? ve : nullptr, f, false)) return rc;
+      if (int rc = pimy_force64(b, U, edamp ? ve : nullptr, f, false)) return rc;
     } else if (int rc = operator_apply64(b, fx_coeffs{1.0, 0.0, 0.0, 0}, U, f)) {
       return rc;
     }

@@ -152,7 +152,7 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     ok = true;
     for (;;) {
       if (b->pimy)  // update_tangent: the tangents at u
-        if (int rc = pimy_force64(b, U, V, SCR, true)) return rc;
+        if (int rc = pimy_force64(b, U, nullptr, SCR, true)) return rc;
       fx_solve_result r;
       if (int rc = b->gm ? krylov64_solver(b, k, so, B, dU, r) : cg64_solver(b, k, so, B, dU, r)) return rc;
       out.krylov_iterations += r.iterations;

@@ -235,6 +235,7 @@ struct fx_batch {
   // time-integration workspace (FX_BATCH_DYNAMIC): 12 * neq doubles; the DOF direction of each
   // equation (reference numbering, host)
   double* dyn = nullptr;
+  bool pimy_damped = false;  // some PIMY parameter set has Rayleigh damping
   std::vector<int8_t> eq_dir;
   // GMRES workspace (FX_BATCH_GMRES): V [(m + 1) n], Z [m n], coefficients [m + 1], CG start [n]
   double* gm = nullptr;

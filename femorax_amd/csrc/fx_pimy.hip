@@ -74,6 +74,7 @@ struct PimyBatchArgs {
   double* T;
   double* tangent;  // [21 * npts] upper triangle row-major, or null
   const double* x;  // u (force) or v (operator)
+  const double* v;  // force: the velocities of the Rayleigh damping force, or null
   double* slots;    // [24 * padded]
   fx_coeffs k;
   double* cres;     // compacted return map: [33 * npts] stress, tangent (upper), strain
@@ -129,11 +130,27 @@ __device__ inline void initial_D(const PimyParams& p, double (&D)[36]) {
     }
 }
 
-// D_eff = cK D_t + c0 D_0 (Rayleigh off; element_operator.hpp:45-75)
-__device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, int64_t pt, double (&D)[36]) {
-  const double cK = a.k.initial ? 0.0 : a.k.stiffness, c0 = a.k.initial ? a.k.stiffness : 0.0;
+// The operator factors (element_operator.hpp:45-56; the PIMY initial stiffness is not its
+// tangent): cK on the stored tangent, c0 on the initial, cc on the committed tangent, cM on the
+// mass.
+__device__ inline void operator_factors(const PimyParams& p, const fx_coeffs& k, double& cK, double& c0, double& cc,
+                                        double& cM) {
+  cK = k.initial ? k.damping * p.betaK : k.stiffness + k.damping * p.betaK;
+  c0 = k.initial ? k.stiffness + k.damping * p.betaKinit : k.damping * p.betaKinit;
+  cc = k.damping * p.betaKcomm;
+  cM = k.mass + k.damping * p.alphaM;
+}
+
+// D_eff = cK D_t + c0 D_0 + cc D_c (effective_tangent, element_operator.hpp:58-75); D_t the stored
+// (symmetric) tangent, D_c the committed tangent of the point's committed history
+__device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, int64_t pt, double (&D)[36],
+                                   double& cM) {
+  double cK, c0, cc;
+  operator_factors(p, a.k, cK, c0, cc, cM);
   double D0[36];
   initial_D(p, D0);
+  double Dc[36];
+  if (cc != 0.0) pimy_committed_tangent(p, a.C + pt, a.npts, Dc);
   int u = 0;
   double Dt[36];
   for (int i = 0; i < 6; ++i)
@@ -141,18 +158,26 @@ __device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, 
   for (int i = 0; i < 36; ++i) {
     double v = cK * Dt[i];
     if (c0 != 0.0) v += c0 * D0[i];
+    if (cc != 0.0) v += cc * Dc[i];
     D[i] = v;
   }
 }
 
+__device__ inline bool stiffness_damping(const PimyParams& p) {
+  return p.betaK != 0.0 || p.betaKinit != 0.0 || p.betaKcomm != 0.0;
+}
+
 // MODE 0: force (update at u; STORE also keeps the tangents; REPLAY takes each point's stress
-// and tangent from the compacted pass instead of updating). MODE 1: operator on the stored
-// tangents. MODE 2: its diagonal.
-template <int MODE, bool STORE, bool REPLAY = false>
+// and tangent from the compacted pass instead of updating; DAMP adds the Rayleigh damping force
+// at the velocities a.v, std_brick.hpp:409-464). MODE 1: operator on the stored tangents. MODE 2:
+// its diagonal. DAMP blocks hold half the points (their LDS keeps 9 values per node).
+template <int MODE, bool STORE, bool REPLAY = false, bool DAMP = false>
 __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchArgs a) {
-  __shared__ double part[kPointsPerBlock][48];  // per point: 8 nodes x (3 first terms, 3 second terms)
-  const int t = threadIdx.x;
-  const int e = blockIdx.x * kElemsPerBlock + t / 8, gp = t % 8;
+  constexpr int PTS = DAMP ? kPointsPerBlock / 2 : kPointsPerBlock, W = DAMP ? 9 : 6;
+  // per point: 8 nodes x (3 first terms, 3 second terms [, 3 damping terms])
+  __shared__ double part[PTS][8 * W];
+  const int t = threadIdx.x;  // the launch has PTS threads
+  const int e = blockIdx.x * (PTS / 8) + t / 8, gp = t % 8;
   const bool live = e < a.count;
   if (live) {
     double X[8], Y[8], Z[8];
@@ -164,13 +189,13 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
     const int64_t pt = static_cast<int64_t>(e) * 8 + gp;
     double* mine = part[t];
     if constexpr (MODE == 0) {
-      double sig[6];
+      double sig[6], D[36];
       if constexpr (REPLAY) {
         for (int c = 0; c < 6; ++c) sig[c] = a.cres[c * a.npts + pt];
         if constexpr (STORE)
           for (int w = 0; w < 21; ++w) a.tangent[w * a.npts + pt] = a.cres[(6 + w) * a.npts + pt];
       } else {
-        double u[24], eps[6], D[36];
+        double u[24], eps[6];
         for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
         point_strain(s, u, eps);
         pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
@@ -184,21 +209,55 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         double rj[3];
         node_force(s, sig, j, rj);
         for (int q = 0; q < 3; ++q) {
-          mine[6 * j + q] = rj[q];
-          mine[6 * j + 3 + q] = s.dvol * p.b[q] * s.N[j];  // body
+          mine[W * j + q] = rj[q];
+          mine[W * j + 3 + q] = s.dvol * p.b[q] * s.N[j];  // body
+        }
+      }
+      if constexpr (DAMP) {
+        // sigma_d = (betaK D + betaKinit D_0 + betaKcomm D_c) B v; the node terms B_j^T sigma_d dvol
+        // + alphaM N_j dvol rho v (point_update_shape :420-442, point_node_contribution :448-464)
+        const bool stiff = stiffness_damping(p);
+        double vel[24], dsig[6] = {0, 0, 0, 0, 0, 0}, vgp[3] = {0, 0, 0};
+        for (int k = 0; k < 24; ++k) vel[k] = eq[k] >= 0 ? a.v[eq[k]] : 0.0;
+        if (stiff) {
+          double rate[6], D0[36], Dc[36];
+          if (p.betaKinit != 0.0) initial_D(p, D0);
+          if (p.betaKcomm != 0.0) pimy_committed_tangent(p, a.C + pt, a.npts, Dc);
+          point_strain(s, vel, rate);
+          for (int r = 0; r < 6; ++r) {
+            double sum = 0.0;
+            for (int c = 0; c < 6; ++c) {
+              double Drc = p.betaK * D[6 * r + c];
+              if (p.betaKinit != 0.0) Drc += p.betaKinit * D0[6 * r + c];
+              if (p.betaKcomm != 0.0) Drc += p.betaKcomm * Dc[6 * r + c];
+              sum += Drc * rate[c];
+            }
+            dsig[r] = sum;
+          }
+        }
+        if (p.alphaM != 0.0 && p.mass_type == 0)
+          for (int j = 0; j < 8; ++j)
+            for (int q = 0; q < 3; ++q) vgp[q] += s.N[j] * vel[3 * j + q];
+        for (int j = 0; j < 8; ++j) {
+          double dj[3] = {0, 0, 0};
+          if (stiff) node_force(s, dsig, j, dj);
+          if (p.alphaM != 0.0) {
+            const double temp = s.N[j] * s.dvol * p.rho;
+            for (int q = 0; q < 3; ++q) dj[q] += p.alphaM * temp * (p.mass_type == 0 ? vgp[q] : vel[3 * j + q]);
+          }
+          for (int q = 0; q < 3; ++q) mine[W * j + 6 + q] = dj[q];
         }
       }
     } else if constexpr (MODE == 1) {
-      double v[24], eps[6], sig[6], D[36];
+      double v[24], eps[6], sig[6], D[36], cM;
       for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
-      effective_D(a, p, pt, D);
+      effective_D(a, p, pt, D, cM);
       point_strain(s, v, eps);
       for (int r = 0; r < 6; ++r) {
         double sum = 0.0;
         for (int c = 0; c < 6; ++c) sum += D[6 * r + c] * eps[c];
         sig[r] = sum;
       }
-      const double cM = a.k.mass;
       double g[3] = {0, 0, 0};
       if (cM != 0.0 && p.mass_type == 0)
         for (int kk = 0; kk < 8; ++kk)
@@ -213,9 +272,8 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         }
       }
     } else {
-      double D[36];
-      effective_D(a, p, pt, D);
-      const double cM = a.k.mass;
+      double D[36], cM;
+      effective_D(a, p, pt, D, cM);
       double de[24];
       for (int k = 0; k < 24; ++k) de[k] = 0.0;
       // B_J^T (D dvol) B_K entry (r, c): with g the gradients, the isotropic-free general form
@@ -273,26 +331,36 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
   }
   __syncthreads();
   if (!live || gp != 0) return;
-  double out[24];
-  for (int k = 0; k < 24; ++k) out[k] = 0.0;
+  double out[24], fdamp[24];
+  for (int k = 0; k < 24; ++k) out[k] = fdamp[k] = 0.0;
+  const PimyParams& p = a.params[a.param[e]];
+  double cM = 0.0;
+  if constexpr (MODE == 1) {
+    double cK, c0, cc;
+    operator_factors(p, a.k, cK, c0, cc, cM);
+  }
   for (int g = 0; g < 8; ++g) {
     const double* c = part[t + g];
     if constexpr (MODE == 0) {
       for (int j = 0; j < 8; ++j)
         for (int q = 0; q < 3; ++q) {
-          out[3 * j + q] += c[6 * j + q];
-          out[3 * j + q] -= c[6 * j + 3 + q];
+          out[3 * j + q] += c[W * j + q];
+          out[3 * j + q] -= c[W * j + 3 + q];
+          if constexpr (DAMP) fdamp[3 * j + q] += c[W * j + 6 + q];
         }
     } else if constexpr (MODE == 1) {
       for (int j = 0; j < 8; ++j)
         for (int q = 0; q < 3; ++q) {
           out[3 * j + q] += c[6 * j + q];
-          if (a.k.mass != 0.0) out[3 * j + q] += c[6 * j + 3 + q];
+          if (cM != 0.0) out[3 * j + q] += c[6 * j + 3 + q];
         }
     } else {
       for (int k = 0; k < 24; ++k) out[k] += c[2 * k];
     }
   }
+  if constexpr (DAMP)  // fe = resid + fdamp where the element damps
+    if (stiffness_damping(p) || p.alphaM != 0.0)
+      for (int k = 0; k < 24; ++k) out[k] += fdamp[k];
   const size_t P = static_cast<size_t>(a.padded);
   for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
 }
@@ -381,7 +449,7 @@ __global__ void pimy_batch_init_kernel(int64_t npts, const PimyParams* __restric
   }
 }
 
-PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k) {
+PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const double* v = nullptr) {
   PimyBatchArgs a{};
   a.count = b->count;
   a.padded = b->padded;
@@ -396,6 +464,7 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k) {
   a.T = b->state_t;
   a.tangent = b->tangent;
   a.x = x;
+  a.v = v;
   a.slots = b->slots;
   a.k = k;
   a.cres = b->cres;
@@ -408,7 +477,10 @@ template <int MODE, bool STORE>
 int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done) {
   const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
   hipStream_t st = b->ctx->stream;
-  if (b->count > 0 && MODE == 0 && b->cres) {  // compacted: predict, correct, replay
+  if (b->count > 0 && MODE == 0 && a.v) {  // the damping force: the fused return map, half blocks
+    const dim3 g2((b->count + kElemsPerBlock / 2 - 1) / (kElemsPerBlock / 2));
+    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE, false, true>), g2, dim3(kPointsPerBlock / 2), 0, st, a);
+  } else if (b->count > 0 && MODE == 0 && b->cres) {  // compacted: predict, correct, replay
     hipLaunchKernelGGL(pimy_predict_kernel, grid, dim3(kPointsPerBlock), 0, st, a);
     const unsigned cgrid = static_cast<unsigned>(std::min<int64_t>((b->npts + 127) / 128, 2048));
     hipLaunchKernelGGL(pimy_correct_kernel, dim3(cgrid), dim3(128), 0, st, a);
@@ -436,7 +508,8 @@ int pimy_batch_init(fx_batch* b) {
 }
 
 int pimy_force64(fx_batch* b, const double* u, const double* v, double* f, bool store) {
-  const PimyBatchArgs a = batch_args(b, u, fx_coeffs{1.0, 0.0, 0.0, 0});
+  // the damping force only where some parameter set damps and velocities are given
+  const PimyBatchArgs a = batch_args(b, u, fx_coeffs{1.0, 0.0, 0.0, 0}, b->pimy_damped ? v : nullptr);
   if (store) {
     if (int rc = run_pimy<0, true>(b, a, f, nullptr)) return rc;
     b->tangents_stored = true;

@@ -37,6 +37,7 @@ struct PimyParams {
   double b[3];
   int32_t mass_type, pad;
   double size[kPimyNS + 1], modulus[kPimyNS + 1];
+  double alphaM, betaK, betaKinit, betaKcomm;  // the element's Rayleigh damping (batches)
 };
 
 }  // namespace fx

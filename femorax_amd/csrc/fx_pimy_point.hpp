@@ -549,6 +549,18 @@ __device__ inline void pimy_update(const PimyParams& p, const double* C, double*
   pt.update(eps, sig, D);
 }
 
+// committed_tangent (:162-168): getTangent with the committed stress, active surface, surfaces
+// and moduli (the surfaces are read from C; nothing is written).
+__device__ inline void pimy_committed_tangent(const PimyParams& p, const double* C, int64_t stride,
+                                              double (&D)[36]) {
+  pimy::Point pt(p, C, const_cast<double*>(C), stride);
+  pt.trial = pt.load_t2(kPStress);
+  pt.active = static_cast<int>(pt.c(kPActive));
+  pt.G = pt.c(kPG);
+  pt.K = pt.c(kPK);
+  pt.tangent(D);
+}
+
 __device__ inline bool pimy_predict(const PimyParams& p, const double* C, double* T, int64_t stride,
                                     const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
   pimy::Point pt(p, C, T, stride);

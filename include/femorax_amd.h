@@ -300,12 +300,16 @@ typedef struct fx_pimy_batch_desc {
   int32_t nnode;
   const double* coords;            /* [3 * nnode] SoA */
   int32_t neq;
+  const fx_rayleigh* damp;         /* [nparam] Rayleigh damping per parameter set, or NULL */
 } fx_pimy_batch_desc;
 /* A batch whose material points (8 per element, point e * 8 + gp) carry PIMY history (committed
  * and trial, at rest at creation). fx_force runs the return map at every point (the trial
- * history; InternalForce::compute), fx_operator_apply / _diagonal / fx_cg use the tangents kept
- * by the last fx_store_tangents (store_tangents, internal_force.cpp:532-543; D_eff = cK D_t +
- * c0 D_0, no Rayleigh damping), fx_explicit_cd commits after every step. No FP32 operator. */
+ * history; InternalForce::compute), with the Rayleigh damping force at v when v is given
+ * (std_brick.hpp:409-464: betaK on the updated tangent, betaKinit on the initial, betaKcomm on the
+ * committed tangent, alphaM on the mass); fx_operator_apply / _diagonal / fx_cg use the tangents
+ * kept by the last fx_store_tangents (store_tangents, internal_force.cpp:532-543; D_eff = cK D_t +
+ * c0 D_0 + cc D_c with the factors of element_operator.hpp:45-56), fx_explicit_cd commits after
+ * every step. No FP32 operator. */
 int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* desc, uint32_t flags, fx_batch** out);
 /* updateMaterialStage for every parameter set (0 elastic, 1 plastic). No-op on elastic batches. */
 int fx_set_material_stage(fx_batch* b, int32_t stage);

@@ -163,25 +163,31 @@ void fxo_pimy_update(const fxo_pimy_params* p, const double* committed, double* 
                      double sig[6], double D[36]);
 
 /* ---- stdBrick<PressureIndependMultiYield> (fxo_brick.c, fxo_ops.c) ------------------------ */
+/* committed_tangent (:162-168): getTangent from the committed history (stress, active surface,
+ * surfaces, moduli) */
+void fxo_pimy_committed_tangent(const fxo_pimy_params* p, const double* committed, double D[36]);
 typedef struct fxo_pimy_elem {
   fxo_pimy_params mat;
   double b[3];       /* body force per unit volume */
   int32_t mass_type; /* 0 consistent, 1 lumped */
+  fxo_rayleigh damp; /* the element's Rayleigh damping */
 } fxo_pimy_elem;
-void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* const* C,
-                          double* const* T, double* fe, double* Dgp);
+/* ve: the element velocities (the Rayleigh damping force, std_brick.hpp:409-464), or NULL */
+void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
+                          const double* const* C, double* const* T, double* fe, double* Dgp);
+/* C: the points' committed histories (the committed tangent of betaKcomm), or NULL */
 void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
-                                 const double* Dgp, const double* ve, double* ye);
+                                 const double* Dgp, const double* const* C, const double* ve, double* ye);
 void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
-                                    const double* Dgp, const int32_t* eq, double* de);
+                                    const double* Dgp, const double* const* C, const int32_t* eq, double* de);
 /* global: material points e * 8 + gp (batch order), history point-major (FXO_PIMY_NSTATE each),
- * tangents D[point][36] */
+ * tangents D[point][36]; v may be NULL (no damping velocity) */
 void fxo_pimy_force(const fxo_compiled* c, const fxo_pimy_elem* params, const double* C, double* T,
-                    const double* u, double* f, double* D, int nthreads);
+                    const double* u, const double* v, double* f, double* D, int nthreads);
 void fxo_apply_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
-                  const double* x, double* y, int nthreads);
+                  const double* C, const double* x, double* y, int nthreads);
 void fxo_diagonal_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
-                     double* d, int nthreads);
+                     const double* C, double* d, int nthreads);
 
 /* ---- implicit Newmark + Newton (fxo_newmark.c) -------------------------------------------- */
 typedef struct fxo_system {

@@ -55,7 +55,7 @@ class PimyParams(C.Structure):
 
 
 class PimyElem(C.Structure):
-    _fields_ = [("mat", PimyParams), ("b", C.c_double * 3), ("mass_type", C.c_int32)]
+    _fields_ = [("mat", PimyParams), ("b", C.c_double * 3), ("mass_type", C.c_int32), ("damp", Rayleigh)]
 
 
 PIMY_NSTATE = 190
@@ -76,6 +76,7 @@ def pimy_elems(materials, stage=0):
         arr[i].mat.load_stage = stage
         arr[i].b = (C.c_double * 3)(*m.get("b", (0.0, 0.0, 0.0)))
         arr[i].mass_type = int(m.get("mass_type", 0))
+        arr[i].damp = Rayleigh(*m.get("rayleigh", (0.0, 0.0, 0.0, 0.0)))
     return arr
 
 
@@ -156,9 +157,13 @@ def lib():
         L.fxo_pimy_setup.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(PimyParams)]
         L.fxo_pimy_nstate.restype = C.c_int
         L.fxo_pimy_initial_state.argtypes = [C.POINTER(PimyParams), C.c_void_p]
-        L.fxo_pimy_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p, f64p, f64p, C.c_void_p, C.c_int]
-        L.fxo_apply_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, f64p, C.c_int]
-        L.fxo_diagonal_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_int]
+        L.fxo_pimy_force.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, f64p, f64p, C.c_void_p, f64p, C.c_void_p,
+                                     C.c_int]
+        L.fxo_apply_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, C.c_void_p, f64p, f64p,
+                                   C.c_int]
+        L.fxo_diagonal_gp.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, C.c_void_p, f64p,
+                                      C.c_int]
+        L.fxo_pimy_committed_tangent.argtypes = [C.POINTER(PimyParams), f64p, f64p]
         L.fxo_gmres_system.restype = C.c_int
         L.fxo_gmres_system.argtypes = [C.POINTER(System), C.POINTER(Coeffs), f64p, f64p, C.c_double, C.c_double,
                                        C.c_int32, C.c_int32, C.c_int, C.POINTER(SolveResult)]
@@ -338,23 +343,29 @@ class OracleMesh:
             S[8 * e:8 * e + 8] = one
         return S
 
-    def pimy_force(self, elems, Cs, Ts, u, want_D=False, nthreads=0):
+    def pimy_force(self, elems, Cs, Ts, u, want_D=False, nthreads=0, v=None):
+        """f(u) with the return map at every point (trial history written); v: the velocities of
+        the Rayleigh damping force, or None."""
         f = np.zeros(self.neq)
         D = np.zeros((self.nelem * 8, 36)) if want_D else None
-        lib().fxo_pimy_force(C.byref(self.c), C.cast(elems, C.c_void_p), Cs, Ts, np.ascontiguousarray(u, np.float64), f,
-                             None if D is None else D.ctypes.data, nthreads)
+        vv = None if v is None else np.ascontiguousarray(v, np.float64)
+        lib().fxo_pimy_force(C.byref(self.c), C.cast(elems, C.c_void_p), Cs, Ts, np.ascontiguousarray(u, np.float64),
+                             None if vv is None else vv.ctypes.data, f, None if D is None else D.ctypes.data, nthreads)
         return (f, D) if want_D else f
 
-    def apply_gp(self, elems, D, x, k=None, nthreads=0):
+    def apply_gp(self, elems, D, x, k=None, nthreads=0, committed=None):
+        """K_eff x on the stored tangents D; committed: the committed histories (betaKcomm)."""
         y = np.zeros(self.neq)
+        cm = None if committed is None else np.ascontiguousarray(committed)
         lib().fxo_apply_gp(C.byref(self.c), C.cast(elems, C.c_void_p), C.byref(k or coeffs()), np.ascontiguousarray(D),
-                           np.ascontiguousarray(x, np.float64), y, nthreads)
+                           None if cm is None else cm.ctypes.data, np.ascontiguousarray(x, np.float64), y, nthreads)
         return y
 
-    def diagonal_gp(self, elems, D, k=None, nthreads=0):
+    def diagonal_gp(self, elems, D, k=None, nthreads=0, committed=None):
         d = np.zeros(self.neq)
+        cm = None if committed is None else np.ascontiguousarray(committed)
         lib().fxo_diagonal_gp(C.byref(self.c), C.cast(elems, C.c_void_p), C.byref(k or coeffs()), np.ascontiguousarray(D),
-                              d, nthreads)
+                              None if cm is None else cm.ctypes.data, d, nthreads)
         return d
 
     def newmark(self, u, v, a, du, dt, nsteps, elastic=None, pimy=None, pimy_state=None, gamma=0.5, beta=0.25,

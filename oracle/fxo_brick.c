@@ -388,35 +388,6 @@ void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, cons
 }
 
 /* ---- stdBrick with PressureIndependMultiYield ------------------------------------------------ */
-/* Brick::compute_with (std_brick.hpp:493-543) with the material update per Gauss point:
- * fe = sum_gp [B^T sigma dvol - b N dvol] in OpenSees' per-node order (resid += residJ;
- * resid -= body). C[gp], T[gp]: the point's committed and trial history (FXO_PIMY_NSTATE
- * doubles each); Dgp: the points' tangents (8 x 36) or NULL. No Rayleigh damping. */
-void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* const* C,
-                          double* const* T, double* fe, double* Dgp) {
-  double xl[3][8];
-  local_coords(xe, xl);
-  double resid[24];
-  for (int a = 0; a < 24; ++a) resid[a] = 0.0;
-  for (int gp = 0; gp < 8; ++gp) {
-    double shp[4][8], dvol, eps[6], sig[6], D[36];
-    fxo_brick_shape(xl, gp, shp, &dvol);
-    point_strain(shp, ue, eps);
-    fxo_pimy_update(&p->mat, C[gp], T[gp], eps, sig, D);
-    if (Dgp) memcpy(Dgp + 36 * gp, D, sizeof D);
-    for (int j = 0; j < 8; ++j) {
-      double rj[3], body[3];
-      point_node_force(shp, dvol, sig, j, rj);
-      for (int q = 0; q < 3; ++q) body[q] = dvol * p->b[q] * shp[3][j];
-      for (int q = 0; q < 3; ++q) {
-        resid[3 * j + q] += rj[q];
-        resid[3 * j + q] -= body[q];
-      }
-    }
-  }
-  for (int a = 0; a < 24; ++a) fe[a] = resid[a];
-}
-
 /* the elastic initial tangent of the material's reference moduli (initial_tangent) */
 static void pimy_initial_D(const fxo_pimy_elem* p, double* D) {
   for (int i = 0; i < 6; ++i)
@@ -429,12 +400,99 @@ static void pimy_initial_D(const fxo_pimy_elem* p, double* D) {
     }
 }
 
-/* element_operator_apply with stored tangents (Source::Stored, element_operator.hpp:82-130):
- * D_eff(gp) = cK D_t(gp) + c0 D_0 (Rayleigh off: cK = stiffness, c0 = 0, or c0 = stiffness for
- * the initial operator); cM' = mass. */
+/* Brick::compute_with (std_brick.hpp:493-543) with the material update per Gauss point
+ * (point_update_shape :409-443, point_node_contribution :448-464): per point and node, resid +=
+ * B_j^T sigma dvol; resid -= b N_j dvol; fdamp += B_j^T sigma_d dvol + alphaM N_j dvol rho v,
+ * sigma_d = (betaK D + betaKinit D_0 + betaKcomm D_c) B v with D the point's updated tangent,
+ * D_0 the initial and D_c the committed tangent; fe = resid + fdamp. C[gp], T[gp]: the point's
+ * committed and trial history; Dgp: the points' tangents (8 x 36) or NULL; ve NULL: no damping
+ * velocity (fe = resid). */
+void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
+                          const double* const* C, double* const* T, double* fe, double* Dgp) {
+  const fxo_rayleigh* dp = &p->damp;
+  const int stiff = dp->betaK != 0.0 || dp->betaKinit != 0.0 || dp->betaKcomm != 0.0;
+  const int damped = ve && (stiff || dp->alphaM != 0.0);
+  double xl[3][8], D0[36];
+  local_coords(xe, xl);
+  if (damped && dp->betaKinit != 0.0) pimy_initial_D(p, D0);
+  double resid[24], fdamp[24];
+  for (int a = 0; a < 24; ++a) resid[a] = fdamp[a] = 0.0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double shp[4][8], dvol, eps[6], sig[6], D[36];
+    fxo_brick_shape(xl, gp, shp, &dvol);
+    point_strain(shp, ue, eps);
+    fxo_pimy_update(&p->mat, C[gp], T[gp], eps, sig, D);
+    if (Dgp) memcpy(Dgp + 36 * gp, D, sizeof D);
+    double dsig[6] = {0, 0, 0, 0, 0, 0}, vgp[3] = {0, 0, 0};
+    if (damped && stiff) {
+      double Dc[36], rate[6];
+      if (dp->betaKcomm != 0.0) fxo_pimy_committed_tangent(&p->mat, C[gp], Dc);
+      point_strain(shp, ve, rate);
+      for (int a = 0; a < 6; ++a) {
+        double sum = 0.0;
+        for (int c = 0; c < 6; ++c) {
+          double Dac = dp->betaK * D[6 * a + c];
+          if (dp->betaKinit != 0.0) Dac += dp->betaKinit * D0[6 * a + c];
+          if (dp->betaKcomm != 0.0) Dac += dp->betaKcomm * Dc[6 * a + c];
+          sum += Dac * rate[c];
+        }
+        dsig[a] = sum;
+      }
+    }
+    if (damped && dp->alphaM != 0.0 && p->mass_type == 0)
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 3; ++q) vgp[q] += shp[3][j] * ve[3 * j + q];
+    for (int j = 0; j < 8; ++j) {
+      double rj[3], body[3], dj[3] = {0, 0, 0};
+      point_node_force(shp, dvol, sig, j, rj);
+      for (int q = 0; q < 3; ++q) body[q] = dvol * p->b[q] * shp[3][j];
+      if (damped) {
+        if (stiff) point_node_force(shp, dvol, dsig, j, dj);
+        if (dp->alphaM != 0.0) {
+          const double temp = shp[3][j] * dvol * p->mat.rho;
+          for (int q = 0; q < 3; ++q) dj[q] += dp->alphaM * temp * (p->mass_type == 0 ? vgp[q] : ve[3 * j + q]);
+        }
+      }
+      for (int q = 0; q < 3; ++q) {
+        resid[3 * j + q] += rj[q];
+        resid[3 * j + q] -= body[q];
+        fdamp[3 * j + q] += dj[q];
+      }
+    }
+  }
+  for (int a = 0; a < 24; ++a) fe[a] = damped ? resid[a] + fdamp[a] : resid[a];
+}
+
+/* The operator factors (element_operator.hpp:45-56; the PIMY initial stiffness is not its
+ * tangent): cK on the stored tangent, c0 on the initial one, cc on the committed one, cM on the
+ * mass. */
+static void pimy_factors(const fxo_pimy_elem* p, const fxo_coeffs* k, double* cK, double* c0, double* cc,
+                         double* cM) {
+  const fxo_rayleigh* d = &p->damp;
+  *cK = k->initial ? k->damping * d->betaK : k->stiffness + k->damping * d->betaK;
+  *c0 = k->initial ? k->stiffness + k->damping * d->betaKinit : k->damping * d->betaKinit;
+  *cc = k->damping * d->betaKcomm;
+  *cM = k->mass + k->damping * d->alphaM;
+}
+
+/* D_eff at point gp (effective_tangent, element_operator.hpp:58-75) */
+static void pimy_effective(const fxo_pimy_elem* p, double cK, double c0, double cc, const double* Dgp,
+                           const double* D0, const double* const* C, int gp, double* Deff) {
+  double Dc[36];
+  if (cc != 0.0) fxo_pimy_committed_tangent(&p->mat, C[gp], Dc);
+  for (int a = 0; a < 36; ++a) {
+    double v = cK * Dgp[36 * gp + a];
+    if (c0 != 0.0) v += c0 * D0[a];
+    if (cc != 0.0) v += cc * Dc[a];
+    Deff[a] = v;
+  }
+}
+
+/* element_operator_apply with stored tangents (Source::Stored, element_operator.hpp:82-130) */
 void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
-                                 const double* Dgp, const double* ve, double* ye) {
-  const double cK = k->initial ? 0.0 : k->stiffness, c0 = k->initial ? k->stiffness : 0.0, cM = k->mass;
+                                 const double* Dgp, const double* const* C, const double* ve, double* ye) {
+  double cK, c0, cc, cM;
+  pimy_factors(p, k, &cK, &c0, &cc, &cM);
   double xl[3][8], D0[36];
   local_coords(xe, xl);
   pimy_initial_D(p, D0);
@@ -446,11 +504,7 @@ void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, co
   for (int gp = 0; gp < 8; ++gp) {
     double shp[4][8], dvol, eps[6], sig[6], Deff[36];
     fxo_brick_shape(xl, gp, shp, &dvol);
-    for (int a = 0; a < 36; ++a) {
-      double v = cK * Dgp[36 * gp + a];
-      if (c0 != 0.0) v += c0 * D0[a];
-      Deff[a] = v;
-    }
+    pimy_effective(p, cK, c0, cc, Dgp, D0, C, gp, Deff);
     point_strain(shp, ve, eps);
     for (int a = 0; a < 6; ++a) {
       double sum = 0.0;
@@ -472,8 +526,9 @@ void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, co
 
 /* element_operator_diagonal with stored tangents, merged DOFs' cross terms on the first */
 void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
-                                    const double* Dgp, const int32_t* eq, double* de) {
-  const double cK = k->initial ? 0.0 : k->stiffness, c0 = k->initial ? k->stiffness : 0.0, cM = k->mass;
+                                    const double* Dgp, const double* const* C, const int32_t* eq, double* de) {
+  double cK, c0, cc, cM;
+  pimy_factors(p, k, &cK, &c0, &cc, &cM);
   double xl[3][8], D0[36];
   local_coords(xe, xl);
   pimy_initial_D(p, D0);
@@ -496,11 +551,7 @@ void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k,
   for (int gp = 0; gp < 8; ++gp) {
     double shp[4][8], dvol, Deff[36];
     fxo_brick_shape(xl, gp, shp, &dvol);
-    for (int a = 0; a < 36; ++a) {
-      double v = cK * Dgp[36 * gp + a];
-      if (c0 != 0.0) v += c0 * D0[a];
-      Deff[a] = v;
-    }
+    pimy_effective(p, cK, c0, cc, Dgp, D0, C, gp, Deff);
     for (int j = 0; j < 8; ++j) {
       double blk[3][3];
       point_block_tangent(shp, dvol, Deff, j, j, blk);

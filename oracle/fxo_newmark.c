@@ -18,19 +18,18 @@
 #include "fxo.h"
 
 static void sys_force(fxo_system* s, const double* u, const double* v, double* f) {
-  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, f, NULL, s->nthreads);
+  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, v, f, NULL, s->nthreads);
   else fxo_force(s->c, s->elastic, u, v, f, s->nthreads);
 }
-static void sys_store(fxo_system* s, const double* u, const double* v, double* scratch) {
-  (void)v;
-  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, scratch, s->D, s->nthreads);
+static void sys_store(fxo_system* s, const double* u, double* scratch) {
+  if (s->pimy) fxo_pimy_force(s->c, s->pimy, s->C, s->T, u, NULL, scratch, s->D, s->nthreads);
 }
 static void sys_apply(fxo_system* s, const fxo_coeffs* k, const double* x, double* y) {
-  if (s->pimy) fxo_apply_gp(s->c, s->pimy, k, s->D, x, y, s->nthreads);
+  if (s->pimy) fxo_apply_gp(s->c, s->pimy, k, s->D, s->C, x, y, s->nthreads);
   else fxo_apply(s->c, s->elastic, k, x, y, s->nthreads);
 }
 static void sys_diagonal(fxo_system* s, const fxo_coeffs* k, double* d) {
-  if (s->pimy) fxo_diagonal_gp(s->c, s->pimy, k, s->D, d, s->nthreads);
+  if (s->pimy) fxo_diagonal_gp(s->c, s->pimy, k, s->D, s->C, d, s->nthreads);
   else fxo_diagonal(s->c, s->elastic, k, d, s->nthreads);
 }
 
@@ -284,7 +283,7 @@ static int nm_attempt(nm_state* w, double h) {
   nm_unbalance(w, ag);
   int iter = 1, iterations = 0, ok = 1;
   for (;;) {
-    sys_store(w->s, w->u, w->v, w->scratch); /* update_tangent: the tangents at u */
+    sys_store(w->s, w->u, w->scratch); /* update_tangent: the tangents at u */
     fxo_solve_result r;
     fxo_cg_system(w->s, &k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, &r);
     w->st->krylov_iterations += r.iterations;

@@ -231,17 +231,18 @@ int fxo_cg(const fxo_compiled* c, const fxo_elastic* params, const fxo_coeffs* k
 
 /* ---- PressureIndependMultiYield batches (points e * 8 + gp, batch order; history point-major) */
 void fxo_pimy_force(const fxo_compiled* c, const fxo_pimy_elem* params, const double* C, double* T,
-                    const double* u, double* f, double* D, int nthreads) {
+                    const double* u, const double* v, double* f, double* D, int nthreads) {
   set_threads(nthreads);
   const int32_t P = c->padded, E = c->nelem;
   const int ns = fxo_pimy_nstate();
 #pragma omp parallel for schedule(static)
   for (int32_t e = 0; e < E; ++e) {
-    double xe[8][3], ue[24], fe[24];
+    double xe[8][3], ue[24], ve[24], fe[24];
     gather_xe(c, e, xe);
     for (int k = 0; k < 24; ++k) {
       const int32_t q = c->eq[(int64_t)k * P + e];
       ue[k] = q >= 0 ? u[q] : 0.0;
+      ve[k] = (v && q >= 0) ? v[q] : 0.0;
     }
     const double* Cp[8];
     double* Tp[8];
@@ -249,14 +250,21 @@ void fxo_pimy_force(const fxo_compiled* c, const fxo_pimy_elem* params, const do
       Cp[gp] = C + ((int64_t)e * 8 + gp) * ns;
       Tp[gp] = T + ((int64_t)e * 8 + gp) * ns;
     }
-    fxo_brick_force_pimy(&params[c->param[e]], (const double(*)[3])xe, ue, Cp, Tp, fe, D ? D + (int64_t)e * 8 * 36 : NULL);
+    fxo_brick_force_pimy(&params[c->param[e]], (const double(*)[3])xe, ue, v ? ve : NULL, Cp, Tp, fe,
+                         D ? D + (int64_t)e * 8 * 36 : NULL);
     for (int k = 0; k < 24; ++k) c->slots[(int64_t)k * P + e] = fe[k];
   }
   gather(c, f);
 }
 
+/* the committed history pointers of element e's points, or NULLs */
+static void point_ptrs(const double* C, int32_t e, const double** Cp) {
+  const int ns = fxo_pimy_nstate();
+  for (int gp = 0; gp < 8; ++gp) Cp[gp] = C ? C + ((int64_t)e * 8 + gp) * ns : NULL;
+}
+
 void fxo_apply_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
-                  const double* x, double* y, int nthreads) {
+                  const double* C, const double* x, double* y, int nthreads) {
   set_threads(nthreads);
   const int32_t P = c->padded, E = c->nelem;
 #pragma omp parallel for schedule(static)
@@ -267,14 +275,16 @@ void fxo_apply_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_
       const int32_t q = c->eq[(int64_t)a * P + e];
       ve[a] = q >= 0 ? x[q] : 0.0;
     }
-    fxo_brick_operator_apply_gp(&params[c->param[e]], k, (const double(*)[3])xe, D + (int64_t)e * 8 * 36, ve, ye);
+    const double* Cp[8];
+    point_ptrs(C, e, Cp);
+    fxo_brick_operator_apply_gp(&params[c->param[e]], k, (const double(*)[3])xe, D + (int64_t)e * 8 * 36, Cp, ve, ye);
     for (int a = 0; a < 24; ++a) c->slots[(int64_t)a * P + e] = ye[a];
   }
   gather(c, y);
 }
 
 void fxo_diagonal_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const fxo_coeffs* k, const double* D,
-                     double* d, int nthreads) {
+                     const double* C, double* d, int nthreads) {
   set_threads(nthreads);
   const int32_t P = c->padded, E = c->nelem;
 #pragma omp parallel for schedule(static)
@@ -283,7 +293,10 @@ void fxo_diagonal_gp(const fxo_compiled* c, const fxo_pimy_elem* params, const f
     int32_t eq[24];
     gather_xe(c, e, xe);
     for (int a = 0; a < 24; ++a) eq[a] = c->eq[(int64_t)a * P + e];
-    fxo_brick_operator_diagonal_gp(&params[c->param[e]], k, (const double(*)[3])xe, D + (int64_t)e * 8 * 36, eq, de);
+    const double* Cp[8];
+    point_ptrs(C, e, Cp);
+    fxo_brick_operator_diagonal_gp(&params[c->param[e]], k, (const double(*)[3])xe, D + (int64_t)e * 8 * 36, Cp, eq,
+                                   de);
     for (int a = 0; a < 24; ++a) c->slots[(int64_t)a * P + e] = de[a];
   }
   gather(c, d);

@@ -600,6 +600,19 @@ static void tangent(const point* w, double* D) {
     }
 }
 
+void fxo_pimy_committed_tangent(const fxo_pimy_params* p, const double* C, double D[36]) {
+  point w;
+  memset(&w, 0, sizeof w);
+  w.p = p;
+  w.C = C;
+  w.T = (double*)C; /* the surfaces are read from the committed buffer; nothing is written */
+  t2_load(&w.trial, C, S_STRESS);
+  w.active = (int)C[S_ACTIVE];
+  w.G = C[S_G];
+  w.K = C[S_K];
+  tangent(&w, D);
+}
+
 void fxo_pimy_update(const fxo_pimy_params* p, const double* C, double* T, const double eps[6], double sig[6],
                      double D[36]) {
   point w;

@@ -122,6 +122,70 @@ void fxref_pimy_update(const double* params, int load_stage, const double* commi
   std::memcpy(D, DD, sizeof DD);
 }
 
+// StdBrick<PressureIndependMultiYield<20>>::compute<false> (std_brick.hpp:493-543): the element
+// force with the return map at each point and Rayleigh damping at v; histories contiguous per
+// point (NSTATE each), committed C, trial T written.
+void fxref_pimy_brick_force(const double* params, int load_stage, const double* b, int mass_type, const double* ray,
+                            const double* xe, const double* u, const double* v, const double* C, double* T,
+                            double* fe) {
+  using M = PressureIndependMultiYield<20>;
+  using PB = StdBrick<M>;
+  PB::Params p{};
+  std::memcpy(&p.mat, params, sizeof p.mat);
+  p.mat.loadStage = load_stage;
+  p.b[0] = b[0];
+  p.b[1] = b[1];
+  p.b[2] = b[2];
+  p.massType = mass_type;
+  double x[8][3];
+  std::memcpy(x, xe, sizeof x);
+  const PointArray<const double> committed{C, 1, M::NSTATE};
+  const PointArray<double> trial{T, 1, M::NSTATE};
+  PB::compute<false>(p, rayleigh(ray), x, u, v, committed, trial, fe, nullptr);
+}
+
+// PressureIndependMultiYield<20>::committed_tangent (:162-168)
+void fxref_pimy_committed_tangent(const double* params, int load_stage, const double* committed, double* D) {
+  using M = PressureIndependMultiYield<20>;
+  M::Params p;
+  std::memcpy(&p, params, sizeof p);
+  p.loadStage = load_stage;
+  double DD[6][6];
+  M::committed_tangent(p, {committed, 1}, DD);
+  std::memcpy(D, DD, sizeof DD);
+}
+
+// element_operator_apply / _diagonal of StdBrick<PressureIndependMultiYield<20>> with the points'
+// tangents Dgp[8][36] (the stored source) and the committed tangent from C (contiguous points).
+void fxref_pimy_operator(const double* params, int load_stage, int mass_type, const double* ray,
+                         const double* coeffs /* k c m */, int initial, const double* xe, const double* Dgp,
+                         const double* C, const double* v, const int* eq, int diagonal, double* out) {
+  using M = PressureIndependMultiYield<20>;
+  using PB = StdBrick<M>;
+  PB::Params p{};
+  std::memcpy(&p.mat, params, sizeof p.mat);
+  p.mat.loadStage = load_stage;
+  p.massType = mass_type;
+  const Rayleigh d = rayleigh(ray);
+  OperatorCoefficients k;
+  k.stiffness = coeffs[0];
+  k.damping = coeffs[1];
+  k.mass = coeffs[2];
+  k.initial = initial != 0;
+  double x[8][3], xl[3][8];
+  std::memcpy(x, xe, sizeof x);
+  PB::local_coords(x, xl);
+  const auto geo = PB::recomputed_geometry(xl, p);
+  auto tangent_at = [&](int gp, const PB::PointShape&, double (*D)[6]) {
+    std::memcpy(D, Dgp + 36 * gp, 36 * sizeof(double));
+  };
+  auto committed_at = [&](int gp, double (*D)[6]) {
+    M::committed_tangent(p.mat, {C + static_cast<std::ptrdiff_t>(gp) * M::NSTATE, 1}, D);
+  };
+  if (diagonal) element_operator_diagonal<PB>(p, d, k, geo, tangent_at, committed_at, eq, out);
+  else element_operator_apply<PB>(p, d, k, geo, tangent_at, committed_at, v, out);
+}
+
 // StdBrick::critical_dt (std_brick.hpp:629-694) with ElasticIsotropic (no state):
 // scheme 0 CentralDifference, 1 ExplicitDifference. Returns omega^2 in *omega_sq.
 double fxref_brick_critical_dt(const double* mat, const double* ray, int scheme, const double* xe,

@@ -220,3 +220,132 @@ def test_compacted_return_map_gives_the_same_bits(ctx):
         fused.commit()
         comp.commit()
     assert plastic > 0
+
+
+DAMP = (0.6, 2e-3, 1.5e-3, 1e-3)  # alphaM, betaK, betaKinit, betaKcomm: every term of the damping
+
+
+def setup_damped(ctx, name, mass_type=None):
+    model, mats = pimy_model(name)
+    pm = core.Mesh(model)
+    per_set = [dict(mats[g]) for g in pm.param_group]
+    for m in per_set:
+        m["rayleigh"] = DAMP
+        if mass_type is not None:
+            m["mass_type"] = mass_type
+    keep = []
+    dev_mats = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep)
+                for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev_mats, rayleigh=[DAMP] * len(per_set))
+    return model, pm, batch, fxo.OracleMesh(model), per_set
+
+
+@pytest.mark.parametrize("name,mass_type", [("column12_tied", None), ("distorted3", 0)])
+def test_damped_force_operator_and_diagonal(ctx, name, mass_type):
+    """Rayleigh damping on PIMY batches (std_brick.hpp:409-464, element_operator.hpp:45-75): the
+    force at (u, v), the operator and diagonal with the Newmark-like and the damping-only
+    coefficients (the committed tangent of betaKcomm from the committed history), against the
+    oracle along a walk into the plastic range."""
+    model, pm, batch, om, per_set = setup_damped(ctx, name, mass_type)
+    n = pm.neq
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    dir_ = meshes.random_vector(n, 41)
+    du, dv, df = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    x = meshes.random_vector(n, 42)
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, n)
+    for step in range(6):
+        u = dir_ * (2.5e-3 * (step + 1) * (1 if step % 3 else -0.5))
+        v = meshes.random_vector(n, 50 + step) * 0.4
+        du.upload(u)
+        dv.upload(v)
+        batch.force(du, df, dv)
+        f_o = om.pimy_force(pe, Co, To, u, v=v)
+        assert rel(df.get(), f_o) <= 1e-10, (step, rel(df.get(), f_o))
+        batch.store_tangents(du)
+        _, D_o = om.pimy_force(pe, Co, To, u, want_D=True)
+        for kk in ((1.0, 400.0, 8e4, False), (0.0, 1.0, 0.0, False), (1.0, 400.0, 8e4, True)):
+            batch.apply(dx, dy, core.coeffs(*kk))
+            want = om.apply_gp(pe, D_o, x, fxo.coeffs(*kk), committed=Co)
+            assert rel(dy.get(), want) <= 1e-10, (step, kk)
+            batch.diagonal(dy, core.coeffs(*kk))
+            want = om.diagonal_gp(pe, D_o, fxo.coeffs(*kk), committed=Co)
+            assert rel(dy.get(), want) <= 1e-10, (step, kk)
+        batch.commit()
+        Co, To = To, Co
+    assert np.any(Co[:, 26] > 0), "the walk reached the plastic range"
+
+
+def test_damped_newmark_stage(ctx):
+    """Newmark + Newton on a damped PIMY column: the unbalance carries the damping force at the
+    trial velocities, the tangent operator the damping terms; against the oracle's stage."""
+    model, pm, batch, om, per_set = setup_damped(ctx, "column12_tied")
+    for m in per_set:
+        m["b"] = (0.0, 0.0, 0.0)
+    keep = []
+    dev_mats = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep)
+                for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev_mats, rayleigh=[DAMP] * len(per_set))
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    n = pm.neq
+    r = np.zeros(n)
+    q = om.node_eq.reshape(-1, 3)[:, 0]
+    r[q[q >= 0]] = 1.0
+    h, nsteps = 0.005, 16
+    ag = 6.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 2)
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, pimy=pe, pimy_state=[Co, To], ground_dir=r, ground_accel=ag, test=0,
+                        tol=1e-3)
+    assert rc == 0 and st["steps"] == nsteps
+    d = [core.DeviceArray(ctx, n) for _ in range(4)]
+    for x in d:
+        x.zero()
+    res = batch.newmark(*d, h, nsteps, ground_dir=0, ground_accel=ag, test=0, tol=1e-3)
+    assert res["failed"] == 0 and res["steps"] == nsteps
+    assert abs(res["newton_iterations"] - st["newton_iterations"]) <= 2
+    for g, w, tol in zip((x.get() for x in d[:3]), (u, v, a), (1e-8, 1e-7, 1e-7)):
+        assert rel(g, w) <= tol
+    assert history_error(batch.pimy_state(True), Co) <= 1e-7
+    assert np.max(Co[:, 26]) > 0
+
+
+def test_damped_explicit_loop(ctx):
+    """CentralDifference on a damped PIMY column (explicit_analysis.cpp:87-117, 712-722): the
+    element force at the garbage velocity -c2 u(n-1) with its damping terms, the lumped alphaM
+    mass on the diagonal."""
+    model, pm, batch, om, per_set = setup_damped(ctx, "column12_tied")
+    n = pm.neq
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    m, am = om.lumped_mass(fxo.elastic_params([dict(E=1.0, nu=0.3, rho=s["rho"], mass_type=s["mass_type"],
+                                                    rayleigh=DAMP) for s in per_set]))
+    dt, nsteps = 2e-4, 40
+    u0 = np.zeros(n)
+    v0 = meshes.random_vector(n, 13) * 0.2
+    um0 = u0 - v0 * dt
+    u, um, v = u0.copy(), um0.copy(), v0.copy()
+    c2, c3 = 0.5 / dt, 1.0 / dt**2
+    for k in range(nsteps):
+        f = om.pimy_force(pe, Co, To, u, v=um * -c2)
+        ag = u * (-2.0 * c3) + um * c3
+        R = -f - m * ag
+        U = R / (m * c3 + am * c2)
+        Ud = (U * 3.0 + u * -4.0 + um) * c2
+        um, u, v = u, U, Ud
+        Co, To = To, Co
+    du, dum = core.DeviceArray.from_host(ctx, u0), core.DeviceArray.from_host(ctx, um0)
+    dv, da = core.DeviceArray.from_host(ctx, v0), core.DeviceArray(ctx, n)
+    da.zero()
+    batch.explicit_cd(dt, nsteps, du, dum, dv, da)
+    assert rel(du.get(), u) <= 1e-9
+    assert rel(dv.get(), v) <= 1e-8
+    assert history_error(batch.pimy_state(True), Co) <= 1e-9
