@@ -63,9 +63,17 @@ void brick_node_weights(const double (&xe)[8][3], double (&w)[8]) {
 }
 
 void brick_stiffness(const double (&xe)[8][3], double lam, double mu, double (&K)[24][24]) {
+  const double L[8] = {lam, lam, lam, lam, lam, lam, lam, lam}, M[8] = {mu, mu, mu, mu, mu, mu, mu, mu};
+  brick_stiffness_points(xe, L, M, K);
+}
+
+// K with an isotropic elastic tangent per Gauss point (lam[gp], mu[gp])
+void brick_stiffness_points(const double (&xe)[8][3], const double (&lamp)[8], const double (&mup)[8],
+                            double (&K)[24][24]) {
   for (auto& row : K)
     for (double& v : row) v = 0.0;
   for (int gp = 0; gp < 8; ++gp) {
+    const double lam = lamp[gp], mu = mup[gp];
     double N[8], g[8][3], det;
     shape_at(xe, gp, N, g, det);
     for (int a = 0; a < 8; ++a)
@@ -113,6 +121,12 @@ double max_eigenvalue_symmetric(double (&A)[24][24]) {
 }
 
 double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho) {
+  const double L[8] = {lam, lam, lam, lam, lam, lam, lam, lam}, M[8] = {mu, mu, mu, mu, mu, mu, mu, mu};
+  return brick_max_omega_sq_points(xe, L, M, rho);
+}
+
+double brick_max_omega_sq_points(const double (&xe)[8][3], const double (&lamp)[8], const double (&mup)[8],
+                                 double rho) {
   double w[8], K[24][24], s[8];
   brick_node_weights(xe, w);
   for (int j = 0; j < 8; ++j) {
@@ -120,7 +134,7 @@ double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, doubl
     if (!(m > 0.0)) return 0.0;  // massless: no limit
     s[j] = 1.0 / std::sqrt(m);
   }
-  brick_stiffness(xe, lam, mu, K);
+  brick_stiffness_points(xe, lamp, mup, K);
   for (int a = 0; a < 24; ++a)
     for (int c = 0; c < 24; ++c) K[a][c] *= s[a / 3] * s[c / 3];
   return max_eigenvalue_symmetric(K);

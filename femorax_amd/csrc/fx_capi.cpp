@@ -628,6 +628,24 @@ int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
     for (const fx::DevParam& p : b->host_params)
       best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(box, p.lam, p.mu, p.rho)));
   } else {
+    // PIMY points: the stability tangent of the committed state (stability_moduli,
+    // pressure_independ_multi_yield.hpp:174-203): the point's moduli, scaled as paramScaling will
+    // scale them while the switch to the plastic stage is pending
+    std::vector<double> vol, e2p, G, K;
+    if (b->pimy && b->npts > 0) {
+      const size_t np = static_cast<size_t>(b->npts);
+      vol.resize(np);
+      e2p.resize(np);
+      G.resize(np);
+      K.resize(np);
+      const std::pair<double*, int> fields[4] = {{vol.data(), fx::kPStress + 12}, {e2p.data(), fx::kPE2P},
+                                                 {G.data(), fx::kPG}, {K.data(), fx::kPK}};
+      FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
+      for (const auto& fd : fields)
+        FX_HIP(b->ctx, hipMemcpyAsync(fd.first, b->state_c + static_cast<size_t>(fd.second) * np, sizeof(double) * np,
+                                      hipMemcpyDeviceToHost, b->ctx->stream));
+      FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+    }
     const int32_t P = b->padded, N = b->nnode;
     for (int32_t e = 0; e < b->count; ++e) {
       double xe[8][3];
@@ -636,7 +654,27 @@ int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
         for (int k = 0; k < 3; ++k) xe[a][k] = b->host_coords[static_cast<size_t>(k) * N + nd];
       }
       const fx::DevParam& p = b->host_params[b->host_param[e]];
-      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(xe, p.lam, p.mu, p.rho)));
+      if (!b->pimy) {
+        best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(xe, p.lam, p.mu, p.rho)));
+        continue;
+      }
+      const fx::PimyParams& q = b->host_pimy[b->host_param[e]];
+      double lam[8], mu[8];
+      for (int gp = 0; gp < 8; ++gp) {
+        const size_t i = static_cast<size_t>(e) * 8 + gp;
+        double g = G[i], k = K[i];
+        if (q.load_stage == 1 && e2p[i] == 0.0 && q.phi != 0.) {
+          double v = vol[i];
+          if (v > 0.) v = 0.;  // elast2Plast drops the tension first
+          const double con = -(v - q.residual_press);
+          const double scale = std::pow(-con / (q.pref - q.residual_press), q.pdc);
+          g *= scale;
+          k *= scale;
+        }
+        lam[gp] = k - 2. * g / 3.;
+        mu[gp] = g;
+      }
+      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq_points(xe, lam, mu, q.rho)));
     }
   }
   *dt = best;

@@ -164,6 +164,9 @@ void brick_node_weights(const double (&xe)[8][3], double (&w)[8]);
 void brick_stiffness(const double (&xe)[8][3], double lam, double mu, double (&K)[24][24]);
 double max_eigenvalue_symmetric(double (&A)[24][24]);
 double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho);
+void brick_stiffness_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8],
+                            double (&K)[24][24]);
+double brick_max_omega_sq_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8], double rho);
 double stable_dt(int scheme, const DevParam& p, double omega_sq);
 }  // namespace fx
 

@@ -186,6 +186,22 @@ void fxref_pimy_operator(const double* params, int load_stage, int mass_type, co
   else element_operator_apply<PB>(p, d, k, geo, tangent_at, committed_at, v, out);
 }
 
+// StdBrick<PressureIndependMultiYield<20>>::critical_dt at the committed state C (contiguous points):
+// the stability tangent of each point (stability_moduli, :174-203).
+double fxref_pimy_critical_dt(const double* params, int load_stage, const double* ray, int scheme, const double* xe,
+                              const double* C) {
+  using M = PressureIndependMultiYield<20>;
+  using PB = StdBrick<M>;
+  PB::Params p{};
+  std::memcpy(&p.mat, params, sizeof p.mat);
+  p.mat.loadStage = load_stage;
+  double x[8][3];
+  std::memcpy(x, xe, sizeof x);
+  const PointArray<const double> committed{C, 1, M::NSTATE};
+  return PB::critical_dt(p, rayleigh(ray), scheme == 0 ? ExplicitScheme::CentralDifference
+                                                      : ExplicitScheme::ExplicitDifference, x, committed);
+}
+
 // StdBrick::critical_dt (std_brick.hpp:629-694) with ElasticIsotropic (no state):
 // scheme 0 CentralDifference, 1 ExplicitDifference. Returns omega^2 in *omega_sq.
 double fxref_brick_critical_dt(const double* mat, const double* ray, int scheme, const double* xe,

@@ -7,6 +7,8 @@ correct / replay) against the fused one, bit for bit.
 Tolerances: forces, operator, diagonal 1e-10 of the largest entry; histories 1e-10 of each
 field's largest value; CG 1e-8 (the reference's time-history tolerance).
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -349,3 +351,57 @@ def test_damped_explicit_loop(ctx):
     assert rel(du.get(), u) <= 1e-9
     assert rel(dv.get(), v) <= 1e-8
     assert history_error(batch.pimy_state(True), Co) <= 1e-9
+
+
+@pytest.mark.skipif(fxo.ref() is None, reason="oracle/_ref not built")
+def test_stable_step_from_the_committed_stability_tangent(ctx):
+    """fx_critical_dt of a PIMY batch (min_critical_dt, element_kernels.hpp:453-474): every
+    point's stability tangent from its committed state (stability_moduli,
+    pressure_independ_multi_yield.hpp:174-203; a frictional soil scales its moduli by the
+    pressure while the switch to the plastic stage is pending). Against the reference's own
+    StdBrick<PressureIndependMultiYield<20>>::critical_dt (oracle/_ref) element by element, on
+    the committed histories read back from the device; 1e-12 (two eigenvalue solvers)."""
+    from tests.test_oracle_pimy_damping import _ref_params
+    R = fxo.ref()
+    R.fxref_pimy_critical_dt.restype = C.c_double
+    R.fxref_pimy_critical_dt.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+    model = meshes.distorted_box(3, layers=1)
+    pm = core.Mesh(model)
+    m = dict(rho=1900.0, G=60e6, K=160e6, cohesion=5e3, gamma_peak=0.1, phi=30.0, pref=80e3, pdc=0.5, nsurf=20,
+             b=(0.0, 0.0, -1900.0 * 9.81), mass_type=0)
+    keep = []
+    dev_mats = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=20, b=m["b"], mass_type=0, keep=keep) for _ in range(pm.nparam)]
+    batch = core.Batch.pimy(ctx, pm, dev_mats)
+    rp = _ref_params(m, R)
+    ray = np.zeros(4)
+    coords = np.asarray(pm.coords).reshape(3, -1)
+    node = np.asarray(pm.node).reshape(8, -1)
+
+    def reference_dt(stage, scheme):
+        Cs = np.ascontiguousarray(batch.pimy_state(True))
+        best = np.inf
+        for e in range(pm.nelem):
+            xe = np.ascontiguousarray(coords[:, node[:, e]].T)
+            Ce = np.ascontiguousarray(Cs[8 * e:8 * e + 8])
+            best = min(best, R.fxref_pimy_critical_dt(rp.ctypes.data, stage, ray.ctypes.data, scheme, xe.ctypes.data,
+                                                      Ce.ctypes.data))
+        return best
+
+    # compress the box (stage 0), commit, switch to the plastic stage: the pending switch scales
+    n = pm.neq
+    du, df = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    du.upload(meshes.random_vector(n, 71) * 2e-4 - 1e-3)
+    batch.force(du, df)
+    batch.commit()
+    at_rest = batch.critical_dt(0)
+    batch.set_stage(1)
+    for scheme in (0, 1):
+        got, want = batch.critical_dt(scheme), reference_dt(1, scheme)
+        assert abs(got - want) <= 1e-12 * want, (scheme, got, want)
+    assert batch.critical_dt(0) != at_rest, "the pending switch changes the moduli"
+    # after a plastic-stage update and its commit the points carry their own moduli
+    batch.force(du, df)
+    batch.commit()
+    got, want = batch.critical_dt(0), reference_dt(1, 0)
+    assert abs(got - want) <= 1e-12 * want, (got, want)
