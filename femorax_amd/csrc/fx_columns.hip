@@ -24,7 +24,7 @@
 //    history into the other), F and S written per element;
 //  - column_solve_kernel<WAVES, MIXED>: one workgroup of 64 WAVES threads per column of at most
 //    64 WAVES elements, one level per thread. The column's own Newmark/Newton state machine:
-//    unbalance B = -f - M a - a_g M r; the test (NormUnbalance ||B||, or NormDispIncr ||dU||
+//    unbalance B = -f - M a - a_g M r (r the rigid unit translation, base included); the test (NormUnbalance ||B||, or NormDispIncr ||dU||
 //    of the last solve); on convergence the commit (buffer swap), the recorded top-level values
 //    and the next step's predictor; otherwise PCG + Jacobi on K_eff = K_t + c3 M (block
 //    tridiagonal, 3 x 3 blocks held in registers; neighbour values through LDS), warm-started
@@ -328,14 +328,16 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   const double V = col.hx * col.hy * col.hz;
   const Reducer<WAVES, double> RA{redA}, RB{redB}, RC{redC};
   Exchange<double, NT> neighbours{xs, t, N, live};
-  // mass of this level: diagonal and couplings (lumped: no coupling)
-  double m = 0.0, mlo = 0.0, mhi = 0.0, body[3] = {0, 0, 0};
+  // mass of this level: diagonal and couplings (lumped: no coupling); mr: the row of M times the
+  // rigid unit translation, the base included (mass_times_ground), for the ground load
+  double m = 0.0, mlo = 0.0, mhi = 0.0, mr = 0.0, body[3] = {0, 0, 0};
   const bool above = live && t + 1 < N;
   if (live) {
     const PimyParams& pe = a.params[a.elem_param[e]];
     const bool lumped = pe.mass_type != 0;
     m += lumped ? pe.rho * V / 2.0 : pe.rho * V / 3.0;
     if (!lumped && t > 0) mlo = pe.rho * V / 6.0;
+    if (!lumped) mr = pe.rho * V / 6.0;  // the coupling to the level below, the fixed base too
     for (int d = 0; d < 3; ++d) body[d] += V * pe.b[d] / 2.0;
     if (above) {
       const PimyParams& pa = a.params[a.elem_param[e + 1]];
@@ -363,7 +365,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       ma += mhi * ap[d];
       double r = -f;
       r -= ma;
-      if (d == col.ground_dir) r -= ag * (m + mlo + mhi);
+      if (d == col.ground_dir) r -= ag * (m + mr + mhi);
       B[d] = r;
     }
   double bb = 0.0, unused = 0.0;

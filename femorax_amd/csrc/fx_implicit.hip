@@ -46,6 +46,11 @@ __global__ void newmark_update_kernel(int32_t n, const double* du, double* u, do
   a[q] += du[q] * c3;
 }
 
+__global__ void mul_kernel(int32_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ z) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) z[q] = x[q] * y[q];
+}
+
 __global__ void newmark_residual_kernel(int32_t n, const double* f, const double* ma, const double* mr, double ag,
                                         double* b) {
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -102,13 +107,18 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
   }
   const fx_coeffs mass{0.0, 0.0, 1.0, 0};
   const bool ground = d.ground_dir >= 0;
-  if (ground) {  // mass_times_ground: M r, r = 1 at the equations of the excitation's direction
+  if (ground) {
+    // mass_times_ground (internal_force.cpp:601-626): M r with r the rigid unit translation at
+    // every element DOF of the direction, fixed ones included. The element mass (consistent or
+    // lumped) times a rigid translation is its row sums rho sum_gp N_j dvol, so M r is the lumped
+    // mass at the direction's equations.
     std::vector<double> r(N, 0.0);
     for (int32_t q = 0; q < n; ++q)
       if (b->eq_dir[q] == d.ground_dir) r[b->internal ? b->internal_of[q] : q] = 1.0;
     FX_HIP(ctx, hipMemcpyAsync(SCR, r.data(), sizeof(double) * N, hipMemcpyHostToDevice, st));
     FX_HIP(ctx, hipStreamSynchronize(st));
-    if (int rc = operator_apply64(b, mass, SCR, MR)) return rc;
+    hipLaunchKernelGGL(mul_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, b->lumped_m, SCR, MR);
+    FX_HIP(ctx, hipGetLastError());
   }
   const double gamma = d.gamma, beta = d.beta;
   fx_solve_options so{d.cg_rtol, 0.0, d.cg_max_it};

@@ -234,10 +234,14 @@ int fxo_cg_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x
 /* FGMRES(m), CGS2, right Jacobi (gmres.cpp:32-123, block_ops.cpp:59-90) on a system */
 int fxo_gmres_system(fxo_system* s, const fxo_coeffs* k, const double* b, double* x, double rtol, double atol,
                      int32_t max_it, int32_t m, int lanes, fxo_solve_result* res);
-/* nsteps Newmark steps with Newton (implicit_stage.cpp); ground_dir: the influence vector of the
- * excitation's direction (1 at its equations), or NULL. Returns 0, or 1 at the first failed step
- * (its state restored). du: the Krylov initial guess in, the last increment out. */
-int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
+/* y = M r, r the rigid unit translation in direction dir at every element DOF (fixed ones
+ * included): InternalForce::mass_times_ground */
+void fxo_mass_times_ground(const fxo_compiled* c, const fxo_elastic* elastic, const fxo_pimy_elem* pimy, int dir,
+                           double* y, int nthreads);
+/* nsteps Newmark steps with Newton (implicit_stage.cpp); ground_direction: the excitation's
+ * direction 0..2 (its load -a_g M r from fxo_mass_times_ground), or -1. Returns 0, or 1 at the
+ * first failed step (its state restored). du: the Krylov initial guess in, the last increment out. */
+int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, int ground_direction, double* u, double* v, double* a,
                 double* du, fxo_newmark_stats* st);
 
 /* ---- Internal force, stable step, CentralDifference (fxo_time.c) ------------------------- */

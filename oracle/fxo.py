@@ -168,8 +168,9 @@ def lib():
         L.fxo_gmres_system.argtypes = [C.POINTER(System), C.POINTER(Coeffs), f64p, f64p, C.c_double, C.c_double,
                                        C.c_int32, C.c_int32, C.c_int, C.POINTER(SolveResult)]
         L.fxo_newmark.restype = C.c_int
-        L.fxo_newmark.argtypes = [C.POINTER(System), C.POINTER(NewmarkOpts), C.c_void_p, f64p, f64p, f64p, f64p,
+        L.fxo_newmark.argtypes = [C.POINTER(System), C.POINTER(NewmarkOpts), C.c_int, f64p, f64p, f64p, f64p,
                                   C.POINTER(NewmarkStats)]
+        L.fxo_mass_times_ground.argtypes = [C.POINTER(Compiled), C.c_void_p, C.c_void_p, C.c_int, f64p, C.c_int]
         L.fxo_cg.restype = C.c_int
         L.fxo_cg.argtypes = [C.POINTER(Compiled), C.c_void_p, C.POINTER(Coeffs), f64p, f64p, C.c_double,
                              C.c_double, C.c_int32, C.c_int, C.c_int, C.POINTER(SolveResult)]
@@ -368,6 +369,25 @@ class OracleMesh:
                               None if cm is None else cm.ctypes.data, d, nthreads)
         return d
 
+    def ground_direction(self, ground_dir):
+        """ground_dir: None, a direction 0..2, or the influence vector of one direction (1 at its
+        free equations) -> the direction, or -1."""
+        if ground_dir is None:
+            return -1
+        if np.isscalar(ground_dir):
+            return int(ground_dir)
+        r = np.asarray(ground_dir)
+        dirs = {d for d in range(3) for q in self.node_eq.reshape(-1, 3)[:, d] if q >= 0 and r[q] != 0.0}
+        assert len(dirs) == 1, "the influence vector names one direction"
+        return dirs.pop()
+
+    def mass_times_ground(self, d, elastic=None, pimy=None, nthreads=0):
+        """M r with r the rigid unit translation in direction d at every element DOF"""
+        y = np.zeros(self.neq)
+        lib().fxo_mass_times_ground(C.byref(self.c), None if elastic is None else C.cast(elastic, C.c_void_p),
+                                    None if pimy is None else C.cast(pimy, C.c_void_p), d, y, nthreads)
+        return y
+
     def newmark(self, u, v, a, du, dt, nsteps, elastic=None, pimy=None, pimy_state=None, gamma=0.5, beta=0.25,
                 test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, ground_dir=None, ground_accel=None,
                 lanes=64, nthreads=0, max_retries=0, num_sublevels=0, num_substeps=2, series=None, ground_factor=1.0,
@@ -396,9 +416,8 @@ class OracleMesh:
         o = NewmarkOpts(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, lanes,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
                         None if ser is None else C.cast(C.pointer(ser), C.c_void_p), ground_factor, time)
-        gd = None if ground_dir is None else np.ascontiguousarray(ground_dir, np.float64)
         st = NewmarkStats()
-        rc = lib().fxo_newmark(C.byref(s), C.byref(o), None if gd is None else gd.ctypes.data, u, v, a, du, C.byref(st))
+        rc = lib().fxo_newmark(C.byref(s), C.byref(o), self.ground_direction(ground_dir), u, v, a, du, C.byref(st))
         if pimy is not None and s.C != pimy_state[0].ctypes.data:  # an odd number of commits
             tmp = pimy_state[0].copy()
             pimy_state[0][...] = pimy_state[1]

@@ -238,7 +238,7 @@ double fxo_series_value(const fxo_series* s, double fact, double t) {
 typedef struct nm_state {
   fxo_system* s;
   const fxo_newmark_opts* o;
-  const double* ground_dir;
+  int ground;
   fxo_newmark_stats* st;
   int64_t n;
   int32_t step;
@@ -253,7 +253,7 @@ static void nm_unbalance(nm_state* w, double ag) {
   for (int64_t q = 0; q < w->n; ++q) {
     double r = -w->f[q];
     r -= w->tmp[q];
-    if (w->ground_dir) r -= ag * w->mr[q];
+    if (w->ground >= 0) r -= ag * w->mr[q];
     w->b[q] = r;
   }
 }
@@ -276,7 +276,7 @@ static int nm_attempt(nm_state* w, double h) {
   w->st->time += h;
   memcpy(w->du0, w->du, (size_t)n * sizeof(double));
   double ag = 0.0;
-  if (w->ground_dir)
+  if (w->ground >= 0)
     ag = o->ground_series ? fxo_series_value(o->ground_series, o->ground_factor, w->st->time)
                           : (o->ground_accel ? o->ground_accel[w->step] : 0.0);
   const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
@@ -345,7 +345,7 @@ static int nm_recover(nm_state* w, double h, int level) {
   return 1;
 }
 
-int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_dir, double* u, double* v, double* a,
+int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, int ground_direction, double* u, double* v, double* a,
                 double* du, fxo_newmark_stats* st) {
   const int64_t n = s->c->neq;
   memset(st, 0, sizeof(*st));
@@ -353,7 +353,7 @@ int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_d
   nm_state w;
   w.s = s;
   w.o = o;
-  w.ground_dir = ground_dir;
+  w.ground = ground_direction;
   w.st = st;
   w.n = n;
   w.step = 0;
@@ -364,8 +364,8 @@ int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, const double* ground_d
   double** bufs[] = {&w.ut, &w.vt, &w.at, &w.du0, &w.f, &w.tmp, &w.b, &w.scratch};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) *bufs[i] = (double*)malloc((size_t)(n + 1) * sizeof(double));
   w.mr = (double*)calloc((size_t)n + 1, sizeof(double));
-  const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
-  if (ground_dir) sys_apply(s, &mass, ground_dir, w.mr); /* mass_times_ground */
+  if (ground_direction >= 0) /* mass_times_ground */
+    fxo_mass_times_ground(s->c, s->elastic, s->pimy, ground_direction, w.mr, s->nthreads);
   int status = 0;
   for (w.step = 0; w.step < o->nsteps; ++w.step) {
     if (!nm_recover(&w, o->dt, 0)) {

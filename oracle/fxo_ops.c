@@ -50,6 +50,32 @@ static void gather(const fxo_compiled* c, double* y) {
   }
 }
 
+/* InternalForce::mass_times_ground (internal_force.cpp:601-626; the unit_dir source of
+ * operator_kernels.hpp:75-141): y = M r with r the rigid unit translation in direction dir at every
+ * element DOF, fixed ones included; the element masses of elastic or PIMY parameter sets. */
+void fxo_mass_times_ground(const fxo_compiled* c, const fxo_elastic* elastic, const fxo_pimy_elem* pimy, int dir,
+                           double* y, int nthreads) {
+  set_threads(nthreads);
+  const int32_t P = c->padded, E = c->nelem;
+  const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
+#pragma omp parallel for schedule(static)
+  for (int32_t e = 0; e < E; ++e) {
+    double xe[8][3], ve[24], ye[24], D[36];
+    gather_xe(c, e, xe);
+    for (int a = 0; a < 24; ++a) ve[a] = a % 3 == dir ? 1.0 : 0.0;
+    memset(D, 0, sizeof D);
+    if (pimy) {
+      double Dgp[8 * 36];
+      memset(Dgp, 0, sizeof Dgp);
+      fxo_brick_operator_apply_gp(&pimy[c->param[e]], &mass, xe, Dgp, NULL, ve, ye);
+    } else {
+      fxo_brick_operator_apply(&elastic[c->param[e]], &mass, xe, D, ve, ye);
+    }
+    for (int a = 0; a < 24; ++a) c->slots[(int64_t)a * P + e] = ye[a];
+  }
+  gather(c, y);
+}
+
 void fxo_apply(const fxo_compiled* c, const fxo_elastic* params, const fxo_coeffs* k,
                const double* x, double* y, int nthreads) {
   set_threads(nthreads);

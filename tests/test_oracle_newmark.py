@@ -191,3 +191,21 @@ def test_retries_and_substeps_compose_single_attempts():
     for key, val in count.items():
         assert st[key] == val, key
     assert st["time"] == clock[0]
+
+
+def test_ground_load_is_the_mass_times_a_rigid_translation():
+    """mass_times_ground (internal_force.cpp:601-626): M r with r the unit translation at every
+    element DOF, fixed ones included. A consistent element mass times a rigid translation is its
+    row sums rho sum_gp N_j dvol (the shape functions sum to one), so M r is the lumped mass at the
+    direction's free equations; the free-DOF product misses the coupling to the fixed base."""
+    model = meshes.distorted_box(3, layers=1)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    assert all(model["params"][g].get("mass_type", 0) == 0 for g in om.param_group)
+    m, _ = om.lumped_mass(op)
+    for d in range(3):
+        r = ground_dir(om, d)
+        mg = om.mass_times_ground(d, elastic=op)
+        assert np.max(np.abs(mg - m * r)) <= 1e-14 * np.max(m)
+        free_only = om.apply(op, r, fxo.coeffs(0.0, 0.0, 1.0))
+        assert np.max(np.abs(free_only - mg)) >= 1e-2 * np.max(m)
