@@ -76,6 +76,29 @@ def cpu_baseline(model, params, x, budget_s=20.0):
                       f"{per * 1e3:.1f} ms per apply"}
 
 
+def bench_refcol(core, S, ctx, side, reps):
+    """The reference's own operator benchmark (benchmarks/bench_operator.cpp:71-111): the side^3
+    soil column of tests/common/synthetic_models.hpp (consistent mass, two Rayleigh regions,
+    laminar ties), K_eff = K + 2e3 C + 1.6e7 M applied to device-resident vectors. Its published
+    H200 times are beside it for context (a different GPU and build; not this line's metric)."""
+    model = S.reference_column(side)
+    mesh = core.Mesh(model)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model))
+    k = core.coeffs(*S.REFERENCE_COLUMN_COEFFS)
+    x = core.DeviceArray.from_host(ctx, np.full(mesh.neq, 1e-3))
+    y = core.DeviceArray(ctx, mesh.neq)
+    batch.time_apply(x, y, 5, 64, k=k)
+    ms = batch.time_apply(x, y, reps, 64, k=k) / reps
+    res = {"model": f"{side}^3 soil column (synthetic_models.hpp: consistent mass, 2 Rayleigh regions, laminar "
+                    f"ties), K + 2e3 C + 1.6e7 M", "elements": mesh.nelem, "equations": mesh.neq,
+           "path": "lattice" if batch.info()["lattice"] else "general", "us_per_apply": round(ms * 1e3, 2),
+           "gdof_s": round(mesh.neq / (ms * 1e-3) / 1e9, 3)}
+    if side == 40:
+        res["reference_h200_us"] = {"gated_default": 242.3, "fastest_variant": 73.1,
+                                    "source": "benchmarks/baselines/tacc-vista-gh-cuda270336.json:1909, :1789"}
+    return res
+
+
 def bench_c1(core, S, ctx, reps=30, cpu=True):
     """C1: 16^3 box, E 30 MPa, 10 kPa top traction, CG + Jacobi to rtol 1e-8 from x0 = 0; each
     solve is one blocking fx_cg call (the latency a caller sees)."""
@@ -338,6 +361,8 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve", action="store_true", help="also time full plain-FP64 and mixed-precision solves")
+    ap.add_argument("--refcol", type=int, default=40, metavar="SIDE",
+                    help="time the reference's own operator benchmark model, a SIDE^3 soil column (0: skip)")
     ap.add_argument("--c1", type=int, default=1, help="time C1 (16^3 static box) solves: solves/s, latency (0: skip)")
     ap.add_argument("--c4", type=int, default=40, metavar="STEPS",
                     help="time STEPS Newmark steps of C4 (4096 PIMY soil columns) and the plain per-model path "
@@ -450,6 +475,11 @@ def main():
                                                              f"{threads} threads), scaled to the {n_it} iterations "
                                                              f"of the GPU solve"}
         out["c2_static_solve"] = res
+    if args.refcol:
+        try:
+            out["reference_operator_benchmark"] = bench_refcol(core, S, ctx, args.refcol, args.steps)
+        except Exception as e:  # noqa: BLE001
+            out["reference_operator_benchmark"] = {"error": repr(e)}
     if args.c1:
         out["c1_static_solves"] = bench_c1(core, S, ctx, cpu=not args.no_cpu)
     if args.c3:

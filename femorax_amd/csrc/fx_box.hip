@@ -461,6 +461,40 @@ __global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __
     if (mask & (1u << d)) y[base + j++] = acc[d];
 }
 
+// y at tied equation sets (nodes that hold the same equations: the laminar ties of a soil
+// column): the partial sums of every tile at every node of the set, one wave per set, the lanes
+// striding over the set's slots, then a butterfly over the lanes (a fixed order).
+template <typename T>
+__global__ __launch_bounds__(256) void hex8_tied_sum_kernel(const uint32_t* __restrict__ word,
+                                                            const int32_t* __restrict__ start,
+                                                            const T* __restrict__ partial, T* __restrict__ y, int n,
+                                                            int accumulate, const int32_t* done) {
+  const int u = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x) / 64;  // one set per wave
+  const int lane = threadIdx.x % 64;
+  if (u >= n) return;  // the whole wave
+  if (done && *done) return;
+  const int s0 = start[u], s1 = start[u + 1];
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int s = s0 + lane; s < s1; s += 64)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += partial[static_cast<size_t>(s) * 3 + d];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += __shfl_xor(acc[d], off, 64);
+  if (lane != 0) return;
+  const uint32_t w = word[u];
+  const uint32_t mask = (w >> 28) & 7u;
+  const int base = static_cast<int>(w & 0x0fffffffu);
+  int j = 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (!(mask & (1u << d))) continue;
+    T* dst = y + base + j++;
+    *dst = accumulate ? *dst + acc[d] : acc[d];
+  }
+}
+
 namespace {
 // Workgroups of a tile kernel resident at once on the device (blocks per CU x CUs), cached.
 int resident_tiles(fx_ctx* ctx, const void* kernel) {
@@ -550,6 +584,9 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   if (b->nshared > 0)
     hipLaunchKernelGGL(hex8_shared_sum_kernel<T>, dim3((b->nshared + 255) / 256), dim3(256), 0, b->ctx->stream,
                        b->shared_word, b->shared_start, b->shared_meta, a.partial, y, b->nshared, accumulate, done);
+  if (b->ntied > 0)
+    hipLaunchKernelGGL(hex8_tied_sum_kernel<T>, dim3((b->ntied + 3) / 4), dim3(256), 0, b->ctx->stream, b->tied_word,
+                       b->tied_start, a.partial, y, b->ntied, accumulate, done);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }

@@ -120,6 +120,18 @@ double max_eigenvalue_symmetric(double (&A)[24][24]) {
   return lmax;
 }
 
+// the consistent mass of unit density, per node pair: sum_gp N_J N_K dvol (std_brick.hpp:361-383)
+void brick_mass_unit(const double (&xe)[8][3], double (&M)[8][8]) {
+  for (auto& row : M)
+    for (double& v : row) v = 0.0;
+  for (int gp = 0; gp < 8; ++gp) {
+    double N[8], dx[8][3], det;
+    shape_at(xe, gp, N, dx, det);
+    for (int a = 0; a < 8; ++a)
+      for (int b = 0; b < 8; ++b) M[a][b] += N[a] * N[b] * det;
+  }
+}
+
 double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho) {
   const double L[8] = {lam, lam, lam, lam, lam, lam, lam, lam}, M[8] = {mu, mu, mu, mu, mu, mu, mu, mu};
   return brick_max_omega_sq_points(xe, L, M, rho);

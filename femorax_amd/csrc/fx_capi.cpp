@@ -2,6 +2,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <new>
@@ -207,7 +209,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
-                  b->shared_meta, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
+                  b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount};
   for (void* p : ptrs)
@@ -326,6 +328,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     b->ntiles = static_cast<int32_t>(lat.tiles.size());
     b->internal = true;
     b->nshared = static_cast<int32_t>(lat.shared_word.size());
+    b->ntied = static_cast<int32_t>(lat.tied_word.size());
     ok(dev_upload(ctx, b->tiles, lat.tiles.data(), lat.tiles.size(), by)) &&
         ok(dev_upload(ctx, b->words, lat.words.data(), lat.words.size(), by)) &&
         ok(dev_upload(ctx, b->tile_slots, lat.slots.data(), lat.slots.size(), by)) &&
@@ -333,6 +336,8 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         ok(lat.shared_word.empty() ? FX_OK : dev_upload(ctx, b->shared_word, lat.shared_word.data(), lat.shared_word.size(), by)) &&
         ok(lat.shared_start.empty() ? FX_OK : dev_upload(ctx, b->shared_start, lat.shared_start.data(), lat.shared_start.size(), by)) &&
         ok(lat.shared_meta.empty() ? FX_OK : dev_upload(ctx, b->shared_meta, lat.shared_meta.data(), lat.shared_meta.size(), by)) &&
+        ok(lat.tied_word.empty() ? FX_OK : dev_upload(ctx, b->tied_word, lat.tied_word.data(), lat.tied_word.size(), by)) &&
+        ok(lat.tied_word.empty() ? FX_OK : dev_upload(ctx, b->tied_start, lat.tied_start.data(), lat.tied_start.size(), by)) &&
         ok(dev_zeros(ctx, b->partial64, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by)) &&
         ok(b->has_f32 ? dev_zeros(ctx, b->partial32, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by) : FX_OK) &&
         ok(dev_upload(ctx, b->perm, lat.perm.data(), lat.perm.size(), by));
@@ -371,10 +376,21 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         for (int k = 0; k < 3; ++k) xe[a][k] = d->coords[static_cast<size_t>(k) * N + nd];
       }
     };
+    // a tied lattice: the unit stiffness parts and consistent mass of its (congruent) box, for
+    // the diagonal's cross terms of DOFs that one element ties together
+    double Kl[24][24], Km[24][24];
+    double Mn[8][8] = {};
+    const bool tied_lattice = use_lattice && !lat.tied_word.empty();
+    std::map<std::pair<int32_t, int32_t>, std::array<double, 3>> cross;  // (equation, set) -> sums
     if (use_lattice) {
       const double box[8][3] = {{0, 0, 0}, {lat.hx, 0, 0}, {lat.hx, lat.hy, 0}, {0, lat.hy, 0},
                                 {0, 0, lat.hz}, {lat.hx, 0, lat.hz}, {lat.hx, lat.hy, lat.hz}, {0, lat.hy, lat.hz}};
       fx::brick_node_weights(box, w);
+      if (tied_lattice) {
+        fx::brick_stiffness(box, 1.0, 0.0, Kl);
+        fx::brick_stiffness(box, 0.0, 1.0, Km);
+        fx::brick_mass_unit(box, Mn);
+      }
     }
     for (int32_t e = 0; e < E; ++e) {
       if (!use_lattice) {
@@ -383,6 +399,18 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         fx::brick_node_weights(xe, w);
       }
       const fx::DevParam& p = b->host_params[d->param[e]];
+      if (tied_lattice)
+        for (int a = 0; a < 24; ++a) {
+          const int32_t q = d->eq[static_cast<size_t>(a) * P + e];
+          if (q < 0) continue;
+          for (int c = 0; c < 24; ++c) {
+            if (c == a || d->eq[static_cast<size_t>(c) * P + e] != q) continue;
+            auto& s = cross[{q, d->param[e]}];
+            s[0] += Kl[a][c];
+            s[1] += Km[a][c];
+            if (a % 3 == c % 3) s[2] += Mn[a / 3][c / 3];
+          }
+        }
       for (int k = 0; k < 24; ++k) {
         const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
         if (q < 0) continue;
@@ -405,6 +433,19 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
       to_int(lm);
       to_int(la);
       to_int(fb);
+      for (const auto& kv : cross) {
+        b->tie_q.push_back(b->internal_of[kv.first.first]);
+        b->tie_p.push_back(kv.first.second);
+        b->tie_kl.push_back(kv.second[0]);
+        b->tie_km.push_back(kv.second[1]);
+        b->tie_mc.push_back(kv.second[2]);
+        if (b->tie_eqs.empty() || b->tie_eqs.back() != b->tie_q.back()) b->tie_eqs.push_back(b->tie_q.back());
+      }
+      if (!b->tie_eqs.empty()) {
+        b->tie_host.assign(b->tie_eqs.size(), 0.0);
+        ok(dev_upload(ctx, b->tie_idx, b->tie_eqs.data(), b->tie_eqs.size(), by)) &&
+            ok(dev_zeros(ctx, b->tie_val, b->tie_eqs.size(), by));
+      }
     } else {
       b->host_node.assign(d->node, d->node + static_cast<size_t>(8) * P);
       b->host_param.assign(d->param, d->param + P);

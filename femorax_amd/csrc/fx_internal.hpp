@@ -77,6 +77,10 @@ struct Lattice {
   // nodes touched by several tiles, in group order: word (first equation | mask), first slot,
   // and (slot stride | tile count << 24); tile j's partial sum sits at slot first + j * stride,
   // tiles in parity-colour order
+  // tied equation sets (nodes holding the same equations): word (first equation | mask) and
+  // [tied_start[u], tied_start[u + 1]) the slots of every (node, tile) of set u
+  std::vector<uint32_t> tied_word;
+  std::vector<int32_t> tied_start;
   std::vector<uint32_t> shared_word;
   std::vector<int32_t> shared_start;
   std::vector<uint32_t> shared_meta;
@@ -164,6 +168,7 @@ void brick_node_weights(const double (&xe)[8][3], double (&w)[8]);
 void brick_stiffness(const double (&xe)[8][3], double lam, double mu, double (&K)[24][24]);
 double max_eigenvalue_symmetric(double (&A)[24][24]);
 double brick_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho);
+void brick_mass_unit(const double (&xe)[8][3], double (&M)[8][8]);
 void brick_stiffness_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8],
                             double (&K)[24][24]);
 double brick_max_omega_sq_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8], double rho);
@@ -197,6 +202,19 @@ struct fx_batch {
   uint16_t* cell_param = nullptr;
   int32_t ntiles = 0;
   int32_t nshared = 0;
+  uint32_t* tied_word = nullptr;  // lattice tied equation sets (laminar ties)
+  int32_t* tied_start = nullptr;
+  int32_t ntied = 0;
+  // the diagonal's cross terms of tied DOF pairs inside an element (element_operator.hpp:
+  // 138-209): per (tied equation, parameter set) the sums of the unit-coefficient stiffness
+  // (lambda part, mu part) and consistent-mass entries K_ab, M_ab over ordered pairs a != b
+  std::vector<int32_t> tie_q;       // internal equation of each entry
+  std::vector<int32_t> tie_p;       // parameter set of each entry
+  std::vector<double> tie_kl, tie_km, tie_mc;
+  std::vector<int32_t> tie_eqs;     // the distinct tied equations (internal numbering)
+  std::vector<double> tie_host;     // their corrections for the current coefficients
+  int32_t* tie_idx = nullptr;
+  double* tie_val = nullptr;
   uint32_t* shared_word = nullptr;
   int32_t* shared_start = nullptr;
   uint32_t* shared_meta = nullptr;

@@ -61,7 +61,10 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     nk = std::max(nk, ck[e] + 1);
     if (param[e] < 0 || param[e] >= 0xffff) return false;
   }
-  // free DOFs of a node must have consecutive equations (no ties across nodes)
+  // ties: two nodes share either all their equations (equalDOF in every free DOF, the laminar
+  // boundary of a soil column: the same equation triple) or none; tied[n]: another node holds n's
+  // equations. Partial ties go to the general path.
+  std::vector<char> tied(nnode, 0);
   {
     std::unordered_map<int32_t, int32_t> owner;
     for (int32_t n = 0; n < nnode; ++n)
@@ -69,7 +72,11 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
         const int32_t q = node_eq[3 * n + d];
         if (q < 0) continue;
         auto it = owner.emplace(q, n);
-        if (!it.second && it.first->second != n) return false;  // an equation shared by two nodes
+        const int32_t m = it.first->second;
+        if (it.second || m == n) continue;
+        for (int c = 0; c < 3; ++c)
+          if (node_eq[3 * m + c] != node_eq[3 * n + c]) return false;
+        tied[m] = tied[n] = 1;
       }
   }
   // lattice -> element and -> node
@@ -173,7 +180,9 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
   for (int32_t n : tile_nodes)
     if (n >= 0) ++touch_count[n];
   std::vector<int32_t> touch_start(static_cast<size_t>(nnode) + 1, 0);
-  for (int32_t n = 0; n < nnode; ++n) touch_start[n + 1] = touch_start[n] + (touch_count[n] > 1 ? touch_count[n] : 0);
+  // a node gets partial-sum slots where several tiles touch it or it is tied
+  auto slotted = [&](int32_t n) { return touch_count[n] > 1 || tied[n]; };
+  for (int32_t n = 0; n < nnode; ++n) touch_start[n + 1] = touch_start[n] + (slotted(n) ? touch_count[n] : 0);
   std::vector<int32_t> touch_tile(touch_start[nnode]), touch_ln(touch_start[nnode]);
   {
     std::vector<int32_t> fill(touch_start.begin(), touch_start.end() - 1);
@@ -183,7 +192,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
         const TileHeader& h = L.tiles[t];
         for (int32_t i = 0; i < tile_nodes_full; ++i) {
           const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
-          if (n < 0 || touch_count[n] < 2) continue;
+          if (n < 0 || !slotted(n)) continue;
           touch_tile[fill[n]] = t;
           touch_ln[fill[n]++] = i;
         }
@@ -198,7 +207,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     std::map<std::vector<int32_t>, int32_t> group_of;
     std::vector<std::vector<int32_t>> groups;
     for (int32_t n : tile_nodes) {
-      if (n < 0 || touch_count[n] < 2 || seen[n]) continue;
+      if (n < 0 || touch_count[n] < 2 || tied[n] || seen[n]) continue;
       seen[n] = 1;
       std::vector<int32_t> key(touch_tile.begin() + touch_start[n], touch_tile.begin() + touch_start[n + 1]);
       auto it = group_of.find(key);
@@ -210,9 +219,26 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     }
     for (const auto& g : groups) shared_nodes.insert(shared_nodes.end(), g.begin(), g.end());
   }
+  // tied nodes grouped by their equations (one set per tied equation triple), sets and their
+  // nodes in first-touch order
+  std::vector<std::vector<int32_t>> tied_sets;
+  {
+    std::unordered_map<int32_t, int32_t> set_of;  // first free equation -> set
+    std::vector<char> seen(nnode, 0);
+    for (int32_t n : tile_nodes) {
+      if (n < 0 || !tied[n] || seen[n]) continue;
+      seen[n] = 1;
+      int32_t key = -1;
+      for (int d = 0; d < 3 && key < 0; ++d) key = node_eq[3 * n + d];
+      auto it = set_of.emplace(key, static_cast<int32_t>(tied_sets.size()));
+      if (it.second) tied_sets.emplace_back();
+      tied_sets[it.first->second].push_back(n);
+    }
+  }
   // internal numbering: first the nodes only one tile touches (tile by tile, lattice order),
-  // then the shared nodes in group order, then untouched equations. A node's free DOFs are
-  // consecutive. perm: internal -> reference equation.
+  // then the shared nodes in group order, then the tied sets, then untouched equations. A node's
+  // free DOFs are consecutive (a tied set's nodes hold the same ones). perm: internal ->
+  // reference equation.
   std::vector<int32_t> internal_of(static_cast<size_t>(neq), -1);
   L.perm.assign(static_cast<size_t>(neq), -1);
   int32_t next = 0;
@@ -227,8 +253,9 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     }
   };
   for (int32_t n : tile_nodes)
-    if (n >= 0 && !numbered[n] && touch_count[n] == 1) number(n);
+    if (n >= 0 && !numbered[n] && touch_count[n] == 1 && !tied[n]) number(n);
   for (int32_t n : shared_nodes) number(n);
+  for (const auto& s : tied_sets) number(s.front());
   for (int32_t q = 0; q < neq; ++q)
     if (internal_of[q] < 0) {
       internal_of[q] = next;
@@ -279,6 +306,19 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     if (m >= (1 << 24) || nslot >= (int64_t(1) << 28)) return false;
     g0 = g1;
   }
+  // tied sets: one slot per (node, touching tile), a set's slots contiguous (nodes in set order,
+  // each node's tiles in colour order)
+  L.tied_word.clear();
+  L.tied_start.assign(1, static_cast<int32_t>(nslot));
+  for (const auto& s : tied_sets) {
+    uint32_t w = 0;
+    if (!word_of(s.front(), w)) return false;
+    L.tied_word.push_back(w);
+    for (int32_t n : s)
+      for (int32_t j = touch_start[n]; j < touch_start[n + 1]; ++j) slot_of_touch[j] = static_cast<int32_t>(nslot++);
+    if (nslot >= (int64_t(1) << 28)) return false;
+    L.tied_start.push_back(static_cast<int32_t>(nslot));
+  }
   L.nslot = nslot;
   if (nslot >= (int64_t(1) << 28)) return false;  // a shared node's slot takes the word's 28 equation bits
   // tile node words: first equation | free-DOF mask | shared flag; and for the shared nodes the
@@ -293,7 +333,7 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     uint32_t w = 0;
     if (n >= 0) {
       if (!word_of(n, w)) return false;
-      if (touch_count[n] > 1) w |= 0x80000000u;
+      if (slotted(n)) w |= 0x80000000u;
     }
     L.words[i] = w;
   }

@@ -13,6 +13,8 @@
 // Colours of blocks are launched in order, so the scatter is deterministic.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "fx_internal.hpp"
 #include "fx_kernels.hpp"
 #include "fx_brick_device.hpp"
@@ -312,12 +314,37 @@ int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, 
                             : launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
   return rc ? rc : zero_orphans(b, y, done);
 }
+__global__ void add_at_kernel(int32_t n, const int32_t* __restrict__ idx, const double* __restrict__ val,
+                              double* __restrict__ d) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[idx[i]] += val[i];
+}
+
 int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
   if (b->pimy) return pimy_diagonal64(b, k, d);
   const int rc = b->lattice
                      ? launch_box<double, true>(b, k, static_cast<const double*>(nullptr), d, 0, nullptr)
                      : launch_operator<double, true>(b, operator_args<double>(b, k, static_cast<const double*>(nullptr), d, 0));
-  return rc ? rc : zero_orphans(b, d, static_cast<const int32_t*>(nullptr));
+  if (rc) return rc;
+  if (!b->tie_eqs.empty()) {
+    // a tied lattice: the cross terms K_ab + M_ab of the DOF pairs an element ties together, for
+    // this call's coefficients, added to their equations
+    std::fill(b->tie_host.begin(), b->tie_host.end(), 0.0);
+    size_t u = 0;
+    for (size_t i = 0; i < b->tie_q.size(); ++i) {
+      while (b->tie_eqs[u] != b->tie_q[i]) ++u;
+      const DevParam& p = b->host_params[b->tie_p[i]];
+      double fD, cM;
+      operator_factors(p, k, fD, cM);
+      b->tie_host[u] += fD * (p.lam * b->tie_kl[i] + p.mu * b->tie_km[i]) + (p.mass_type == 0 ? cM * p.rho * b->tie_mc[i] : 0.0);
+    }
+    const int32_t n = static_cast<int32_t>(b->tie_eqs.size());
+    FX_HIP(b->ctx, hipMemcpyAsync(b->tie_val, b->tie_host.data(), sizeof(double) * n, hipMemcpyHostToDevice,
+                                  b->ctx->stream));
+    hipLaunchKernelGGL(add_at_kernel, dim3((n + 255) / 256), dim3(256), 0, b->ctx->stream, n, b->tie_idx, b->tie_val, d);
+    FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));  // the host vector is reused by the next call
+  }
+  return zero_orphans(b, d, static_cast<const int32_t*>(nullptr));
 }
 
 }  // namespace fx

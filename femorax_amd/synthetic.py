@@ -106,6 +106,47 @@ def soil_column(nz, h=1.0, params=None, layer_of=None, nx=1, ny=1):
                 h=(h, h, h))
 
 
+def reference_column(side=40, lumped=False, damping=True):
+    """The reference's operator-benchmark model (tests/common/synthetic_models.hpp:24-94 with
+    benchmarks/bench_operator.cpp:71-84): side^3 stdBrick grid of 1 x 1 x 0.5 m bricks, base
+    fixed, the side nodes of each elevation tied to the first one in DOFs 1 2 3, two
+    ElasticIsotropic materials (lower and upper half) with their body forces, consistent mass
+    unless lumped, and two Rayleigh regions: one over every element (alphaM 0.6, betaK 5.8e-4)
+    and one over the top element layer (alphaM 0.9, betaK 3e-4, betaKinit 1e-4; the later region
+    wins). Groups: 0 lower half, 1 upper half below the top layer, 2 the top layer."""
+    nx = ny = nz = side
+    dz = 0.5
+    xyz, conn, ek = grid(nx, ny, nz, 1.0, 1.0, dz, z0=-nz * dz)
+    NX, NY = nx + 1, ny + 1
+    ties = []
+    for k in range(1, nz + 1):
+        master = -1
+        for j in range(NY):
+            for i in range(NX):
+                if not (i == 0 or i == nx or j == 0 or j == ny):
+                    continue
+                t = i + NX * (j + NY * k)
+                if master < 0:
+                    master = t
+                    continue
+                for d in range(3):
+                    ties.append((master, t, d))
+    group = np.where(ek < nz // 2, 0, np.where(ek < nz - 1, 1, 2)).astype(np.int32)
+    mass_type = 1 if lumped else 0
+    ray_all = (0.6, 5.8e-4, 0.0, 0.0) if damping else (0.0, 0.0, 0.0, 0.0)
+    ray_top = (0.9, 3e-4, 1e-4, 0.0) if damping else (0.0, 0.0, 0.0, 0.0)
+    lower = dict(E=377000.0, nu=0.3, rho=2.0285, b=(0.0, 0.0, -19.9), mass_type=mass_type, rayleigh=ray_all)
+    upper = dict(E=109200.0, nu=0.3, rho=2.0183, b=(0.0, 0.0, -19.8), mass_type=mass_type, rayleigh=ray_all)
+    top = dict(upper, rayleigh=ray_top)
+    return dict(nnode=len(xyz), xyz=xyz, nelem=len(conn), conn=conn, group=group, fix=fixed_base(nx, ny),
+                tie=np.array(ties, np.int32).reshape(-1, 3), params=[lower, upper, top], grid=(nx, ny, nz),
+                h=(1.0, 1.0, dz))
+
+
+# the operator coefficients of the reference's operator benchmark (bench_operator.cpp:84)
+REFERENCE_COLUMN_COEFFS = (1.0, 2e3, 1.6e7)
+
+
 def top_traction_nodal(model, traction_z):
     """Consistent nodal loads of a -z traction on the top face (z = max): each top quad gives
     t * A / 4 to its 4 corners. traction_z: scalar or per-top-node array (model order of the

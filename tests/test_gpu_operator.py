@@ -290,3 +290,27 @@ def test_tile8_shape_variants_and_partial_tiles(ctx):
     dx, ya = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, pm.neq)
     a.apply(dx, ya)
     assert rel_err(ya.get(), om.apply(op, x)) <= 1e-12
+
+
+@pytest.mark.parametrize("side,tile8", [(6, False), (20, False), (20, True)])
+def test_reference_benchmark_column_operator(ctx, side, tile8):
+    """The reference's operator-benchmark model (synthetic_models.hpp:24-94: laminar ties,
+    consistent mass, two Rayleigh regions, the top one with betaKinit): the lattice path with
+    tied equation sets (every side node of a level holds the level's equations), the operator
+    with the benchmark's coefficients K + 2e3 C + 1.6e7 M, K alone and the diagonal, against the
+    oracle, on one tile column and on several tiles per level."""
+    model = S.reference_column(side)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    a = core.Batch(ctx, pm, params, tile8=tile8)
+    assert a.info()["lattice"] == 1
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    kk = S.REFERENCE_COLUMN_COEFFS
+    x = meshes.random_vector(pm.neq, 93)
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, pm.neq)
+    for k in (kk, (1.0, 0.0, 0.0)):
+        a.apply(dx, dy, core.coeffs(*k))
+        assert rel_err(dy.get(), om.apply(op, x, fxo.coeffs(*k))) <= 1e-12
+    a.diagonal(dy, core.coeffs(*kk))
+    assert rel_err(dy.get(), om.diagonal(op, fxo.coeffs(*kk))) <= 1e-12
