@@ -96,6 +96,44 @@ def bench_refcol(core, S, ctx, side, reps):
     if side == 40:
         res["reference_h200_us"] = {"gated_default": 242.3, "fastest_variant": 73.1,
                                     "source": "benchmarks/baselines/tacc-vista-gh-cuda270336.json:1909, :1789"}
+    # 20 CG and GMRES(30) iterations with Jacobi on the Newmark (dt = 1e-3) operator, b = 1, x0 = 0,
+    # no tolerance (bench_operator.cpp:113-185); the Jacobi setup is inside our timed call
+    dt = 1e-3
+    kn = core.coeffs(1.0, 0.5 / (0.25 * dt), 1.0 / (0.25 * dt * dt))
+    gb = core.Batch(ctx, mesh, mesh.params_for(model), gmres=True)
+    b = core.DeviceArray.from_host(ctx, np.ones(mesh.neq))
+    xs = core.DeviceArray(ctx, mesh.neq)
+    solves = {}
+    for name, fn in (("cg20", gb.cg), ("gmres20", gb.gmres)):
+        per = []
+        for _ in range(6):
+            xs.zero()
+            ctx.sync()
+            t0 = time.perf_counter()
+            r = fn(b, xs, rtol=0.0, max_it=20, k=kn)
+            ctx.sync()
+            per.append(time.perf_counter() - t0)
+        solves[name] = {"ms": round(float(np.median(per[1:])) * 1e3, 3), "iterations": r["iterations"]}
+    res["solves"] = solves
+    # the explicit internal force with damping at u = 1e-4, v = 1e-3 on the lumped-mass column
+    # (bench_explicit.cpp:76-140)
+    fmodel = S.reference_column(side, lumped=True)
+    fmesh = core.Mesh(fmodel)
+    fb_ = core.Batch(ctx, fmesh, fmesh.params_for(fmodel))
+    du = core.DeviceArray.from_host(ctx, np.full(fmesh.neq, 1e-4))
+    dv = core.DeviceArray.from_host(ctx, np.full(fmesh.neq, 1e-3))
+    df = core.DeviceArray(ctx, fmesh.neq)
+    for _ in range(3):
+        fb_.force(du, df, dv)
+    ctx.sync()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fb_.force(du, df, dv)
+    ctx.sync()
+    res["explicit_force_damped_us"] = round((time.perf_counter() - t0) / reps * 1e6, 2)
+    if side == 40:
+        res["reference_h200"] = {"cg20_ms": 7.676, "gmres20_ms": 8.447, "explicit_force_damped_us": 335.2,
+                                 "source": "benchmarks/baselines/tacc-vista-gh-cuda270336.json:2089, :2134, :394"}
     return res
 
 

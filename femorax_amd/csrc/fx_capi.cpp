@@ -465,7 +465,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(13) * std::max(1, d->neq), by));
     if (flags & FX_BATCH_GMRES)
       ok(dev_zeros(ctx, b->gm,
-                   static_cast<size_t>(2 * fx::kGmresRestart + 3) * std::max(1, d->neq) + fx::kGmresRestart + 1, by));
+                   fx::gmres_workspace(d->neq), by));
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);

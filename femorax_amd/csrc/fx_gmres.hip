@@ -62,6 +62,7 @@ int gmres64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, c
   }
   double *w = b->work, *r = b->work + n, *inv = b->work + 4 * n, *d = b->work + 5 * n;
   double *V = b->gm, *Z = b->gm + (m + 1) * n, *coef = b->gm + (2 * m + 2) * n;
+  double* dscratch = b->gm + (2 * m + 3) * n + m + 1;  // the block dots' partials
   if (int rc = jacobi_inverse_public(b, k, d, inv)) return rc;
   res = fx_solve_result{};
   auto dot = [&](const double* x, const double* y, double& out) { return dot64(ctx, n, x, y, &out); };
@@ -102,10 +103,9 @@ int gmres64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, c
       double *vj = V + static_cast<size_t>(j) * n, *zj = Z + static_cast<size_t>(j) * n;
       hipLaunchKernelGGL(gm_precond_kernel, grid(n), dim3(kThreads), 0, st, n, inv, vj, zj);
       if (int rc = operator_apply64(b, k, zj, w)) return rc;
-      for (int pass = 0; pass < 2; ++pass) {
+      for (int pass = 0; pass < 2; ++pass) {  // CGS2: one block of dots, one block update per pass
         std::vector<double>& hh = pass == 0 ? h : h2;
-        for (int c = 0; c <= j; ++c)
-          if (int rc = dot(V + static_cast<size_t>(c) * n, w, hh[c])) return rc;
+        if (int rc = multidot64(ctx, n, V, j + 1, w, dscratch, hh.data())) return rc;
         if (int rc = combine(V, j + 1, hh, -1.0, w)) return rc;
       }
       for (int i = 0; i <= j; ++i) Hij(i, j) = h[i] + h2[i];

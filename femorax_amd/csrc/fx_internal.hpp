@@ -157,6 +157,13 @@ namespace fx {
 constexpr int kMaxPointLoads = 32;
 constexpr int kMaxPointMaterials = 8;
 constexpr int kGmresRestart = 30;  // GMRES(30) (gmres.hpp default restart)
+// doubles of the GMRES workspace (FX_BATCH_GMRES): V (m + 1 columns), Z (m), the initial guess,
+// the combine coefficients, and the block dots' chunk partials and level buffers
+inline size_t gmres_workspace(int64_t n) {
+  const int64_t ch = (n + 2047) / 2048, lvl = 2 * ((ch + 2047) / 2048 + 2);
+  return static_cast<size_t>(2 * kGmresRestart + 3) * static_cast<size_t>(n > 0 ? n : 1) + kGmresRestart + 1 +
+         static_cast<size_t>(kGmresRestart + 1) * static_cast<size_t>(ch + lvl + 1);
+}
 struct PimyParams;
 // PressureIndependMultiYield (fx_pimy_host.cpp, fx_pimy.hip)
 int pimy_setup(const fx_pimy_material& m, PimyParams& p, std::string& err);

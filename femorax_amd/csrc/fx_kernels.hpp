@@ -58,6 +58,9 @@ int to_external(fx_batch* b, const T* in, T* ext);
 // Solver-side launchers (fx_solver.hip).
 int copy64(fx_ctx* ctx, int64_t n, const double* a, double* b);
 int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
+// out[c] = V(:, c) . w for c < ncol (V column-major, stride n), each in dot64's order, one host
+// synchronisation; scratch: ncol * (chunks + level buffers + 1) doubles (gmres_workspace)
+int multidot64(fx_ctx* ctx, int64_t n, const double* V, int ncol, const double* w, double* scratch, double* out);
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& r);
 int jacobi_inverse_public(fx_batch* b, const fx_coeffs& k, double* d, double* inv);

@@ -280,6 +280,81 @@ int ensure_red(fx_ctx* ctx, int64_t n) {
 
 }  // namespace
 
+// chunk partials of the columns' dots: block (chunk, column), the lane order of chunk_dot_kernel
+__global__ __launch_bounds__(kLanes) void multi_chunk_dot_kernel(int64_t n, const double* __restrict__ V,
+                                                                 const double* __restrict__ w,
+                                                                 double* __restrict__ partial, int64_t nch) {
+  __shared__ double lane_sum[kLanes];
+  const int64_t c = blockIdx.x, col = blockIdx.y;
+  const double* x = V + col * n;
+  const int64_t begin = c * kChunk;
+  const int64_t len = (n - begin < kChunk) ? n - begin : kChunk;
+  const int l = threadIdx.x;
+  double s = 0.0;
+  for (int64_t k = l; k < len; k += kLanes) s += x[begin + k] * w[begin + k];
+  lane_sum[l] = s;
+  __syncthreads();
+  if (l == 0) {
+    double t = 0.0;
+    for (int j = 0; j < kLanes; ++j) t += lane_sum[j];
+    partial[col * nch + c] = t;
+  }
+}
+
+// one workgroup per column: its chunk partials reduced level by level as reduce_kernel does
+__global__ __launch_bounds__(kLanes) void multi_reduce_kernel(int64_t npart, const double* __restrict__ partial,
+                                                              double* __restrict__ levels, int64_t lvl,
+                                                              double* __restrict__ out) {
+  __shared__ double lane_sum[kLanes];
+  const int col = blockIdx.x, l = threadIdx.x;
+  const double* src = partial + col * npart;
+  double* bufs[2] = {levels + col * lvl, levels + col * lvl + lvl / 2};
+  int which = 0;
+  int64_t m = npart;
+  while (m > 1) {
+    const int64_t nch = (m + kChunk - 1) / kChunk;
+    double* dst = bufs[which];
+    for (int64_t c = 0; c < nch; ++c) {
+      const int64_t begin = c * kChunk;
+      const int64_t len = (m - begin < kChunk) ? m - begin : kChunk;
+      double s = 0.0;
+      for (int64_t k = l; k < len; k += kLanes) s += src[begin + k];
+      lane_sum[l] = s;
+      __syncthreads();
+      if (l == 0) {
+        double t = 0.0;
+        for (int j = 0; j < kLanes; ++j) t += lane_sum[j];
+        dst[c] = t;
+      }
+      __syncthreads();
+    }
+    m = nch;
+    src = dst;
+    which ^= 1;
+  }
+  if (l == 0) out[col] = src[0];
+}
+
+int multidot64(fx_ctx* ctx, int64_t n, const double* V, int ncol, const double* w, double* scratch, double* out) {
+  if (ncol <= 0) return FX_OK;
+  if (n <= 0) {
+    for (int c = 0; c < ncol; ++c) out[c] = 0.0;
+    return FX_OK;
+  }
+  const int64_t nch = chunks(n), lvl = 2 * (chunks(nch) + 2);
+  double* partial = scratch;
+  double* levels = partial + static_cast<size_t>(ncol) * nch;
+  double* dres = levels + static_cast<size_t>(ncol) * lvl;
+  hipLaunchKernelGGL(multi_chunk_dot_kernel, dim3(static_cast<unsigned>(nch), static_cast<unsigned>(ncol)), dim3(kLanes),
+                     0, ctx->stream, n, V, w, partial, nch);
+  hipLaunchKernelGGL(multi_reduce_kernel, dim3(static_cast<unsigned>(ncol)), dim3(kLanes), 0, ctx->stream, nch,
+                     partial, levels, lvl, dres);
+  FX_HIP(ctx, hipGetLastError());
+  FX_HIP(ctx, hipMemcpyAsync(out, dres, sizeof(double) * ncol, hipMemcpyDeviceToHost, ctx->stream));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
 int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out) {
   if (n <= 0) {
     *out = 0.0;
