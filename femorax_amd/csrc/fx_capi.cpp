@@ -210,7 +210,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->dyn, b->gm,
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -840,6 +840,9 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
       if (q >= 0) ++start[q + 1];
     }
   for (int32_t q = 0; q < n; ++q) start[q + 1] += start[q];
+  std::vector<int32_t> long_eqs;  // gathered one wave each
+  for (int32_t q = 0; q < n; ++q)
+    if (start[q + 1] - start[q] > 64) long_eqs.push_back(q);
   list.resize(start[n]);
   std::vector<int32_t> fill(start.begin(), start.end() - 1);
   for (int32_t e = 0; e < d->count; ++e)
@@ -860,6 +863,8 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
       ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
       ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&
       ok(list.empty() ? FX_OK : dev_upload(ctx, b->gather_list, list.data(), list.size(), by)) &&
+      ok(long_eqs.empty() ? FX_OK : dev_upload(ctx, b->long_eqs, long_eqs.data(), long_eqs.size(), by)) &&
+      ok((b->nlong = static_cast<int32_t>(long_eqs.size()), FX_OK)) &&
       (!(flags & FX_BATCH_COMPACT) ||
        (ok(dev_zeros(ctx, b->cres, np * 33, by)) && ok(dev_zeros(ctx, b->clist, np, by)) &&
         ok(dev_zeros(ctx, b->ccount, static_cast<size_t>(1), by)))) &&

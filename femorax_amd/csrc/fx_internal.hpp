@@ -264,6 +264,8 @@ struct fx_batch {
   // equation (reference numbering, host)
   double* dyn = nullptr;
   bool pimy_damped = false;  // some PIMY parameter set has Rayleigh damping
+  int32_t* long_eqs = nullptr;  // PIMY gather: the equations with more than 64 slots (tied levels)
+  int32_t nlong = 0;
   std::vector<int8_t> eq_dir;
   // GMRES workspace (FX_BATCH_GMRES): V [(m + 1) n], Z [m n], coefficients [m + 1], CG start [n]
   double* gm = nullptr;

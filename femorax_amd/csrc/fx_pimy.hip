@@ -119,6 +119,15 @@ __device__ inline void node_force(const Shape<double>& s, const double (&sig)[6]
   out[2] = dz * t[2] + dy * t[4] + dx * t[5];
 }
 
+// entry (i, j) of the elastic initial tangent of the reference moduli (initial_D)
+__device__ inline double initial_entry(const PimyParams& p, int i, int j) {
+  double v = 0.;
+  if (i == j) v += p.G;
+  if (i < 3 && j < 3 && i == j) v += p.G;
+  if (i < 3 && j < 3) v += (p.K - 2. * p.G / 3.);
+  return v;
+}
+
 __device__ inline void initial_D(const PimyParams& p, double (&D)[36]) {
   for (int i = 0; i < 6; ++i)
     for (int j = 0; j < 6; ++j) {
@@ -141,10 +150,18 @@ __device__ inline void operator_factors(const PimyParams& p, const fx_coeffs& k,
   cM = k.mass + k.damping * p.alphaM;
 }
 
+// The stored tangents: the 21 upper-triangle entries of each point, entry w of point gp of
+// element e at [(w * 8 + gp) * count + e], so that threads on consecutive elements read
+// consecutive addresses.
+__device__ inline int64_t tangent_at(const PimyBatchArgs& a, int64_t e, int gp, int w) {
+  return (static_cast<int64_t>(w) * 8 + gp) * a.count + e;
+}
+
 // D_eff = cK D_t + c0 D_0 + cc D_c (effective_tangent, element_operator.hpp:58-75); D_t the stored
 // (symmetric) tangent, D_c the committed tangent of the point's committed history
-__device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, int64_t pt, double (&D)[36],
+__device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, int64_t e, int gp, double (&D)[36],
                                    double& cM) {
+  const int64_t pt = e * 8 + gp;
   double cK, c0, cc;
   operator_factors(p, a.k, cK, c0, cc, cM);
   double D0[36];
@@ -154,7 +171,7 @@ __device__ inline void effective_D(const PimyBatchArgs& a, const PimyParams& p, 
   int u = 0;
   double Dt[36];
   for (int i = 0; i < 6; ++i)
-    for (int j = i; j < 6; ++j, ++u) Dt[6 * i + j] = Dt[6 * j + i] = a.tangent[u * a.npts + pt];
+    for (int j = i; j < 6; ++j, ++u) Dt[6 * i + j] = Dt[6 * j + i] = a.tangent[tangent_at(a, e, gp, u)];
   for (int i = 0; i < 36; ++i) {
     double v = cK * Dt[i];
     if (c0 != 0.0) v += c0 * D0[i];
@@ -167,12 +184,13 @@ __device__ inline bool stiffness_damping(const PimyParams& p) {
   return p.betaK != 0.0 || p.betaKinit != 0.0 || p.betaKcomm != 0.0;
 }
 
-// MODE 0: force (update at u; STORE also keeps the tangents; REPLAY takes each point's stress
-// and tangent from the compacted pass instead of updating; DAMP adds the Rayleigh damping force
-// at the velocities a.v, std_brick.hpp:409-464). MODE 1: operator on the stored tangents. MODE 2:
-// its diagonal. DAMP blocks hold half the points (their LDS keeps 9 values per node).
+// The internal force: the update at u at every point (STORE also keeps the tangents; REPLAY takes
+// each point's stress and tangent from the compacted pass instead of updating; DAMP adds the
+// Rayleigh damping force at the velocities a.v, std_brick.hpp:409-464). One thread per point;
+// DAMP blocks hold half the points (their LDS keeps 9 values per node).
 template <int MODE, bool STORE, bool REPLAY = false, bool DAMP = false>
 __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchArgs a) {
+  static_assert(MODE == 0, "the operator and its diagonal run in pimy_operator_kernel");
   constexpr int PTS = DAMP ? kPointsPerBlock / 2 : kPointsPerBlock, W = DAMP ? 9 : 6;
   // per point: 8 nodes x (3 first terms, 3 second terms [, 3 damping terms])
   __shared__ double part[PTS][8 * W];
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
       if constexpr (REPLAY) {
         for (int c = 0; c < 6; ++c) sig[c] = a.cres[c * a.npts + pt];
         if constexpr (STORE)
-          for (int w = 0; w < 21; ++w) a.tangent[w * a.npts + pt] = a.cres[(6 + w) * a.npts + pt];
+          for (int w = 0; w < 21; ++w) a.tangent[tangent_at(a, e, gp, w)] = a.cres[(6 + w) * a.npts + pt];
       } else {
         double u[24], eps[6];
         for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
@@ -202,7 +220,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         if constexpr (STORE) {
           int w = 0;
           for (int i = 0; i < 6; ++i)
-            for (int j = i; j < 6; ++j, ++w) a.tangent[w * a.npts + pt] = D[6 * i + j];
+            for (int j = i; j < 6; ++j, ++w) a.tangent[tangent_at(a, e, gp, w)] = D[6 * i + j];
         }
       }
       for (int j = 0; j < 8; ++j) {
@@ -248,14 +266,69 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
           for (int q = 0; q < 3; ++q) mine[W * j + 6 + q] = dj[q];
         }
       }
-    } else if constexpr (MODE == 1) {
-      double v[24], eps[6], sig[6], D[36], cM;
-      for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
-      effective_D(a, p, pt, D, cM);
+    }
+  }
+  __syncthreads();
+  if (!live || gp != 0) return;
+  double out[24], fdamp[24];
+  for (int k = 0; k < 24; ++k) out[k] = fdamp[k] = 0.0;
+  const PimyParams& p = a.params[a.param[e]];
+  for (int g = 0; g < 8; ++g) {
+    const double* c = part[t + g];
+    for (int j = 0; j < 8; ++j)
+      for (int q = 0; q < 3; ++q) {
+        out[3 * j + q] += c[W * j + q];
+        out[3 * j + q] -= c[W * j + 3 + q];
+        if constexpr (DAMP) fdamp[3 * j + q] += c[W * j + 6 + q];
+      }
+  }
+  if constexpr (DAMP)  // fe = resid + fdamp where the element damps
+    if (stiffness_damping(p) || p.alphaM != 0.0)
+      for (int k = 0; k < 24; ++k) out[k] += fdamp[k];
+  const size_t P = static_cast<size_t>(a.padded);
+  for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
+}
+
+// The operator (DIAG false) and its diagonal on the stored tangents: one thread per element, its
+// 8 points in order, each point's node terms added to the element's 24 values as they come
+// (element_operator_apply / _diagonal, element_operator.hpp:82-209). Registers only. COMM: some
+// parameter set has a committed-tangent term (betaKcomm with damping in the coefficients).
+template <bool DIAG, bool COMM>
+__global__ __launch_bounds__(128) void pimy_operator_kernel(PimyBatchArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.count) return;
+  double X[8], Y[8], Z[8];
+  int eq[24];
+  element_geometry(a, e, X, Y, Z, eq);
+  const PimyParams& p = a.params[a.param[e]];
+  double v[24];
+  if constexpr (!DIAG)
+    for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+  double out[24];
+  for (int k = 0; k < 24; ++k) out[k] = 0.0;
+  double cK, c0, cc, cM;
+  operator_factors(p, a.k, cK, c0, cc, cM);
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    Shape<double> s;
+    gauss_shape(X, Y, Z, gp, s);
+    if constexpr (!DIAG) {
+      // sigma = D_eff eps with D_eff's entries formed one at a time as effective_D forms them
+      double eps[6], sig[6], Dc[36];
+      if constexpr (COMM)
+        if (cc != 0.0) pimy_committed_tangent(p, a.C + static_cast<int64_t>(e) * 8 + gp, a.npts, Dc);
       point_strain(s, v, eps);
       for (int r = 0; r < 6; ++r) {
         double sum = 0.0;
-        for (int c = 0; c < 6; ++c) sum += D[6 * r + c] * eps[c];
+        for (int c = 0; c < 6; ++c) {
+          const int lo = r < c ? r : c, hi = r < c ? c : r;
+          const int w = lo * 6 - lo * (lo - 1) / 2 + (hi - lo);  // packed upper triangle, row-major
+          double d = cK * a.tangent[tangent_at(a, e, gp, w)];
+          if (c0 != 0.0) d += c0 * initial_entry(p, r, c);
+          if constexpr (COMM)
+            if (cc != 0.0) d += cc * Dc[6 * r + c];
+          sum += d * eps[c];
+        }
         sig[r] = sum;
       }
       double g[3] = {0, 0, 0};
@@ -267,13 +340,13 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         node_force(s, sig, j, fj);
         const double temp = s.N[j] * s.dvol * p.rho;
         for (int q = 0; q < 3; ++q) {
-          mine[6 * j + q] = fj[q];
-          mine[6 * j + 3 + q] = cM != 0.0 ? cM * (temp * (p.mass_type == 0 ? g[q] : v[3 * j + q])) : 0.0;
+          out[3 * j + q] += fj[q];
+          if (cM != 0.0) out[3 * j + q] += cM * (temp * (p.mass_type == 0 ? g[q] : v[3 * j + q]));
         }
       }
     } else {
-      double D[36], cM;
-      effective_D(a, p, pt, D, cM);
+      double D[36], cMd;
+      effective_D(a, p, e, gp, D, cMd);
       double de[24];
       for (int k = 0; k < 24; ++k) de[k] = 0.0;
       // B_J^T (D dvol) B_K entry (r, c): with g the gradients, the isotropic-free general form
@@ -284,13 +357,13 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         BJ[4][1] = gJ[2]; BJ[4][2] = gJ[1]; BJ[5][0] = gJ[2]; BJ[5][2] = gJ[0];
         BK[0][0] = gK[0]; BK[1][1] = gK[1]; BK[2][2] = gK[2]; BK[3][0] = gK[1]; BK[3][1] = gK[0];
         BK[4][1] = gK[2]; BK[4][2] = gK[1]; BK[5][0] = gK[2]; BK[5][2] = gK[0];
-        double out = 0.0;
+        double o = 0.0;
         for (int m = 0; m < 6; ++m) {
           double bd = 0.0;
           for (int l = 0; l < 6; ++l) bd += BJ[l][r] * (D[6 * l + m] * s.dvol);
-          out += bd * BK[m][c];
+          o += bd * BK[m][c];
         }
-        return out;
+        return o;
       };
       auto mass_entry = [&](int J, int K) {
         const double temp = s.N[J] * s.dvol * p.rho;
@@ -323,44 +396,9 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
                 const double m = (cM != 0.0 && J != K && r == c) ? cM * mass_entry(J, K) : 0.0;
                 de[first[ia]] += block(J, K, r, c) + m;
               }
-      for (int k = 0; k < 24; ++k) {
-        mine[2 * k] = de[k];
-        mine[2 * k + 1] = 0.0;
-      }
+      for (int k = 0; k < 24; ++k) out[k] += de[k];
     }
   }
-  __syncthreads();
-  if (!live || gp != 0) return;
-  double out[24], fdamp[24];
-  for (int k = 0; k < 24; ++k) out[k] = fdamp[k] = 0.0;
-  const PimyParams& p = a.params[a.param[e]];
-  double cM = 0.0;
-  if constexpr (MODE == 1) {
-    double cK, c0, cc;
-    operator_factors(p, a.k, cK, c0, cc, cM);
-  }
-  for (int g = 0; g < 8; ++g) {
-    const double* c = part[t + g];
-    if constexpr (MODE == 0) {
-      for (int j = 0; j < 8; ++j)
-        for (int q = 0; q < 3; ++q) {
-          out[3 * j + q] += c[W * j + q];
-          out[3 * j + q] -= c[W * j + 3 + q];
-          if constexpr (DAMP) fdamp[3 * j + q] += c[W * j + 6 + q];
-        }
-    } else if constexpr (MODE == 1) {
-      for (int j = 0; j < 8; ++j)
-        for (int q = 0; q < 3; ++q) {
-          out[3 * j + q] += c[6 * j + q];
-          if (cM != 0.0) out[3 * j + q] += c[6 * j + 3 + q];
-        }
-    } else {
-      for (int k = 0; k < 24; ++k) out[k] += c[2 * k];
-    }
-  }
-  if constexpr (DAMP)  // fe = resid + fdamp where the element damps
-    if (stiffness_damping(p) || p.alphaM != 0.0)
-      for (int k = 0; k < 24; ++k) out[k] += fdamp[k];
   const size_t P = static_cast<size_t>(a.padded);
   for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
 }
@@ -425,14 +463,38 @@ __global__ void pimy_clear_count_kernel(int32_t* count) {
   if (threadIdx.x == 0) count[0] = 0;
 }
 
+// y[q] = the sum of equation q's element slots, in (element, DOF) order (assembly.cpp:64-95).
+// An equation with more than kLongGather slots (a tied level of a soil column) is left to
+// long_gather_kernel.
+constexpr int kLongGather = 64;
+
 __global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
                                    const double* __restrict__ slots, double* __restrict__ y, const int32_t* done) {
   if (done && *done) return;
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= neq) return;
+  const int32_t j0 = start[q], j1 = start[q + 1];
+  if (j1 - j0 > kLongGather) return;
   double s = 0.0;
-  for (int32_t j = start[q]; j < start[q + 1]; ++j) s += slots[list[j]];
+  for (int32_t j = j0; j < j1; ++j) s += slots[list[j]];
   y[q] = s;
+}
+
+// The long equations: one wave each, lane l summing slots l, l + 64, ... in order, then a fixed
+// butterfly over the lanes.
+__global__ __launch_bounds__(256) void long_gather_kernel(int32_t nlong, const int32_t* __restrict__ eqs,
+                                                          const int32_t* __restrict__ start,
+                                                          const int32_t* __restrict__ list,
+                                                          const double* __restrict__ slots, double* __restrict__ y,
+                                                          const int32_t* done) {
+  const int u = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x) / 64, lane = threadIdx.x % 64;
+  if (u >= nlong) return;  // the whole wave
+  if (done && *done) return;
+  const int32_t q = eqs[u];
+  double s = 0.0;
+  for (int32_t j = start[q] + lane; j < start[q + 1]; j += 64) s += slots[list[j]];
+  for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+  if (lane == 0) y[q] = s;
 }
 
 __global__ void pimy_batch_init_kernel(int64_t npts, const PimyParams* __restrict__ params,
@@ -473,25 +535,40 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const
   return a;
 }
 
+// MODE 0: the force (STORE: also the tangents); 1: the operator on the stored tangents; 2: its
+// diagonal. The elements' 24 values go to their slots, then each equation gathers its slots.
 template <int MODE, bool STORE>
 int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done) {
   const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
   hipStream_t st = b->ctx->stream;
-  if (b->count > 0 && MODE == 0 && a.v) {  // the damping force: the fused return map, half blocks
-    const dim3 g2((b->count + kElemsPerBlock / 2 - 1) / (kElemsPerBlock / 2));
-    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE, false, true>), g2, dim3(kPointsPerBlock / 2), 0, st, a);
-  } else if (b->count > 0 && MODE == 0 && b->cres) {  // compacted: predict, correct, replay
-    hipLaunchKernelGGL(pimy_predict_kernel, grid, dim3(kPointsPerBlock), 0, st, a);
-    const unsigned cgrid = static_cast<unsigned>(std::min<int64_t>((b->npts + 127) / 128, 2048));
-    hipLaunchKernelGGL(pimy_correct_kernel, dim3(cgrid), dim3(128), 0, st, a);
-    hipLaunchKernelGGL(pimy_clear_count_kernel, dim3(1), dim3(64), 0, st, a.ccount);
-    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE, true>), grid, dim3(kPointsPerBlock), 0, st, a);
-  } else if (b->count > 0) {
-    hipLaunchKernelGGL((pimy_batch_kernel<MODE, STORE>), grid, dim3(kPointsPerBlock), 0, st, a);
+  if constexpr (MODE == 0) {
+    if (b->count > 0 && a.v) {  // the damping force: the fused return map, half blocks
+      const dim3 g2((b->count + kElemsPerBlock / 2 - 1) / (kElemsPerBlock / 2));
+      hipLaunchKernelGGL((pimy_batch_kernel<0, STORE, false, true>), g2, dim3(kPointsPerBlock / 2), 0, st, a);
+    } else if (b->count > 0 && b->cres) {  // compacted: predict, correct, replay
+      hipLaunchKernelGGL(pimy_predict_kernel, grid, dim3(kPointsPerBlock), 0, st, a);
+      const unsigned cgrid = static_cast<unsigned>(std::min<int64_t>((b->npts + 127) / 128, 2048));
+      hipLaunchKernelGGL(pimy_correct_kernel, dim3(cgrid), dim3(128), 0, st, a);
+      hipLaunchKernelGGL(pimy_clear_count_kernel, dim3(1), dim3(64), 0, st, a.ccount);
+      hipLaunchKernelGGL((pimy_batch_kernel<0, STORE, true>), grid, dim3(kPointsPerBlock), 0, st, a);
+    } else if (b->count > 0) {
+      hipLaunchKernelGGL((pimy_batch_kernel<0, STORE>), grid, dim3(kPointsPerBlock), 0, st, a);
+    }
+  } else if (b->count > 0) {  // one element per thread
+    bool comm = false;
+    for (const PimyParams& q : b->host_pimy) comm |= a.k.damping * q.betaKcomm != 0.0;
+    const dim3 g((b->count + 127) / 128);
+    if (comm)
+      hipLaunchKernelGGL((pimy_operator_kernel<MODE == 2, true>), g, dim3(128), 0, st, a);
+    else
+      hipLaunchKernelGGL((pimy_operator_kernel<MODE == 2, false>), g, dim3(128), 0, st, a);
   }
   if (b->neq > 0)
     hipLaunchKernelGGL(slot_gather_kernel, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
                        b->gather_start, b->gather_list, b->slots, y, done);
+  if (b->nlong > 0)
+    hipLaunchKernelGGL(long_gather_kernel, dim3((b->nlong + 3) / 4), dim3(256), 0, b->ctx->stream, b->nlong,
+                       b->long_eqs, b->gather_start, b->gather_list, b->slots, y, done);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }
