@@ -131,9 +131,42 @@ def bench_refcol(core, S, ctx, side, reps):
         fb_.force(du, df, dv)
     ctx.sync()
     res["explicit_force_damped_us"] = round((time.perf_counter() - t0) / reps * 1e6, 2)
+    # the stored-tangent operator (the path of a history material): a PressureIndependMultiYield
+    # batch on the same column, tangents stored at a state, K_eff = K_t + 2e3 C + 1.6e7 M; and
+    # its return-map force at an elastic and at a yielding displacement
+    keep = []
+    mats, rays = [], []
+    for p in model["params"]:
+        G = p["E"] / (2.0 * (1.0 + p["nu"]))
+        K = p["E"] / (3.0 * (1.0 - 2.0 * p["nu"]))
+        mats.append(core.pimy_material(p["rho"], G, K, 30.0, 0.1, 0.0, 100.0, 0.0, nsurf=20, b=p["b"], mass_type=0,
+                                       keep=keep))
+        rays.append(p["rayleigh"])
+    pb = core.Batch.pimy(ctx, mesh, [mats[g] for g in range(len(mats))], rayleigh=rays)
+    pb.set_stage(1)
+    z = np.linspace(0.0, 1.0, mesh.neq)
+    dup = core.DeviceArray.from_host(ctx, 2e-4 * np.sin(40.0 * z))
+    dfp = core.DeviceArray(ctx, mesh.neq)
+    pb.store_tangents(dup)
+    pb.time_apply(x, y, 3, 64, k=k)
+    ms_t = pb.time_apply(x, y, reps, 64, k=k) / reps
+    res["pimy_stored_tangent_apply_us"] = round(ms_t * 1e3, 2)
+    for name, amp in (("elastic", 1e-7), ("yielding", 2e-3)):
+        dup.upload(amp * np.sin(40.0 * z))
+        pb.force(dup, dfp)
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            pb.force(dup, dfp)
+        ctx.sync()
+        res[f"pimy_force_{name}_us"] = round((time.perf_counter() - t0) / 5 * 1e6, 1)
+    active = pb.pimy_state(False)[:, 26]
+    res["pimy_yielding_points_frac"] = round(float(np.mean(active > 0)), 3)
     if side == 40:
         res["reference_h200"] = {"cg20_ms": 7.676, "gmres20_ms": 8.447, "explicit_force_damped_us": 335.2,
-                                 "source": "benchmarks/baselines/tacc-vista-gh-cuda270336.json:2089, :2134, :394"}
+                                 "stored_tangent_apply_us": 487.1,
+                                 "source": "benchmarks/baselines/tacc-vista-gh-cuda270336.json:2089, :2134, :394, "
+                                           ":1939"}
     return res
 
 
