@@ -60,6 +60,9 @@ __global__ void pimy_init_kernel(int64_t npts, const PimyParams* __restrict__ pa
 namespace {
 
 constexpr int kPointsPerBlock = 128;  // 16 elements
+// batches below this many elements run the operator one thread per point (enough threads to
+// fill the device), larger ones one thread per element (no LDS, sums in registers)
+constexpr int kPointOperatorBelow = 32768;
 constexpr int kElemsPerBlock = kPointsPerBlock / 8;
 
 struct PimyBatchArgs {
@@ -289,6 +292,60 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
   for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
 }
 
+// diag(B^T D_eff B dvol + cM M) at one point, tied DOFs of the element merged onto the first
+// (element_operator_diagonal, element_operator.hpp:138-209)
+__device__ inline void element_point_diagonal(const Shape<double>& s, const PimyParams& p, const double (&D)[36],
+                                              double cM, const int (&eq)[24], double (&de)[24]) {
+  for (int k = 0; k < 24; ++k) de[k] = 0.0;
+  // B_J^T (D dvol) B_K entry (r, c): with g the gradients, the isotropic-free general form
+  auto block = [&](int J, int K, int r, int c) {
+    double BJ[6][3] = {}, BK[6][3] = {};
+    const double gJ[3] = {s.dx[J], s.dy[J], s.dz[J]}, gK[3] = {s.dx[K], s.dy[K], s.dz[K]};
+    BJ[0][0] = gJ[0]; BJ[1][1] = gJ[1]; BJ[2][2] = gJ[2]; BJ[3][0] = gJ[1]; BJ[3][1] = gJ[0];
+    BJ[4][1] = gJ[2]; BJ[4][2] = gJ[1]; BJ[5][0] = gJ[2]; BJ[5][2] = gJ[0];
+    BK[0][0] = gK[0]; BK[1][1] = gK[1]; BK[2][2] = gK[2]; BK[3][0] = gK[1]; BK[3][1] = gK[0];
+    BK[4][1] = gK[2]; BK[4][2] = gK[1]; BK[5][0] = gK[2]; BK[5][2] = gK[0];
+    double o = 0.0;
+    for (int m = 0; m < 6; ++m) {
+      double bd = 0.0;
+      for (int l = 0; l < 6; ++l) bd += BJ[l][r] * (D[6 * l + m] * s.dvol);
+      o += bd * BK[m][c];
+    }
+    return o;
+  };
+  auto mass_entry = [&](int J, int K) {
+    const double temp = s.N[J] * s.dvol * p.rho;
+    if (p.mass_type == 0) return temp * s.N[K];
+    return J == K ? temp : 0.0;
+  };
+  int first[24];
+  bool merged = false;
+  for (int k = 0; k < 24; ++k) {
+    first[k] = k;
+    if (eq[k] < 0) continue;
+    for (int b = 0; b < k; ++b)
+      if (eq[b] == eq[k]) {
+        first[k] = first[b];
+        merged = true;
+        break;
+      }
+  }
+  for (int j = 0; j < 8; ++j) {
+    const double m = cM != 0.0 ? cM * mass_entry(j, j) : 0.0;
+    for (int q = 0; q < 3; ++q) de[3 * j + q] += block(j, j, q, q) + m;
+  }
+  if (merged)
+    for (int J = 0; J < 8; ++J)
+      for (int K = 0; K < 8; ++K)
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c) {
+            const int ia = 3 * J + r, ic = 3 * K + c;
+            if (ia == ic || eq[ia] < 0 || eq[ia] != eq[ic]) continue;
+            const double m = (cM != 0.0 && J != K && r == c) ? cM * mass_entry(J, K) : 0.0;
+            de[first[ia]] += block(J, K, r, c) + m;
+          }
+}
+
 // The operator (DIAG false) and its diagonal on the stored tangents: one thread per element, its
 // 8 points in order, each point's node terms added to the element's 24 values as they come
 // (element_operator_apply / _diagonal, element_operator.hpp:82-209). Registers only. COMM: some
@@ -348,55 +405,81 @@ __global__ __launch_bounds__(128) void pimy_operator_kernel(PimyBatchArgs a) {
       double D[36], cMd;
       effective_D(a, p, e, gp, D, cMd);
       double de[24];
-      for (int k = 0; k < 24; ++k) de[k] = 0.0;
-      // B_J^T (D dvol) B_K entry (r, c): with g the gradients, the isotropic-free general form
-      auto block = [&](int J, int K, int r, int c) {
-        double BJ[6][3] = {}, BK[6][3] = {};
-        const double gJ[3] = {s.dx[J], s.dy[J], s.dz[J]}, gK[3] = {s.dx[K], s.dy[K], s.dz[K]};
-        BJ[0][0] = gJ[0]; BJ[1][1] = gJ[1]; BJ[2][2] = gJ[2]; BJ[3][0] = gJ[1]; BJ[3][1] = gJ[0];
-        BJ[4][1] = gJ[2]; BJ[4][2] = gJ[1]; BJ[5][0] = gJ[2]; BJ[5][2] = gJ[0];
-        BK[0][0] = gK[0]; BK[1][1] = gK[1]; BK[2][2] = gK[2]; BK[3][0] = gK[1]; BK[3][1] = gK[0];
-        BK[4][1] = gK[2]; BK[4][2] = gK[1]; BK[5][0] = gK[2]; BK[5][2] = gK[0];
-        double o = 0.0;
-        for (int m = 0; m < 6; ++m) {
-          double bd = 0.0;
-          for (int l = 0; l < 6; ++l) bd += BJ[l][r] * (D[6 * l + m] * s.dvol);
-          o += bd * BK[m][c];
-        }
-        return o;
-      };
-      auto mass_entry = [&](int J, int K) {
-        const double temp = s.N[J] * s.dvol * p.rho;
-        if (p.mass_type == 0) return temp * s.N[K];
-        return J == K ? temp : 0.0;
-      };
-      int first[24];
-      bool merged = false;
-      for (int k = 0; k < 24; ++k) {
-        first[k] = k;
-        if (eq[k] < 0) continue;
-        for (int b = 0; b < k; ++b)
-          if (eq[b] == eq[k]) {
-            first[k] = first[b];
-            merged = true;
-            break;
-          }
-      }
-      for (int j = 0; j < 8; ++j) {
-        const double m = cM != 0.0 ? cM * mass_entry(j, j) : 0.0;
-        for (int q = 0; q < 3; ++q) de[3 * j + q] += block(j, j, q, q) + m;
-      }
-      if (merged)
-        for (int J = 0; J < 8; ++J)
-          for (int K = 0; K < 8; ++K)
-            for (int r = 0; r < 3; ++r)
-              for (int c = 0; c < 3; ++c) {
-                const int ia = 3 * J + r, ic = 3 * K + c;
-                if (ia == ic || eq[ia] < 0 || eq[ia] != eq[ic]) continue;
-                const double m = (cM != 0.0 && J != K && r == c) ? cM * mass_entry(J, K) : 0.0;
-                de[first[ia]] += block(J, K, r, c) + m;
-              }
+      element_point_diagonal(s, p, D, cMd, eq, de);
       for (int k = 0; k < 24; ++k) out[k] += de[k];
+    }
+  }
+  const size_t P = static_cast<size_t>(a.padded);
+  for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
+}
+
+// The same for small batches, one thread per point (8 lanes per element), so that a batch of a
+// few hundred elements still runs thousands of threads: each point leaves its node terms in
+// LDS and the element's first lane adds them in point order (the same sums as above).
+template <bool DIAG>
+__global__ __launch_bounds__(kPointsPerBlock) void pimy_point_operator_kernel(PimyBatchArgs a) {
+  __shared__ double part[kPointsPerBlock][48];  // per point: 8 nodes x (3 stiffness, 3 mass) or 24 x 2
+  const int t = threadIdx.x;
+  const int e = blockIdx.x * kElemsPerBlock + t / 8, gp = t % 8;
+  const bool live = e < a.count;
+  if (live) {
+    double X[8], Y[8], Z[8];
+    int eq[24];
+    element_geometry(a, e, X, Y, Z, eq);
+    const PimyParams& p = a.params[a.param[e]];
+    Shape<double> s;
+    gauss_shape(X, Y, Z, gp, s);
+    double* mine = part[t];
+    double D[36], cM;
+    effective_D(a, p, e, gp, D, cM);
+    if constexpr (!DIAG) {
+      double v[24], eps[6], sig[6];
+      for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+      point_strain(s, v, eps);
+      for (int r = 0; r < 6; ++r) {
+        double sum = 0.0;
+        for (int c = 0; c < 6; ++c) sum += D[6 * r + c] * eps[c];
+        sig[r] = sum;
+      }
+      double g[3] = {0, 0, 0};
+      if (cM != 0.0 && p.mass_type == 0)
+        for (int kk = 0; kk < 8; ++kk)
+          for (int q = 0; q < 3; ++q) g[q] += s.N[kk] * v[3 * kk + q];
+      for (int j = 0; j < 8; ++j) {
+        double fj[3];
+        node_force(s, sig, j, fj);
+        const double temp = s.N[j] * s.dvol * p.rho;
+        for (int q = 0; q < 3; ++q) {
+          mine[6 * j + q] = fj[q];
+          mine[6 * j + 3 + q] = cM != 0.0 ? cM * (temp * (p.mass_type == 0 ? g[q] : v[3 * j + q])) : 0.0;
+        }
+      }
+    } else {
+      double de[24];
+      element_point_diagonal(s, p, D, cM, eq, de);
+      for (int k = 0; k < 24; ++k) {
+        mine[2 * k] = de[k];
+        mine[2 * k + 1] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (!live || gp != 0) return;
+  const PimyParams& p = a.params[a.param[e]];
+  double cK, c0, cc, cM;
+  operator_factors(p, a.k, cK, c0, cc, cM);
+  double out[24];
+  for (int k = 0; k < 24; ++k) out[k] = 0.0;
+  for (int g = 0; g < 8; ++g) {
+    const double* c = part[t + g];
+    if constexpr (!DIAG) {
+      for (int j = 0; j < 8; ++j)
+        for (int q = 0; q < 3; ++q) {
+          out[3 * j + q] += c[6 * j + q];
+          if (cM != 0.0) out[3 * j + q] += c[6 * j + 3 + q];
+        }
+    } else {
+      for (int k = 0; k < 24; ++k) out[k] += c[2 * k];
     }
   }
   const size_t P = static_cast<size_t>(a.padded);
@@ -554,6 +637,8 @@ int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done
     } else if (b->count > 0) {
       hipLaunchKernelGGL((pimy_batch_kernel<0, STORE>), grid, dim3(kPointsPerBlock), 0, st, a);
     }
+  } else if (b->count > 0 && b->count < kPointOperatorBelow) {  // small: one thread per point
+    hipLaunchKernelGGL((pimy_point_operator_kernel<MODE == 2>), grid, dim3(kPointsPerBlock), 0, st, a);
   } else if (b->count > 0) {  // one element per thread
     bool comm = false;
     for (const PimyParams& q : b->host_pimy) comm |= a.k.damping * q.betaKcomm != 0.0;

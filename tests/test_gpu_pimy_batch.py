@@ -405,3 +405,36 @@ def test_stable_step_from_the_committed_stability_tangent(ctx):
     batch.commit()
     got, want = batch.critical_dt(0), reference_dt(1, 0)
     assert abs(got - want) <= 1e-12 * want, (got, want)
+
+
+def test_large_batch_operator_one_element_per_thread(ctx):
+    """A batch of 33^3 elements (above the size where the operator and its diagonal run one
+    thread per element): the operator with the Newmark-like coefficients and damping terms, and
+    the diagonal, on tangents stored at a yielding state, against the oracle."""
+    model = S.elastic_box(33, 33.0, layers=3, body=True, seed=17)
+    pm = core.Mesh(model)
+    assert pm.nelem >= 32768
+    mats = soil(20261215, 3)
+    per_set = [dict(mats[g % 3], rayleigh=DAMP) for g in range(pm.nparam)]
+    keep = []
+    dev_mats = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"],
+                                   m["pdc"], nsurf=m["nsurf"], b=m["b"], mass_type=m["mass_type"], keep=keep)
+                for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev_mats, rayleigh=[DAMP] * len(per_set))
+    batch.set_stage(1)
+    om = fxo.OracleMesh(model)
+    pe = fxo.pimy_elems([per_set[g] for g in range(pm.nparam)], stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    n = pm.neq
+    u = meshes.random_vector(n, 61) * 2e-3
+    du = core.DeviceArray.from_host(ctx, u)
+    batch.store_tangents(du)
+    _, D_o = om.pimy_force(pe, Co, To, u, want_D=True)
+    x = meshes.random_vector(n, 62)
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, n)
+    for kk in ((1.0, 400.0, 8e4, False), (1.0, 0.0, 0.0, True)):
+        batch.apply(dx, dy, core.coeffs(*kk))
+        assert rel(dy.get(), om.apply_gp(pe, D_o, x, fxo.coeffs(*kk), committed=Co)) <= 1e-10, kk
+        batch.diagonal(dy, core.coeffs(*kk))
+        assert rel(dy.get(), om.diagonal_gp(pe, D_o, fxo.coeffs(*kk), committed=Co)) <= 1e-10, kk
