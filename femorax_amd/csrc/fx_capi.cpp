@@ -210,7 +210,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->dyn, b->gm,
+                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -859,6 +859,7 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
   const size_t np = static_cast<size_t>(std::max<int64_t>(b->npts, 1));
   ok(dev_upload(ctx, b->pimy_params, pp.data(), pp.size(), by)) &&
       ok(dev_zeros(ctx, b->state_c, np * fx::kPimyNState, by)) && ok(dev_zeros(ctx, b->state_t, np * fx::kPimyNState, by)) &&
+      ok(dev_zeros(ctx, b->pimy_touched, np, by)) &&
       ok(dev_zeros(ctx, b->tangent, np * 21, by)) &&
       ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
       ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&

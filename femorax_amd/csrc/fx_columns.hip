@@ -79,6 +79,7 @@ struct PoolArgs {
   const int32_t* elem_param;    // [cap] parameter set
   const PimyParams* params;
   double* state;                // 2 buffers of [kPimyNState * cap]
+  int32_t* touched;             // [cap] the highest surface whose centre an update has written
   double* eout;                 // [9 * cap]: F (3), S upper triangle (6)
   double* u;                    // [3 cap] each
   double* v;
@@ -120,7 +121,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
   const double* C = a.state + (col.cur ? NS : 0) + i;
   double* T = a.state + (col.cur ? 0 : NS) + i;
   double sig[6], D[36];
-  pimy_update(a.params[a.elem_param[i]], C, T, a.cap, eps, sig, D);
+  pimy_update(a.params[a.elem_param[i]], C, T, a.cap, eps, sig, D, a.touched + i);
   column_outputs(a, i, A, hz, sig, D);
 }
 
@@ -163,7 +164,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
       const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(a.cap);
       double sig[6], D[36];
       plastic = pimy_predict(a.params[a.elem_param[i]], a.state + (col.cur ? NS : 0) + i,
-                             a.state + (col.cur ? 0 : NS) + i, a.cap, eps, sig, D);
+                             a.state + (col.cur ? 0 : NS) + i, a.cap, eps, sig, D, a.touched + i);
       if (plastic) {
         a.ceps[i] = eps[2];
         a.ceps[a.cap + i] = eps[4];
@@ -193,7 +194,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE))) void
     double eps[6] = {0.0, 0.0, a.ceps[i], 0.0, a.ceps[a.cap + i], a.ceps[2 * a.cap + i]};
     double sig[6], D[36];
     pimy_update(a.params[a.elem_param[i]], a.state + (col.cur ? NS : 0) + i, a.state + (col.cur ? 0 : NS) + i, a.cap,
-                eps, sig, D);
+                eps, sig, D, a.touched + i);
     column_outputs(a, i, col.hx * col.hy, col.hz, sig, D);
   }
 }
@@ -597,10 +598,11 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
 }
 
 __global__ void column_init_kernel(int64_t cap, int64_t e0, int32_t n, const PimyParams* __restrict__ params,
-                                   const int32_t* __restrict__ elem_param, double* state) {
+                                   const int32_t* __restrict__ elem_param, double* state, int32_t* touched) {
   const int64_t k = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (k >= n) return;
   const int64_t i = e0 + k;
+  touched[i] = 0;  // every centre at its initial value in both buffers
   const PimyParams& p = params[elem_param[i]];
   const size_t NS = static_cast<size_t>(kPimyNState) * static_cast<size_t>(cap);
   for (double* S : {state, state + NS}) {
@@ -826,7 +828,8 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   double* rec = nullptr;
   if ((rc = pool_alloc(p, cols, d->max_columns)) || (rc = pool_alloc(p, elem_col, cap)) ||
       (rc = pool_alloc(p, elem_param, cap)) || (rc = pool_alloc(p, params, d->max_materials)) ||
-      (rc = pool_alloc(p, a.state, 2 * kPimyNState * cap)) || (rc = pool_alloc(p, a.eout, 9 * cap)) ||
+      (rc = pool_alloc(p, a.state, 2 * kPimyNState * cap)) || (rc = pool_alloc(p, a.touched, cap)) ||
+      (rc = pool_alloc(p, a.eout, 9 * cap)) ||
       (rc = pool_alloc(p, a.u, 3 * cap)) || (rc = pool_alloc(p, a.v, 3 * cap)) || (rc = pool_alloc(p, a.a, 3 * cap)) ||
       (rc = pool_alloc(p, a.ut, 3 * cap)) || (rc = pool_alloc(p, a.vt, 3 * cap)) ||
       (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) ||
@@ -958,7 +961,7 @@ int fx_column_add(fx_column_pool* p, const fx_column_desc* d, int32_t* id) {
                              hipMemcpyHostToDevice, st));
   for (double* q : {a.u, a.v, a.a, a.ut, a.vt, a.at, a.du}) FX_HIP(ctx, hipMemsetAsync(q + 3 * e0, 0, n3, st));
   hipLaunchKernelGGL(column_init_kernel, dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, st, a.cap, e0, N,
-                     a.params, a.elem_param, a.state);
+                     a.params, a.elem_param, a.state, a.touched);
   FX_HIP(ctx, hipGetLastError());
   // the column joins the running set last: slot ownership, then its status
   FX_HIP(ctx, hipMemcpyAsync(const_cast<int32_t*>(a.elem_col) + e0, ecol.data(), sizeof(int32_t) * N,

@@ -265,6 +265,7 @@ struct fx_batch {
   double* dyn = nullptr;
   bool pimy_damped = false;  // some PIMY parameter set has Rayleigh damping
   int32_t* long_eqs = nullptr;  // PIMY gather: the equations with more than 64 slots (tied levels)
+  int32_t* pimy_touched = nullptr;  // per point: the highest surface whose centre an update has written
   int32_t nlong = 0;
   std::vector<int8_t> eq_dir;
   // GMRES workspace (FX_BATCH_GMRES): V [(m + 1) n], Z [m n], coefficients [m + 1], CG start [n]

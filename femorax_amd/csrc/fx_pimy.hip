@@ -78,6 +78,7 @@ struct PimyBatchArgs {
   double* tangent;  // [21 * npts] upper triangle row-major, or null
   const double* x;  // u (force) or v (operator)
   const double* v;  // force: the velocities of the Rayleigh damping force, or null
+  int32_t* touched; // per point: the highest surface whose centre an update has written
   double* slots;    // [24 * padded]
   fx_coeffs k;
   double* cres;     // compacted return map: [33 * npts] stress, tangent (upper), strain
@@ -219,7 +220,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
         double u[24], eps[6];
         for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
         point_strain(s, u, eps);
-        pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+        pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D, a.touched + pt);
         if constexpr (STORE) {
           int w = 0;
           for (int i = 0; i < 6; ++i)
@@ -507,7 +508,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_predict_kernel(PimyBatch
     double u[24], eps[6], sig[6], D[36];
     for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
     point_strain(s, u, eps);
-    plastic = pimy_predict(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+    plastic = pimy_predict(p, a.C + pt, a.T + pt, a.npts, eps, sig, D, a.touched + pt);
     if (plastic) {
       for (int c = 0; c < 6; ++c) a.cres[(27 + c) * a.npts + pt] = eps[c];
     } else {
@@ -534,7 +535,7 @@ __global__ __launch_bounds__(128) void pimy_correct_kernel(PimyBatchArgs a) {
     const PimyParams& p = a.params[a.param[pt / 8]];
     double eps[6], sig[6], D[36];
     for (int c = 0; c < 6; ++c) eps[c] = a.cres[(27 + c) * a.npts + pt];
-    pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D);
+    pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D, a.touched + pt);
     for (int c = 0; c < 6; ++c) a.cres[c * a.npts + pt] = sig[c];
     int w = 0;
     for (int i = 0; i < 6; ++i)
@@ -610,6 +611,7 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const
   a.tangent = b->tangent;
   a.x = x;
   a.v = v;
+  a.touched = b->pimy_touched;
   a.slots = b->slots;
   a.k = k;
   a.cres = b->cres;

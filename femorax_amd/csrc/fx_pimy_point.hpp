@@ -151,8 +151,15 @@ struct Point {
   T2 cur_stress, cur_strain, trial, rate, sub_rate;
   int committed_active = 0, active = 0, e2p = 0;
   double G = 0, K = 0;
+  // The surfaces whose centres get_stress copies from C into T: a batch keeps per point the
+  // highest surface whose centre any update has written, and every centre above it holds its
+  // initial value in both buffers, so the copy can stop there (ncopy) with T equal to C after
+  // it, as if all were copied. moved: the highest surface this update writes.
+  int ncopy;
+  mutable int moved = 0;
 
-  __device__ Point(const PimyParams& pp, const double* c, double* t, int64_t stride) : p(pp), C(c), T(t), s(stride) {}
+  __device__ Point(const PimyParams& pp, const double* c, double* t, int64_t stride, int copy = -1)
+      : p(pp), C(c), T(t), s(stride), ncopy(copy < 0 || copy > pp.nsurf ? pp.nsurf : copy) {}
 
   __device__ double c(int k) const { return C[k * s]; }
   __device__ double& t(int k) const { return T[k * s]; }
@@ -173,6 +180,7 @@ struct Point {
     return o;
   }
   __device__ void set_center(int i, const V6& o) const {
+    moved = i > moved ? i : moved;
     for (int k = 0; k < 6; ++k) t(kPCenters + 6 * (i - 1) + k) = o.v[k];
   }
   __device__ T2 load_t2(int k) const {
@@ -473,7 +481,7 @@ struct Point {
     // committed surfaces into the trial buffer (centres of the nsurf surfaces in use)
     if (!fixed_surfaces())
       for (int k = kPSizes; k < kPCenters; ++k) t(k) = c(k);
-    for (int k = kPCenters; k < kPCenters + 6 * p.nsurf; ++k) t(k) = c(k);
+    for (int k = kPCenters; k < kPCenters + 6 * ncopy; ++k) t(k) = c(k);
     if (p.load_stage == 1 && e2p == 0) elastic_to_plastic();
     if (p.load_stage != 1) {
       double De[36];
@@ -543,10 +551,15 @@ struct Point {
 
 }  // namespace pimy
 
+// touched (may be null: every centre copied): the point's highest surface whose centre an update
+// has written, read and raised here
 __device__ inline void pimy_update(const PimyParams& p, const double* C, double* T, int64_t stride,
-                                   const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
-  pimy::Point pt(p, C, T, stride);
+                                   const double (&eps)[6], double (&sig)[6], double (&D)[36],
+                                   int32_t* touched = nullptr) {
+  const int copy = touched ? *touched : -1;
+  pimy::Point pt(p, C, T, stride, copy);
   pt.update(eps, sig, D);
+  if (touched && pt.moved > copy) *touched = pt.moved;
 }
 
 // committed_tangent (:162-168): getTangent with the committed stress, active surface, surfaces
@@ -562,9 +575,13 @@ __device__ inline void pimy_committed_tangent(const PimyParams& p, const double*
 }
 
 __device__ inline bool pimy_predict(const PimyParams& p, const double* C, double* T, int64_t stride,
-                                    const double (&eps)[6], double (&sig)[6], double (&D)[36]) {
-  pimy::Point pt(p, C, T, stride);
-  return pt.predict(eps, sig, D);
+                                    const double (&eps)[6], double (&sig)[6], double (&D)[36],
+                                    int32_t* touched = nullptr) {
+  const int copy = touched ? *touched : -1;
+  pimy::Point pt(p, C, T, stride, copy);
+  const bool plastic = pt.predict(eps, sig, D);
+  if (touched && pt.moved > copy) *touched = pt.moved;
+  return plastic;
 }
 
 }  // namespace fx
