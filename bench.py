@@ -289,7 +289,7 @@ def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cp
            "column_steps_per_s": round(steps / sec, 1), "pool_iterations": calls * per_call,
            "ms_per_pool_iteration": round(sec / (calls * per_call) * 1e3, 4),
            "newton_per_step": round(newton / max(steps, 1), 3), "cg_per_newton": round(krylov / max(newton, 1), 2),
-           "state_bytes_per_iteration": total * 2 * 150 * 8,
+           "state_bytes_full_copy": total * 2 * 150 * 8,  # upper bound: unmoved surface centres are not copied
            "return_map": "compacted (predict, compact, correct)" if compact else "fused",
            "waves_per_simd": {0: 3, 1: "compiler"}.get(waves, waves),
            "solve": "FP32 inner PCG in FP64 refinement" if mixed else "FP64 PCG",

@@ -80,7 +80,7 @@ class CdDesc(C.Structure):
 
 class TimeSeries(C.Structure):
     _fields_ = [("values", C.c_void_p), ("count", C.c_int32), ("dt", C.c_double), ("factor", C.c_double),
-                ("start", C.c_double), ("use_last", C.c_int32)]
+                ("start", C.c_double), ("use_last", C.c_int32), ("time", C.c_void_p), ("loc", C.c_void_p)]
 
 
 class NewmarkDesc(C.Structure):
@@ -118,6 +118,7 @@ FX_COLUMN_COMPACT = 4
 FX_BATCH_COMPACT = 128
 FX_BATCH_TILE4 = 256
 FX_BATCH_TILE8 = 512
+FX_BATCH_TILE16 = 1024
 
 
 class ColumnPoolDesc(C.Structure):
@@ -458,7 +459,7 @@ class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
     def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False,
-                 gmres=False, tile8=None):
+                 gmres=False, tile8=None, tile_depth=None):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -478,6 +479,8 @@ class Batch:
         flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
             (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0) | \
             (FX_BATCH_GMRES if gmres else 0) | (0 if tile8 is None else FX_BATCH_TILE8 if tile8 else FX_BATCH_TILE4)
+        if tile_depth is not None:  # cells per lattice tile in z: 4, 8 or 16
+            flags |= {4: FX_BATCH_TILE4, 8: FX_BATCH_TILE8, 16: FX_BATCH_TILE16}[tile_depth]
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)
@@ -521,11 +524,18 @@ class Batch:
         series: the excitation as a uniform path series, dict(values, dt, factor=1, start=0,
         use_last=False), evaluated at every attempt's end time (needed to substep)."""
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
-        sv = ts = None
+        sv = ts = tv = None
         if series is not None:
+            # {"values", "dt"[, "factor", "start", "use_last"]}: PathSeries; {"values", "time"[, "factor",
+            # "loc"]}: PathTimeSeries, "loc" an int32 array of one entry holding the interval walk
             sv = np.ascontiguousarray(series["values"], np.float64)
-            ts = TimeSeries(sv.ctypes.data, len(sv), series["dt"], series.get("factor", 1.0), series.get("start", 0.0),
-                            int(bool(series.get("use_last", False))))
+            tv = None if series.get("time") is None else np.ascontiguousarray(series["time"], np.float64)
+            loc = series.get("loc")
+            if loc is not None:
+                assert isinstance(loc, np.ndarray) and loc.dtype == np.int32 and loc.size == 1 and loc.flags.c_contiguous
+            ts = TimeSeries(sv.ctypes.data, len(sv), series.get("dt", 0.0), series.get("factor", 1.0),
+                            series.get("start", 0.0), int(bool(series.get("use_last", False))),
+                            None if tv is None else tv.ctypes.data, None if loc is None else loc.ctypes.data)
         d = NewmarkDesc(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, ground_dir,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
                         None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time)

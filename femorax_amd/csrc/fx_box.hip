@@ -264,18 +264,26 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t,
   }
 }
 
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
 // One pass of a tile: cell (cx, cy, cz = wave + 4 pass). MODE: 0 apply without mass terms, 1
 // apply with mass terms, 2 diagonal. The cell's results are summed per node and committed.
 //
 // Every lane runs the full 8 x 8 plane: a cell past the end of a cut tile has no parameter set
 // (zero results) and a node past it has no word (nothing fetched, nothing committed), so the
-// node sums need no bounds. Node (x, y, z) of the tile sits at x + 9 (y + 9 z).
-template <typename T, int MODE, int TZ>
+// node sums need no bounds. Node (x, y, z) of the tile sits at x + 9 (y + 9 z') in LDS, with
+// z' = z (RING = 0: every node plane of the tile held) or z mod RING (a ring of RING planes).
+// mid() runs after the barrier that ends the pass's reads of x, before its commits.
+template <typename T, int MODE, int TZ, int RING = 0, typename Mid = NoHook>
 __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T* xs, const uint32_t* wsh, T* top,
-                                         int t, int pass) {
+                                         int t, int pass, const Mid& mid = Mid{}) {
   using G = TileGeom<TZ>;
   constexpr bool DIAG = MODE == 2;
   constexpr int NX = kTileX + 1;
+  constexpr int LDS_NODES = RING ? RING * kPlane : G::kNodes;  // component stride of xs
+  auto lp = [](int z) { return RING ? z % RING : z; };
   const int cx = t % kTileX, cy = (t / kTileX) % kTileY;
   const int cz = __builtin_amdgcn_readfirstlane(t / (kTileX * kTileY)) + 4 * pass;  // one plane per wave
   T out[3][8];
@@ -302,8 +310,8 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
     } else {
       int ln[8];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * ((cy + ((q >> 1) & 1)) + (kTileY + 1) * (cz + (q >> 2)));
-      box_apply<T, MODE == 1, G::kNodes>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
+      for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * (cy + ((q >> 1) & 1)) + kPlane * lp(cz + (q >> 2));
+      box_apply<T, MODE == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
                                          static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
     }
   }
@@ -340,6 +348,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
       }
     }
   __syncthreads();
+  mid();
   // commit: plane z = cz (bottom face + the plane below's top face), and for the last plane of
   // cells also plane z = TZ (its own top face)
   auto commit = [&](int ln, const T (&val)[3]) {
@@ -372,12 +381,12 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
     T val[3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) val[c] = cz > 0 ? F0[c][r] + top[slot(c, cz - 1) + pi[r]] : F0[c][r];
-    commit(pi[r] + kPlane * cz, val);
+    commit(pi[r] + kPlane * lp(cz), val);
     if (cz == TZ - 1) {
       T up[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) up[c] = top[slot(c, cz) + pi[r]];
-      commit(pi[r] + kPlane * (cz + 1), up);
+      commit(pi[r] + kPlane * lp(cz + 1), up);
     }
   }
 }
@@ -417,6 +426,105 @@ __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_
     __syncthreads();  // the LDS takes the next tile
     tile = next;
     f = fn;
+  }
+}
+
+// Streamed tiles: the x and the words of only five node planes sit in LDS, in rings (node plane
+// z at ring plane z mod 5, ring index = tile node index mod 405). The first five planes are
+// fetched before the first pass; each later pass's four new planes are loaded while the pass
+// before it commits, and stored over the planes that pass has finished with. 21 KB of LDS per
+// workgroup against 30 KB with every plane held: seven resident workgroups per CU, not five.
+constexpr int kRingPlanes = 5;
+constexpr int kRingNodes = kRingPlanes * kPlane;
+
+template <typename T>
+struct PlaneFetch {
+  uint32_t w[2];
+  T v[2][3];
+};
+
+// tile nodes [n0, n0 + cnt) (whole node planes, cnt <= 512) into registers: thread t takes
+// nodes n0 + t and n0 + t + 256
+template <typename T, bool DIAG>
+__device__ __forceinline__ void fetch_planes(const BoxArgs<T>& a, const uint32_t* words, const int32_t* slots, int n0,
+                                             int cnt, int t, PlaneFetch<T>& f) {
+  uint32_t w[2];
+  int32_t s[2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int j = t + r * kTileThreads;
+    w[r] = j < cnt ? words[n0 + j] : 0u;
+    s[r] = j < cnt ? slots[n0 + j] : 0;
+  }
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const uint32_t mask = DIAG ? 0u : (w[r] >> 28) & 7u;
+    const T* src = a.x + (w[r] & 0x0fffffffu);
+    if (mask == 7u) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d) f.v[r][d] = src[d];
+    } else {
+      int j = 0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        f.v[r][d] = (mask & (1u << d)) ? src[j] : T(0);
+        j += (mask >> d) & 1u;
+      }
+    }
+    f.w[r] = (w[r] & 0x80000000u) ? ((w[r] & 0xf0000000u) | static_cast<uint32_t>(s[r])) : w[r];
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ void store_planes(const PlaneFetch<T>& f, int n0, int cnt, int t, T* xs, uint32_t* wsh) {
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int j = t + r * kTileThreads;
+    if (j < cnt) {
+      const int ln = (n0 + j) % kRingNodes;
+      wsh[ln] = f.w[r];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) xs[d * kRingNodes + ln] = f.v[r][d];
+    }
+  }
+}
+
+template <typename T, int MODE, int WAVES, int TZ>
+__global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_stream_kernel(BoxArgs<T> a) {
+  using G = TileGeom<TZ>;
+  constexpr bool DIAG = MODE == 2;
+  static_assert(G::kPasses >= 2, "streamed tiles have at least two passes");
+  __shared__ T xs[3 * kRingNodes];
+  __shared__ uint32_t wsh[kRingNodes];
+  __shared__ T top[3 * G::kRing * kPlane];
+  const int t = threadIdx.x;
+  if (a.done && *a.done) return;
+  const int tile = blockIdx.x;
+  const uint32_t* words = a.words + static_cast<size_t>(tile) * G::kNodes;
+  const int32_t* slots = a.slots + static_cast<size_t>(tile) * G::kNodes;
+  int pidx[G::kPasses];
+#pragma unroll
+  for (int p = 0; p < G::kPasses; ++p) pidx[p] = a.cell_param[static_cast<size_t>(tile) * G::kCells + t + kTileThreads * p];
+  {
+    PlaneFetch<T> f;
+    fetch_planes<T, DIAG>(a, words, slots, 0, kRingNodes, t, f);
+    store_planes(f, 0, kRingNodes, t, xs, wsh);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int pass = 0; pass < G::kPasses; ++pass) {
+    if (pass + 1 < G::kPasses) {
+      // planes 4 pass + 5 .. 4 pass + 8 replace planes 4 pass .. 4 pass + 3 (ring slots)
+      const int n0 = (4 * pass + kRingPlanes) * kPlane;
+      PlaneFetch<T> f;
+      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass,
+                                         [&] { fetch_planes<T, DIAG>(a, words, slots, n0, 4 * kPlane, t, f); });
+      __syncthreads();  // this pass's commits have read the words of its planes
+      store_planes(f, n0, 4 * kPlane, t, xs, wsh);
+      __syncthreads();
+    } else {
+      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass);
+    }
   }
 }
 
@@ -541,10 +649,26 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   static const Kernel table4[3][8] = FX_BOX_TABLE(4);
   static const Kernel table8[3][8] = FX_BOX_TABLE(8);
 #undef FX_BOX_TABLE
-  const int v = b->box_variant & 7;
-  const Kernel kernel = (b->tile_z == 8 ? table8 : table4)[DIAG ? 2 : mass ? 1 : 0][v];
+  // bit 3: streamed tiles (rings of five node planes in LDS; 8-deep tiles). 16-deep tiles are
+  // always streamed (bits 0-1 the register budget as above, bit 2 ignored).
+#define FX_STREAM_TABLE(TZ)                                                                                         \
+  {{hex8_stream_kernel<T, 0, 0, TZ>, hex8_stream_kernel<T, 0, 5, TZ>, hex8_stream_kernel<T, 0, 7, TZ>,             \
+    hex8_stream_kernel<T, 0, 8, TZ>},                                                                              \
+   {hex8_stream_kernel<T, 1, 0, TZ>, hex8_stream_kernel<T, 1, 0, TZ>, hex8_stream_kernel<T, 1, 0, TZ>,             \
+    hex8_stream_kernel<T, 1, 0, TZ>},                                                                              \
+   {hex8_stream_kernel<T, 2, 0, TZ>, hex8_stream_kernel<T, 2, 0, TZ>, hex8_stream_kernel<T, 2, 0, TZ>,             \
+    hex8_stream_kernel<T, 2, 0, TZ>}}
+  static const Kernel stream8[3][4] = FX_STREAM_TABLE(8);
+  static const Kernel stream16[3][4] = FX_STREAM_TABLE(16);
+#undef FX_STREAM_TABLE
+  const int v = b->box_variant & 15;
+  const int m = DIAG ? 2 : mass ? 1 : 0;
+  const bool stream = b->tile_z == 16 || ((v & 8) && b->tile_z == 8);
+  const Kernel kernel = b->tile_z == 16 ? stream16[m][v & 3]
+                        : stream        ? stream8[m][v & 3]
+                                        : (b->tile_z == 8 ? table8 : table4)[m][v & 7];
   int grid = b->ntiles;
-  if (v & 4) {
+  if ((v & 4) && !stream) {
     const int resident = resident_tiles(b->ctx, reinterpret_cast<const void*>(kernel));
     if (resident > 0 && resident < grid) grid = resident >= 8 ? resident / 8 * 8 : resident;
   }

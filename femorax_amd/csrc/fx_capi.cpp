@@ -290,8 +290,9 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   const bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
                            fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
                                              (flags & FX_BATCH_XCD_SLABS) != 0,
-                                             (flags & FX_BATCH_TILE4)   ? 4
-                                             : (flags & FX_BATCH_TILE8) ? 8
+                                             (flags & FX_BATCH_TILE4)    ? 4
+                                             : (flags & FX_BATCH_TILE8)  ? 8
+                                             : (flags & FX_BATCH_TILE16) ? 16
                                              : E >= (1 << 20)           ? 8
                                                                         : 4,
                                              lat);
@@ -498,14 +499,17 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
 int fx_batch_tune(fx_batch* b, int variant) {
   if (!b) return fail(nullptr, FX_EINVAL, "fx_batch_tune: null batch");
   if (variant >= 0) {
-    if (variant > 7) return fail(b->ctx, FX_EINVAL, "fx_batch_tune: variant out of range");
+    if (variant > 15) return fail(b->ctx, FX_EINVAL, "fx_batch_tune: variant out of range");
     b->box_variant = variant;
     b->info.variant = variant;
     return FX_OK;
   }
   if (!b->lattice) return FX_OK;  // one variant only
-  // time each variant on scratch vectors: a warm-up, then the best of three (the reference's
-  // force-settings autotune, src/exec/force_settings.cpp:135-155)
+  // time each variant on scratch vectors (the reference's force-settings autotune,
+  // src/exec/force_settings.cpp:135-155): three rounds over the variants, each sample a warm-up
+  // apply then ten back-to-back applies between two events (one apply is a few tens of
+  // microseconds: single-apply samples picked a different variant from run to run), and the
+  // best sample per variant kept
   fx_ctx* ctx = b->ctx;
   const fx_coeffs k{1.0, 0.0, 0.0, 0};
   double* x = b->work;
@@ -513,26 +517,27 @@ int fx_batch_tune(fx_batch* b, int variant) {
   hipEvent_t e0, e1;
   FX_HIP(ctx, hipEventCreate(&e0));
   FX_HIP(ctx, hipEventCreate(&e1));
-  float best = 1e30f;
-  int best_v = 0;
-  for (int v = 0; v < 8; ++v) {
-    b->box_variant = v;
-    if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
-    float vbest = 1e30f;
-    for (int r = 0; r < 3; ++r) {
-      FX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+  // 16-deep tiles: the four register budgets of the streamed kernel; 8-deep: held and streamed
+  // tiles; 4-deep: held tiles only
+  const int nvar = b->tile_z == 16 ? 4 : b->tile_z == 8 ? 12 : 8;
+  constexpr int kSampleApplies = 10;
+  std::vector<float> vbest(static_cast<size_t>(nvar), 1e30f);
+  for (int round = 0; round < 3; ++round)
+    for (int v = 0; v < nvar; ++v) {
+      b->box_variant = v;
       if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
+      FX_HIP(ctx, hipEventRecord(e0, ctx->stream));
+      for (int r = 0; r < kSampleApplies; ++r)
+        if (int rc = fx::operator_apply64(b, k, x, y)) return rc;
       FX_HIP(ctx, hipEventRecord(e1, ctx->stream));
       FX_HIP(ctx, hipEventSynchronize(e1));
       float ms = 0;
       FX_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
-      vbest = std::min(vbest, ms);
+      vbest[v] = std::min(vbest[v], ms);
     }
-    if (vbest < best) {
-      best = vbest;
-      best_v = v;
-    }
-  }
+  int best_v = 0;
+  for (int v = 1; v < nvar; ++v)
+    if (vbest[v] < vbest[best_v]) best_v = v;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->box_variant = best_v;
@@ -928,8 +933,17 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
     return fail(b->ctx, FX_EINVAL, "fx_newmark: max_retries and num_sublevels must be non-negative, num_substeps positive");
   if (d->ground_dir >= 0 && d->num_sublevels > 0 && !d->ground_series)
     return fail(b->ctx, FX_EINVAL, "fx_newmark: substeps under an excitation need its time series (ground_series)");
-  if (d->ground_series && (!d->ground_series->values || d->ground_series->count < 1 || !(d->ground_series->dt > 0)))
-    return fail(b->ctx, FX_EINVAL, "fx_newmark: bad ground_series");
+  if (const fx_time_series* gs = d->ground_series) {
+    if (!gs->values || gs->count < 1 || (!gs->time && !(gs->dt > 0)))
+      return fail(b->ctx, FX_EINVAL, "fx_newmark: bad ground_series");
+    if (gs->time) {
+      for (int32_t i = 0; i < gs->count; ++i)
+        if (!std::isfinite(gs->time[i]) || (i > 0 && gs->time[i] < gs->time[i - 1]))
+          return fail(b->ctx, FX_EINVAL, "fx_newmark: ground_series times must be finite and nondecreasing");
+      if (gs->loc && (*gs->loc < 0 || *gs->loc >= gs->count))
+        return fail(b->ctx, FX_EINVAL, "fx_newmark: ground_series interval (*loc) out of range");
+    }
+  }
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
   return fx::newmark_run(b, *d, u, v, a, du, *result);
 }

@@ -63,10 +63,45 @@ __global__ void newmark_residual_kernel(int32_t n, const double* f, const double
 
 }  // namespace
 
-// PathSeries::getFactor for a uniform series (ground_motion.cpp:17-33), times the pattern factor;
-// 0 before time 0 (GroundMotions::evaluate, :121-145).
-double series_value(const fx_time_series& s, double fact, double t) {
+// PathTimeSeries::getFactor (ground_motion.cpp:43-74): the interval [time[l], time[l + 1]] found
+// by walking on from the previous call's l, forward or back (loc keeps it); a time equal to the
+// walk's lower end gives that sample exactly; 0 outside the data.
+double path_time(const fx_time_series& s, int32_t& loc, double t) {
+  const double* v = s.values;
+  const double* tm = s.time;
+  const int32_t n = s.count;
+  int32_t l = loc;
+  double t1 = tm[l];
+  if (t < t1 && l == 0) return 0.0;
+  if (t == t1) return s.factor * v[l];
+  if (t > t1 && l == n - 1) return 0.0;
+  double t2 = tm[l + 1];
+  if (t > t2) {
+    while (t > t2 && l < n - 2) {
+      ++l;
+      t1 = t2;
+      t2 = tm[l + 1];
+    }
+    loc = l;
+    if (t > t2) return 0.0;
+  } else if (t < t1) {
+    while (t < t1 && l > 0) {
+      --l;
+      t2 = t1;
+      t1 = tm[l];
+    }
+    loc = l;
+    if (t < t1) return 0.0;
+  }
+  return s.factor * (v[l] + (v[l + 1] - v[l]) * (t - t1) / (t2 - t1));
+}
+
+// The series' value times the pattern factor: PathSeries::getFactor for a uniform series
+// (ground_motion.cpp:17-33), PathTimeSeries for a time array; 0 before time 0
+// (GroundMotions::evaluate, :121-145).
+double series_value(const fx_time_series& s, double fact, double t, int32_t& loc) {
   if (t < 0.0) return 0.0;
+  if (s.time) return fact * path_time(s, loc, t);
   double a = 0.0;
   if (!(t < s.start)) {
     const double incr = (t - s.start) / s.dt;
@@ -89,6 +124,9 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
   const int32_t n = b->neq;
   out = fx_newmark_result{};
   out.time = d.time;
+  // a time-array series walks on from the caller's interval (or interval 0) and keeps its walk
+  // from attempt to attempt
+  int32_t loc = d.ground_series && d.ground_series->loc ? *d.ground_series->loc : 0;
   if (n == 0 || d.nsteps <= 0) return FX_OK;
   const size_t N = static_cast<size_t>(n);
   double* W = b->dyn;
@@ -155,7 +193,7 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
                        a2, a3, a4);
     out.time += h;
     if (int rc = copy64(ctx, n, dU, dU0)) return rc;  // the solver's initial guess, restored on failure
-    ag = !ground ? 0.0 : d.ground_series ? series_value(*d.ground_series, d.ground_factor, out.time) : d.ground_accel[step];
+    ag = !ground ? 0.0 : d.ground_series ? series_value(*d.ground_series, d.ground_factor, out.time, loc) : d.ground_accel[step];
     const fx_coeffs k{1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
     if (int rc = unbalance()) return rc;
     int iter = 1, iterations = 0;
@@ -226,6 +264,7 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     }
     out.steps = step + 1;
   }
+  if (d.ground_series && d.ground_series->loc) *d.ground_series->loc = loc;
   if (b->internal) {
     if (int rc = to_external<double>(b, U, u)) return rc;
     if (int rc = to_external<double>(b, V, v)) return rc;

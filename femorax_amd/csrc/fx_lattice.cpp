@@ -20,7 +20,7 @@ namespace fx {
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
                    const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, int tile_z,
                    Lattice& L) {
-  if (count <= 0 || (tile_z != 4 && tile_z != 8)) return false;
+  if (count <= 0 || (tile_z != 4 && tile_z != 8 && tile_z != 16)) return false;
   const double* X = coords;
   const double* Y = coords + nnode;
   const double* Z = coords + 2 * static_cast<size_t>(nnode);

@@ -156,6 +156,8 @@ typedef struct fx_batch_desc {
 #define FX_BATCH_TILE8 512u   /* lattice path: tiles of 8 x 8 x 8 cells (two passes). Neither flag: 8-deep
                                  tiles from 2^20 cells up (C2: 0.0625 ms against 0.0670 ms), 4-deep below
                                  (more workgroups for small latency-bound models) */
+#define FX_BATCH_TILE16 1024u /* lattice path: tiles of 8 x 8 x 16 cells (four passes, streamed: x and words
+                                 of five node planes in LDS at a time) */
 #define FX_BATCH_COMPACT 128u /* stdBrick<PressureIndependMultiYield> batches: the compacted return map
                                  (compacted_kernels.hpp:85-202): an elastic predictor per Gauss point, the
                                  plastic points listed, one full update per listed point, the element
@@ -174,10 +176,11 @@ typedef struct fx_batch_info {
 int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
 /* Kernel-variant choice (the reference's force-settings autotune, src/exec/force_settings.cpp:
  * 135-205): variant >= 0 sets it, -1 times the variants on this batch and keeps the fastest.
- * Lattice batches have variants 0-7 (bits 0-1: the tile kernel's register budget, the compiler's
+ * Lattice batches have variants 0-15 (bits 0-1: the tile kernel's register budget, the compiler's
  * choice or at least 5, 7, 8 waves per SIMD; bit 2: persistent workgroups that prefetch their
- * next tile); general batches have one. Every variant gives the
- * same bits. */
+ * next tile; bit 3: streamed 8-deep tiles, x and words of five node planes in LDS at a time;
+ * 16-deep tiles are always streamed); general batches have one. The autotune times 0-7 (4-deep
+ * tiles), 0-11 (8-deep) or 0-3 (16-deep). Every variant gives the same bits. */
 int fx_batch_tune(fx_batch* b, int variant);
 
 /* y = K_eff x, K_eff = cK K_t + cC C + cM M (InternalForce::apply_operator,
@@ -326,11 +329,20 @@ int fx_pimy_state(fx_batch* b, int committed, double* dst);
  * t is factor * (linear interpolation of values[] at (t - start) / dt); 0 before start and, past
  * the last value, 0 or (use_last) factor * values[count - 1]. A series the reference reads with
  * a leading zero (prepend_zero) is passed with that zero in values[]. */
+/* An excitation's time series (GroundMotions, src/exec/ground_motion.cpp:17-74): with time NULL,
+ * PathSeries (uniform step dt from start, use_last holding the last value past the end; the
+ * caller prepends OpenSees' leading zero if it wants one); with time set, PathTimeSeries
+ * (linear between the given nondecreasing times, 0 outside them, use_last, dt and start unused),
+ * whose interval walk is kept in *loc between calls as GroundMotions keeps it per pattern (loc
+ * NULL: each fx_newmark call walks from interval 0). Either form is 0 before time 0. */
 typedef struct fx_time_series {
   const double* values;       /* host [count] */
   int32_t count;
   double dt, factor, start;
   int32_t use_last;
+  const double* time;         /* host [count] or NULL */
+  int32_t* loc;               /* host: the walk's interval, read at the start and written at the end of
+                                 fx_newmark, or NULL */
 } fx_time_series;
 typedef struct fx_newmark_desc {
   double gamma, beta, dt;     /* Newmark constants and step */

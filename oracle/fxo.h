@@ -199,12 +199,16 @@ typedef struct fxo_system {
   double* D;                  /* PIMY stored tangents [points * 36] */
   int nthreads;
 } fxo_system;
-/* PathSeries with a uniform step (ground_motion.cpp:17-33) */
+/* PathSeries with a uniform step (ground_motion.cpp:17-33), or, with time set, PathTimeSeries
+ * (:43-74: linear between the given times, its interval walk kept in *loc between calls; a NULL
+ * loc starts every call from interval 0) */
 typedef struct fxo_series {
   const double* values;
   int32_t count;
   double dt, factor, start;
   int32_t use_last;
+  const double* time;
+  int32_t* loc;
 } fxo_series;
 double fxo_series_value(const fxo_series* s, double fact, double t);
 typedef struct fxo_newmark_opts {

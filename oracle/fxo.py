@@ -87,14 +87,34 @@ class System(C.Structure):
 
 class Series(C.Structure):
     _fields_ = [("values", C.c_void_p), ("count", C.c_int32), ("dt", C.c_double), ("factor", C.c_double),
-                ("start", C.c_double), ("use_last", C.c_int32)]
+                ("start", C.c_double), ("use_last", C.c_int32), ("time", C.c_void_p), ("loc", C.c_void_p)]
 
 
 def series_value(values, dt, t, factor=1.0, start=0.0, use_last=False, fact=1.0):
     """PathSeries value at time t (uniform step), times the pattern factor."""
     sv = np.ascontiguousarray(values, np.float64)
-    s = Series(sv.ctypes.data, len(sv), dt, factor, start, int(bool(use_last)))
+    s = Series(sv.ctypes.data, len(sv), dt, factor, start, int(bool(use_last)), None, None)
     return lib().fxo_series_value(C.cast(C.pointer(s), C.c_void_p), fact, t)
+
+
+class TimeArraySeries:
+    """PathTimeSeries (values at given times) with its interval walk kept between calls, as
+    GroundMotions keeps it per pattern (ground_motion.cpp:43-74, :133-139)."""
+
+    def __init__(self, values, time, factor=1.0):
+        self.values = np.ascontiguousarray(values, np.float64)
+        self.time = np.ascontiguousarray(time, np.float64)
+        assert len(self.values) == len(self.time) >= 1
+        self.factor = factor
+        self.loc = C.c_int32(0)
+
+    def struct(self):
+        return Series(self.values.ctypes.data, len(self.values), 0.0, self.factor, 0.0, 0,
+                      self.time.ctypes.data, C.cast(C.pointer(self.loc), C.c_void_p))
+
+    def __call__(self, t, fact=1.0):
+        s = self.struct()
+        return lib().fxo_series_value(C.cast(C.pointer(s), C.c_void_p), fact, t)
 
 
 class NewmarkOpts(C.Structure):
@@ -408,10 +428,13 @@ class OracleMesh:
         s.nthreads = nthreads
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
         ser = None
-        if series is not None:
+        if isinstance(series, TimeArraySeries):
+            ser = series.struct()
+            keep += [series, ser]
+        elif series is not None:
             sv = np.ascontiguousarray(series["values"], np.float64)
             ser = Series(sv.ctypes.data, len(sv), series["dt"], series.get("factor", 1.0), series.get("start", 0.0),
-                         int(bool(series.get("use_last", False))))
+                         int(bool(series.get("use_last", False))), None, None)
             keep += [sv, ser]
         o = NewmarkOpts(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, lanes,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,

@@ -218,8 +218,45 @@ int fxo_gmres_system(fxo_system* s, const fxo_coeffs* k, const double* b, double
   return status;
 }
 
+/* PathTimeSeries::getFactor (ground_motion.cpp:43-74): the interval [time[l], time[l + 1]] is
+ * found by walking from the last call's l, forward or back; a time equal to the walk's lower
+ * end returns that sample exactly; outside the data, 0 */
+static double path_time(const fxo_series* s, int32_t* loc, double t) {
+  const double* v = s->values;
+  const double* tm = s->time;
+  const int32_t n = s->count;
+  int32_t l = *loc;
+  double t1 = tm[l];
+  if (t < t1 && l == 0) return 0.0;
+  if (t == t1) return s->factor * v[l];
+  if (t > t1 && l == n - 1) return 0.0;
+  double t2 = tm[l + 1];
+  if (t > t2) {
+    while (t > t2 && l < n - 2) {
+      ++l;
+      t1 = t2;
+      t2 = tm[l + 1];
+    }
+    *loc = l;
+    if (t > t2) return 0.0;
+  } else if (t < t1) {
+    while (t < t1 && l > 0) {
+      --l;
+      t2 = t1;
+      t1 = tm[l];
+    }
+    *loc = l;
+    if (t < t1) return 0.0;
+  }
+  return s->factor * (v[l] + (v[l + 1] - v[l]) * (t - t1) / (t2 - t1));
+}
+
 double fxo_series_value(const fxo_series* s, double fact, double t) {
   if (t < 0.0) return 0.0; /* GroundMotion::getAccel: 0 before time 0 */
+  if (s->time) {
+    int32_t l0 = 0;
+    return fact * path_time(s, s->loc ? s->loc : &l0, t);
+  }
   double a = 0.0;
   if (!(t < s->start)) {   /* PathSeries::getFactor */
     const double incr = (t - s->start) / s->dt;
@@ -243,6 +280,8 @@ typedef struct nm_state {
   int64_t n;
   int32_t step;
   double *u, *v, *a, *du, *ut, *vt, *at, *du0, *f, *tmp, *b, *mr, *scratch;
+  fxo_series series; /* the ground series with the run's interval walk (time-array series) */
+  int32_t loc;
 } nm_state;
 
 /* form_unbalance (:212-219) and form_residual (:55-68): B = -f(u, v) - M a - a_g M r */
@@ -277,7 +316,7 @@ static int nm_attempt(nm_state* w, double h) {
   memcpy(w->du0, w->du, (size_t)n * sizeof(double));
   double ag = 0.0;
   if (w->ground >= 0)
-    ag = o->ground_series ? fxo_series_value(o->ground_series, o->ground_factor, w->st->time)
+    ag = o->ground_series ? fxo_series_value(&w->series, o->ground_factor, w->st->time)
                           : (o->ground_accel ? o->ground_accel[w->step] : 0.0);
   const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
   nm_unbalance(w, ag);
@@ -361,6 +400,13 @@ int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, int ground_direction, 
   w.v = v;
   w.a = a;
   w.du = du;
+  /* a time-array series walks on from the caller's interval (GroundMotions keeps it for the
+   * pattern's life) or from interval 0, and the run keeps its walk from attempt to attempt */
+  if (o->ground_series) {
+    w.series = *o->ground_series;
+    w.loc = w.series.loc ? *w.series.loc : 0;
+    w.series.loc = &w.loc;
+  }
   double** bufs[] = {&w.ut, &w.vt, &w.at, &w.du0, &w.f, &w.tmp, &w.b, &w.scratch};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) *bufs[i] = (double*)malloc((size_t)(n + 1) * sizeof(double));
   w.mr = (double*)calloc((size_t)n + 1, sizeof(double));
@@ -376,5 +422,6 @@ int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, int ground_direction, 
   }
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) free(*bufs[i]);
   free(w.mr);
+  if (o->ground_series && o->ground_series->loc) *o->ground_series->loc = w.loc;
   return status;
 }

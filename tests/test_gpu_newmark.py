@@ -159,6 +159,60 @@ def test_pimy_steps_retried_and_substepped(ctx):
     assert history_error(batch.pimy_state(True), Cs) <= 1e-7
 
 
+def test_pimy_steps_under_a_time_array_series(ctx):
+    """A PathTimeSeries excitation (ground_motion.cpp:43-74; times off the step grid) on the
+    recovering PIMY column: a failed attempt restores the time, so the series' interval walk
+    goes back as well as forward. The same commits, retries, substeps and time as the oracle,
+    the state to the plastic-column tolerance, and the same final interval in loc."""
+    from tests.test_oracle_newmark import _pimy_column_case
+    model, om, per_set, pe, h, nsteps, _ = _pimy_column_case()
+    tm = np.concatenate([[0.0], np.cumsum(h * (0.6 + 0.5 * np.abs(np.sin(np.arange(1, 3 * nsteps)))))])
+    vals = 12.0 * np.sin(tm * 2 * np.pi * 2)
+    pm = core.Mesh(model)
+    keep = []
+    dev = [core.pimy_material(m["rho"], m["G"], m["K"], m["cohesion"], m["gamma_peak"], m["phi"], m["pref"], m["pdc"],
+                              nsurf=20, b=m["b"], mass_type=m["mass_type"], keep=keep) for m in per_set]
+    batch = core.Batch.pimy(ctx, pm, dev)
+    batch.set_stage(1)
+    n = pm.neq
+    kw = dict(test=0, tol=1e-5, max_iter=2, max_retries=1, num_sublevels=2, num_substeps=2, ground_factor=0.5)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    ser_o = fxo.TimeArraySeries(vals, tm)
+    rc, st = om.newmark(u, v, a, du, h, nsteps, pimy=pe, pimy_state=[Cs, Ts], ground_dir=ground_dir(om, 0),
+                        series=ser_o, **kw)
+    assert rc == 0 and st["substepped"] > 0 and st["failed_attempts"] > 0
+    d = [core.DeviceArray(ctx, n) for _ in range(4)]
+    for x in d:
+        x.zero()
+    loc = np.zeros(1, np.int32)
+    r = batch.newmark(*d, h, nsteps, ground_dir=0, series=dict(values=vals, time=tm, loc=loc), **kw)
+    assert r["failed"] == 0 and r["steps"] == nsteps, r
+    for key in ("commits", "retries", "substepped", "failed_attempts", "newton_iterations", "max_newton_iterations"):
+        assert r[key] == st[key], (key, r, st)
+    assert r["time"] == st["time"]
+    assert loc[0] == ser_o.loc.value > 0
+    for g, w, tol in zip((x.get() for x in d[:3]), (u, v, a), (1e-8, 1e-7, 1e-7)):
+        assert rel(g, w) <= tol
+    assert history_error(batch.pimy_state(True), Cs) <= 1e-7
+
+
+def test_time_array_series_is_validated(ctx):
+    model = meshes.column(4)
+    pm = core.Mesh(model)
+    batch = core.Batch(ctx, pm, pm.params_for(model), dynamic=True)
+    d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+    for x in d:
+        x.zero()
+    for bad in (dict(values=[0.0, 1.0], time=[0.0, np.nan]), dict(values=[0.0, 1.0, 2.0], time=[0.0, 0.2, 0.1]),
+                dict(values=[0.0, 1.0], time=[0.0, 0.1], loc=np.array([2], np.int32))):
+        with pytest.raises(core.FxError):
+            batch.newmark(*d, 0.01, 2, ground_dir=0, series=bad)
+    r = batch.newmark(*d, 0.01, 2, ground_dir=0, series=dict(values=[0.0, 1.0], time=[0.0, 0.1]))
+    assert r["steps"] == 2
+
+
 def test_substeps_under_excitation_need_a_series(ctx):
     model = meshes.column(4)
     pm = core.Mesh(model)

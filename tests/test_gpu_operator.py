@@ -125,15 +125,44 @@ def test_kernel_variants_and_tile_placement_give_the_same_bits(ctx):
     assert a.info()["variant"] in range(8)
 
 
-@pytest.mark.parametrize("tile8", [False, True], ids=["tile8x8x4", "tile8x8x8"])
-def test_lattice_fp32_diagonal_and_mass_paths(ctx, tile8):
-    """Lattice path at a size with many tiles (both tile shapes): FP32 apply, diagonal and the
-    mass-term kernel against the oracle."""
+@pytest.mark.parametrize("depth", [8, 16])
+def test_streamed_tile_variants_give_the_same_bits(ctx, depth):
+    """8-deep tiles: the held (0-7) and streamed (8-11) kernels give the same bits; 16-deep tiles
+    (always streamed) the same bits for every register budget. 42^3 cuts the last tiles in z
+    (42 = 5 x 8 + 2 = 2 x 16 + 10), so the later plane fetches see missing nodes."""
+    model = S.config_c2(n=42)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    a = core.Batch(ctx, pm, params, tile_depth=depth, tune=False)
+    x = meshes.random_vector(pm.neq, 79)
+    dx = core.DeviceArray.from_host(ctx, x)
+    ya = core.DeviceArray(ctx, pm.neq)
+    variants = range(12) if depth == 8 else range(4)
+    for kk in COEFFS:
+        outs = []
+        for v in variants:
+            a.tune(v)
+            a.apply(dx, ya, core.coeffs(*kk))
+            outs.append(ya.get())
+        for o in outs[1:]:
+            np.testing.assert_array_equal(outs[0], o)
+    a.tune(-1)
+    assert a.info()["variant"] in variants
+
+
+@pytest.mark.parametrize("shape", [(4, None), (8, None), (8, 8), (16, None)],
+                         ids=["tile8x8x4", "tile8x8x8", "tile8x8x8_streamed", "tile8x8x16"])
+def test_lattice_fp32_diagonal_and_mass_paths(ctx, shape):
+    """Lattice path at a size with many tiles (every tile shape, held and streamed): FP32 apply,
+    diagonal and the mass-term kernel against the oracle."""
+    depth, variant = shape
     model = S.config_c2(n=40)
     pm = core.Mesh(model)
     params = pm.params_for(model)
-    a = core.Batch(ctx, pm, params, fp32=True, tile8=tile8)
-    assert a.info()["lattice"] == 1 and a.info()["tiles"] > 64
+    a = core.Batch(ctx, pm, params, fp32=True, tile_depth=depth)
+    if variant is not None:
+        a.tune(variant)
+    assert a.info()["lattice"] == 1 and a.info()["tiles"] > 16
     om = fxo.OracleMesh(model)
     op = fxo.elastic_params(params)
     x = meshes.random_vector(pm.neq, 77)

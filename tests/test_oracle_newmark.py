@@ -104,6 +104,74 @@ def test_path_series_values():
     assert [fxo.series_value([0.0] + ag, 0.25, 0.25 * (k + 1)) for k in range(2)] == ag[:2]
 
 
+# the series of the reference's own unit test (tests/unit/test_ground_motion.cpp:40-55)
+REF_TIME_SERIES = dict(values=[0.0, 0.3, -0.2, 0.5, 0.1], time=[0.0, 0.01, 0.035, 0.05, 0.08], factor=9.81)
+
+
+def _interp(t, s, fact):
+    """Linear interpolation between the samples (numpy's), 0 outside the data."""
+    tm, v = np.array(s["time"]), np.array(s["values"])
+    return 0.0 if t < tm[0] or t > tm[-1] else fact * s["factor"] * float(np.interp(t, tm, v))
+
+
+def test_path_time_series_walk_matches_linear_interpolation():
+    """PathTimeSeries (ground_motion.cpp:43-74) on the reference test's series and pattern factor
+    (test_ground_motion.cpp:95-110): increasing times including the data points, then decreasing
+    times (the walk backwards), against numpy's interpolation, 0 before the first and after the
+    last time and before time 0; a time equal to the walk's lower end gives that sample exactly."""
+    tol = 1e-15 * 9.81 * 1.5
+    s = fxo.TimeArraySeries(REF_TIME_SERIES["values"], REF_TIME_SERIES["time"], REF_TIME_SERIES["factor"])
+    ts = sorted(list(np.arange(-0.01, 0.1, 0.0025)) + REF_TIME_SERIES["time"])
+    for t in ts:
+        assert abs(s(t, 1.5) - _interp(t, REF_TIME_SERIES, 1.5)) <= tol, t
+    assert s.loc.value == 3  # the walk stopped in the last interval
+    for t in reversed(ts):
+        if t >= 0.0:
+            assert abs(s(t, 1.5) - _interp(t, REF_TIME_SERIES, 1.5)) <= tol, t
+    assert s.loc.value == 0
+    # exact samples where the walk's lower end is hit
+    s.loc.value = 2
+    assert s(0.035, 1.0) == 9.81 * -0.2
+    assert s.loc.value == 2
+    s.loc.value = 0
+    assert s(0.0, 1.0) == 0.0 and s(0.01, 2.0) == 2.0 * 9.81 * 0.3
+    # outside the data, and a series that starts after time 0
+    assert s(0.09, 1.0) == 0.0 and s(-0.001, 1.0) == 0.0
+    late = fxo.TimeArraySeries([1.0, 2.0], [0.02, 0.04])
+    assert late(0.01) == 0.0 and late(0.02) == 1.0 and abs(late(0.03) - 1.5) <= 1e-15
+    one = fxo.TimeArraySeries([4.0], [0.5])
+    assert one(0.4) == 0.0 and one(0.5) == 4.0 and one(0.6) == 0.0
+
+
+def test_newmark_with_a_time_array_series_uses_it_at_each_step_end():
+    """The elastic Newmark run with a PathTimeSeries excitation (times not on the step grid)
+    equals the run with the accelerations that series gives at the accumulated step-end times
+    (the same walk, evaluated ahead), bit for bit; the run leaves its walk in loc."""
+    model = meshes.column(6)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    h, nsteps = 0.004, 30
+    tm = np.concatenate([[0.0], np.cumsum(0.003 + 0.004 * np.abs(np.sin(np.arange(1, 60))))])
+    vals = 2.0 * np.sin(tm * 2 * np.pi * 5)
+    ahead = fxo.TimeArraySeries(vals, tm, 1.0)
+    t, ag = 0.0, []
+    for _ in range(nsteps):
+        t += h
+        ag.append(ahead(t, 0.7))
+    states = []
+    for kw in (dict(series=fxo.TimeArraySeries(vals, tm, 1.0), ground_factor=0.7), dict(ground_accel=np.array(ag))):
+        u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        rc, st = om.newmark(u, v, a, du, h, nsteps, elastic=op, ground_dir=0, tol=1e-10, test=1, **kw)
+        assert rc == 0 and st["steps"] == nsteps
+        states.append((u, v, a))
+        if "series" in kw:
+            assert kw["series"].loc.value == ahead.loc.value > 0
+    for x, y in zip(*states):
+        np.testing.assert_array_equal(x, y)
+    assert np.max(np.abs(states[0][0])) > 0
+
+
 def test_energy_incr_is_half_the_last_increments_work():
     """EnergyIncr (implicit_stage.cpp:239-246): 0.5 |dU . B| with B the unbalance after the
     update; B recomputed from the oracle's force, mass and ground load at the final state."""
