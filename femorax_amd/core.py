@@ -88,7 +88,8 @@ class NewmarkDesc(C.Structure):
                 ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
                 ("cg_max_it", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p),
                 ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32),
-                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double)]
+                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double),
+                ("mixed", C.c_int32), ("inner_rtol", C.c_double)]
 
 
 class NewmarkResult(C.Structure):
@@ -486,7 +487,7 @@ class Batch:
         ctx.adopt(self)
 
     @classmethod
-    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None):
+    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None, fp32=False):
         """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
         fx_pimy_material per compiled parameter set; compact: the compacted return map;
         rayleigh: per parameter set (alphaM, betaK, betaKinit, betaKcomm), or None."""
@@ -511,18 +512,19 @@ class Batch:
             self._damp = (Rayleigh * len(materials))(*[Rayleigh(*r) for r in rayleigh])
             d.damp = C.cast(self._damp, C.c_void_p)
         h = C.c_void_p()
-        _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), FX_BATCH_COMPACT if compact else 0, C.byref(h)), ctx)
+        flags = (FX_BATCH_COMPACT if compact else 0) | (FX_BATCH_FP32 if fp32 else 0)
+        _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)
         return self
 
     def newmark(self, u, v, a, du, dt, nsteps, gamma=0.5, beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12,
                 cg_max_it=10000, ground_dir=-1, ground_accel=None, max_retries=0, num_sublevels=0, num_substeps=2,
-                series=None, ground_factor=1.0, time=0.0):
+                series=None, ground_factor=1.0, time=0.0, mixed=False, inner_rtol=1e-3):
         """nsteps Newmark + Newton steps on device arrays u, v, a, du (in place); returns the
         fx_newmark_result as a dict. test: 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr.
-        series: the excitation as a uniform path series, dict(values, dt, factor=1, start=0,
-        use_last=False), evaluated at every attempt's end time (needed to substep)."""
+        series: the excitation's series (below), evaluated at every attempt's end time (needed to
+        substep). mixed: the Newton systems by mixed-precision CG (FP32 batches)."""
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
         sv = ts = tv = None
         if series is not None:
@@ -538,7 +540,8 @@ class Batch:
                             None if tv is None else tv.ctypes.data, None if loc is None else loc.ctypes.data)
         d = NewmarkDesc(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, ground_dir,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
-                        None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time)
+                        None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time,
+                        int(bool(mixed)), inner_rtol)
         r = NewmarkResult()
         _check(lib().fx_newmark(self.h, C.byref(d), u.ptr, v.ptr, a.ptr, du.ptr, C.byref(r)), self.ctx)
         return {f: getattr(r, f) for f, _ in NewmarkResult._fields_}

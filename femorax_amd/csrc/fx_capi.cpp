@@ -210,7 +210,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
-                  b->state_c, b->state_t, b->tangent, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
+                  b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -866,6 +866,10 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
       ok(dev_zeros(ctx, b->state_c, np * fx::kPimyNState, by)) && ok(dev_zeros(ctx, b->state_t, np * fx::kPimyNState, by)) &&
       ok(dev_zeros(ctx, b->pimy_touched, np, by)) &&
       ok(dev_zeros(ctx, b->tangent, np * 21, by)) &&
+      // FX_BATCH_FP32: the FP32 operator (an FP32 copy of the tangents) and the mixed CG's FP32 vectors
+      ok((flags & FX_BATCH_FP32) ? dev_zeros(ctx, b->tangent32, np * 21, by) : FX_OK) &&
+      ok((flags & FX_BATCH_FP32) ? dev_zeros(ctx, b->work32, static_cast<size_t>(9) * std::max(1, d->neq), by) : FX_OK) &&
+      ok((b->has_f32 = (flags & FX_BATCH_FP32) != 0, FX_OK)) &&
       ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
       ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&
       ok(list.empty() ? FX_OK : dev_upload(ctx, b->gather_list, list.data(), list.size(), by)) &&
@@ -933,6 +937,8 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
     return fail(b->ctx, FX_EINVAL, "fx_newmark: max_retries and num_sublevels must be non-negative, num_substeps positive");
   if (d->ground_dir >= 0 && d->num_sublevels > 0 && !d->ground_series)
     return fail(b->ctx, FX_EINVAL, "fx_newmark: substeps under an excitation need its time series (ground_series)");
+  if (d->mixed && (!b->has_f32 || !(d->inner_rtol > 0 && d->inner_rtol < 1)))
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: mixed needs a batch created with FX_BATCH_FP32 and inner_rtol in (0, 1)");
   if (const fx_time_series* gs = d->ground_series) {
     if (!gs->values || gs->count < 1 || (!gs->time && !(gs->dt > 0)))
       return fail(b->ctx, FX_EINVAL, "fx_newmark: bad ground_series");

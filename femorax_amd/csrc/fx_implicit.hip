@@ -202,7 +202,10 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
       if (b->pimy)  // update_tangent: the tangents at u
         if (int rc = pimy_force64(b, U, nullptr, SCR, true)) return rc;
       fx_solve_result r;
-      if (int rc = b->gm ? krylov64_solver(b, k, so, B, dU, r) : cg64_solver(b, k, so, B, dU, r)) return rc;
+      if (int rc = d.mixed  ? cg_mixed_solver(b, k, so, d.inner_rtol, B, dU, r)
+                   : b->gm ? krylov64_solver(b, k, so, B, dU, r)
+                           : cg64_solver(b, k, so, B, dU, r))
+        return rc;
       out.krylov_iterations += r.iterations;
       if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
         ok = false;

@@ -251,6 +251,7 @@ struct fx_batch {
   double* state_c = nullptr;          // committed history [kPimyNState * npts]
   double* state_t = nullptr;          // trial history
   double* tangent = nullptr;          // [21 * npts] upper triangles
+  float* tangent32 = nullptr;         // their FP32 copy (FX_BATCH_FP32 PIMY batches)
   bool tangents_stored = false;
   double* slots = nullptr;            // [24 * padded]
   int32_t* gather_start = nullptr;    // [neq + 1]

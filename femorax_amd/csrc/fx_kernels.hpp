@@ -77,11 +77,15 @@ int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, cons
                 fx_solve_result& r);
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
              double* x, fx_solve_result& r);
+// the same on internal-numbering vectors (bi, xi outside the batch's work arrays)
+int cg_mixed_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* bi,
+                    double* xi, fx_solve_result& r);
 
 // stdBrick<PressureIndependMultiYield> batches (fx_pimy.hip).
 int pimy_batch_init(fx_batch* b);
 int pimy_force64(fx_batch* b, const double* u, const double* v, double* f, bool store_tangents);
 int pimy_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
+int pimy_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done = nullptr);
 int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 
 // Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.

@@ -306,10 +306,7 @@ int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y
   return rc ? rc : zero_orphans(b, y, done);
 }
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done) {
-  if (b->pimy) {
-    b->ctx->error = "operator: no FP32 operator for PressureIndependMultiYield batches";
-    return FX_EUNSUP;
-  }
+  if (b->pimy) return pimy_apply32(b, k, x, y, done);
   const int rc = b->lattice ? launch_box<float, false>(b, k, x, y, 0, done)
                             : launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
   return rc ? rc : zero_orphans(b, y, done);

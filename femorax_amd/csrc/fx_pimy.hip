@@ -552,30 +552,32 @@ __global__ void pimy_clear_count_kernel(int32_t* count) {
 // long_gather_kernel.
 constexpr int kLongGather = 64;
 
+template <typename T>
 __global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
-                                   const double* __restrict__ slots, double* __restrict__ y, const int32_t* done) {
+                                   const T* __restrict__ slots, T* __restrict__ y, const int32_t* done) {
   if (done && *done) return;
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= neq) return;
   const int32_t j0 = start[q], j1 = start[q + 1];
   if (j1 - j0 > kLongGather) return;
-  double s = 0.0;
+  T s = T(0);
   for (int32_t j = j0; j < j1; ++j) s += slots[list[j]];
   y[q] = s;
 }
 
 // The long equations: one wave each, lane l summing slots l, l + 64, ... in order, then a fixed
 // butterfly over the lanes.
+template <typename T>
 __global__ __launch_bounds__(256) void long_gather_kernel(int32_t nlong, const int32_t* __restrict__ eqs,
                                                           const int32_t* __restrict__ start,
                                                           const int32_t* __restrict__ list,
-                                                          const double* __restrict__ slots, double* __restrict__ y,
+                                                          const T* __restrict__ slots, T* __restrict__ y,
                                                           const int32_t* done) {
   const int u = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x) / 64, lane = threadIdx.x % 64;
   if (u >= nlong) return;  // the whole wave
   if (done && *done) return;
   const int32_t q = eqs[u];
-  double s = 0.0;
+  T s = T(0);
   for (int32_t j = start[q] + lane; j < start[q + 1]; j += 64) s += slots[list[j]];
   for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
   if (lane == 0) y[q] = s;
@@ -620,6 +622,104 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const
   return a;
 }
 
+// y[q] = the sum of equation q's element slots (slot_gather_kernel, long_gather_kernel)
+template <typename T>
+int gather(fx_batch* b, const T* slots, T* y, const int32_t* done) {
+  if (b->neq > 0)
+    hipLaunchKernelGGL(slot_gather_kernel<T>, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
+                       b->gather_start, b->gather_list, slots, y, done);
+  if (b->nlong > 0)
+    hipLaunchKernelGGL(long_gather_kernel<T>, dim3((b->nlong + 3) / 4), dim3(256), 0, b->ctx->stream, b->nlong,
+                       b->long_eqs, b->gather_start, b->gather_list, slots, y, done);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+// The FP32 copy of the stored tangents that the FP32 operator reads (half the bytes).
+__global__ void tangent_to_f32_kernel(int64_t n, const double* __restrict__ t, float* __restrict__ t32) {
+  const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) t32[i] = static_cast<float>(t[i]);
+}
+
+// The FP32 operator on the stored tangents, for the inner solves of mixed-precision CG: one
+// thread per element as pimy_operator_kernel, its terms in the same order, with x, the
+// tangents, strains, stresses and sums in FP32. The geometry (Jacobian, shape derivatives,
+// dvol), the operator factors and the committed tangent are formed in FP64 and rounded once.
+template <bool COMM>
+__global__ __launch_bounds__(128) void pimy_operator32_kernel(PimyBatchArgs a, const float* __restrict__ x,
+                                                              const float* __restrict__ tangent, float* slots) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.count) return;
+  double X[8], Y[8], Z[8];
+  int eq[24];
+  element_geometry(a, e, X, Y, Z, eq);
+  const PimyParams& p = a.params[a.param[e]];
+  float v[24], out[24];
+  for (int k = 0; k < 24; ++k) {
+    v[k] = eq[k] >= 0 ? x[eq[k]] : 0.0f;
+    out[k] = 0.0f;
+  }
+  double cK, c0, cc, cM;
+  operator_factors(p, a.k, cK, c0, cc, cM);
+  const float fK = static_cast<float>(cK), fM = static_cast<float>(cM);
+  float D0[36];
+  for (int r = 0; r < 6; ++r)
+    for (int c = 0; c < 6; ++c) D0[6 * r + c] = static_cast<float>(c0 * initial_entry(p, r, c));
+#pragma unroll 1
+  for (int gp = 0; gp < 8; ++gp) {
+    Shape<double> s;
+    gauss_shape(X, Y, Z, gp, s);
+    float dx[8], dy[8], dz[8];
+    for (int j = 0; j < 8; ++j) {
+      dx[j] = static_cast<float>(s.dx[j]);
+      dy[j] = static_cast<float>(s.dy[j]);
+      dz[j] = static_cast<float>(s.dz[j]);
+    }
+    const float dvol = static_cast<float>(s.dvol);
+    float eps[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < 8; ++j) {
+      const float u0 = v[3 * j], u1 = v[3 * j + 1], u2 = v[3 * j + 2];
+      eps[0] += dx[j] * u0;
+      eps[1] += dy[j] * u1;
+      eps[2] += dz[j] * u2;
+      eps[3] += dy[j] * u0 + dx[j] * u1;
+      eps[4] += dz[j] * u1 + dy[j] * u2;
+      eps[5] += dz[j] * u0 + dx[j] * u2;
+    }
+    double Dc[36];
+    if constexpr (COMM)
+      if (cc != 0.0) pimy_committed_tangent(p, a.C + static_cast<int64_t>(e) * 8 + gp, a.npts, Dc);
+    float sig[6];
+    for (int r = 0; r < 6; ++r) {
+      float sum = 0.0f;
+      for (int c = 0; c < 6; ++c) {
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        const int w = lo * 6 - lo * (lo - 1) / 2 + (hi - lo);  // packed upper triangle, row-major
+        float d = fK * tangent[tangent_at(a, e, gp, w)] + D0[6 * r + c];
+        if constexpr (COMM)
+          if (cc != 0.0) d += static_cast<float>(cc * Dc[6 * r + c]);
+        sum += d * eps[c];
+      }
+      sig[r] = sum * dvol;
+    }
+    float g[3] = {0, 0, 0};
+    if (cM != 0.0 && p.mass_type == 0)
+      for (int kk = 0; kk < 8; ++kk)
+        for (int q = 0; q < 3; ++q) g[q] += static_cast<float>(s.N[kk]) * v[3 * kk + q];
+    for (int j = 0; j < 8; ++j) {
+      out[3 * j] += dx[j] * sig[0] + dy[j] * sig[3] + dz[j] * sig[5];
+      out[3 * j + 1] += dy[j] * sig[1] + dx[j] * sig[3] + dz[j] * sig[4];
+      out[3 * j + 2] += dz[j] * sig[2] + dy[j] * sig[4] + dx[j] * sig[5];
+      if (cM != 0.0) {
+        const float temp = static_cast<float>(s.N[j] * s.dvol * p.rho);
+        for (int q = 0; q < 3; ++q) out[3 * j + q] += fM * (temp * (p.mass_type == 0 ? g[q] : v[3 * j + q]));
+      }
+    }
+  }
+  const size_t P = static_cast<size_t>(a.padded);
+  for (int k = 0; k < 24; ++k) slots[k * P + e] = out[k];
+}
+
 // MODE 0: the force (STORE: also the tangents); 1: the operator on the stored tangents; 2: its
 // diagonal. The elements' 24 values go to their slots, then each equation gathers its slots.
 template <int MODE, bool STORE>
@@ -650,14 +750,7 @@ int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done
     else
       hipLaunchKernelGGL((pimy_operator_kernel<MODE == 2, false>), g, dim3(128), 0, st, a);
   }
-  if (b->neq > 0)
-    hipLaunchKernelGGL(slot_gather_kernel, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
-                       b->gather_start, b->gather_list, b->slots, y, done);
-  if (b->nlong > 0)
-    hipLaunchKernelGGL(long_gather_kernel, dim3((b->nlong + 3) / 4), dim3(256), 0, b->ctx->stream, b->nlong,
-                       b->long_eqs, b->gather_start, b->gather_list, b->slots, y, done);
-  FX_HIP(b->ctx, hipGetLastError());
-  return FX_OK;
+  return gather<double>(b, b->slots, y, done);
 }
 
 }  // namespace
@@ -677,6 +770,12 @@ int pimy_force64(fx_batch* b, const double* u, const double* v, double* f, bool 
   if (store) {
     if (int rc = run_pimy<0, true>(b, a, f, nullptr)) return rc;
     b->tangents_stored = true;
+    if (b->tangent32 && b->npts > 0) {
+      const int64_t nt = b->npts * 21;
+      hipLaunchKernelGGL(tangent_to_f32_kernel, dim3(static_cast<unsigned>((nt + 255) / 256)), dim3(256), 0,
+                         b->ctx->stream, nt, b->tangent, b->tangent32);
+      FX_HIP(b->ctx, hipGetLastError());
+    }
     return FX_OK;
   }
   return run_pimy<0, false>(b, a, f, nullptr);
@@ -688,6 +787,29 @@ int pimy_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, co
     return FX_EINVAL;
   }
   return run_pimy<1, false>(b, batch_args(b, x, k), y, done);
+}
+
+int pimy_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done) {
+  if (!b->tangent32) {
+    b->ctx->error = "operator: no FP32 operator (PressureIndependMultiYield batch created without FX_BATCH_FP32)";
+    return FX_EUNSUP;
+  }
+  if (!b->tangents_stored && !k.initial && k.stiffness != 0.0) {
+    b->ctx->error = "operator: no stored tangents (call fx_store_tangents first)";
+    return FX_EINVAL;
+  }
+  const PimyBatchArgs a = batch_args(b, nullptr, k);
+  float* slots = reinterpret_cast<float*>(b->slots);  // [24 * padded] floats in the FP64 slots' space
+  if (b->count > 0) {
+    bool comm = false;
+    for (const PimyParams& q : b->host_pimy) comm |= k.damping * q.betaKcomm != 0.0;
+    const dim3 g((b->count + 127) / 128);
+    if (comm)
+      hipLaunchKernelGGL(pimy_operator32_kernel<true>, g, dim3(128), 0, b->ctx->stream, a, x, b->tangent32, slots);
+    else
+      hipLaunchKernelGGL(pimy_operator32_kernel<false>, g, dim3(128), 0, b->ctx->stream, a, x, b->tangent32, slots);
+  }
+  return gather<float>(b, slots, y, done);
 }
 
 int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {

@@ -611,17 +611,25 @@ int krylov64(fx_batch* b, int method, const fx_coeffs& k, const fx_solve_options
 
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,
              double* x, fx_solve_result& res) {
+  const int64_t n = b->neq;
+  double *bi = b->work + 6 * n, *xi = b->work + 7 * n;
+  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
+  if (int rc = to_internal<double>(b, x, xi)) return rc;
+  if (int rc = cg_mixed_solver(b, k, o, inner_rtol, bi, xi, res)) return rc;
+  return to_external<double>(b, xi, x);
+}
+
+int cg_mixed_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* bi,
+                    double* xi, fx_solve_result& res) {
   fx_ctx* ctx = b->ctx;
   if (!b->has_f32) {
     ctx->error = "fx_cg_mixed: the batch was created without FX_BATCH_FP32";
     return FX_EUNSUP;
   }
   const int64_t n = b->neq;
-  double *r64 = b->work, *ap64 = r64 + 3 * n, *d64 = r64 + 5 * n, *bi = r64 + 6 * n, *xi = r64 + 7 * n;
+  double *r64 = b->work, *ap64 = r64 + 3 * n, *d64 = r64 + 5 * n;
   float *inv = b->work32, *r = inv + n, *z = r + n, *p = z + n, *ap = p + n, *dx = ap + n, *rhs32 = dx + n;
   if (int rc = jacobi_inverse(b, k, d64, inv, true)) return rc;
-  if (int rc = to_internal<double>(b, rhs, bi)) return rc;
-  if (int rc = to_internal<double>(b, x, xi)) return rc;
   CgState* st = static_cast<CgState*>(ctx->scalars);
   hipStream_t s = ctx->stream;
   double bb = 0;
@@ -661,7 +669,7 @@ int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double 
     if (h.it == 0) break;  // no progress possible
   }
   res.iterations = total_inner;
-  return to_external<double>(b, xi, x);
+  return FX_OK;
 }
 
 }  // namespace fx

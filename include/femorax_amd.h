@@ -142,7 +142,8 @@ typedef struct fx_batch_desc {
 } fx_batch_desc;
 
 /* Flags for fx_batch_create. */
-#define FX_BATCH_FP32 1u    /* also build the FP32 operator (mixed precision solves) */
+#define FX_BATCH_FP32 1u    /* also build the FP32 operator (mixed precision solves); on
+                               fx_pimy_batch_create, an FP32 copy of the stored tangents for it */
 #define FX_BATCH_GENERAL 2u  /* do not use the structured-lattice fast path (testing) */
 #define FX_BATCH_DYNAMIC 32u  /* allocate the time-integration workspace (12 * neq doubles) that
                                  fx_newmark needs (PIMY batches always have it) */
@@ -364,6 +365,11 @@ typedef struct fx_newmark_desc {
                                  under an excitation */
   double ground_factor;       /* UniformExcitation factor applied to ground_series (OpenSees -fact) */
   double time;                /* the stage's start time (the domain time before the first step) */
+  int32_t mixed;              /* 1: each Newton system by mixed-precision CG (FP32 operator and inner
+                                 CG to inner_rtol, FP64 residuals and refinement to cg_rtol; batches
+                                 created with FX_BATCH_FP32), in place of FP64 CG (and its GMRES
+                                 fallback) */
+  double inner_rtol;
 } fx_newmark_desc;
 typedef struct fx_newmark_result {
   int32_t steps;              /* converged steps of size dt (each may have been substepped) */
