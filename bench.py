@@ -142,7 +142,7 @@ def bench_refcol(core, S, ctx, side, reps):
         mats.append(core.pimy_material(p["rho"], G, K, 30.0, 0.1, 0.0, 100.0, 0.0, nsurf=20, b=p["b"], mass_type=0,
                                        keep=keep))
         rays.append(p["rayleigh"])
-    pb = core.Batch.pimy(ctx, mesh, [mats[g] for g in range(len(mats))], rayleigh=rays)
+    pb = core.Batch.pimy(ctx, mesh, [mats[g] for g in range(len(mats))], rayleigh=rays, fp32=True)
     pb.set_stage(1)
     z = np.linspace(0.0, 1.0, mesh.neq)
     dup = core.DeviceArray.from_host(ctx, 2e-4 * np.sin(40.0 * z))
@@ -151,6 +151,11 @@ def bench_refcol(core, S, ctx, side, reps):
     pb.time_apply(x, y, 3, 64, k=k)
     ms_t = pb.time_apply(x, y, reps, 64, k=k) / reps
     res["pimy_stored_tangent_apply_us"] = round(ms_t * 1e3, 2)
+    # the same operator in FP32 (FP32 tangent copy; the inner solves of mixed-precision CG)
+    x32 = core.DeviceArray.from_host(ctx, np.sin(0.001 * np.arange(mesh.neq)).astype(np.float32))
+    y32 = core.DeviceArray(ctx, mesh.neq, np.float32)
+    pb.time_apply(x32, y32, 3, 32, k=k)
+    res["pimy_stored_tangent_apply_fp32_us"] = round(pb.time_apply(x32, y32, reps, 32, k=k) / reps * 1e3, 2)
     for name, amp in (("elastic", 1e-7), ("yielding", 2e-3)):
         dup.upload(amp * np.sin(40.0 * z))
         pb.force(dup, dfp)
