@@ -126,7 +126,8 @@ class ColumnPoolDesc(C.Structure):
     _fields_ = [("max_elements", C.c_int64), ("max_columns", C.c_int32), ("max_materials", C.c_int32),
                 ("max_record", C.c_int64), ("gamma", C.c_double), ("beta", C.c_double), ("dt", C.c_double),
                 ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
-                ("cg_max_it", C.c_int32), ("flags", C.c_uint32), ("inner_rtol", C.c_double)]
+                ("cg_max_it", C.c_int32), ("flags", C.c_uint32), ("inner_rtol", C.c_double),
+                ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32)]
 
 
 class ColumnDesc(C.Structure):
@@ -137,7 +138,9 @@ class ColumnDesc(C.Structure):
 
 class ColumnResult(C.Structure):
     _fields_ = [("status", C.c_int32), ("steps", C.c_int32), ("newton_iterations", C.c_int32),
-                ("max_newton_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("last_test", C.c_double)]
+                ("max_newton_iterations", C.c_int32), ("krylov_iterations", C.c_int32), ("last_test", C.c_double),
+                ("commits", C.c_int32), ("failed_attempts", C.c_int32), ("retries", C.c_int32),
+                ("substepped", C.c_int32)]
 
 
 def pimy_material(rho, G, K, cohesion, gamma_peak, phi, pref, pdc, nsurf=20, gredu=None, b=(0.0, 0.0, 0.0),
@@ -662,13 +665,14 @@ class ColumnPool:
 
     def __init__(self, ctx, max_elements, max_columns, dt, max_materials=None, max_record=None, gamma=0.5,
                  beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, graph=True, mixed=False,
-                 inner_rtol=1e-5, compact=False, waves=0, one_stream=False):
+                 inner_rtol=1e-5, compact=False, waves=0, one_stream=False, max_retries=0, num_sublevels=0,
+                 num_substeps=2):
         self.ctx = ctx
         flags = ((0 if graph else FX_COLUMN_NO_GRAPH) | (FX_COLUMN_MIXED if mixed else 0) |
                  (FX_COLUMN_COMPACT if compact else 0) | (int(waves) << 8) | (8 if one_stream else 0))
         d = ColumnPoolDesc(int(max_elements), int(max_columns), int(max_materials or 16 * max_columns),
                            int(max_record or 1 << 20), gamma, beta, dt, test, tol, max_iter, cg_rtol, cg_max_it, flags,
-                           inner_rtol)
+                           inner_rtol, max_retries, num_sublevels, num_substeps)
         h = C.c_void_p()
         _check(lib().fx_column_pool_create(ctx.h, C.byref(d), C.byref(h)), ctx)
         self.h = h

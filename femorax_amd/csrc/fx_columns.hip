@@ -54,6 +54,7 @@ namespace fx {
 namespace {
 
 enum : int32_t { kIdle = 0, kRunning = 1, kFinished = 2, kFailed = 3 };
+constexpr int kMaxSublevels = 6;  // recovery depth of a failed step (num_sublevels)
 
 struct ColumnDev {
   int32_t status, step, nsteps, iter;
@@ -65,6 +66,12 @@ struct ColumnDev {
   double hx, hy, hz;
   double dnorm;      // ||dU|| of the last solve
   double last_test;  // the last test value
+  // recovery of a failed step (implicit_stage.cpp:385-402): the attempt's level (its size is
+  // dt / num_substeps^level), the retries it has used, the substep reached at each level, and
+  // the counters of fx_newmark's result
+  int32_t level, tries;
+  int32_t commits, failed_attempts, retries, substepped;
+  int32_t sub[kMaxSublevels + 1];
 };
 
 struct PoolArgs {
@@ -72,8 +79,10 @@ struct PoolArgs {
   int32_t max_columns;
   int32_t test, max_iter, cg_max_it;
   double tol, cg_rtol, inner_rtol;
-  double a1, a2, a3, a4;        // predictor constants
-  double c2, c3;                // update constants gamma / (beta h), 1 / (beta h^2)
+  // per recovery level (size h = dt / num_substeps^level): predictor constants a1..a4, update
+  // constants c2 = gamma / (beta h), c3 = 1 / (beta h^2)
+  double nk[kMaxSublevels + 1][6];
+  int32_t max_retries, num_sublevels, num_substeps, pad;
   ColumnDev* cols;
   const int32_t* elem_col;      // [cap] column slot or -1
   const int32_t* elem_param;    // [cap] parameter set
@@ -88,6 +97,7 @@ struct PoolArgs {
   double* vt;
   double* at;
   double* du;
+  double* du0;                  // du at the attempt's start (the solver's initial guess, restored)
   const double* record;         // record arena
   double* rec_out;              // same arena, written (histories)
   // compacted return map (FX_COLUMN_COMPACT): the plastic points' strains and list
@@ -355,7 +365,58 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   double am[3], ap[3];
   neighbours(av, am, ap);
   const int step = col.step;
-  const double ag = a.record[col.rec + step];
+  const int level = col.level;
+  const double* K0 = a.nk[level];  // a1 a2 a3 a4 c2 c3 of this attempt's size
+  double ag = a.record[col.rec + step];
+  if (level > 0) {
+    // a substep: the record read as a path series with a leading zero (PathSeries,
+    // ground_motion.cpp:17-33), linear within the step from the previous step's value to this
+    // step's, at the attempt's end (fraction pos / num_substeps^level of the step)
+    int64_t pos = 1, scale = 1;
+    for (int l = level; l >= 1; --l) {
+      pos += col.sub[l] * scale;
+      scale *= a.num_substeps;
+    }
+    const double prev = step > 0 ? a.record[col.rec + step - 1] : 0.0;
+    ag = prev + (ag - prev) * (static_cast<double>(pos) / static_cast<double>(scale));
+  }
+  // a failed attempt (implicit_stage.cpp:385-402): its start restored with the solver's initial
+  // guess; then a retry at the same size, else substeps one level down, else the column fails.
+  // Every thread derives the same choice from the state read above; thread 0 stores it.
+  auto fail_attempt = [&](int newton_its, int krylov_its) {
+    const int tries = col.tries;
+    const int outcome = tries < a.max_retries ? 0 : level < a.num_sublevels ? 1 : 2;
+    const int ln = outcome == 1 ? level + 1 : level;
+    const double* K1 = a.nk[ln];
+    if (live)
+      for (int d = 0; d < 3; ++d) {
+        const int64_t q = 3 * e + d;
+        const double u0 = a.ut[q], v0 = a.vt[q], a0 = a.at[q];
+        a.u[q] = u0;
+        a.v[q] = outcome == 2 ? v0 : v0 * K1[0] + a0 * K1[1];
+        a.a[q] = outcome == 2 ? a0 : a0 * K1[3] + v0 * K1[2];
+        a.du[q] = a.du0[q];
+      }
+    __syncthreads();
+    if (t == 0) {
+      col.krylov += krylov_its;
+      col.newton += newton_its;
+      col.max_newton = max(col.max_newton, newton_its);
+      ++col.failed_attempts;
+      col.iter = 0;
+      if (outcome == 0) {
+        ++col.tries;
+        ++col.retries;
+      } else if (outcome == 1) {
+        ++col.substepped;
+        col.level = ln;
+        col.sub[ln] = 0;
+        col.tries = 0;
+      } else {
+        col.status = kFailed;
+      }
+    }
+  };
   if (live)
     for (int d = 0; d < 3; ++d) {
       double f = a.eout[d * P + e];
@@ -377,9 +438,17 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
   if (iter > 0) {
     const double value = a.test == 0 ? bnorm : col.dnorm;
     bool done = value <= a.tol;
-    if (done) {  // commit, record, next step's predictor
-      const int next = step + 1;
-      if (live && t == N - 1 && col.hist >= 0) {
+    if (done) {  // commit; the next attempt's predictor; the record once the step is whole
+      // a converged substep moves on within its level or, as the last of its level, completes the
+      // attempt one level up (the recursion of recover, implicit_stage.cpp:385-402)
+      int sub[kMaxSublevels + 1];
+      for (int l = 0; l <= kMaxSublevels; ++l) sub[l] = col.sub[l];
+      int L = level;
+      while (L > 0 && ++sub[L] >= a.num_substeps) --L;
+      const bool whole = L == 0;
+      const int next = whole ? step + 1 : step;
+      const double* K1 = a.nk[L];
+      if (whole && live && t == N - 1 && col.hist >= 0) {
         double* h = a.rec_out + col.hist + 9 * static_cast<int64_t>(step);
         for (int d = 0; d < 3; ++d) {
           h[d] = a.u[3 * e + d];
@@ -387,43 +456,36 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
           h[6 + d] = a.a[3 * e + d];
         }
       }
-      if (live && next < col.nsteps)
+      if (live && !(whole && next >= col.nsteps))
         for (int d = 0; d < 3; ++d) {
           const int64_t q = 3 * e + d;
           const double u0 = a.u[q], v0 = a.v[q], a0 = a.a[q];
           a.ut[q] = u0;
           a.vt[q] = v0;
           a.at[q] = a0;
-          a.v[q] = v0 * a.a1 + a0 * a.a2;
-          a.a[q] = a0 * a.a4 + v0 * a.a3;
+          a.du0[q] = a.du[q];
+          a.v[q] = v0 * K1[0] + a0 * K1[1];
+          a.a[q] = a0 * K1[3] + v0 * K1[2];
         }
       __syncthreads();
       if (t == 0) {
         col.last_test = value;
         col.newton += iter;
         col.max_newton = max(col.max_newton, iter);
+        ++col.commits;
         col.cur ^= 1;
         col.step = next;
         col.iter = 0;
-        if (next >= col.nsteps) col.status = kFinished;
+        col.level = L;
+        col.tries = 0;
+        for (int l = 1; l <= kMaxSublevels; ++l) col.sub[l] = sub[l];
+        if (whole && next >= col.nsteps) col.status = kFinished;
       }
       return;
     }
-    if (iter >= a.max_iter) {  // restore the step's start
-      if (live)
-        for (int d = 0; d < 3; ++d) {
-          const int64_t q = 3 * e + d;
-          a.u[q] = a.ut[q];
-          a.v[q] = a.vt[q];
-          a.a[q] = a.at[q];
-        }
-      __syncthreads();
-      if (t == 0) {
-        col.last_test = value;
-        col.newton += iter;
-        col.max_newton = max(col.max_newton, iter);
-        col.status = kFailed;
-      }
+    if (iter >= a.max_iter) {  // the attempt failed
+      if (t == 0) col.last_test = value;
+      fail_attempt(iter, 0);
       return;
     }
   }
@@ -441,9 +503,9 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     }
     constexpr int diag[3] = {0, 3, 5};
     for (int d = 0; d < 3; ++d) {
-      Kd.m[diag[d]] += a.c3 * m;
-      Klo.m[diag[d]] += a.c3 * mlo;
-      Khi.m[diag[d]] += a.c3 * mhi;
+      Kd.m[diag[d]] += K0[5] * m;
+      Klo.m[diag[d]] += K0[5] * mlo;
+      Khi.m[diag[d]] += K0[5] * mhi;
     }
     for (int d = 0; d < 3; ++d) inv[d] = 1.0 / Kd.m[diag[d]];
   }
@@ -561,21 +623,8 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     }
   }
   const bool accept = converged || rnorm <= 1e-6 * bnorm;
-  if (!accept) {  // the solve failed: restore the step's start (implicit_stage.cpp:226-238)
-    if (live)
-      for (int d = 0; d < 3; ++d) {
-        const int64_t q = 3 * e + d;
-        a.u[q] = a.ut[q];
-        a.v[q] = a.vt[q];
-        a.a[q] = a.at[q];
-      }
-    __syncthreads();
-    if (t == 0) {
-      col.krylov += its;
-      col.newton += iter + 1;
-      col.max_newton = max(col.max_newton, iter + 1);
-      col.status = kFailed;
-    }
+  if (!accept) {  // the solve failed: the attempt fails (implicit_stage.cpp:226-238)
+    fail_attempt(iter + 1, its);
     return;
   }
   double dd = 0.0, unused2 = 0.0;
@@ -584,8 +633,8 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       const int64_t q = 3 * e + d;
       a.du[q] = x[d];
       a.u[q] += x[d];
-      a.v[q] += x[d] * a.c2;
-      a.a[q] += x[d] * a.c3;
+      a.v[q] += x[d] * K0[4];
+      a.a[q] += x[d] * K0[5];
       dd += x[d] * x[d];
     }
   RA.sum2(dd, unused2);
@@ -799,6 +848,9 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   if (d->max_elements <= 0 || d->max_columns <= 0 || d->max_materials <= 0 || d->max_record <= 0 ||
       !(d->dt > 0.0) || !(d->beta > 0.0) || d->max_iter <= 0 || d->cg_max_it <= 0 || (d->test != 0 && d->test != 1))
     return fail(ctx, FX_EINVAL, "fx_column_pool_create: bad descriptor");
+  if (d->max_retries < 0 || d->num_sublevels < 0 || d->num_sublevels > kMaxSublevels ||
+      (d->num_sublevels > 0 && d->num_substeps < 1))
+    return fail(ctx, FX_EINVAL, "fx_column_pool_create: max_retries >= 0, num_sublevels in [0, 6], num_substeps >= 1");
   FX_HIP(ctx, hipSetDevice(ctx->device));
   fx_column_pool* p = new (std::nothrow) fx_column_pool;
   if (!p) return fail(ctx, FX_ENOMEM, "host allocation failed");
@@ -814,12 +866,22 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   a.cg_rtol = d->cg_rtol;
   a.inner_rtol = d->inner_rtol > 0.0 ? d->inner_rtol : 1e-5;
   const double gamma = d->gamma, beta = d->beta, h = d->dt;
-  a.a1 = 1.0 - gamma / beta;
-  a.a2 = h * (1.0 - 0.5 * gamma / beta);
-  a.a3 = -1.0 / (beta * h);
-  a.a4 = 1.0 - 0.5 / beta;
-  a.c2 = gamma / (beta * h);
-  a.c3 = 1.0 / (beta * h * h);
+  // the constants of each recovery level's size, h / num_substeps per level down (as fx_newmark
+  // divides it)
+  a.max_retries = d->max_retries;
+  a.num_sublevels = d->num_sublevels;
+  a.num_substeps = d->num_sublevels > 0 ? d->num_substeps : 1;
+  double hl = h;
+  for (int l = 0; l <= kMaxSublevels; ++l) {
+    double* K = a.nk[l];
+    K[0] = 1.0 - gamma / beta;
+    K[1] = hl * (1.0 - 0.5 * gamma / beta);
+    K[2] = -1.0 / (beta * hl);
+    K[3] = 1.0 - 0.5 / beta;
+    K[4] = gamma / (beta * hl);
+    K[5] = 1.0 / (beta * hl * hl);
+    hl = hl / a.num_substeps;
+  }
   const size_t cap = static_cast<size_t>(a.cap);
   int rc = FX_OK;
   ColumnDev* cols = nullptr;
@@ -832,7 +894,7 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
       (rc = pool_alloc(p, a.eout, 9 * cap)) ||
       (rc = pool_alloc(p, a.u, 3 * cap)) || (rc = pool_alloc(p, a.v, 3 * cap)) || (rc = pool_alloc(p, a.a, 3 * cap)) ||
       (rc = pool_alloc(p, a.ut, 3 * cap)) || (rc = pool_alloc(p, a.vt, 3 * cap)) ||
-      (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) ||
+      (rc = pool_alloc(p, a.at, 3 * cap)) || (rc = pool_alloc(p, a.du, 3 * cap)) || (rc = pool_alloc(p, a.du0, 3 * cap)) ||
       (rc = pool_alloc(p, rec, static_cast<size_t>(d->max_record))) ||
       ((d->flags & FX_COLUMN_COMPACT) &&
        ((rc = pool_alloc(p, a.ceps, 3 * cap)) || (rc = pool_alloc(p, a.plist, cap)) ||
@@ -959,7 +1021,7 @@ int fx_column_add(fx_column_pool* p, const fx_column_desc* d, int32_t* id) {
                              hipMemcpyHostToDevice, st));
   FX_HIP(ctx, hipMemcpyAsync(const_cast<double*>(a.record) + r0, d->ground_accel, sizeof(double) * d->nsteps,
                              hipMemcpyHostToDevice, st));
-  for (double* q : {a.u, a.v, a.a, a.ut, a.vt, a.at, a.du}) FX_HIP(ctx, hipMemsetAsync(q + 3 * e0, 0, n3, st));
+  for (double* q : {a.u, a.v, a.a, a.ut, a.vt, a.at, a.du, a.du0}) FX_HIP(ctx, hipMemsetAsync(q + 3 * e0, 0, n3, st));
   hipLaunchKernelGGL(column_init_kernel, dim3(static_cast<unsigned>((N + 255) / 256)), dim3(256), 0, st, a.cap, e0, N,
                      a.params, a.elem_param, a.state, a.touched);
   FX_HIP(ctx, hipGetLastError());
@@ -1019,6 +1081,10 @@ int fx_column_status(fx_column_pool* p, int32_t id, fx_column_result* r) {
   r->max_newton_iterations = cd.max_newton;
   r->krylov_iterations = cd.krylov;
   r->last_test = cd.last_test;
+  r->commits = cd.commits;
+  r->failed_attempts = cd.failed_attempts;
+  r->retries = cd.retries;
+  r->substepped = cd.substepped;
   return FX_OK;
 }
 

@@ -429,6 +429,12 @@ typedef struct fx_column_pool_desc {
   int32_t cg_max_it;
   uint32_t flags;             /* FX_COLUMN_* */
   double inner_rtol;          /* FX_COLUMN_MIXED: relative reduction of each FP32 inner solve */
+  /* recovery of a failed step, per column, as fx_newmark (implicit_stage.cpp:385-402): max_retries
+   * more attempts at the same size, then num_substeps steps of 1/num_substeps the size down to
+   * num_sublevels levels (at most 6); a substep's ground acceleration is the record read as a path
+   * series with a leading zero (linear within a step from the previous step's value). 0, 0: a
+   * failed step fails the column. */
+  int32_t max_retries, num_sublevels, num_substeps;
 } fx_column_pool_desc;
 typedef struct fx_column_desc {
   int32_t nelem;              /* elements, bottom to top (1..FX_COLUMN_MAX_ELEMS) */
@@ -447,6 +453,8 @@ typedef struct fx_column_result {
   int32_t steps;              /* converged (committed) steps */
   int32_t newton_iterations, max_newton_iterations, krylov_iterations;
   double last_test;
+  int32_t commits;            /* converged attempts (substeps included) */
+  int32_t failed_attempts, retries, substepped;
 } fx_column_result;
 /* Allocates all device memory of the pool (nothing is allocated per column); FX_ENOMEM over the
  * context's budget. */

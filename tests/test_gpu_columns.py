@@ -218,6 +218,43 @@ def test_a_step_that_does_not_converge_fails_and_keeps_its_start(ctx):
     assert np.all(res["u"] == 0.0) and np.all(res["v"] == 0.0) and np.all(res["a"] == 0.0)
 
 
+def test_failed_steps_are_retried_and_substepped_as_the_oracle_stage(ctx):
+    """Recovery inside the pool (implicit_stage.cpp:385-402) with Newton capped at 3 iterations:
+    a step that needs more fails, is retried once, then split into halves, each half's ground
+    acceleration the record read as a path series with a leading zero. Against the oracle's stage
+    with the same recovery and that series: the same commits, failed attempts, retries and
+    substeps, the state to 1e-8 (the two evaluate the series' fraction differently)."""
+    n, nsteps = 10, 10
+    mats, mat_of = column_case(n, 302)
+    ag = S.ground_motion(nsteps, DT, 7, pga=6.0)
+    kw = dict(max_retries=1, num_sublevels=3, num_substeps=2)
+    pool = core.ColumnPool(ctx, max_elements=n + 64, max_columns=2, dt=DT, max_record=10 * nsteps + 16, max_iter=3,
+                           tol=1e-10, test=1, **kw)
+    res = run_pool(ctx, [(mats, mat_of)], nsteps, ag, pool=pool)[0]
+    pool.close()
+    st = res["status"]
+    model = column_model(n, len(mats), mat_of)
+    pm = core.Mesh(model)
+    om = fxo.OracleMesh(model)
+    pe = fxo.pimy_elems([mats[g] for g in pm.param_group], stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    N = om.neq
+    u, v, a, du = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+    gd = np.zeros(N)
+    q = om.node_eq.reshape(-1, 3)[:, 0]
+    gd[q[q >= 0]] = 1.0
+    rc, ost = om.newmark(u, v, a, du, DT, nsteps, pimy=pe, pimy_state=[Cs, Ts], ground_dir=gd,
+                         series=dict(values=np.concatenate([[0.0], ag]), dt=DT), test=1, tol=1e-10, max_iter=3, **kw)
+    assert rc == 0 and ost["substepped"] > 0 and ost["retries"] > 0, ost
+    assert st["status"] == 1 and st["steps"] == nsteps, st
+    for key in ("commits", "failed_attempts", "retries", "substepped"):
+        assert st[key] == ost[key], (key, st, ost)
+    for got, want in zip((res["u"], res["v"], res["a"]), (u, v, a)):
+        assert rel(got, level_values(pm, want, n)) <= 1e-8
+    assert history_error(res["state"], Cs[8 * np.argsort(pm.elem_order)]) <= 1e-8
+
+
 def test_mixed_precision_columns_match_fp64(ctx):
     """FP32 inner PCG on FP32 blocks inside FP64 refinement to the same relative residual: the
     error against the all-FP64 pool is stated per result (printed) and bounded by 1e-10."""
