@@ -524,9 +524,16 @@ int fx_batch_tune(fx_batch* b, int variant) {
     FX_HIP(ctx, hipMemcpyAsync(x, pattern.data(), sizeof(double) * pattern.size(), hipMemcpyHostToDevice, ctx->stream));
     FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   }
-  hipEvent_t e0, e1;
-  FX_HIP(ctx, hipEventCreate(&e0));
-  FX_HIP(ctx, hipEventCreate(&e1));
+  struct Events {  // destroyed on every return
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  FX_HIP(ctx, hipEventCreate(&ev.e0));
+  FX_HIP(ctx, hipEventCreate(&ev.e1));
+  const hipEvent_t e0 = ev.e0, e1 = ev.e1;
   // 16-deep tiles: the four register budgets of the streamed kernel; 8-deep: held and streamed
   // tiles; 4-deep: held tiles only
   const int nvar = b->tile_z == 16 ? 4 : b->tile_z == 8 ? 12 : 8;
@@ -552,8 +559,6 @@ int fx_batch_tune(fx_batch* b, int variant) {
   int best_v = 0;
   for (int v = 1; v < nvar; ++v)
     if (vbest[v] + mbest[v] < vbest[best_v] + mbest[best_v]) best_v = v;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   b->box_variant = best_v;
   b->info.variant = best_v;
   return FX_OK;
