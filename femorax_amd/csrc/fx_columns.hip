@@ -265,6 +265,17 @@ __device__ inline T wave_sum(T x) {
 template <int WAVES, typename T>
 struct Reducer {
   T* red;  // LDS [2 * WAVES]
+  // one value: the same sums, in the same order, as the first of sum2
+  __device__ void sum1(T& x) const {
+    x = wave_sum(x);
+    if constexpr (WAVES > 1) {
+      const int w = threadIdx.x / 64;
+      if ((threadIdx.x & 63) == 0) red[2 * w] = x;
+      __syncthreads();
+      x = red[0];
+      for (int k = 1; k < WAVES; ++k) x += red[2 * k];
+    }
+  }
   __device__ void sum2(T& x, T& y) const {
     x = wave_sum(x);
     y = wave_sum(y);
@@ -430,9 +441,9 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       if (d == col.ground_dir) r -= ag * (m + mr + mhi);
       B[d] = r;
     }
-  double bb = 0.0, unused = 0.0;
+  double bb = 0.0;
   for (int d = 0; d < 3; ++d) bb += B[d] * B[d];
-  RA.sum2(bb, unused);
+  RA.sum1(bb);
   const double bnorm = sqrt(bb);
   const int iter = col.iter;
   if (iter > 0) {
@@ -534,9 +545,9 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     converged = rnorm <= tol;
     while (!converged && its < a.cg_max_it) {
       apply(p, Ap);
-      double pAp = 0.0, dummy = 0.0;
+      double pAp = 0.0;
       for (int d = 0; d < 3; ++d) pAp += p[d] * Ap[d];
-      RC.sum2(pAp, dummy);
+      RC.sum1(pAp);
       if (!(pAp > 0.0)) break;  // indefinite
       const double alpha = rz / pAp;
       double rr2 = 0.0, rz2 = 0.0;
@@ -574,12 +585,12 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     const float inner_rtol = static_cast<float>(a.inner_rtol);
     for (int cycle = 0;; ++cycle) {
       apply(x, Ap);
-      double rr = 0.0, unused_r = 0.0;
+      double rr = 0.0;
       for (int d = 0; d < 3; ++d) {
         r[d] = B[d] - Ap[d];
         rr += r[d] * r[d];
       }
-      RB.sum2(rr, unused_r);
+      RB.sum1(rr);
       rnorm = sqrt(rr);
       converged = rnorm <= tol;
       if (converged || its >= a.cg_max_it || cycle >= kMaxRefinements) break;
@@ -599,9 +610,9 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       const float itol = inner_rtol * sqrtf(rrf);
       while (its < a.cg_max_it) {
         column_apply(neighbours32, Klo32, Kd32, Khi32, pf, Apf);
-        float pAp = 0.f, dummy = 0.f;
+        float pAp = 0.f;
         for (int d = 0; d < 3; ++d) pAp += pf[d] * Apf[d];
-        RE.sum2(pAp, dummy);
+        RE.sum1(pAp);
         if (!(pAp > 0.f)) break;
         const float alpha = rzf / pAp;
         float rr2 = 0.f, rz2 = 0.f;
@@ -627,7 +638,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
     fail_attempt(iter + 1, its);
     return;
   }
-  double dd = 0.0, unused2 = 0.0;
+  double dd = 0.0;
   if (live)
     for (int d = 0; d < 3; ++d) {
       const int64_t q = 3 * e + d;
@@ -637,7 +648,7 @@ __global__ __launch_bounds__(64 * WAVES) void column_solve_kernel(PoolArgs a) {
       a.a[q] += x[d] * K0[5];
       dd += x[d] * x[d];
     }
-  RA.sum2(dd, unused2);
+  RA.sum1(dd);
   __syncthreads();
   if (t == 0) {
     col.dnorm = sqrt(dd);
