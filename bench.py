@@ -565,6 +565,11 @@ def main():
     if args.c4:
         try:
             out["c4_columns"] = bench_c4(core, S, ctx, args.c4, ncol=args.c4_columns, cpu=not args.no_cpu)
+            # the same pool with FP32 inner PCG in FP64 refinement (same relative tolerance)
+            m = bench_c4(core, S, ctx, args.c4, ncol=args.c4_columns, cpu=False, plain=False, mixed=True)
+            out["c4_columns"]["mixed_precision"] = {k: m[k] for k in ("column_steps_per_s", "ms_per_pool_iteration",
+                                                                     "newton_per_step", "cg_per_newton",
+                                                                     "failed_columns", "solve")}
         except Exception as e:  # noqa: BLE001
             out["c4_columns"] = {"error": repr(e)}
     if args.c5:
