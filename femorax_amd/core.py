@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(HERE, "libfemorax_amd.so")
 _LIB = None
 
 FX_OK = 0
+FX_EINVAL = -1
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP"}
 FX_BATCH_FP32 = 1
 FX_BATCH_GENERAL = 2

@@ -55,6 +55,8 @@ void dev_free(fx_ctx* ctx, void* p, uint64_t& bytes, size_t sz) {
 
 }  // namespace
 
+void fx::set_thread_error(const std::string& msg) { g_error = msg; }
+
 extern "C" {
 
 int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out) {

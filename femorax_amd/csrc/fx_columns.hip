@@ -762,6 +762,7 @@ namespace {
 
 int fail(fx_ctx* ctx, int code, const std::string& msg) {
   if (ctx) ctx->error = msg;
+  else set_thread_error(msg);
   return code;
 }
 
@@ -854,14 +855,26 @@ void fx_column_pool_destroy(fx_column_pool* p) {
 
 int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_pool** out) {
   using namespace fx;
-  if (!ctx || !d || !out) return fail(ctx, FX_EINVAL, "fx_column_pool_create: null argument");
+  if (!d || !out) return fail(ctx, FX_EINVAL, "fx_column_pool_create: null argument");
   *out = nullptr;
+  // the descriptor is checked before the context, so a bad one is refused without a device
   if (d->max_elements <= 0 || d->max_columns <= 0 || d->max_materials <= 0 || d->max_record <= 0 ||
       !(d->dt > 0.0) || !(d->beta > 0.0) || d->max_iter <= 0 || d->cg_max_it <= 0 || (d->test != 0 && d->test != 1))
     return fail(ctx, FX_EINVAL, "fx_column_pool_create: bad descriptor");
   if (d->max_retries < 0 || d->num_sublevels < 0 || d->num_sublevels > kMaxSublevels ||
       (d->num_sublevels > 0 && d->num_substeps < 1))
     return fail(ctx, FX_EINVAL, "fx_column_pool_create: max_retries >= 0, num_sublevels in [0, 6], num_substeps >= 1");
+  {
+    // a substep's position within its step is counted in units of num_substeps^level (int64 on
+    // the device); the deepest level's count must stay within INT32_MAX
+    int64_t units = 1;
+    for (int l = 0; l < d->num_sublevels; ++l) {
+      units *= d->num_substeps;
+      if (units > INT32_MAX)
+        return fail(ctx, FX_EINVAL, "fx_column_pool_create: num_substeps^num_sublevels exceeds 2^31 - 1");
+    }
+  }
+  if (!ctx) return fail(ctx, FX_EINVAL, "fx_column_pool_create: null context");
   FX_HIP(ctx, hipSetDevice(ctx->device));
   fx_column_pool* p = new (std::nothrow) fx_column_pool;
   if (!p) return fail(ctx, FX_ENOMEM, "host allocation failed");

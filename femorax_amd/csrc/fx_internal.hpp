@@ -128,6 +128,8 @@ struct fx_ctx {
 };
 
 namespace fx {
+// The message fx_last_error(NULL) returns (errors raised before a context exists).
+void set_thread_error(const std::string& msg);
 // Drop the captured CG graphs of one owner (nullptr: all).
 void release_cg_graphs(fx_ctx* ctx, const void* owner);
 }  // namespace fx

@@ -399,6 +399,10 @@ struct CgGraphKey {
   int32_t tsize, initial;
   double ks, kd, km, rtol, atol;
   const void* ptr[10];
+  // the batch arrays the operator kernels read whose pointers change between calls: the
+  // committed and trial histories swap on every commit (the committed tangent is formed from
+  // the committed history), so each parity of the swap has its own graph
+  const void* state[2];
 };
 
 // The captured graph of kGraphIters iterations for this key, capturing it on first use.
@@ -437,7 +441,7 @@ hipGraphExec_t cg_graph(fx_ctx* ctx, const CgGraphKey& key, Body body) {
 // and arguments, so the same bits as launching them).
 template <typename T, typename Apply>
 int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
-              CgState* st, double rtol, double atol, int max_it, CgState& host, const void* owner = nullptr,
+              CgState* st, double rtol, double atol, int max_it, CgState& host, const fx_batch* owner = nullptr,
               const fx_coeffs* kc = nullptr) {
   if (int rc = ensure_red(ctx, n)) return rc;
   hipStream_t s = ctx->stream;
@@ -494,6 +498,8 @@ int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* 
     key.atol = atol;
     const void* ptrs[10] = {inv, b, x, r, z, p, ap, st, part, tmp};
     for (int i = 0; i < 10; ++i) key.ptr[i] = ptrs[i];
+    key.state[0] = owner->state_c;
+    key.state[1] = owner->state_t;
     exec = cg_graph(ctx, key, body);
   }
   for (;;) {

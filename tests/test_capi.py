@@ -82,3 +82,21 @@ def test_padding_is_inert():
     np.testing.assert_array_equal(pm.eq.reshape(24, P)[:, :P1], pm1.eq.reshape(24, P1))
     assert np.all(pm.eq.reshape(24, P)[:, P1:] == -1)
     assert np.all(pm.node.reshape(8, P)[:, P1:] == -1)
+
+
+def test_column_pool_refuses_substep_counts_beyond_int32():
+    """The pool counts a substep's position in units of num_substeps^level; descriptors whose
+    deepest level would exceed 2^31 - 1 units are refused before any device work."""
+    import ctypes as C
+
+    d = core.ColumnPoolDesc(max_elements=64, max_columns=1, max_materials=1, max_record=64, gamma=0.5, beta=0.25,
+                            dt=0.005, test=1, tol=1e-10, max_iter=10, cg_rtol=1e-12, cg_max_it=100, flags=0,
+                            inner_rtol=0.0, max_retries=1, num_sublevels=6, num_substeps=1500)
+    out = C.c_void_p()
+    L = core.lib()
+    assert L.fx_column_pool_create(None, C.byref(d), C.byref(out)) == core.FX_EINVAL
+    assert b"num_substeps^num_sublevels" in L.fx_last_error(None)
+    assert not out.value
+    d.num_substeps, d.num_sublevels = 2, 6  # 64 units: passes the descriptor checks, then needs a context
+    assert L.fx_column_pool_create(None, C.byref(d), C.byref(out)) == core.FX_EINVAL
+    assert b"null context" in L.fx_last_error(None)

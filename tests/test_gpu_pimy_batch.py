@@ -438,3 +438,33 @@ def test_large_batch_operator_one_element_per_thread(ctx):
         assert rel(dy.get(), om.apply_gp(pe, D_o, x, fxo.coeffs(*kk), committed=Co)) <= 1e-10, kk
         batch.diagonal(dy, core.coeffs(*kk))
         assert rel(dy.get(), om.diagonal_gp(pe, D_o, fxo.coeffs(*kk), committed=Co)) <= 1e-10, kk
+
+
+def test_graph_replayed_newmark_keeps_the_committed_history(ctx):
+    """Graph-replayed CG inside a Newmark stage on a damped PIMY column (betaKcomm: the operator
+    reads the committed history, whose buffer swaps with the trial one at every commit): the
+    same bits as the launched iterations over several committed steps."""
+    results = []
+    for graphs in (False, True):
+        model, pm, batch, om, per_set = setup_damped(ctx, "column12_tied")
+        batch.set_stage(1)
+        n = pm.neq
+        h, nsteps = 0.005, 6
+        ag = 6.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 2)
+        d = [core.DeviceArray(ctx, n) for _ in range(4)]
+        for x in d:
+            x.zero()
+        ctx.set_graphs(graphs)
+        try:
+            res = batch.newmark(*d, h, nsteps, ground_dir=0, ground_accel=ag, test=0, tol=1e-3)
+        finally:
+            ctx.set_graphs(False)
+        assert res["failed"] == 0 and res["steps"] == nsteps, res
+        results.append(([x.get() for x in d[:3]], batch.pimy_state(True), res))
+        batch.close()
+    (u0, s0, r0), (u1, s1, r1) = results
+    assert r0["krylov_iterations"] == r1["krylov_iterations"], (r0, r1)
+    for a, b in zip(u0, u1):
+        np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(s0, s1)
+    assert np.max(s0[:, 26]) > 0, "the stage reached the plastic range"
