@@ -219,6 +219,7 @@ SIGNATURES = [
     ("fx_operator_time", C.c_int, [vp, C.POINTER(Coeffs), C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
     ("fx_copy", C.c_int, [vp, i64, vp, vp]),
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
+    ("fx_dot_static", C.c_int, [vp, i64, vp, vp, vp]),
     ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
     ("fx_gmres", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),

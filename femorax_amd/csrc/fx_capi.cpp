@@ -83,6 +83,14 @@ int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out) {
     delete c;
     return fail(nullptr, FX_EDEVICE, "hipMalloc failed");
   }
+  if (int rc = fx::static_scratch_init(c)) {
+    const std::string msg = c->error;
+    (void)hipFree(c->scalars);
+    (void)hipFree(c->pimy_params);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail(nullptr, rc, msg);
+  }
   *out = c;
   return FX_OK;
 }

@@ -50,7 +50,8 @@ Graph build_node_graph(const fx_model_desc& m) {
     auto& l = nbr[v];
     std::sort(l.begin(), l.end());
     l.erase(std::unique(l.begin(), l.end()), l.end());
-    l.erase(std::remove(l.begin(), l.end(), v), l.end());
+    // a node listed at two corners of one element keeps itself as a neighbour (the pairs are
+    // distinct corners, not distinct nodes: renumber.cpp:38-41), which counts in its degree
     total += l.size();
     g.start[v + 1] = static_cast<int64_t>(total);
   }

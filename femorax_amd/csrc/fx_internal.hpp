@@ -130,6 +130,9 @@ struct fx_ctx {
 namespace fx {
 // The message fx_last_error(NULL) returns (errors raised before a context exists).
 void set_thread_error(const std::string& msg);
+// The device address of fx_dot_static's fixed scratch array, looked up once per device (at
+// context creation, with the device current).
+int static_scratch_init(fx_ctx* ctx);
 // Drop the captured CG graphs of one owner (nullptr: all).
 void release_cg_graphs(fx_ctx* ctx, const void* owner);
 }  // namespace fx

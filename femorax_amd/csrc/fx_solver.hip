@@ -15,6 +15,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
+#include <string>
 
 #include "fx_internal.hpp"
 #include "fx_kernels.hpp"
@@ -679,3 +681,91 @@ int cg_mixed_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, 
 }
 
 }  // namespace fx
+
+// ---- VectorOps::dot without allocation (fx_dot_static) ------------------------------------
+// The reduction scratch of fx_dot_static: a fixed array of the library on every device it runs
+// on, sized for FX_DOT_STATIC_MAX_N entries (one chunk partial per 2048 entries, then the level
+// buffers reduce_kernel walks: 131,220 doubles, 1.05 MB). In host code `scratch` names a host
+// stand-in, never the device memory: host code uses only the address hipGetSymbolAddress gives
+// for the current device, kept per device in scratch_ptr.
+namespace fx {
+namespace {
+
+constexpr int64_t kStaticDotChunks = FX_DOT_STATIC_MAX_N / kChunk;
+constexpr size_t kStaticScratch = kStaticDotChunks + 2 * (kStaticDotChunks / kChunk + 2) + 16;
+static_assert(FX_DOT_STATIC_MAX_N % kChunk == 0, "whole chunks");
+
+static __device__ __align__(256) double scratch[kStaticScratch];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[kMaxDevices] = {};
+
+__global__ void store_zero_kernel(double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = 0.0;
+}
+
+}  // namespace
+
+// Once per device, with that device current (fx_ctx_create): the device address of the array.
+int static_scratch_init(fx_ctx* ctx) {
+  if (ctx->device < 0 || ctx->device >= kMaxDevices) {
+    ctx->error = "fx_ctx_create: device index beyond the static scratch table";
+    return FX_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(scratch_mutex);
+  if (scratch_ptr[ctx->device]) return FX_OK;
+  void* p = nullptr;
+  FX_HIP(ctx, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) {
+    ctx->error = "hipGetSymbolAddress(scratch): null address";
+    return FX_EDEVICE;
+  }
+  scratch_ptr[ctx->device] = static_cast<double*>(p);
+  return FX_OK;
+}
+
+// The kernels and arguments of dot64 on the device's fixed scratch, the result left in device
+// memory: no allocation, no host synchronisation.
+int dot64_static(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out_dev) {
+  double* base = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mutex);
+    base = scratch_ptr[ctx->device];
+  }
+  if (!base) {
+    ctx->error = "fx_dot_static: no static scratch on this device (context set-up failed)";
+    return FX_EDEVICE;
+  }
+  hipStream_t s = ctx->stream;
+  if (n == 0) {
+    hipLaunchKernelGGL(store_zero_kernel, dim3(1), dim3(64), 0, s, out_dev);
+    FX_HIP(ctx, hipGetLastError());
+    return FX_OK;
+  }
+  double* part = base;
+  double* tmp = base + chunks(n);
+  hipLaunchKernelGGL(chunk_dot_kernel<double>, dim3(static_cast<unsigned>(chunks(n))), dim3(kLanes), 0, s, n, x, y,
+                     static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
+                     static_cast<const int32_t*>(nullptr));
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp,
+                     static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, out_dev);
+  FX_HIP(ctx, hipGetLastError());
+  return FX_OK;
+}
+
+}  // namespace fx
+
+extern "C" int fx_dot_static(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out_dev) {
+  if (!ctx || !out_dev || (n > 0 && (!x || !y))) {
+    if (ctx) ctx->error = "fx_dot_static: null argument";
+    else fx::set_thread_error("fx_dot_static: null argument");
+    return FX_EINVAL;
+  }
+  if (n < 0 || n > FX_DOT_STATIC_MAX_N) {
+    ctx->error = "fx_dot_static: n = " + std::to_string(n) + " outside [0, FX_DOT_STATIC_MAX_N = " +
+                 std::to_string(static_cast<int64_t>(FX_DOT_STATIC_MAX_N)) + "]";
+    return FX_EINVAL;
+  }
+  return fx::dot64_static(ctx, n, x, y, out_dev);
+}

@@ -1,0 +1,69 @@
+"""Profiling drivers for single kernels, run under rocprofv3 (kernel trace or counters):
+
+    python femorax_amd/profile_kernels.py general N REPS   general-geometry operator (hex8_block_kernel) on a
+                                                             distorted N^3 layered box, FP64 and FP32 applies
+    python femorax_amd/profile_kernels.py pimy SIDE REPS   stdBrick<PressureIndependMultiYield> force at a yielding
+                                                             state (pimy_batch_kernel) on the SIDE^3 reference column
+"""
+import os
+import sys
+
+import numpy as np
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+from femorax_amd import core  # noqa: E402
+from femorax_amd import synthetic as S  # noqa: E402
+
+
+def general(n, reps):
+    model = S.distorted_box(n)
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=True)
+    info = batch.info()
+    assert info["lattice"] == 0, info
+    print("batch", info, flush=True)
+    x = np.sin(0.001 * np.arange(mesh.neq))
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, mesh.neq)
+    dx32, dy32 = core.DeviceArray.from_host(ctx, x.astype(np.float32)), core.DeviceArray(ctx, mesh.neq, np.float32)
+    ms64 = batch.time_apply(dx, dy, reps, 64) / reps
+    ms32 = batch.time_apply(dx32, dy32, reps, 32) / reps
+    print(f"general path {mesh.nelem} elements {mesh.neq} DoF: FP64 {ms64 * 1e3:.1f} us, FP32 {ms32 * 1e3:.1f} us "
+          f"per apply", flush=True)
+    batch.close()
+    ctx.close()
+
+
+def pimy(side, reps):
+    model = S.reference_column(side)
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    keep = []
+    mats, rays = [], []
+    for p in model["params"]:
+        G = p["E"] / (2.0 * (1.0 + p["nu"]))
+        K = p["E"] / (3.0 * (1.0 - 2.0 * p["nu"]))
+        mats.append(core.pimy_material(p["rho"], G, K, 30.0, 0.1, 0.0, 100.0, 0.0, nsurf=20, b=p["b"], mass_type=0,
+                                       keep=keep))
+        rays.append(p["rayleigh"])
+    pb = core.Batch.pimy(ctx, mesh, mats, rayleigh=rays)
+    pb.set_stage(1)
+    z = np.linspace(0.0, 1.0, mesh.neq)
+    du = core.DeviceArray.from_host(ctx, 2e-3 * np.sin(40.0 * z))
+    df = core.DeviceArray(ctx, mesh.neq)
+    for _ in range(reps):
+        pb.force(du, df)
+    ctx.sync()
+    active = pb.pimy_state(False)[:, 26]
+    print(f"pimy force on {mesh.nelem} elements, {int(np.sum(active > 0))} of {active.size} points past the "
+          f"first surface", flush=True)
+    pb.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1]
+    size = int(sys.argv[2]) if len(sys.argv) > 2 else (128 if what == "general" else 40)
+    reps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
+    {"general": general, "pimy": pimy}[what](size, reps)

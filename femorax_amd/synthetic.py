@@ -82,6 +82,24 @@ def elastic_box(n=16, L=1.0, E=30e6, nu=0.3, rho=2000.0, layers=None, body=False
                 grid=(n, n, n), h=(h, h, h))
 
 
+def distorted_box(n=128, L=None, amp=0.2, layers=8, seed=SEED_BASE + 5):
+    """The layered n^3 box of elastic_box (h = 1 m unless L is given) with every interior node
+    moved by up to amp * h in each direction (numpy's generator from `seed`): no element is an
+    axis-aligned box, so the operator runs on the general-geometry path."""
+    L = float(n) if L is None else L
+    m = elastic_box(n, L, layers=layers, body=True, seed=seed)
+    h = L / n
+    N1 = n + 1
+    idx = np.arange(m["nnode"])
+    i, j, k = idx % N1, (idx // N1) % N1, idx // (N1 * N1)
+    inner = (i > 0) & (i < n) & (j > 0) & (j < n) & (k > 0) & (k < n)
+    rng = np.random.default_rng(seed)
+    xyz = m["xyz"].copy()
+    xyz[inner] += amp * h * rng.uniform(-1.0, 1.0, size=(int(inner.sum()), 3))
+    m["xyz"] = xyz
+    return m
+
+
 def soil_column(nz, h=1.0, params=None, layer_of=None, nx=1, ny=1):
     """A 1x1xN (or nx x ny x N) column: base fixed, side nodes of each level tied to the first
     side node in DOFs 1 2 3 (the laminar boundary of synthetic_models.hpp:64-79)."""

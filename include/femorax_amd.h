@@ -217,6 +217,16 @@ int fx_copy(fx_ctx* ctx, int64_t n, const double* src, double* dst);
 /* VectorOps::dot (vector_ops.cpp:66-73): chunks of 2048, 64 lanes, fixed order. */
 int fx_dot(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out);
 
+/* The same dot into device memory, allocating nothing (no counterpart in the reference; for
+ * callers that cannot allocate device memory: code under HIP graph capture, tight serving
+ * loops). *out_dev (device) receives fx_dot's bits; the work is enqueued on the context's stream
+ * and the call returns without waiting (fx_sync or a later blocking call waits). The reduction
+ * scratch is a fixed 1.05 MB array of the library on each device, so n is limited to
+ * FX_DOT_STATIC_MAX_N (FX_EINVAL above it), and calls that overlap on one device (several
+ * contexts or threads on one GPU) race on it. */
+#define FX_DOT_STATIC_MAX_N 268435456
+int fx_dot_static(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out_dev);
+
 /* Preconditioned CG with Jacobi, FP64 (ConjugateGradient::solve + Jacobi). x: initial guess in,
  * solution out. The plain drop-in path. */
 int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,

@@ -37,7 +37,9 @@ static int cmp_i32(const void* a, const void* b) {
   return (x > y) - (x < y);
 }
 
-/* CSR node graph, neighbour lists sorted and unique, no self loops (renumber.cpp:30-57). */
+/* CSR node graph, neighbour lists sorted and unique (renumber.cpp:30-57). Pairs are of distinct
+ * corners (:38-41), so a node listed twice in one element keeps itself as a neighbour and in its
+ * degree. */
 typedef struct { int32_t n; int64_t* start; int32_t* adj; } graph;
 
 static graph node_graph(const fxo_model* m) {
@@ -76,7 +78,6 @@ static graph node_graph(const fxo_model* m) {
     int64_t r0 = w;
     for (int64_t k = b; k < e; ++k) {
       int32_t v = tmp[k];
-      if (v == i) continue; /* self loop (a node listed twice in one element) */
       if (w > r0 && tmp[w - 1] == v) continue;
       tmp[w++] = v;
     }
