@@ -54,15 +54,17 @@ def cpu_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(model, params, x, budget_s=20.0):
-    """The oracle (C restatement of the reference's CPU default: gather assembly, recomputed
-    geometry... constant D) timed on the host cores; bounded to about budget_s seconds."""
+def cpu_baseline(model, params, x, y_gpu=None, budget_s=20.0):
+    """The oracle (C restatement of the reference's CPU operator: gather assembly, constant D,
+    element geometry recomputed per apply where the reference's CPU default caches it) timed on
+    the host cores, bounded to about budget_s seconds; its y also checks the GPU's y of the same
+    x at full size (max |y_gpu - y_oracle| over max |y_oracle|)."""
     from oracle import fxo
 
     threads = cpu_threads()
     om = fxo.OracleMesh(model, coloring=False)
     op = fxo.elastic_params(params)
-    om.apply(op, x, nthreads=threads)  # warm-up
+    y = om.apply(op, x, nthreads=threads)  # warm-up
     reps, t_total = 0, 0.0
     while t_total < budget_s and reps < 50:
         t0 = time.perf_counter()
@@ -70,10 +72,13 @@ def cpu_baseline(model, params, x, budget_s=20.0):
         t_total += time.perf_counter() - t0
         reps += 1
     per = t_total / reps
-    return {"value": om.neq / per / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
-            "sample": f"{reps} full FP64 operator applies of the same {om.neq}-DoF mesh "
-                      f"(oracle/liboracle.so, -O3 -fopenmp -ffp-contract=off, {threads} threads), "
-                      f"{per * 1e3:.1f} ms per apply"}
+    out = {"value": om.neq / per / 1e9, "unit": "GDoF/s", "cores": threads, "kind": "port",
+           "sample": f"{reps} full FP64 operator applies of the same {om.neq}-DoF mesh "
+                     f"(oracle/liboracle.so, -O3 -fopenmp -ffp-contract=off, {threads} threads; geometry "
+                     f"recomputed per apply, the reference's CPU default caches it), {per * 1e3:.1f} ms per apply"}
+    if y_gpu is not None:
+        out["gpu_y_max_rel_error"] = float(np.max(np.abs(y_gpu - y)) / np.max(np.abs(y)))
+    return out
 
 
 def bench_refcol(core, S, ctx, side, reps):
@@ -271,7 +276,8 @@ def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cp
     cols, rec = S.config_c4(ncol=ncol, nsteps=max(nsteps, 150))
     total = sum(c["nelem"] for c in cols)
     pool = core.ColumnPool(ctx, max_elements=total + 1024, max_columns=ncol, dt=0.005, max_materials=10 * ncol,
-                           max_record=ncol * 10 * nsteps + 1024, compact=compact, mixed=mixed, waves=waves)
+                           max_record=ncol * 10 * nsteps + 1024, compact=compact, mixed=mixed, waves=waves,
+                           tune=waves == 0)
     keep = []
     t0 = time.perf_counter()
     ids = [pool.add(core.pimy_materials(c["mats"], keep), c["mat_of"], nsteps, rec, record=True) for c in cols]
@@ -296,7 +302,7 @@ def bench_c4(core, S, ctx, nsteps, ncol=4096, plain_columns=2, plain_steps=5, cp
            "newton_per_step": round(newton / max(steps, 1), 3), "cg_per_newton": round(krylov / max(newton, 1), 2),
            "state_bytes_full_copy": total * 2 * 150 * 8,  # upper bound: unmoved surface centres are not copied
            "return_map": "compacted (predict, compact, correct)" if compact else "fused",
-           "waves_per_simd": {0: 3, 1: "compiler"}.get(waves, waves),
+           "waves_per_simd": {0: "autotuned (1, 3 or 4; timed in the first call)", 1: "compiler"}.get(waves, waves),
            "solve": "FP32 inner PCG in FP64 refinement" if mixed else "FP64 PCG",
            "note": "one material point per tied element (uniform strain), return map + per-column PCG per "
                    "Newton iteration, hipGraph replay"}
@@ -339,11 +345,14 @@ def bench_c5(core, S, duration=60.0, rate=200.0, burst=2000, plain_columns=2, pl
     from femorax_amd import serve
 
     reqs = serve.mixed_requests(duration=duration, rate=rate)
-    srv = serve.Server(hbm_budget=200e9, mixed=True)
+    t0 = time.perf_counter()
+    srv = serve.Server(hbm_budget=200e9, mixed=True, block_sizes=range(8, 49))
+    setup_s = time.perf_counter() - t0
     try:
         recs = srv.run(reqs, log=log)
         served = serve.summary(recs)
         served["loop"] = srv.last_stats
+        served["setup_s"] = round(setup_s, 2)  # block models made resident before the first request
         burst_reqs = serve.mixed_requests(duration=1.0, rate=float(burst), seed=S.SEED_BASE + 5)
         for r in burst_reqs:
             r.arrival = 0.0
@@ -494,6 +503,7 @@ def main():
     for _ in range(nd):
         batch.apply(dx, dy)
     ms_dropin = (time.perf_counter() - t0) / nd * 1e3
+    y_dropin = dy.get()  # the drop-in apply's y (reference numbering): checked against the oracle below
     B64 = algorithmic_bytes(mesh.neq, mesh.nelem, mesh.nnode, 8)
     B32 = algorithmic_bytes(mesh.neq, mesh.nelem, mesh.nnode, 4)
     gbs64 = B64 / (ms64 * 1e-3) / 1e9
@@ -580,7 +590,7 @@ def main():
             out["c5_serving"] = {"error": repr(e)}
     if not args.no_cpu:
         print("[bench] CPU baseline ...", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(model, params, x)
+        out["cpu_baseline"] = cpu_baseline(model, params, x, y_gpu=y_dropin)
     print(json.dumps(out), flush=True)
     batch.close()
     ctx.close()

@@ -24,6 +24,9 @@ FX_BATCH_XCD_SLABS = 8
 FX_BATCH_DYNAMIC = 32
 FX_BATCH_GMRES = 64
 FX_BATCH_NO_TUNE = 16
+FX_BATCH_SLOTS = 2048
+FX_OUT_PLAIN, FX_OUT_ATOMIC, FX_OUT_SLOTS = 0, 1, 2
+FX_TANGENT_SPD, FX_TANGENT_SPSD = 0, 1
 
 
 class FxError(RuntimeError):
@@ -66,7 +69,7 @@ class BatchDesc(C.Structure):
 
 class BatchInfo(C.Structure):
     _fields_ = [(n, C.c_int32) for n in ("count", "neq", "nnode", "nblocks", "ncolors", "tied", "lattice",
-                                         "tiles", "variant")] + [("device_bytes", C.c_uint64)]
+                                         "tiles", "variant", "tangent_symmetry")] + [("device_bytes", C.c_uint64)]
 
 
 class SolveOptions(C.Structure):
@@ -76,7 +79,9 @@ class SolveOptions(C.Structure):
 
 class CdDesc(C.Structure):
     _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("dash", C.c_void_p), ("nload", C.c_int32),
-                ("load_eq", C.c_void_p), ("load_hist", C.c_void_p)]
+                ("load_eq", C.c_void_p), ("load_hist", C.c_void_p), ("scheme", C.c_int32), ("ground_dir", C.c_int32),
+                ("ground_accel", C.c_void_p), ("ground_series", C.c_void_p), ("ground_factor", C.c_double),
+                ("time", C.c_double)]
 
 
 class TimeSeries(C.Structure):
@@ -90,7 +95,9 @@ class NewmarkDesc(C.Structure):
                 ("cg_max_it", C.c_int32), ("ground_dir", C.c_int32), ("ground_accel", C.c_void_p),
                 ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32),
                 ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double),
-                ("mixed", C.c_int32), ("inner_rtol", C.c_double)]
+                ("mixed", C.c_int32), ("inner_rtol", C.c_double), ("analysis", C.c_int32), ("algorithm", C.c_int32),
+                ("modified_initial", C.c_int32), ("factor_once", C.c_int32), ("load_increment", C.c_double),
+                ("load_min_increment", C.c_double), ("load_max_increment", C.c_double), ("load_num_iter", C.c_int32)]
 
 
 class NewmarkResult(C.Structure):
@@ -117,6 +124,7 @@ FX_COLUMN_MAX_ELEMS = 512
 FX_COLUMN_MIXED = 1
 FX_COLUMN_NO_GRAPH = 2
 FX_COLUMN_COMPACT = 4
+FX_COLUMN_TUNE = 16
 FX_BATCH_COMPACT = 128
 FX_BATCH_TILE4 = 256
 FX_BATCH_TILE8 = 512
@@ -216,6 +224,9 @@ SIGNATURES = [
     ("fx_operator_apply", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
     ("fx_operator_apply_f32", C.c_int, [vp, C.POINTER(Coeffs), vp, vp]),
     ("fx_operator_diagonal", C.c_int, [vp, C.POINTER(Coeffs), vp]),
+    ("fx_run_operator", C.c_int, [vp, C.c_int, C.POINTER(Coeffs), C.c_int, vp, vp, vp, i64, C.c_int]),
+    ("fx_batch_set_nodal_mass", C.c_int, [vp, vp, vp]),
+    ("fx_mass_times_ground", C.c_int, [vp, C.c_int, vp]),
     ("fx_operator_time", C.c_int, [vp, C.POINTER(Coeffs), C.c_int, vp, vp, C.c_int, C.POINTER(C.c_float)]),
     ("fx_copy", C.c_int, [vp, i64, vp, vp]),
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
@@ -465,7 +476,7 @@ class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
     def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False,
-                 gmres=False, tile8=None, tile_depth=None):
+                 gmres=False, tile8=None, tile_depth=None, slots=False):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -484,7 +495,8 @@ class Batch:
         h = C.c_void_p()
         flags = (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GENERAL if general else 0) | \
             (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0) | \
-            (FX_BATCH_GMRES if gmres else 0) | (0 if tile8 is None else FX_BATCH_TILE8 if tile8 else FX_BATCH_TILE4)
+            (FX_BATCH_GMRES if gmres else 0) | (0 if tile8 is None else FX_BATCH_TILE8 if tile8 else FX_BATCH_TILE4) | \
+            (FX_BATCH_SLOTS if slots else 0)
         if tile_depth is not None:  # cells per lattice tile in z: 4, 8 or 16
             flags |= {4: FX_BATCH_TILE4, 8: FX_BATCH_TILE8, 16: FX_BATCH_TILE16}[tile_depth]
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
@@ -492,7 +504,7 @@ class Batch:
         ctx.adopt(self)
 
     @classmethod
-    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None, fp32=False):
+    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None, fp32=False, gmres=False):
         """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
         fx_pimy_material per compiled parameter set; compact: the compacted return map;
         rayleigh: per parameter set (alphaM, betaK, betaKinit, betaKcomm), or None."""
@@ -517,7 +529,7 @@ class Batch:
             self._damp = (Rayleigh * len(materials))(*[Rayleigh(*r) for r in rayleigh])
             d.damp = C.cast(self._damp, C.c_void_p)
         h = C.c_void_p()
-        flags = (FX_BATCH_COMPACT if compact else 0) | (FX_BATCH_FP32 if fp32 else 0)
+        flags = (FX_BATCH_COMPACT if compact else 0) | (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GMRES if gmres else 0)
         _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)
@@ -525,7 +537,9 @@ class Batch:
 
     def newmark(self, u, v, a, du, dt, nsteps, gamma=0.5, beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12,
                 cg_max_it=10000, ground_dir=-1, ground_accel=None, max_retries=0, num_sublevels=0, num_substeps=2,
-                series=None, ground_factor=1.0, time=0.0, mixed=False, inner_rtol=1e-3):
+                series=None, ground_factor=1.0, time=0.0, mixed=False, inner_rtol=1e-3, static=False, algorithm=0,
+                modified_initial=False, factor_once=False, load_increment=None, load_min_increment=0.0,
+                load_max_increment=0.0, load_num_iter=1):
         """nsteps Newmark + Newton steps on device arrays u, v, a, du (in place); returns the
         fx_newmark_result as a dict. test: 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr.
         series: the excitation's series (below), evaluated at every attempt's end time (needed to
@@ -546,7 +560,9 @@ class Batch:
         d = NewmarkDesc(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, ground_dir,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
                         None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time,
-                        int(bool(mixed)), inner_rtol)
+                        int(bool(mixed)), inner_rtol, int(bool(static)), algorithm, int(bool(modified_initial)),
+                        int(bool(factor_once)), dt if load_increment is None else load_increment, load_min_increment,
+                        load_max_increment, load_num_iter)
         r = NewmarkResult()
         _check(lib().fx_newmark(self.h, C.byref(d), u.ptr, v.ptr, a.ptr, du.ptr, C.byref(r)), self.ctx)
         return {f: getattr(r, f) for f, _ in NewmarkResult._fields_}
@@ -597,6 +613,23 @@ class Batch:
     def diagonal(self, d, k=None):
         _check(lib().fx_operator_diagonal(self.h, C.byref(k or coeffs()), d.ptr), self.ctx)
 
+    def run_operator(self, out, x=None, y=None, slots=None, slot_offset=0, diagonal=False, unit_dir=-1, k=None):
+        """fx_run_operator (InternalForce::Batch::run_operator): out FX_OUT_SLOTS writes this batch's
+        element values into slots (a DeviceArray) at slot_offset + a * padded + e; FX_OUT_PLAIN /
+        FX_OUT_ATOMIC add into y."""
+        _check(lib().fx_run_operator(self.h, int(bool(diagonal)), C.byref(k or coeffs()), out,
+                                     x.ptr if x is not None else None, y.ptr if y is not None else None,
+                                     slots.ptr if slots is not None else None, slot_offset, unit_dir), self.ctx)
+
+    def set_nodal_mass(self, mass=None, alpha_mass=None):
+        m = None if mass is None else np.ascontiguousarray(mass, np.float64)
+        am = None if alpha_mass is None else np.ascontiguousarray(alpha_mass, np.float64)
+        _check(lib().fx_batch_set_nodal_mass(self.h, None if m is None else m.ctypes.data,
+                                             None if am is None else am.ctypes.data), self.ctx)
+
+    def mass_times_ground(self, direction, y):
+        _check(lib().fx_mass_times_ground(self.h, direction, y.ptr), self.ctx)
+
     def time_apply(self, x, y, reps, precision=64, k=None):
         ms = C.c_float()
         _check(lib().fx_operator_time(self.h, C.byref(k or coeffs()), precision, x.ptr, y.ptr, reps, C.byref(ms)),
@@ -642,12 +675,26 @@ class Batch:
         _check(lib().fx_critical_dt(self.h, scheme, C.byref(dt)), self.ctx)
         return dt.value
 
-    def explicit_cd(self, dt, nsteps, u, um, v, a, dash=None, load_eq=None, load_hist=None):
-        """nsteps CentralDifference steps on device arrays u, um, v, a (in place)."""
+    def explicit_cd(self, dt, nsteps, u, um, v, a, dash=None, load_eq=None, load_hist=None, scheme=0, ground_dir=-1,
+                    ground_accel=None, series=None, ground_factor=1.0, time=0.0):
+        """nsteps explicit steps on device arrays u, um, v, a (in place): scheme 0 CentralDifference
+        (um = U(n-1)), 1 ExplicitDifference (um = the element velocity); a uniform excitation in
+        ground_dir from ground_accel (per step) or series ({"values", "dt"} or {"values", "time"})."""
         le = np.ascontiguousarray(load_eq if load_eq is not None else np.zeros(0), np.int32)
         lh = np.ascontiguousarray(load_hist if load_hist is not None else np.zeros(0), np.float64)
+        ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        keep = []
+        ts = None
+        if series is not None:
+            sv = np.ascontiguousarray(series["values"], np.float64)
+            tv = None if series.get("time") is None else np.ascontiguousarray(series["time"], np.float64)
+            keep += [sv, tv]
+            ts = TimeSeries(sv.ctypes.data, len(sv), series.get("dt", 0.0), series.get("factor", 1.0),
+                            series.get("start", 0.0), int(bool(series.get("use_last", False))),
+                            None if tv is None else tv.ctypes.data, None)
         d = CdDesc(dt, nsteps, dash.ptr if dash is not None else None, len(le), le.ctypes.data if len(le) else None,
-                   lh.ctypes.data if len(lh) else None)
+                   lh.ctypes.data if len(lh) else None, scheme, ground_dir, None if ga is None else ga.ctypes.data,
+                   None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time)
         _check(lib().fx_explicit_cd(self.h, C.byref(d), u.ptr, um.ptr, v.ptr, a.ptr), self.ctx)
 
 
@@ -668,10 +715,11 @@ class ColumnPool:
     def __init__(self, ctx, max_elements, max_columns, dt, max_materials=None, max_record=None, gamma=0.5,
                  beta=0.25, test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, graph=True, mixed=False,
                  inner_rtol=1e-5, compact=False, waves=0, one_stream=False, max_retries=0, num_sublevels=0,
-                 num_substeps=2):
+                 num_substeps=2, tune=False):
         self.ctx = ctx
         flags = ((0 if graph else FX_COLUMN_NO_GRAPH) | (FX_COLUMN_MIXED if mixed else 0) |
-                 (FX_COLUMN_COMPACT if compact else 0) | (int(waves) << 8) | (8 if one_stream else 0))
+                 (FX_COLUMN_COMPACT if compact else 0) | (int(waves) << 8) | (8 if one_stream else 0) |
+                 (FX_COLUMN_TUNE if tune else 0))
         d = ColumnPoolDesc(int(max_elements), int(max_columns), int(max_materials or 16 * max_columns),
                            int(max_record or 1 << 20), gamma, beta, dt, test, tol, max_iter, cg_rtol, cg_max_it, flags,
                            inner_rtol, max_retries, num_sublevels, num_substeps)

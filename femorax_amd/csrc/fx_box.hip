@@ -230,18 +230,30 @@ struct TileFetch {
   int pidx[TileGeom<TZ>::kPasses];
 };
 
+// A strictly-interior node (x, y, z in 1..7, 1..7, 1..TZ-1) of a regular tile: its word from the
+// tile's interior base (three free DOFs, interior order x fastest), or 0xffffffff.
+template <int TZ>
+__device__ __forceinline__ uint32_t interior_word(int ib, int ln) {
+  const int x = ln % (kTileX + 1), y = (ln / (kTileX + 1)) % (kTileY + 1), z = ln / kPlane;
+  if (ib < 0 || x < 1 || x >= kTileX || y < 1 || y >= kTileY || z < 1 || z >= TZ) return 0xffffffffu;
+  return (7u << 28) | static_cast<uint32_t>(ib + 3 * ((x - 1) + (kTileX - 1) * ((y - 1) + (kTileY - 1) * (z - 1))));
+}
+
 template <typename T, bool DIAG, int TZ>
 __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t, TileFetch<T, TZ>& f) {
   using G = TileGeom<TZ>;
   const uint32_t* words = a.words + static_cast<size_t>(tile) * G::kNodes;
   const int32_t* slots = a.slots + static_cast<size_t>(tile) * G::kNodes;
+  const int ib = __builtin_amdgcn_readfirstlane(a.tiles[tile].interior_base);
   uint32_t w[G::kRounds];
   int32_t s[G::kRounds];
 #pragma unroll
   for (int r = 0; r < G::kRounds; ++r) {
     const int ln = t + r * kTileThreads;
-    w[r] = ln < G::kNodes ? words[ln] : 0u;
-    s[r] = ln < G::kNodes ? slots[ln] : 0;
+    const uint32_t iw = interior_word<TZ>(ib, ln);
+    const bool load = ln < G::kNodes && iw == 0xffffffffu;
+    w[r] = load ? words[ln] : ln < G::kNodes ? iw : 0u;
+    s[r] = load ? slots[ln] : 0;
   }
 #pragma unroll
   for (int p = 0; p < G::kPasses; ++p) f.pidx[p] = a.cell_param[static_cast<size_t>(tile) * G::kCells + t + kTileThreads * p];
@@ -558,11 +570,19 @@ __global__ __launch_bounds__(256) void hex8_shared_sum_kernel(const uint32_t* __
       j += (mask >> d) & 1u;
     }
   }
-  for (int t = 1; t < k; ++t) {
-    const T* p = partial + (static_cast<size_t>(s0) + static_cast<size_t>(t) * stride) * 3;
+  // the other tiles' partials (at most 8 tiles touch a node): loads first, then the adds in
+  // colour order
+  T v[7][3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) acc[d] += p[d];
-  }
+  for (int t = 1; t < 8; ++t)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+      v[t - 1][d] = t < k ? partial[(static_cast<size_t>(s0) + static_cast<size_t>(t) * stride) * 3 + d] : T(0);
+#pragma unroll
+  for (int t = 1; t < 8; ++t)
+    if (t < k)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) acc[d] += v[t - 1][d];
   int j = 0;
 #pragma unroll
   for (int d = 0; d < 3; ++d)

@@ -50,12 +50,36 @@ void dev_free(fx_ctx* ctx, void* p, uint64_t& bytes, size_t sz) {
   if (!p) return;
   (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, sz);
-  (void)bytes;
+  bytes -= std::min<uint64_t>(bytes, sz);
 }
 
 }  // namespace
 
 void fx::set_thread_error(const std::string& msg) { g_error = msg; }
+
+// The gather of a batch's element slots k * padded + e: each equation's slots in (element, DOF)
+// order (SlotGather's pairs, internal_force.cpp:121-134), and the equations with more than 64 of
+// them (gathered one wave each).
+static void build_gather(int32_t count, int32_t P, const int32_t* eq, int32_t neq, std::vector<int32_t>& start,
+                  std::vector<int32_t>& list, std::vector<int32_t>& long_eqs) {
+  start.assign(static_cast<size_t>(neq) + 1, 0);
+  for (int32_t e = 0; e < count; ++e)
+    for (int k = 0; k < 24; ++k) {
+      const int32_t q = eq[static_cast<size_t>(k) * P + e];
+      if (q >= 0) ++start[q + 1];
+    }
+  for (int32_t q = 0; q < neq; ++q) start[q + 1] += start[q];
+  long_eqs.clear();
+  for (int32_t q = 0; q < neq; ++q)
+    if (start[q + 1] - start[q] > 64) long_eqs.push_back(q);
+  list.assign(start[neq], 0);
+  std::vector<int32_t> fill(start.begin(), start.end() - 1);
+  for (int32_t e = 0; e < count; ++e)
+    for (int k = 0; k < 24; ++k) {
+      const int32_t q = eq[static_cast<size_t>(k) * P + e];
+      if (q >= 0) list[fill[q]++] = k * P + e;
+    }
+}
 
 extern "C" {
 
@@ -221,16 +245,78 @@ void fx_batch_destroy(fx_batch* b) {
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
-                  b->cres, b->clist, b->ccount};
+                  b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
   delete b;
 }
 
+namespace {
+constexpr uint32_t kDepthFlags = FX_BATCH_TILE4 | FX_BATCH_TILE8 | FX_BATCH_TILE16;
+constexpr int kTuneMinElements = 32768;
+
+// The model half of the autotune key (tune_key, force_settings.cpp:157-177): lattice size and the
+// operator forms' terms.
+std::string lattice_model_key(const fx_batch_desc* d, int forced_depth) {
+  bool mass = false, damping = false;
+  for (int32_t i = 0; i < d->nparam; ++i) {
+    mass |= d->params[i].rho != 0.0;
+    const fx_rayleigh& r = d->params[i].damp;
+    damping |= r.alphaM != 0.0 || r.betaK != 0.0 || r.betaKinit != 0.0 || r.betaKcomm != 0.0;
+  }
+  return fx::tune_model_key(d->count, mass, damping, forced_depth);
+}
+
+uint32_t depth_flag(int depth) {
+  return depth == 4 ? FX_BATCH_TILE4 : depth == 16 ? FX_BATCH_TILE16 : FX_BATCH_TILE8;
+}
+}  // namespace
+
 int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batch** out) {
   if (!ctx || !d || !out) return fail(ctx, FX_EINVAL, "fx_batch_create: null argument");
   *out = nullptr;
+  // Tile depth by autotune (choose_force_settings, force_settings.cpp:179-205): a large lattice
+  // without a depth flag tries two depths (8 and 16 cells from 2^20 cells up, 4 and 8 below), each
+  // with its variant autotune, and keeps the faster; the choice is kept in the autotune cache per
+  // device and model, so a known model is built once, with the kept depth and variant.
+  if (!(flags & (kDepthFlags | FX_BATCH_NO_TUNE | FX_BATCH_GENERAL)) && d->count >= kTuneMinElements && d->params &&
+      d->nparam > 0) {
+    const std::string key = fx::tune_machine_key(ctx->device) + "|" + lattice_model_key(d, 0);
+    int depth = 0, variant = -1;
+    if (fx::tune_cache_get(key, depth, variant)) {
+      fx_batch* b = nullptr;
+      if (int rc = fx_batch_create(ctx, d, flags | depth_flag(depth) | FX_BATCH_NO_TUNE, &b)) return rc;
+      if (b->lattice) fx_batch_tune(b, variant);
+      *out = b;
+      return FX_OK;
+    }
+    const int cand[2] = {d->count >= (1 << 20) ? 8 : 4, d->count >= (1 << 20) ? 16 : 8};
+    fx_batch* best = nullptr;
+    int best_depth = 0;
+    for (int c : cand) {
+      fx_batch* b = nullptr;
+      if (int rc = fx_batch_create(ctx, d, flags | depth_flag(c), &b)) {
+        fx_batch_destroy(best);
+        return rc;
+      }
+      if (!b->lattice) {  // not a lattice: no depth to choose
+        fx_batch_destroy(best);
+        *out = b;
+        return FX_OK;
+      }
+      if (!best || b->tuned_ms < best->tuned_ms) {
+        fx_batch_destroy(best);
+        best = b;
+        best_depth = c;
+      } else {
+        fx_batch_destroy(b);
+      }
+    }
+    fx::tune_cache_put(key, best_depth, best->box_variant);
+    *out = best;
+    return FX_OK;
+  }
   if (d->count < 0 || d->padded < d->count || d->nparam <= 0 || d->nnode <= 0 || d->neq < 0 || !d->node ||
       !d->param || !d->params || !d->coords || (!d->eq))
     return fail(ctx, FX_EINVAL, "fx_batch_create: bad descriptor");
@@ -462,6 +548,8 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
       b->host_param.assign(d->param, d->param + P);
       b->host_coords.assign(d->coords, d->coords + static_cast<size_t>(3) * N);
     }
+    b->host_lm = lm;
+    b->host_la = la;
     ok(n > 0 ? dev_upload(ctx, b->lumped_m, lm.data(), lm.size(), by) : FX_OK) &&
         ok(any_am ? dev_upload(ctx, b->lumped_am, la.data(), la.size(), by) : FX_OK) &&
         ok(any_body ? dev_upload(ctx, b->body, fb.data(), fb.size(), by) : FX_OK);
@@ -473,10 +561,31 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
         if (q >= 0) b->eq_dir[q] = static_cast<int8_t>(k % 3);
       }
+    {
+      std::vector<int8_t> dir(static_cast<size_t>(std::max(1, d->neq)), -1);
+      for (int32_t q = 0; q < d->neq; ++q) dir[b->internal ? b->internal_of[q] : q] = b->eq_dir[q];
+      ok(dev_upload(ctx, b->eq_dir_dev, dir.data(), dir.size(), by));
+    }
     if (flags & FX_BATCH_DYNAMIC) ok(dev_zeros(ctx, b->dyn, static_cast<size_t>(13) * std::max(1, d->neq), by));
     if (flags & FX_BATCH_GMRES)
       ok(dev_zeros(ctx, b->gm,
                    fx::gmres_workspace(d->neq), by));
+  }
+  if (rc == FX_OK && (flags & FX_BATCH_SLOTS)) {
+    // fx_run_operator's per-element forms: the connectivity the lattice path does not keep, an
+    // element slot buffer and its gather (reference numbering)
+    if (use_lattice)
+      ok(dev_upload(ctx, b->node, d->node, static_cast<size_t>(8) * P, by)) &&
+          ok(dev_upload(ctx, b->param, d->param, static_cast<size_t>(P), by)) &&
+          ok(dev_upload(ctx, b->node_eq, node_eq.data(), node_eq.size(), by)) &&
+          ok(dev_upload(ctx, b->coords, d->coords, static_cast<size_t>(3) * N, by));
+    std::vector<int32_t> start, list, long_eqs;
+    build_gather(E, P, d->eq, d->neq, start, list, long_eqs);
+    ok(dev_zeros(ctx, b->slots, static_cast<size_t>(24) * std::max(1, P), by)) &&
+        ok(dev_upload(ctx, b->gather_start, start.data(), start.size(), by)) &&
+        ok(list.empty() ? FX_OK : dev_upload(ctx, b->gather_list, list.data(), list.size(), by)) &&
+        ok(long_eqs.empty() ? FX_OK : dev_upload(ctx, b->long_eqs, long_eqs.data(), long_eqs.size(), by));
+    b->nlong = static_cast<int32_t>(long_eqs.size());
   }
   if (rc != FX_OK) {
     fx_batch_destroy(b);
@@ -492,11 +601,19 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   b->info.ncolors = blk.ncolors;
   b->info.tied = b->tied ? 1 : 0;
   b->info.lattice = b->lattice ? 1 : 0;
-  if (b->lattice && !(flags & FX_BATCH_NO_TUNE) && E >= 32768) {
-    const int rc2 = fx_batch_tune(b, -1);
-    if (rc2 != FX_OK) {
-      fx_batch_destroy(b);
-      return rc2;
+  if (b->lattice && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {
+    // the variant autotune, kept per device and model (with this tile depth) in the cache
+    const std::string key = fx::tune_machine_key(ctx->device) + "|" + lattice_model_key(d, b->tile_z);
+    int depth = 0, variant = -1;
+    if (fx::tune_cache_get(key, depth, variant) && depth == b->tile_z) {
+      fx_batch_tune(b, variant);
+    } else {
+      const int rc2 = fx_batch_tune(b, -1);
+      if (rc2 != FX_OK) {
+        fx_batch_destroy(b);
+        return rc2;
+      }
+      fx::tune_cache_put(key, b->tile_z, b->box_variant);
     }
   }
   b->info.variant = b->box_variant;
@@ -571,6 +688,7 @@ int fx_batch_tune(fx_batch* b, int variant) {
     if (vbest[v] + mbest[v] < vbest[best_v] + mbest[best_v]) best_v = v;
   b->box_variant = best_v;
   b->info.variant = best_v;
+  b->tuned_ms = vbest[best_v] + mbest[best_v];
   return FX_OK;
 }
 
@@ -578,6 +696,7 @@ int fx_batch_info_get(const fx_batch* b, fx_batch_info* info) {
   if (!b || !info) return fail(nullptr, FX_EINVAL, "fx_batch_info_get: null argument");
   *info = b->info;
   info->device_bytes = b->device_bytes;
+  info->tangent_symmetry = b->tangent_symmetry;
   return FX_OK;
 }
 
@@ -598,6 +717,129 @@ int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float
 int fx_operator_diagonal(fx_batch* b, const fx_coeffs* k, double* d) {
   if (!b || !k || !d) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_operator_diagonal: null argument");
   return fx::external_diagonal64(b, *k, d);
+}
+
+int fx_run_operator(fx_batch* b, int diagonal, const fx_coeffs* k, int out, const double* x, double* y, double* slots,
+                    int64_t slot_offset, int unit_dir) {
+  if (!b || !k) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_run_operator: null argument");
+  fx_ctx* ctx = b->ctx;
+  if (out != FX_OUT_PLAIN && out != FX_OUT_ATOMIC && out != FX_OUT_SLOTS)
+    return fail(ctx, FX_EINVAL, "fx_run_operator: out must be FX_OUT_PLAIN, FX_OUT_ATOMIC or FX_OUT_SLOTS");
+  if (unit_dir < -1 || unit_dir > 2) return fail(ctx, FX_EINVAL, "fx_run_operator: unit_dir outside -1..2");
+  if (diagonal != 0 && diagonal != 1) return fail(ctx, FX_EINVAL, "fx_run_operator: diagonal must be 0 or 1");
+  if (out == FX_OUT_SLOTS && (!slots || slot_offset < 0))
+    return fail(ctx, FX_EINVAL, "fx_run_operator: FX_OUT_SLOTS needs slots and slot_offset >= 0");
+  if (out != FX_OUT_SLOTS && !y) return fail(ctx, FX_EINVAL, "fx_run_operator: null y");
+  if (!diagonal && unit_dir < 0 && !x) return fail(ctx, FX_EINVAL, "fx_run_operator: null x");
+  if (!diagonal && unit_dir < 0 && out != FX_OUT_SLOTS && x == y)
+    return fail(ctx, FX_EINVAL, "fx_run_operator: x and y alias");
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  const bool per_element = out == FX_OUT_SLOTS || unit_dir >= 0 || b->pimy;
+  if (per_element && !b->slots)
+    return fail(ctx, FX_EINVAL,
+                "fx_run_operator: FX_OUT_SLOTS and unit_dir >= 0 need the per-element forms (create the batch with "
+                "FX_BATCH_SLOTS)");
+  int rc = FX_OK;
+  if (per_element) {
+    double* dst = out == FX_OUT_SLOTS ? slots + slot_offset : b->slots;
+    rc = b->pimy ? fx::pimy_slots64(b, diagonal != 0, *k, x, unit_dir, dst)
+                 : fx::element_slots64(b, diagonal != 0, *k, x, unit_dir, dst);
+    if (!rc && out != FX_OUT_SLOTS) rc = fx::slots_gather64(b, b->slots, y, 1);
+  } else {
+    rc = fx::add_elements64(b, diagonal != 0, *k, x, y);
+  }
+  if (rc) return rc;
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
+}
+
+int fx_batch_set_nodal_mass(fx_batch* b, const double* mass, const double* alpha_mass) {
+  if (!b) return fail(nullptr, FX_EINVAL, "fx_batch_set_nodal_mass: null batch");
+  fx_ctx* ctx = b->ctx;
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  const int32_t n = b->neq;
+  const size_t N = static_cast<size_t>(n);
+  uint64_t& by = b->device_bytes;
+  for (double** p : {&b->nodal_m, &b->nodal_am}) dev_free(ctx, *p, by, sizeof(double) * N), *p = nullptr;
+  for (float** p : {&b->nodal_m32, &b->nodal_am32}) dev_free(ctx, *p, by, sizeof(float) * N), *p = nullptr;
+  // the solver numbering of the batch (internal_of maps a reference equation to it)
+  auto to_solver = [&](const double* src, std::vector<double>& dst) {
+    dst.assign(N, 0.0);
+    for (int32_t q = 0; q < n; ++q) {
+      if (!std::isfinite(src[q]) || src[q] < 0.0) return false;
+      dst[b->internal ? b->internal_of[q] : q] = src[q];
+    }
+    return true;
+  };
+  std::vector<double> m, am;
+  if (mass && !to_solver(mass, m)) return fail(ctx, FX_EINVAL, "fx_batch_set_nodal_mass: negative or non-finite mass");
+  if (alpha_mass && !to_solver(alpha_mass, am))
+    return fail(ctx, FX_EINVAL, "fx_batch_set_nodal_mass: negative or non-finite alphaM mass");
+  int rc = FX_OK;
+  auto ok = [&](int r) {
+    if (r != FX_OK && rc == FX_OK) rc = r;
+    return rc == FX_OK;
+  };
+  auto any = [](const std::vector<double>& v) {
+    for (double x : v)
+      if (x != 0.0) return true;
+    return false;
+  };
+  const bool has_m = any(m), has_am = any(am);
+  if (has_m) {
+    ok(dev_upload(ctx, b->nodal_m, m.data(), N, by));
+    if (b->has_f32) {
+      std::vector<float> m32(m.begin(), m.end());
+      ok(dev_upload(ctx, b->nodal_m32, m32.data(), N, by));
+    }
+  }
+  if (has_am) {
+    ok(dev_upload(ctx, b->nodal_am, am.data(), N, by));
+    if (b->has_f32) {
+      std::vector<float> a32(am.begin(), am.end());
+      ok(dev_upload(ctx, b->nodal_am32, a32.data(), N, by));
+    }
+  }
+  // the lumped masses: the elements' sums, then the nodal masses (batches.cpp:249-261)
+  if (rc == FX_OK && n > 0) {
+    std::vector<double> lm = b->host_lm, la = b->host_la;
+    bool any_am = false;
+    for (size_t q = 0; q < N; ++q) {
+      if (has_m) lm[q] += m[q];
+      if (has_am) la[q] += am[q];
+      any_am |= la[q] != 0.0;
+    }
+    FX_HIP(ctx, hipMemcpy(b->lumped_m, lm.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+    if (any_am && !b->lumped_am) ok(dev_zeros(ctx, b->lumped_am, N, by));
+    if (rc == FX_OK && b->lumped_am)
+      FX_HIP(ctx, hipMemcpy(b->lumped_am, la.data(), sizeof(double) * N, hipMemcpyHostToDevice));
+  }
+  fx::release_cg_graphs(ctx, b);  // captured applies would miss the nodal terms
+  b->info.device_bytes = by;
+  return rc;
+}
+
+int fx_mass_times_ground(fx_batch* b, int direction, double* y) {
+  if (!b || !y || direction < 0 || direction > 2)
+    return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_mass_times_ground: bad argument");
+  fx_ctx* ctx = b->ctx;
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  // The element mass times a rigid translation is its row sums rho sum_gp N_j dvol (sum_k N_k = 1),
+  // so the elements' M r is their lumped mass at the direction's equations; the nodal masses of
+  // the direction follow (internal_force.cpp:619-625), as the lumped mass holds them.
+  const size_t N = static_cast<size_t>(b->neq);
+  std::vector<double> r(N, 0.0);
+  for (int32_t q = 0; q < b->neq; ++q)
+    if (b->eq_dir[q] == direction) r[b->internal ? b->internal_of[q] : q] = 1.0;
+  if (N == 0) return FX_OK;
+  double* scr = b->work + 6 * N;
+  double* mr = b->work + 7 * N;
+  FX_HIP(ctx, hipMemcpyAsync(scr, r.data(), sizeof(double) * N, hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = fx::multiply64(ctx, b->neq, b->lumped_m, scr, mr)) return rc;
+  if (int rc = fx::to_external<double>(b, mr, y)) return rc;
+  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return FX_OK;
 }
 
 int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void* x, void* y, int reps, float* ms) {
@@ -759,8 +1001,27 @@ int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
 int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, double* v, double* a) {
   if (!b || !d || !u || !um || !v || !a) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_explicit_cd: null argument");
   if (!(d->dt > 0) || d->nsteps < 0 || d->nload < 0 || d->nload > FX_MAX_POINT_LOADS ||
-      (d->nload > 0 && (!d->load_eq || !d->load_hist)))
+      (d->nload > 0 && (!d->load_eq || !d->load_hist)) || (d->scheme != 0 && d->scheme != 1) || d->ground_dir < -1 ||
+      d->ground_dir > 2 || (d->ground_dir >= 0 && !d->ground_accel && !d->ground_series))
     return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: bad descriptor");
+  if (const fx_time_series* gs = d->ground_series) {
+    if (!gs->values || gs->count < 1 || (!gs->time && !(gs->dt > 0)))
+      return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: bad ground_series");
+    if (gs->time) {
+      for (int32_t i = 0; i < gs->count; ++i)
+        if (!std::isfinite(gs->time[i]) || (i > 0 && gs->time[i] < gs->time[i - 1]))
+          return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: ground_series times must be finite and nondecreasing");
+      if (gs->loc && (*gs->loc < 0 || *gs->loc >= gs->count))
+        return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: ground_series interval (*loc) out of range");
+    }
+  }
+  if (d->scheme == 1 && b->neq > 0) {  // check_masses: a = R / m needs a mass at every equation
+    std::vector<double> m(static_cast<size_t>(b->neq));
+    FX_HIP(b->ctx, hipMemcpy(m.data(), b->lumped_m, sizeof(double) * m.size(), hipMemcpyDeviceToHost));
+    for (double x : m)
+      if (!(x > 0.0))
+        return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: ExplicitDifference needs a positive lumped mass at every equation");
+  }
   for (int j = 0; j < d->nload; ++j)
     if (d->load_eq[j] < 0 || d->load_eq[j] >= b->neq) return fail(b->ctx, FX_EINVAL, "fx_explicit_cd: load equation out of range");
   FX_HIP(b->ctx, hipSetDevice(b->ctx->device));
@@ -857,33 +1118,21 @@ int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flag
   ed.coords = d->coords;
   ed.neq = d->neq;
   fx_batch* b = nullptr;
-  if (int rc = fx_batch_create(ctx, &ed, (flags & ~FX_BATCH_FP32) | FX_BATCH_GENERAL | FX_BATCH_NO_TUNE | FX_BATCH_DYNAMIC,
+  if (int rc = fx_batch_create(ctx, &ed,
+                               (flags & ~(FX_BATCH_FP32 | FX_BATCH_SLOTS)) | FX_BATCH_GENERAL | FX_BATCH_NO_TUNE |
+                                   FX_BATCH_DYNAMIC,
                                &b))
     return rc;
   b->pimy = true;
+  b->tangent_symmetry = FX_TANGENT_SPSD;
   b->host_pimy = pp;
   for (const auto& q : pp)
     b->pimy_damped |= q.alphaM != 0.0 || q.betaK != 0.0 || q.betaKinit != 0.0 || q.betaKcomm != 0.0;
   b->npts = static_cast<int64_t>(d->count) * 8;
   // the gather: each equation's slots k * padded + e in (element, DOF) order
-  const int32_t P = d->padded, n = d->neq;
-  std::vector<int32_t> start(static_cast<size_t>(n) + 1, 0), list;
-  for (int32_t e = 0; e < d->count; ++e)
-    for (int k = 0; k < 24; ++k) {
-      const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
-      if (q >= 0) ++start[q + 1];
-    }
-  for (int32_t q = 0; q < n; ++q) start[q + 1] += start[q];
-  std::vector<int32_t> long_eqs;  // gathered one wave each
-  for (int32_t q = 0; q < n; ++q)
-    if (start[q + 1] - start[q] > 64) long_eqs.push_back(q);
-  list.resize(start[n]);
-  std::vector<int32_t> fill(start.begin(), start.end() - 1);
-  for (int32_t e = 0; e < d->count; ++e)
-    for (int k = 0; k < 24; ++k) {
-      const int32_t q = d->eq[static_cast<size_t>(k) * P + e];
-      if (q >= 0) list[fill[q]++] = k * P + e;
-    }
+  const int32_t P = d->padded;
+  std::vector<int32_t> start, list, long_eqs;  // long equations: gathered one wave each
+  build_gather(d->count, P, d->eq, d->neq, start, list, long_eqs);
   uint64_t& by = b->device_bytes;
   int rc = FX_OK;
   auto ok = [&](int r) {
@@ -959,9 +1208,36 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
                fx_newmark_result* result) {
   if (!b || !d || !u || !v || !a || !du || !result) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_newmark: null argument");
   if (!b->dyn) return fail(b->ctx, FX_EINVAL, "fx_newmark: the batch was created without FX_BATCH_DYNAMIC");
-  if (!(d->dt > 0) || !(d->beta > 0) || d->nsteps < 0 || d->max_iter <= 0 || d->ground_dir > 2 || d->test < 0 ||
-      d->test > 2 || (d->ground_dir >= 0 && !d->ground_accel && !d->ground_series))
+  const bool is_static = d->analysis == 1;
+  if ((d->analysis != 0 && d->analysis != 1) || (!is_static && (!(d->dt > 0) || !(d->beta > 0))) || d->nsteps < 0 ||
+      d->max_iter <= 0 || d->ground_dir > 2 || d->test < 0 || d->test > 2 ||
+      (d->ground_dir >= 0 && !d->ground_accel && !d->ground_series))
     return fail(b->ctx, FX_EINVAL, "fx_newmark: bad descriptor");
+  if (d->algorithm < 0 || d->algorithm > 2)
+    return fail(b->ctx, FX_EINVAL, "fx_newmark: algorithm must be 0 (Newton), 1 (Linear) or 2 (ModifiedNewton)");
+  if (d->algorithm == 1 && b->pimy)  // implicit_stage.cpp:130-140
+    return fail(b->ctx, FX_EUNSUP,
+                "fx_newmark: the Linear algorithm with a material that has history state (OpenSees would commit a "
+                "stress inconsistent with the strain); use Newton or ModifiedNewton");
+  if (is_static) {
+    const bool adapt = d->load_min_increment != 0.0 && d->load_max_increment != 0.0;
+    if (!std::isfinite(d->load_increment) || (adapt && (!(d->load_min_increment <= d->load_max_increment) ||
+                                                        d->load_num_iter <= 0)))
+      return fail(b->ctx, FX_EINVAL, "fx_newmark: bad LoadControl increment, bounds or numIter");
+  }
+  {
+    const double h = is_static ? 1.0 : d->dt;
+    const fx_coeffs k = is_static ? fx_coeffs{1.0, 0.0, 0.0, 0}
+                                  : fx_coeffs{1.0, d->gamma / (d->beta * h), 1.0 / (d->beta * h * h), 0};
+    if (!fx::operator_spd(b, k)) {
+      if (d->mixed)
+        return fail(b->ctx, FX_EINVAL, "fx_newmark: mixed-precision CG needs a symmetric positive definite operator");
+      if (!b->gm)
+        return fail(b->ctx, FX_EINVAL,
+                    "fx_newmark: the stage's operator is not symmetric positive definite (a history material in a "
+                    "static stage), so its solver is GMRES: create the batch with FX_BATCH_GMRES");
+    }
+  }
   if (d->max_retries < 0 || d->num_sublevels < 0 || (d->num_sublevels > 0 && d->num_substeps < 1))
     return fail(b->ctx, FX_EINVAL, "fx_newmark: max_retries and num_sublevels must be non-negative, num_substeps positive");
   if (d->ground_dir >= 0 && d->num_sublevels > 0 && !d->ground_series)

@@ -753,6 +753,9 @@ struct fx_column_pool {
   int32_t* count_host = nullptr;  // pinned copy of the same
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  int wpe = 3;                    // the return-map kernels' register budget (waves per SIMD)
+  bool tuned = true;              // false until FX_COLUMN_TUNE has chosen wpe
+  bool return_map_only = false;   // launch_iteration stops after the return maps (the tuner's samples)
   hipStream_t side[4] = {nullptr, nullptr, nullptr, nullptr};  // default priority
   hipEvent_t fork = nullptr, join[4] = {nullptr, nullptr, nullptr, nullptr};
 };
@@ -786,8 +789,7 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
   const PoolArgs& a = p->args;
   const unsigned slots = static_cast<unsigned>((a.cap + 255) / 256);
   // register budget of the return-map kernels: waves per SIMD the compiler must allow
-  const uint32_t bits = (p->d.flags >> 8) & 0xfu;
-  const int wpe = bits ? static_cast<int>(bits) : 3;  // 3: measured best on C4
+  const int wpe = p->wpe;
   auto return_map = [&](auto w) {
     constexpr int W = decltype(w)::value;
     if (p->d.flags & FX_COLUMN_COMPACT) {
@@ -801,6 +803,7 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
   if (wpe >= 4) return_map(std::integral_constant<int, 4>{});
   else if (wpe == 3) return_map(std::integral_constant<int, 3>{});
   else return_map(std::integral_constant<int, 1>{});
+  if (p->return_map_only) return FX_OK;
   // the four solve sizes touch disjoint columns: forked onto side streams so they overlap
   // (graph branches when captured), joined back before the next iteration
   const unsigned nc = static_cast<unsigned>(a.max_columns);
@@ -827,6 +830,79 @@ int launch_iteration(fx_column_pool* p, hipStream_t st) {
     }
   }
   FX_HIP(p->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+}  // namespace
+}  // namespace fx
+
+namespace fx {
+namespace {
+
+// The iteration captured once as a hipGraph (the forks as branches), unless FX_COLUMN_NO_GRAPH.
+int capture_iteration(fx_column_pool* p) {
+  if (p->d.flags & FX_COLUMN_NO_GRAPH) return FX_OK;
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(ctx, FX_EDEVICE, "hipStreamBeginCapture failed");
+  const int lrc = launch_iteration(p, st);
+  const hipError_t ec = hipStreamEndCapture(st, &p->graph);
+  if (lrc || ec != hipSuccess || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess)
+    return fail(ctx, FX_EDEVICE, "column pool: graph capture failed");
+  return FX_OK;
+}
+
+// FX_COLUMN_TUNE: the return maps alone (a pure function of the columns' displacements and
+// committed histories: a repeat writes the same trial histories, element results and copy
+// bounds, so the iteration that follows is unchanged) timed for each register budget on the
+// running columns, the fastest kept, per device and pool in the autotune cache (the reference's
+// choose_force_settings, force_settings.cpp:179-205).
+int tune_return_map(fx_column_pool* p) {
+  fx_ctx* ctx = p->ctx;
+  hipStream_t st = ctx->stream;
+  const int64_t cap = p->args.cap;
+  const int log2cap = cap > 0 ? static_cast<int>(std::floor(std::log2(static_cast<double>(cap)))) : 0;
+  const std::string key = tune_machine_key(ctx->device) + "|column_pool,slots=2^" + std::to_string(log2cap) +
+                          ",compact=" + ((p->d.flags & FX_COLUMN_COMPACT) ? "1" : "0");
+  int depth = 0, v = -1;
+  if (tune_cache_get(key, depth, v) && (v == 1 || v == 3 || v == 4)) {
+    p->wpe = v;
+    return FX_OK;
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  FX_HIP(ctx, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(ctx, FX_EDEVICE, "hipEventCreate failed");
+  }
+  const int cand[3] = {3, 4, 1};
+  float best_ms = 1e30f;
+  int best = 3, rc = FX_OK;
+  p->return_map_only = true;
+  for (int round = 0; round < 2 && rc == FX_OK; ++round)
+    for (int w : cand) {
+      p->wpe = w;
+      if ((rc = launch_iteration(p, st))) break;  // warm-up
+      if (hipEventRecord(e0, st) != hipSuccess) rc = FX_EDEVICE;
+      for (int r = 0; r < 3 && rc == FX_OK; ++r) rc = launch_iteration(p, st);
+      float ms = 0.0f;
+      if (rc || hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        rc = rc ? rc : fail(ctx, FX_EDEVICE, "column pool: autotune timing failed");
+        break;
+      }
+      if (ms < best_ms) {
+        best_ms = ms;
+        best = w;
+      }
+    }
+  p->return_map_only = false;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  p->wpe = best;
+  tune_cache_put(key, 8, best);
   return FX_OK;
 }
 
@@ -956,17 +1032,13 @@ int fx_column_pool_create(fx_ctx* ctx, const fx_column_pool_desc* d, fx_column_p
   p->mats.reset(d->max_materials);
   p->recs.reset(d->max_record);
   p->host.assign(d->max_columns, HostColumn{});
-  if (!(d->flags & FX_COLUMN_NO_GRAPH)) {
-    hipStream_t st = ctx->stream;
-    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+  const uint32_t bits = (d->flags >> 8) & 0xfu;
+  p->wpe = bits ? static_cast<int>(bits) : 3;  // 3: measured best on C4
+  p->tuned = bits != 0 || !(d->flags & FX_COLUMN_TUNE);
+  if (p->tuned) {
+    if (int rc = capture_iteration(p)) {
       fx_column_pool_destroy(p);
-      return fail(ctx, FX_EDEVICE, "hipStreamBeginCapture failed");
-    }
-    const int lrc = launch_iteration(p, st);
-    const hipError_t ec = hipStreamEndCapture(st, &p->graph);
-    if (lrc || ec != hipSuccess || hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0) != hipSuccess) {
-      fx_column_pool_destroy(p);
-      return fail(ctx, FX_EDEVICE, "column pool: graph capture failed");
+      return rc;
     }
   }
   *out = p;
@@ -1063,6 +1135,15 @@ int fx_column_run(fx_column_pool* p, int32_t iterations, int32_t* running) {
   if (!p || iterations < 0) return fail(p ? p->ctx : nullptr, FX_EINVAL, "fx_column_run: bad argument");
   fx_ctx* ctx = p->ctx;
   hipStream_t st = ctx->stream;
+  if (!p->tuned && iterations > 0) {
+    bool any = false;
+    for (const auto& h : p->host) any |= h.used;
+    if (any) {  // the first running columns: choose the register budget, then capture
+      if (int rc = tune_return_map(p)) return rc;
+      p->tuned = true;
+      if (int rc = capture_iteration(p)) return rc;
+    }
+  }
   for (int32_t k = 0; k < iterations; ++k) {
     if (p->exec) FX_HIP(ctx, hipGraphLaunch(p->exec, st));
     else if (int rc = launch_iteration(p, st)) return rc;

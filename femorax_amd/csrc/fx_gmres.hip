@@ -160,6 +160,15 @@ int gmres64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, c
 int krylov64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
                     fx_solve_result& res) {
   const int64_t n = b->neq;
+  if (!operator_spd(b, k)) {  // KrylovSolver::setup (krylov.cpp:10-22): GMRES from the start
+    if (!b->gm) {
+      b->ctx->error = "Krylov solve: the operator is not symmetric positive definite, so it needs GMRES (create "
+                      "the batch with FX_BATCH_GMRES)";
+      return FX_EINVAL;
+    }
+    res = fx_solve_result{};
+    return gmres64_solver(b, k, o, bi, xi, res);
+  }
   double* x0 = b->gm ? b->gm + (2 * kGmresRestart + 2) * n + kGmresRestart + 1 : nullptr;
   if (x0)
     if (int rc = copy64(b->ctx, n, xi, x0)) return rc;

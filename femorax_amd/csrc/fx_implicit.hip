@@ -46,17 +46,32 @@ __global__ void newmark_update_kernel(int32_t n, const double* du, double* u, do
   a[q] += du[q] * c3;
 }
 
+__global__ void axpy1_kernel(int32_t n, const double* __restrict__ du, double* __restrict__ u) {
+  const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) u[q] += du[q];
+}
+
 __global__ void mul_kernel(int32_t n, const double* __restrict__ x, const double* __restrict__ y, double* __restrict__ z) {
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q < n) z[q] = x[q] * y[q];
 }
+
+}  // namespace
+
+int multiply64(fx_ctx* ctx, int32_t n, const double* x, const double* y, double* z) {
+  if (n > 0) hipLaunchKernelGGL(mul_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, n, x, y, z);
+  FX_HIP(ctx, hipGetLastError());
+  return FX_OK;
+}
+
+namespace {
 
 __global__ void newmark_residual_kernel(int32_t n, const double* f, const double* ma, const double* mr, double ag,
                                         double* b) {
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   double r = -f[q];
-  r -= ma[q];
+  if (ma) r -= ma[q];
   if (mr) r -= ag * mr[q];
   b[q] = r;
 }
@@ -159,13 +174,25 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     FX_HIP(ctx, hipGetLastError());
   }
   const double gamma = d.gamma, beta = d.beta;
+  const bool is_static = d.analysis == 1;
+  const bool linear = d.algorithm == 1, modified = d.algorithm == 2;
+  const bool factor_once = modified && d.factor_once, initial = modified && d.modified_initial;
   fx_solve_options so{d.cg_rtol, 0.0, d.cg_max_it};
   double ag = 0.0;
+  // the operator's coefficients per step size (:146-150): static K (or K_0), transient
+  // K + gamma/(beta h) C + 1/(beta h^2) M
+  auto coefficients = [&](double h) -> fx_coeffs {
+    if (is_static) return fx_coeffs{1.0, 0.0, 0.0, initial ? 1 : 0};
+    return fx_coeffs{1.0, gamma / (beta * h), 1.0 / (beta * h * h), initial ? 1 : 0};
+  };
+  // form_unbalance (:212-219): B = -f(u, v) - M a - a_g M r; static stages: the elements see v = 0,
+  // no inertia
   auto unbalance = [&]() -> int {
-    if (int rc = force64(b, U, V, F, SCR)) return rc;
-    if (int rc = operator_apply64(b, mass, A, MA)) return rc;
-    hipLaunchKernelGGL(newmark_residual_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, F, MA,
-                       ground ? MR : nullptr, ag, B);
+    if (int rc = force64(b, U, is_static ? nullptr : V, F, SCR)) return rc;
+    if (!is_static)
+      if (int rc = operator_apply64(b, mass, A, MA)) return rc;
+    hipLaunchKernelGGL(newmark_residual_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, F,
+                       is_static ? nullptr : MA, ground ? MR : nullptr, ag, B);
     FX_HIP(ctx, hipGetLastError());
     return FX_OK;
   };
@@ -182,54 +209,113 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     val = std::sqrt(s);
     return FX_OK;
   };
+  // update_tangent (:173-193): the tangents at u (stored tangents of history materials); the
+  // Jacobi diagonal is formed by each solve from the current tangents
+  double tangent_h = d.dt;
+  bool kept_tangent = false;
+  auto update_tangent = [&](double h) -> int {
+    tangent_h = h;
+    if (b->pimy)
+      if (int rc = pimy_force64(b, U, nullptr, SCR, true)) return rc;
+    return FX_OK;
+  };
+  // The solver KrylovSolver::setup picks (krylov.cpp:10-22): CG for an SPD operator (with the
+  // GMRES fallback when the workspace is there), GMRES otherwise; or the mixed-precision CG.
+  auto solve = [&](const fx_coeffs& k, fx_solve_result& r) -> int {
+    if (d.mixed) return cg_mixed_solver(b, k, so, d.inner_rtol, B, dU, r);
+    if (b->gm) return krylov64_solver(b, k, so, B, dU, r);
+    return cg64_solver(b, k, so, B, dU, r);
+  };
+  // LoadControl (:262-269): the increment, its bounds, numIter and the last attempt's iterations
+  double dlambda = d.load_increment;
+  const bool adapt = d.load_min_increment != 0.0 && d.load_max_increment != 0.0;
+  const double dmin = adapt ? d.load_min_increment : d.load_increment;
+  const double dmax = adapt ? d.load_max_increment : d.load_increment;
+  const double spec_iter = adapt ? static_cast<double>(d.load_num_iter) : 1.0;
+  double last_iter = spec_iter;
   int32_t step = 0;  // the step of size dt being recovered (its ground_accel entry)
-  // One attempt of size h (:274-372): predictor, unbalance, Newton; a failure restores u, v, a,
-  // the initial guess and the time.
+  // One attempt of size h (:274-372): predictor (transient) or the load increment (static),
+  // unbalance, the algorithm; a failure restores u (and v, a), the initial guess and the time.
   auto attempt = [&](double h, bool& ok) -> int {
     const double time0 = out.time;
-    const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
-    const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
-    hipLaunchKernelGGL(newmark_predict_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, U, V, A, Ut, Vt, At, a1,
-                       a2, a3, a4);
-    out.time += h;
+    if (is_static) {
+      dlambda *= spec_iter / last_iter;
+      if (dlambda < dmin) dlambda = dmin;
+      else if (dlambda > dmax) dlambda = dmax;
+      if (int rc = copy64(ctx, n, U, Ut)) return rc;
+      out.time += dlambda;
+      last_iter = 0;
+    } else {
+      const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
+      const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+      hipLaunchKernelGGL(newmark_predict_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, U, V, A, Ut, Vt, At,
+                         a1, a2, a3, a4);
+      out.time += h;
+    }
     if (int rc = copy64(ctx, n, dU, dU0)) return rc;  // the solver's initial guess, restored on failure
     ag = !ground ? 0.0 : d.ground_series ? series_value(*d.ground_series, d.ground_factor, out.time, loc) : d.ground_accel[step];
-    const fx_coeffs k{1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
-    if (int rc = unbalance()) return rc;
-    int iter = 1, iterations = 0;
+    const fx_coeffs k = coefficients(h);
+    auto update = [&]() -> int {
+      if (is_static)
+        hipLaunchKernelGGL(axpy1_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, dU, U);
+      else
+        hipLaunchKernelGGL(newmark_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, dU, U, V, A,
+                           k.damping, k.mass);
+      FX_HIP(ctx, hipGetLastError());
+      return FX_OK;
+    };
+    int iterations = 0;
     ok = true;
-    for (;;) {
-      if (b->pimy)  // update_tangent: the tangents at u
-        if (int rc = pimy_force64(b, U, nullptr, SCR, true)) return rc;
-      fx_solve_result r;
-      if (int rc = d.mixed  ? cg_mixed_solver(b, k, so, d.inner_rtol, B, dU, r)
-                   : b->gm ? krylov64_solver(b, k, so, B, dU, r)
-                           : cg64_solver(b, k, so, B, dU, r))
-        return rc;
-      out.krylov_iterations += r.iterations;
-      if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
-        ok = false;
-        break;
-      }
-      hipLaunchKernelGGL(newmark_update_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, dU, U, V, A, k.damping,
-                         k.mass);
+    if (linear) {  // :301-313: the tangent, the unbalance, one solve that must converge
+      if (int rc = update_tangent(h)) return rc;
       if (int rc = unbalance()) return rc;
-      double value = 0.0;
-      if (int rc = test_value(value)) return rc;
-      iterations = iter;
-      out.last_test = value;
-      if (value <= d.tol) break;
-      if (iter >= d.max_iter) {
-        ok = false;
-        break;
+      fx_solve_result r;
+      if (int rc = solve(k, r)) return rc;
+      out.krylov_iterations += r.iterations;
+      if (!r.converged) ok = false;
+      else if (int rc = update()) return rc;
+      iterations = 1;
+    } else {
+      if (int rc = unbalance()) return rc;
+      // ModifiedNewton: the tangent once per attempt, after the first unbalance; -factoronce: once
+      // while it stays valid (:324-328)
+      if (modified && !(factor_once && kept_tangent && h == tangent_h))
+        if (int rc = update_tangent(h)) return rc;
+      kept_tangent = factor_once;
+      int iter = 1;
+      for (;;) {
+        if (!modified)
+          if (int rc = update_tangent(h)) return rc;
+        fx_solve_result r;
+        if (int rc = solve(k, r)) return rc;
+        out.krylov_iterations += r.iterations;
+        if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
+          ok = false;
+          break;
+        }
+        if (int rc = update()) return rc;
+        if (int rc = unbalance()) return rc;
+        double value = 0.0;
+        if (int rc = test_value(value)) return rc;
+        iterations = iter;
+        out.last_test = value;
+        if (value <= d.tol) break;
+        if (iter >= d.max_iter) {
+          ok = false;
+          kept_tangent = false;  // OpenSees: factorOnce = 1 after a failed test
+          break;
+        }
+        ++iter;
       }
-      ++iter;
     }
     out.newton_iterations += iterations;
+    if (is_static) last_iter = iterations;  // LoadControl counts the failed attempt's updates too
     if (!ok) {
       if (int rc = copy64(ctx, n, Ut, U)) return rc;
-      if (int rc = copy64(ctx, n, Vt, V)) return rc;
-      if (int rc = copy64(ctx, n, At, A)) return rc;
+      if (!is_static) {
+        if (int rc = copy64(ctx, n, Vt, V)) return rc;
+        if (int rc = copy64(ctx, n, At, A)) return rc;
+      }
       if (int rc = copy64(ctx, n, dU0, dU)) return rc;
       out.time = time0;
       ++out.failed_attempts;
@@ -249,7 +335,7 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
       if (int rc = attempt(h, ok)) return rc;
       if (ok) return FX_OK;
     }
-    if (level >= d.num_sublevels) return FX_OK;
+    if (is_static || level >= d.num_sublevels) return FX_OK;  // static stages retry only (:262-264)
     ++out.substepped;
     const double hs = h / static_cast<double>(d.num_substeps);
     for (int i = 0; i < d.num_substeps; ++i) {

@@ -62,7 +62,9 @@ struct TileHeader {
   int32_t nx, ny, nz;       // cells of this tile (<= kTileX, kTileY, tile depth)
   int32_t word_offset;      // into the node words: tile * 81 (tz + 1), node (x, y, z) at x + 9 (y + 9 z)
   int32_t cell_offset;      // into the cell parameters: tile * 64 tz (full-tile order, x fastest)
-  int32_t pad[3];
+  int32_t interior_base;    // a full tile whose strictly-interior nodes are its own, with three free DOFs
+                            // each, numbered in interior order (x fastest): their first equation; else -1
+  int32_t pad[2];
 };
 
 struct Lattice {
@@ -135,6 +137,12 @@ void set_thread_error(const std::string& msg);
 int static_scratch_init(fx_ctx* ctx);
 // Drop the captured CG graphs of one owner (nullptr: all).
 void release_cg_graphs(fx_ctx* ctx, const void* owner);
+// The persistent autotune cache (fx_tune.cpp): keys for the device and the lattice model, and the
+// tile depth and kernel variant kept for a key.
+std::string tune_machine_key(int device);
+std::string tune_model_key(int64_t cells, bool mass, bool damping, int forced_depth);
+bool tune_cache_get(const std::string& key, int& depth, int& variant);
+void tune_cache_put(const std::string& key, int depth, int variant);
 }  // namespace fx
 
 namespace fx {
@@ -233,6 +241,7 @@ struct fx_batch {
   double* partial64 = nullptr;        // [nslot * 3] per-tile sums at shared nodes
   float* partial32 = nullptr;
   int box_variant = 0;                // register-allocation variant of the mass-free kernel (tuned)
+  float tuned_ms = 0.0f;              // the tuned variant's sample time (both operator forms), 0 untuned
   int tile_z = 8;                     // lattice tiles of 8 x 8 x tile_z cells
   // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference
   bool internal = false;
@@ -274,8 +283,17 @@ struct fx_batch {
   int32_t* pimy_touched = nullptr;  // per point: the highest surface whose centre an update has written
   int32_t nlong = 0;
   std::vector<int8_t> eq_dir;
+  int8_t* eq_dir_dev = nullptr;  // the same on the device, solver numbering (explicit ground loads)
   // GMRES workspace (FX_BATCH_GMRES): V [(m + 1) n], Z [m n], coefficients [m + 1], CG start [n]
   double* gm = nullptr;
+  // nodal masses of the model (fx_batch_set_nodal_mass), solver numbering; the elements' own
+  // lumped masses (host, solver numbering) that the lumped arrays add them to
+  double* nodal_m = nullptr;
+  double* nodal_am = nullptr;
+  float* nodal_m32 = nullptr;
+  float* nodal_am32 = nullptr;
+  std::vector<double> host_lm, host_la;
+  int32_t tangent_symmetry = FX_TANGENT_SPD;
   // solver work (allocated at create: compute allocates nothing)
   double* work = nullptr;       // 8 * neq doubles: r z p ap inv d, permuted b and x
   float* work32 = nullptr;      // 5 * neq floats
@@ -292,3 +310,15 @@ struct fx_batch {
       return FX_EDEVICE;                                                           \
     }                                                                              \
   } while (0)
+
+namespace fx {
+// operator_symmetry (matrix_free_operator.cpp:5-16): whether K_eff of k is symmetric positive
+// definite, from the batch's tangent symmetry (elastic SPD, PressureIndependMultiYield SPSD: made
+// definite by a mass term) and the coefficients' signs.
+inline bool operator_spd(const fx_batch* b, const fx_coeffs& k) {
+  const bool positive =
+      k.stiffness >= 0.0 && k.damping >= 0.0 && k.mass >= 0.0 && (k.stiffness > 0.0 || k.mass > 0.0);
+  if (b->tangent_symmetry == FX_TANGENT_SPD) return positive;
+  return positive && k.mass > 0.0;
+}
+}  // namespace fx

@@ -43,6 +43,18 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
 int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done = nullptr);
 int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
+// The same without the nodal masses (one batch's elements only).
+int elements_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
+int elements_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
+// fx_run_operator's pieces: the per-element values into slots[a * padded + e] (elastic,
+// fx_operator.hip; PressureIndependMultiYield, fx_pimy.hip), the batch's gather of such slots into
+// y (accumulate: y += the sums), and y += the fast-path operator or diagonal (reference numbering).
+int element_slots64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, int unit_dir, double* slots);
+int pimy_slots64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, int unit_dir, double* slots);
+int slots_gather64(fx_batch* b, const double* slots, double* y, int accumulate);
+int add_elements64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, double* y);
+// f += am v with the nodal masses' alphaM part (solver numbering).
+int nodal_damping_force64(fx_batch* b, const double* v, double* f);
 
 // The C-ABI operator entry points (reference equation numbering; lattice batches permute to
 // and from their internal numbering around the internal apply), fx_operator.hip.
@@ -91,6 +103,13 @@ int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 // Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch);
 int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a);
+
+// z = x * y elementwise (fx_implicit.hip).
+int multiply64(fx_ctx* ctx, int32_t n, const double* x, const double* y, double* z);
+
+// An excitation's value at time t (PathSeries / PathTimeSeries times the pattern factor,
+// ground_motion.cpp:17-74, 121-145); loc: the time array's interval walk (fx_implicit.hip).
+double series_value(const fx_time_series& s, double fact, double t, int32_t& loc);
 
 // The implicit Newmark stage (fx_implicit.hip).
 int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, double* a, double* du,

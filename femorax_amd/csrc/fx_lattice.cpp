@@ -252,8 +252,34 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
       L.perm[next++] = q;
     }
   };
-  for (int32_t n : tile_nodes)
-    if (n >= 0 && !numbered[n] && touch_count[n] == 1 && !tied[n]) number(n);
+  // a tile's nodes only it touches, tile by tile: its strictly-interior nodes first (x fastest),
+  // then the others (on the lattice's boundary); a full tile whose interior nodes all have three
+  // free DOFs then finds their equations from one base (TileHeader::interior_base)
+  const int NXY = (TX + 1) * (TY + 1);
+  for (TileHeader& h : L.tiles) {
+    h.interior_base = -1;
+    const bool full = h.nx == TX && h.ny == TY && h.nz == TZ;
+    bool regular = full;
+    const int32_t base = next;
+    for (int32_t z = 1; z < TZ; ++z)
+      for (int32_t y = 1; y < TY; ++y)
+        for (int32_t x = 1; x < TX; ++x) {
+          const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + x + (TX + 1) * y + NXY * z];
+          if (n < 0 || numbered[n] || touch_count[n] != 1 || tied[n]) {
+            regular = false;
+            continue;
+          }
+          bool all = true;
+          for (int d = 0; d < 3; ++d) all &= node_eq[3 * n + d] >= 0;
+          regular &= all;
+          number(n);
+        }
+    if (regular && next == base + 3 * (TX - 1) * (TY - 1) * (TZ - 1)) h.interior_base = base;
+    for (int32_t i = 0; i < tile_nodes_full; ++i) {
+      const int32_t n = tile_nodes[static_cast<size_t>(h.word_offset) + i];
+      if (n >= 0 && !numbered[n] && touch_count[n] == 1 && !tied[n]) number(n);
+    }
+  }
   for (int32_t n : shared_nodes) number(n);
   for (const auto& s : tied_sets) number(s.front());
   for (int32_t q = 0; q < neq; ++q)

@@ -223,6 +223,46 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
   }
 }
 
+// One element per thread: element e's 24 operator values (or its diagonal) into slots[a * padded
+// + e], for every element e < count (launch_operator with emit's Out::Slots,
+// operator_kernels.hpp:75-141, assembly.hpp:28-36). unit_dir >= 0 takes the rigid unit
+// translation in that direction as the input at every DOF, fixed ones included (:96-98).
+template <bool DIAG>
+__global__ __launch_bounds__(128) void element_slots_kernel(KernelArgs<double> a, int32_t count, int unit_dir,
+                                                             double* __restrict__ slots) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= count) return;
+  double X[8], Y[8], Z[8], V[24], out[24];
+  int eqe[24];
+  const size_t P = static_cast<size_t>(a.padded);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    const int n = a.node[k * P + e];
+    X[k] = a.coords64[n];
+    Y[k] = a.coords64[static_cast<size_t>(a.nnode) + n];
+    Z[k] = a.coords64[2 * static_cast<size_t>(a.nnode) + n];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int q = a.node_eq[3 * n + d];
+      eqe[3 * k + d] = q;
+      if (unit_dir >= 0) V[3 * k + d] = d == unit_dir ? 1.0 : 0.0;
+      else V[3 * k + d] = (!DIAG && q >= 0) ? a.x[q] : 0.0;
+    }
+  }
+  const DevParam& p = a.params[a.param[e]];
+  double fD, cM;
+  operator_factors(p, a.k, fD, cM);
+  ElemCoef<double> c;
+  c.lam = fD * p.lam;
+  c.mu = fD * p.mu;
+  c.rhoM = cM * p.rho;
+  c.consistent = p.mass_type == 0;
+  if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
+  else element_apply(X, Y, Z, V, c, out);
+#pragma unroll
+  for (int k = 0; k < 24; ++k) slots[k * P + e] = out[k];
+}
+
 // y[q] (+)= sum of the node-space values of q's node DOFs, in ascending DOF order.
 template <typename T>
 __global__ void contract_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
@@ -299,17 +339,81 @@ int zero_orphans(fx_batch* b, T* y, const int32_t* done) {
   return FX_OK;
 }
 
-int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
+// y += c m x (or c m on the diagonal): add_nodal_mass (internal_force.cpp:442-448).
+template <typename T>
+__global__ void nodal_mass_kernel(int32_t n, T c, const T* __restrict__ m, const T* __restrict__ x, T* __restrict__ y,
+                                  int diagonal, const int32_t* done) {
+  if (done && *done) return;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < n) y[q] += c * m[q] * (diagonal ? T(1) : x[q]);
+}
+
+// The nodal masses' terms of the operator (cM m x, then cC am x) or of its diagonal
+// (internal_force.cpp:571-572, 597-598).
+template <typename T>
+int add_nodal(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int diagonal, const int32_t* done) {
+  const T* m;
+  const T* am;
+  if constexpr (sizeof(T) == 8) {
+    m = b->nodal_m;
+    am = b->nodal_am;
+  } else {
+    m = b->nodal_m32;
+    am = b->nodal_am32;
+  }
+  const dim3 g((b->neq + 255) / 256);
+  if (m && k.mass != 0.0)
+    hipLaunchKernelGGL(nodal_mass_kernel<T>, g, dim3(256), 0, b->ctx->stream, b->neq, static_cast<T>(k.mass), m, x, y,
+                       diagonal, done);
+  if (am && k.damping != 0.0)
+    hipLaunchKernelGGL(nodal_mass_kernel<T>, g, dim3(256), 0, b->ctx->stream, b->neq, static_cast<T>(k.damping), am, x,
+                       y, diagonal, done);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+int nodal_damping_force64(fx_batch* b, const double* v, double* f) {
+  return add_nodal<double>(b, fx_coeffs{0.0, 1.0, 0.0, 0}, v, f, 0, nullptr);
+}
+
+int elements_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
   if (b->pimy) return pimy_apply64(b, k, x, y, done);
   const int rc = b->lattice ? launch_box<double, false>(b, k, x, y, 0, done)
                             : launch_operator<double, false>(b, operator_args<double>(b, k, x, y, 0, done));
   return rc ? rc : zero_orphans(b, y, done);
 }
+int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done) {
+  if (int rc = elements_apply64(b, k, x, y, done)) return rc;
+  return (b->nodal_m || b->nodal_am) ? add_nodal<double>(b, k, x, y, 0, done) : FX_OK;
+}
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done) {
-  if (b->pimy) return pimy_apply32(b, k, x, y, done);
-  const int rc = b->lattice ? launch_box<float, false>(b, k, x, y, 0, done)
-                            : launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
-  return rc ? rc : zero_orphans(b, y, done);
+  int rc;
+  if (b->pimy) {
+    rc = pimy_apply32(b, k, x, y, done);
+  } else {
+    rc = b->lattice ? launch_box<float, false>(b, k, x, y, 0, done)
+                    : launch_operator<float, false>(b, operator_args<float>(b, k, x, y, 0, done));
+    if (!rc) rc = zero_orphans(b, y, done);
+  }
+  if (rc) return rc;
+  return (b->nodal_m32 || b->nodal_am32) ? add_nodal<float>(b, k, x, y, 0, done) : FX_OK;
+}
+
+int element_slots64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, int unit_dir, double* slots) {
+  if (!b->node || !b->node_eq || !b->coords || !b->param) {
+    b->ctx->error = "fx_run_operator: the batch keeps no element connectivity on the device (create it with "
+                    "FX_BATCH_SLOTS)";
+    return FX_EINVAL;
+  }
+  if (b->count == 0) return FX_OK;
+  const KernelArgs<double> a = operator_args<double>(b, k, x, static_cast<double*>(nullptr), 0);
+  const dim3 g((b->count + 127) / 128);
+  if (diagonal)
+    hipLaunchKernelGGL(element_slots_kernel<true>, g, dim3(128), 0, b->ctx->stream, a, b->count, unit_dir, slots);
+  else
+    hipLaunchKernelGGL(element_slots_kernel<false>, g, dim3(128), 0, b->ctx->stream, a, b->count, unit_dir, slots);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
 }
 __global__ void add_at_kernel(int32_t n, const int32_t* __restrict__ idx, const double* __restrict__ val,
                               double* __restrict__ d) {
@@ -318,6 +422,12 @@ __global__ void add_at_kernel(int32_t n, const int32_t* __restrict__ idx, const 
 }
 
 int operator_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
+  if (int rc = elements_diagonal64(b, k, d)) return rc;
+  return (b->nodal_m || b->nodal_am) ? add_nodal<double>(b, k, static_cast<const double*>(nullptr), d, 1, nullptr)
+                                     : FX_OK;
+}
+
+int elements_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
   if (b->pimy) return pimy_diagonal64(b, k, d);
   const int rc = b->lattice
                      ? launch_box<double, true>(b, k, static_cast<const double*>(nullptr), d, 0, nullptr)
@@ -406,6 +516,34 @@ int external_call(fx_batch* b, const T* x, T* y, T* xi, T* yi, Op op) {
   return FX_OK;
 }
 }  // namespace
+
+__global__ void add_perm_kernel(int32_t n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                                double* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm ? perm[i] : i] += src[i];
+}
+
+// y (reference numbering) += this batch's operator or diagonal without the nodal masses, computed
+// on the fast path into the batch's scratch (fx_run_operator's FX_OUT_PLAIN / FX_OUT_ATOMIC).
+int add_elements64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, double* y) {
+  if (b->neq == 0) return FX_OK;
+  double* xi = b->work + 6 * static_cast<size_t>(b->neq);
+  double* yi = b->work + 7 * static_cast<size_t>(b->neq);
+  if (diagonal) {
+    if (int rc = elements_diagonal64(b, k, yi)) return rc;
+  } else {
+    const double* xs = x;
+    if (b->internal) {
+      if (int rc = to_internal<double>(b, x, xi)) return rc;
+      xs = xi;
+    }
+    if (int rc = elements_apply64(b, k, xs, yi)) return rc;
+  }
+  hipLaunchKernelGGL(add_perm_kernel, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
+                     b->internal ? b->perm : static_cast<const int32_t*>(nullptr), yi, y);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
 
 int external_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y) {
   double* xi = b->work + 6 * static_cast<size_t>(b->neq);

@@ -84,7 +84,15 @@ struct PimyBatchArgs {
   double* cres;     // compacted return map: [33 * npts] stress, tangent (upper), strain
   int32_t* clist;
   int32_t* ccount;
+  int32_t unit_dir;  // operator: >= 0 replaces x by the rigid unit translation in that direction
 };
+
+// The operator's input at local DOF k: x at the DOF's equation (0 at a fixed DOF), or with unit_dir
+// >= 0 the rigid unit translation at every DOF, fixed ones included (operator_kernels.hpp:96-98).
+__device__ inline double operator_input(const PimyBatchArgs& a, const int (&eq)[24], int k) {
+  if (a.unit_dir >= 0) return k % 3 == a.unit_dir ? 1.0 : 0.0;
+  return eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+}
 
 __device__ inline void element_geometry(const PimyBatchArgs& a, int e, double (&X)[8], double (&Y)[8], double (&Z)[8],
                                         int (&eq)[24]) {
@@ -361,7 +369,7 @@ __global__ __launch_bounds__(128) void pimy_operator_kernel(PimyBatchArgs a) {
   const PimyParams& p = a.params[a.param[e]];
   double v[24];
   if constexpr (!DIAG)
-    for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+    for (int k = 0; k < 24; ++k) v[k] = operator_input(a, eq, k);
   double out[24];
   for (int k = 0; k < 24; ++k) out[k] = 0.0;
   double cK, c0, cc, cM;
@@ -435,7 +443,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_point_operator_kernel(Pi
     effective_D(a, p, e, gp, D, cM);
     if constexpr (!DIAG) {
       double v[24], eps[6], sig[6];
-      for (int k = 0; k < 24; ++k) v[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+      for (int k = 0; k < 24; ++k) v[k] = operator_input(a, eq, k);
       point_strain(s, v, eps);
       for (int r = 0; r < 6; ++r) {
         double sum = 0.0;
@@ -554,7 +562,8 @@ constexpr int kLongGather = 64;
 
 template <typename T>
 __global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ start, const int32_t* __restrict__ list,
-                                   const T* __restrict__ slots, T* __restrict__ y, const int32_t* done) {
+                                   const T* __restrict__ slots, T* __restrict__ y, const int32_t* done,
+                                   int accumulate) {
   if (done && *done) return;
   const int32_t q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= neq) return;
@@ -562,7 +571,7 @@ __global__ void slot_gather_kernel(int32_t neq, const int32_t* __restrict__ star
   if (j1 - j0 > kLongGather) return;
   T s = T(0);
   for (int32_t j = j0; j < j1; ++j) s += slots[list[j]];
-  y[q] = s;
+  y[q] = accumulate ? y[q] + s : s;
 }
 
 // The long equations: one wave each, lane l summing slots l, l + 64, ... in order, then a fixed
@@ -572,7 +581,7 @@ __global__ __launch_bounds__(256) void long_gather_kernel(int32_t nlong, const i
                                                           const int32_t* __restrict__ start,
                                                           const int32_t* __restrict__ list,
                                                           const T* __restrict__ slots, T* __restrict__ y,
-                                                          const int32_t* done) {
+                                                          const int32_t* done, int accumulate) {
   const int u = static_cast<int>(blockIdx.x) * 4 + static_cast<int>(threadIdx.x) / 64, lane = threadIdx.x % 64;
   if (u >= nlong) return;  // the whole wave
   if (done && *done) return;
@@ -580,7 +589,7 @@ __global__ __launch_bounds__(256) void long_gather_kernel(int32_t nlong, const i
   T s = T(0);
   for (int32_t j = start[q] + lane; j < start[q + 1]; j += 64) s += slots[list[j]];
   for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-  if (lane == 0) y[q] = s;
+  if (lane == 0) y[q] = accumulate ? y[q] + s : s;
 }
 
 __global__ void pimy_batch_init_kernel(int64_t npts, const PimyParams* __restrict__ params,
@@ -619,18 +628,19 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const
   a.cres = b->cres;
   a.clist = b->clist;
   a.ccount = b->ccount;
+  a.unit_dir = -1;
   return a;
 }
 
 // y[q] = the sum of equation q's element slots (slot_gather_kernel, long_gather_kernel)
 template <typename T>
-int gather(fx_batch* b, const T* slots, T* y, const int32_t* done) {
+int gather(fx_batch* b, const T* slots, T* y, const int32_t* done, int accumulate = 0) {
   if (b->neq > 0)
     hipLaunchKernelGGL(slot_gather_kernel<T>, dim3((b->neq + 255) / 256), dim3(256), 0, b->ctx->stream, b->neq,
-                       b->gather_start, b->gather_list, slots, y, done);
+                       b->gather_start, b->gather_list, slots, y, done, accumulate);
   if (b->nlong > 0)
     hipLaunchKernelGGL(long_gather_kernel<T>, dim3((b->nlong + 3) / 4), dim3(256), 0, b->ctx->stream, b->nlong,
-                       b->long_eqs, b->gather_start, b->gather_list, slots, y, done);
+                       b->long_eqs, b->gather_start, b->gather_list, slots, y, done, accumulate);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }
@@ -720,6 +730,25 @@ __global__ __launch_bounds__(128) void pimy_operator32_kernel(PimyBatchArgs a, c
   for (int k = 0; k < 24; ++k) slots[k * P + e] = out[k];
 }
 
+// The operator (DIAG false) or its diagonal on the stored tangents into a.slots, element values
+// a * padded + e: one thread per point below kPointOperatorBelow elements, one per element above.
+template <bool DIAG>
+void launch_operator_slots(fx_batch* b, const PimyBatchArgs& a) {
+  hipStream_t st = b->ctx->stream;
+  if (b->count > 0 && b->count < kPointOperatorBelow) {  // small: one thread per point
+    const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
+    hipLaunchKernelGGL((pimy_point_operator_kernel<DIAG>), grid, dim3(kPointsPerBlock), 0, st, a);
+  } else if (b->count > 0) {  // one element per thread
+    bool comm = false;
+    for (const PimyParams& q : b->host_pimy) comm |= a.k.damping * q.betaKcomm != 0.0;
+    const dim3 g((b->count + 127) / 128);
+    if (comm)
+      hipLaunchKernelGGL((pimy_operator_kernel<DIAG, true>), g, dim3(128), 0, st, a);
+    else
+      hipLaunchKernelGGL((pimy_operator_kernel<DIAG, false>), g, dim3(128), 0, st, a);
+  }
+}
+
 // MODE 0: the force (STORE: also the tangents); 1: the operator on the stored tangents; 2: its
 // diagonal. The elements' 24 values go to their slots, then each equation gathers its slots.
 template <int MODE, bool STORE>
@@ -739,16 +768,8 @@ int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done
     } else if (b->count > 0) {
       hipLaunchKernelGGL((pimy_batch_kernel<0, STORE>), grid, dim3(kPointsPerBlock), 0, st, a);
     }
-  } else if (b->count > 0 && b->count < kPointOperatorBelow) {  // small: one thread per point
-    hipLaunchKernelGGL((pimy_point_operator_kernel<MODE == 2>), grid, dim3(kPointsPerBlock), 0, st, a);
-  } else if (b->count > 0) {  // one element per thread
-    bool comm = false;
-    for (const PimyParams& q : b->host_pimy) comm |= a.k.damping * q.betaKcomm != 0.0;
-    const dim3 g((b->count + 127) / 128);
-    if (comm)
-      hipLaunchKernelGGL((pimy_operator_kernel<MODE == 2, true>), g, dim3(128), 0, st, a);
-    else
-      hipLaunchKernelGGL((pimy_operator_kernel<MODE == 2, false>), g, dim3(128), 0, st, a);
+  } else {
+    launch_operator_slots<MODE == 2>(b, a);
   }
   return gather<double>(b, b->slots, y, done);
 }
@@ -818,6 +839,24 @@ int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d) {
     return FX_EINVAL;
   }
   return run_pimy<2, false>(b, batch_args(b, nullptr, k), d, nullptr);
+}
+
+int pimy_slots64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, int unit_dir, double* slots) {
+  if (!b->tangents_stored && !k.initial && k.stiffness != 0.0) {
+    b->ctx->error = "fx_run_operator: no stored tangents (call fx_store_tangents first)";
+    return FX_EINVAL;
+  }
+  PimyBatchArgs a = batch_args(b, x, k);
+  a.unit_dir = unit_dir;
+  a.slots = slots;
+  if (diagonal) launch_operator_slots<true>(b, a);
+  else launch_operator_slots<false>(b, a);
+  FX_HIP(b->ctx, hipGetLastError());
+  return FX_OK;
+}
+
+int slots_gather64(fx_batch* b, const double* slots, double* y, int accumulate) {
+  return gather<double>(b, slots, y, nullptr, accumulate);
 }
 
 int pimy_points(fx_ctx* ctx, int64_t npts, const PimyParams* params, const int32_t* param, const double* C, double* T,

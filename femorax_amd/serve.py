@@ -13,7 +13,8 @@ while earlier requests are still on the device.
   has its own context, so its own HIP stream: block solves overlap with the column iterations
   and with each other on the device. A worker keeps the compiled batch of each block size it
   has served resident, within the block share of the budget, evicting the least recently
-  used. Blocks solve with FP64 CG (fx_cg) by default: up to 48^3 (346k DoF) a solve is
+  used; the block sizes the service is set up for (block_sizes) are made resident before the
+  first request. Blocks solve with FP64 CG (fx_cg) by default: up to 48^3 (346k DoF) a solve is
   latency-bound, and FP32 inner CG inside FP64 refinement (fx_cg_mixed, block_mixed=True) needs
   1.5-2.6x the iterations at about the same time per iteration (measured 1.4-2.6x slower). Columns
   use the mixed PCG (mixed=True), which measures slightly faster on the pool.
@@ -94,11 +95,14 @@ class BlockProblems:
 class BlockWorker(threading.Thread):
     """Solves block requests on its own context (stream), keeping batches resident by size."""
 
-    def __init__(self, device, problems, inbox, done, budget, mixed, rtol=1e-8, keep_x=False, inner_rtol=1e-6):
+    def __init__(self, device, problems, inbox, done, budget, mixed, rtol=1e-8, keep_x=False, inner_rtol=1e-6,
+                 prewarm=()):
         super().__init__(daemon=True)
         self.device, self.problems, self.inbox, self.done = device, problems, inbox, done
         self.budget, self.mixed, self.rtol, self.keep_x = budget, mixed, rtol, keep_x
         self.inner_rtol = inner_rtol
+        self.prewarm = tuple(prewarm)
+        self.ready = threading.Event()
         self.error = None
 
     def run(self):
@@ -109,6 +113,16 @@ class BlockWorker(threading.Thread):
             self.error = e
         resident = collections.OrderedDict()  # n -> (batch, b, x, bytes)
         self.used = 0
+        # the block models the service is set up for: resident on the device before traffic (a
+        # model seen for the first time would otherwise pay its compile and upload in its latency)
+        for n in self.prewarm:
+            if ctx is None:
+                break
+            try:
+                self._resident(ctx, resident, n)
+            except Exception as e:  # noqa: BLE001 - the request path reports it again
+                self.error = e
+        self.ready.set()
         while True:
             r = self.inbox.get()
             if r is None:
@@ -130,24 +144,29 @@ class BlockWorker(threading.Thread):
         if ctx is not None:
             ctx.close()
 
+    def _resident(self, ctx, resident, n):
+        """The resident batch of block size n (built and uploaded on first use, least recently
+        used ones evicted to stay within the worker's share of the budget)."""
+        if n in resident:
+            resident.move_to_end(n)
+            return resident[n]
+        model, mesh, rhs = self.problems.get(n)
+        need = _block_bytes(mesh.neq, self.mixed)
+        while resident and self.used + need > self.budget:  # admission within the block share
+            _, (batch, b, x, nb) = resident.popitem(last=False)
+            batch.close()
+            b.free()
+            x.free()
+            self.used -= nb
+        batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=self.mixed, tune=False)
+        b = core.DeviceArray.from_host(ctx, rhs)
+        x = core.DeviceArray(ctx, mesh.neq)
+        resident[n] = (batch, b, x, need)
+        self.used += need
+        return resident[n]
+
     def _solve(self, ctx, resident, r):
-        if r.n in resident:
-            resident.move_to_end(r.n)
-        else:
-            model, mesh, rhs = self.problems.get(r.n)
-            need = _block_bytes(mesh.neq, self.mixed)
-            while resident and self.used + need > self.budget:  # admission within the block share
-                _, (batch, b, x, nb) = resident.popitem(last=False)
-                batch.close()
-                b.free()
-                x.free()
-                self.used -= nb
-            batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=self.mixed, tune=False)
-            b = core.DeviceArray.from_host(ctx, rhs)
-            x = core.DeviceArray(ctx, mesh.neq)
-            resident[r.n] = (batch, b, x, need)
-            self.used += need
-        batch, b, x, _ = resident[r.n]
+        batch, b, x, _ = self._resident(ctx, resident, r.n)
         x.zero()
         if self.mixed:
             res = batch.cg_mixed(b, x, rtol=self.rtol, inner_rtol=self.inner_rtol)
@@ -177,21 +196,25 @@ class Server:
 
     def __init__(self, device=0, hbm_budget=200e9, column_share=0.5, block_workers=2, mixed=True, dt=0.005,
                  max_columns=4096, iterations_per_call=4, keep_results=False, max_record_steps=500,
-                 one_stream=True, block_mixed=False):
+                 one_stream=True, block_mixed=False, block_sizes=()):
         self.ctx = core.Context(device)
         self.mixed, self.iters, self.keep = mixed, iterations_per_call, keep_results
         pool_elems = int(column_share * hbm_budget / COLUMN_BYTES_PER_ELEMENT)
         pool_elems = min(pool_elems, max_columns * core.FX_COLUMN_MAX_ELEMS)
         self.pool = core.ColumnPool(self.ctx, max_elements=pool_elems, max_columns=max_columns, dt=dt,
                                     max_materials=10 * max_columns, max_record=max_columns * 10 * max_record_steps,
-                                    mixed=mixed, one_stream=one_stream)
+                                    mixed=mixed, one_stream=one_stream, tune=True)
         self.problems = BlockProblems()
         self.inbox, self.done = queue.Queue(), queue.Queue()
         per_worker = (1.0 - column_share) * hbm_budget / max(1, block_workers)
+        for n in block_sizes:  # the host compile of the served block models, shared by the workers
+            self.problems.get(n)
         self.workers = [BlockWorker(device, self.problems, self.inbox, self.done, per_worker, block_mixed,
-                                    keep_x=keep_results) for _ in range(block_workers)]
+                                    keep_x=keep_results, prewarm=block_sizes) for _ in range(block_workers)]
         for w in self.workers:
             w.start()
+        for w in self.workers:
+            w.ready.wait()
 
     def run(self, requests, log=None):
         """Serve `requests` (sorted by arrival, seconds from the start) in real time; returns

@@ -159,6 +159,10 @@ typedef struct fx_batch_desc {
                                  (more workgroups for small latency-bound models) */
 #define FX_BATCH_TILE16 1024u /* lattice path: tiles of 8 x 8 x 16 cells (four passes, streamed: x and words
                                  of five node planes in LDS at a time) */
+#define FX_BATCH_SLOTS 2048u  /* keep the per-element operator forms of fx_run_operator ready: on elastic batches
+                                 an element slot buffer [24 * padded] and its gather list (and, on the lattice
+                                 path, the connectivity, coordinates and node equations on the device);
+                                 PressureIndependMultiYield batches always have them */
 #define FX_BATCH_COMPACT 128u /* stdBrick<PressureIndependMultiYield> batches: the compacted return map
                                  (compacted_kernels.hpp:85-202): an elastic predictor per Gauss point, the
                                  plastic points listed, one full update per listed point, the element
@@ -167,11 +171,16 @@ typedef struct fx_batch_desc {
 int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* desc, uint32_t flags, fx_batch** out);
 void fx_batch_destroy(fx_batch* b);
 
+/* TangentSymmetry of the batch's material (src/physics/contracts.hpp:36-45). */
+#define FX_TANGENT_SPD 0    /* SymmetricPositiveDefinite (ElasticIsotropic, elastic_isotropic.hpp:27) */
+#define FX_TANGENT_SPSD 1   /* SymmetricPositiveSemidefinite (PressureIndependMultiYield,
+                               pressure_independ_multi_yield.hpp:70) */
 typedef struct fx_batch_info {
   int32_t count, neq, nnode, nblocks, ncolors, tied;
   int32_t lattice;          /* 1: the batch runs on the structured-lattice fast path */
   int32_t tiles;            /* lattice tiles (workgroups per apply) */
   int32_t variant;          /* tuned kernel variant */
+  int32_t tangent_symmetry; /* FX_TANGENT_* (Batch::tangent_symmetry, internal_force_batch.hpp:84) */
   uint64_t device_bytes;
 } fx_batch_info;
 int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
@@ -193,6 +202,41 @@ int fx_operator_apply_f32(fx_batch* b, const fx_coeffs* k, const float* x, float
 /* d = diag(K_eff) (InternalForce::operator_diagonal, internal_force.cpp:575-599;
  * element_operator_diagonal, element_operator.hpp:138-209). */
 int fx_operator_diagonal(fx_batch* b, const fx_coeffs* k, double* d);
+/* The per-batch verb the reference's InternalForce calls once per batch per apply
+ * (InternalForce::Batch::run_operator, internal_force_batch.hpp:80-82; launch_operator,
+ * operator_kernels.hpp:75-141; emit, assembly.hpp:28-36), for a binding that keeps
+ * InternalForce::apply_operator / operator_diagonal / mass_times_ground unchanged
+ * (internal_force.cpp:545-626):
+ *   out = FX_OUT_SLOTS: slots[slot_offset + a * padded + e] = value a (0..23) of element e of the
+ *     batch (e < count), fixed DOFs included, as Out::Slots writes them; the caller's SlotGather
+ *     then sums them (assembly.cpp:30-38). Elastic batches need FX_BATCH_SLOTS.
+ *   out = FX_OUT_PLAIN or FX_OUT_ATOMIC: y[q] += this batch's sum at equation q (q >= 0); the
+ *     caller zeroed y before the first batch, as apply_operator does (internal_force.cpp:565).
+ *     The adds run in a fixed order here (the reference's Atomic order varies between runs).
+ *   diagonal = 1: the element diagonals (element_operator_diagonal, element_operator.hpp:138-209);
+ *     x unused.
+ *   unit_dir = 0..2: x is replaced by the rigid unit translation in that direction at every
+ *     element DOF, fixed ones included (the ground-load vector M r of mass_times_ground with
+ *     k = {0, 0, 1}); -1: x is used. With PLAIN/ATOMIC, elastic batches need FX_BATCH_SLOTS.
+ * The nodal masses of fx_batch_set_nodal_mass are not added (InternalForce adds them). x, y and
+ * slots are device pointers in the reference's equation numbering. */
+#define FX_OUT_PLAIN 0
+#define FX_OUT_ATOMIC 1
+#define FX_OUT_SLOTS 2
+int fx_run_operator(fx_batch* b, int diagonal, const fx_coeffs* k, int out, const double* x, double* y,
+                    double* slots, int64_t slot_offset, int unit_dir);
+/* Nodal masses of the model (Compiled::nodal_mass / nodal_alpha_mass, batches.cpp:249-261): host
+ * [neq] in the reference numbering, NULL for none. Like InternalForce, the batch then adds
+ * cM m x + cC am x to fx_operator_apply and its diagonal (add_nodal_mass, internal_force.cpp:
+ * 442-448, 571-572, 597-598; also inside fx_cg, fx_newmark and the other solvers), am v to
+ * fx_force (:366-374), m and am to the lumped masses of fx_lumped_mass and fx_explicit_cd
+ * (batches.cpp:252-258), and m to fx_mass_times_ground (:619-625). In a model of several batches
+ * give them to one batch. Set-up call (uploads; replaces the previous nodal masses). */
+int fx_batch_set_nodal_mass(fx_batch* b, const double* mass, const double* alpha_mass);
+/* InternalForce::mass_times_ground (internal_force.cpp:601-626) for a model of this one batch:
+ * y = M r for the rigid unit translation r in `direction` (0..2) at every element DOF, fixed ones
+ * included, plus the nodal masses of that direction. y: device [neq], reference numbering. */
+int fx_mass_times_ground(fx_batch* b, int direction, double* y);
 /* Timing aid: the operator kernel alone, `reps` times back to back, device time in ms
  * (HIP events on the batch's stream). */
 int fx_operator_time(fx_batch* b, const fx_coeffs* k, int precision, const void* x, void* y, int reps,
@@ -237,8 +281,13 @@ int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const do
 /* FGMRES(30), CGS2, right Jacobi (Gmres::solve, gmres.cpp:32-123). Needs FX_BATCH_GMRES. */
 int fx_gmres(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
              fx_solve_result* res);
-/* KrylovSolver::solve (krylov.cpp:25-42): CG; if it meets p^T A p <= 0, GMRES from the same
- * initial guess (iterations add up, indefinite set). Without FX_BATCH_GMRES: CG only. */
+/* KrylovSolver::setup + solve (krylov.cpp:10-42) on the operator of k, whose symmetry is
+ * operator_symmetry(batch tangent symmetry, k) (matrix_free_operator.cpp:5-16): a symmetric
+ * positive definite operator runs CG and, if CG meets p^T A p <= 0, GMRES from the same initial
+ * guess (iterations add up, indefinite set); any other operator (a PressureIndependMultiYield
+ * batch without a mass term, negative coefficients) runs GMRES from the start. Needs
+ * FX_BATCH_GMRES for the GMRES runs (FX_EINVAL when one is needed and the workspace is missing;
+ * without it an SPD operator runs CG only). */
 int fx_krylov(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
               fx_solve_result* res);
 int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol,
@@ -257,6 +306,22 @@ int fx_force(fx_batch* b, const double* u, const double* v, double* f);
  * lattice batches solve one eigenproblem per parameter set. */
 int fx_critical_dt(fx_batch* b, int scheme, double* dt);
 
+/* An excitation's time series (GroundMotions, src/exec/ground_motion.cpp:17-74): with time NULL,
+ * PathSeries (uniform step dt from start, use_last holding the last value past the end; the
+ * caller prepends OpenSees' leading zero if it wants one); with time set, PathTimeSeries
+ * (linear between the given nondecreasing times, 0 outside them, use_last, dt and start unused),
+ * whose interval walk is kept in *loc between calls as GroundMotions keeps it per pattern (loc
+ * NULL: each fx_newmark / fx_explicit_cd call walks from interval 0). Either form is 0 before
+ * time 0. */
+typedef struct fx_time_series {
+  const double* values;       /* host [count] */
+  int32_t count;
+  double dt, factor, start;
+  int32_t use_last;
+  const double* time;         /* host [count] or NULL */
+  int32_t* loc;               /* host: the walk's interval, read at the start and written at the end of
+                                 the call, or NULL */
+} fx_time_series;
 #define FX_MAX_POINT_LOADS 32
 typedef struct fx_cd_desc {
   double dt;
@@ -264,14 +329,29 @@ typedef struct fx_cd_desc {
   const double* dash;        /* [neq] device: diagonal dampers outside the elements (e.g. Lysmer-
                                 Kuhlemeyer boundaries), NULL = none: their force dash * v_e joins
                                 the element forces at the garbage velocity and dash joins the
-                                tangent's C */
+                                tangent's C (ExplicitDifference: the force dash * Utdot, no tangent
+                                term) */
   int32_t nload;             /* point loads, <= FX_MAX_POINT_LOADS */
   const int32_t* load_eq;    /* [nload] host: equations */
   const double* load_hist;   /* [nsteps * nload] host: P(t_k) of load j at [k * nload + j] */
+  int32_t scheme;            /* 0 CentralDifference, 1 ExplicitDifference */
+  int32_t ground_dir;        /* UniformExcitation direction 0..2, or -1 */
+  const double* ground_accel; /* host [nsteps]: a_g at each step's load time (t(n) for CentralDifference,
+                                 t(n+1) for ExplicitDifference), used when ground_series is NULL */
+  const fx_time_series* ground_series; /* the excitation's series, evaluated at those times, or NULL */
+  double ground_factor;      /* UniformExcitation factor applied to ground_series */
+  double time;               /* the stage's time before the first step */
 } fx_cd_desc;
-/* nsteps CentralDifference steps (explicit_analysis.cpp:87-117, 661-662, 712-722) with the
- * batch's lumped mass and lumped alphaM mass: from u = U(n), um = U(n-1), v, a (device, in/out)
- * to u = U(n+nsteps), um = U(n+nsteps-1) and the recorded v, a of the last step. */
+/* nsteps explicit steps with the batch's lumped mass (the nodal masses included), the material
+ * history committed after every step (Domain::commit), and the ground load -m a_g of a uniform
+ * excitation at the equations of its direction (ground_load, explicit_analysis.cpp:48-58):
+ *   scheme 0, CentralDifference (:87-117, 661-662, 712-722), with the lumped alphaM mass: from
+ *     u = U(n), um = U(n-1), v, a (device, in/out) to u = U(n+nsteps), um = U(n+nsteps-1) and the
+ *     recorded v, a of the last step;
+ *   scheme 1, ExplicitDifference (:60-85, 699-710): u, a and um = the element velocity Utdot
+ *     (the caller sets it to the committed velocity at a stage's start, :680) in/out, v the
+ *     recorded velocity out; the elements' damping acts at Utdot; needs a positive lumped mass
+ *     at every equation (check_masses). */
 int fx_explicit_cd(fx_batch* b, const fx_cd_desc* d, double* u, double* um, double* v, double* a);
 
 /* ---- PressureIndependMultiYield (J2 multi-surface soil) ------------------------------------ */
@@ -336,25 +416,6 @@ int fx_commit_state(fx_batch* b);
 int fx_pimy_state(fx_batch* b, int committed, double* dst);
 
 /* ---- implicit Newmark stage with Newton (implicit_stage.cpp) ---------------------------------- */
-/* A uniform path time series (PathSeries::getFactor, ground_motion.cpp:17-33): the value at time
- * t is factor * (linear interpolation of values[] at (t - start) / dt); 0 before start and, past
- * the last value, 0 or (use_last) factor * values[count - 1]. A series the reference reads with
- * a leading zero (prepend_zero) is passed with that zero in values[]. */
-/* An excitation's time series (GroundMotions, src/exec/ground_motion.cpp:17-74): with time NULL,
- * PathSeries (uniform step dt from start, use_last holding the last value past the end; the
- * caller prepends OpenSees' leading zero if it wants one); with time set, PathTimeSeries
- * (linear between the given nondecreasing times, 0 outside them, use_last, dt and start unused),
- * whose interval walk is kept in *loc between calls as GroundMotions keeps it per pattern (loc
- * NULL: each fx_newmark call walks from interval 0). Either form is 0 before time 0. */
-typedef struct fx_time_series {
-  const double* values;       /* host [count] */
-  int32_t count;
-  double dt, factor, start;
-  int32_t use_last;
-  const double* time;         /* host [count] or NULL */
-  int32_t* loc;               /* host: the walk's interval, read at the start and written at the end of
-                                 fx_newmark, or NULL */
-} fx_time_series;
 typedef struct fx_newmark_desc {
   double gamma, beta, dt;     /* Newmark constants and step */
   int32_t nsteps;
@@ -380,6 +441,21 @@ typedef struct fx_newmark_desc {
                                  created with FX_BATCH_FP32), in place of FP64 CG (and its GMRES
                                  fallback) */
   double inner_rtol;
+  /* the stage's analysis and algorithm (ImplicitStage::run, implicit_stage.cpp:118-402) */
+  int32_t analysis;           /* 0 transient (Newmark form D), 1 static (LoadControl: the elements see
+                                 v = 0, no inertia, u += dU; gamma, beta and dt unused; v and a kept) */
+  int32_t algorithm;          /* 0 Newton; 1 Linear (one solve per step, which must converge; refused
+                                 for materials with history, :130-140); 2 ModifiedNewton (the tangent once
+                                 per attempt, after the first unbalance, :324-328) */
+  int32_t modified_initial;   /* ModifiedNewton -initial: the initial stiffness K_0 in the stiffness term
+                                 (OperatorCoefficients::initial, :146-150) */
+  int32_t factor_once;        /* ModifiedNewton -factoronce: the tangent kept across steps until a test
+                                 fails or the step size changes (:166-170, 324-328, 343) */
+  double load_increment;      /* LoadControl dlambda (static): the time advances by it per step */
+  double load_min_increment, load_max_increment; /* LoadControl dlambdaMin, dlambdaMax; both 0: not
+                                 given (the increment stays fixed) */
+  int32_t load_num_iter;      /* LoadControl numIter: with min and max given, dlambda *= numIter / (the last
+                                 attempt's iterations), clamped to [min, max] (:262-269, 277-283) */
 } fx_newmark_desc;
 typedef struct fx_newmark_result {
   int32_t steps;              /* converged steps of size dt (each may have been substepped) */
@@ -390,12 +466,17 @@ typedef struct fx_newmark_result {
   int32_t failed_attempts, retries, substepped;
   double time;                /* the domain time after the last converged attempt */
 } fx_newmark_result;
-/* nsteps Newmark steps, each solved by Newton with PCG + Jacobi on K + gamma/(beta dt) C +
- * 1/(beta dt^2) M (stored tangents for history materials), committing the material history after
- * each converged attempt. u, v, a, du: device vectors in/out (du: the Krylov initial guess, the
- * last increment out). Needs FX_BATCH_DYNAMIC (or a PIMY batch). A step that fails is retried
- * and substepped as desc asks (the reference's recover, :385-402); one that cannot be recovered
- * returns FX_OK with result.failed = 1 (the reference stops the stage with a ModelError). */
+/* nsteps steps of an implicit stage (ImplicitStage::run): transient Newmark steps, each solved by
+ * the algorithm on K + gamma/(beta dt) C + 1/(beta dt^2) M, or static LoadControl steps on K
+ * (stored tangents for history materials), committing the material history after each converged
+ * attempt. The solver is the reference's KrylovSolver choice (krylov.cpp:10-22): PCG + Jacobi for
+ * a symmetric positive definite operator (with the GMRES fallback when the batch has
+ * FX_BATCH_GMRES), GMRES(30) + Jacobi from the start otherwise (a PressureIndependMultiYield
+ * batch in a static stage: FX_BATCH_GMRES needed). u, v, a, du: device vectors in/out (du: the
+ * Krylov initial guess, the last increment out). Needs FX_BATCH_DYNAMIC (or a PIMY batch). A step
+ * that fails is retried and (transient only) substepped as desc asks (the reference's recover,
+ * :385-402); one that cannot be recovered returns FX_OK with result.failed = 1 (the reference
+ * stops the stage with a ModelError). */
 int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, double* a, double* du,
                fx_newmark_result* result);
 
@@ -422,6 +503,10 @@ int fx_newmark(fx_batch* b, const fx_newmark_desc* d, double* u, double* v, doub
 #define FX_COLUMN_ONE_STREAM 8u /* the four solve sizes one after another on the pool's stream (default:
                                  forked onto four side streams so they overlap; one stream leaves the
                                  other hardware queues to co-running work) */
+#define FX_COLUMN_TUNE 16u    /* the return-map kernels' register budget (1, 3 or 4 waves per SIMD) by autotune:
+                                 each timed on the first running columns (the same results: the budgets
+                                 differ only in register allocation), the fastest kept per device and pool
+                                 in the autotune cache; FX_COLUMN_WAVES(n) fixes it instead */
 #define FX_COLUMN_COMPACT 4u  /* compacted return map (compacted_kernels.hpp:85-202): an elastic predictor
                                  per point, the plastic points listed, one full update per listed point
                                  (same bits as the fused kernel) */

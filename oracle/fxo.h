@@ -225,6 +225,12 @@ typedef struct fxo_newmark_opts {
   const fxo_series* ground_series; /* a_g = ground_factor * series(t) at each attempt's end time, or NULL */
   double ground_factor;
   double time;           /* the stage's start time */
+  /* ImplicitStage::run (implicit_stage.cpp:118-402) */
+  int32_t analysis;      /* 0 transient, 1 static (LoadControl) */
+  int32_t algorithm;     /* 0 Newton, 1 Linear, 2 ModifiedNewton */
+  int32_t modified_initial, factor_once;
+  double load_increment, load_min_increment, load_max_increment; /* min = max = 0: not given */
+  int32_t load_num_iter;
 } fxo_newmark_opts;
 typedef struct fxo_newmark_stats {
   int32_t steps, newton_iterations, max_newton_iterations, krylov_iterations;
@@ -269,6 +275,14 @@ double fxo_critical_dt(const fxo_compiled* c, const fxo_elastic* params, int sch
 void fxo_cd_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
                 const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt, int32_t nsteps,
                 double* u, double* um, double* v, double* a, int nthreads);
+/* The explicit stage loop (explicit_analysis.cpp:48-117, 687-722): scheme 0 CentralDifference (um =
+ * U(n-1)), 1 ExplicitDifference (um = the element velocity Utdot); ground_dir >= 0: the uniform
+ * excitation's load -m a_g at the equations of that direction, ground_accel[k] at step k's load
+ * time (t(n) CentralDifference, t(n+1) ExplicitDifference). */
+void fxo_explicit_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
+                      const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt,
+                      int32_t nsteps, int scheme, int ground_dir, const double* ground_accel, double* u, double* um,
+                      double* v, double* a, int nthreads);
 
 #ifdef __cplusplus
 }

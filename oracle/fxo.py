@@ -122,7 +122,10 @@ class NewmarkOpts(C.Structure):
                 ("test", C.c_int32), ("tol", C.c_double), ("max_iter", C.c_int32), ("cg_rtol", C.c_double),
                 ("cg_max_it", C.c_int32), ("lanes", C.c_int32), ("ground_accel", C.c_void_p),
                 ("max_retries", C.c_int32), ("num_sublevels", C.c_int32), ("num_substeps", C.c_int32),
-                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double)]
+                ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double),
+                ("analysis", C.c_int32), ("algorithm", C.c_int32), ("modified_initial", C.c_int32),
+                ("factor_once", C.c_int32), ("load_increment", C.c_double), ("load_min_increment", C.c_double),
+                ("load_max_increment", C.c_double), ("load_num_iter", C.c_int32)]
 
 
 class NewmarkStats(C.Structure):
@@ -174,6 +177,9 @@ def lib():
         L.fxo_critical_dt.argtypes = [C.POINTER(Compiled), C.c_void_p, C.c_int]
         L.fxo_cd_run.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                  C.c_void_p, C.c_double, C.c_int32, f64p, f64p, f64p, f64p, C.c_int]
+        L.fxo_explicit_run.argtypes = [C.POINTER(Compiled), C.c_void_p, f64p, C.c_void_p, C.c_void_p, C.c_int32,
+                                       C.c_void_p, C.c_void_p, C.c_double, C.c_int32, C.c_int, C.c_int, C.c_void_p,
+                                       f64p, f64p, f64p, f64p, C.c_int]
         L.fxo_pimy_setup.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(PimyParams)]
         L.fxo_pimy_nstate.restype = C.c_int
         L.fxo_pimy_initial_state.argtypes = [C.POINTER(PimyParams), C.c_void_p]
@@ -355,6 +361,20 @@ class OracleMesh:
                          None if cmv is None else cmv.ctypes.data, None if dsv is None else dsv.ctypes.data, len(le), le.ctypes.data, lh.ctypes.data, dt,
                          nsteps, u, um, v, a, nthreads)
 
+    def explicit_run(self, params, m, cm, dt, nsteps, u, um, v, a, scheme=0, ground_dir=-1, ground_accel=None,
+                     load_eq=None, load_hist=None, dash=None, nthreads=0):
+        """nsteps explicit steps (scheme 0 CentralDifference, 1 ExplicitDifference), a uniform
+        excitation in ground_dir with ground_accel[k] at each step's load time; in place."""
+        cmv = None if cm is None else np.ascontiguousarray(cm, np.float64)
+        dsv = None if dash is None else np.ascontiguousarray(dash, np.float64)
+        le = np.ascontiguousarray(load_eq if load_eq is not None else np.zeros(0), np.int32)
+        lh = np.ascontiguousarray(load_hist if load_hist is not None else np.zeros(0), np.float64)
+        ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
+        lib().fxo_explicit_run(C.byref(self.c), C.cast(params, C.c_void_p), np.ascontiguousarray(m, np.float64),
+                               None if cmv is None else cmv.ctypes.data, None if dsv is None else dsv.ctypes.data,
+                               len(le), le.ctypes.data, lh.ctypes.data, dt, nsteps, scheme, ground_dir,
+                               None if ga is None else ga.ctypes.data, u, um, v, a, nthreads)
+
     # ---- PressureIndependMultiYield batches: points e * 8 + gp, history [point, PIMY_NSTATE]
     def pimy_initial_state(self, elems):
         S = np.zeros((self.nelem * 8, PIMY_NSTATE))
@@ -411,10 +431,12 @@ class OracleMesh:
     def newmark(self, u, v, a, du, dt, nsteps, elastic=None, pimy=None, pimy_state=None, gamma=0.5, beta=0.25,
                 test=1, tol=1e-10, max_iter=25, cg_rtol=1e-12, cg_max_it=10000, ground_dir=None, ground_accel=None,
                 lanes=64, nthreads=0, max_retries=0, num_sublevels=0, num_substeps=2, series=None, ground_factor=1.0,
-                time=0.0):
-        """nsteps Newmark + Newton steps (implicit_stage.cpp) on u, v, a, du (numpy, in place).
-        pimy_state: [C, T] point-major histories (swapped in place on commits). Returns
-        (status, stats dict)."""
+                time=0.0, static=False, algorithm=0, modified_initial=False, factor_once=False, load_increment=None,
+                load_min_increment=0.0, load_max_increment=0.0, load_num_iter=1):
+        """nsteps steps of an implicit stage (implicit_stage.cpp) on u, v, a, du (numpy, in place):
+        Newmark (static=False) or LoadControl (static=True) steps; algorithm 0 Newton, 1 Linear,
+        2 ModifiedNewton. pimy_state: [C, T] point-major histories (swapped in place on commits).
+        Returns (status, stats dict)."""
         s = System()
         s.c = C.cast(C.pointer(self.c), C.c_void_p)
         s.elastic = C.cast(elastic, C.c_void_p) if elastic is not None else None
@@ -438,7 +460,10 @@ class OracleMesh:
             keep += [sv, ser]
         o = NewmarkOpts(gamma, beta, dt, nsteps, test, tol, max_iter, cg_rtol, cg_max_it, lanes,
                         None if ga is None else ga.ctypes.data, max_retries, num_sublevels, num_substeps,
-                        None if ser is None else C.cast(C.pointer(ser), C.c_void_p), ground_factor, time)
+                        None if ser is None else C.cast(C.pointer(ser), C.c_void_p), ground_factor, time,
+                        int(bool(static)), algorithm, int(bool(modified_initial)), int(bool(factor_once)),
+                        dt if load_increment is None else load_increment, load_min_increment, load_max_increment,
+                        load_num_iter)
         st = NewmarkStats()
         rc = lib().fxo_newmark(C.byref(s), C.byref(o), self.ground_direction(ground_dir), u, v, a, du, C.byref(st))
         if pimy is not None and s.C != pimy_state[0].ctypes.data:  # an odd number of commits

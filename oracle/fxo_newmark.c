@@ -5,7 +5,10 @@
  * form_residual :55-68, the integrator constants and operator coefficients :150-159, 283-286,
  * update_tangent :180-193, form_unbalance :212-219, solve :223-238 (Newton accepts an increment
  * whose solve reduced the residual by 1e-6), test_value :239-246, the Newton attempt :274-372,
- * recover :385-402 (retries, substeps) and the stage loop :404-418, the ground acceleration at
+ * recover :385-402 (retries, substeps) and the stage loop :404-418, static LoadControl stages, the
+ * Linear and ModifiedNewton (-initial, -factoronce) algorithms :118-372, the solver KrylovSolver
+ * picks from the operator's symmetry (krylov.cpp:10-42, matrix_free_operator.cpp:5-16: CG with
+ * the GMRES fallback for an SPD operator, GMRES otherwise), the ground acceleration at
  * each attempt's end time (GroundMotions::evaluate, ground_motion.cpp:17-33, 121-145), commit
  * :365-372; the PCG of src/solvers/cg.cpp:24-72 with
  * Jacobi (jacobi.cpp:10-26) on a generic system (elastic operator, or the stored tangents of
@@ -279,22 +282,55 @@ typedef struct nm_state {
   fxo_newmark_stats* st;
   int64_t n;
   int32_t step;
-  double *u, *v, *a, *du, *ut, *vt, *at, *du0, *f, *tmp, *b, *mr, *scratch;
+  double *u, *v, *a, *du, *ut, *vt, *at, *du0, *f, *tmp, *b, *mr, *scratch, *x0;
   fxo_series series; /* the ground series with the run's interval walk (time-array series) */
   int32_t loc;
+  /* LoadControl (:262-269) and the kept tangent of ModifiedNewton -factoronce (:166-170) */
+  double dlambda, dmin, dmax, spec_iter, last_iter, tangent_h;
+  int kept_tangent;
 } nm_state;
 
-/* form_unbalance (:212-219) and form_residual (:55-68): B = -f(u, v) - M a - a_g M r */
+/* form_unbalance (:212-219) and form_residual (:55-68): B = -f(u, v) - M a - a_g M r; static
+ * stages: the elements see v = 0 and there is no inertia */
 static void nm_unbalance(nm_state* w, double ag) {
   const fxo_coeffs mass = {0.0, 0.0, 1.0, 0};
-  sys_force(w->s, w->u, w->v, w->f);
-  sys_apply(w->s, &mass, w->a, w->tmp);
+  const int is_static = w->o->analysis == 1;
+  sys_force(w->s, w->u, is_static ? NULL : w->v, w->f);
+  if (!is_static) sys_apply(w->s, &mass, w->a, w->tmp);
   for (int64_t q = 0; q < w->n; ++q) {
     double r = -w->f[q];
-    r -= w->tmp[q];
+    if (!is_static) r -= w->tmp[q];
     if (w->ground >= 0) r -= ag * w->mr[q];
     w->b[q] = r;
   }
+}
+
+/* operator_symmetry (matrix_free_operator.cpp:5-16): 1 when the operator is SPD (ElasticIsotropic
+ * SPD; PressureIndependMultiYield SPSD, made definite by a mass term) */
+static int nm_spd(const fxo_system* s, const fxo_coeffs* k) {
+  const int positive = k->stiffness >= 0.0 && k->damping >= 0.0 && k->mass >= 0.0 &&
+                       (k->stiffness > 0.0 || k->mass > 0.0);
+  if (!s->pimy) return positive;
+  return positive && k->mass > 0.0;
+}
+
+/* KrylovSolver::solve (krylov.cpp:25-42): CG, GMRES(30) from the same initial guess when CG meets
+ * p^T A p <= 0 (iterations added, indefinite set); GMRES from the start for a non-SPD operator */
+static void nm_solve(nm_state* w, const fxo_coeffs* k, fxo_solve_result* r) {
+  const fxo_newmark_opts* o = w->o;
+  const int64_t n = w->n;
+  if (!nm_spd(w->s, k)) {
+    fxo_gmres_system(w->s, k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, 30, o->lanes, r);
+    return;
+  }
+  memcpy(w->x0, w->du, (size_t)n * sizeof(double));
+  fxo_cg_system(w->s, k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, r);
+  if (!r->indefinite) return;
+  const int32_t cg_it = r->iterations;
+  memcpy(w->du, w->x0, (size_t)n * sizeof(double));
+  fxo_gmres_system(w->s, k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, 30, o->lanes, r);
+  r->iterations += cg_it;
+  r->indefinite = 1;
 }
 
 /* one attempt of size h (:274-372); 1 = converged and committed */
@@ -302,58 +338,104 @@ static int nm_attempt(nm_state* w, double h) {
   const fxo_newmark_opts* o = w->o;
   const int64_t n = w->n;
   const double gamma = o->gamma, beta = o->beta;
+  const int is_static = o->analysis == 1, linear = o->algorithm == 1, modified = o->algorithm == 2;
+  const int factor_once = modified && o->factor_once, initial = modified && o->modified_initial;
   const double time0 = w->st->time;
-  const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
-  const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
-  for (int64_t q = 0; q < n; ++q) { /* newmark_predict (:29-40) */
-    w->ut[q] = w->u[q];
-    w->vt[q] = w->v[q];
-    w->at[q] = w->a[q];
-    w->v[q] = w->v[q] * a1 + w->at[q] * a2;
-    w->a[q] = w->a[q] * a4 + w->vt[q] * a3;
+  if (is_static) { /* LoadControl::newStep */
+    w->dlambda *= w->spec_iter / w->last_iter;
+    if (w->dlambda < w->dmin) w->dlambda = w->dmin;
+    else if (w->dlambda > w->dmax) w->dlambda = w->dmax;
+    memcpy(w->ut, w->u, (size_t)n * sizeof(double));
+    w->st->time += w->dlambda;
+    w->last_iter = 0;
+  } else {
+    const double a1 = 1.0 - gamma / beta, a2 = h * (1.0 - 0.5 * gamma / beta);
+    const double a3 = -1.0 / (beta * h), a4 = 1.0 - 0.5 / beta;
+    for (int64_t q = 0; q < n; ++q) { /* newmark_predict (:29-40) */
+      w->ut[q] = w->u[q];
+      w->vt[q] = w->v[q];
+      w->at[q] = w->a[q];
+      w->v[q] = w->v[q] * a1 + w->at[q] * a2;
+      w->a[q] = w->a[q] * a4 + w->vt[q] * a3;
+    }
+    w->st->time += h;
   }
-  w->st->time += h;
   memcpy(w->du0, w->du, (size_t)n * sizeof(double));
   double ag = 0.0;
   if (w->ground >= 0)
     ag = o->ground_series ? fxo_series_value(&w->series, o->ground_factor, w->st->time)
                           : (o->ground_accel ? o->ground_accel[w->step] : 0.0);
-  const fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), 0};
-  nm_unbalance(w, ag);
-  int iter = 1, iterations = 0, ok = 1;
-  for (;;) {
-    sys_store(w->s, w->u, w->scratch); /* update_tangent: the tangents at u */
-    fxo_solve_result r;
-    fxo_cg_system(w->s, &k, w->b, w->du, o->cg_rtol, 0.0, o->cg_max_it, o->lanes, &r);
-    w->st->krylov_iterations += r.iterations;
-    if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
-      ok = 0;
-      break;
-    }
-    for (int64_t q = 0; q < n; ++q) { /* newmark_update (:43-52) */
-      w->u[q] += w->du[q];
-      w->v[q] += w->du[q] * k.damping;
-      w->a[q] += w->du[q] * k.mass;
-    }
+  fxo_coeffs k = {1.0, gamma / (beta * h), 1.0 / (beta * h * h), initial};
+  if (is_static) k = (fxo_coeffs){1.0, 0.0, 0.0, initial};
+  int iterations = 0, ok = 1;
+  if (linear) { /* :301-313 */
+    w->tangent_h = h;
+    sys_store(w->s, w->u, w->scratch);
     nm_unbalance(w, ag);
-    double value;
-    if (o->test == 2) value = 0.5 * fabs(fxo_dot(n, w->du, w->b, o->lanes));
-    else if (o->test == 0) value = sqrt(fxo_dot(n, w->b, w->b, o->lanes));
-    else value = sqrt(fxo_dot(n, w->du, w->du, o->lanes));
-    iterations = iter;
-    w->st->last_test = value;
-    if (value <= o->tol) break;
-    if (iter >= o->max_iter) {
-      ok = 0;
-      break;
+    fxo_solve_result r;
+    nm_solve(w, &k, &r);
+    w->st->krylov_iterations += r.iterations;
+    if (!r.converged) ok = 0;
+    else
+      for (int64_t q = 0; q < n; ++q) {
+        w->u[q] += w->du[q];
+        if (!is_static) {
+          w->v[q] += w->du[q] * k.damping;
+          w->a[q] += w->du[q] * k.mass;
+        }
+      }
+    iterations = 1;
+  } else {
+    nm_unbalance(w, ag);
+    if (modified && !(factor_once && w->kept_tangent && h == w->tangent_h)) { /* :324-328 */
+      w->tangent_h = h;
+      sys_store(w->s, w->u, w->scratch);
     }
-    ++iter;
+    w->kept_tangent = factor_once;
+    int iter = 1;
+    for (;;) {
+      if (!modified) { /* update_tangent: the tangents at u */
+        w->tangent_h = h;
+        sys_store(w->s, w->u, w->scratch);
+      }
+      fxo_solve_result r;
+      nm_solve(w, &k, &r);
+      w->st->krylov_iterations += r.iterations;
+      if (!r.converged && !(r.final_residual <= 1e-6 * r.rhs_norm)) {
+        ok = 0;
+        break;
+      }
+      for (int64_t q = 0; q < n; ++q) { /* newmark_update (:43-52), or u += dU */
+        w->u[q] += w->du[q];
+        if (!is_static) {
+          w->v[q] += w->du[q] * k.damping;
+          w->a[q] += w->du[q] * k.mass;
+        }
+      }
+      nm_unbalance(w, ag);
+      double value;
+      if (o->test == 2) value = 0.5 * fabs(fxo_dot(n, w->du, w->b, o->lanes));
+      else if (o->test == 0) value = sqrt(fxo_dot(n, w->b, w->b, o->lanes));
+      else value = sqrt(fxo_dot(n, w->du, w->du, o->lanes));
+      iterations = iter;
+      w->st->last_test = value;
+      if (value <= o->tol) break;
+      if (iter >= o->max_iter) {
+        ok = 0;
+        w->kept_tangent = 0; /* OpenSees: factorOnce = 1 after a failed test */
+        break;
+      }
+      ++iter;
+    }
   }
   w->st->newton_iterations += iterations;
+  if (is_static) w->last_iter = iterations; /* LoadControl counts the failed attempt's updates too */
   if (!ok) {
     memcpy(w->u, w->ut, (size_t)n * sizeof(double));
-    memcpy(w->v, w->vt, (size_t)n * sizeof(double));
-    memcpy(w->a, w->at, (size_t)n * sizeof(double));
+    if (!is_static) {
+      memcpy(w->v, w->vt, (size_t)n * sizeof(double));
+      memcpy(w->a, w->at, (size_t)n * sizeof(double));
+    }
     memcpy(w->du, w->du0, (size_t)n * sizeof(double));
     w->st->time = time0;
     ++w->st->failed_attempts;
@@ -369,14 +451,14 @@ static int nm_attempt(nm_state* w, double h) {
   return 1;
 }
 
-/* recover (:385-402) */
+/* recover (:385-402); static stages retry only */
 static int nm_recover(nm_state* w, double h, int level) {
   if (nm_attempt(w, h)) return 1;
   for (int r = 1; r <= w->o->max_retries; ++r) {
     ++w->st->retries;
     if (nm_attempt(w, h)) return 1;
   }
-  if (level >= w->o->num_sublevels) return 0;
+  if (w->o->analysis == 1 || level >= w->o->num_sublevels) return 0;
   ++w->st->substepped;
   const double hs = h / (double)w->o->num_substeps;
   for (int i = 0; i < w->o->num_substeps; ++i)
@@ -407,7 +489,15 @@ int fxo_newmark(fxo_system* s, const fxo_newmark_opts* o, int ground_direction, 
     w.loc = w.series.loc ? *w.series.loc : 0;
     w.series.loc = &w.loc;
   }
-  double** bufs[] = {&w.ut, &w.vt, &w.at, &w.du0, &w.f, &w.tmp, &w.b, &w.scratch};
+  const int adapt = o->load_min_increment != 0.0 && o->load_max_increment != 0.0;
+  w.dlambda = o->load_increment;
+  w.dmin = adapt ? o->load_min_increment : o->load_increment;
+  w.dmax = adapt ? o->load_max_increment : o->load_increment;
+  w.spec_iter = adapt ? (double)o->load_num_iter : 1.0;
+  w.last_iter = w.spec_iter;
+  w.tangent_h = o->dt;
+  w.kept_tangent = 0;
+  double** bufs[] = {&w.ut, &w.vt, &w.at, &w.du0, &w.f, &w.tmp, &w.b, &w.scratch, &w.x0};
   for (size_t i = 0; i < sizeof(bufs) / sizeof(bufs[0]); ++i) *bufs[i] = (double*)malloc((size_t)(n + 1) * sizeof(double));
   w.mr = (double*)calloc((size_t)n + 1, sizeof(double));
   if (ground_direction >= 0) /* mass_times_ground */

@@ -135,25 +135,53 @@ double fxo_critical_dt(const fxo_compiled* c, const fxo_elastic* params, int sch
   return dt;
 }
 
-void fxo_cd_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
-                const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt, int32_t nsteps,
-                double* u, double* um, double* v, double* a, int nthreads) {
+void fxo_explicit_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
+                      const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt,
+                      int32_t nsteps, int scheme, int ground_dir, const double* ground_accel, double* u, double* um,
+                      double* v, double* a, int nthreads) {
   const int32_t n = c->neq;
   const double c2 = 0.5 / dt, c3 = 1.0 / (dt * dt);
   double* f = (double*)malloc(((size_t)n + 1) * sizeof(double));
   double* ve = (double*)malloc(((size_t)n + 1) * sizeof(double));
   double* P = (double*)calloc((size_t)n + 1, sizeof(double));
+  signed char* dir = (signed char*)malloc((size_t)n + 1);
+  for (int32_t q = 0; q < n; ++q) dir[q] = -1;
+  for (int32_t i = 0; i < 3 * c->nnode; ++i)
+    if (c->node_eq[i] >= 0) dir[c->node_eq[i]] = (signed char)(i % 3);
   for (int32_t k = 0; k < nsteps; ++k) {
-    /* loads at t(n) */
     for (int32_t j = 0; j < nload; ++j) P[load_eq[j]] = load_hist[(int64_t)k * nload + j];
-    /* garbage velocity for the element damping force */
+    const double ag_k = ground_dir >= 0 ? ground_accel[k] : 0.0;
+    if (scheme == 1) {
+      /* ExplicitDifference (explicit_analysis.cpp:60-85): vh (in um) += dt a; u += dt vh; the
+       * force at vh; a = R / m; the recorded velocity vh + dt/8 (3 a(n+1) + a(n)) */
+      for (int32_t q = 0; q < n; ++q) {
+        um[q] += a[q] * dt;
+        u[q] += um[q] * dt;
+      }
+      fxo_force(c, params, u, um, f, nthreads);
+      if (dash)
+        for (int32_t q = 0; q < n; ++q) f[q] += dash[q] * um[q];
+      for (int32_t q = 0; q < n; ++q) {
+        double g = 0.0; /* ground_load (:48-58) */
+        if (dir[q] == ground_dir) g -= m[q] * ag_k;
+        const double R = -f[q] + P[q] + g;
+        const double a_new = R / m[q];
+        const double t = 3.0 * a_new + a[q];
+        v[q] = um[q] + t * (dt * 0.125);
+        a[q] = a_new;
+      }
+      continue;
+    }
+    /* CentralDifference: loads at t(n); the garbage velocity for the element damping force */
     for (int32_t q = 0; q < n; ++q) ve[q] = um[q] * -c2;
     fxo_force(c, params, u, ve, f, nthreads);
     if (dash)
       for (int32_t q = 0; q < n; ++q) f[q] += dash[q] * ve[q];
     for (int32_t q = 0; q < n; ++q) {
+      double g = 0.0;
+      if (dir[q] == ground_dir) g -= m[q] * ag_k;
       const double ag = u[q] * (-2.0 * c3) + um[q] * c3;
-      const double R = -f[q] - m[q] * ag + P[q];
+      const double R = -f[q] - m[q] * ag + P[q] + g;
       const double A = m[q] * c3 + (cm ? cm[q] : 0.0) * c2;
       const double U = R / A;
       double Udot = U * 3.0;
@@ -170,4 +198,11 @@ void fxo_cd_run(const fxo_compiled* c, const fxo_elastic* params, const double* 
   free(f);
   free(ve);
   free(P);
+  free(dir);
+}
+
+void fxo_cd_run(const fxo_compiled* c, const fxo_elastic* params, const double* m, const double* cm,
+                const double* dash, int32_t nload, const int32_t* load_eq, const double* load_hist, double dt, int32_t nsteps,
+                double* u, double* um, double* v, double* a, int nthreads) {
+  fxo_explicit_run(c, params, m, cm, dash, nload, load_eq, load_hist, dt, nsteps, 0, -1, NULL, u, um, v, a, nthreads);
 }

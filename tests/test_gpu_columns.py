@@ -58,12 +58,12 @@ def level_values(pm, x, n):
 
 
 def run_pool(ctx, cases, nsteps, ag, max_iter=25, tol=1e-10, iters_per_call=4, pool=None, mixed=False,
-             compact=False, test=1):
+             compact=False, test=1, tune=False, waves=0):
     own = pool is None
     if own:
         pool = core.ColumnPool(ctx, max_elements=sum(len(mo) for _, mo in cases) + 64, max_columns=len(cases) + 2,
                                dt=DT, max_record=10 * nsteps * (len(cases) + 1) + 16, max_iter=max_iter, tol=tol,
-                               mixed=mixed, compact=compact, test=test)
+                               mixed=mixed, compact=compact, test=test, tune=tune, waves=waves)
     keep = []
     ids = [pool.add(dev_mats(m, keep), mo, nsteps, ag) for m, mo in cases]
     assert all(i is not None for i in ids)
@@ -86,6 +86,32 @@ def compare(res, hist_u, hist_v, hist_a, u, v, a, label):
             rel(res["v"], v), rel(res["a"], a))
     print(label, "errors u v a (history), u v a (final):", ["%.2e" % e for e in errs])
     assert max(errs) <= 1e-10, errs
+
+
+def oracle_column(mats, mat_of, nsteps, ag, test=1, tol=1e-10):
+    """The oracle's eight-point implicit stage on the column model: the top level's u, v, a per
+    step, the levels' final u, v, a and the committed histories at Gauss point 0 of each element
+    (bottom to top)."""
+    n = len(mat_of)
+    model = column_model(n, len(mats), mat_of)
+    pm = core.Mesh(model)
+    om = fxo.OracleMesh(model)
+    pe = fxo.pimy_elems([mats[g] for g in pm.param_group], stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    N = om.neq
+    u, v, a, du = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+    gd = np.zeros(N)
+    q = om.node_eq.reshape(-1, 3)[:, 0]
+    gd[q[q >= 0]] = 1.0
+    hist = []
+    for k in range(nsteps):
+        rc, st = om.newmark(u, v, a, du, DT, 1, pimy=pe, pimy_state=[Cs, Ts], ground_dir=gd, ground_accel=ag[k:k + 1],
+                            test=test, tol=tol)
+        assert rc == 0 and st["steps"] == 1
+        hist.append(np.concatenate([level_values(pm, w, n)[-1] for w in (u, v, a)]))
+    final = [level_values(pm, w, n) for w in (u, v, a)]
+    return np.array(hist), final, Cs[8 * np.argsort(pm.elem_order)]
 
 
 @pytest.mark.parametrize("n,mass_type,test", [(6, 1, 1), (9, 0, 1), (8, 1, 0)])
@@ -126,6 +152,47 @@ def test_columns_match_the_oracle(ctx, n, mass_type, test):
     print("history error", err)
     assert err <= 1e-10
     assert np.max(res["state"][:, 26]) > 0, "the record drives the column into the plastic range"
+
+
+def test_a_tall_column_matches_the_oracle_over_many_plastic_steps(ctx):
+    """A column of 400 elements (the top of C4's height range, the 8-wave solve workgroup) over
+    80 steps of a strong record, against the oracle's eight-point stage step by step: top-level
+    history, final state and committed histories within 1e-10, with points past their first
+    surface in at least 50 of the steps."""
+    n, nsteps = 400, 80
+    mats, mat_of = column_case(n, 4242, nlayer=10)
+    ag = S.ground_motion(nsteps, DT, 13, pga=6.0)
+    res = run_pool(ctx, [(mats, mat_of)], nsteps, ag)[0]
+    assert res["status"]["status"] == 1 and res["status"]["steps"] == nsteps, res["status"]
+    model = column_model(n, len(mats), mat_of)
+    pm = core.Mesh(model)
+    om = fxo.OracleMesh(model)
+    per_set = [mats[g] for g in pm.param_group]
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    N = om.neq
+    u, v, a, du = np.zeros(N), np.zeros(N), np.zeros(N), np.zeros(N)
+    gd = np.zeros(N)
+    q = om.node_eq.reshape(-1, 3)[:, 0]
+    gd[q[q >= 0]] = 1.0
+    hu, hv, ha = [], [], []
+    plastic_steps = 0
+    for k in range(nsteps):
+        rc, st = om.newmark(u, v, a, du, DT, 1, pimy=pe, pimy_state=[Cs, Ts], ground_dir=gd, ground_accel=ag[k:k + 1],
+                            test=1, tol=1e-10)
+        assert rc == 0 and st["steps"] == 1
+        plastic_steps += int(np.any(Cs[:, 26] > 0))
+        hu.append(level_values(pm, u, n)[-1])
+        hv.append(level_values(pm, v, n)[-1])
+        ha.append(level_values(pm, a, n)[-1])
+    compare(res, np.array(hu), np.array(hv), np.array(ha), level_values(pm, u, n), level_values(pm, v, n),
+            level_values(pm, a, n), f"oracle n={n}, {nsteps} steps")
+    elem_of_level = np.argsort(pm.elem_order)
+    err = history_error(res["state"], Cs[8 * elem_of_level])
+    print("history error", err, "plastic steps", plastic_steps)
+    assert err <= 1e-10
+    assert plastic_steps >= 50, plastic_steps
 
 
 @pytest.mark.parametrize("n", [40, 100, 200, 330])
@@ -275,7 +342,7 @@ def test_mixed_precision_columns_match_fp64(ctx):
 def test_compacted_return_map_gives_the_same_bits(ctx):
     """Predict / compact / correct (compacted_kernels.hpp:85-202): the corrector redoes the whole
     update, so every result equals the fused kernel's bit for bit, with elastic and plastic
-    points in the same iterations."""
+    points in the same iterations; the compacted columns also match the oracle (1e-10)."""
     nsteps = 14
     ag = S.ground_motion(nsteps, DT, 23, pga=4.0)
     cases = [column_case(n, 700 + n) for n in (7, 64, 65, 250, 257, 512)]
@@ -288,3 +355,30 @@ def test_compacted_return_map_gives_the_same_bits(ctx):
         assert y["status"] == x["status"]
         plastic += int(np.sum(x["state"][:, 26] > 0))
     assert plastic > 0
+    # and the compacted columns against the oracle's eight-point stage (two of the sizes)
+    for (mats, mat_of), y in list(zip(cases, comp))[:2]:
+        hist, final, state = oracle_column(mats, mat_of, nsteps, ag)
+        compare(y, hist[:, 0:3], hist[:, 3:6], hist[:, 6:9], *final, f"compacted vs oracle n={len(mat_of)}")
+        assert history_error(y["state"], state) <= 1e-10
+
+
+@pytest.mark.parametrize("compact", [False, True])
+def test_autotuned_register_budget_gives_the_same_bits(ctx, compact, tmp_path, monkeypatch):
+    """FX_COLUMN_TUNE: the return maps are timed at each register budget on the first running
+    columns (repeats of a pure function of the columns' state) and the fastest kept; every
+    result equals the fixed-budget pool's bit for bit, with the cache empty and again with the
+    choice read back from it."""
+    monkeypatch.setenv("FX_TUNE_CACHE", str(tmp_path / "autotune.txt"))
+    nsteps = 10
+    ag = S.ground_motion(nsteps, DT, 29, pga=4.0)
+    cases = [column_case(n, 900 + n) for n in (9, 70, 130)]
+    fixed = run_pool(ctx, cases, nsteps, ag, compact=compact, waves=3)
+    for attempt in range(2):
+        tuned = run_pool(ctx, cases, nsteps, ag, compact=compact, tune=True)
+        for x, y in zip(fixed, tuned):
+            for key in ("u", "v", "a", "history", "state"):
+                np.testing.assert_array_equal(y[key], x[key], err_msg=key)
+            assert y["status"] == x["status"]
+        text = (tmp_path / "autotune.txt").read_text()
+        assert "column_pool" in text, text
+

@@ -128,3 +128,59 @@ def test_central_difference_matches_oracle(ctx, name, nsteps):
     assert rel(dum.get(), um) <= 1e-10
     assert rel(dv.get(), v) <= 1e-8
     assert rel(da.get(), a) <= 1e-8
+
+
+@pytest.mark.parametrize("name,nsteps", [("c3_small_lattice", 80), ("distorted4_damped", 41), ("column10_tied", 60)])
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_explicit_schemes_under_ground_motion(ctx, name, nsteps, scheme):
+    """CentralDifference and ExplicitDifference (explicit_analysis.cpp:60-117, 687-722) with a
+    uniform excitation's load -m a_g (:48-58) from a time series evaluated at each step's load
+    time (t(n) for CentralDifference, t(n+1) for ExplicitDifference), point loads and dampers
+    where the model has them: against the oracle's loop (per-step accelerations)."""
+    model, pm, batch, om, op = setup(ctx, name)
+    n = pm.neq
+    dt = 0.5 * om.critical_dt(op, scheme=scheme)
+    m, am = om.lumped_mass(op)
+    dash = pm.to_equations(model["dashpots"]) if "dashpots" in model else None
+    series = dict(values=np.concatenate([[0.0], 2.5 * np.sin(np.arange(1, nsteps + 4) * 0.21)]), dt=dt * 0.9,
+                  factor=1.3)
+    t0 = 0.37 * dt
+    times = t0 + dt * (np.arange(nsteps) + (1 if scheme == 1 else 0))
+    ag = np.array([fxo.series_value(series["values"], series["dt"], t, factor=series["factor"], fact=0.8)
+                   for t in times])
+    load_eq = np.array([n - 1, n // 2], np.int32)
+    t = np.arange(nsteps) * dt
+    hist = np.stack([1e3 * np.sin(60 * t), -3e2 * np.cos(35 * t)], axis=1)
+    u0 = meshes.random_vector(n, 7) * 1e-5
+    v0 = meshes.random_vector(n, 8) * 1e-4
+    um0 = u0 - v0 * dt if scheme == 0 else v0.copy()  # ED: the element velocity starts at v
+    u, um, v, a = u0.copy(), um0.copy(), v0.copy(), np.zeros(n)
+    cm = None
+    if scheme == 0 and (np.any(am) or dash is not None):
+        cm = (am if np.any(am) else 0.0) + (dash if dash is not None else 0.0)
+    om.explicit_run(op, m, cm, dt, nsteps, u, um, v, a, scheme=scheme, ground_dir=0, ground_accel=ag,
+                    load_eq=load_eq, load_hist=hist.ravel(), dash=dash)
+    du, dum = core.DeviceArray.from_host(ctx, u0), core.DeviceArray.from_host(ctx, um0)
+    dv, da = core.DeviceArray.from_host(ctx, v0), core.DeviceArray(ctx, n)
+    da.zero()
+    ddash = core.DeviceArray.from_host(ctx, dash) if dash is not None else None
+    print(f"explicit {name} scheme {scheme}: {nsteps} steps", flush=True)
+    batch.explicit_cd(dt, nsteps, du, dum, dv, da, dash=ddash, load_eq=load_eq, load_hist=hist.ravel(), scheme=scheme,
+                      ground_dir=0, series=series, ground_factor=0.8, time=t0)
+    assert np.max(np.abs(u - u0)) > 1e-3 * np.max(np.abs(u0))
+    assert rel(du.get(), u) <= 1e-10
+    assert rel(dum.get(), um) <= 1e-10
+    assert rel(dv.get(), v) <= 1e-8
+    assert rel(da.get(), a) <= 1e-8
+
+
+def test_explicit_difference_needs_masses_everywhere(ctx):
+    model = meshes.column(6)
+    model["params"] = [dict(p, rho=0.0) for p in model["params"]]
+    pm = core.Mesh(model)
+    batch = core.Batch(ctx, pm, pm.params_for(model))
+    d = [core.DeviceArray(ctx, pm.neq) for _ in range(4)]
+    for x in d:
+        x.zero()
+    with pytest.raises(core.FxError, match="positive lumped mass"):
+        batch.explicit_cd(1e-4, 2, *d, scheme=1)

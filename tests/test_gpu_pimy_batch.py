@@ -176,6 +176,44 @@ def test_explicit_loop_commits_the_history(ctx):
     assert history_error(batch.pimy_state(True), Co) <= 1e-9
 
 
+def test_explicit_difference_commits_the_history(ctx):
+    """ExplicitDifference (explicit_analysis.cpp:60-85) on a PressureIndependMultiYield column under
+    a ground acceleration: the oracle's loop of predictor, return-map force at u, a = R / m with
+    R = -f - m a_g at the x equations, recorded velocity, commit."""
+    model, pm, batch, om, per_set = setup(ctx, "column12_tied")
+    n = pm.neq
+    batch.set_stage(1)
+    pe = fxo.pimy_elems(per_set, stage=1)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
+    m, _ = om.lumped_mass(pe_as_elastic(per_set))
+    dt, nsteps = 2e-4, 40
+    ag = 8.0 * np.sin(np.arange(1, nsteps + 1) * dt * 90.0)
+    xdir = np.zeros(n, bool)
+    for i, q in enumerate(pm.node_eq):
+        if q >= 0 and i % 3 == 0:
+            xdir[q] = True
+    u, vh, v, a = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+    for k in range(nsteps):
+        vh = vh + a * dt
+        u = u + vh * dt
+        f = om.pimy_force(pe, Co, To, u)
+        R = -f + np.where(xdir, -m * ag[k], 0.0)
+        an = R / m
+        v = vh + (3.0 * an + a) * (dt * 0.125)
+        a = an
+        Co, To = To, Co
+    d = [core.DeviceArray(ctx, n) for _ in range(4)]
+    for x in d:
+        x.zero()
+    batch.explicit_cd(dt, nsteps, d[0], d[1], d[2], d[3], scheme=1, ground_dir=0, ground_accel=ag)
+    assert rel(d[0].get(), u) <= 1e-9
+    assert rel(d[1].get(), vh) <= 1e-9
+    assert rel(d[2].get(), v) <= 1e-8
+    assert history_error(batch.pimy_state(True), Co) <= 1e-9
+    assert np.max(Co[:, 26]) > 0, "the shaking reached the plastic range"
+
+
 def pe_as_elastic(per_set):
     """The lumped mass only needs rho (elastic parameter records with the PIMY sets' rho)."""
     return fxo.elastic_params([dict(E=1.0, nu=0.3, rho=m["rho"], mass_type=m["mass_type"]) for m in per_set])
@@ -183,8 +221,8 @@ def pe_as_elastic(per_set):
 
 def test_compacted_return_map_gives_the_same_bits(ctx):
     """FX_BATCH_COMPACT (compacted_kernels.hpp:85-202): forces, stored tangents (through the
-    operator) and trial histories equal to the fused kernel's bit for bit, along a walk with
-    elastic and plastic points, commits between steps."""
+    operator) and trial histories equal to the fused kernel's bit for bit, and to the oracle within
+    1e-10, along a walk with elastic and plastic points, commits between steps."""
     model, mats = pimy_model("distorted3")
     pm = core.Mesh(model)
     per_set = [mats[g] for g in pm.param_group]
@@ -200,27 +238,40 @@ def test_compacted_return_map_gives_the_same_bits(ctx):
     dir_ = meshes.random_vector(n, 31)
     du = core.DeviceArray(ctx, n)
     f1, f2 = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
-    x = core.DeviceArray.from_host(ctx, meshes.random_vector(n, 32))
+    xh = meshes.random_vector(n, 32)
+    x = core.DeviceArray.from_host(ctx, xh)
     y1, y2 = core.DeviceArray(ctx, n), core.DeviceArray(ctx, n)
+    # the oracle along the same walk: the compacted results are also checked against it
+    om = fxo.OracleMesh(model)
+    pe = fxo.pimy_elems(per_set, stage=0)
+    Co = om.pimy_initial_state(pe)
+    To = Co.copy()
     plastic = 0
     for step in range(7):
         if step == 1:
             fused.set_stage(1)
             comp.set_stage(1)
-        du.upload(dir_ * (3e-3 * (step + 1) * (1 if step % 3 else -0.6)))
+            pe = fxo.pimy_elems(per_set, stage=1)
+        u = dir_ * (3e-3 * (step + 1) * (1 if step % 3 else -0.6))
+        du.upload(u)
         fused.store_tangents(du, f1)
         comp.store_tangents(du, f2)
         np.testing.assert_array_equal(f2.get(), f1.get())
         np.testing.assert_array_equal(comp.pimy_state(False), fused.pimy_state(False))
+        f_o, D_o = om.pimy_force(pe, Co, To, u, want_D=True)
+        assert rel(f2.get(), f_o) <= 1e-10, step
+        assert history_error(comp.pimy_state(False), To) <= 1e-10, step
         fused.apply(x, y1)
         comp.apply(x, y2)
         np.testing.assert_array_equal(y2.get(), y1.get())
+        assert rel(y2.get(), om.apply_gp(pe, D_o, xh)) <= 1e-10, step
         fused.force(du, f1)
         comp.force(du, f2)
         np.testing.assert_array_equal(f2.get(), f1.get())
         plastic = max(plastic, int(np.sum(fused.pimy_state(False)[:, 26] > 0)))
         fused.commit()
         comp.commit()
+        Co, To = To, Co
     assert plastic > 0
 
 

@@ -277,3 +277,110 @@ def test_ground_load_is_the_mass_times_a_rigid_translation():
         assert np.max(np.abs(mg - m * r)) <= 1e-14 * np.max(m)
         free_only = om.apply(op, r, fxo.coeffs(0.0, 0.0, 1.0))
         assert np.max(np.abs(free_only - mg)) >= 1e-2 * np.max(m)
+
+
+def _dense(om, op):
+    E = np.eye(om.neq)
+    K = np.column_stack([om.apply(op, e) for e in E])
+    M = np.column_stack([om.apply(op, e, fxo.coeffs(0.0, 0.0, 1.0)) for e in E])
+    return K, M
+
+
+def test_static_stage_solves_the_body_load_and_keeps_velocities():
+    """A static LoadControl stage (implicit_stage.cpp:118-402, analysis Static) on an elastic
+    column: K u = F_body by a dense solve (the body force does not scale with the load factor),
+    Newton converging in one iteration per step, Linear and ModifiedNewton (plain, -initial,
+    -factoronce) giving the same u, v and a untouched, the time advancing by the increment."""
+    model = meshes.column(8)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    K, _ = _dense(om, op)
+    U = np.linalg.solve(K, om.body_load(op))
+    v0 = meshes.random_vector(n, 3)
+    a0 = meshes.random_vector(n, 4)
+    for algorithm, modified_initial, factor_once in ((0, False, False), (1, False, False), (2, False, False),
+                                                     (2, True, False), (2, False, True)):
+        u, v, a, du = np.zeros(n), v0.copy(), a0.copy(), np.zeros(n)
+        rc, st = om.newmark(u, v, a, du, 1.0, 3, elastic=op, static=True, algorithm=algorithm,
+                            modified_initial=modified_initial, factor_once=factor_once, load_increment=0.1, test=0,
+                            tol=1e-6, time=2.0)
+        assert rc == 0 and st["steps"] == 3 and st["commits"] == 3, (algorithm, st)
+        assert np.max(np.abs(u - U)) <= 1e-9 * np.max(np.abs(U)), algorithm
+        np.testing.assert_array_equal(v, v0)
+        np.testing.assert_array_equal(a, a0)
+        assert abs(st["time"] - 2.3) <= 1e-12
+        if algorithm == 1:
+            assert st["newton_iterations"] == 3  # one solve per step
+        else:
+            # step 1: one iteration; steps 2-3 start in equilibrium: one iteration each
+            assert st["newton_iterations"] == 3 and st["max_newton_iterations"] == 1, st
+
+
+def test_load_control_increment_adapts_to_the_iterations():
+    """LoadControl with dlambdaMin, dlambdaMax and numIter (implicit_stage.cpp:262-269, 277-283,
+    351): dlambda *= numIter / (the last attempt's iterations), clamped; hand-derived sequence for
+    an elastic column whose steps take one Newton iteration each: 0.1, 0.2, 0.4, 0.5, 0.5."""
+    model = meshes.column(6)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    v, a = np.zeros(n), np.zeros(n)
+    # one call carries the LoadControl state: run the five steps together
+    u, du = np.zeros(n), np.zeros(n)
+    rc, st = om.newmark(u, v, a, du, 1.0, 5, elastic=op, static=True, load_increment=0.1, load_min_increment=0.05,
+                        load_max_increment=0.5, load_num_iter=2, test=0, tol=1e-6)
+    assert rc == 0 and abs(st["time"] - (0.1 + 0.2 + 0.4 + 0.5 + 0.5)) <= 1e-12, st["time"]
+
+
+def test_linear_and_modified_newton_transient_match_the_textbook_recurrence():
+    """Transient Linear and ModifiedNewton (plain, -initial, -factoronce) on an elastic column:
+    the same textbook Newmark recurrence as Newton (the system is linear: one solve per step)."""
+    model = meshes.column(8)
+    model["params"] = [dict(p, rayleigh=(0.4, 2e-4, 0.0, 0.0)) for p in model["params"]]
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params([model["params"][g] for g in om.param_group])
+    n = om.neq
+    h, nsteps = 0.004, 12
+    ag = 2.0 * np.sin(np.arange(1, nsteps + 1) * h * 2 * np.pi * 3)
+    r = ground_dir(om, 0)
+    ref = None
+    for algorithm, mi, fo in ((0, False, False), (1, False, False), (2, False, False), (2, True, False),
+                              (2, False, True)):
+        u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+        rc, st = om.newmark(u, v, a, du, h, nsteps, elastic=op, ground_dir=r, ground_accel=ag, tol=1e-12, test=1,
+                            algorithm=algorithm, modified_initial=mi, factor_once=fo)
+        assert rc == 0 and st["steps"] == nsteps
+        if ref is None:
+            ref = (u.copy(), v.copy(), a.copy())
+        else:
+            assert np.max(np.abs(u - ref[0])) <= 1e-9 * np.max(np.abs(ref[0])), algorithm
+            assert np.max(np.abs(a - ref[2])) <= 1e-7 * np.max(np.abs(ref[2])), algorithm
+
+
+def test_pimy_static_stage_by_gmres_ends_in_equilibrium():
+    """A PressureIndependMultiYield column in a static stage: the operator is symmetric
+    semi-definite without a mass term, so the solver is GMRES (matrix_free_operator.cpp:5-16,
+    krylov.cpp:10-22); Newton, ModifiedNewton and ModifiedNewton -initial each end every step in
+    equilibrium, -f(u) = 0 to the test tolerance, under the body load."""
+    model = meshes.column(10)
+    om = fxo.OracleMesh(model)
+    mats = [dict(rho=2000.0, G=G, K=2.17 * G, cohesion=c, gamma_peak=0.1, phi=0.0, pref=100e3, pdc=0.0,
+                 b=(0.0, 0.0, -2000.0 * 9.81), mass_type=1) for G, c in ((40e6, 30e3), (80e6, 60e3), (150e6, 90e3))]
+    sets = [mats[g] for g in om.param_group]
+    n = om.neq
+    for algorithm, mi in ((0, False), (2, False), (2, True)):
+        for stage in (0, 1):
+            pe = fxo.pimy_elems(sets, stage=stage)
+            Cs = om.pimy_initial_state(pe)
+            Ts = Cs.copy()
+            u, v, a, du = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(n)
+            rc, st = om.newmark(u, v, a, du, 1.0, 2, pimy=pe, pimy_state=[Cs, Ts], static=True, algorithm=algorithm,
+                                modified_initial=mi, load_increment=1.0, test=0, tol=1e-3, max_iter=60)
+            assert rc == 0 and st["steps"] == 2, (algorithm, mi, stage, st)
+            T2 = Cs.copy()
+            f = om.pimy_force(pe, Ts, T2, u)
+            assert np.linalg.norm(f) <= 1e-3 * 1.0001, (algorithm, mi, stage, np.linalg.norm(f))
+            assert np.max(np.abs(u)) > 0
+            if stage == 1:
+                assert np.max(Cs[:, 26]) > 0, "the body load reached the plastic range"

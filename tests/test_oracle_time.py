@@ -147,3 +147,69 @@ def test_central_difference_against_the_recurrence(damping):
     assert np.max(np.abs(um - Um)) <= 1e-10 * scale
     assert np.max(np.abs(v - V)) <= 1e-8 * np.max(np.abs(V))
     assert np.max(np.abs(a - A)) <= 1e-6 * np.max(np.abs(A))
+
+
+def test_explicit_difference_matches_the_closed_form_leapfrog():
+    """ExplicitDifference (explicit_analysis.cpp:60-85) started in one undamped mode of a dense
+    system: the predictor and update form the leapfrog recurrence u(n+1) - 2 u(n) + u(n-1) =
+    dt^2 a(n), whose solution in a mode of circular frequency w is u(n) = cos(n t) + c sin(n t)
+    with cos t = 1 - (w dt)^2 / 2 and c = (cos t - 1) / sin t (u(0) = 1, vh(0) = 0,
+    a(0) = -w^2); the recorded velocity is vh + dt/8 (3 a(n+1) + a(n))."""
+    model = meshes.column(6)
+    om, op = compiled(model)
+    n = om.neq
+    m, _ = om.lumped_mass(op)
+    K = np.column_stack([om.apply(op, e) for e in np.eye(n)])
+    # the mass-normalised modes of M^-1/2 K M^-1/2
+    Mh = 1.0 / np.sqrt(m)
+    w2, Q = np.linalg.eigh(Mh[:, None] * K * Mh[None, :])
+    for mode in (0, 3):
+        w = np.sqrt(w2[mode])
+        phi = Mh * Q[:, mode]
+        dt = 0.4 * 2.0 / np.sqrt(w2[-1])  # 0.4 of the stable step
+        nsteps = 40
+        u = phi.copy()
+        vh = np.zeros(n)
+        a = -w * w * phi
+        v = np.zeros(n)
+        # no body force: the column's own gravity would add a static part
+        op0 = fxo.elastic_params([dict(model["params"][g], b=(0.0, 0.0, 0.0)) for g in om.param_group])
+        om.explicit_run(op0, m, None, dt, nsteps, u, vh, v, a, scheme=1)
+        th = np.arccos(1.0 - 0.5 * (w * dt) ** 2)
+        cc = (np.cos(th) - 1.0) / np.sin(th)
+        want = np.cos(nsteps * th) + cc * np.sin(nsteps * th)
+        scale = np.max(np.abs(phi))
+        assert np.max(np.abs(u - want * phi)) <= 1e-9 * scale, (mode, np.max(np.abs(u - want * phi)))
+        # the element velocity: vh(n) = (u(n) - u(n-1)) / dt
+        prev = np.cos((nsteps - 1) * th) + cc * np.sin((nsteps - 1) * th)
+        assert np.max(np.abs(vh - (want - prev) / dt * phi)) <= 1e-8 * w * scale
+        # a(n) = -w^2 u(n), and the recorded velocity from it
+        assert np.max(np.abs(a + w * w * want * phi)) <= 1e-8 * w * w * scale
+        a_prev = -w * w * prev * phi
+        assert np.max(np.abs(v - (vh + dt / 8 * (3 * a + a_prev)))) <= 1e-8 * w * scale
+
+
+@pytest.mark.parametrize("scheme", [0, 1])
+def test_explicit_ground_motion_is_a_rigid_base_translation(scheme):
+    """A uniform excitation's load -m a_g (ground_load, explicit_analysis.cpp:48-58) on a column
+    whose base is fixed: for a constant a_g applied from rest, the relative motion is that of the
+    force -M r a_g, i.e. the same run as a point-load history -m a_g at every equation of the
+    direction (a closed-form equivalence of the two load paths); checked for both schemes."""
+    model = meshes.column(6)
+    om, op = compiled(model)
+    op0 = fxo.elastic_params([dict(model["params"][g], b=(0.0, 0.0, 0.0)) for g in om.param_group])
+    n = om.neq
+    m, _ = om.lumped_mass(op0)
+    dt, nsteps = 2e-4, 30
+    ag = 0.3 * 9.81 * np.sin(np.arange(nsteps) * 0.3)
+    x_eqs = np.array(sorted({q for i, q in enumerate(om.node_eq) if q >= 0 and i % 3 == 0}), np.int32)
+    s1 = [np.zeros(n) for _ in range(4)]
+    om.explicit_run(op0, m, None, dt, nsteps, *s1, scheme=scheme, ground_dir=0, ground_accel=ag)
+    # the same loads as point loads: at most 32 equations per run in the device form, so compare
+    # with the oracle's own point-load path over all of them
+    hist = np.array([[-m[q] * ag[k] for q in x_eqs] for k in range(nsteps)])
+    s2 = [np.zeros(n) for _ in range(4)]
+    om.explicit_run(op0, m, None, dt, nsteps, *s2, scheme=scheme, load_eq=x_eqs, load_hist=hist)
+    for a1, a2 in zip(s1, s2):
+        assert np.max(np.abs(a1 - a2)) <= 1e-12 * max(np.max(np.abs(a2)), 1e-300)
+    assert np.max(np.abs(s1[0])) > 0
