@@ -180,6 +180,32 @@ def bench_refcol(core, S, ctx, side, reps):
     return res
 
 
+def bench_ssp(core, ctx, mesh, params, model, x, dx, dy, args, B64, check=True):
+    """The SSPbrick<ElasticIsotropic> operator (one mean-strain point plus the stabilisation term) on
+    the same lattice: it moves the same bytes as stdBrick's apply, so the same roofline bytes apply;
+    with the oracle's SSPbrick apply on the whole model when the CPU leg runs."""
+    sp = [dict(p, element="SSPbrick") for p in params]
+    t0 = time.perf_counter()
+    sb = core.Batch(ctx, mesh, sp, element="SSPbrick")
+    setup = time.perf_counter() - t0
+    sb.time_apply(dx, dy, max(1, args.warmup), 64)
+    ms = sb.time_apply(dx, dy, args.steps, 64) / args.steps
+    gbs = B64 / (ms * 1e-3) / 1e9
+    res = {"kernel_ms_per_apply": round(ms, 5), "gdof_s": round(mesh.neq / (ms * 1e-3) / 1e9, 3),
+           "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4), "setup_s": round(setup, 2),
+           "kernel": "hex8_box_kernel<double, 3, 0, false, 8> + hex8_shared_sum_kernel<double>",
+           "lattice_path": sb.info()["lattice"]}
+    if check:
+        from oracle import fxo
+
+        sb.apply(dx, dy)
+        om = fxo.OracleMesh(model, coloring=False)
+        want = om.apply(fxo.elastic_params(sp), x, nthreads=cpu_threads())
+        res["gpu_y_max_rel_error"] = float(np.max(np.abs(dy.get() - want)) / np.max(np.abs(want)))
+    sb.close()
+    return res
+
+
 def bench_c1(core, S, ctx, reps=30, cpu=True):
     """C1: 16^3 box, E 30 MPa, 10 kPa top traction, CG + Jacobi to rtol 1e-8 from x0 = 0; each
     solve is one blocking fx_cg call (the latency a caller sees)."""
@@ -441,22 +467,24 @@ def c4_cpu_baseline(core, S, cols, rec, steps=150):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=50)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--solve", action="store_true", help="also time full plain-FP64 and mixed-precision solves")
+    ap.add_argument("--no-solve", dest="solve", action="store_false",
+                    help="skip the full plain-FP64 and mixed-precision C2 solves")
     ap.add_argument("--refcol", type=int, default=40, metavar="SIDE",
                     help="time the reference's own operator benchmark model, a SIDE^3 soil column (0: skip)")
     ap.add_argument("--c1", type=int, default=1, help="time C1 (16^3 static box) solves: solves/s, latency (0: skip)")
-    ap.add_argument("--c4", type=int, default=40, metavar="STEPS",
+    ap.add_argument("--c4", type=int, default=4000, metavar="STEPS",
                     help="time STEPS Newmark steps of C4 (4096 PIMY soil columns) and the plain per-model path "
                          "(0: skip)")
     ap.add_argument("--c4-columns", type=int, default=4096)
     ap.add_argument("--c5", type=float, default=60.0, metavar="SECONDS",
                     help="serve C5 (Poisson 200/s mixed columns and blocks) for SECONDS of arrivals (0: skip)")
-    ap.add_argument("--c3", type=int, default=0, metavar="STEPS",
-                    help="also time STEPS explicit CentralDifference steps of C3 (200x200x100, h 5 m)")
+    ap.add_argument("--ssp", type=int, default=1, help="time the SSPbrick operator on the same lattice (0: skip)")
+    ap.add_argument("--c3", type=int, default=2000, metavar="STEPS",
+                    help="time STEPS explicit CentralDifference steps of C3 (200x200x100, h 5 m; 0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the path would shard by "
@@ -530,11 +558,15 @@ def main():
                           "achieved_GBs": round(gbs32, 1), "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                           "algorithmic_bytes_per_apply": B32},
     }
+    if args.ssp:
+        out["ssp_operator"] = bench_ssp(core, ctx, mesh, params, model, x, dx, dy, args, B64,
+                                        check=not args.no_cpu)
     if args.solve:
         b = mesh.to_equations(model["nodal_load"])
         db = core.DeviceArray.from_host(ctx, b)
         xs = core.DeviceArray(ctx, mesh.neq)
-        res = {}
+        res, sols = {}, {}
+        ctx.set_graphs(True)  # the CG iterations replayed from captured hipGraphs (the same kernels and bits)
         for name, fn in (("plain_fp64", lambda: batch.cg(db, xs, rtol=1e-8, max_it=100000)),
                          ("mixed_fp32_ir", lambda: batch.cg_mixed(db, xs, rtol=1e-8, inner_rtol=1e-6))):
             xs.zero()
@@ -544,13 +576,24 @@ def main():
             ctx.sync()
             r["seconds"] = round(time.perf_counter() - t0, 3)
             res[name] = r
+            sols[name] = xs.get()
             print(f"[bench] {name}: {r}", file=sys.stderr, flush=True)
-        if not args.no_cpu:  # the oracle's CG for a few iterations: time per iteration, whole solve estimated
-            from oracle import fxo
+        from oracle import fxo
 
-            threads = cpu_threads()
-            om = fxo.OracleMesh(model, coloring=False)
-            op = fxo.elastic_params(params)
+        threads = cpu_threads()
+        om = fxo.OracleMesh(model, coloring=False)
+        op = fxo.elastic_params(params)
+        # each solution's residual with the FP64 oracle's operator, and the two solutions' difference
+        for name in sols:
+            r = b - om.apply(op, sols[name], nthreads=threads)
+            res[name]["oracle_relative_residual"] = float(np.linalg.norm(r) / np.linalg.norm(b))
+        res["mixed_vs_plain_max_rel_difference"] = float(np.max(np.abs(sols["mixed_fp32_ir"] - sols["plain_fp64"])) /
+                                                         np.max(np.abs(sols["plain_fp64"])))
+        res["mixed_speedup"] = round(res["plain_fp64"]["seconds"] / res["mixed_fp32_ir"]["seconds"], 3)
+        ctx.set_graphs(False)
+        for name in ("plain_fp64", "mixed_fp32_ir"):
+            res[name]["ms_per_iteration"] = round(res[name]["seconds"] * 1e3 / max(1, res[name]["iterations"]), 4)
+        if not args.no_cpu:  # the oracle's CG for a few iterations: time per iteration, whole solve estimated
             its = 5
             t0 = time.perf_counter()
             om.cg(op, b, rtol=1e-8, max_it=its, lanes=1, nthreads=threads)

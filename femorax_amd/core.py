@@ -25,6 +25,7 @@ FX_BATCH_DYNAMIC = 32
 FX_BATCH_GMRES = 64
 FX_BATCH_NO_TUNE = 16
 FX_BATCH_SLOTS = 2048
+FX_BATCH_SSPBRICK = 4096
 FX_OUT_PLAIN, FX_OUT_ATOMIC, FX_OUT_SLOTS = 0, 1, 2
 FX_TANGENT_SPD, FX_TANGENT_SPSD = 0, 1
 
@@ -476,7 +477,7 @@ class Batch:
     """An element batch on the device (fx_batch_create): the operator, its diagonal, CG."""
 
     def __init__(self, ctx, mesh, params, fp32=False, general=False, xcd=False, tune=True, dynamic=False,
-                 gmres=False, tile8=None, tile_depth=None, slots=False):
+                 gmres=False, tile8=None, tile_depth=None, slots=False, element="stdBrick"):
         self.ctx = ctx
         self.neq = mesh.neq
         keep = []
@@ -497,6 +498,10 @@ class Batch:
             (FX_BATCH_XCD_SLABS if xcd else 0) | (0 if tune else FX_BATCH_NO_TUNE) | (FX_BATCH_DYNAMIC if dynamic else 0) | \
             (FX_BATCH_GMRES if gmres else 0) | (0 if tile8 is None else FX_BATCH_TILE8 if tile8 else FX_BATCH_TILE4) | \
             (FX_BATCH_SLOTS if slots else 0)
+        if element not in ("stdBrick", "SSPbrick"):
+            raise ValueError(f"element {element!r}: stdBrick or SSPbrick")
+        if element == "SSPbrick":
+            flags |= FX_BATCH_SSPBRICK
         if tile_depth is not None:  # cells per lattice tile in z: 4, 8 or 16
             flags |= {4: FX_BATCH_TILE4, 8: FX_BATCH_TILE8, 16: FX_BATCH_TILE16}[tile_depth]
         _check(lib().fx_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)

@@ -52,7 +52,9 @@ struct BoxArgs {
   T* partial;  // [slots][3]: the tiles' sums at their shared nodes
   int ntiles;
   int accumulate;
+  int use_interior;  // regular tiles' interior nodes addressed from their base (internal numbering only)
   const int32_t* done;
+  const double* ssp;  // SSPbrick: per parameter set, the modal stabilisation and diag(K_e) (kSspBox each)
 };
 
 // The element operator in the modal basis of the trilinear field.
@@ -97,18 +99,12 @@ __device__ __forceinline__ void hadamard8_transposed(T (&g)[8]) {
 // y[c][q] = (K_e v)[c][q] (+ the mass term), v[c][q] from the tile's node array in LDS (xs[c][ln],
 // ln[q] the tile node of corner q). lam, mu carry dvol and the operator factor; mc = cM rho dvol
 // (MASS only; lumped: the reference's lumped mass, else consistent).
-template <typename T, bool MASS, int NODES>
-__device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam, T mu, T mc, bool lumped,
-                                          const BoxConst<T>& K, T (&y)[3][8]) {
-  T b[3][8];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * NODES + ln[q]];
-    hadamard8(b[c]);
-  }
+// The constant-strain part (the three linear modes of each component): F[d][c] = kappa_d b[c][e_d],
+// g[c][e_d] = 8 kappa_d sigma_dc.
+template <typename T>
+__device__ __forceinline__ void box_constant_strain(const T (&b)[3][8], T lam, T mu, const BoxConst<T>& K,
+                                                    T (&y)[3][8]) {
   const T m2 = mu + mu;
-  // constant strain: F[d][c] = kappa_d b[c][e_d]; g[c][e_d] = 8 kappa_d sigma_dc
   {
     T F[3][3];
 #pragma unroll
@@ -127,6 +123,20 @@ __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam
         y[d][1 << c] = K.w0[c] * s;
       }
   }
+}
+
+template <typename T, bool MASS, int NODES>
+__device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam, T mu, T mc, bool lumped,
+                                          const BoxConst<T>& K, T (&y)[3][8]) {
+  T b[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * NODES + ln[q]];
+    hadamard8(b[c]);
+  }
+  const T m2 = mu + mu;
+  box_constant_strain(b, lam, mu, K, y);
   // the bilinear modes that vary along p: rows d1, d2 (the other axes) of the gradient's p-linear
   // term, F[d][c] = kappa_d b[c][e_d | e_p]; g[c][e_d | e_p] gets 8/3 kappa_d sigma_dc. Mode
   // (d, p) hears from p and from d; the smaller of the two assigns, the other adds.
@@ -175,6 +185,59 @@ __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam
 #pragma unroll
   for (int c = 0; c < 3; ++c) hadamard8_transposed(y[c]);
   if (MASS && lumped) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+      for (int q = 0; q < 8; ++q) y[c][q] += mc * xs[c * NODES + ln[q]];
+  }
+}
+
+// SSPbrick on an axis-aligned box (ssp_brick.hpp; element_operator_apply with its element term,
+// element_operator.hpp:82-130): the one-point mean strain is the gradient of the linear modes, so
+// the mean-strain stiffness is box_constant_strain exactly; the stabilisation Kstab = Mben^T
+// interior Mben acts on the hourglass modes only, whose vectors on a box are the modes xi eta,
+// eta zeta, xi zeta, xi eta zeta (3, 6, 5, 7 here): Mben v = (H v)_m / 8, so the twelve modal
+// values go through one symmetric 12 x 12 product with S = interior / 64 (per parameter set,
+// packed upper triangle), scaled by the operator factor fE. The lumped mass mc x at each node.
+template <typename T, bool MASS, int NODES>
+__device__ __forceinline__ void box_apply_ssp(const T* xs, const int (&ln)[8], T lam, T mu, T mc, T fE,
+                                              const double* __restrict__ S, const BoxConst<T>& K, T (&y)[3][8]) {
+  T b[3][8];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) b[c][q] = xs[c * NODES + ln[q]];
+    hadamard8(b[c]);
+  }
+  box_constant_strain(b, lam, mu, K, y);
+  constexpr int kHourglass[4] = {3, 6, 5, 7};
+  T w[12], z[12];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      w[3 * g + c] = b[c][kHourglass[g]];
+      z[3 * g + c] = T(0);
+    }
+  int idx = 0;
+#pragma unroll
+  for (int r = 0; r < 12; ++r)
+#pragma unroll
+    for (int q = r; q < 12; ++q) {
+      const T sv = static_cast<T>(S[idx++]);
+      z[r] += sv * w[q];
+      if (q != r) z[q] += sv * w[r];
+    }
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) y[c][kHourglass[g]] = fE * z[3 * g + c];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    y[c][0] = T(0);
+    hadamard8_transposed(y[c]);
+  }
+  if (MASS) {
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
@@ -244,7 +307,7 @@ __device__ __forceinline__ void fetch_tile(const BoxArgs<T>& a, int tile, int t,
   using G = TileGeom<TZ>;
   const uint32_t* words = a.words + static_cast<size_t>(tile) * G::kNodes;
   const int32_t* slots = a.slots + static_cast<size_t>(tile) * G::kNodes;
-  const int ib = __builtin_amdgcn_readfirstlane(a.tiles[tile].interior_base);
+  const int ib = a.use_interior ? __builtin_amdgcn_readfirstlane(a.tiles[tile].interior_base) : -1;
   uint32_t w[G::kRounds];
   int32_t s[G::kRounds];
 #pragma unroll
@@ -281,7 +344,8 @@ struct NoHook {
 };
 
 // One pass of a tile: cell (cx, cy, cz = wave + 4 pass). MODE: 0 apply without mass terms, 1
-// apply with mass terms, 2 diagonal. The cell's results are summed per node and committed.
+// apply with mass terms, 2 diagonal; 3, 4, 5 the same for SSPbrick cells. The cell's results are
+// summed per node and committed.
 //
 // Every lane runs the full 8 x 8 plane: a cell past the end of a cut tile has no parameter set
 // (zero results) and a node past it has no word (nothing fetched, nothing committed), so the
@@ -292,7 +356,8 @@ template <typename T, int MODE, int TZ, int RING = 0, typename Mid = NoHook>
 __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T* xs, const uint32_t* wsh, T* top,
                                          int t, int pass, const Mid& mid = Mid{}) {
   using G = TileGeom<TZ>;
-  constexpr bool DIAG = MODE == 2;
+  constexpr bool DIAG = MODE % 3 == 2;
+  constexpr bool SSP = MODE >= 3;
   constexpr int NX = kTileX + 1;
   constexpr int LDS_NODES = RING ? RING * kPlane : G::kNodes;  // component stride of xs
   auto lp = [](int z) { return RING ? z % RING : z; };
@@ -308,7 +373,15 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
     double fD, cM;
     operator_factors(p, a.k, fD, cM);
     const double dvol = a.hx * a.hy * a.hz * 0.125;
-    if constexpr (DIAG) {
+    if constexpr (DIAG && SSP) {
+      // diag(K_e) per corner, kept per parameter set, and the lumped mass
+      const double* dk = a.ssp + static_cast<size_t>(pidx) * kSspBox + 78;
+      const double mass = cM * p.rho * dvol;
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) out[c][q] = static_cast<T>(fD * dk[8 * c + q] + mass);
+    } else if constexpr (DIAG) {
       // diag(K_e): Gauss sums of (dN/dx_d)^2 are 8 / (9 h_d^2) at every node of a box; of N^2, 8/27
       const double S[3] = {8.0 / (9.0 * a.hx * a.hx), 8.0 / (9.0 * a.hy * a.hy), 8.0 / (9.0 * a.hz * a.hz)};
       const double lam = fD * p.lam, mu = fD * p.mu;
@@ -323,8 +396,13 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
       int ln[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * (cy + ((q >> 1) & 1)) + kPlane * lp(cz + (q >> 2));
-      box_apply<T, MODE == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
-                                         static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
+      if constexpr (SSP)
+        box_apply_ssp<T, MODE == 4, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
+                                               static_cast<T>(cM * p.rho * dvol), static_cast<T>(fD),
+                                               a.ssp + static_cast<size_t>(pidx) * kSspBox, a.K, out);
+      else
+        box_apply<T, MODE == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
+                                           static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
     }
   }
   // in-plane sums, x first then y: the cell's corners a (cx, cy), b (cx+1, cy), c (cx, cy+1),
@@ -409,7 +487,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
 template <typename T, int MODE, int WAVES, bool PERSIST, int TZ>
 __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_kernel(BoxArgs<T> a) {
   using G = TileGeom<TZ>;
-  constexpr bool DIAG = MODE == 2;
+  constexpr bool DIAG = MODE % 3 == 2;
   __shared__ T xs[3 * G::kNodes];               // x at the tile's nodes
   __shared__ uint32_t wsh[G::kNodes];           // node words (slot for shared nodes)
   __shared__ T top[3 * G::kRing * kPlane];      // top-face sums of the cell planes
@@ -643,7 +721,11 @@ int resident_tiles(fx_ctx* ctx, const void* kernel) {
 }  // namespace
 
 template <typename T, bool DIAG>
-int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done) {
+int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done, int ref) {
+  if (ref && !b->words_ref) {
+    b->ctx->error = "operator: no reference-numbering words for this batch";
+    return FX_EINVAL;
+  }
   // the mass-free kernel unless some parameter set has a mass factor cM' = cM + cC alphaM
   bool mass = false;
   for (const DevParam& p : b->host_params) {
@@ -681,12 +763,18 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   static const Kernel stream8[3][4] = FX_STREAM_TABLE(8);
   static const Kernel stream16[3][4] = FX_STREAM_TABLE(16);
 #undef FX_STREAM_TABLE
-  const int v = b->box_variant & 15;
+  // SSPbrick lattices: one kernel per mode (4- or 8-deep tiles, the compiler's register budget)
+  static const Kernel ssp4[3] = {hex8_box_kernel<T, 3, 0, false, 4>, hex8_box_kernel<T, 4, 0, false, 4>,
+                                 hex8_box_kernel<T, 5, 0, false, 4>};
+  static const Kernel ssp8[3] = {hex8_box_kernel<T, 3, 0, false, 8>, hex8_box_kernel<T, 4, 0, false, 8>,
+                                 hex8_box_kernel<T, 5, 0, false, 8>};
+  const int v = b->ssp ? 0 : b->box_variant & 15;
   const int m = DIAG ? 2 : mass ? 1 : 0;
-  const bool stream = b->tile_z == 16 || ((v & 8) && b->tile_z == 8);
-  const Kernel kernel = b->tile_z == 16 ? stream16[m][v & 3]
-                        : stream        ? stream8[m][v & 3]
-                                        : (b->tile_z == 8 ? table8 : table4)[m][v & 7];
+  const bool stream = !b->ssp && (b->tile_z == 16 || ((v & 8) && b->tile_z == 8));
+  const Kernel kernel = b->ssp            ? (b->tile_z == 8 ? ssp8 : ssp4)[m]
+                        : b->tile_z == 16 ? stream16[m][v & 3]
+                        : stream          ? stream8[m][v & 3]
+                                          : (b->tile_z == 8 ? table8 : table4)[m][v & 7];
   int grid = b->ntiles;
   if ((v & 4) && !stream) {
     const int resident = resident_tiles(b->ctx, reinterpret_cast<const void*>(kernel));
@@ -694,8 +782,9 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   }
   BoxArgs<T> a{};
   a.tiles = b->tiles;
-  a.words = b->words;
+  a.words = ref ? b->words_ref : b->words;
   a.slots = b->tile_slots;
+  a.use_interior = ref ? 0 : 1;
   a.cell_param = b->cell_param;
   a.params = b->params;
   a.k = k;
@@ -720,6 +809,7 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   a.ntiles = b->ntiles;
   a.accumulate = accumulate;
   a.done = done;
+  a.ssp = b->ssp_box;
   if (!a.partial) {
     b->ctx->error = "operator: no partial-sum buffer for this precision";
     return FX_EINVAL;
@@ -727,16 +817,17 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   hipLaunchKernelGGL(kernel, dim3(grid), dim3(kTileThreads), 0, b->ctx->stream, a);
   if (b->nshared > 0)
     hipLaunchKernelGGL(hex8_shared_sum_kernel<T>, dim3((b->nshared + 255) / 256), dim3(256), 0, b->ctx->stream,
-                       b->shared_word, b->shared_start, b->shared_meta, a.partial, y, b->nshared, accumulate, done);
+                       ref ? b->shared_word_ref : b->shared_word, b->shared_start, b->shared_meta, a.partial, y,
+                       b->nshared, accumulate, done);
   if (b->ntied > 0)
-    hipLaunchKernelGGL(hex8_tied_sum_kernel<T>, dim3((b->ntied + 3) / 4), dim3(256), 0, b->ctx->stream, b->tied_word,
-                       b->tied_start, a.partial, y, b->ntied, accumulate, done);
+    hipLaunchKernelGGL(hex8_tied_sum_kernel<T>, dim3((b->ntied + 3) / 4), dim3(256), 0, b->ctx->stream,
+                       ref ? b->tied_word_ref : b->tied_word, b->tied_start, a.partial, y, b->ntied, accumulate, done);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }
 
-template int launch_box<double, false>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*);
-template int launch_box<double, true>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*);
-template int launch_box<float, false>(fx_batch*, const fx_coeffs&, const float*, float*, int, const int32_t*);
+template int launch_box<double, false>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*, int);
+template int launch_box<double, true>(fx_batch*, const fx_coeffs&, const double*, double*, int, const int32_t*, int);
+template int launch_box<float, false>(fx_batch*, const fx_coeffs&, const float*, float*, int, const int32_t*, int);
 
 }  // namespace fx

@@ -7,6 +7,8 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <thread>
+#include <vector>
 
 #include "fx_internal.hpp"
 #include "fx_kernels.hpp"
@@ -245,7 +247,8 @@ void fx_batch_destroy(fx_batch* b) {
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
-                  b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev};
+                  b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev,
+                  b->words_ref, b->shared_word_ref, b->tied_word_ref, b->orphans_ref, b->ssp_elem, b->ssp_box};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -280,8 +283,8 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   // without a depth flag tries two depths (8 and 16 cells from 2^20 cells up, 4 and 8 below), each
   // with its variant autotune, and keeps the faster; the choice is kept in the autotune cache per
   // device and model, so a known model is built once, with the kept depth and variant.
-  if (!(flags & (kDepthFlags | FX_BATCH_NO_TUNE | FX_BATCH_GENERAL)) && d->count >= kTuneMinElements && d->params &&
-      d->nparam > 0) {
+  if (!(flags & (kDepthFlags | FX_BATCH_NO_TUNE | FX_BATCH_GENERAL | FX_BATCH_SSPBRICK)) &&
+      d->count >= kTuneMinElements && d->params && d->nparam > 0) {
     const std::string key = fx::tune_machine_key(ctx->device) + "|" + lattice_model_key(d, 0);
     int depth = 0, variant = -1;
     if (fx::tune_cache_get(key, depth, variant)) {
@@ -356,6 +359,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   b->neq = d->neq;
   b->nparam = d->nparam;
   b->has_f32 = (flags & FX_BATCH_FP32) != 0;
+  b->ssp = (flags & FX_BATCH_SSPBRICK) != 0;
   for (int32_t i = 0; i < d->nparam; ++i) {
     const auto& p = d->params[i];
     fx::DevParam q{};
@@ -365,7 +369,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     q.mu = 0.5 * mu2;
     q.rho = p.rho;
     for (int k = 0; k < 3; ++k) q.b[k] = p.b[k];
-    q.mass_type = p.mass_type;
+    q.mass_type = b->ssp ? 1 : p.mass_type;  // SSPbrick's mass is always lumped
     q.alphaM = p.damp.alphaM;
     q.betaK = p.damp.betaK;
     q.betaKinit = p.damp.betaKinit;
@@ -383,15 +387,20 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   for (int32_t q = 0; q < d->neq; ++q)
     if (!covered[q]) orphans.push_back(q);
   fx::Lattice lat;
-  const bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
-                           fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
-                                             (flags & FX_BATCH_XCD_SLABS) != 0,
-                                             (flags & FX_BATCH_TILE4)    ? 4
-                                             : (flags & FX_BATCH_TILE8)  ? 8
-                                             : (flags & FX_BATCH_TILE16) ? 16
-                                             : E >= (1 << 20)           ? 8
-                                                                        : 4,
-                                             lat);
+  const int depth = (flags & FX_BATCH_TILE4)    ? 4
+                    : (flags & FX_BATCH_TILE8)  ? 8
+                    : (flags & FX_BATCH_TILE16) ? (b->ssp ? 8 : 16)
+                    : E >= (1 << 20)           ? 8
+                                               : 4;
+  bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
+                     fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
+                                       (flags & FX_BATCH_XCD_SLABS) != 0, depth, lat);
+  // an SSPbrick lattice with DOFs one element ties together: the general path (the diagonal's
+  // cross terms come from each element's constants there)
+  if (use_lattice && b->ssp && !lat.tied_word.empty()) {
+    use_lattice = false;
+    lat = fx::Lattice{};
+  }
   if (use_lattice) {  // orphans in the internal numbering
     std::vector<int32_t> internal_orphans;
     for (int32_t i = 0; i < d->neq; ++i)
@@ -438,6 +447,34 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         ok(dev_zeros(ctx, b->partial64, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by)) &&
         ok(b->has_f32 ? dev_zeros(ctx, b->partial32, static_cast<size_t>(std::max<int64_t>(lat.nslot, 1)) * 3, by) : FX_OK) &&
         ok(dev_upload(ctx, b->perm, lat.perm.data(), lat.perm.size(), by));
+    // the node words in the reference numbering, where each node's free DOFs have consecutive
+    // reference equations (so a word's first equation and mask still name them)
+    bool consecutive = true;
+    auto to_ref = [&](uint32_t w) {
+      const uint32_t mask = (w >> 28) & 7u;
+      if (mask == 0u) return w;
+      const int32_t i0 = static_cast<int32_t>(w & 0x0fffffffu);
+      const int cnt = ((mask >> 0) & 1u) + ((mask >> 1) & 1u) + ((mask >> 2) & 1u);
+      if (i0 + cnt > d->neq) {
+        consecutive = false;
+        return w;
+      }
+      const int32_t q0 = lat.perm[i0];
+      for (int j = 1; j < cnt; ++j) consecutive &= lat.perm[i0 + j] == q0 + j;
+      return (w & 0xf0000000u) | static_cast<uint32_t>(q0);
+    };
+    std::vector<uint32_t> wr(lat.words.size()), sr(lat.shared_word.size()), tr(lat.tied_word.size());
+    for (size_t i = 0; i < wr.size(); ++i) wr[i] = to_ref(lat.words[i]);
+    for (size_t i = 0; i < sr.size(); ++i) sr[i] = to_ref(lat.shared_word[i]);
+    for (size_t i = 0; i < tr.size(); ++i) tr[i] = to_ref(lat.tied_word[i]);
+    if (consecutive && rc == FX_OK) {
+      std::vector<int32_t> orr(orphans.size());
+      for (size_t i = 0; i < orr.size(); ++i) orr[i] = lat.perm[orphans[i]];
+      ok(dev_upload(ctx, b->words_ref, wr.data(), wr.size(), by)) &&
+          ok(sr.empty() ? FX_OK : dev_upload(ctx, b->shared_word_ref, sr.data(), sr.size(), by)) &&
+          ok(tr.empty() ? FX_OK : dev_upload(ctx, b->tied_word_ref, tr.data(), tr.size(), by)) &&
+          ok(orr.empty() ? FX_OK : dev_upload(ctx, b->orphans_ref, orr.data(), orr.size(), by));
+    }
   } else {
     std::vector<float> c32(d->coords, d->coords + static_cast<size_t>(3) * N);
     ok(dev_upload(ctx, b->node, d->node, static_cast<size_t>(8) * P, by)) &&
@@ -457,6 +494,38 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
           ok(b->has_f32 ? dev_zeros(ctx, b->ynode32, static_cast<size_t>(3) * N, by) : FX_OK) &&
           ok(dev_upload(ctx, b->con_start, con.start.data(), con.start.size(), by)) &&
           ok(dev_upload(ctx, b->con_list, con.list.data(), con.list.size(), by));
+    }
+  }
+  // SSPbrick constants: per parameter set on a lattice (the modal stabilisation of its box and
+  // diag(K_e)); per element on the general path and for the per-element forms of fx_run_operator
+  // (SSPbrick::point_store_geometry / element_constants, ssp_brick.hpp:890-914, kept as the
+  // reference's geometry cache and constants arrays, element_kernels.hpp:47)
+  if (rc == FX_OK && b->ssp) {
+    if (use_lattice) {
+      std::vector<double> box(static_cast<size_t>(d->nparam) * fx::kSspBox);
+      for (int32_t i = 0; i < d->nparam; ++i)
+        fx::ssp_box_constants(lat.hx, lat.hy, lat.hz, b->host_params[i].lam, b->host_params[i].mu,
+                              box.data() + static_cast<size_t>(i) * fx::kSspBox);
+      ok(dev_upload(ctx, b->ssp_box, box.data(), box.size(), by));
+    }
+    if (!use_lattice || (flags & FX_BATCH_SLOTS)) {
+      std::vector<double> cst(static_cast<size_t>(fx::kSspConsts) * std::max(1, P), 0.0);
+      const int nth = static_cast<int>(std::min<int64_t>(16, std::max<int64_t>(1, E / 4096)));
+      std::vector<std::thread> pool;
+      for (int t = 0; t < nth; ++t)
+        pool.emplace_back([&, t] {
+          for (int32_t e = t; e < E; e += nth) {
+            double xe[8][3];
+            for (int a = 0; a < 8; ++a) {
+              const int32_t nd = d->node[static_cast<size_t>(a) * P + e];
+              for (int k = 0; k < 3; ++k) xe[a][k] = d->coords[static_cast<size_t>(k) * N + nd];
+            }
+            const fx::DevParam& q = b->host_params[d->param[e]];
+            fx::ssp_constants(xe, q.lam, q.mu, cst.data() + e, P);
+          }
+        });
+      for (auto& th : pool) th.join();
+      ok(dev_upload(ctx, b->ssp_elem, cst.data(), cst.size(), by));
     }
   }
   // lumped mass, lumped alphaM mass and body load (batches.cpp:204-224; std_brick.hpp:519-542,
@@ -601,7 +670,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   b->info.ncolors = blk.ncolors;
   b->info.tied = b->tied ? 1 : 0;
   b->info.lattice = b->lattice ? 1 : 0;
-  if (b->lattice && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {
+  if (b->lattice && !b->ssp && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {
     // the variant autotune, kept per device and model (with this tile depth) in the cache
     const std::string key = fx::tune_machine_key(ctx->device) + "|" + lattice_model_key(d, b->tile_z);
     int depth = 0, variant = -1;
@@ -631,7 +700,7 @@ int fx_batch_tune(fx_batch* b, int variant) {
     b->info.variant = variant;
     return FX_OK;
   }
-  if (!b->lattice) return FX_OK;  // one variant only
+  if (!b->lattice || b->ssp) return FX_OK;  // one variant only
   // time each variant on scratch vectors (the reference's force-settings autotune,
   // src/exec/force_settings.cpp:135-155): three rounds over the variants, each sample a warm-up
   // apply then ten back-to-back applies between two events (one apply is a few tens of
@@ -943,7 +1012,9 @@ int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
     const double hx = b->hx, hy = b->hy, hz = b->hz;
     const double box[8][3] = {{0, 0, 0}, {hx, 0, 0}, {hx, hy, 0}, {0, hy, 0}, {0, 0, hz}, {hx, 0, hz}, {hx, hy, hz}, {0, hy, hz}};
     for (const fx::DevParam& p : b->host_params)
-      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(box, p.lam, p.mu, p.rho)));
+      best = std::min(best, fx::stable_dt(scheme, p,
+                                          b->ssp ? fx::ssp_max_omega_sq(box, p.lam, p.mu, p.rho)
+                                                 : fx::brick_max_omega_sq(box, p.lam, p.mu, p.rho)));
   } else {
     // PIMY points: the stability tangent of the committed state (stability_moduli,
     // pressure_independ_multi_yield.hpp:174-203): the point's moduli, scaled as paramScaling will
@@ -972,7 +1043,9 @@ int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
       }
       const fx::DevParam& p = b->host_params[b->host_param[e]];
       if (!b->pimy) {
-        best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(xe, p.lam, p.mu, p.rho)));
+        best = std::min(best, fx::stable_dt(scheme, p,
+                                            b->ssp ? fx::ssp_max_omega_sq(xe, p.lam, p.mu, p.rho)
+                                                   : fx::brick_max_omega_sq(xe, p.lam, p.mu, p.rho)));
         continue;
       }
       const fx::PimyParams& q = b->host_pimy[b->host_param[e]];
@@ -1083,6 +1156,8 @@ int fx_pimy_points(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, cons
 int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flags, fx_batch** out) {
   if (!ctx || !d || !out || !d->params || d->nparam <= 0) return fail(ctx, FX_EINVAL, "fx_pimy_batch_create: null argument");
   *out = nullptr;
+  if (flags & FX_BATCH_SSPBRICK)
+    return fail(ctx, FX_EUNSUP, "fx_pimy_batch_create: SSPbrick<PressureIndependMultiYield> is not built");
   std::vector<fx::PimyParams> pp(d->nparam);
   std::vector<fx_elastic_params> ep(d->nparam);
   for (int32_t i = 0; i < d->nparam; ++i) {

@@ -192,6 +192,19 @@ void brick_mass_unit(const double (&xe)[8][3], double (&M)[8][8]);
 void brick_stiffness_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8],
                             double (&K)[24][24]);
 double brick_max_omega_sq_points(const double (&xe)[8][3], const double (&lam)[8], const double (&mu)[8], double rho);
+// Host SSPbrick helpers (fx_ssp_host.cpp). An element's constants, kSspConsts values at c[k * stride]:
+// dNmod (node-major, 24), mVol, the node volume weights (8), the hourglass vectors gst gut gus gstu
+// (32), interior's upper triangle (78).
+constexpr int kSspConsts = 143;
+constexpr int kSspGamma = 33;
+constexpr int kSspInterior = 65;
+// A lattice cell's: the modal stabilisation (interior / 64, upper triangle, 78) and diag(K_e) of unit
+// operator factor by lattice corner (component-major, 24).
+constexpr int kSspBox = 102;
+void ssp_constants(const double (&xe)[8][3], double lam, double mu, double* c, int64_t stride);
+void ssp_stiffness(const double* c, double lam, double mu, double (&K)[24][24]);
+double ssp_max_omega_sq(const double (&xe)[8][3], double lam, double mu, double rho);
+void ssp_box_constants(double hx, double hy, double hz, double lam, double mu, double* out);
 double stable_dt(int scheme, const DevParam& p, double omega_sq);
 }  // namespace fx
 
@@ -206,6 +219,11 @@ struct fx_batch {
   double* coords = nullptr;     // [3 * nnode]
   float* coords32 = nullptr;    // [3 * nnode]
   fx::DevParam* params = nullptr;
+  // SSPbrick batches (FX_BATCH_SSPBRICK): the element constants ([kSspConsts][padded], the general
+  // path and the per-element forms) or the lattice's per-set modal constants ([nparam][kSspBox])
+  bool ssp = false;
+  double* ssp_elem = nullptr;
+  double* ssp_box = nullptr;
   fx::DevBlocks blk;
   // tied meshes: node-space output and its contraction
   bool tied = false;
@@ -246,6 +264,12 @@ struct fx_batch {
   // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference
   bool internal = false;
   int32_t* perm = nullptr;
+  // the same node words in the reference numbering (where every node's free DOFs have consecutive
+  // reference equations): the drop-in apply reads x and writes y in place, with no permutation
+  uint32_t* words_ref = nullptr;
+  uint32_t* shared_word_ref = nullptr;
+  uint32_t* tied_word_ref = nullptr;
+  int32_t* orphans_ref = nullptr;
   // free equations no element touches: zeroed after each apply
   int32_t* orphans = nullptr;
   int32_t norphans = 0;

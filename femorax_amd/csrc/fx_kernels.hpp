@@ -25,6 +25,7 @@ struct KernelArgs {
   T* ynode;
   int accumulate;
   const int32_t* done;  // solver stop flag: the launch is a no-op once it is set
+  const double* ssp;    // SSPbrick: the elements' constants, value[k * padded + e], k < kSspConsts
 };
 
 // Effective factors of element_operator.hpp:45-56 for an elastic material (D_0 = D_c = D):
@@ -37,8 +38,9 @@ __host__ __device__ inline void operator_factors(const DevParam& p, const fx_coe
   cM = k.mass + k.damping * p.alphaM;
 }
 
+// ref = 1: x and y in the reference numbering (the batch's reference-numbering words)
 template <typename T, bool DIAG>
-int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done);
+int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate, const int32_t* done, int ref = 0);
 
 int operator_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y, const int32_t* done = nullptr);
 int operator_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y, const int32_t* done = nullptr);

@@ -5,7 +5,8 @@
 //   compute  y_e = sum over the 8 Gauss points of B^T (D_eff B x_e) dvol + cM' (M x_e)
 //            (element_operator_apply, src/physics/operators/element_operator.hpp:82-130, with
 //            stdBrick's shape functions std_brick.hpp:45-152 and the isotropic D of
-//            elastic_isotropic.hpp:40-94)
+//            elastic_isotropic.hpp:40-94), by sum factorisation on the trilinear modes
+//            (element_apply_modal)
 //   stage    the 24 values in LDS (the reference's slots, assembly.hpp:28-36, without the HBM
 //            round trip)
 //   reduce   per local node in a fixed (element, corner) order, then one store (first writer
@@ -29,66 +30,6 @@ struct ElemCoef {
   int consistent;
 };
 
-
-template <typename T>
-__device__ __forceinline__ void element_apply(const T (&x)[8], const T (&y)[8], const T (&z)[8], const T (&v)[24],
-                                              const ElemCoef<T>& c, T (&out)[24]) {
-#pragma unroll
-  for (int a = 0; a < 24; ++a) out[a] = T(0);
-#pragma unroll 1
-  for (int gp = 0; gp < 8; ++gp) {
-    Shape<T> s;
-    gauss_shape(x, y, z, gp, s);
-    T e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const T u0 = v[3 * j], u1 = v[3 * j + 1], u2 = v[3 * j + 2];
-      e0 += s.dx[j] * u0;
-      e1 += s.dy[j] * u1;
-      e2 += s.dz[j] * u2;
-      e3 += s.dy[j] * u0 + s.dx[j] * u1;
-      e4 += s.dz[j] * u1 + s.dy[j] * u2;
-      e5 += s.dz[j] * u0 + s.dx[j] * u2;
-    }
-    const T lt = c.lam * (e0 + e1 + e2);
-    const T m2 = T(2) * c.mu;
-    const T s0 = (m2 * e0 + lt) * s.dvol, s1 = (m2 * e1 + lt) * s.dvol, s2 = (m2 * e2 + lt) * s.dvol;
-    const T s3 = c.mu * e3 * s.dvol, s4 = c.mu * e4 * s.dvol, s5 = c.mu * e5 * s.dvol;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      out[3 * j] += s.dx[j] * s0 + s.dy[j] * s3 + s.dz[j] * s5;
-      out[3 * j + 1] += s.dy[j] * s1 + s.dx[j] * s3 + s.dz[j] * s4;
-      out[3 * j + 2] += s.dz[j] * s2 + s.dy[j] * s4 + s.dx[j] * s5;
-    }
-    if (c.rhoM != T(0)) {
-      const T w = c.rhoM * s.dvol;
-      if (c.consistent) {
-        T g0 = 0, g1 = 0, g2 = 0;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          g0 += s.N[k] * v[3 * k];
-          g1 += s.N[k] * v[3 * k + 1];
-          g2 += s.N[k] * v[3 * k + 2];
-        }
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const T wj = w * s.N[j];
-          out[3 * j] += wj * g0;
-          out[3 * j + 1] += wj * g1;
-          out[3 * j + 2] += wj * g2;
-        }
-      } else {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const T wj = w * s.N[j];
-          out[3 * j] += wj * v[3 * j];
-          out[3 * j + 1] += wj * v[3 * j + 1];
-          out[3 * j + 2] += wj * v[3 * j + 2];
-        }
-      }
-    }
-  }
-}
 
 // Diagonal of the element operator, with the cross terms of local DOFs that share an equation
 // added to the first of them (element_operator_diagonal, element_operator.hpp:138-209).
@@ -143,6 +84,162 @@ __device__ __forceinline__ void element_diagonal(const T (&x)[8], const T (&y)[8
   }
 }
 
+// SSPbrick (ssp_brick.hpp) from its constants c[k * P + e] (dNmod, mVol, node weights, hourglass
+// vectors, interior's upper triangle): the element operator with its element term
+// (element_operator_apply, element_operator.hpp:82-130): fD (mVol Bnot^T D Bnot + Kstab) v plus
+// the lumped mass, Kstab v = Mben^T (interior (Mben v)) without the 24 x 24 matrix
+// (element_term_reduce / _node, ssp_brick.hpp:964-989). For an elastic material every stiffness
+// and damping term has the same tangent, so the point's factor (cK + c0 + cc) and the element
+// term's (cK + cC (betaK + betaKinit + betaKcomm), element_term.hpp:34-37) are one fD.
+template <typename T>
+__device__ __forceinline__ T ssp_at(const double* __restrict__ c, size_t P, size_t e, int k) {
+  return static_cast<T>(c[static_cast<size_t>(k) * P + e]);
+}
+
+template <typename T>
+__device__ __forceinline__ void ssp_element_apply(const double* __restrict__ c, size_t P, size_t e, const T (&v)[24],
+                                                  const ElemCoef<T>& cf, T fE, T (&out)[24]) {
+  T g[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) g[k] = ssp_at<T>(c, P, e, k);
+  const T V = ssp_at<T>(c, P, e, 24);
+  T e0 = 0, e1 = 0, e2 = 0, e3 = 0, e4 = 0, e5 = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const T bx = g[3 * j], by = g[3 * j + 1], bz = g[3 * j + 2];
+    const T u0 = v[3 * j], u1 = v[3 * j + 1], u2 = v[3 * j + 2];
+    e0 += bx * u0;
+    e1 += by * u1;
+    e2 += bz * u2;
+    e3 += by * u0 + bx * u1;
+    e4 += bz * u1 + by * u2;
+    e5 += bz * u0 + bx * u2;
+  }
+  const T lt = cf.lam * (e0 + e1 + e2), m2 = T(2) * cf.mu;
+  const T s0 = (m2 * e0 + lt) * V, s1 = (m2 * e1 + lt) * V, s2 = (m2 * e2 + lt) * V;
+  const T s3 = cf.mu * e3 * V, s4 = cf.mu * e4 * V, s5 = cf.mu * e5 * V;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    const T bx = g[3 * j], by = g[3 * j + 1], bz = g[3 * j + 2];
+    out[3 * j] = bx * s0 + by * s3 + bz * s5;
+    out[3 * j + 1] = by * s1 + bx * s3 + bz * s4;
+    out[3 * j + 2] = bz * s2 + by * s4 + bx * s5;
+  }
+  if (cf.rhoM != T(0)) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const T m = cf.rhoM * ssp_at<T>(c, P, e, 25 + j);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) out[3 * j + d] += m * v[3 * j + d];
+    }
+  }
+  if (fE != T(0)) {
+    T gam[32], w[12], z[12];
+#pragma unroll
+    for (int k = 0; k < 32; ++k) gam[k] = ssp_at<T>(c, P, e, kSspGamma + k);
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T s = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s += gam[8 * m + i] * v[3 * i + d];
+        w[3 * m + d] = s;
+        z[3 * m + d] = T(0);
+      }
+    int idx = kSspInterior;
+#pragma unroll
+    for (int r = 0; r < 12; ++r)
+#pragma unroll
+      for (int q = r; q < 12; ++q) {
+        const T sv = ssp_at<T>(c, P, e, idx++);
+        z[r] += sv * w[q];
+        if (q != r) z[q] += sv * w[r];
+      }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        T s = 0;
+#pragma unroll
+        for (int m = 0; m < 4; ++m) s += gam[8 * m + j] * z[3 * m + d];
+        out[3 * j + d] += fE * s;
+      }
+  }
+}
+
+// interior(r, q) from the packed upper triangle
+template <typename T>
+__device__ __forceinline__ T ssp_interior(const double* __restrict__ c, size_t P, size_t e, int r, int q) {
+  if (r > q) {
+    const int t = r;
+    r = q;
+    q = t;
+  }
+  return ssp_at<T>(c, P, e, kSspInterior + 12 * r - r * (r - 1) / 2 + (q - r));
+}
+
+// Kstab(a, b) = sum_gh gamma_g(i) interior(3g + d, 3h + e) gamma_h(k) (element_term_entry,
+// ssp_brick.hpp:991-1005)
+template <typename T>
+__device__ __forceinline__ T ssp_term_entry(const double* __restrict__ c, size_t P, size_t e, int a, int b) {
+  const int i = a / 3, d = a % 3, k = b / 3, f = b % 3;
+  T s = 0;
+  for (int g = 0; g < 4; ++g) {
+    const T gi = ssp_at<T>(c, P, e, kSspGamma + 8 * g + i);
+    for (int h = 0; h < 4; ++h)
+      s += gi * ssp_interior<T>(c, P, e, 3 * g + d, 3 * h + f) * ssp_at<T>(c, P, e, kSspGamma + 8 * h + k);
+  }
+  return s;
+}
+
+// Diagonal of the SSPbrick operator, the cross terms of local DOFs that share an equation added to
+// the first of them (element_operator_diagonal, element_operator.hpp:138-209; the lumped mass has
+// no off-diagonal term).
+template <typename T>
+__device__ __forceinline__ void ssp_element_diagonal(const double* __restrict__ c, size_t P, size_t e,
+                                                     const int (&eq)[24], const ElemCoef<T>& cf, T fE, T (&out)[24]) {
+  int first[24];
+  bool merged = false;
+#pragma unroll
+  for (int a = 0; a < 24; ++a) first[a] = a;
+  for (int a = 0; a < 24; ++a) {
+    if (eq[a] < 0) continue;
+    for (int b = 0; b < a; ++b)
+      if (eq[b] == eq[a]) {
+        first[a] = first[b];
+        merged = true;
+        break;
+      }
+  }
+  const T V = ssp_at<T>(c, P, e, 24), l2 = cf.lam + T(2) * cf.mu;
+#pragma unroll 1
+  for (int j = 0; j < 8; ++j) {
+    const T dx = ssp_at<T>(c, P, e, 3 * j), dy = ssp_at<T>(c, P, e, 3 * j + 1), dz = ssp_at<T>(c, P, e, 3 * j + 2);
+    const T m = cf.rhoM != T(0) ? cf.rhoM * ssp_at<T>(c, P, e, 25 + j) : T(0);
+    out[3 * j] = (l2 * dx * dx + cf.mu * (dy * dy + dz * dz)) * V + m;
+    out[3 * j + 1] = (l2 * dy * dy + cf.mu * (dx * dx + dz * dz)) * V + m;
+    out[3 * j + 2] = (l2 * dz * dz + cf.mu * (dx * dx + dy * dy)) * V + m;
+  }
+  if (fE != T(0))
+    for (int a = 0; a < 24; ++a) out[a] += fE * ssp_term_entry<T>(c, P, e, a, a);
+  if (!merged) return;
+  for (int a = 0; a < 24; ++a) {
+    if (eq[a] < 0) continue;
+    for (int b = 0; b < 24; ++b) {
+      if (b == a || eq[b] != eq[a]) continue;
+      const int J = a / 3, p = a % 3, K = b / 3, q = b % 3;
+      const T gJ[3] = {ssp_at<T>(c, P, e, 3 * J), ssp_at<T>(c, P, e, 3 * J + 1), ssp_at<T>(c, P, e, 3 * J + 2)};
+      const T gK[3] = {ssp_at<T>(c, P, e, 3 * K), ssp_at<T>(c, P, e, 3 * K + 1), ssp_at<T>(c, P, e, 3 * K + 2)};
+      T kab = cf.lam * gJ[p] * gK[q] + cf.mu * gJ[q] * gK[p];
+      if (p == q) kab += cf.mu * (gJ[0] * gK[0] + gJ[1] * gK[1] + gJ[2] * gK[2]);
+      kab *= V;
+      if (fE != T(0)) kab += fE * ssp_term_entry<T>(c, P, e, a < b ? a : b, a < b ? b : a);
+      out[first[a]] += kab;
+    }
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ T load_coord(const double* c64, const float* c32, size_t i) {
   if constexpr (sizeof(T) == 8) return c64[i];
@@ -150,7 +247,7 @@ __device__ __forceinline__ T load_coord(const double* c64, const float* c32, siz
 }
 
 // The block kernel: y (or the node-space ynode) += K_eff x over one colour of blocks.
-template <typename T, bool DIAG>
+template <typename T, bool DIAG, bool SSP>
 __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a, const int32_t* __restrict__ cblocks) {
   extern __shared__ unsigned char smem_raw[];
   T* slots = reinterpret_cast<T*>(smem_raw);  // [24][kBlockElems]
@@ -165,9 +262,11 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
       const int n = a.node[k * P + e];
-      X[k] = load_coord<T>(a.coords64, a.coords32, n);
-      Y[k] = load_coord<T>(a.coords64, a.coords32, static_cast<size_t>(a.nnode) + n);
-      Z[k] = load_coord<T>(a.coords64, a.coords32, 2 * static_cast<size_t>(a.nnode) + n);
+      if constexpr (!SSP) {
+        X[k] = load_coord<T>(a.coords64, a.coords32, n);
+        Y[k] = load_coord<T>(a.coords64, a.coords32, static_cast<size_t>(a.nnode) + n);
+        Z[k] = load_coord<T>(a.coords64, a.coords32, 2 * static_cast<size_t>(a.nnode) + n);
+      }
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int q = a.node_eq[3 * n + d];
@@ -183,8 +282,13 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
     c.mu = static_cast<T>(fD * p.mu);
     c.rhoM = static_cast<T>(cM * p.rho);
     c.consistent = p.mass_type == 0;
-    if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
-    else element_apply(X, Y, Z, V, c, out);
+    if constexpr (SSP) {
+      if constexpr (DIAG) ssp_element_diagonal<T>(a.ssp, P, e, eqe, c, static_cast<T>(fD), out);
+      else ssp_element_apply<T>(a.ssp, P, e, V, c, static_cast<T>(fD), out);
+    } else {
+      if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
+      else element_apply_modal<T>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out);
+    }
 #pragma unroll
     for (int k = 0; k < 24; ++k) slots[k * kBlockElems + t] = out[k];
   }
@@ -227,7 +331,7 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
 // + e], for every element e < count (launch_operator with emit's Out::Slots,
 // operator_kernels.hpp:75-141, assembly.hpp:28-36). unit_dir >= 0 takes the rigid unit
 // translation in that direction as the input at every DOF, fixed ones included (:96-98).
-template <bool DIAG>
+template <bool DIAG, bool SSP>
 __global__ __launch_bounds__(128) void element_slots_kernel(KernelArgs<double> a, int32_t count, int unit_dir,
                                                              double* __restrict__ slots) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -257,8 +361,13 @@ __global__ __launch_bounds__(128) void element_slots_kernel(KernelArgs<double> a
   c.mu = fD * p.mu;
   c.rhoM = cM * p.rho;
   c.consistent = p.mass_type == 0;
-  if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
-  else element_apply(X, Y, Z, V, c, out);
+  if constexpr (SSP) {
+    if constexpr (DIAG) ssp_element_diagonal<double>(a.ssp, P, e, eqe, c, fD, out);
+    else ssp_element_apply<double>(a.ssp, P, e, V, c, fD, out);
+  } else {
+    if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
+    else element_apply_modal<double>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out);
+  }
 #pragma unroll
   for (int k = 0; k < 24; ++k) slots[k * P + e] = out[k];
 }
@@ -282,8 +391,12 @@ int launch_operator(fx_batch* b, const KernelArgs<T>& args) {
   for (int c = 0; c < b->blk.ncolors; ++c) {
     const int n = b->blk.colour_start[c + 1] - b->blk.colour_start[c];
     if (n == 0) continue;
-    hipLaunchKernelGGL((hex8_block_kernel<T, DIAG>), dim3(n), dim3(kBlockElems), smem, st, args,
-                       b->blk.colour_blocks + b->blk.colour_start[c]);
+    if (b->ssp)
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true>), dim3(n), dim3(kBlockElems), smem, st, args,
+                         b->blk.colour_blocks + b->blk.colour_start[c]);
+    else
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false>), dim3(n), dim3(kBlockElems), smem, st, args,
+                         b->blk.colour_blocks + b->blk.colour_start[c]);
   }
   if (b->tied) {
     const int nb = (b->neq + 255) / 256;
@@ -316,6 +429,7 @@ KernelArgs<T> operator_args(fx_batch* b, const fx_coeffs& k, const T* x, T* y, i
   a.x = x;
   a.y = y;
   a.accumulate = accumulate;
+  a.ssp = b->ssp_elem;
   if (b->tied) {
     if constexpr (sizeof(T) == 8) a.ynode = b->ynode;
     else a.ynode = b->ynode32;
@@ -408,10 +522,13 @@ int element_slots64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double
   if (b->count == 0) return FX_OK;
   const KernelArgs<double> a = operator_args<double>(b, k, x, static_cast<double*>(nullptr), 0);
   const dim3 g((b->count + 127) / 128);
-  if (diagonal)
-    hipLaunchKernelGGL(element_slots_kernel<true>, g, dim3(128), 0, b->ctx->stream, a, b->count, unit_dir, slots);
-  else
-    hipLaunchKernelGGL(element_slots_kernel<false>, g, dim3(128), 0, b->ctx->stream, a, b->count, unit_dir, slots);
+  if (b->ssp && !b->ssp_elem) {
+    b->ctx->error = "fx_run_operator: the SSPbrick batch keeps no element constants (create it with FX_BATCH_SLOTS)";
+    return FX_EINVAL;
+  }
+  auto* kern = diagonal ? (b->ssp ? element_slots_kernel<true, true> : element_slots_kernel<true, false>)
+                        : (b->ssp ? element_slots_kernel<false, true> : element_slots_kernel<false, false>);
+  hipLaunchKernelGGL(kern, g, dim3(128), 0, b->ctx->stream, a, b->count, unit_dir, slots);
   FX_HIP(b->ctx, hipGetLastError());
   return FX_OK;
 }
@@ -501,6 +618,29 @@ template int to_external<double>(fx_batch*, const double*, double*);
 template int to_external<float>(fx_batch*, const float*, float*);
 
 namespace {
+// The drop-in apply in place on reference-numbered vectors: lattice batches with reference words,
+// no nodal masses (their arrays are in the solver numbering)
+bool direct_ref(const fx_batch* b) {
+  return b->lattice && b->internal && b->words_ref && !b->pimy && !b->nodal_m && !b->nodal_am;
+}
+
+template <typename T>
+__global__ void zero_list_kernel(int32_t n, const int32_t* __restrict__ list, T* __restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[list[i]] = T(0);
+}
+
+template <typename T>
+int apply_ref(fx_batch* b, const fx_coeffs& k, const T* x, T* y) {
+  if (int rc = launch_box<T, false>(b, k, x, y, 0, nullptr, 1)) return rc;
+  if (b->norphans > 0)
+    hipLaunchKernelGGL(zero_list_kernel<T>, dim3((b->norphans + 255) / 256), dim3(256), 0, b->ctx->stream, b->norphans,
+                       b->orphans_ref, y);
+  FX_HIP(b->ctx, hipGetLastError());
+  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  return FX_OK;
+}
+
 template <typename T, typename Op>
 int external_call(fx_batch* b, const T* x, T* y, T* xi, T* yi, Op op) {
   if (b->internal) {
@@ -527,6 +667,8 @@ __global__ void add_perm_kernel(int32_t n, const int32_t* __restrict__ perm, con
 // on the fast path into the batch's scratch (fx_run_operator's FX_OUT_PLAIN / FX_OUT_ATOMIC).
 int add_elements64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double* x, double* y) {
   if (b->neq == 0) return FX_OK;
+  if (!diagonal && b->lattice && b->internal && b->words_ref)  // in place: y += the sums (orphans get none)
+    return launch_box<double, false>(b, k, x, y, 1, nullptr, 1);
   double* xi = b->work + 6 * static_cast<size_t>(b->neq);
   double* yi = b->work + 7 * static_cast<size_t>(b->neq);
   if (diagonal) {
@@ -546,12 +688,14 @@ int add_elements64(fx_batch* b, bool diagonal, const fx_coeffs& k, const double*
 }
 
 int external_apply64(fx_batch* b, const fx_coeffs& k, const double* x, double* y) {
+  if (direct_ref(b)) return apply_ref<double>(b, k, x, y);
   double* xi = b->work + 6 * static_cast<size_t>(b->neq);
   double* yi = b->work + 7 * static_cast<size_t>(b->neq);
   return external_call<double>(b, x, y, xi, yi,
                                [&](const double* xx, double* yy) { return operator_apply64(b, k, xx, yy); });
 }
 int external_apply32(fx_batch* b, const fx_coeffs& k, const float* x, float* y) {
+  if (direct_ref(b)) return apply_ref<float>(b, k, x, y);
   float* xi = b->work32 + 7 * static_cast<size_t>(b->neq);
   float* yi = b->work32 + 8 * static_cast<size_t>(b->neq);
   return external_call<float>(b, x, y, xi, yi, [&](const float* xx, float* yy) { return operator_apply32(b, k, xx, yy); });

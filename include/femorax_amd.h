@@ -163,6 +163,12 @@ typedef struct fx_batch_desc {
                                  an element slot buffer [24 * padded] and its gather list (and, on the lattice
                                  path, the connectivity, coordinates and node equations on the device);
                                  PressureIndependMultiYield batches always have them */
+#define FX_BATCH_SSPBRICK 4096u /* the element is SSPbrick<ElasticIsotropic> (ssp_brick.hpp: one mean-strain
+                                 point plus the stabilisation Kstab = Mben^T interior Mben, lumped mass; the
+                                 reference's batch of that (element, material) pair, batches.cpp:54-227)
+                                 instead of stdBrick; mass_type is ignored. Lattice path: 4- or 8-deep tiles
+                                 (TILE16 takes 8), no kernel variants; a lattice with equalDOF ties inside an
+                                 element runs on the general path */
 #define FX_BATCH_COMPACT 128u /* stdBrick<PressureIndependMultiYield> batches: the compacted return map
                                  (compacted_kernels.hpp:85-202): an elastic predictor per Gauss point, the
                                  plastic points listed, one full update per listed point, the element

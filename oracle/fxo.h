@@ -16,6 +16,7 @@
  * Pinning (how each part is checked, in tests/):
  *   brick geometry/force/tangent/mass/damping  reference goldens tests/reference/data/element
  *                                              (Brick_*), and oracle/_ref (reference headers)
+ *   SSPbrick force/tangent/mass/damping        reference goldens element/SSPbrick_* (16 files)
  *   ElasticIsotropic                           goldens nd/ElasticIsotropic_*, and oracle/_ref
  *   PIMY return map                            goldens nd/PressureIndependMultiYield_* (34)
  *   element operator / diagonal                K·v and diag(K) from the Brick goldens, oracle/_ref
@@ -95,12 +96,15 @@ int32_t fxo_node_bandwidth(const fxo_model* m, const int32_t* node_rank);
 /* ---- Element parameters ------------------------------------------------------------------ */
 typedef struct fxo_rayleigh { double alphaM, betaK, betaKinit, betaKcomm; } fxo_rayleigh;
 
-/* stdBrick<ElasticIsotropic>::Params (std_brick.hpp:181-185, elastic_isotropic.hpp:30-34) */
+/* stdBrick<ElasticIsotropic>::Params (std_brick.hpp:181-185, elastic_isotropic.hpp:30-34), or
+ * SSPbrick<ElasticIsotropic>::Params (ssp_brick.hpp:860-863; always lumped) when element = 1:
+ * the element functions below and every mesh-level function dispatch on it */
 typedef struct fxo_elastic {
   double E, nu, rho;
   double b[3];
   int32_t mass_type; /* 0 consistent, 1 lumped */
   fxo_rayleigh damp;
+  int32_t element;   /* 0 stdBrick, 1 SSPbrick */
 } fxo_elastic;
 
 typedef struct fxo_coeffs { double stiffness, damping, mass; int32_t initial; } fxo_coeffs;
@@ -122,6 +126,21 @@ void fxo_brick_operator_apply(const fxo_elastic* p, const fxo_coeffs* k, const d
                               const double* D, const double* ve, double* ye);
 void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3],
                                  const double* D, const int32_t* eq, double* de);
+
+/* ---- SSPbrick (ssp_brick.hpp, fxo_ssp.c) ---------------------------------------------------- */
+/* geom[33]: dNmod (node-major), mVol, the nodes' volume weights (opensees_mass: OpenSees' polyJac
+ * formula, else the exact integral of N_i); consts[110]: gst gut gus gstu, then interior's upper
+ * triangle (element_constants, ssp_brick.hpp:898-914) */
+void fxo_ssp_element(const fxo_elastic* p, const double xe[8][3], int opensees_mass, double* geom, double* consts);
+/* compute<WantK> (ssp_brick.hpp:1142-1193) with the set's Rayleigh damping */
+void fxo_ssp_force(const fxo_elastic* p, const double xe[8][3], int opensees_mass, const double* ue, const double* ve,
+                   double* fe, double* Ke);
+void fxo_ssp_mass(const fxo_elastic* p, const double xe[8][3], int opensees_mass, double* Me);
+void fxo_ssp_operator_apply(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3], const double* ve,
+                            double* ye);
+void fxo_ssp_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3], const int32_t* eq,
+                               double* de);
+double fxo_ssp_max_omega_sq(const fxo_elastic* p, const double xe[8][3]);
 
 /* ---- Global operator (gather assembly, internal_force.cpp:121-134, 545-599) ---------------- */
 /* y = K_eff x; params[nparam]. x, y in equation space. Deterministic slot-gather order. */

@@ -41,7 +41,7 @@ class Rayleigh(C.Structure):
 
 class Elastic(C.Structure):
     _fields_ = [("E", C.c_double), ("nu", C.c_double), ("rho", C.c_double), ("b", C.c_double * 3),
-                ("mass_type", C.c_int32), ("damp", Rayleigh)]
+                ("mass_type", C.c_int32), ("damp", Rayleigh), ("element", C.c_int32)]
 
 
 class Coeffs(C.Structure):
@@ -234,13 +234,16 @@ def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
 
 
 def elastic_params(params):
-    """params: list of dicts with E, nu, rho, b (3), mass_type, rayleigh (4) -> ctypes array."""
+    """params: list of dicts with E, nu, rho, b (3), mass_type, rayleigh (4), element ("stdBrick" or
+    "SSPbrick", default stdBrick) -> ctypes array."""
     arr = (Elastic * max(1, len(params)))()
     for i, p in enumerate(params):
         b = p.get("b", (0.0, 0.0, 0.0))
         r = p.get("rayleigh", (0.0, 0.0, 0.0, 0.0))
+        el = p.get("element", 0)
+        el = {"stdBrick": 0, "SSPbrick": 1}.get(el, el)
         arr[i] = Elastic(p["E"], p["nu"], p.get("rho", 0.0), (C.c_double * 3)(*b), int(p.get("mass_type", 0)),
-                         Rayleigh(*r))
+                         Rayleigh(*r), int(el))
     return arr
 
 

@@ -166,6 +166,10 @@ static void elastic_stress(double E, double nu, const double eps[6], double sig[
 
 void fxo_brick_force_elastic(const fxo_elastic* p, const double xe[8][3], const double* ue,
                              const double* ve, double* fe, double* Ke) {
+  if (p->element == 1) {
+    fxo_ssp_force(p, xe, 0, ue, ve, fe, Ke);
+    return;
+  }
   double xl[3][8];
   local_coords(xe, xl);
   double resid[24], fdamp[24];
@@ -302,6 +306,10 @@ static void point_mass_apply(const fxo_elastic* p, const double shp[4][8], doubl
 
 void fxo_brick_operator_apply(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3],
                               const double* D, const double* ve, double* ye) {
+  if (p->element == 1) {
+    fxo_ssp_operator_apply(p, k, xe, ve, ye);
+    return;
+  }
   double cK, c0, cc, cM;
   factors(&p->damp, k, &cK, &c0, &cc, &cM);
   double xl[3][8], Deff[36];
@@ -338,6 +346,10 @@ static double point_mass_entry(const fxo_elastic* p, const double shp[4][8], dou
 
 void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, const double xe[8][3],
                                  const double* D, const int32_t* eq, double* de) {
+  if (p->element == 1) {
+    fxo_ssp_operator_diagonal(p, k, xe, eq, de);
+    return;
+  }
   double cK, c0, cc, cM;
   factors(&p->damp, k, &cK, &c0, &cc, &cM);
   double xl[3][8], Deff[36];

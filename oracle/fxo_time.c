@@ -94,6 +94,7 @@ double fxo_max_eigenvalue_symmetric24(double A[24][24]) {
 }
 
 double fxo_brick_max_omega_sq(const fxo_elastic* p, const double xe[8][3]) {
+  if (p->element == 1) return fxo_ssp_max_omega_sq(p, xe);
   double m[8], K[576], fe[24], zero[24];
   memset(zero, 0, sizeof(zero));
   fxo_brick_lumped_mass(p->rho, xe, m);

@@ -6,6 +6,7 @@ reads the files the hot path needs and stores them unchanged in value (float64 r
 %.17g exactly) as compressed .npz, so the tests never need the reference tree:
 
   brick.npz        element/Brick_*_{state,mass}.txt   (stdBrick + ElasticIsotropic, 16 files)
+  ssp_brick.npz    element/SSPbrick_*_{state,mass}.txt (SSPbrick + ElasticIsotropic, 16 files)
   elastic.npz      nd/ElasticIsotropic_*.txt          (20 strain paths)
   pimy.npz         nd/PressureIndependMultiYield_*.txt (34 strain paths)
   constraints.npz  constraints/cases/*.txt + constraints/*.txt (OpenSees equation classes)
@@ -102,6 +103,9 @@ def main():
     el = sorted(os.path.join(data, "element", f) for f in os.listdir(os.path.join(data, "element"))
                 if f.startswith("Brick_"))
     pack(el, "brick.npz")
+    ssp = sorted(os.path.join(data, "element", f) for f in os.listdir(os.path.join(data, "element"))
+                 if f.startswith("SSPbrick_"))
+    pack(ssp, "ssp_brick.npz")
     nd = os.path.join(data, "nd")
     pack(sorted(os.path.join(nd, f) for f in os.listdir(nd) if f.startswith("ElasticIsotropic_")), "elastic.npz")
     pack(sorted(os.path.join(nd, f) for f in os.listdir(nd) if f.startswith("PressureIndependMultiYield_")),

@@ -343,3 +343,44 @@ def test_reference_benchmark_column_operator(ctx, side, tile8):
         assert rel_err(dy.get(), om.apply(op, x, fxo.coeffs(*k))) <= 1e-12
     a.diagonal(dy, core.coeffs(*kk))
     assert rel_err(dy.get(), om.diagonal(op, fxo.coeffs(*kk))) <= 1e-12
+
+
+@pytest.mark.parametrize("tied", [False, True])
+def test_reference_numbering_apply_gives_the_solver_numbering_bits(ctx, tied):
+    """fx_operator_apply on a lattice batch reads x and writes y in the reference numbering in
+    place (node words in that numbering); the solver-numbering apply (the one fx_cg runs) gives
+    the same values at every equation: the two results are the same multiset of bits, and match
+    the oracle. FP64 and FP32, a 42^3 box (cut tiles) and a tied column (tied-set sums);
+    fx_run_operator's in-place add as well."""
+    from oracle import fxo
+    model = meshes.column(30) if tied else S.config_c2(n=42)
+    pm = core.Mesh(model)
+    params = pm.params_for(model)
+    batch = core.Batch(ctx, pm, params, fp32=True, tune=False)
+    assert batch.info()["lattice"] == 1
+    x = meshes.random_vector(pm.neq, 81)
+    dx, dy, di = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, pm.neq), core.DeviceArray(ctx, pm.neq)
+    om = fxo.OracleMesh(model)
+    op = fxo.elastic_params(params)
+    for kk in COEFFS:
+        k = core.coeffs(*kk)
+        batch.apply(dx, dy, k)
+        want = om.apply(op, x, fxo.coeffs(*kk))
+        assert np.max(np.abs(dy.get() - want)) <= 1e-12 * np.max(np.abs(want)), kk
+        # the solver-numbering apply of the same x (fx_operator_time runs it on the batch's own
+        # numbering; x is a permutation-invariant input only for the multiset check below)
+        xs = np.full(pm.neq, 0.75)
+        dxs = core.DeviceArray.from_host(ctx, xs)
+        batch.time_apply(dxs, di, 1, 64, k)
+        batch.apply(dxs, dy, k)
+        np.testing.assert_array_equal(np.sort(dy.get()), np.sort(di.get()))
+        # fx_run_operator's add (y += K x in place): onto y = x, against x + the oracle's K x
+        dy.upload(x)
+        batch.run_operator(core.FX_OUT_PLAIN, x=dx, y=dy, k=k)
+        want = x + om.apply(op, x, fxo.coeffs(*kk))
+        assert np.max(np.abs(dy.get() - want)) <= 1e-12 * np.max(np.abs(want)), kk
+    x32 = core.DeviceArray.from_host(ctx, x.astype(np.float32))
+    y32 = core.DeviceArray(ctx, pm.neq, np.float32)
+    batch.apply_f32(x32, y32)
+    want = om.apply(op, x.astype(np.float32).astype(np.float64))
+    assert np.max(np.abs(y32.get() - want)) <= 2e-6 * np.max(np.abs(want))
