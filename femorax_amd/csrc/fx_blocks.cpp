@@ -4,10 +4,10 @@
 // in a fixed order (src/exec/assembly.cpp:28-38, pair order internal_force.cpp:121-134): every
 // element value is written to and read back from HBM (24 doubles per element). Here the slot
 // round trip stays in LDS: a workgroup owns a compact block of elements, writes their 24 values
-// to LDS, sums them per node in a fixed order and writes each node once per block. Blocks of
-// one colour share no node, so a colour is one launch with plain stores; across colours a
-// node's partial sums are added in colour order. The result is deterministic (the same bits
-// for every run and co-runner) without atomics.
+// to LDS and sums them per node in a fixed order. A node only its block touches is written
+// there; a node on a block boundary gets one partial-sum slot per block that touches it, and a
+// second kernel adds a node's slots in block order. Every block runs in one launch, and the
+// result is deterministic (the same bits for every run and co-runner) without atomics.
 #include <algorithm>
 #include <numeric>
 
@@ -74,7 +74,7 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
   const int32_t nb = static_cast<int32_t>(blocks.size());
   B.nblocks = nb;
   B.elems.assign(static_cast<size_t>(nb) * kBlockElems, -1);
-  B.corner.assign(static_cast<size_t>(nb) * 8 * kBlockElems, 0);
+  B.pos.assign(static_cast<size_t>(count), -1);
   B.slot_list.assign(static_cast<size_t>(nb) * 8 * kBlockElems, 0);
   B.ln_start.assign(static_cast<size_t>(nb) + 1, 0);
   B.lnode.clear();
@@ -86,6 +86,7 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
     nodes.reserve(el.size() * 8);
     for (size_t i = 0; i < el.size(); ++i) {
       B.elems[static_cast<size_t>(b) * kBlockElems + i] = el[i];
+      B.pos[el[i]] = b * kBlockElems + static_cast<int32_t>(i);
       for (int a = 0; a < 8; ++a) nodes.push_back(node[static_cast<size_t>(a) * padded + el[i]]);
     }
     std::sort(nodes.begin(), nodes.end());
@@ -97,7 +98,6 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
       for (int a = 0; a < 8; ++a) {
         const int32_t n = node[static_cast<size_t>(a) * padded + el[i]];
         const auto ln = static_cast<uint16_t>(std::lower_bound(nodes.begin(), nodes.end(), n) - nodes.begin());
-        B.corner[(static_cast<size_t>(b) * 8 + a) * kBlockElems + i] = ln;
         per[ln].push_back(static_cast<uint16_t>(i * 8 + a));
       }
     uint32_t pos = 0;
@@ -108,45 +108,30 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
     B.ln_start[b + 1] = B.ln_start[b] + static_cast<int32_t>(nodes.size());
     block_nodes[b] = std::move(nodes);
   }
-  // block colouring: greedy in block order, conflicts through shared nodes
-  std::vector<std::vector<int32_t>> blocks_at(nnode);
+  // nodes several blocks touch: one partial-sum slot per (node, block), a node's slots
+  // consecutive and in block order
+  std::vector<int32_t> nblk(nnode, 0);
   for (int32_t b = 0; b < nb; ++b)
-    for (int32_t n : block_nodes[b]) blocks_at[n].push_back(b);
-  std::vector<int32_t> colour(nb, -1);
-  int32_t ncol = 0;
-  std::vector<char> busy;
-  for (int32_t b = 0; b < nb; ++b) {
-    busy.assign(static_cast<size_t>(ncol) + 1, 0);
-    for (int32_t n : block_nodes[b])
-      for (int32_t o : blocks_at[n])
-        if (colour[o] >= 0) busy[colour[o]] = 1;
-    int32_t c = 0;
-    while (busy[c]) ++c;
-    colour[b] = c;
-    ncol = std::max(ncol, c + 1);
-  }
-  B.ncolors = ncol;
-  B.colour_start.assign(static_cast<size_t>(ncol) + 1, 0);
-  for (int32_t b = 0; b < nb; ++b) ++B.colour_start[colour[b] + 1];
-  for (int32_t c = 0; c < ncol; ++c) B.colour_start[c + 1] += B.colour_start[c];
-  B.colour_blocks.resize(nb);
-  {
-    std::vector<int32_t> fill(B.colour_start.begin(), B.colour_start.end() - 1);
-    for (int32_t b = 0; b < nb; ++b) B.colour_blocks[fill[colour[b]]++] = b;
-  }
-  // first writer of each node in launch order (colour, then block): it stores, later ones add
-  std::vector<char> touched(nnode, 0);
+    for (int32_t n : block_nodes[b]) ++nblk[n];
+  std::vector<int32_t> first_slot(nnode, -1);
+  B.sh_node.clear();
+  B.sh_start.assign(1, 0);
+  for (int32_t n = 0; n < nnode; ++n)
+    if (nblk[n] > 1) {
+      first_slot[n] = B.sh_start.back();
+      B.sh_node.push_back(n);
+      B.sh_start.push_back(B.sh_start.back() + nblk[n]);
+    }
+  std::vector<int32_t> taken(nnode, 0);
   B.lnode.assign(static_cast<size_t>(B.ln_start[nb]), 0);
-  for (int32_t b : B.colour_blocks) {
+  B.ln_part.assign(static_cast<size_t>(B.ln_start[nb]), -1);
+  for (int32_t b = 0; b < nb; ++b) {
     const auto& nodes = block_nodes[b];
     for (size_t ln = 0; ln < nodes.size(); ++ln) {
       const int32_t n = nodes[ln];
-      uint32_t v = static_cast<uint32_t>(n);
-      if (!touched[n]) {
-        touched[n] = 1;
-        v |= 0x80000000u;
-      }
-      B.lnode[static_cast<size_t>(B.ln_start[b]) + ln] = v;
+      const size_t at = static_cast<size_t>(B.ln_start[b]) + ln;
+      B.lnode[at] = static_cast<uint32_t>(n);
+      if (first_slot[n] >= 0) B.ln_part[at] = first_slot[n] + taken[n]++;
     }
   }
 }

@@ -199,9 +199,9 @@ __device__ __forceinline__ void box_apply(const T* xs, const int (&ln)[8], T lam
 // eta zeta, xi zeta, xi eta zeta (3, 6, 5, 7 here): Mben v = (H v)_m / 8, so the twelve modal
 // values go through one symmetric 12 x 12 product with S = interior / 64 (per parameter set,
 // packed upper triangle), scaled by the operator factor fE. The lumped mass mc x at each node.
-template <typename T, bool MASS, int NODES>
-__device__ __forceinline__ void box_apply_ssp(const T* xs, const int (&ln)[8], T lam, T mu, T mc, T fE,
-                                              const double* __restrict__ S, const BoxConst<T>& K, T (&y)[3][8]) {
+template <typename T, bool MASS, int NODES, typename SPtr>
+__device__ __forceinline__ void box_apply_ssp(const T* xs, const int (&ln)[8], T lam, T mu, T mc, T fE, SPtr S,
+                                              const BoxConst<T>& K, T (&y)[3][8]) {
   T b[3][8];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
@@ -396,11 +396,20 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
       int ln[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) ln[q] = (cx + (q & 1)) + NX * (cy + ((q >> 1) & 1)) + kPlane * lp(cz + (q >> 2));
-      if constexpr (SSP)
-        box_apply_ssp<T, MODE == 4, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
-                                               static_cast<T>(cM * p.rho * dvol), static_cast<T>(fD),
-                                               a.ssp + static_cast<size_t>(pidx) * kSspBox, a.K, out);
-      else
+      if constexpr (SSP) {
+        const T lam = static_cast<T>(fD * p.lam * dvol), mu = static_cast<T>(fD * p.mu * dvol);
+        const T mc = static_cast<T>(cM * p.rho * dvol), fE = static_cast<T>(fD);
+        // one parameter set across the wave (the usual case): its constants by scalar loads
+        const int pu = __builtin_amdgcn_readfirstlane(pidx);
+        if (__all(pidx == pu)) {
+          using ConstPtr = const __attribute__((address_space(4))) double*;
+          box_apply_ssp<T, MODE == 4, LDS_NODES>(xs, ln, lam, mu, mc, fE,
+                                                 (ConstPtr)(a.ssp + static_cast<size_t>(pu) * kSspBox), a.K, out);
+        } else {
+          box_apply_ssp<T, MODE == 4, LDS_NODES>(xs, ln, lam, mu, mc, fE, a.ssp + static_cast<size_t>(pidx) * kSspBox,
+                                                 a.K, out);
+        }
+      } else
         box_apply<T, MODE == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
                                            static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
     }

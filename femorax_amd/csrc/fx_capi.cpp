@@ -242,8 +242,8 @@ void fx_batch_destroy(fx_batch* b) {
   (void)hipStreamSynchronize(ctx->stream);
   fx::release_cg_graphs(ctx, b);  // they read the batch's arrays
   void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
-                  b->blk.ln_start, b->blk.lnode, b->blk.corner, b->blk.ln_slots, b->blk.slot_list,
-                  b->blk.colour_blocks, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
+                  b->blk.ln_start, b->blk.lnode, b->blk.ln_part, b->blk.ln_slots, b->blk.slot_list,
+                  b->blk.sh_node, b->blk.sh_start, b->blk.pos, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
@@ -485,10 +485,16 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         ok(dev_upload(ctx, b->blk.elems, blk.elems.data(), blk.elems.size(), by)) &&
         ok(dev_upload(ctx, b->blk.ln_start, blk.ln_start.data(), blk.ln_start.size(), by)) &&
         ok(dev_upload(ctx, b->blk.lnode, blk.lnode.data(), blk.lnode.size(), by)) &&
-        ok(dev_upload(ctx, b->blk.corner, blk.corner.data(), blk.corner.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.ln_part, blk.ln_part.data(), blk.ln_part.size(), by)) &&
         ok(dev_upload(ctx, b->blk.ln_slots, blk.ln_slots.data(), blk.ln_slots.size(), by)) &&
         ok(dev_upload(ctx, b->blk.slot_list, blk.slot_list.data(), blk.slot_list.size(), by)) &&
-        ok(dev_upload(ctx, b->blk.colour_blocks, blk.colour_blocks.data(), blk.colour_blocks.size(), by));
+        ok(blk.sh_node.empty() ? FX_OK : dev_upload(ctx, b->blk.sh_node, blk.sh_node.data(), blk.sh_node.size(), by)) &&
+        ok(dev_upload(ctx, b->blk.sh_start, blk.sh_start.data(), blk.sh_start.size(), by)) &&
+        ok(b->ssp ? dev_upload(ctx, b->blk.pos, blk.pos.data(), blk.pos.size(), by) : FX_OK);
+    // the block-boundary nodes' partial sums (three values per (node, block))
+    const size_t npart = static_cast<size_t>(std::max(1, blk.sh_start.back())) * 3;
+    ok(dev_zeros(ctx, b->partial64, npart, by)) && ok(b->has_f32 ? dev_zeros(ctx, b->partial32, npart, by) : FX_OK);
+    b->blk.nshared = static_cast<int32_t>(blk.sh_node.size());
     if (rc == FX_OK && b->tied) {
       ok(dev_zeros(ctx, b->ynode, static_cast<size_t>(3) * N, by)) &&
           ok(b->has_f32 ? dev_zeros(ctx, b->ynode32, static_cast<size_t>(3) * N, by) : FX_OK) &&
@@ -509,7 +515,10 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
       ok(dev_upload(ctx, b->ssp_box, box.data(), box.size(), by));
     }
     if (!use_lattice || (flags & FX_BATCH_SLOTS)) {
-      std::vector<double> cst(static_cast<size_t>(fx::kSspConsts) * std::max(1, P), 0.0);
+      // the general path reads them in block order (position b * kBlockElems + i), the lattice
+      // path's per-element forms in element order
+      const int64_t stride = use_lattice ? P : static_cast<int64_t>(blk.nblocks) * fx::kBlockElems;
+      std::vector<double> cst(static_cast<size_t>(fx::kSspConsts) * std::max<int64_t>(1, stride), 0.0);
       const int nth = static_cast<int>(std::min<int64_t>(16, std::max<int64_t>(1, E / 4096)));
       std::vector<std::thread> pool;
       for (int t = 0; t < nth; ++t)
@@ -521,7 +530,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
               for (int k = 0; k < 3; ++k) xe[a][k] = d->coords[static_cast<size_t>(k) * N + nd];
             }
             const fx::DevParam& q = b->host_params[d->param[e]];
-            fx::ssp_constants(xe, q.lam, q.mu, cst.data() + e, P);
+            fx::ssp_constants(xe, q.lam, q.mu, cst.data() + (use_lattice ? e : blk.pos[e]), stride);
           }
         });
       for (auto& th : pool) th.join();
@@ -661,13 +670,11 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     return rc;
   }
   b->blk.nblocks = blk.nblocks;
-  b->blk.ncolors = blk.ncolors;
-  b->blk.colour_start = blk.colour_start;
   b->info.count = E;
   b->info.neq = d->neq;
   b->info.nnode = N;
   b->info.nblocks = blk.nblocks;
-  b->info.ncolors = blk.ncolors;
+  b->info.ncolors = blk.nblocks > 0 ? 1 : 0;  // one launch of every block
   b->info.tied = b->tied ? 1 : 0;
   b->info.lattice = b->lattice ? 1 : 0;
   if (b->lattice && !b->ssp && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {

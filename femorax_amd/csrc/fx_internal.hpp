@@ -32,21 +32,22 @@ constexpr int kBlockElems = 256;
 
 // Host view of the block decomposition the kernels walk (built once per batch).
 struct Blocks {
-  int32_t nblocks = 0, ncolors = 0;
+  int32_t nblocks = 0;
   std::vector<int32_t> elems;        // [nblocks * kBlockElems] batch position or -1
+  std::vector<int32_t> pos;          // [count] block position b * kBlockElems + i of each element
   std::vector<int32_t> ln_start;     // [nblocks + 1] into lnode
-  std::vector<uint32_t> lnode;       // global node id | first-toucher flag (bit 31)
-  std::vector<uint16_t> corner;      // [nblocks * 8 * kBlockElems] local node of (corner, element)
+  std::vector<uint32_t> lnode;       // global node id of each block-local node
+  std::vector<int32_t> ln_part;      // per local node: its partial-sum slot (a node several blocks touch), or -1
   std::vector<uint32_t> ln_slots;    // per local node: start (low 16) | count (high 16)
   std::vector<uint16_t> slot_list;   // [nblocks * 8 * kBlockElems] elem_local * 8 + corner
-  std::vector<int32_t> colour_start; // [ncolors + 1] into colour_blocks
-  std::vector<int32_t> colour_blocks;
+  std::vector<int32_t> sh_node;      // nodes several blocks touch
+  std::vector<int32_t> sh_start;     // [nshared + 1]: node i's partial slots, in block order
   int32_t max_ln = 0;
 };
 
 // Partition the batch's elements into compact blocks (recursive coordinate bisection of the
-// element centroids), colour the blocks so that blocks of one colour share no node, and mark
-// each node's first writer in launch order.
+// element centroids). A node only one block touches is written by that block; the others get a
+// partial-sum slot per (node, block), summed in block order by a second kernel.
 void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nnode, const double* coords,
                   Blocks& out);
 
@@ -147,15 +148,16 @@ void tune_cache_put(const std::string& key, int depth, int variant);
 
 namespace fx {
 struct DevBlocks {
-  int32_t nblocks = 0, ncolors = 0;
+  int32_t nblocks = 0, nshared = 0;
   int32_t* elems = nullptr;
+  int32_t* pos = nullptr;  // SSPbrick batches: each element's block position (its constants' index)
   int32_t* ln_start = nullptr;
   uint32_t* lnode = nullptr;
-  uint16_t* corner = nullptr;
+  int32_t* ln_part = nullptr;
   uint32_t* ln_slots = nullptr;
   uint16_t* slot_list = nullptr;
-  int32_t* colour_blocks = nullptr;
-  std::vector<int32_t> colour_start;  // host copy for launches
+  int32_t* sh_node = nullptr;
+  int32_t* sh_start = nullptr;
 };
 
 struct DevParam {  // per parameter set, device

@@ -20,12 +20,15 @@ struct KernelArgs {
   const uint32_t* lnode;
   const uint32_t* ln_slots;
   const uint16_t* slot_list;
+  const int32_t* ln_part;
   const T* x;
   T* y;
   T* ynode;
   int accumulate;
   const int32_t* done;  // solver stop flag: the launch is a no-op once it is set
-  const double* ssp;    // SSPbrick: the elements' constants, value[k * padded + e], k < kSspConsts
+  const double* ssp;    // SSPbrick: the elements' constants, value[k * ssp_stride + at], k < kSspConsts
+  int64_t ssp_stride;   // padded (element order: the lattice path's per-element forms) or nblocks * kBlockElems
+  const int32_t* ssp_pos;  // block order: each element's index at (null: at = e)
 };
 
 // Effective factors of element_operator.hpp:45-56 for an elastic material (D_0 = D_c = D):

@@ -9,9 +9,10 @@
 //            (element_apply_modal)
 //   stage    the 24 values in LDS (the reference's slots, assembly.hpp:28-36, without the HBM
 //            round trip)
-//   reduce   per local node in a fixed (element, corner) order, then one store (first writer
-//            of the node in launch order) or read-add-write per node and equation
-// Colours of blocks are launched in order, so the scatter is deterministic.
+//   reduce   per local node in a fixed (element, corner) order; a node only this block touches
+//            is written (stored, or added with accumulate), a node on a block boundary goes to
+//            its partial-sum slot, and block_shared_sum_kernel adds a node's slots in block order
+// Every block runs in one launch; the scatter is deterministic.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -248,11 +249,11 @@ __device__ __forceinline__ T load_coord(const double* c64, const float* c32, siz
 
 // The block kernel: y (or the node-space ynode) += K_eff x over one colour of blocks.
 template <typename T, bool DIAG, bool SSP>
-__global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a, const int32_t* __restrict__ cblocks) {
+__global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a, T* __restrict__ partial) {
   extern __shared__ unsigned char smem_raw[];
   T* slots = reinterpret_cast<T*>(smem_raw);  // [24][kBlockElems]
   if (a.done && *a.done) return;
-  const int b = cblocks[blockIdx.x];
+  const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int e = a.elems[static_cast<size_t>(b) * kBlockElems + t];
   if (e >= 0) {
@@ -283,8 +284,10 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
     c.rhoM = static_cast<T>(cM * p.rho);
     c.consistent = p.mass_type == 0;
     if constexpr (SSP) {
-      if constexpr (DIAG) ssp_element_diagonal<T>(a.ssp, P, e, eqe, c, static_cast<T>(fD), out);
-      else ssp_element_apply<T>(a.ssp, P, e, V, c, static_cast<T>(fD), out);
+      // the constants in block order: coalesced loads
+      const size_t at = static_cast<size_t>(b) * kBlockElems + t;
+      if constexpr (DIAG) ssp_element_diagonal<T>(a.ssp, a.ssp_stride, at, eqe, c, static_cast<T>(fD), out);
+      else ssp_element_apply<T>(a.ssp, a.ssp_stride, at, V, c, static_cast<T>(fD), out);
     } else {
       if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
       else element_apply_modal<T>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out);
@@ -299,30 +302,66 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
     const uint32_t sc = a.ln_slots[ln];
     const int s0 = sc & 0xffff, ns = sc >> 16;
     T acc0 = 0, acc1 = 0, acc2 = 0;
-    for (int j = 0; j < ns; ++j) {
-      const int s = list[s0 + j];
-      const int el = s >> 3, corner = s & 7;
-      acc0 += slots[(3 * corner) * kBlockElems + el];
-      acc1 += slots[(3 * corner + 1) * kBlockElems + el];
-      acc2 += slots[(3 * corner + 2) * kBlockElems + el];
+    // the node's (element, corner) slots, eight at a time: their list entries are loaded before
+    // the LDS reads use them (one round of memory latency per eight, not one per slot)
+    for (int j0 = 0; j0 < ns; j0 += 8) {
+      int sl[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) sl[j] = j0 + j < ns ? list[s0 + j0 + j] : -1;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if (sl[j] < 0) break;
+        const int el = sl[j] >> 3, corner = sl[j] & 7;
+        acc0 += slots[(3 * corner) * kBlockElems + el];
+        acc1 += slots[(3 * corner + 1) * kBlockElems + el];
+        acc2 += slots[(3 * corner + 2) * kBlockElems + el];
+      }
     }
-    const uint32_t lv = a.lnode[ln];
-    const bool first = (lv & 0x80000000u) && !a.accumulate;
-    const int n = static_cast<int>(lv & 0x7fffffffu);
     const T acc[3] = {acc0, acc1, acc2};
+    const int part = a.ln_part[ln];
+    if (part >= 0) {  // a node other blocks touch too: this block's partial sum
+#pragma unroll
+      for (int d = 0; d < 3; ++d) partial[3 * static_cast<size_t>(part) + d] = acc[d];
+      continue;
+    }
+    const int n = static_cast<int>(a.lnode[ln]);
     if (a.ynode) {  // tied mesh: node space, contracted afterwards
 #pragma unroll
-      for (int d = 0; d < 3; ++d) {
-        T* dst = a.ynode + 3 * static_cast<size_t>(n) + d;
-        *dst = first ? acc[d] : *dst + acc[d];
-      }
+      for (int d = 0; d < 3; ++d) a.ynode[3 * static_cast<size_t>(n) + d] = acc[d];
     } else {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
         const int q = a.node_eq[3 * n + d];
         if (q < 0) continue;
-        a.y[q] = first ? acc[d] : a.y[q] + acc[d];
+        a.y[q] = a.accumulate ? a.y[q] + acc[d] : acc[d];
       }
+    }
+  }
+}
+
+// The nodes several blocks touch: their blocks' partial sums added in block order.
+template <typename T>
+__global__ void block_shared_sum_kernel(int32_t nshared, const int32_t* __restrict__ sh_node,
+                                        const int32_t* __restrict__ sh_start, const T* __restrict__ partial,
+                                        const int32_t* __restrict__ node_eq, T* __restrict__ y, T* __restrict__ ynode,
+                                        int accumulate, const int32_t* done) {
+  if (done && *done) return;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nshared) return;
+  const int n = sh_node[i];
+  T acc[3] = {T(0), T(0), T(0)};
+  for (int s = sh_start[i]; s < sh_start[i + 1]; ++s)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) acc[d] += partial[3 * static_cast<size_t>(s) + d];
+  if (ynode) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) ynode[3 * static_cast<size_t>(n) + d] = acc[d];
+  } else {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int q = node_eq[3 * n + d];
+      if (q < 0) continue;
+      y[q] = accumulate ? y[q] + acc[d] : acc[d];
     }
   }
 }
@@ -362,8 +401,9 @@ __global__ __launch_bounds__(128) void element_slots_kernel(KernelArgs<double> a
   c.rhoM = cM * p.rho;
   c.consistent = p.mass_type == 0;
   if constexpr (SSP) {
-    if constexpr (DIAG) ssp_element_diagonal<double>(a.ssp, P, e, eqe, c, fD, out);
-    else ssp_element_apply<double>(a.ssp, P, e, V, c, fD, out);
+    const size_t at = a.ssp_pos ? static_cast<size_t>(a.ssp_pos[e]) : static_cast<size_t>(e);
+    if constexpr (DIAG) ssp_element_diagonal<double>(a.ssp, a.ssp_stride, at, eqe, c, fD, out);
+    else ssp_element_apply<double>(a.ssp, a.ssp_stride, at, V, c, fD, out);
   } else {
     if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
     else element_apply_modal<double>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out);
@@ -388,16 +428,25 @@ template <typename T, bool DIAG>
 int launch_operator(fx_batch* b, const KernelArgs<T>& args) {
   hipStream_t st = b->ctx->stream;
   const size_t smem = sizeof(T) * 24 * kBlockElems;
-  for (int c = 0; c < b->blk.ncolors; ++c) {
-    const int n = b->blk.colour_start[c + 1] - b->blk.colour_start[c];
-    if (n == 0) continue;
-    if (b->ssp)
-      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true>), dim3(n), dim3(kBlockElems), smem, st, args,
-                         b->blk.colour_blocks + b->blk.colour_start[c]);
-    else
-      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false>), dim3(n), dim3(kBlockElems), smem, st, args,
-                         b->blk.colour_blocks + b->blk.colour_start[c]);
+  T* partial;
+  if constexpr (sizeof(T) == 8) partial = b->partial64;
+  else partial = b->partial32;
+  if (b->blk.nshared > 0 && !partial) {
+    b->ctx->error = "operator: no partial-sum buffer for this precision";
+    return FX_EINVAL;
   }
+  if (b->blk.nblocks > 0) {
+    if (b->ssp)
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true>), dim3(b->blk.nblocks), dim3(kBlockElems), smem, st, args,
+                         partial);
+    else
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false>), dim3(b->blk.nblocks), dim3(kBlockElems), smem, st,
+                         args, partial);
+  }
+  if (b->blk.nshared > 0)
+    hipLaunchKernelGGL(block_shared_sum_kernel<T>, dim3((b->blk.nshared + 255) / 256), dim3(256), 0, st,
+                       b->blk.nshared, b->blk.sh_node, b->blk.sh_start, partial, args.node_eq, args.y, args.ynode,
+                       args.accumulate, args.done);
   if (b->tied) {
     const int nb = (b->neq + 255) / 256;
     hipLaunchKernelGGL(contract_kernel<T>, dim3(nb), dim3(256), 0, st, b->neq, b->con_start, b->con_list,
@@ -426,10 +475,13 @@ KernelArgs<T> operator_args(fx_batch* b, const fx_coeffs& k, const T* x, T* y, i
   a.lnode = b->blk.lnode;
   a.ln_slots = b->blk.ln_slots;
   a.slot_list = b->blk.slot_list;
+  a.ln_part = b->blk.ln_part;
   a.x = x;
   a.y = y;
   a.accumulate = accumulate;
   a.ssp = b->ssp_elem;
+  a.ssp_stride = b->lattice ? b->padded : static_cast<int64_t>(b->blk.nblocks) * kBlockElems;
+  a.ssp_pos = b->lattice ? nullptr : b->blk.pos;
   if (b->tied) {
     if constexpr (sizeof(T) == 8) a.ynode = b->ynode;
     else a.ynode = b->ynode32;
