@@ -55,6 +55,10 @@ struct BoxArgs {
   int use_interior;  // regular tiles' interior nodes addressed from their base (internal numbering only)
   const int32_t* done;
   const double* ssp;  // SSPbrick: per parameter set, the modal stabilisation and diag(K_e) (kSspBox each)
+  // rectilinear lattices: (kappa, h) of each cell interval along x, y, z
+  const double* kx;
+  const double* ky;
+  const double* kz;
 };
 
 // The element operator in the modal basis of the trilinear field.
@@ -344,7 +348,8 @@ struct NoHook {
 };
 
 // One pass of a tile: cell (cx, cy, cz = wave + 4 pass). MODE: 0 apply without mass terms, 1
-// apply with mass terms, 2 diagonal; 3, 4, 5 the same for SSPbrick cells. The cell's results are
+// apply with mass terms, 2 diagonal; 3, 4, 5 the same for SSPbrick cells; 6, 7, 8 the same on a
+// rectilinear lattice (each cell's spacings from the tile's origin). The cell's results are
 // summed per node and committed.
 //
 // Every lane runs the full 8 x 8 plane: a cell past the end of a cut tile has no parameter set
@@ -354,10 +359,11 @@ struct NoHook {
 // mid() runs after the barrier that ends the pass's reads of x, before its commits.
 template <typename T, int MODE, int TZ, int RING = 0, typename Mid = NoHook>
 __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T* xs, const uint32_t* wsh, T* top,
-                                         int t, int pass, const Mid& mid = Mid{}) {
+                                         int t, int pass, int tile, const Mid& mid = Mid{}) {
   using G = TileGeom<TZ>;
   constexpr bool DIAG = MODE % 3 == 2;
-  constexpr bool SSP = MODE >= 3;
+  constexpr bool SSP = MODE >= 3 && MODE < 6;
+  constexpr bool RECT = MODE >= 6;
   constexpr int NX = kTileX + 1;
   constexpr int LDS_NODES = RING ? RING * kPlane : G::kNodes;  // component stride of xs
   auto lp = [](int z) { return RING ? z % RING : z; };
@@ -372,7 +378,27 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
     const DevParam& p = a.params[pidx];
     double fD, cM;
     operator_factors(p, a.k, fD, cM);
-    const double dvol = a.hx * a.hy * a.hz * 0.125;
+    // the cell's geometry: the lattice's cell size, or on a rectilinear lattice the spacings of
+    // the cell's plane intervals
+    double hc[3] = {a.hx, a.hy, a.hz};
+    BoxConst<T> Kc = a.K;
+    if constexpr (RECT) {
+      const TileHeader& th = a.tiles[tile];
+      const double* kh[3] = {a.kx + 2 * (th.ox + cx), a.ky + 2 * (th.oy + cy), a.kz + 2 * (th.oz + cz)};
+      double ks = 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const double kap = kh[d][0];
+        hc[d] = kh[d][1];
+        Kc.kap[d] = static_cast<T>(kap);
+        Kc.w0[d] = static_cast<T>(8.0 * kap);
+        Kc.w1[d] = static_cast<T>(8.0 / 3.0 * kap);
+        Kc.kk[d] = static_cast<T>(8.0 / 9.0 * kap * kap);
+        ks += kap * kap;
+      }
+      Kc.kks = static_cast<T>(8.0 / 9.0 * ks);
+    }
+    const double dvol = hc[0] * hc[1] * hc[2] * 0.125;
     if constexpr (DIAG && SSP) {
       // diag(K_e) per corner, kept per parameter set, and the lumped mass
       const double* dk = a.ssp + static_cast<size_t>(pidx) * kSspBox + 78;
@@ -383,7 +409,7 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
         for (int q = 0; q < 8; ++q) out[c][q] = static_cast<T>(fD * dk[8 * c + q] + mass);
     } else if constexpr (DIAG) {
       // diag(K_e): Gauss sums of (dN/dx_d)^2 are 8 / (9 h_d^2) at every node of a box; of N^2, 8/27
-      const double S[3] = {8.0 / (9.0 * a.hx * a.hx), 8.0 / (9.0 * a.hy * a.hy), 8.0 / (9.0 * a.hz * a.hz)};
+      const double S[3] = {8.0 / (9.0 * hc[0] * hc[0]), 8.0 / (9.0 * hc[1] * hc[1]), 8.0 / (9.0 * hc[2] * hc[2])};
       const double lam = fD * p.lam, mu = fD * p.mu;
       const double mass = cM * p.rho * dvol * (p.mass_type == 0 ? 8.0 / 27.0 : 1.0);
 #pragma unroll
@@ -410,8 +436,9 @@ __device__ __forceinline__ void box_tile(const BoxArgs<T>& a, int pidx, const T*
                                                  a.K, out);
         }
       } else
-        box_apply<T, MODE == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol), static_cast<T>(fD * p.mu * dvol),
-                                           static_cast<T>(cM * p.rho * dvol), p.mass_type != 0, a.K, out);
+        box_apply<T, MODE % 3 == 1, LDS_NODES>(xs, ln, static_cast<T>(fD * p.lam * dvol),
+                                               static_cast<T>(fD * p.mu * dvol), static_cast<T>(cM * p.rho * dvol),
+                                               p.mass_type != 0, Kc, out);
     }
   }
   // in-plane sums, x first then y: the cell's corners a (cx, cy), b (cx+1, cy), c (cx, cy+1),
@@ -520,7 +547,7 @@ __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_box_
     TileFetch<T, TZ> fn = f;
     if (PERSIST && next < a.ntiles) fetch_tile<T, DIAG, TZ>(a, next, t, fn);
 #pragma unroll
-    for (int pass = 0; pass < G::kPasses; ++pass) box_tile<T, MODE, TZ>(a, f.pidx[pass], xs, wsh, top, t, pass);
+    for (int pass = 0; pass < G::kPasses; ++pass) box_tile<T, MODE, TZ>(a, f.pidx[pass], xs, wsh, top, t, pass, tile);
     if (next >= a.ntiles) break;
     __syncthreads();  // the LDS takes the next tile
     tile = next;
@@ -616,13 +643,13 @@ __global__ __launch_bounds__(kTileThreads, WAVES > 0 ? WAVES : 1) void hex8_stre
       // planes 4 pass + 5 .. 4 pass + 8 replace planes 4 pass .. 4 pass + 3 (ring slots)
       const int n0 = (4 * pass + kRingPlanes) * kPlane;
       PlaneFetch<T> f;
-      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass,
+      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass, tile,
                                          [&] { fetch_planes<T, DIAG>(a, words, slots, n0, 4 * kPlane, t, f); });
       __syncthreads();  // this pass's commits have read the words of its planes
       store_planes(f, n0, 4 * kPlane, t, xs, wsh);
       __syncthreads();
     } else {
-      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass);
+      box_tile<T, MODE, TZ, kRingPlanes>(a, pidx[pass], xs, wsh, top, t, pass, tile);
     }
   }
 }
@@ -777,10 +804,17 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
                                  hex8_box_kernel<T, 5, 0, false, 4>};
   static const Kernel ssp8[3] = {hex8_box_kernel<T, 3, 0, false, 8>, hex8_box_kernel<T, 4, 0, false, 8>,
                                  hex8_box_kernel<T, 5, 0, false, 8>};
-  const int v = b->ssp ? 0 : b->box_variant & 15;
+  // rectilinear lattices: the same, each cell's spacings from its plane intervals
+  static const Kernel rect4[3] = {hex8_box_kernel<T, 6, 0, false, 4>, hex8_box_kernel<T, 7, 0, false, 4>,
+                                  hex8_box_kernel<T, 8, 0, false, 4>};
+  static const Kernel rect8[3] = {hex8_box_kernel<T, 6, 0, false, 8>, hex8_box_kernel<T, 7, 0, false, 8>,
+                                  hex8_box_kernel<T, 8, 0, false, 8>};
+  const bool one = b->ssp || b->rect;  // a single kernel per mode
+  const int v = one ? 0 : b->box_variant & 15;
   const int m = DIAG ? 2 : mass ? 1 : 0;
-  const bool stream = !b->ssp && (b->tile_z == 16 || ((v & 8) && b->tile_z == 8));
+  const bool stream = !one && (b->tile_z == 16 || ((v & 8) && b->tile_z == 8));
   const Kernel kernel = b->ssp            ? (b->tile_z == 8 ? ssp8 : ssp4)[m]
+                        : b->rect         ? (b->tile_z == 8 ? rect8 : rect4)[m]
                         : b->tile_z == 16 ? stream16[m][v & 3]
                         : stream          ? stream8[m][v & 3]
                                           : (b->tile_z == 8 ? table8 : table4)[m][v & 7];
@@ -819,6 +853,9 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   a.accumulate = accumulate;
   a.done = done;
   a.ssp = b->ssp_box;
+  a.kx = b->kx;
+  a.ky = b->ky;
+  a.kz = b->kz;
   if (!a.partial) {
     b->ctx->error = "operator: no partial-sum buffer for this precision";
     return FX_EINVAL;

@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <set>
 #include <thread>
 #include <vector>
 
@@ -248,7 +249,8 @@ void fx_batch_destroy(fx_batch* b) {
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev,
-                  b->words_ref, b->shared_word_ref, b->tied_word_ref, b->orphans_ref, b->ssp_elem, b->ssp_box};
+                  b->words_ref, b->shared_word_ref, b->tied_word_ref, b->orphans_ref, b->ssp_elem, b->ssp_box,
+                  b->kx, b->ky, b->kz};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -392,9 +394,14 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
                     : (flags & FX_BATCH_TILE16) ? (b->ssp ? 8 : 16)
                     : E >= (1 << 20)           ? 8
                                                : 4;
+  // SSPbrick's lattice constants assume one cell size: rectilinear SSPbrick lattices take the
+  // general path; rectilinear lattices run 4- or 8-deep tiles
   bool use_lattice = !(flags & FX_BATCH_GENERAL) &&
                      fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
-                                       (flags & FX_BATCH_XCD_SLABS) != 0, depth, lat);
+                                       (flags & FX_BATCH_XCD_SLABS) != 0, depth, !b->ssp, lat);
+  if (use_lattice && lat.rect && depth == 16)
+    use_lattice = fx::build_lattice(E, P, d->node, d->param, N, d->coords, node_eq.data(), d->neq,
+                                    (flags & FX_BATCH_XCD_SLABS) != 0, 8, true, lat);
   // an SSPbrick lattice with DOFs one element ties together: the general path (the diagonal's
   // cross terms come from each element's constants there)
   if (use_lattice && b->ssp && !lat.tied_word.empty()) {
@@ -431,6 +438,22 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     b->hx = lat.hx;
     b->hy = lat.hy;
     b->hz = lat.hz;
+    b->rect = lat.rect;
+    if (lat.rect) {
+      // each cell interval's (kappa = 1 / (4 h), h) along each axis
+      const std::vector<double>* planes[3] = {&lat.px, &lat.py, &lat.pz};
+      double** dst[3] = {&b->kx, &b->ky, &b->kz};
+      for (int ax = 0; ax < 3; ++ax) {
+        const auto& pl = *planes[ax];
+        std::vector<double> k(2 * (pl.size() - 1));
+        for (size_t i = 0; i + 1 < pl.size(); ++i) {
+          const double h = pl[i + 1] - pl[i];
+          k[2 * i] = 0.25 / h;
+          k[2 * i + 1] = h;
+        }
+        ok(dev_upload(ctx, *dst[ax], k.data(), k.size(), by));
+      }
+    }
     b->ntiles = static_cast<int32_t>(lat.tiles.size());
     b->internal = true;
     b->nshared = static_cast<int32_t>(lat.shared_word.size());
@@ -567,11 +590,30 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         fx::brick_mass_unit(box, Mn);
       }
     }
+    std::set<std::array<double, 4>> rect_boxes;  // (hx, hy, hz, set) of a rectilinear lattice's cells
     for (int32_t e = 0; e < E; ++e) {
-      if (!use_lattice) {
+      if (!use_lattice || lat.rect) {
         double xe[8][3];
         element_xe(e, xe);
         fx::brick_node_weights(xe, w);
+        if (lat.rect) {
+          const double hx = xe[1][0] - xe[0][0], hy = xe[3][1] - xe[0][1], hz = xe[4][2] - xe[0][2];
+          rect_boxes.insert({hx, hy, hz, static_cast<double>(d->param[e])});
+          if (tied_lattice) {  // this cell's own unit stiffness parts and mass (when it ties DOFs)
+            bool ties = false;
+            for (int a = 0; a < 24 && !ties; ++a) {
+              const int32_t q = d->eq[static_cast<size_t>(a) * P + e];
+              for (int c = a + 1; c < 24 && q >= 0; ++c) ties |= d->eq[static_cast<size_t>(c) * P + e] == q;
+            }
+            if (ties) {
+              const double box[8][3] = {{0, 0, 0}, {hx, 0, 0}, {hx, hy, 0}, {0, hy, 0},
+                                        {0, 0, hz}, {hx, 0, hz}, {hx, hy, hz}, {0, hy, hz}};
+              fx::brick_stiffness(box, 1.0, 0.0, Kl);
+              fx::brick_stiffness(box, 0.0, 1.0, Km);
+              fx::brick_mass_unit(box, Mn);
+            }
+          }
+        }
       }
       const fx::DevParam& p = b->host_params[d->param[e]];
       if (tied_lattice)
@@ -597,6 +639,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         any_body |= p.b[k % 3] != 0.0;
       }
     }
+    for (const auto& bx : rect_boxes) b->rect_boxes.push_back(bx);
     if (use_lattice) {
       b->internal_of.assign(n, -1);
       for (int32_t i = 0; i < n; ++i) b->internal_of[lat.perm[i]] = i;
@@ -677,7 +720,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   b->info.ncolors = blk.nblocks > 0 ? 1 : 0;  // one launch of every block
   b->info.tied = b->tied ? 1 : 0;
   b->info.lattice = b->lattice ? 1 : 0;
-  if (b->lattice && !b->ssp && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {
+  if (b->lattice && !b->ssp && !b->rect && !(flags & FX_BATCH_NO_TUNE) && E >= kTuneMinElements) {
     // the variant autotune, kept per device and model (with this tile depth) in the cache
     const std::string key = fx::tune_machine_key(ctx->device) + "|" + lattice_model_key(d, b->tile_z);
     int depth = 0, variant = -1;
@@ -707,7 +750,7 @@ int fx_batch_tune(fx_batch* b, int variant) {
     b->info.variant = variant;
     return FX_OK;
   }
-  if (!b->lattice || b->ssp) return FX_OK;  // one variant only
+  if (!b->lattice || b->ssp || b->rect) return FX_OK;  // one variant only
   // time each variant on scratch vectors (the reference's force-settings autotune,
   // src/exec/force_settings.cpp:135-155): three rounds over the variants, each sample a warm-up
   // apply then ten back-to-back applies between two events (one apply is a few tens of
@@ -1015,7 +1058,15 @@ int fx_force(fx_batch* b, const double* u, const double* v, double* f) {
 int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
   if (!b || !dt || (scheme != 0 && scheme != 1)) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_critical_dt: bad argument");
   double best = INFINITY;
-  if (b->lattice) {
+  if (b->lattice && b->rect) {
+    // a rectilinear lattice: each distinct (cell size, parameter set) once
+    for (const auto& bx : b->rect_boxes) {
+      const double hx = bx[0], hy = bx[1], hz = bx[2];
+      const double box[8][3] = {{0, 0, 0}, {hx, 0, 0}, {hx, hy, 0}, {0, hy, 0}, {0, 0, hz}, {hx, 0, hz}, {hx, hy, hz}, {0, hy, hz}};
+      const fx::DevParam& p = b->host_params[static_cast<size_t>(bx[3])];
+      best = std::min(best, fx::stable_dt(scheme, p, fx::brick_max_omega_sq(box, p.lam, p.mu, p.rho)));
+    }
+  } else if (b->lattice) {
     const double hx = b->hx, hy = b->hy, hz = b->hz;
     const double box[8][3] = {{0, 0, 0}, {hx, 0, 0}, {hx, hy, 0}, {0, hy, 0}, {0, 0, hz}, {hx, 0, hz}, {hx, hy, hz}, {0, hy, hz}};
     for (const fx::DevParam& p : b->host_params)

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <array>
 #include <vector>
 
 #include "../../include/femorax_amd.h"
@@ -65,11 +66,17 @@ struct TileHeader {
   int32_t cell_offset;      // into the cell parameters: tile * 64 tz (full-tile order, x fastest)
   int32_t interior_base;    // a full tile whose strictly-interior nodes are its own, with three free DOFs
                             // each, numbered in interior order (x fastest): their first equation; else -1
-  int32_t pad[2];
+  int32_t ox, oy, oz;       // the lattice cell of the tile's first cell (rectilinear lattices: its spacings)
+  int32_t pad;
 };
 
 struct Lattice {
-  double hx = 0, hy = 0, hz = 0;
+  double hx = 0, hy = 0, hz = 0;      // the cell size (a uniform lattice)
+  // a rectilinear lattice (the plane spacings differ along some axis): the plane coordinates along
+  // each axis and each element's lattice cell (i, j, k)
+  bool rect = false;
+  std::vector<double> px, py, pz;
+  std::vector<int32_t> ecell;         // [3 * count]
   int tz = 4;                         // cells per tile in z
   std::vector<TileHeader> tiles;      // in launch order (XCD-aware slabs or lattice order)
   std::vector<uint32_t> words;        // per tile node: first equation (28 bits) | free-DOF mask << 28 | shared << 31
@@ -91,9 +98,10 @@ struct Lattice {
 };
 
 // tile_z: cells per tile in z (4 or 8; tiles are 8 x 8 x tile_z cells)
+// allow_rect: accept rectilinear lattices (else only uniform ones)
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
                    const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, int tile_z,
-                   Lattice& out);
+                   bool allow_rect, Lattice& out);
 
 // Contraction of node-space values into equations when equations are shared by several
 // nodes (equalDOF ties): eq -> list of node DOFs (3 n + d), in ascending order.
@@ -226,6 +234,13 @@ struct fx_batch {
   bool ssp = false;
   double* ssp_elem = nullptr;
   double* ssp_box = nullptr;
+  // rectilinear lattices: (kappa = 1 / (4 h), h) of every cell interval along x, y, z, and the
+  // distinct (hx, hy, hz, parameter set) of the cells (the stable step)
+  bool rect = false;
+  double* kx = nullptr;
+  double* ky = nullptr;
+  double* kz = nullptr;
+  std::vector<std::array<double, 4>> rect_boxes;
   fx::DevBlocks blk;
   // tied meshes: node-space output and its contraction
   bool tied = false;

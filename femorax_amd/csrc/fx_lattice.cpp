@@ -1,10 +1,12 @@
 // Lattice detection for the structured fast path.
 //
-// Femora-built soil models are mostly regular grids of axis-aligned bricks. When every element
-// of a batch is an axis-aligned box of one size (hx, hy, hz) sitting on a common lattice, with
-// its nodes in the OpenSees Brick order, the kernels need neither the connectivity nor the
-// coordinates: a brick of lattice cells is a workgroup's block, its nodes are addressed by
-// lattice position, and the element geometry is three numbers. Per block the device then holds
+// Femora-built soil models are mostly regular grids of axis-aligned bricks, often graded (layers
+// of different element heights). When every element of a batch is an axis-aligned box spanning
+// one interval of a common set of planes along each axis, with its nodes in the OpenSees Brick
+// order, the kernels need neither the connectivity nor the coordinates: a brick of lattice cells
+// is a workgroup's block, its nodes are addressed by lattice position, and the element geometry
+// is three numbers (one cell size for a uniform lattice, the cell's plane spacings for a
+// rectilinear one). Per block the device then holds
 // one 32-bit word per lattice node (first equation | free-DOF mask | shared flag) and a 16-bit
 // parameter index per cell. Anything else (distorted or rotated elements, holes inside
 // a tile are fine, ties that merge a node's DOFs with another node's) uses the general blocks.
@@ -19,48 +21,97 @@ namespace fx {
 
 bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int32_t* param, int32_t nnode,
                    const double* coords, const int32_t* node_eq, int32_t neq, bool xcd_order, int tile_z,
-                   Lattice& L) {
+                   bool allow_rect, Lattice& L) {
   if (count <= 0 || (tile_z != 4 && tile_z != 8 && tile_z != 16)) return false;
   const double* X = coords;
   const double* Y = coords + nnode;
   const double* Z = coords + 2 * static_cast<size_t>(nnode);
   auto nd = [&](int a, int32_t e) { return node[static_cast<size_t>(a) * padded + e]; };
-  // element 0 fixes the cell size
-  const int32_t n0 = nd(0, 0);
-  const double hx = X[nd(1, 0)] - X[n0], hy = Y[nd(3, 0)] - Y[n0], hz = Z[nd(4, 0)] - Z[n0];
-  if (!(hx > 0 && hy > 0 && hz > 0)) return false;
-  const double tol = 1e-9 * std::min(hx, std::min(hy, hz));
-  // corner offsets of the OpenSees Brick order
   static const int OX[8] = {0, 1, 1, 0, 0, 1, 1, 0}, OY[8] = {0, 0, 1, 1, 0, 0, 1, 1}, OZ[8] = {0, 0, 0, 0, 1, 1, 1, 1};
-  double x0 = 1e300, y0 = 1e300, z0 = 1e300;
-  for (int32_t e = 0; e < count; ++e) {
-    const int32_t b = nd(0, e);
-    x0 = std::min(x0, X[b]);
-    y0 = std::min(y0, Y[b]);
-    z0 = std::min(z0, Z[b]);
-  }
+  const double* C3[3] = {X, Y, Z};
+  const int* O3[3] = {OX, OY, OZ};
   std::vector<int32_t> ci(count), cj(count), ck(count);
+  int32_t* cidx[3] = {ci.data(), cj.data(), ck.data()};
   int32_t ni = 0, nj = 0, nk = 0;
-  for (int32_t e = 0; e < count; ++e) {
-    const int32_t b = nd(0, e);
-    for (int a = 0; a < 8; ++a) {
-      const int32_t n = nd(a, e);
-      if (std::fabs(X[n] - X[b] - OX[a] * hx) > tol || std::fabs(Y[n] - Y[b] - OY[a] * hy) > tol ||
-          std::fabs(Z[n] - Z[b] - OZ[a] * hz) > tol)
-        return false;
+  double h0[3] = {0, 0, 0};
+  std::vector<double> planes[3];
+  // a uniform lattice: element 0 fixes the cell size; every element an axis-aligned box of that
+  // size at a whole number of cells from the lowest corner
+  auto uniform_lattice = [&]() {
+    const int32_t n0 = nd(0, 0);
+    h0[0] = X[nd(1, 0)] - X[n0];
+    h0[1] = Y[nd(3, 0)] - Y[n0];
+    h0[2] = Z[nd(4, 0)] - Z[n0];
+    if (!(h0[0] > 0 && h0[1] > 0 && h0[2] > 0)) return false;
+    const double tol = 1e-9 * std::min(h0[0], std::min(h0[1], h0[2]));
+    double lo[3] = {1e300, 1e300, 1e300};
+    for (int32_t e = 0; e < count; ++e)
+      for (int ax = 0; ax < 3; ++ax) lo[ax] = std::min(lo[ax], C3[ax][nd(0, e)]);
+    int32_t* n3[3] = {&ni, &nj, &nk};
+    *n3[0] = *n3[1] = *n3[2] = 0;
+    for (int32_t e = 0; e < count; ++e) {
+      const int32_t b = nd(0, e);
+      for (int a = 0; a < 8; ++a)
+        for (int ax = 0; ax < 3; ++ax)
+          if (std::fabs(C3[ax][nd(a, e)] - C3[ax][b] - O3[ax][a] * h0[ax]) > tol) return false;
+      for (int ax = 0; ax < 3; ++ax) {
+        const double f = (C3[ax][b] - lo[ax]) / h0[ax], r = std::nearbyint(f);
+        if (std::fabs(f - r) > 1e-6 || r > 1e6) return false;
+        cidx[ax][e] = static_cast<int32_t>(r);
+        *n3[ax] = std::max(*n3[ax], cidx[ax][e] + 1);
+      }
     }
-    const double fi = (X[b] - x0) / hx, fj = (Y[b] - y0) / hy, fk = (Z[b] - z0) / hz;
-    const double ri = std::nearbyint(fi), rj = std::nearbyint(fj), rk = std::nearbyint(fk);
-    if (std::fabs(fi - ri) > 1e-6 || std::fabs(fj - rj) > 1e-6 || std::fabs(fk - rk) > 1e-6) return false;
-    if (ri > 1e6 || rj > 1e6 || rk > 1e6) return false;
-    ci[e] = static_cast<int32_t>(ri);
-    cj[e] = static_cast<int32_t>(rj);
-    ck[e] = static_cast<int32_t>(rk);
-    ni = std::max(ni, ci[e] + 1);
-    nj = std::max(nj, cj[e] + 1);
-    nk = std::max(nk, ck[e] + 1);
+    return true;
+  };
+  // a rectilinear lattice: the planes along each axis are the distinct node coordinates of the
+  // batch (merged within 1e-9 of the smallest cell size); every element spans one interval along
+  // each axis with its nodes in the OpenSees Brick order
+  auto rect_lattice = [&]() {
+    double hmin = 1e300;
+    for (int32_t e = 0; e < count; ++e)
+      for (int ax = 0; ax < 3; ++ax) {
+        const double h = C3[ax][nd(ax == 0 ? 1 : ax == 1 ? 3 : 4, e)] - C3[ax][nd(0, e)];
+        if (!(h > 0)) return false;
+        hmin = std::min(hmin, h);
+      }
+    const double tol = 1e-9 * hmin;
+    for (int ax = 0; ax < 3; ++ax) {
+      std::vector<double> v;
+      v.reserve(static_cast<size_t>(count) * 2);
+      for (int32_t e = 0; e < count; ++e) {
+        v.push_back(C3[ax][nd(0, e)]);
+        v.push_back(C3[ax][nd(6, e)]);
+      }
+      std::sort(v.begin(), v.end());
+      planes[ax].clear();
+      for (double c : v)
+        if (planes[ax].empty() || c - planes[ax].back() > tol) planes[ax].push_back(c);
+      if (planes[ax].size() < 2 || planes[ax].size() > 1000001) return false;
+    }
+    auto plane_of = [&](int ax, double c) -> int32_t {
+      const auto& P = planes[ax];
+      auto it = std::lower_bound(P.begin(), P.end(), c - tol);
+      if (it == P.end() || std::fabs(*it - c) > tol) return -1;
+      return static_cast<int32_t>(it - P.begin());
+    };
+    for (int32_t e = 0; e < count; ++e)
+      for (int ax = 0; ax < 3; ++ax) {
+        const int32_t i0 = plane_of(ax, C3[ax][nd(0, e)]);
+        if (i0 < 0) return false;
+        for (int a = 1; a < 8; ++a)
+          if (plane_of(ax, C3[ax][nd(a, e)]) != i0 + O3[ax][a]) return false;
+        cidx[ax][e] = i0;
+      }
+    ni = static_cast<int32_t>(planes[0].size()) - 1;
+    nj = static_cast<int32_t>(planes[1].size()) - 1;
+    nk = static_cast<int32_t>(planes[2].size()) - 1;
+    for (int ax = 0; ax < 3; ++ax) h0[ax] = planes[ax][1] - planes[ax][0];
+    return true;
+  };
+  const bool uniform = uniform_lattice();
+  if (!uniform && !(allow_rect && rect_lattice())) return false;
+  for (int32_t e = 0; e < count; ++e)
     if (param[e] < 0 || param[e] >= 0xffff) return false;
-  }
   // ties: two nodes share either all their equations (equalDOF in every free DOF, the laminar
   // boundary of a soil column: the same equation triple) or none; tied[n]: another node holds n's
   // equations. Partial ties go to the general path.
@@ -103,9 +154,21 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
   const int32_t bi = (ni + TX - 1) / TX, bj = (nj + TY - 1) / TY, bk = (nk + TZ - 1) / TZ;
   L = Lattice{};
   L.tz = tile_z;
-  L.hx = hx;
-  L.hy = hy;
-  L.hz = hz;
+  L.hx = h0[0];
+  L.hy = h0[1];
+  L.hz = h0[2];
+  L.rect = !uniform;
+  if (L.rect) {
+    L.px = planes[0];
+    L.py = planes[1];
+    L.pz = planes[2];
+    L.ecell.resize(static_cast<size_t>(count) * 3);
+    for (int32_t e = 0; e < count; ++e) {
+      L.ecell[3 * static_cast<size_t>(e)] = ci[e];
+      L.ecell[3 * static_cast<size_t>(e) + 1] = cj[e];
+      L.ecell[3 * static_cast<size_t>(e) + 2] = ck[e];
+    }
+  }
   std::vector<int32_t> order;
   order.reserve(static_cast<size_t>(bi) * bj * bk);
   if (xcd_order) {
@@ -142,6 +205,9 @@ bool build_lattice(int32_t count, int32_t padded, const int32_t* node, const int
     h.nx = nx;
     h.ny = ny;
     h.nz = nz;
+    h.ox = i0;
+    h.oy = j0;
+    h.oz = k0;
     // every tile owns a full block of 81 (TZ + 1) words (node (x, y, z) at x + 9 (y + 9 z), also
     // in a tile cut by the lattice's end: its missing nodes have no word) and 64 TZ cell
     // parameters in full-tile order (x + 8 (y + 8 z)), so a workgroup finds its words and cells

@@ -4,6 +4,10 @@
                                                              distorted N^3 layered box, FP64 and FP32 applies
     python femorax_amd/profile_kernels.py pimy SIDE REPS   stdBrick<PressureIndependMultiYield> force at a yielding
                                                              state (pimy_batch_kernel) on the SIDE^3 reference column
+    python femorax_amd/profile_kernels.py ssp N REPS       SSPbrick operator on the N^3 layered box: the lattice path
+                                                             (hex8_box_kernel<T, 3..5>) and the general path
+    python femorax_amd/profile_kernels.py rect N REPS      stdBrick operator on the N^3 layered box graded in z
+                                                             (rectilinear lattice, hex8_box_kernel<T, 6..8>)
 """
 import os
 import sys
@@ -62,8 +66,42 @@ def pimy(side, reps):
     ctx.close()
 
 
+def ssp(n, reps):
+    model = S.config_c2(n=n)
+    mesh = core.Mesh(model)
+    params = [dict(p, element="SSPbrick") for p in mesh.params_for(model)]
+    ctx = core.Context(0)
+    x = np.sin(0.001 * np.arange(mesh.neq))
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, mesh.neq)
+    for general in (False, True):
+        batch = core.Batch(ctx, mesh, params, element="SSPbrick", general=general)
+        ms = batch.time_apply(dx, dy, reps, 64) / reps
+        print(f"SSPbrick {'general' if general else 'lattice'} path, {mesh.nelem} elements: {ms * 1e3:.1f} us per "
+              f"apply", flush=True)
+        batch.close()
+    ctx.close()
+
+
+def rect(n, reps):
+    model = S.config_c2(n=n)
+    z = model["xyz"][:, 2]
+    lo, hi = z.min(), z.max()
+    model["xyz"][:, 2] = lo + (hi - lo) * (1.0 - (1.0 - (z - lo) / (hi - lo)) ** 1.5)  # finer toward the top
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=True)
+    info = batch.info()
+    assert info["lattice"] == 1, info
+    x = np.sin(0.001 * np.arange(mesh.neq))
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, mesh.neq)
+    ms = batch.time_apply(dx, dy, reps, 64) / reps
+    print(f"rectilinear lattice {mesh.nelem} elements: {ms * 1e3:.1f} us per apply", flush=True)
+    batch.close()
+    ctx.close()
+
+
 if __name__ == "__main__":
     what = sys.argv[1]
-    size = int(sys.argv[2]) if len(sys.argv) > 2 else (128 if what == "general" else 40)
+    size = int(sys.argv[2]) if len(sys.argv) > 2 else (40 if what == "pimy" else 128)
     reps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
-    {"general": general, "pimy": pimy}[what](size, reps)
+    {"general": general, "pimy": pimy, "ssp": ssp, "rect": rect}[what](size, reps)
