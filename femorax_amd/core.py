@@ -233,6 +233,8 @@ SIGNATURES = [
     ("fx_dot", C.c_int, [vp, i64, vp, vp, C.POINTER(dbl)]),
     ("fx_dot_static", C.c_int, [vp, i64, vp, vp, vp]),
     ("fx_cg", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
+    ("fx_cg_many", C.c_int, [C.c_int32, C.POINTER(vp), C.POINTER(Coeffs), C.POINTER(SolveOptions), C.POINTER(vp),
+                             C.POINTER(vp), C.POINTER(SolveResult)]),
     ("fx_cg_mixed", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), dbl, vp, vp, C.POINTER(SolveResult)]),
     ("fx_gmres", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
     ("fx_krylov", C.c_int, [vp, C.POINTER(Coeffs), C.POINTER(SolveOptions), vp, vp, C.POINTER(SolveResult)]),
@@ -471,6 +473,20 @@ def elastic_array(params):
 
 def coeffs(stiffness=1.0, damping=0.0, mass=0.0, initial=False):
     return Coeffs(stiffness, damping, mass, int(initial))
+
+
+def cg_many(ctx, batches, bs, xs, rtol=1e-8, atol=0.0, max_it=10000, k=None):
+    """fx_cg_many: one CG + Jacobi solve per batch, advanced together (batch i solves A_i x_i = b_i);
+    the results as fx_cg gives them, in order."""
+    n = len(batches)
+    vp = C.c_void_p
+    hs = (vp * max(1, n))(*[b.h.value for b in batches])
+    rb = (vp * max(1, n))(*[b.ptr for b in bs])
+    rx = (vp * max(1, n))(*[x.ptr for x in xs])
+    res = (SolveResult * max(1, n))()
+    o = SolveOptions(rtol, atol, max_it)
+    _check(lib().fx_cg_many(n, hs, C.byref(k or coeffs()), C.byref(o), rb, rx, res), ctx)
+    return [res[i].as_dict() for i in range(n)]
 
 
 class Batch:

@@ -250,7 +250,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev,
                   b->words_ref, b->shared_word_ref, b->tied_word_ref, b->orphans_ref, b->ssp_elem, b->ssp_box,
-                  b->kx, b->ky, b->kz};
+                  b->kx, b->ky, b->kz, b->cg_state, b->cg_red};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);
@@ -430,6 +430,8 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
   };
   ok(dev_upload(ctx, b->params, b->host_params.data(), b->host_params.size(), by)) &&
       ok(dev_zeros(ctx, b->work, static_cast<size_t>(8) * std::max(1, d->neq), by)) &&
+      ok(dev_zeros(ctx, b->cg_state, 32, by)) &&
+      ok(dev_zeros(ctx, b->cg_red, fx::cg_red_doubles(std::max(1, d->neq)), by)) &&
       ok(b->has_f32 ? dev_zeros(ctx, b->work32, static_cast<size_t>(9) * std::max(1, d->neq), by) : FX_OK) &&
       ok(orphans.empty() ? FX_OK : dev_upload(ctx, b->orphans, orphans.data(), orphans.size(), by));
   b->norphans = static_cast<int32_t>(orphans.size());
@@ -998,6 +1000,25 @@ int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const do
           fx_solve_result* res) {
   if (!b || !k || !opt || !rhs || !x || !res) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_cg: null argument");
   return fx::cg64(b, *k, *opt, rhs, x, *res);
+}
+
+int fx_cg_many(int32_t n, fx_batch* const* batches, const fx_coeffs* k, const fx_solve_options* opt,
+               const double* const* rhs, double* const* x, fx_solve_result* res) {
+  if (n < 0 || (n > 0 && (!batches || !k || !opt || !rhs || !x || !res)))
+    return fail(nullptr, FX_EINVAL, "fx_cg_many: null argument");
+  if (n == 0) return FX_OK;
+  fx_ctx* ctx = batches[0] ? batches[0]->ctx : nullptr;
+  for (int32_t i = 0; i < n; ++i) {
+    fx_batch* b = batches[i];
+    if (!b || !rhs[i] || !x[i]) return fail(ctx, FX_EINVAL, "fx_cg_many: null batch or vector");
+    if (b->ctx != ctx) return fail(ctx, FX_EINVAL, "fx_cg_many: the batches belong to different contexts");
+    if (!b->cg_state || !b->cg_red)
+      return fail(ctx, FX_EUNSUP, "fx_cg_many: elastic batches (fx_batch_create) only");
+    for (int32_t j = 0; j < i; ++j)
+      if (batches[j] == b) return fail(ctx, FX_EINVAL, "fx_cg_many: a batch appears twice (one solve per batch)");
+  }
+  FX_HIP(ctx, hipSetDevice(ctx->device));
+  return fx::cg64_many(n, batches, *k, *opt, rhs, x, res);
 }
 
 int fx_cg_mixed(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, double inner_rtol, const double* rhs,

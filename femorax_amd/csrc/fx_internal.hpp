@@ -231,6 +231,9 @@ struct fx_batch {
   fx::DevParam* params = nullptr;
   // SSPbrick batches (FX_BATCH_SSPBRICK): the element constants ([kSspConsts][padded], the general
   // path and the per-element forms) or the lattice's per-set modal constants ([nparam][kSspBox])
+  // a CG solve's own state block and reduction scratch (fx_cg_many: several solves in flight)
+  double* cg_state = nullptr;  // a CgState (fx_solver.hip)
+  double* cg_red = nullptr;
   bool ssp = false;
   double* ssp_elem = nullptr;
   double* ssp_box = nullptr;

@@ -90,6 +90,9 @@ int krylov64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, 
 int krylov64(fx_batch* b, int method, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
              fx_solve_result& r);
 // CG + Jacobi in the solver numbering; uses the batch's work vectors 0-5.
+int cg64_many(int32_t nb, fx_batch* const* bs, const fx_coeffs& k, const fx_solve_options& o,
+              const double* const* rhs, double* const* x, fx_solve_result* res);
+size_t cg_red_doubles(int64_t n);
 int cg64_solver(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* bi, double* xi,
                 fx_solve_result& r);
 int cg_mixed(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, double inner_rtol, const double* rhs,

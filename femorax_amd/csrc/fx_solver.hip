@@ -393,7 +393,7 @@ void release_cg_graphs(fx_ctx* ctx, const void* owner) {
 namespace {
 
 constexpr int kGraphIters = 8;     // CG iterations per captured graph
-constexpr size_t kMaxGraphs = 16;  // per context
+constexpr size_t kMaxGraphs = 64;  // per context (a serving worker keeps a few dozen block sizes)
 
 struct CgGraphKey {
   const void* owner;
@@ -436,19 +436,14 @@ hipGraphExec_t cg_graph(fx_ctx* ctx, const CgGraphKey& key, Body body) {
   return exec;
 }
 
-// The CG iteration on the device for vector type T. apply(x, y) computes y = A x for the
-// operator of `owner` with coefficients k. The iterations run in groups between host polls of
-// the device stop flag (4, 8, ..., 64 iterations); iterations after the stop are no-ops. With
-// graphs on, each group replays a captured graph of kGraphIters iterations (the same kernels
-// and arguments, so the same bits as launching them).
+// The CG setup on the device for vector type T: ||b||, r = b - A x, z = inv r, p = z, r.z; the
+// state block st and the reduction scratch red (2 chunks(n) + levels) are the solve's own.
 template <typename T, typename Apply>
-int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
-              CgState* st, double rtol, double atol, int max_it, CgState& host, const fx_batch* owner = nullptr,
-              const fx_coeffs* kc = nullptr) {
-  if (int rc = ensure_red(ctx, n)) return rc;
+int cg_setup(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
+             CgState* st, double* red, double rtol, double atol, int max_it) {
   hipStream_t s = ctx->stream;
-  double* part = ctx->red;
-  double* tmp = ctx->red + 2 * chunks(n);
+  double* part = red;
+  double* tmp = red + 2 * chunks(n);
   const dim3 g = grid_of(n), cg_chunks(static_cast<unsigned>(chunks(n)));
   CgState init{};
   init.max_it = max_it;
@@ -473,37 +468,70 @@ int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* 
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kRz0), rtol, atol, static_cast<double*>(nullptr));
   FX_HIP(ctx, hipGetLastError());
-  int launched = 0, poll = 4;
+  return FX_OK;
+}
+
+// One CG iteration (cg.cpp:44-69): A p, p.Ap, the fused x, r, z update with r.r and r.z, the
+// direction; a no-op once the state's stop flag is set.
+template <typename T, typename Apply>
+int cg_iteration(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, T* x, T* r, T* z, T* p, T* ap, CgState* st,
+                 double* red, double rtol, double atol) {
+  hipStream_t s = ctx->stream;
+  double* part = red;
+  double* tmp = red + 2 * chunks(n);
+  const dim3 g = grid_of(n), cg_chunks(static_cast<unsigned>(chunks(n)));
+  const T* nt = nullptr;
   const int32_t* done = &st->done;
-  auto body = [&]() -> int {
-    if (int rc = apply(p, ap, done)) return rc;
-    hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
-                       static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
-    hipLaunchKernelGGL(update_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, st, x, r, z, p, ap, inv, part);
-    hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
-                       static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
-    hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
-    return FX_OK;
-  };
-  hipGraphExec_t exec = nullptr;
-  if (ctx->use_graphs && owner && kc && max_it > 0) {
-    CgGraphKey key{};
-    key.owner = owner;
-    key.n = n;
-    key.tsize = static_cast<int32_t>(sizeof(T));
-    key.initial = kc->initial;
-    key.ks = kc->stiffness;
-    key.kd = kc->damping;
-    key.km = kc->mass;
-    key.rtol = rtol;
-    key.atol = atol;
-    const void* ptrs[10] = {inv, b, x, r, z, p, ap, st, part, tmp};
-    for (int i = 0; i < 10; ++i) key.ptr[i] = ptrs[i];
-    key.state[0] = owner->state_c;
-    key.state[1] = owner->state_t;
-    exec = cg_graph(ctx, key, body);
-  }
+  if (int rc = apply(p, ap, done)) return rc;
+  hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+                     static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
+  hipLaunchKernelGGL(update_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, st, x, r, z, p, ap, inv, part);
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
+                     static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
+  hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
+  return FX_OK;
+}
+
+// The captured graph of a solve's iterations (null: graphs off, or the capture failed).
+template <typename T, typename Body>
+hipGraphExec_t cg_solve_graph(fx_ctx* ctx, int64_t n, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
+                              CgState* st, double* red, double rtol, double atol, int max_it, const fx_batch* owner,
+                              const fx_coeffs* kc, Body body) {
+  if (!ctx->use_graphs || !owner || !kc || max_it <= 0) return nullptr;
+  CgGraphKey key{};
+  key.owner = owner;
+  key.n = n;
+  key.tsize = static_cast<int32_t>(sizeof(T));
+  key.initial = kc->initial;
+  key.ks = kc->stiffness;
+  key.kd = kc->damping;
+  key.km = kc->mass;
+  key.rtol = rtol;
+  key.atol = atol;
+  const void* ptrs[10] = {inv, b, x, r, z, p, ap, st, red, red + 2 * chunks(n)};
+  for (int i = 0; i < 10; ++i) key.ptr[i] = ptrs[i];
+  key.state[0] = owner->state_c;
+  key.state[1] = owner->state_t;
+  return cg_graph(ctx, key, body);
+}
+
+// The CG iteration on the device for vector type T. apply(x, y) computes y = A x for the
+// operator of `owner` with coefficients k. The iterations run in groups between host polls of
+// the device stop flag (4, 8, ..., 64 iterations); iterations after the stop are no-ops. With
+// graphs on, each group replays a captured graph of kGraphIters iterations (the same kernels
+// and arguments, so the same bits as launching them).
+template <typename T, typename Apply>
+int cg_device(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x, T* r, T* z, T* p, T* ap,
+              CgState* st, double rtol, double atol, int max_it, CgState& host, const fx_batch* owner = nullptr,
+              const fx_coeffs* kc = nullptr) {
+  if (int rc = ensure_red(ctx, n)) return rc;
+  double* red = ctx->red;
+  hipStream_t s = ctx->stream;
+  if (int rc = cg_setup<T>(ctx, n, apply, inv, b, x, r, z, p, ap, st, red, rtol, atol, max_it)) return rc;
+  auto body = [&]() -> int { return cg_iteration<T>(ctx, n, apply, inv, x, r, z, p, ap, st, red, rtol, atol); };
+  hipGraphExec_t exec = cg_solve_graph<T>(ctx, n, inv, b, x, r, z, p, ap, st, red, rtol, atol, max_it, owner, kc, body);
+  int launched = 0, poll = 4;
   for (;;) {
     FX_HIP(ctx, hipMemcpyAsync(&host, st, sizeof(CgState), hipMemcpyDeviceToHost, s));
     FX_HIP(ctx, hipStreamSynchronize(s));
@@ -597,6 +625,115 @@ int cg64_once(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const 
 int cg64(fx_batch* b, const fx_coeffs& k, const fx_solve_options& o, const double* rhs, double* x,
          fx_solve_result& res) {
   return cg64_once(b, k, o, rhs, x, res);
+}
+
+size_t cg_red_doubles(int64_t n) { return static_cast<size_t>(2 * chunks(n) + 2 * (chunks(chunks(n)) + 2) + 16); }
+
+// Several FP64 CG solves advanced together on one stream (the serving loop's admission round): each
+// poll round launches every unfinished solve's next group of iterations (a captured graph per
+// solve with graphs on) and then reads every solve's state with one synchronisation. A solve
+// keeps its own state block and reduction scratch (the batch's), and its kernels, arguments and
+// order are those of cg64 on its own batch, so its iterates, result and x are the same bits.
+int cg64_many(int32_t nb, fx_batch* const* bs, const fx_coeffs& k, const fx_solve_options& o,
+              const double* const* rhs, double* const* x, fx_solve_result* res) {
+  if (nb <= 0) return FX_OK;
+  fx_ctx* ctx = bs[0]->ctx;
+  hipStream_t s = ctx->stream;
+  struct Solve {
+    fx_batch* b;
+    double *r, *z, *p, *ap, *inv, *d, *bi, *xi;
+    CgState* st;
+    double* red;
+    hipGraphExec_t exec;
+    int launched;
+    bool active;
+  };
+  std::vector<Solve> sv(static_cast<size_t>(nb));
+  std::vector<CgState> host(static_cast<size_t>(nb));
+  for (int32_t i = 0; i < nb; ++i) {
+    fx_batch* b = bs[i];
+    const int64_t n = b->neq;
+    Solve& v = sv[i];
+    v.b = b;
+    v.r = b->work;
+    v.z = v.r + n;
+    v.p = v.z + n;
+    v.ap = v.p + n;
+    v.inv = v.ap + n;
+    v.d = v.inv + n;
+    v.bi = v.r + 6 * n;
+    v.xi = v.r + 7 * n;
+    v.st = reinterpret_cast<CgState*>(b->cg_state);
+    v.red = b->cg_red;
+    v.launched = 0;
+    v.active = true;
+    if (int rc = to_internal<double>(b, rhs[i], v.bi)) return rc;
+    if (int rc = to_internal<double>(b, x[i], v.xi)) return rc;
+    if (int rc = jacobi_inverse(b, k, v.d, v.inv, false)) return rc;
+    auto apply = [b, &k](const double* xx, double* yy, const int32_t* dn) { return operator_apply64(b, k, xx, yy, dn); };
+    if (int rc = cg_setup<double>(ctx, n, apply, v.inv, v.bi, v.xi, v.r, v.z, v.p, v.ap, v.st, v.red,
+                                  o.relative_tolerance, o.absolute_tolerance, o.max_iterations))
+      return rc;
+    auto body = [&, b, n]() -> int {
+      return cg_iteration<double>(ctx, n, apply, v.inv, v.xi, v.r, v.z, v.p, v.ap, v.st, v.red, o.relative_tolerance,
+                                  o.absolute_tolerance);
+    };
+    v.exec = cg_solve_graph<double>(ctx, n, v.inv, v.bi, v.xi, v.r, v.z, v.p, v.ap, v.st, v.red, o.relative_tolerance,
+                                    o.absolute_tolerance, o.max_iterations, b, &k, body);
+  }
+  const int max_it = o.max_iterations;
+  for (int poll = 4, left = nb; left > 0; poll = std::min(poll * 2, 64)) {
+    for (int32_t i = 0; i < nb; ++i)
+      if (sv[i].active)
+        FX_HIP(ctx, hipMemcpyAsync(&host[i], sv[i].st, sizeof(CgState), hipMemcpyDeviceToHost, s));
+    FX_HIP(ctx, hipStreamSynchronize(s));
+    for (int32_t i = 0; i < nb; ++i) {
+      Solve& v = sv[i];
+      if (!v.active) continue;
+      const CgState& h = host[i];
+      if (h.done || v.launched >= max_it) {
+        // finished: cg64_solver's result, then cg64_once's true residual and permutation
+        fx_batch* b = v.b;
+        const int64_t n = b->neq;
+        fx_solve_result& rs = res[i];
+        rs = fx_solve_result{};
+        rs.converged = h.converged;
+        rs.iterations = h.it;
+        rs.indefinite = h.indefinite;
+        rs.rhs_norm = h.rhs_norm;
+        rs.initial_residual = h.init_res;
+        rs.final_residual = h.final_res;
+        if (int rc = operator_apply64(b, k, v.xi, v.ap)) return rc;
+        hipLaunchKernelGGL(residual_kernel<double>, grid_of(n), dim3(256), 0, s, n, v.bi, v.ap, v.r);
+        double rr = 0;
+        if (int rc = dot64(ctx, n, v.r, v.r, &rr)) return rc;
+        rs.true_residual = std::sqrt(rr);
+        if (int rc = to_external<double>(b, v.xi, x[i])) return rc;
+        v.active = false;
+        --left;
+        continue;
+      }
+      const int batch = std::min(poll, max_it - v.launched);
+      if (v.exec) {
+        const int graphs = (batch + kGraphIters - 1) / kGraphIters;
+        for (int g = 0; g < graphs; ++g) FX_HIP(ctx, hipGraphLaunch(v.exec, s));
+        v.launched += graphs * kGraphIters;
+      } else {
+        const int64_t n = v.b->neq;
+        fx_batch* b = v.b;
+        auto apply = [b, &k](const double* xx, double* yy, const int32_t* dn) {
+          return operator_apply64(b, k, xx, yy, dn);
+        };
+        for (int it = 0; it < batch; ++it)
+          if (int rc = cg_iteration<double>(ctx, n, apply, v.inv, v.xi, v.r, v.z, v.p, v.ap, v.st, v.red,
+                                            o.relative_tolerance, o.absolute_tolerance))
+            return rc;
+        v.launched += batch;
+      }
+      FX_HIP(ctx, hipGetLastError());
+    }
+  }
+  return FX_OK;
 }
 
 // GMRES (method 1) or CG with the GMRES fallback (method 2) at the C ABI: permutations around

@@ -284,6 +284,14 @@ int fx_cg(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const do
 /* FP32 inner CG (FP32 operator, FP32 vectors, FP64 dot products) inside FP64 iterative
  * refinement (FP64 residual b - A x with the FP64 operator, FP64 update); same stop rule on the
  * FP64 residual. inner_rtol: each inner solve's relative tolerance. */
+/* Several CG + Jacobi solves advanced together, one per batch (no counterpart in the reference,
+ * whose solvers take one system at a time; for serving loops that admit many small systems per
+ * round): each poll round launches every unfinished solve's next iterations on the context's
+ * stream (captured graphs with FX_OPT_GRAPHS) and reads every solve's state with one
+ * synchronisation. Solve i gives the bits fx_cg(batches[i], k, opt, rhs[i], x[i], &res[i]) gives.
+ * Elastic batches (fx_batch_create) of one context, each at most once. */
+int fx_cg_many(int32_t n, fx_batch* const* batches, const fx_coeffs* k, const fx_solve_options* opt,
+               const double* const* rhs, double* const* x, fx_solve_result* res);
 /* FGMRES(30), CGS2, right Jacobi (Gmres::solve, gmres.cpp:32-123). Needs FX_BATCH_GMRES. */
 int fx_gmres(fx_batch* b, const fx_coeffs* k, const fx_solve_options* opt, const double* rhs, double* x,
              fx_solve_result* res);
