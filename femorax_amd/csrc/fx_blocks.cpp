@@ -76,6 +76,7 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
   B.elems.assign(static_cast<size_t>(nb) * kBlockElems, -1);
   B.pos.assign(static_cast<size_t>(count), -1);
   B.slot_list.assign(static_cast<size_t>(nb) * 8 * kBlockElems, 0);
+  B.lcorner.assign(static_cast<size_t>(nb) * 8 * kBlockElems, 0);
   B.ln_start.assign(static_cast<size_t>(nb) + 1, 0);
   B.lnode.clear();
   B.ln_slots.clear();
@@ -99,6 +100,7 @@ void build_blocks(int32_t count, int32_t padded, const int32_t* node, int32_t nn
         const int32_t n = node[static_cast<size_t>(a) * padded + el[i]];
         const auto ln = static_cast<uint16_t>(std::lower_bound(nodes.begin(), nodes.end(), n) - nodes.begin());
         per[ln].push_back(static_cast<uint16_t>(i * 8 + a));
+        B.lcorner[(static_cast<size_t>(b) * 8 + a) * kBlockElems + i] = ln;
       }
     uint32_t pos = 0;
     for (size_t ln = 0; ln < nodes.size(); ++ln) {

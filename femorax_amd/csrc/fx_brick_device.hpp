@@ -110,11 +110,13 @@ __host__ __device__ __forceinline__ void hadamard8t_hd(T (&g)[8]) {
 
 // x, y, z: node coordinates, v: node values (node-major, 3 per node), reference node order.
 // lam, mu: the effective D's Lame factors; rhoM: cM' rho (0: no mass term); consistent: the
-// consistent mass (else lumped, sum_gp N_j dvol).
-template <typename T>
+// consistent mass (else lumped, sum_gp N_j dvol). vl(i) returns v[i] again for the lumped mass
+// term at the end (a caller whose values sit in LDS reloads them there instead of keeping 24
+// registers live through the Gauss loop).
+template <typename T, typename VL>
 __host__ __device__ __forceinline__ void element_apply_modal(const T (&x)[8], const T (&y)[8], const T (&z)[8],
                                                              const T (&v)[24], T lam, T mu, T rhoM, bool consistent,
-                                                             T (&out)[24]) {
+                                                             T (&out)[24], VL vl) {
   T c[3][8], a[3][8], G[3][8], W[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) {
@@ -224,10 +226,17 @@ __host__ __device__ __forceinline__ void element_apply_modal(const T (&x)[8], co
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       T o = T(0.125) * G[d][q];
-      if (rhoM != T(0) && !consistent) o += rhoM * (T(0.125) * W[q]) * v[3 * n + d];
+      if (rhoM != T(0) && !consistent) o += rhoM * (T(0.125) * W[q]) * vl(3 * n + d);
       out[3 * n + d] = o;
     }
   }
+}
+
+template <typename T>
+__host__ __device__ __forceinline__ void element_apply_modal(const T (&x)[8], const T (&y)[8], const T (&z)[8],
+                                                             const T (&v)[24], T lam, T mu, T rhoM, bool consistent,
+                                                             T (&out)[24]) {
+  element_apply_modal<T>(x, y, z, v, lam, mu, rhoM, consistent, out, [&](int i) { return v[i]; });
 }
 
 }  // namespace fx

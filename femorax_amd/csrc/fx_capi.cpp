@@ -244,7 +244,8 @@ void fx_batch_destroy(fx_batch* b) {
   fx::release_cg_graphs(ctx, b);  // they read the batch's arrays
   void* ptrs[] = {b->node, b->param, b->node_eq, b->coords, b->coords32, b->params, b->blk.elems,
                   b->blk.ln_start, b->blk.lnode, b->blk.ln_part, b->blk.ln_slots, b->blk.slot_list,
-                  b->blk.sh_node, b->blk.sh_start, b->blk.pos, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
+                  b->blk.sh_node, b->blk.sh_start, b->blk.pos, b->blk.lcorner, b->blk.ln_eq, b->blk.ln_xyz,
+                  b->blk.ln_xyz32, b->ynode, b->ynode32, b->con_start, b->con_list, b->work, b->work32,
                   b->tiles, b->words, b->tile_slots, b->cell_param, b->orphans, b->shared_word, b->shared_start,
                   b->shared_meta, b->tied_word, b->tied_start, b->tie_idx, b->tie_val, b->partial64, b->partial32, b->perm, b->lumped_m, b->lumped_am, b->body, b->pimy_params,
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
@@ -516,6 +517,25 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
         ok(blk.sh_node.empty() ? FX_OK : dev_upload(ctx, b->blk.sh_node, blk.sh_node.data(), blk.sh_node.size(), by)) &&
         ok(dev_upload(ctx, b->blk.sh_start, blk.sh_start.data(), blk.sh_start.size(), by)) &&
         ok(b->ssp ? dev_upload(ctx, b->blk.pos, blk.pos.data(), blk.pos.size(), by) : FX_OK);
+    // the blocks' local nodes in block order: their equations and coordinates (loaded once per
+    // block and node, coalesced), and each element corner's local node
+    {
+      const int64_t tot = static_cast<int64_t>(blk.lnode.size());
+      std::vector<int32_t> leq(static_cast<size_t>(3 * tot));
+      std::vector<double> lxyz(static_cast<size_t>(3 * tot));
+      for (int64_t ln = 0; ln < tot; ++ln)
+        for (int dd = 0; dd < 3; ++dd) {
+          leq[dd * tot + ln] = node_eq[3 * static_cast<size_t>(blk.lnode[ln]) + dd];
+          lxyz[dd * tot + ln] = d->coords[static_cast<size_t>(dd) * N + blk.lnode[ln]];
+        }
+      std::vector<float> lxyz32(b->has_f32 ? lxyz.size() : 0);
+      for (size_t i = 0; i < lxyz32.size(); ++i) lxyz32[i] = static_cast<float>(lxyz[i]);
+      b->blk.ln_total = tot;
+      ok(dev_upload(ctx, b->blk.lcorner, blk.lcorner.data(), blk.lcorner.size(), by)) &&
+          ok(dev_upload(ctx, b->blk.ln_eq, leq.data(), leq.size(), by)) &&
+          ok(dev_upload(ctx, b->blk.ln_xyz, lxyz.data(), lxyz.size(), by)) &&
+          ok(b->has_f32 ? dev_upload(ctx, b->blk.ln_xyz32, lxyz32.data(), lxyz32.size(), by) : FX_OK);
+    }
     // the block-boundary nodes' partial sums (three values per (node, block))
     const size_t npart = static_cast<size_t>(std::max(1, blk.sh_start.back())) * 3;
     ok(dev_zeros(ctx, b->partial64, npart, by)) && ok(b->has_f32 ? dev_zeros(ctx, b->partial32, npart, by) : FX_OK);

@@ -43,8 +43,14 @@ struct Blocks {
   std::vector<uint16_t> slot_list;   // [nblocks * 8 * kBlockElems] elem_local * 8 + corner
   std::vector<int32_t> sh_node;      // nodes several blocks touch
   std::vector<int32_t> sh_start;     // [nshared + 1]: node i's partial slots, in block order
+  std::vector<uint16_t> lcorner;     // [nblocks * 8 * kBlockElems] at b * 8 * kBlockElems + corner * kBlockElems + i:
+                                     // the block-local node of element i's corner
   int32_t max_ln = 0;
 };
+
+// A block whose local nodes number at most kStageNodes stages them (coordinates, equations, x)
+// in the LDS its element slots later use: 6 values per node in the 24 * kBlockElems slots.
+constexpr int kStageNodes = 4 * kBlockElems;
 
 // Partition the batch's elements into compact blocks (recursive coordinate bisection of the
 // element centroids). A node only one block touches is written by that block; the others get a
@@ -166,6 +172,11 @@ struct DevBlocks {
   uint16_t* slot_list = nullptr;
   int32_t* sh_node = nullptr;
   int32_t* sh_start = nullptr;
+  uint16_t* lcorner = nullptr;
+  int64_t ln_total = 0;      // local nodes over all blocks
+  int32_t* ln_eq = nullptr;  // [3 * ln_total] at d * ln_total + ln: the local node's equations
+  double* ln_xyz = nullptr;  // [3 * ln_total] at d * ln_total + ln: its coordinates
+  float* ln_xyz32 = nullptr;
 };
 
 struct DevParam {  // per parameter set, device

@@ -21,6 +21,11 @@ struct KernelArgs {
   const uint32_t* ln_slots;
   const uint16_t* slot_list;
   const int32_t* ln_part;
+  const uint16_t* lcorner;  // block-local node of each element corner (block order)
+  int64_t ln_total;
+  const int32_t* ln_eq;     // local nodes' equations and coordinates, [d * ln_total + ln]
+  const double* ln_xyz64;
+  const float* ln_xyz32;
   const T* x;
   T* y;
   T* ynode;

@@ -247,18 +247,87 @@ __device__ __forceinline__ T load_coord(const double* c64, const float* c32, siz
   else return c32[i];
 }
 
-// The block kernel: y (or the node-space ynode) += K_eff x over one colour of blocks.
+// The block kernel: y (or the node-space ynode) += K_eff x over the block's elements. A block
+// whose local nodes fit (kStageNodes) first loads them, in block order, into the LDS its slots
+// use afterwards: each node's coordinates, equations and x once (coalesced, where an element-wise
+// gather would fetch them for each of up to eight elements); an element then reads its corners
+// there. Larger blocks gather per element. The values are the same either way, so are the bits.
+// Registers are held to 256, two waves per SIMD (FP64 element work needs a few more otherwise,
+// which leaves one wave and no other to hide the loads behind).
 template <typename T, bool DIAG, bool SSP>
-__global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a, T* __restrict__ partial) {
+__global__ __launch_bounds__(kBlockElems, 2) void hex8_block_kernel(KernelArgs<T> a, T* __restrict__ partial) {
   extern __shared__ unsigned char smem_raw[];
   T* slots = reinterpret_cast<T*>(smem_raw);  // [24][kBlockElems]
   if (a.done && *a.done) return;
   const int b = blockIdx.x;
   const int t = threadIdx.x;
   const int e = a.elems[static_cast<size_t>(b) * kBlockElems + t];
-  if (e >= 0) {
-    T X[8], Y[8], Z[8], V[24], out[24];
-    int eqe[24];
+  const int ln0 = a.ln_start[b], ln1 = a.ln_start[b + 1];
+  const size_t tot = static_cast<size_t>(a.ln_total);
+  T X[8], Y[8], Z[8], V[24], out[24];
+  int eqe[24];
+  // the element's operator values (or diagonal) from its corners; vl(i) reloads V[i]
+  auto element = [&](auto vl) {
+    const DevParam& p = a.params[a.param[e]];
+    double fD, cM;
+    operator_factors(p, a.k, fD, cM);
+    ElemCoef<T> c;
+    c.lam = static_cast<T>(fD * p.lam);
+    c.mu = static_cast<T>(fD * p.mu);
+    c.rhoM = static_cast<T>(cM * p.rho);
+    c.consistent = p.mass_type == 0;
+    if constexpr (SSP) {
+      // the constants in block order: coalesced loads
+      const size_t at = static_cast<size_t>(b) * kBlockElems + t;
+      if constexpr (DIAG) ssp_element_diagonal<T>(a.ssp, a.ssp_stride, at, eqe, c, static_cast<T>(fD), out);
+      else ssp_element_apply<T>(a.ssp, a.ssp_stride, at, V, c, static_cast<T>(fD), out);
+    } else {
+      if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
+      else element_apply_modal<T>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out, vl);
+    }
+  };
+  if (ln1 - ln0 <= kStageNodes) {  // uniform over the workgroup
+    // staging: x, y, z, then the three x values (or, for the diagonal, equations) per node
+    T* sxyz = slots;
+    T* sv = slots + 3 * kStageNodes;
+    int* seq = reinterpret_cast<int*>(sv);
+    for (int i = t; i < ln1 - ln0; i += kBlockElems) {
+      const size_t ln = static_cast<size_t>(ln0) + i;
+      if constexpr (!SSP)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if constexpr (sizeof(T) == 8) sxyz[d * kStageNodes + i] = a.ln_xyz64[d * tot + ln];
+          else sxyz[d * kStageNodes + i] = a.ln_xyz32[d * tot + ln];
+        }
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        const int q = a.ln_eq[d * tot + ln];
+        if constexpr (DIAG) seq[d * kStageNodes + i] = q;
+        else sv[d * kStageNodes + i] = q >= 0 ? a.x[q] : T(0);
+      }
+    }
+    __syncthreads();
+    if (e >= 0) {
+      const uint16_t* lc = a.lcorner + static_cast<size_t>(b) * 8 * kBlockElems + t;
+      int li[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        li[k] = lc[k * kBlockElems];
+        if constexpr (!SSP) {
+          X[k] = sxyz[li[k]];
+          Y[k] = sxyz[kStageNodes + li[k]];
+          Z[k] = sxyz[2 * kStageNodes + li[k]];
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if constexpr (DIAG) eqe[3 * k + d] = seq[d * kStageNodes + li[k]];
+          else V[3 * k + d] = sv[d * kStageNodes + li[k]];
+        }
+      }
+      element([&](int i) { return sv[(i % 3) * kStageNodes + li[i / 3]]; });
+    }
+    __syncthreads();  // the staging area becomes the slots
+  } else if (e >= 0) {
     const size_t P = static_cast<size_t>(a.padded);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -275,28 +344,13 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
         if constexpr (!DIAG) V[3 * k + d] = q >= 0 ? a.x[q] : T(0);
       }
     }
-    const DevParam& p = a.params[a.param[e]];
-    double fD, cM;
-    operator_factors(p, a.k, fD, cM);
-    ElemCoef<T> c;
-    c.lam = static_cast<T>(fD * p.lam);
-    c.mu = static_cast<T>(fD * p.mu);
-    c.rhoM = static_cast<T>(cM * p.rho);
-    c.consistent = p.mass_type == 0;
-    if constexpr (SSP) {
-      // the constants in block order: coalesced loads
-      const size_t at = static_cast<size_t>(b) * kBlockElems + t;
-      if constexpr (DIAG) ssp_element_diagonal<T>(a.ssp, a.ssp_stride, at, eqe, c, static_cast<T>(fD), out);
-      else ssp_element_apply<T>(a.ssp, a.ssp_stride, at, V, c, static_cast<T>(fD), out);
-    } else {
-      if constexpr (DIAG) element_diagonal(X, Y, Z, eqe, c, out);
-      else element_apply_modal<T>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out);
-    }
+    element([&](int i) { return V[i]; });
+  }
+  if (e >= 0) {
 #pragma unroll
     for (int k = 0; k < 24; ++k) slots[k * kBlockElems + t] = out[k];
   }
   __syncthreads();
-  const int ln0 = a.ln_start[b], ln1 = a.ln_start[b + 1];
   const uint16_t* list = a.slot_list + static_cast<size_t>(b) * 8 * kBlockElems;
   for (int ln = ln0 + t; ln < ln1; ln += kBlockElems) {
     const uint32_t sc = a.ln_slots[ln];
@@ -324,14 +378,14 @@ __global__ __launch_bounds__(kBlockElems) void hex8_block_kernel(KernelArgs<T> a
       for (int d = 0; d < 3; ++d) partial[3 * static_cast<size_t>(part) + d] = acc[d];
       continue;
     }
-    const int n = static_cast<int>(a.lnode[ln]);
     if (a.ynode) {  // tied mesh: node space, contracted afterwards
+      const int n = static_cast<int>(a.lnode[ln]);
 #pragma unroll
       for (int d = 0; d < 3; ++d) a.ynode[3 * static_cast<size_t>(n) + d] = acc[d];
     } else {
 #pragma unroll
       for (int d = 0; d < 3; ++d) {
-        const int q = a.node_eq[3 * n + d];
+        const int q = a.ln_eq[d * tot + ln];
         if (q < 0) continue;
         a.y[q] = a.accumulate ? a.y[q] + acc[d] : acc[d];
       }
@@ -476,6 +530,11 @@ KernelArgs<T> operator_args(fx_batch* b, const fx_coeffs& k, const T* x, T* y, i
   a.ln_slots = b->blk.ln_slots;
   a.slot_list = b->blk.slot_list;
   a.ln_part = b->blk.ln_part;
+  a.lcorner = b->blk.lcorner;
+  a.ln_total = b->blk.ln_total;
+  a.ln_eq = b->blk.ln_eq;
+  a.ln_xyz64 = b->blk.ln_xyz;
+  a.ln_xyz32 = b->blk.ln_xyz32;
   a.x = x;
   a.y = y;
   a.accumulate = accumulate;

@@ -196,16 +196,31 @@ __device__ inline bool stiffness_damping(const PimyParams& p) {
   return p.betaK != 0.0 || p.betaKinit != 0.0 || p.betaKcomm != 0.0;
 }
 
+// N_j at Gauss point gp as gauss_shape forms it (the same operations on the same constants, so
+// the same bits): the summing lane forms the body terms dvol b N_j from each point's dvol.
+__device__ __forceinline__ double gauss_N(int gp, int k) {
+  const double r3 = 0.57735026918962576451;
+  const double xi = (gp & 4) ? r3 : -r3, eta = (gp & 2) ? r3 : -r3, zeta = (gp & 1) ? r3 : -r3;
+  const double fx = kSx[k] > 0 ? 1.0 + xi : 1.0 - xi;
+  const double fy = kSy[k] > 0 ? 1.0 + eta : 1.0 - eta;
+  const double fz = kSz[k] > 0 ? 1.0 + zeta : 1.0 - zeta;
+  const double cxy = 0.125 * fx * fy;
+  return cxy * fz;
+}
+
 // The internal force: the update at u at every point (STORE also keeps the tangents; REPLAY takes
 // each point's stress and tangent from the compacted pass instead of updating; DAMP adds the
-// Rayleigh damping force at the velocities a.v, std_brick.hpp:409-464). One thread per point;
-// DAMP blocks hold half the points (their LDS keeps 9 values per node).
+// Rayleigh damping force at the velocities a.v, std_brick.hpp:409-464). One thread per point.
+// A point leaves in LDS its node forces (and damping terms) and its dvol, 25 (49) values: the
+// element's first lane forms the body terms itself. Without DAMP the registers are held to 256,
+// two waves per SIMD (the return map's history loads then overlap another wave's arithmetic); the
+// damping terms need more, one wave. DAMP blocks hold half the points.
 template <int MODE, bool STORE, bool REPLAY = false, bool DAMP = false>
-__global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchArgs a) {
+__global__ __launch_bounds__(kPointsPerBlock, DAMP ? 1 : 2) void pimy_batch_kernel(PimyBatchArgs a) {
   static_assert(MODE == 0, "the operator and its diagonal run in pimy_operator_kernel");
-  constexpr int PTS = DAMP ? kPointsPerBlock / 2 : kPointsPerBlock, W = DAMP ? 9 : 6;
-  // per point: 8 nodes x (3 first terms, 3 second terms [, 3 damping terms])
-  __shared__ double part[PTS][8 * W];
+  constexpr int PTS = DAMP ? kPointsPerBlock / 2 : kPointsPerBlock, W = DAMP ? 6 : 3;
+  // per point: 8 nodes x (3 force terms [, 3 damping terms]), then dvol
+  __shared__ double part[PTS][8 * W + 1];
   const int t = threadIdx.x;  // the launch has PTS threads
   const int e = blockIdx.x * (PTS / 8) + t / 8, gp = t % 8;
   const bool live = e < a.count;
@@ -238,11 +253,9 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
       for (int j = 0; j < 8; ++j) {
         double rj[3];
         node_force(s, sig, j, rj);
-        for (int q = 0; q < 3; ++q) {
-          mine[W * j + q] = rj[q];
-          mine[W * j + 3 + q] = s.dvol * p.b[q] * s.N[j];  // body
-        }
+        for (int q = 0; q < 3; ++q) mine[W * j + q] = rj[q];
       }
+      mine[8 * W] = s.dvol;  // the body terms s.dvol b N_j are formed by the summing lane
       if constexpr (DAMP) {
         // sigma_d = (betaK D + betaKinit D_0 + betaKcomm D_c) B v; the node terms B_j^T sigma_d dvol
         // + alphaM N_j dvol rho v (point_update_shape :420-442, point_node_contribution :448-464)
@@ -275,7 +288,7 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
             const double temp = s.N[j] * s.dvol * p.rho;
             for (int q = 0; q < 3; ++q) dj[q] += p.alphaM * temp * (p.mass_type == 0 ? vgp[q] : vel[3 * j + q]);
           }
-          for (int q = 0; q < 3; ++q) mine[W * j + 6 + q] = dj[q];
+          for (int q = 0; q < 3; ++q) mine[W * j + 3 + q] = dj[q];
         }
       }
     }
@@ -287,12 +300,15 @@ __global__ __launch_bounds__(kPointsPerBlock) void pimy_batch_kernel(PimyBatchAr
   const PimyParams& p = a.params[a.param[e]];
   for (int g = 0; g < 8; ++g) {
     const double* c = part[t + g];
-    for (int j = 0; j < 8; ++j)
+    const double dvol = c[8 * W];
+    for (int j = 0; j < 8; ++j) {
+      const double Nj = gauss_N(g, j);
       for (int q = 0; q < 3; ++q) {
         out[3 * j + q] += c[W * j + q];
-        out[3 * j + q] -= c[W * j + 3 + q];
-        if constexpr (DAMP) fdamp[3 * j + q] += c[W * j + 6 + q];
+        out[3 * j + q] -= dvol * p.b[q] * Nj;  // body
+        if constexpr (DAMP) fdamp[3 * j + q] += c[W * j + 3 + q];
       }
+    }
   }
   if constexpr (DAMP)  // fe = resid + fdamp where the element damps
     if (stiffness_damping(p) || p.alphaM != 0.0)

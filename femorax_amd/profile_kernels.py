@@ -4,6 +4,10 @@
                                                              distorted N^3 layered box, FP64 and FP32 applies
     python femorax_amd/profile_kernels.py pimy SIDE REPS   stdBrick<PressureIndependMultiYield> force at a yielding
                                                              state (pimy_batch_kernel) on the SIDE^3 reference column
+    python femorax_amd/profile_kernels.py pimy_time SIDE REPS   wall time per force call (every kernel of the call) on
+                                                             the same column, elastic and yielding, fused and compacted
+    python femorax_amd/profile_kernels.py general_time N REPS   wall time per apply (hex8_block_kernel and the
+                                                             shared-node sum) on the distorted N^3 box, FP64 and FP32
     python femorax_amd/profile_kernels.py ssp N REPS       SSPbrick operator on the N^3 layered box: the lattice path
                                                              (hex8_box_kernel<T, 3..5>) and the general path
     python femorax_amd/profile_kernels.py rect N REPS      stdBrick operator on the N^3 layered box graded in z
@@ -44,13 +48,8 @@ def pimy(side, reps):
     mesh = core.Mesh(model)
     ctx = core.Context(0)
     keep = []
-    mats, rays = [], []
-    for p in model["params"]:
-        G = p["E"] / (2.0 * (1.0 + p["nu"]))
-        K = p["E"] / (3.0 * (1.0 - 2.0 * p["nu"]))
-        mats.append(core.pimy_material(p["rho"], G, K, 30.0, 0.1, 0.0, 100.0, 0.0, nsurf=20, b=p["b"], mass_type=0,
-                                       keep=keep))
-        rays.append(p["rayleigh"])
+    mats = _pimy_materials(model, keep)
+    rays = [p["rayleigh"] for p in model["params"]]
     pb = core.Batch.pimy(ctx, mesh, mats, rayleigh=rays)
     pb.set_stage(1)
     z = np.linspace(0.0, 1.0, mesh.neq)
@@ -63,6 +62,64 @@ def pimy(side, reps):
     print(f"pimy force on {mesh.nelem} elements, {int(np.sum(active > 0))} of {active.size} points past the "
           f"first surface", flush=True)
     pb.close()
+    ctx.close()
+
+
+def _pimy_materials(model, keep):
+    mats = []
+    for p in model["params"]:
+        G = p["E"] / (2.0 * (1.0 + p["nu"]))
+        K = p["E"] / (3.0 * (1.0 - 2.0 * p["nu"]))
+        mats.append(core.pimy_material(p["rho"], G, K, 30.0, 0.1, 0.0, 100.0, 0.0, nsurf=20, b=p["b"], mass_type=0,
+                                       keep=keep))
+    return mats
+
+
+def pimy_time(side, reps):
+    import time
+    model = S.reference_column(side)
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    keep = []
+    mats = _pimy_materials(model, keep)
+    z = np.linspace(0.0, 1.0, mesh.neq)
+    for compact in (False, True):
+        pb = core.Batch.pimy(ctx, mesh, mats, compact=compact)
+        pb.set_stage(1)
+        du = core.DeviceArray.from_host(ctx, np.zeros(mesh.neq))
+        df = core.DeviceArray(ctx, mesh.neq)
+        for name, amp in (("elastic", 1e-7), ("yielding", 2e-3)):
+            du.upload(amp * np.sin(40.0 * z))
+            pb.force(du, df)
+            ctx.sync()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                pb.force(du, df)
+            ctx.sync()
+            us = (time.perf_counter() - t0) / reps * 1e6
+            print(f"pimy force, {mesh.nelem} elements, {'compacted' if compact else 'fused'}, {name}: {us:.1f} us",
+                  flush=True)
+        pb.close()
+    ctx.close()
+
+
+def general_time(n, reps):
+    model = S.distorted_box(n)
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model), fp32=True)
+    assert batch.info()["lattice"] == 0
+    x = np.sin(0.001 * np.arange(mesh.neq))
+    dx, dy = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, mesh.neq)
+    batch.time_apply(dx, dy, 3, 64)
+    ms = min(batch.time_apply(dx, dy, reps, 64) / reps for _ in range(3))
+    print(f"general apply FP64, {mesh.nelem} elements: {ms * 1e3:.1f} us", flush=True)
+    dx32 = core.DeviceArray.from_host(ctx, x.astype(np.float32), dtype=np.float32)
+    dy32 = core.DeviceArray(ctx, mesh.neq, dtype=np.float32)
+    batch.time_apply(dx32, dy32, 3, 32)
+    ms = min(batch.time_apply(dx32, dy32, reps, 32) / reps for _ in range(3))
+    print(f"general apply FP32, {mesh.nelem} elements: {ms * 1e3:.1f} us", flush=True)
+    batch.close()
     ctx.close()
 
 
@@ -102,6 +159,7 @@ def rect(n, reps):
 
 if __name__ == "__main__":
     what = sys.argv[1]
-    size = int(sys.argv[2]) if len(sys.argv) > 2 else (40 if what == "pimy" else 128)
+    size = int(sys.argv[2]) if len(sys.argv) > 2 else (40 if what.startswith("pimy") else 128)
     reps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
-    {"general": general, "pimy": pimy, "ssp": ssp, "rect": rect}[what](size, reps)
+    {"general": general, "pimy": pimy, "ssp": ssp, "rect": rect, "pimy_time": pimy_time,
+     "general_time": general_time}[what](size, reps)
