@@ -735,6 +735,7 @@ int fx_batch_create(fx_ctx* ctx, const fx_batch_desc* d, uint32_t flags, fx_batc
     return rc;
   }
   b->blk.nblocks = blk.nblocks;
+  b->blk.max_ln = blk.max_ln;
   b->info.count = E;
   b->info.neq = d->neq;
   b->info.nnode = N;

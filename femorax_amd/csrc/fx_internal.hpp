@@ -163,6 +163,7 @@ void tune_cache_put(const std::string& key, int depth, int variant);
 namespace fx {
 struct DevBlocks {
   int32_t nblocks = 0, nshared = 0;
+  int32_t max_ln = 0;  // the most local nodes of a block
   int32_t* elems = nullptr;
   int32_t* pos = nullptr;  // SSPbrick batches: each element's block position (its constants' index)
   int32_t* ln_start = nullptr;

@@ -251,18 +251,21 @@ __device__ __forceinline__ T load_coord(const double* c64, const float* c32, siz
 // whose local nodes fit (kStageNodes) first loads them, in block order, into the LDS its slots
 // use afterwards: each node's coordinates, equations and x once (coalesced, where an element-wise
 // gather would fetch them for each of up to eight elements); an element then reads its corners
-// there. Larger blocks gather per element. The values are the same either way, so are the bits.
-// Registers are held to 256, two waves per SIMD (FP64 element work needs a few more otherwise,
-// which leaves one wave and no other to hide the loads behind).
-template <typename T, bool DIAG, bool SSP>
-__global__ __launch_bounds__(kBlockElems, 2) void hex8_block_kernel(KernelArgs<T> a, T* __restrict__ partial) {
+// there. Larger blocks gather per element (GATHER: a second launch over the same grid, made
+// only when the batch has such blocks; each launch skips the other's blocks). The values are the
+// same either way, so are the bits. Registers are held to two waves per SIMD for FP64 (the
+// staged FP64 element work fits in 228), four for FP32.
+template <typename T, bool DIAG, bool SSP, bool GATHER = false>
+__global__ __launch_bounds__(kBlockElems, sizeof(T) == 8 ? 2 : 4) void hex8_block_kernel(KernelArgs<T> a,
+                                                                                         T* __restrict__ partial) {
   extern __shared__ unsigned char smem_raw[];
   T* slots = reinterpret_cast<T*>(smem_raw);  // [24][kBlockElems]
   if (a.done && *a.done) return;
   const int b = blockIdx.x;
   const int t = threadIdx.x;
-  const int e = a.elems[static_cast<size_t>(b) * kBlockElems + t];
   const int ln0 = a.ln_start[b], ln1 = a.ln_start[b + 1];
+  if ((ln1 - ln0 > kStageNodes) != GATHER) return;  // the other launch's block (uniform)
+  const int e = a.elems[static_cast<size_t>(b) * kBlockElems + t];
   const size_t tot = static_cast<size_t>(a.ln_total);
   T X[8], Y[8], Z[8], V[24], out[24];
   int eqe[24];
@@ -286,7 +289,7 @@ __global__ __launch_bounds__(kBlockElems, 2) void hex8_block_kernel(KernelArgs<T
       else element_apply_modal<T>(X, Y, Z, V, c.lam, c.mu, c.rhoM, c.consistent != 0, out, vl);
     }
   };
-  if (ln1 - ln0 <= kStageNodes) {  // uniform over the workgroup
+  if constexpr (!GATHER) {
     // staging: x, y, z, then the three x values (or, for the diagonal, equations) per node
     T* sxyz = slots;
     T* sv = slots + 3 * kStageNodes;
@@ -327,7 +330,7 @@ __global__ __launch_bounds__(kBlockElems, 2) void hex8_block_kernel(KernelArgs<T
       element([&](int i) { return sv[(i % 3) * kStageNodes + li[i / 3]]; });
     }
     __syncthreads();  // the staging area becomes the slots
-  } else if (e >= 0) {
+  } else if (e >= 0) {  // per-element gathers
     const size_t P = static_cast<size_t>(a.padded);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
@@ -490,12 +493,19 @@ int launch_operator(fx_batch* b, const KernelArgs<T>& args) {
     return FX_EINVAL;
   }
   if (b->blk.nblocks > 0) {
+    const dim3 grid(b->blk.nblocks);
     if (b->ssp)
-      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true>), dim3(b->blk.nblocks), dim3(kBlockElems), smem, st, args,
-                         partial);
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true>), grid, dim3(kBlockElems), smem, st, args, partial);
     else
-      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false>), dim3(b->blk.nblocks), dim3(kBlockElems), smem, st,
-                         args, partial);
+      hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false>), grid, dim3(kBlockElems), smem, st, args, partial);
+    if (b->blk.max_ln > kStageNodes) {  // blocks whose local nodes do not fit the staging area
+      if (b->ssp)
+        hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, true, true>), grid, dim3(kBlockElems), smem, st, args,
+                           partial);
+      else
+        hipLaunchKernelGGL((hex8_block_kernel<T, DIAG, false, true>), grid, dim3(kBlockElems), smem, st, args,
+                           partial);
+    }
   }
   if (b->blk.nshared > 0)
     hipLaunchKernelGGL(block_shared_sum_kernel<T>, dim3((b->blk.nshared + 255) / 256), dim3(256), 0, st,

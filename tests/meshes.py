@@ -28,6 +28,24 @@ def column(nz=24, seed=11):
     return S.soil_column(nz, 1.0, params=params, layer_of=lambda k: np.minimum(k * 3 // nz, 2))
 
 
+def partly_exploded(n=12, lo=0.3, hi=0.7):
+    """The distorted n^3 box whose elements with centroid height in [lo, hi) get their own copies
+    of their eight nodes (no node shared with any other element): a block of such elements has
+    8 local nodes per element, more than the general path stages in LDS, so one apply runs both
+    the staged blocks and the per-element-gather blocks (and blocks with some of each)."""
+    m = distorted_box(n)
+    xyz = m["xyz"]
+    conn = m["conn"].copy()
+    zc = xyz[conn, 2].mean(axis=1)
+    cut = np.nonzero((zc >= lo) & (zc < hi))[0]
+    extra = xyz[conn[cut].ravel()]
+    conn[cut] = m["nnode"] + np.arange(8 * len(cut), dtype=conn.dtype).reshape(-1, 8)
+    m["xyz"] = np.concatenate([xyz, extra])
+    m["nnode"] = len(m["xyz"])
+    m["conn"] = conn
+    return m
+
+
 def random_vector(n, seed):
     rng = S.Lcg(seed)
     return np.array([rng.symmetric() for _ in range(n)])

@@ -384,3 +384,33 @@ def test_reference_numbering_apply_gives_the_solver_numbering_bits(ctx, tied):
     batch.apply_f32(x32, y32)
     want = om.apply(op, x.astype(np.float32).astype(np.float64))
     assert np.max(np.abs(y32.get() - want)) <= 2e-6 * np.max(np.abs(want))
+
+
+def test_general_path_staged_and_gathered_blocks(ctx):
+    """A mesh with blocks whose local nodes fit the LDS staging and blocks that gather per element
+    (disjoint elements): apply, diagonal, FP32 apply, the internal force and a repeat for bits."""
+    model = meshes.partly_exploded(12)
+    model["region"] = model["group"].copy()
+    model["rayleigh"] = [(0.6, 5.8e-4, 1e-4, 0.0), (0.9, 3e-4, 0.0, 2e-5)]
+    pm, batch, om, op = setup(ctx, model, fp32=True, general=True)
+    assert batch.info()["lattice"] == 0
+    x = meshes.random_vector(pm.neq, 23)
+    dx, dy, dd = core.DeviceArray.from_host(ctx, x), core.DeviceArray(ctx, pm.neq), core.DeviceArray(ctx, pm.neq)
+    for kk in COEFFS:
+        batch.apply(dx, dy, core.coeffs(*kk))
+        assert rel_err(dy.get(), om.apply(op, x, fxo.coeffs(*kk))) <= 1e-12, kk
+        batch.diagonal(dd, core.coeffs(*kk))
+        assert rel_err(dd.get(), om.diagonal(op, fxo.coeffs(*kk))) <= 1e-12, kk
+    batch.apply(dx, dy)
+    a = dy.get()
+    batch.apply(dx, dy)
+    np.testing.assert_array_equal(dy.get(), a)
+    dx32 = core.DeviceArray.from_host(ctx, x.astype(np.float32), dtype=np.float32)
+    dy32 = core.DeviceArray(ctx, pm.neq, dtype=np.float32)
+    batch.apply_f32(dx32, dy32)
+    assert rel_err(dy32.get().astype(np.float64), om.apply(op, x.astype(np.float32).astype(np.float64))) <= 2e-6
+    u = meshes.random_vector(pm.neq, 41) * 1e-3
+    v = meshes.random_vector(pm.neq, 42) * 1e-2
+    du, dv, df = core.DeviceArray.from_host(ctx, u), core.DeviceArray.from_host(ctx, v), core.DeviceArray(ctx, pm.neq)
+    batch.force(du, df, dv)
+    assert rel_err(df.get(), om.force(op, u, v)) <= 1e-12
