@@ -10,6 +10,9 @@ reads the files the hot path needs and stores them unchanged in value (float64 r
   elastic.npz      nd/ElasticIsotropic_*.txt          (20 strain paths)
   pimy.npz         nd/PressureIndependMultiYield_*.txt (34 strain paths)
   constraints.npz  constraints/cases/*.txt + constraints/*.txt (OpenSees equation classes)
+  system_pimy.npz  system/{element_std_clay_implicit,column4_pimy_std_implicit}/opensees_{disp,vel,accel}.txt
+                   (OpenSees node recorders of the implicit stdBrick<PressureIndependMultiYield>
+                   soil-column cases, one row per shaking step: time, then nodes x DOFs)
 
 Each data file becomes three arrays: "<name>/data" (rows x columns), "<name>/columns" and
 "<name>/header" (the '# key value...' lines as a JSON object). Run:
@@ -97,6 +100,19 @@ def pack_constraints(ref, out):
     print(out, len(names), "cases")
 
 
+SYSTEM_CASES = ("element_std_clay_implicit", "column4_pimy_std_implicit")
+
+
+def pack_system(ref, out):
+    arrays = {}
+    for case in SYSTEM_CASES:
+        for resp in ("disp", "vel", "accel"):
+            path = os.path.join(ref, "tests/reference/data/system", case, f"opensees_{resp}.txt")
+            arrays[f"{case}/{resp}"] = np.loadtxt(path, ndmin=2)
+    np.savez_compressed(os.path.join(HERE, out), **arrays)
+    print(out, len(SYSTEM_CASES), "cases")
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     data = os.path.join(ref, "tests/reference/data")
@@ -111,6 +127,7 @@ def main():
     pack(sorted(os.path.join(nd, f) for f in os.listdir(nd) if f.startswith("PressureIndependMultiYield_")),
          "pimy.npz")
     pack_constraints(ref, "constraints.npz")
+    pack_system(ref, "system_pimy.npz")
 
 
 if __name__ == "__main__":

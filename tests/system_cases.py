@@ -1,0 +1,127 @@
+"""The reference's implicit stdBrick<PressureIndependMultiYield> soil-column system cases, rebuilt
+as models of this package, with a stage driver for the oracle and one for the device.
+
+The reference stores OpenSees' node-recorder output for each case
+(tests/reference/data/system/<case>/opensees_{disp,vel,accel}.txt, packed into
+tests/golden/system_pimy.npz); the model is written by
+tests/system/soil_column/make_nonlinear.py (its arguments: nonlinear_cases.sh:6-28) on the
+mesh of make_models.py:68-107:
+
+  - nx x ny x nz stdBrick column, 1 m wide, elements 0.5 m high, z from -nz/2 to 0, consistent
+    mass, body force (0, 0, -9.81 rho) (make_nonlinear.py:151-156, make_models.py:68-82);
+  - the base fixed, every level's side nodes tied to its first side node in DOFs 1 2 3
+    (make_models.py:84-88);
+  - soils (make_nonlinear.py:60-66): clay (Femora's basin clay) and frictional;
+    layered_clay = frictional in the lower half (k < nz // 2), clay above (:285-288);
+  - one Rayleigh region over every element, Femora's frequency_rayleigh(2.76 Hz, 13.84 Hz, 3%)
+    (:97-99, 161-163);
+  - stages (:101-128, 166-172): updateMaterialStage 0; gravity, Newmark 0.6 / 0.3025, dt 1, 30
+    steps, Newton, NormUnbalance 1e-7 (30 iterations); updateMaterialStage 1; re-equilibration,
+    dt 0.01, 100 steps, ModifiedNewton -factoronce, NormUnbalance 1e-7; the UniformExcitation in x
+    (a Path series with explicit times: make_models.py:90-95's tapered sine sweep over 1 s times
+    9.81 amp, amp 3), setTime 0; shaking, Newmark 0.5 / 0.25 at dt 5e-3 while the time printed
+    with %f is below 0.5 (100 steps), Newton, NormDispIncr 1e-10 (25 iterations);
+  - recorders: displacement, velocity, acceleration of the node lines of make_models.py:97-99
+    (for a 1 x 1 column the corner line three times), DOFs 1 2 3, every shaking step.
+
+The reference's tolerances (nonlinear_cases.sh:21,24; docs/deviations.md): the largest
+difference relative to the response's largest magnitude within 1e-5 (element) and 5e-6
+(column) on host backends, 5e-5 and 2e-5 on GPUs.
+"""
+import math
+import os
+
+import numpy as np
+
+from femorax_amd import synthetic as S
+
+GRAVITY = 9.81
+DURATION, DT, AMP = 0.5, 5e-3, 3.0
+# PressureIndependMultiYield parameters (make_nonlinear.py:60-66): rho, refShearModul,
+# refBulkModul, cohesi, peakShearStra, frictionAng, refPress, pressDependCoe; 20 surfaces
+SOILS = {
+    "clay": dict(rho=1.8, G=1.8 * 150.0 ** 2, K=1.8 * (300.0 ** 2 - 4.0 * 150.0 ** 2 / 3.0), cohesion=8.0,
+                 gamma_peak=0.1, phi=0.0, pref=100.0, pdc=0.0),
+    "frictional": dict(rho=2.0, G=60.0e3, K=150.0e3, cohesion=5.0, gamma_peak=0.1, phi=30.0, pref=80.0, pdc=0.5),
+}
+CASES = {
+    "element_std_clay_implicit": dict(nz=1, soils=("clay",), tol_host=1e-5, tol_gpu=5e-5),
+    "column4_pimy_std_implicit": dict(nz=4, soils=("frictional", "clay"), tol_host=5e-6, tol_gpu=2e-5),
+}
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "system_pimy.npz")
+
+
+def rayleigh():
+    """(alphaM, betaK, 0, 0) for 3% damping at 2.76 and 13.84 Hz."""
+    w1, w2, xi = 2 * math.pi * 2.76, 2 * math.pi * 13.84, 0.03
+    return (2 * xi * w1 * w2 / (w1 + w2), 2 * xi / (w1 + w2), 0.0, 0.0)
+
+
+def motion(length=1.0):
+    """The base acceleration record (times, values): a 1-12 Hz sine sweep of amplitude 0.1 over
+    `length` seconds, tapered over 0.1 s at both ends, sampled every 0.005 s."""
+    t = np.arange(0.0, length + 1e-12, 0.005)
+    f0, f1 = 1.0, 12.0
+    acc = 0.1 * np.sin(2 * math.pi * (f0 * t + 0.5 * (f1 - f0) / length * t * t))
+    acc *= np.minimum(1.0, np.minimum(t, length - t) / 0.1)
+    return t, acc
+
+
+def model(case):
+    """The case's column as a model dict (1 x 1 x nz, model node i + 2 (j + 2 k))."""
+    c = CASES[case]
+    nz = c["nz"]
+    col = S.soil_column(nz, 1.0)
+    xyz, conn, ek = S.grid(1, 1, nz, 1.0, 1.0, 0.5, z0=-nz * 0.5)
+    col["xyz"], col["conn"], col["h"] = xyz, conn, (1.0, 1.0, 0.5)
+    two = len(c["soils"]) == 2
+    col["group"] = (ek >= nz // 2).astype(np.int32) if two else np.zeros(len(conn), np.int32)
+    col["region"] = np.zeros(len(conn), np.int32)
+    col["params"] = []
+    return col
+
+
+def recorded_nodes(nz):
+    """Model nodes of the recorder columns: the corner, edge-middle and centre lines, which on a
+    1 x 1 column are all the corner line (node (0, 0, k))."""
+    line = [2 * 2 * k for k in range(nz + 1)]
+    return line * 3
+
+
+def soil_of_set(case, groups):
+    """The material dict (oracle form) of each compiled parameter set, by its element group."""
+    soils = CASES[case]["soils"]
+    out = []
+    for g in groups:
+        m = dict(SOILS[soils[g]])
+        m.update(nsurf=20, b=(0.0, 0.0, -GRAVITY * m["rho"]), mass_type=0, rayleigh=rayleigh())
+        out.append(m)
+    return out
+
+
+def opensees(case):
+    """OpenSees' recorded (disp, vel, accel), each [steps, 1 + 3 * nodes] with the time first."""
+    with np.load(FIXTURE) as f:
+        return tuple(f[f"{case}/{r}"] for r in ("disp", "vel", "accel"))
+
+
+def record(node_eq, node_rank, nodes, x):
+    """The recorder's row for equation-space x: nodes x DOFs (a fixed DOF 0)."""
+    eqs = np.asarray(node_eq).reshape(-1, 3)[np.asarray(node_rank)[nodes]]
+    out = np.where(eqs >= 0, x[np.maximum(eqs, 0)], 0.0)
+    return out.ravel()
+
+
+def shaking_steps():
+    """Steps of `while {[getTime] < 0.5} {analyze 1 dt}` with the time compared as %f prints it."""
+    t, n = 0.0, 0
+    while float(f"{t:f}") < DURATION:
+        t += DT
+        n += 1
+    return n
+
+
+def relative_difference(got, want):
+    """The largest difference relative to the response's largest magnitude (the reference's
+    tests/system/soil_column/compare.py:43-47), over the value columns."""
+    return float(np.max(np.abs(got[:, 1:] - want[:, 1:])) / np.max(np.abs(want[:, 1:])))

@@ -1,0 +1,65 @@
+"""The oracle's implicit stage path against OpenSees on the reference's system cases.
+
+The oracle runs each case's whole analysis (tests/system_cases.py: gravity with the soils
+elastic, the plastic re-equilibration by ModifiedNewton -factoronce, 100 steps of Newmark +
+Newton shaking under the uniform excitation) and its recorded displacements, velocities and
+accelerations must match OpenSees' node recorders (tests/golden/system_pimy.npz) within the
+reference's host tolerance for the case. This pins, end to end, the oracle's Newmark step and
+Newton loop, both convergence tests, ModifiedNewton's kept tangent, the material-stage switch,
+PressureIndependMultiYield inside stdBrick, Rayleigh damping, the laminar ties and the
+PathTimeSeries excitation.
+"""
+import numpy as np
+import pytest
+
+from oracle import fxo
+from tests import system_cases as SC
+
+
+def run_oracle(case):
+    """The case's analysis on the oracle: the recorded rows (time, disp), (time, vel), (time, accel)."""
+    m = SC.model(case)
+    om = fxo.OracleMesh(m)
+    pe = fxo.pimy_elems(SC.soil_of_set(case, om.param_group), stage=0)
+    Cs = om.pimy_initial_state(pe)
+    Ts = Cs.copy()
+    n = om.neq
+    u, v, a, du = (np.zeros(n) for _ in range(4))
+    kw = dict(pimy=pe, pimy_state=[Cs, Ts])
+    rc, st = om.newmark(u, v, a, du, 1.0, 30, gamma=0.6, beta=0.3025, test=0, tol=1e-7, max_iter=30, time=0.0, **kw)
+    assert rc == 0 and st["steps"] == 30, st
+    for p in pe:
+        p.mat.load_stage = 1
+    rc, st = om.newmark(u, v, a, du, 0.01, 100, gamma=0.6, beta=0.3025, test=0, tol=1e-7, max_iter=30, algorithm=2,
+                        factor_once=True, time=st["time"], **kw)
+    assert rc == 0 and st["steps"] == 100, st
+    t, acc = SC.motion()
+    series = fxo.TimeArraySeries(acc, t, factor=SC.GRAVITY * SC.AMP)
+    nodes = SC.recorded_nodes(SC.CASES[case]["nz"])
+    rows = [[], [], []]
+    time = 0.0
+    for _ in range(SC.shaking_steps()):
+        rc, st = om.newmark(u, v, a, du, SC.DT, 1, test=1, tol=1e-10, max_iter=25, ground_dir=0, series=series,
+                            time=time, **kw)
+        assert rc == 0 and st["steps"] == 1, st
+        time = st["time"]
+        for r, x in zip(rows, (u, v, a)):
+            r.append(np.concatenate([[time], SC.record(om.node_eq, om.node_rank, nodes, x)]))
+    return tuple(np.array(r) for r in rows), Cs
+
+
+@pytest.mark.parametrize("case", sorted(SC.CASES))
+def test_oracle_matches_opensees_system_case(case):
+    (disp, vel, acc), Cs = run_oracle(case)
+    want = SC.opensees(case)
+    for got, w in zip((disp, vel, acc), want):
+        assert got.shape == w.shape
+        np.testing.assert_allclose(got[:, 0], w[:, 0], rtol=0, atol=1e-12)
+    errs = [SC.relative_difference(g, w) for g, w in zip((disp, vel, acc), want)]
+    assert max(errs) <= SC.CASES[case]["tol_host"], errs
+    if case == "column4_pimy_std_implicit":  # the frictional layer yields (surfaces up to 12 active)
+        assert np.max(Cs[:, 26]) > 0
+
+
+def test_shaking_loop_steps():
+    assert SC.shaking_steps() == 100
