@@ -68,48 +68,64 @@ __global__ __launch_bounds__(kLanes) void chunk_dot_kernel(int64_t n, const T* _
 
 enum ReduceOp : int { kStore = 0, kRhsNorm, kInitRes, kRz0, kPap, kIter };
 
-// Reduce `nv` interleaved sequences of chunk partials level by level in one workgroup (one
-// wave per chunk, VectorOps' order), then apply the solver step `op` to the totals.
-__global__ __launch_bounds__(kLanes) void reduce_kernel(int64_t npart, int nv, double* __restrict__ partial,
-                                                        double* __restrict__ tmp, CgState* st, int op,
-                                                        double rtol, double atol, double* out) {
+// Lane j's value of v, in every lane (j uniform).
+__device__ __forceinline__ double lane_value(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), j);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), j);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+
+// One chunk of a level, summed by one wave in VectorOps' order: lane l adds entries l, l + 64,
+// ... in turn (its 32 loads issued together; entries past len enter as +0.0, which leaves a sum
+// that starts at +0.0 unchanged), then the 64 lane sums are added in lane order. Every lane
+// returns the total.
+__device__ __forceinline__ double wave_chunk_sum(const double* src, int64_t stride, int64_t begin, int64_t len, int l) {
+  double v[kChunk / kLanes];
+#pragma unroll
+  for (int j = 0; j < kChunk / kLanes; ++j) {
+    const int64_t k = l + static_cast<int64_t>(j) * kLanes;
+    v[j] = k < len ? src[(begin + k) * stride] : 0.0;
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < kChunk / kLanes; ++j) s += v[j];
+  double t = 0.0;
+  for (int j = 0; j < kLanes; ++j) t += lane_value(s, j);
+  return t;
+}
+
+// Reduce `nv` (1 or 2) interleaved sequences of chunk partials level by level in one workgroup,
+// in VectorOps' order, then apply the solver step `op` to the totals. The first level's chunks
+// go to the workgroup's waves (their totals in LDS); wave 0 sums those (one chunk: fewer than
+// 2^31 entries have at most kRedTot first-level chunks). The same bits as one wave walking the
+// levels in turn.
+constexpr int kRedWaves = 8, kRedThreads = kRedWaves * kLanes, kRedTot = 512;
+static_assert(static_cast<int64_t>(kRedTot) * kChunk * kChunk >= (int64_t(1) << 31), "first-level totals of 2^31 entries");
+__global__ __launch_bounds__(kRedThreads) void reduce_kernel(int64_t npart, int nv, double* __restrict__ partial,
+                                                             double* __restrict__ tmp, CgState* st, int op,
+                                                             double rtol, double atol, double* out) {
   if (op == kPap || op == kIter)
     if (st->done) return;
-  __shared__ double lane_sum[kLanes];
-  const int l = threadIdx.x;
+  (void)tmp;
+  __shared__ double tot[2][kRedTot];
+  const int l = threadIdx.x % kLanes, w = threadIdx.x / kLanes;
+  const int64_t nch = (npart + kChunk - 1) / kChunk;
   double total[2] = {0.0, 0.0};
-  for (int v = 0; v < nv; ++v) {
-    double* src = partial;
-    double* dst = tmp;
-    int64_t m = npart;
-    int stride = nv, off = v;  // first level reads interleaved
-    while (m > 1) {
-      const int64_t nch = (m + kChunk - 1) / kChunk;
-      for (int64_t c = 0; c < nch; ++c) {
+  if (npart > 1) {
+    for (int v = 0; v < nv; ++v)
+      for (int64_t c = w; c < nch; c += kRedWaves) {
         const int64_t begin = c * kChunk;
-        const int64_t len = (m - begin < kChunk) ? m - begin : kChunk;
-        double s = 0.0;
-        for (int64_t k = l; k < len; k += kLanes) s += src[(begin + k) * stride + off];
-        lane_sum[l] = s;
-        __syncthreads();
-        if (l == 0) {
-          double t = 0.0;
-          for (int j = 0; j < kLanes; ++j) t += lane_sum[j];
-          dst[c] = t;
-        }
-        __syncthreads();
+        const double t = wave_chunk_sum(partial + v, nv, begin, npart - begin < kChunk ? npart - begin : kChunk, l);
+        if (l == 0) tot[v][c] = t;
       }
-      m = nch;
-      double* sw = src == partial ? tmp + ((npart + kChunk - 1) / kChunk + 1) : src;
-      src = dst;
-      dst = sw;
-      stride = 1;
-      off = 0;
-    }
-    total[v] = src[off];
     __syncthreads();
+    if (w == 0)
+      for (int v = 0; v < nv; ++v) total[v] = nch > 1 ? wave_chunk_sum(tot[v], 1, 0, nch, l) : tot[v][0];
+  } else {
+    for (int v = 0; v < nv; ++v) total[v] = partial[v];
   }
-  if (l != 0) return;
+  if (threadIdx.x != 0) return;
   switch (op) {
     case kStore:
       out[0] = total[0];
@@ -369,7 +385,7 @@ int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out)
                      n, x, y, static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
                      static_cast<const int32_t*>(nullptr));
   double* dres = tmp + chunks(chunks(n)) + 8;
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, ctx->stream, chunks(n), 1, part, tmp,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, chunks(n), 1, part, tmp,
                      static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, dres);
   FX_HIP(ctx, hipGetLastError());
   FX_HIP(ctx, hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -452,20 +468,20 @@ int cg_setup(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x
   // ||b||
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, b, b, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kRhsNorm), rtol, atol, static_cast<double*>(nullptr));
   // r = b - A x
   if (int rc = apply(x, ap, static_cast<const int32_t*>(nullptr))) return rc;
   hipLaunchKernelGGL(residual_kernel<T>, g, dim3(256), 0, s, n, b, ap, r);
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kInitRes), rtol, atol, static_cast<double*>(nullptr));
   // z = inv r, p = z, rz = r.z
   hipLaunchKernelGGL(precond_first_kernel<T>, g, dim3(256), 0, s, n, inv, r, z, p);
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, z, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kRz0), rtol, atol, static_cast<double*>(nullptr));
   FX_HIP(ctx, hipGetLastError());
   return FX_OK;
@@ -484,10 +500,10 @@ int cg_iteration(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, T* x, T* r, 
   const int32_t* done = &st->done;
   if (int rc = apply(p, ap, done)) return rc;
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
   hipLaunchKernelGGL(update_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, st, x, r, z, p, ap, inv, part);
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 2, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 2, part, tmp, st,
                      static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
   hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
   return FX_OK;
@@ -885,7 +901,7 @@ int dot64_static(fx_ctx* ctx, int64_t n, const double* x, const double* y, doubl
   hipLaunchKernelGGL(chunk_dot_kernel<double>, dim3(static_cast<unsigned>(chunks(n))), dim3(kLanes), 0, s, n, x, y,
                      static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kLanes), 0, s, chunks(n), 1, part, tmp,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp,
                      static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, out_dev);
   FX_HIP(ctx, hipGetLastError());
   return FX_OK;

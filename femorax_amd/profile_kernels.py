@@ -8,6 +8,10 @@
                                                              the same column, elastic and yielding, fused and compacted
     python femorax_amd/profile_kernels.py general_time N REPS   wall time per apply (hex8_block_kernel and the
                                                              shared-node sum) on the distorted N^3 box, FP64 and FP32
+    python femorax_amd/profile_kernels.py cg N REPS        CG + Jacobi to rtol 1e-8 on the C2 N^3 box (nodal load), the
+                                                             iterations replayed from graphs: REPS solves after a warm one,
+                                                             wall time per iteration (cg_eager: the same with every
+                                                             kernel launched one by one, for kernel traces)
     python femorax_amd/profile_kernels.py ssp N REPS       SSPbrick operator on the N^3 layered box: the lattice path
                                                              (hex8_box_kernel<T, 3..5>) and the general path
     python femorax_amd/profile_kernels.py rect N REPS      stdBrick operator on the N^3 layered box graded in z
@@ -123,6 +127,29 @@ def general_time(n, reps):
     ctx.close()
 
 
+def cg(n, reps, graphs=True):
+    import time
+    model = S.config_c2(n=n)
+    mesh = core.Mesh(model)
+    ctx = core.Context(0)
+    batch = core.Batch(ctx, mesh, mesh.params_for(model))
+    db = core.DeviceArray.from_host(ctx, mesh.to_equations(model["nodal_load"]))
+    xs = core.DeviceArray(ctx, mesh.neq)
+    ctx.set_graphs(graphs)
+    for rep in range(reps + 1):
+        xs.zero()
+        ctx.sync()
+        t0 = time.perf_counter()
+        r = batch.cg(db, xs, rtol=1e-8, max_it=100000)
+        ctx.sync()
+        s = time.perf_counter() - t0
+        print(f"CG {'warm' if rep == 0 else rep}: {r['iterations']} iterations, {s:.3f} s, "
+              f"{s * 1e3 / r['iterations']:.4f} ms per iteration", flush=True)
+    ctx.set_graphs(False)
+    batch.close()
+    ctx.close()
+
+
 def ssp(n, reps):
     model = S.config_c2(n=n)
     mesh = core.Mesh(model)
@@ -164,4 +191,4 @@ if __name__ == "__main__":
     size = int(sys.argv[2]) if len(sys.argv) > 2 else (40 if what.startswith("pimy") else 128)
     reps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
     {"general": general, "pimy": pimy, "ssp": ssp, "rect": rect, "pimy_time": pimy_time,
-     "general_time": general_time}[what](size, reps)
+     "general_time": general_time, "cg": cg, "cg_eager": lambda n, r: cg(n, r, graphs=False)}[what](size, reps)

@@ -203,6 +203,17 @@ def test_dot_matches_oracle_order(ctx):
         assert abs(d - want) <= 1e-14 * np.sum(np.abs(x * y)) + 1e-300, n
 
 
+def test_dot_reduction_order_bitwise(ctx):
+    """With y = 1 every product is exact, so the dot is the summation order alone: the device's
+    chunk, lane and level sums (the reduction's waves included) equal the oracle's VectorOps order
+    bit for bit, across one, two and many first-level chunks."""
+    for n in (1, 2, 63, 64, 2047, 2048, 2049, 4097, 100003, 2048 * 2048 + 5, 6_390_144):
+        x = np.resize(meshes.random_vector(min(n, 100003), n + 7), n) * (1.0 + np.arange(n) % 13)
+        ones = np.ones(n)
+        d = core.dot(ctx, core.DeviceArray.from_host(ctx, x), core.DeviceArray.from_host(ctx, ones))
+        assert d == fxo.dot(x, ones, lanes=64), n
+
+
 @pytest.mark.parametrize("name", ["c1_box16", "distorted6", "column24_tied"])
 def test_cg_matches_oracle(ctx, name):
     model = CASES[name]()
