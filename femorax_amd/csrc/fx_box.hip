@@ -810,8 +810,8 @@ int launch_box(fx_batch* b, const fx_coeffs& k, const T* x, T* y, int accumulate
   static const Kernel rect8[3] = {hex8_box_kernel<T, 6, 0, false, 8>, hex8_box_kernel<T, 7, 0, false, 8>,
                                   hex8_box_kernel<T, 8, 0, false, 8>};
   const bool one = b->ssp || b->rect;  // a single kernel per mode
-  const int v = one ? 0 : b->box_variant & 15;
   const int m = DIAG ? 2 : mass ? 1 : 0;
+  const int v = one ? 0 : (m == 0 ? b->box_variant : b->box_variant_mass) & 15;
   const bool stream = !one && (b->tile_z == 16 || ((v & 8) && b->tile_z == 8));
   const Kernel kernel = b->ssp            ? (b->tile_z == 8 ? ssp8 : ssp4)[m]
                         : b->rect         ? (b->tile_z == 8 ? rect8 : rect4)[m]

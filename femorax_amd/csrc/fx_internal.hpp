@@ -290,8 +290,9 @@ struct fx_batch {
   uint32_t* shared_meta = nullptr;
   double* partial64 = nullptr;        // [nslot * 3] per-tile sums at shared nodes
   float* partial32 = nullptr;
-  int box_variant = 0;                // register-allocation variant of the mass-free kernel (tuned)
-  float tuned_ms = 0.0f;              // the tuned variant's sample time (both operator forms), 0 untuned
+  int box_variant = 0;                // lattice kernel variant of the operator without mass terms (tuned)
+  int box_variant_mass = 0;           // the same with mass terms, and for the diagonal
+  float tuned_ms = 0.0f;              // the tuned variants' sample times (both operator forms), 0 untuned
   int tile_z = 8;                     // lattice tiles of 8 x 8 x tile_z cells
   // internal (tile-major) equation numbering of lattice batches: perm[internal] = reference
   bool internal = false;

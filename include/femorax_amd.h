@@ -196,7 +196,9 @@ int fx_batch_info_get(const fx_batch* b, fx_batch_info* info);
  * choice or at least 5, 7, 8 waves per SIMD; bit 2: persistent workgroups that prefetch their
  * next tile; bit 3: streamed 8-deep tiles, x and words of five node planes in LDS at a time;
  * 16-deep tiles are always streamed); general batches have one. The autotune times 0-7 (4-deep
- * tiles), 0-11 (8-deep) or 0-3 (16-deep). Every variant gives the same bits. */
+ * tiles), 0-11 (8-deep) or 0-3 (16-deep), for the operator without mass terms and with them, and
+ * keeps a variant for each (16 (m + 1) + v: v without mass terms, m with them; 0-15 sets both).
+ * fx_batch_info.variant reports v. Every variant gives the same bits. */
 int fx_batch_tune(fx_batch* b, int variant);
 
 /* y = K_eff x, K_eff = cK K_t + cC C + cM M (InternalForce::apply_operator,

@@ -138,14 +138,18 @@ def test_streamed_tile_variants_give_the_same_bits(ctx, depth):
     dx = core.DeviceArray.from_host(ctx, x)
     ya = core.DeviceArray(ctx, pm.neq)
     variants = range(12) if depth == 8 else range(4)
+    # the forms without and with mass terms may run different variants: 16 (m + 1) + v
+    pairs = [(2, 9), (9, 4), (3, 0)] if depth == 8 else [(1, 3), (3, 0)]
     for kk in COEFFS:
         outs = []
-        for v in variants:
+        for v in list(variants) + [16 * (m + 1) + v for v, m in pairs]:
             a.tune(v)
             a.apply(dx, ya, core.coeffs(*kk))
             outs.append(ya.get())
         for o in outs[1:]:
             np.testing.assert_array_equal(outs[0], o)
+    a.tune(16 * (pairs[0][1] + 1) + pairs[0][0])
+    assert a.info()["variant"] == pairs[0][0]
     a.tune(-1)
     assert a.info()["variant"] in variants
 
