@@ -68,40 +68,45 @@ __global__ __launch_bounds__(kLanes) void chunk_dot_kernel(int64_t n, const T* _
 
 enum ReduceOp : int { kStore = 0, kRhsNorm, kInitRes, kRz0, kPap, kIter };
 
-// Lane j's value of v, in every lane (j uniform).
-__device__ __forceinline__ double lane_value(double v, int j) {
-  const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), j);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), j);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-
 // One chunk of a level, summed by one wave in VectorOps' order: lane l adds entries l, l + 64,
-// ... in turn (its 32 loads issued together; entries past len enter as +0.0, which leaves a sum
-// that starts at +0.0 unchanged), then the 64 lane sums are added in lane order. Every lane
-// returns the total.
-__device__ __forceinline__ double wave_chunk_sum(const double* src, int64_t stride, int64_t begin, int64_t len, int l) {
-  double v[kChunk / kLanes];
-#pragma unroll
-  for (int j = 0; j < kChunk / kLanes; ++j) {
-    const int64_t k = l + static_cast<int64_t>(j) * kLanes;
-    v[j] = k < len ? src[(begin + k) * stride] : 0.0;
-  }
+// ... in turn, then lane 0 adds the 64 lane sums in lane order (through this wave's lane_sum
+// row in LDS: a wave's LDS accesses complete in program order). A full chunk issues each lane's
+// 32 loads together; a partial one (the last of a level, or a small vector) walks its entries.
+// Returns the total in lane 0.
+__device__ __forceinline__ double wave_chunk_sum(const double* src, int64_t stride, int64_t begin, int64_t len, int l,
+                                                 double* lane_sum) {
   double s = 0.0;
+  if (len == kChunk) {
+    double v[kChunk / kLanes];
 #pragma unroll
-  for (int j = 0; j < kChunk / kLanes; ++j) s += v[j];
+    for (int j = 0; j < kChunk / kLanes; ++j) v[j] = src[(begin + l + static_cast<int64_t>(j) * kLanes) * stride];
+#pragma unroll
+    for (int j = 0; j < kChunk / kLanes; ++j) s += v[j];
+  } else {
+    for (int64_t k = l; k < len; k += kLanes) s += src[(begin + k) * stride];
+  }
+  lane_sum[l] = s;
+  __builtin_amdgcn_wave_barrier();
   double t = 0.0;
-  for (int j = 0; j < kLanes; ++j) t += lane_value(s, j);
+  if (l == 0)
+    for (int j = 0; j < kLanes; ++j) t += lane_sum[j];
+  __builtin_amdgcn_wave_barrier();
   return t;
 }
 
 // Reduce `nv` (1 or 2) interleaved sequences of chunk partials level by level in one workgroup,
 // in VectorOps' order, then apply the solver step `op` to the totals. The first level's chunks
-// go to the workgroup's waves (their totals in LDS); wave 0 sums those (one chunk: fewer than
-// 2^31 entries have at most kRedTot first-level chunks). The same bits as one wave walking the
-// levels in turn.
+// go to the workgroup's waves (red_threads: one wave per chunk, at most kRedWaves; their totals
+// in LDS); wave 0 sums those (one chunk: fewer than 2^31 entries have at most kRedTot first-level
+// chunks). The same bits as one wave walking the levels in turn.
 constexpr int kRedWaves = 8, kRedThreads = kRedWaves * kLanes, kRedTot = 512;
 static_assert(static_cast<int64_t>(kRedTot) * kChunk * kChunk >= (int64_t(1) << 31), "first-level totals of 2^31 entries");
+__host__ __device__ inline int64_t red_chunks(int64_t n) { return (n + kChunk - 1) / kChunk; }
+// the reduction's workgroup for n entries: a wave per first-level chunk of the n / 2048 partials
+inline dim3 red_threads(int64_t n) {
+  const int64_t c = red_chunks(red_chunks(n));
+  return dim3(static_cast<unsigned>(kLanes * (c < 1 ? 1 : c > kRedWaves ? kRedWaves : c)));
+}
 __global__ __launch_bounds__(kRedThreads) void reduce_kernel(int64_t npart, int nv, double* __restrict__ partial,
                                                              double* __restrict__ tmp, CgState* st, int op,
                                                              double rtol, double atol, double* out) {
@@ -109,19 +114,21 @@ __global__ __launch_bounds__(kRedThreads) void reduce_kernel(int64_t npart, int 
     if (st->done) return;
   (void)tmp;
   __shared__ double tot[2][kRedTot];
+  __shared__ double lane_sum[kRedWaves][kLanes];
   const int l = threadIdx.x % kLanes, w = threadIdx.x / kLanes;
   const int64_t nch = (npart + kChunk - 1) / kChunk;
   double total[2] = {0.0, 0.0};
   if (npart > 1) {
     for (int v = 0; v < nv; ++v)
-      for (int64_t c = w; c < nch; c += kRedWaves) {
+      for (int64_t c = w; c < nch; c += static_cast<int64_t>(blockDim.x / kLanes)) {
         const int64_t begin = c * kChunk;
-        const double t = wave_chunk_sum(partial + v, nv, begin, npart - begin < kChunk ? npart - begin : kChunk, l);
+        const double t =
+            wave_chunk_sum(partial + v, nv, begin, npart - begin < kChunk ? npart - begin : kChunk, l, lane_sum[w]);
         if (l == 0) tot[v][c] = t;
       }
     __syncthreads();
     if (w == 0)
-      for (int v = 0; v < nv; ++v) total[v] = nch > 1 ? wave_chunk_sum(tot[v], 1, 0, nch, l) : tot[v][0];
+      for (int v = 0; v < nv; ++v) total[v] = nch > 1 ? wave_chunk_sum(tot[v], 1, 0, nch, l, lane_sum[0]) : tot[v][0];
   } else {
     for (int v = 0; v < nv; ++v) total[v] = partial[v];
   }
@@ -385,7 +392,7 @@ int dot64(fx_ctx* ctx, int64_t n, const double* x, const double* y, double* out)
                      n, x, y, static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
                      static_cast<const int32_t*>(nullptr));
   double* dres = tmp + chunks(chunks(n)) + 8;
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, ctx->stream, chunks(n), 1, part, tmp,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, ctx->stream, chunks(n), 1, part, tmp,
                      static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, dres);
   FX_HIP(ctx, hipGetLastError());
   FX_HIP(ctx, hipMemcpyAsync(out, dres, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -468,20 +475,20 @@ int cg_setup(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, const T* b, T* x
   // ||b||
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, b, b, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kRhsNorm), rtol, atol, static_cast<double*>(nullptr));
   // r = b - A x
   if (int rc = apply(x, ap, static_cast<const int32_t*>(nullptr))) return rc;
   hipLaunchKernelGGL(residual_kernel<T>, g, dim3(256), 0, s, n, b, ap, r);
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, r, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kInitRes), rtol, atol, static_cast<double*>(nullptr));
   // z = inv r, p = z, rz = r.z
   hipLaunchKernelGGL(precond_first_kernel<T>, g, dim3(256), 0, s, n, inv, r, z, p);
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, r, z, nt, nt, part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kRz0), rtol, atol, static_cast<double*>(nullptr));
   FX_HIP(ctx, hipGetLastError());
   return FX_OK;
@@ -500,10 +507,10 @@ int cg_iteration(fx_ctx* ctx, int64_t n, Apply apply, const T* inv, T* x, T* r, 
   const int32_t* done = &st->done;
   if (int rc = apply(p, ap, done)) return rc;
   hipLaunchKernelGGL(chunk_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, p, ap, nt, nt, part, done);
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 1, part, tmp, st,
                      static_cast<int>(kPap), rtol, atol, static_cast<double*>(nullptr));
   hipLaunchKernelGGL(update_dot_kernel<T>, cg_chunks, dim3(kLanes), 0, s, n, st, x, r, z, p, ap, inv, part);
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 2, part, tmp, st,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 2, part, tmp, st,
                      static_cast<int>(kIter), rtol, atol, static_cast<double*>(nullptr));
   hipLaunchKernelGGL(direction_kernel<T>, g, dim3(256), 0, s, n, st, z, p);
   return FX_OK;
@@ -901,7 +908,7 @@ int dot64_static(fx_ctx* ctx, int64_t n, const double* x, const double* y, doubl
   hipLaunchKernelGGL(chunk_dot_kernel<double>, dim3(static_cast<unsigned>(chunks(n))), dim3(kLanes), 0, s, n, x, y,
                      static_cast<const double*>(nullptr), static_cast<const double*>(nullptr), part,
                      static_cast<const int32_t*>(nullptr));
-  hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(kRedThreads), 0, s, chunks(n), 1, part, tmp,
+  hipLaunchKernelGGL(reduce_kernel, dim3(1), red_threads(n), 0, s, chunks(n), 1, part, tmp,
                      static_cast<CgState*>(nullptr), static_cast<int>(kStore), 0.0, 0.0, out_dev);
   FX_HIP(ctx, hipGetLastError());
   return FX_OK;
