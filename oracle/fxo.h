@@ -188,9 +188,23 @@ void fxo_pimy_committed_tangent(const fxo_pimy_params* p, const double* committe
 typedef struct fxo_pimy_elem {
   fxo_pimy_params mat;
   double b[3];       /* body force per unit volume */
-  int32_t mass_type; /* 0 consistent, 1 lumped */
+  int32_t mass_type; /* 0 consistent, 1 lumped (SSPbrick: always lumped) */
   fxo_rayleigh damp; /* the element's Rayleigh damping */
+  int32_t element;   /* 0 stdBrick (8 points), 1 SSPbrick (one point: the element's point 0) */
 } fxo_pimy_elem;
+/* the elastic tangent of the material's reference moduli (initial_tangent, :156-158) */
+void fxo_pimy_initial_tangent(const fxo_pimy_params* p, double D[36]);
+/* SSPbrick<PressureIndependMultiYield> (fxo_ssp.c): compute_with (ssp_brick.hpp:1107-1156) with the
+ * point's update (point_update_shape :1056-1083, point_node_contribution :1087-1098) on history
+ * C0 -> T0, Kstab from the material's initial tangent (element_constants :881); D: the point's
+ * tangent out, or NULL. The operator and its diagonal on the stored tangent (element_operator.hpp
+ * with initial_stiffness_is_tangent; the committed tangent from C0 when betaKcomm enters). */
+void fxo_ssp_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
+                        const double* C0, double* T0, double* fe, double* D);
+void fxo_ssp_operator_apply_pimy(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3], const double* Dt,
+                                 const double* C0, const double* ve, double* ye);
+void fxo_ssp_operator_diagonal_pimy(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                    const double* Dt, const double* C0, const int32_t* eq, double* de);
 /* ve: the element velocities (the Rayleigh damping force, std_brick.hpp:409-464), or NULL */
 void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
                           const double* const* C, double* const* T, double* fe, double* Dgp);

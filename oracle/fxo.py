@@ -55,14 +55,16 @@ class PimyParams(C.Structure):
 
 
 class PimyElem(C.Structure):
-    _fields_ = [("mat", PimyParams), ("b", C.c_double * 3), ("mass_type", C.c_int32), ("damp", Rayleigh)]
+    _fields_ = [("mat", PimyParams), ("b", C.c_double * 3), ("mass_type", C.c_int32), ("damp", Rayleigh),
+                ("element", C.c_int32)]
 
 
 PIMY_NSTATE = 190
 
 
 def pimy_elems(materials, stage=0):
-    """materials: dicts with rho G K cohesion gamma_peak phi pref pdc [nsurf gredu b mass_type]
+    """materials: dicts with rho G K cohesion gamma_peak phi pref pdc [nsurf gredu b mass_type rayleigh
+    element ("stdBrick" or "SSPbrick": one material point, the element's point 0)]
     -> ctypes array of fxo_pimy_elem (setUpSurfaces by the oracle)."""
     arr = (PimyElem * max(1, len(materials)))()
     for i, m in enumerate(materials):
@@ -77,6 +79,7 @@ def pimy_elems(materials, stage=0):
         arr[i].b = (C.c_double * 3)(*m.get("b", (0.0, 0.0, 0.0)))
         arr[i].mass_type = int(m.get("mass_type", 0))
         arr[i].damp = Rayleigh(*m.get("rayleigh", (0.0, 0.0, 0.0, 0.0)))
+        arr[i].element = 1 if m.get("element", "stdBrick") == "SSPbrick" else 0
     return arr
 
 

@@ -401,16 +401,17 @@ void fxo_brick_operator_diagonal(const fxo_elastic* p, const fxo_coeffs* k, cons
 
 /* ---- stdBrick with PressureIndependMultiYield ------------------------------------------------ */
 /* the elastic initial tangent of the material's reference moduli (initial_tangent) */
-static void pimy_initial_D(const fxo_pimy_elem* p, double* D) {
+void fxo_pimy_initial_tangent(const fxo_pimy_params* m, double D[36]) {
   for (int i = 0; i < 6; ++i)
     for (int j = 0; j < 6; ++j) {
       double v = 0.;
-      if (i == j) v += p->mat.G;
-      if (i < 3 && j < 3 && i == j) v += p->mat.G;
-      if (i < 3 && j < 3) v += (p->mat.K - 2. * p->mat.G / 3.);
+      if (i == j) v += m->G;
+      if (i < 3 && j < 3 && i == j) v += m->G;
+      if (i < 3 && j < 3) v += (m->K - 2. * m->G / 3.);
       D[6 * i + j] = v;
     }
 }
+static void pimy_initial_D(const fxo_pimy_elem* p, double* D) { fxo_pimy_initial_tangent(&p->mat, D); }
 
 /* Brick::compute_with (std_brick.hpp:493-543) with the material update per Gauss point
  * (point_update_shape :409-443, point_node_contribution :448-464): per point and node, resid +=
@@ -421,6 +422,10 @@ static void pimy_initial_D(const fxo_pimy_elem* p, double* D) {
  * velocity (fe = resid). */
 void fxo_brick_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
                           const double* const* C, double* const* T, double* fe, double* Dgp) {
+  if (p->element == 1) {
+    fxo_ssp_force_pimy(p, xe, ue, ve, C[0], T[0], fe, Dgp);
+    return;
+  }
   const fxo_rayleigh* dp = &p->damp;
   const int stiff = dp->betaK != 0.0 || dp->betaKinit != 0.0 || dp->betaKcomm != 0.0;
   const int damped = ve && (stiff || dp->alphaM != 0.0);
@@ -503,6 +508,10 @@ static void pimy_effective(const fxo_pimy_elem* p, double cK, double c0, double 
 /* element_operator_apply with stored tangents (Source::Stored, element_operator.hpp:82-130) */
 void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
                                  const double* Dgp, const double* const* C, const double* ve, double* ye) {
+  if (p->element == 1) {
+    fxo_ssp_operator_apply_pimy(p, k, xe, Dgp, C ? C[0] : NULL, ve, ye);
+    return;
+  }
   double cK, c0, cc, cM;
   pimy_factors(p, k, &cK, &c0, &cc, &cM);
   double xl[3][8], D0[36];
@@ -539,6 +548,10 @@ void fxo_brick_operator_apply_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, co
 /* element_operator_diagonal with stored tangents, merged DOFs' cross terms on the first */
 void fxo_brick_operator_diagonal_gp(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
                                     const double* Dgp, const double* const* C, const int32_t* eq, double* de) {
+  if (p->element == 1) {
+    fxo_ssp_operator_diagonal_pimy(p, k, xe, Dgp, C ? C[0] : NULL, eq, de);
+    return;
+  }
   double cK, c0, cc, cM;
   pimy_factors(p, k, &cK, &c0, &cc, &cM);
   double xl[3][8], D0[36];

@@ -326,7 +326,9 @@ static void ssp_node_volumes(const double xl[3][8], const ssp_geom* g, int opens
   }
 }
 
-void fxo_ssp_element(const fxo_elastic* p, const double xe[8][3], int opensees_mass, double* geom, double* consts) {
+/* geometry and constants with interior from the tangent D (C1 = D00, C2 = D01, C3 = D33) */
+static void ssp_element_D(const double xe[8][3], int opensees_mass, const double D[6][6], double* geom,
+                          double* consts) {
   double xl[3][8];
   for (int i = 0; i < 8; ++i)
     for (int d = 0; d < 3; ++d) xl[d][i] = xe[i][d];
@@ -335,8 +337,6 @@ void fxo_ssp_element(const fxo_elastic* p, const double xe[8][3], int opensees_m
   for (int k = 0; k < 24; ++k) geom[k] = g.dN[k / 3][k % 3];
   geom[24] = g.mVol;
   ssp_node_volumes(xl, &g, opensees_mass, geom + 25);
-  double D[6][6];
-  fxo_elastic_tangent(p->E, p->nu, D);
   double interior[12][12];
   ssp_interior(&g, D[0][0], D[0][1], D[3][3], interior);
   for (int m = 0; m < 4; ++m)
@@ -344,6 +344,12 @@ void fxo_ssp_element(const fxo_elastic* p, const double xe[8][3], int opensees_m
   int k = 32;
   for (int r = 0; r < 12; ++r)
     for (int q = r; q < 12; ++q) consts[k++] = interior[r][q];
+}
+
+void fxo_ssp_element(const fxo_elastic* p, const double xe[8][3], int opensees_mass, double* geom, double* consts) {
+  double D[6][6];
+  fxo_elastic_tangent(p->E, p->nu, D);
+  ssp_element_D(xe, opensees_mass, (const double(*)[6])D, geom, consts);
 }
 
 /* interior(r, q) from the packed upper triangle (ssp_brick.hpp:917-926) */
@@ -652,4 +658,164 @@ double fxo_ssp_max_omega_sq(const fxo_elastic* p, const double xe[8][3]) {
   for (int a = 0; a < 24; ++a)
     for (int b = 0; b < 24; ++b) A[a][b] *= s[a / 3] * s[b / 3];
   return fxo_max_eigenvalue_symmetric24(A);
+}
+
+/* ---- SSPbrick<PressureIndependMultiYield> ---------------------------------------------------- */
+static void pimy_element_data(const fxo_pimy_elem* p, const double xe[8][3], double geom[33], double c[110]) {
+  double D0[36];
+  fxo_pimy_initial_tangent(&p->mat, D0);
+  ssp_element_D(xe, 0, (const double(*)[6])D0, geom, c);
+}
+
+void fxo_ssp_force_pimy(const fxo_pimy_elem* p, const double xe[8][3], const double* ue, const double* ve,
+                        const double* C0, double* T0, double* fe, double* Dout) {
+  double geom[33], c[110];
+  pimy_element_data(p, xe, geom, c);
+  const fxo_rayleigh* r = &p->damp;
+  const int stiff = r->betaK != 0.0 || r->betaKinit != 0.0 || r->betaKcomm != 0.0;
+  const int damped = ve && (stiff || r->alphaM != 0.0);
+  double zu[12], zv[12];
+  term_reduce(c, ue, zu);
+  if (damped) term_reduce(c, ve, zv);
+  const double beta = r->betaK + r->betaKinit + r->betaKcomm;
+  double resid[24], fdamp[24];
+  for (int j = 0; j < 8; ++j) {
+    term_node(c, zu, j, resid + 3 * j);
+    if (damped && beta != 0.0) {
+      double kv[3];
+      term_node(c, zv, j, kv);
+      for (int q = 0; q < 3; ++q) fdamp[3 * j + q] = beta * kv[q];
+    } else {
+      for (int q = 0; q < 3; ++q) fdamp[3 * j + q] = 0.0;
+    }
+  }
+  double eps[6], sig[6], D[36], dsig[6] = {0, 0, 0, 0, 0, 0};
+  mean_strain(geom, ue, eps);
+  fxo_pimy_update(&p->mat, C0, T0, eps, sig, D);
+  if (Dout) memcpy(Dout, D, sizeof D);
+  if (damped && stiff) {
+    double Dc[36], rate[6];
+    if (r->betaKcomm != 0.0) fxo_pimy_committed_tangent(&p->mat, C0, Dc);
+    mean_strain(geom, ve, rate);
+    for (int a = 0; a < 6; ++a) {
+      double s = 0.0;
+      for (int q = 0; q < 6; ++q) {
+        double Daq = r->betaK * D[6 * a + q];
+        if (r->betaKinit != 0.0) Daq += r->betaKinit * D[6 * a + q]; /* the initial stiffness is the tangent */
+        if (r->betaKcomm != 0.0) Daq += r->betaKcomm * Dc[6 * a + q];
+        s += Daq * rate[q];
+      }
+      dsig[a] = s;
+    }
+  }
+  for (int j = 0; j < 8; ++j) {
+    double fj[3], dj[3] = {0, 0, 0};
+    node_force(geom, sig, j, fj);
+    if (damped) {
+      if (stiff) node_force(geom, dsig, j, dj);
+      if (r->alphaM != 0.0) {
+        const double m = p->mat.rho * geom[25 + j];
+        for (int q = 0; q < 3; ++q) dj[q] += r->alphaM * m * ve[3 * j + q];
+      }
+    }
+    for (int q = 0; q < 3; ++q) {
+      resid[3 * j + q] += fj[q];
+      resid[3 * j + q] -= p->b[q] * geom[25 + j];
+      fdamp[3 * j + q] += dj[q];
+    }
+  }
+  for (int a = 0; a < 24; ++a) fe[a] = damped ? resid[a] + fdamp[a] : resid[a];
+}
+
+/* D_eff = cK Dt + cc Dc (c0 = 0: the initial stiffness is the tangent) */
+static void pimy_ssp_effective(const fxo_pimy_elem* p, double cK, double cc, const double* Dt, const double* C0,
+                               double* Deff) {
+  double Dc[36];
+  if (cc != 0.0) fxo_pimy_committed_tangent(&p->mat, C0, Dc);
+  for (int a = 0; a < 36; ++a) {
+    double v = cK * Dt[a];
+    if (cc != 0.0) v += cc * Dc[a];
+    Deff[a] = v;
+  }
+}
+
+void fxo_ssp_operator_apply_pimy(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3], const double* Dt,
+                                 const double* C0, const double* ve, double* ye) {
+  double geom[33], c[110];
+  pimy_element_data(p, xe, geom, c);
+  double cK, cc, cM, cE, Deff[36];
+  ssp_factors(&p->damp, k, &cK, &cc, &cM, &cE);
+  pimy_ssp_effective(p, cK, cc, Dt, C0, Deff);
+  double eps[6], sig[6];
+  mean_strain(geom, ve, eps);
+  for (int a = 0; a < 6; ++a) {
+    double s = 0.0;
+    for (int q = 0; q < 6; ++q) s += Deff[6 * a + q] * eps[q];
+    sig[a] = s;
+  }
+  for (int j = 0; j < 8; ++j) {
+    double fj[3];
+    node_force(geom, sig, j, fj);
+    for (int q = 0; q < 3; ++q) {
+      ye[3 * j + q] = fj[q];
+      if (cM != 0.0) ye[3 * j + q] += cM * (p->mat.rho * geom[25 + j] * ve[3 * j + q]);
+    }
+  }
+  if (cE != 0.0) {
+    double z[12];
+    term_reduce(c, ve, z);
+    for (int j = 0; j < 8; ++j) {
+      double kj[3];
+      term_node(c, z, j, kj);
+      for (int q = 0; q < 3; ++q) ye[3 * j + q] += cE * kj[q];
+    }
+  }
+}
+
+void fxo_ssp_operator_diagonal_pimy(const fxo_pimy_elem* p, const fxo_coeffs* k, const double xe[8][3],
+                                    const double* Dt, const double* C0, const int32_t* eq, double* de) {
+  double geom[33], c[110];
+  pimy_element_data(p, xe, geom, c);
+  double cK, cc, cM, cE, Deff[36];
+  ssp_factors(&p->damp, k, &cK, &cc, &cM, &cE);
+  pimy_ssp_effective(p, cK, cc, Dt, C0, Deff);
+  int first[24], merged = 0;
+  for (int a = 0; a < 24; ++a) {
+    de[a] = 0.0;
+    first[a] = a;
+    if (eq[a] < 0) continue;
+    for (int b = 0; b < a; ++b)
+      if (eq[b] == eq[a]) {
+        first[a] = first[b];
+        merged = 1;
+        break;
+      }
+  }
+  for (int j = 0; j < 8; ++j) {
+    double blk[3][3];
+    block_tangent(geom, Deff, j, j, blk);
+    const double m = cM != 0.0 ? cM * (p->mat.rho * geom[25 + j]) : 0.0;
+    for (int q = 0; q < 3; ++q) de[3 * j + q] += blk[q][q] + m;
+  }
+  if (merged)
+    for (int J = 0; J < 8; ++J)
+      for (int K = 0; K < 8; ++K) {
+        double blk[3][3];
+        block_tangent(geom, Deff, J, K, blk);
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) {
+            const int ia = 3 * J + a, ib = 3 * K + b;
+            if (ia == ib || eq[ia] < 0 || eq[ia] != eq[ib]) continue;
+            de[first[ia]] += blk[a][b];
+          }
+      }
+  if (cE != 0.0) {
+    for (int a = 0; a < 24; ++a) de[a] += cE * term_entry(c, a, a);
+    if (merged)
+      for (int a = 0; a < 24; ++a)
+        for (int b = 0; b < 24; ++b) {
+          if (a == b || eq[a] < 0 || eq[a] != eq[b]) continue;
+          de[first[a]] += cE * term_entry(c, a, b);
+        }
+  }
 }

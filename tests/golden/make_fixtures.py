@@ -10,9 +10,9 @@ reads the files the hot path needs and stores them unchanged in value (float64 r
   elastic.npz      nd/ElasticIsotropic_*.txt          (20 strain paths)
   pimy.npz         nd/PressureIndependMultiYield_*.txt (34 strain paths)
   constraints.npz  constraints/cases/*.txt + constraints/*.txt (OpenSees equation classes)
-  system_pimy.npz  system/{element_std_clay_implicit,column4_pimy_std_implicit}/opensees_{disp,vel,accel}.txt
-                   (OpenSees node recorders of the implicit stdBrick<PressureIndependMultiYield>
-                   soil-column cases, one row per shaking step: time, then nodes x DOFs)
+  system_pimy.npz  system/<case>/opensees_{disp,vel,accel}.txt for the PressureIndependMultiYield
+                   soil-column cases (implicit stdBrick; SSPbrick with explicit shaking): OpenSees'
+                   node recorders, one row per shaking step: time, then nodes x DOFs
 
 Each data file becomes three arrays: "<name>/data" (rows x columns), "<name>/columns" and
 "<name>/header" (the '# key value...' lines as a JSON object). Run:
@@ -100,7 +100,8 @@ def pack_constraints(ref, out):
     print(out, len(names), "cases")
 
 
-SYSTEM_CASES = ("element_std_clay_implicit", "column4_pimy_std_implicit")
+SYSTEM_CASES = ("element_std_clay_implicit", "column4_pimy_std_implicit", "element_ssp_frictional_explicit",
+                "column4_pimy_ssp_explicit_newton", "column4_pimy_ssp_explicit_vtkhdf")
 
 
 def pack_system(ref, out):

@@ -48,6 +48,18 @@ CASES = {
     "element_std_clay_implicit": dict(nz=1, soils=("clay",), tol_host=1e-5, tol_gpu=5e-5),
     "column4_pimy_std_implicit": dict(nz=4, soils=("frictional", "clay"), tol_host=5e-6, tol_gpu=2e-5),
 }
+# SSPbrick (always lumped) with explicit shaking: ExplicitDifference at dt 5e-4 after the same
+# implicit gravity and re-equilibration stages (make_nonlinear.py:124-126; equilibration
+# ModifiedNewton -factoronce, or Newton for "newton"); tolerances nonlinear_cases.sh:19-24
+EXPLICIT_CASES = {
+    "element_ssp_frictional_explicit": dict(nz=1, soils=("frictional",), equilibration="modified", tol_host=1e-12,
+                                            tol_gpu=5e-6),
+    "column4_pimy_ssp_explicit_newton": dict(nz=4, soils=("frictional", "clay"), equilibration="newton",
+                                             tol_host=1e-12, tol_gpu=5e-6),
+    "column4_pimy_ssp_explicit_vtkhdf": dict(nz=4, soils=("frictional", "clay"), equilibration="modified",
+                                             tol_host=1e-6, tol_gpu=1e-6),
+}
+EXPLICIT_DT = 5e-4
 FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "system_pimy.npz")
 
 
@@ -67,9 +79,13 @@ def motion(length=1.0):
     return t, acc
 
 
+def case_of(case):
+    return CASES[case] if case in CASES else EXPLICIT_CASES[case]
+
+
 def model(case):
     """The case's column as a model dict (1 x 1 x nz, model node i + 2 (j + 2 k))."""
-    c = CASES[case]
+    c = case_of(case)
     nz = c["nz"]
     col = S.soil_column(nz, 1.0)
     xyz, conn, ek = S.grid(1, 1, nz, 1.0, 1.0, 0.5, z0=-nz * 0.5)
@@ -90,11 +106,13 @@ def recorded_nodes(nz):
 
 def soil_of_set(case, groups):
     """The material dict (oracle form) of each compiled parameter set, by its element group."""
-    soils = CASES[case]["soils"]
+    soils = case_of(case)["soils"]
+    ssp = case in EXPLICIT_CASES
     out = []
     for g in groups:
         m = dict(SOILS[soils[g]])
-        m.update(nsurf=20, b=(0.0, 0.0, -GRAVITY * m["rho"]), mass_type=0, rayleigh=rayleigh())
+        m.update(nsurf=20, b=(0.0, 0.0, -GRAVITY * m["rho"]), mass_type=1 if ssp else 0, rayleigh=rayleigh(),
+                 element="SSPbrick" if ssp else "stdBrick")
         out.append(m)
     return out
 
@@ -112,11 +130,11 @@ def record(node_eq, node_rank, nodes, x):
     return out.ravel()
 
 
-def shaking_steps():
+def shaking_steps(dt=DT):
     """Steps of `while {[getTime] < 0.5} {analyze 1 dt}` with the time compared as %f prints it."""
     t, n = 0.0, 0
     while float(f"{t:f}") < DURATION:
-        t += DT
+        t += dt
         n += 1
     return n
 
