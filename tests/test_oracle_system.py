@@ -26,7 +26,7 @@ def oracle_stages(case):
     Ts = Cs.copy()
     n = om.neq
     u, v, a, du = (np.zeros(n) for _ in range(4))
-    kw = dict(pimy=pe, pimy_state=[Cs, Ts])
+    kw = dict(pimy=pe, pimy_state=[Cs, Ts], nthreads=1)  # a few elements: one thread
     rc, st = om.newmark(u, v, a, du, 1.0, 30, gamma=0.6, beta=0.3025, test=0, tol=1e-7, max_iter=30, time=0.0, **kw)
     assert rc == 0 and st["steps"] == 30, st
     for p in pe:
@@ -99,7 +99,7 @@ def run_oracle_explicit(case):
         u += vh * dt
         time += dt
         ag = series(time)
-        f = om.pimy_force(pe, Cs, Ts, u, v=vh)
+        f = om.pimy_force(pe, Cs, Ts, u, v=vh, nthreads=1)
         R = -f + np.where(xdir, -m * ag, 0.0)
         an = R / m
         v = vh + (3.0 * an + a) * (dt * 0.125)
