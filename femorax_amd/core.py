@@ -525,10 +525,11 @@ class Batch:
         ctx.adopt(self)
 
     @classmethod
-    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None, fp32=False, gmres=False):
-        """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create); materials: one
-        fx_pimy_material per compiled parameter set; compact: the compacted return map;
-        rayleigh: per parameter set (alphaM, betaK, betaKinit, betaKcomm), or None."""
+    def pimy(cls, ctx, mesh, materials, compact=False, rayleigh=None, fp32=False, gmres=False, element="stdBrick"):
+        """A stdBrick<PressureIndependMultiYield> batch (fx_pimy_batch_create), or SSPbrick's with
+        element="SSPbrick"; materials: one fx_pimy_material per compiled parameter set; compact: the
+        compacted return map; rayleigh: per parameter set (alphaM, betaK, betaKinit, betaKcomm), or
+        None."""
         self = cls.__new__(cls)
         self.ctx = ctx
         self.neq = mesh.neq
@@ -551,6 +552,8 @@ class Batch:
             d.damp = C.cast(self._damp, C.c_void_p)
         h = C.c_void_p()
         flags = (FX_BATCH_COMPACT if compact else 0) | (FX_BATCH_FP32 if fp32 else 0) | (FX_BATCH_GMRES if gmres else 0)
+        if element == "SSPbrick":
+            flags |= FX_BATCH_SSPBRICK
         _check(lib().fx_pimy_batch_create(ctx.h, C.byref(d), flags, C.byref(h)), ctx)
         self.h = h
         ctx.adopt(self)

@@ -1117,6 +1117,8 @@ int fx_force(fx_batch* b, const double* u, const double* v, double* f) {
 
 int fx_critical_dt(fx_batch* b, int scheme, double* dt) {
   if (!b || !dt || (scheme != 0 && scheme != 1)) return fail(b ? b->ctx : nullptr, FX_EINVAL, "fx_critical_dt: bad argument");
+  if (b->pimy && b->ssp)
+    return fail(b->ctx, FX_EUNSUP, "fx_critical_dt: no stable step for SSPbrick<PressureIndependMultiYield> batches");
   double best = INFINITY;
   if (b->lattice && b->rect) {
     // a rectilinear lattice: each distinct (cell size, parameter set) once
@@ -1274,8 +1276,9 @@ int fx_pimy_points(fx_ctx* ctx, int32_t nmat, const fx_pimy_material* mats, cons
 int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* d, uint32_t flags, fx_batch** out) {
   if (!ctx || !d || !out || !d->params || d->nparam <= 0) return fail(ctx, FX_EINVAL, "fx_pimy_batch_create: null argument");
   *out = nullptr;
-  if (flags & FX_BATCH_SSPBRICK)
-    return fail(ctx, FX_EUNSUP, "fx_pimy_batch_create: SSPbrick<PressureIndependMultiYield> is not built");
+  if ((flags & FX_BATCH_SSPBRICK) && (flags & (FX_BATCH_COMPACT | FX_BATCH_FP32)))
+    return fail(ctx, FX_EUNSUP,
+                "fx_pimy_batch_create: SSPbrick<PressureIndependMultiYield> has no compacted return map or FP32 operator");
   std::vector<fx::PimyParams> pp(d->nparam);
   std::vector<fx_elastic_params> ep(d->nparam);
   for (int32_t i = 0; i < d->nparam; ++i) {

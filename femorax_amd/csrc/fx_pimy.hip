@@ -85,6 +85,12 @@ struct PimyBatchArgs {
   int32_t* clist;
   int32_t* ccount;
   int32_t unit_dir;  // operator: >= 0 replaces x by the rigid unit translation in that direction
+  // SSPbrick batches: the elements' constants c[k * ssp_stride + ssp_pos[e]] (dNmod, mVol, node
+  // volume weights, hourglass vectors, interior's upper triangle), Kstab from the material's initial
+  // tangent
+  const double* ssp;
+  int64_t ssp_stride;
+  const int32_t* ssp_pos;
 };
 
 // The operator's input at local DOF k: x at the DOF's equation (0 at a fixed DOF), or with unit_dir
@@ -645,6 +651,9 @@ PimyBatchArgs batch_args(fx_batch* b, const double* x, const fx_coeffs& k, const
   a.clist = b->clist;
   a.ccount = b->ccount;
   a.unit_dir = -1;
+  a.ssp = b->ssp_elem;
+  a.ssp_stride = static_cast<int64_t>(b->blk.nblocks) * kBlockElems;
+  a.ssp_pos = b->blk.pos;
   return a;
 }
 
@@ -746,12 +755,298 @@ __global__ __launch_bounds__(128) void pimy_operator32_kernel(PimyBatchArgs a, c
   for (int k = 0; k < 24; ++k) slots[k * P + e] = out[k];
 }
 
+// ---- SSPbrick<PressureIndependMultiYield>: one material point per element (the element's point 0
+// of the batch's eight-point history), one thread per element (ssp_brick.hpp:1056-1156) ----------
+struct SspRef {
+  const double* c;
+  int64_t stride, at;
+  __device__ double operator[](int k) const { return c[k * stride + at]; }
+};
+
+__device__ inline SspRef ssp_ref(const PimyBatchArgs& a, int e) {
+  return SspRef{a.ssp, a.ssp_stride, static_cast<int64_t>(a.ssp_pos[e])};
+}
+
+// eps = Bnot u, engineering shear (point_strain, ssp_brick.hpp:1018-1033)
+__device__ inline void ssp_mean_strain(const SspRef& c, const double (&u)[24], double (&eps)[6]) {
+  for (int k = 0; k < 6; ++k) eps[k] = 0.0;
+  for (int i = 0; i < 8; ++i) {
+    const double bx = c[3 * i], by = c[3 * i + 1], bz = c[3 * i + 2];
+    const double u0 = u[3 * i], u1 = u[3 * i + 1], u2 = u[3 * i + 2];
+    eps[0] += bx * u0;
+    eps[3] += by * u0;
+    eps[5] += bz * u0;
+    eps[1] += by * u1;
+    eps[3] += bx * u1;
+    eps[4] += bz * u1;
+    eps[2] += bz * u2;
+    eps[4] += by * u2;
+    eps[5] += bx * u2;
+  }
+}
+
+// mVol Bnot_j^T sigma (point_node_force, :1035-1041)
+__device__ inline void ssp_node_force(const SspRef& c, const double (&sig)[6], int j, double (&f)[3]) {
+  const double bx = c[3 * j], by = c[3 * j + 1], bz = c[3 * j + 2], V = c[24];
+  f[0] = (bx * sig[0] + by * sig[3] + bz * sig[5]) * V;
+  f[1] = (by * sig[1] + bx * sig[3] + bz * sig[4]) * V;
+  f[2] = (bz * sig[2] + by * sig[4] + bx * sig[5]) * V;
+}
+
+// interior(r, q) from its packed upper triangle
+__device__ inline double ssp_int(const SspRef& c, int r, int q) {
+  if (r > q) {
+    const int t = r;
+    r = q;
+    q = t;
+  }
+  return c[kSspInterior + 12 * r - r * (r - 1) / 2 + (q - r)];
+}
+
+// Kstab v at the 24 DOFs: z = interior (Mben v), node j gets Mben_j^T z (element_term_reduce /
+// _node, :964-989)
+__device__ inline void ssp_kstab(const SspRef& c, const double (&v)[24], double (&out)[24]) {
+  double w[12], z[12];
+  for (int m = 0; m < 4; ++m)
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      for (int i = 0; i < 8; ++i) s += c[kSspGamma + 8 * m + i] * v[3 * i + d];
+      w[3 * m + d] = s;
+    }
+  for (int r = 0; r < 12; ++r) {
+    double s = 0.0;
+    for (int q = 0; q < 12; ++q) s += ssp_int(c, r, q) * w[q];
+    z[r] = s;
+  }
+  for (int j = 0; j < 8; ++j)
+    for (int d = 0; d < 3; ++d) {
+      double s = 0.0;
+      for (int m = 0; m < 4; ++m) s += c[kSspGamma + 8 * m + j] * z[3 * m + d];
+      out[3 * j + d] = s;
+    }
+}
+
+// Kstab(a, b), in the order of (min, max) (element_term_entry, :991-1005)
+__device__ inline double ssp_kstab_entry(const SspRef& c, int a, int b) {
+  if (a > b) {
+    const int t = a;
+    a = b;
+    b = t;
+  }
+  const int i = a / 3, d = a % 3, k = b / 3, f = b % 3;
+  double s = 0.0;
+  for (int g = 0; g < 4; ++g)
+    for (int h = 0; h < 4; ++h) s += c[kSspGamma + 8 * g + i] * ssp_int(c, 3 * g + d, 3 * h + f) * c[kSspGamma + 8 * h + k];
+  return s;
+}
+
+// (mVol Bnot_J^T D Bnot_K)[p][q] (point_block_tangent, :1049-1067)
+__device__ inline double ssp_block_entry(const SspRef& c, const double (&D)[36], int J, int K, int p, int q) {
+  auto B = [&](int n, int r, int col) {  // Bnot_n[r][col]
+    const double bx = c[3 * n], by = c[3 * n + 1], bz = c[3 * n + 2];
+    switch (r) {
+      case 0: return col == 0 ? bx : 0.0;
+      case 1: return col == 1 ? by : 0.0;
+      case 2: return col == 2 ? bz : 0.0;
+      case 3: return col == 0 ? by : col == 1 ? bx : 0.0;
+      case 4: return col == 1 ? bz : col == 2 ? by : 0.0;
+      default: return col == 0 ? bz : col == 2 ? bx : 0.0;
+    }
+  };
+  const double V = c[24];
+  double s = 0.0;
+  for (int r = 0; r < 6; ++r) {
+    double bd = 0.0;
+    for (int t = 0; t < 6; ++t) bd += B(J, t, p) * (D[6 * t + r] * V);
+    s += bd * B(K, r, q);
+  }
+  return s;
+}
+
+// The force (STORE: also the point's tangent): Kstab u, then mVol Bnot^T sigma - the body force,
+// and with velocities the damping force (betaK + betaKinit + betaKcomm) Kstab v + mVol Bnot^T
+// sigma_d + alphaM m v, sigma_d = (betaK D + betaKinit D + betaKcomm D_c) Bnot v (the initial
+// stiffness is the tangent); fe = resid + fdamp (compute_with, :1107-1156).
+template <bool STORE>
+__global__ __launch_bounds__(128) void pimy_ssp_kernel(PimyBatchArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.count) return;
+  const size_t P = static_cast<size_t>(a.padded);
+  int eq[24];
+  for (int k = 0; k < 8; ++k) {
+    const int n = a.node[k * P + e];
+    for (int d = 0; d < 3; ++d) eq[3 * k + d] = a.node_eq[3 * n + d];
+  }
+  const PimyParams& p = a.params[a.param[e]];
+  const SspRef c = ssp_ref(a, e);
+  const bool stiff = stiffness_damping(p);
+  const bool damped = a.v && (stiff || p.alphaM != 0.0);
+  double u[24], v[24], resid[24], fdamp[24];
+  for (int k = 0; k < 24; ++k) {
+    u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+    v[k] = (damped && eq[k] >= 0) ? a.v[eq[k]] : 0.0;
+  }
+  ssp_kstab(c, u, resid);
+  const double beta = p.betaK + p.betaKinit + p.betaKcomm;
+  if (damped && beta != 0.0) {
+    ssp_kstab(c, v, fdamp);
+    for (int k = 0; k < 24; ++k) fdamp[k] = beta * fdamp[k];
+  } else {
+    for (int k = 0; k < 24; ++k) fdamp[k] = 0.0;
+  }
+  const int64_t pt = static_cast<int64_t>(e) * 8;
+  double eps[6], sig[6], D[36], dsig[6] = {0, 0, 0, 0, 0, 0};
+  ssp_mean_strain(c, u, eps);
+  pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D, a.touched + pt);
+  if constexpr (STORE) {
+    int w = 0;
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j, ++w) a.tangent[tangent_at(a, e, 0, w)] = D[6 * i + j];
+  }
+  if (damped && stiff) {
+    double Dc[36], rate[6];
+    if (p.betaKcomm != 0.0) pimy_committed_tangent(p, a.C + pt, a.npts, Dc);
+    ssp_mean_strain(c, v, rate);
+    for (int i = 0; i < 6; ++i) {
+      double s = 0.0;
+      for (int q = 0; q < 6; ++q) {
+        double Diq = p.betaK * D[6 * i + q];
+        if (p.betaKinit != 0.0) Diq += p.betaKinit * D[6 * i + q];
+        if (p.betaKcomm != 0.0) Diq += p.betaKcomm * Dc[6 * i + q];
+        s += Diq * rate[q];
+      }
+      dsig[i] = s;
+    }
+  }
+  double out[24];
+  for (int j = 0; j < 8; ++j) {
+    double fj[3], dj[3] = {0, 0, 0};
+    ssp_node_force(c, sig, j, fj);
+    const double wj = c[25 + j];
+    if (damped) {
+      if (stiff) ssp_node_force(c, dsig, j, dj);
+      if (p.alphaM != 0.0) {
+        const double m = p.rho * wj;
+        for (int q = 0; q < 3; ++q) dj[q] += p.alphaM * m * v[3 * j + q];
+      }
+    }
+    for (int q = 0; q < 3; ++q) {
+      double r = resid[3 * j + q];
+      r += fj[q];
+      r -= p.b[q] * wj;
+      const double fd = fdamp[3 * j + q] + dj[q];
+      out[3 * j + q] = damped ? r + fd : r;
+    }
+  }
+  for (int k = 0; k < 24; ++k) a.slots[k * P + e] = out[k];
+}
+
+// The operator on the stored tangent (DIAG: its diagonal, merged DOFs' cross terms on the first):
+// mVol Bnot^T D_eff Bnot + cM m + cE Kstab, D_eff = cK D_t + cc D_c with cK = cS + cC (betaK +
+// betaKinit), cE = cS + cC (betaK + betaKinit + betaKcomm) (element_operator.hpp:45-56 with the
+// initial stiffness taken as the tangent; element_term.hpp:34-37).
+template <bool DIAG>
+__global__ __launch_bounds__(128) void pimy_ssp_operator_kernel(PimyBatchArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.count) return;
+  const size_t P = static_cast<size_t>(a.padded);
+  int eq[24];
+  for (int k = 0; k < 8; ++k) {
+    const int n = a.node[k * P + e];
+    for (int d = 0; d < 3; ++d) eq[3 * k + d] = a.node_eq[3 * n + d];
+  }
+  const PimyParams& p = a.params[a.param[e]];
+  const SspRef c = ssp_ref(a, e);
+  const fx_coeffs& k = a.k;
+  const double cK = k.stiffness + k.damping * p.betaK + k.damping * p.betaKinit;
+  const double cc = k.damping * p.betaKcomm;
+  const double cM = k.mass + k.damping * p.alphaM;
+  const double cE = k.stiffness + k.damping * (p.betaK + p.betaKinit + p.betaKcomm);
+  const int64_t pt = static_cast<int64_t>(e) * 8;
+  double Deff[36];
+  {
+    double Dc[36];
+    if (cc != 0.0) pimy_committed_tangent(p, a.C + pt, a.npts, Dc);
+    int w = 0;
+    double Dt[36];
+    for (int i = 0; i < 6; ++i)
+      for (int j = i; j < 6; ++j, ++w) Dt[6 * i + j] = Dt[6 * j + i] = cK != 0.0 ? a.tangent[tangent_at(a, e, 0, w)] : 0.0;
+    for (int i = 0; i < 36; ++i) {
+      double v = cK * Dt[i];
+      if (cc != 0.0) v += cc * Dc[i];
+      Deff[i] = v;
+    }
+  }
+  double out[24];
+  if constexpr (DIAG) {
+    int first[24];
+    bool merged = false;
+    for (int i = 0; i < 24; ++i) {
+      out[i] = 0.0;
+      first[i] = i;
+      if (eq[i] < 0) continue;
+      for (int j = 0; j < i; ++j)
+        if (eq[j] == eq[i]) {
+          first[i] = first[j];
+          merged = true;
+          break;
+        }
+    }
+    for (int j = 0; j < 8; ++j) {
+      const double m = cM != 0.0 ? cM * (p.rho * c[25 + j]) : 0.0;
+      for (int q = 0; q < 3; ++q) out[3 * j + q] += ssp_block_entry(c, Deff, j, j, q, q) + m;
+    }
+    if (merged)
+      for (int ia = 0; ia < 24; ++ia)
+        for (int ib = 0; ib < 24; ++ib) {
+          if (ia == ib || eq[ia] < 0 || eq[ia] != eq[ib]) continue;
+          out[first[ia]] += ssp_block_entry(c, Deff, ia / 3, ib / 3, ia % 3, ib % 3);
+        }
+    if (cE != 0.0) {
+      for (int i = 0; i < 24; ++i) out[i] += cE * ssp_kstab_entry(c, i, i);
+      if (merged)
+        for (int ia = 0; ia < 24; ++ia)
+          for (int ib = 0; ib < 24; ++ib) {
+            if (ia == ib || eq[ia] < 0 || eq[ia] != eq[ib]) continue;
+            out[first[ia]] += cE * ssp_kstab_entry(c, ia, ib);
+          }
+    }
+  } else {
+    double v[24];
+    for (int i = 0; i < 24; ++i) v[i] = operator_input(a, eq, i);
+    double eps[6], sig[6];
+    ssp_mean_strain(c, v, eps);
+    for (int i = 0; i < 6; ++i) {
+      double s = 0.0;
+      for (int q = 0; q < 6; ++q) s += Deff[6 * i + q] * eps[q];
+      sig[i] = s;
+    }
+    for (int j = 0; j < 8; ++j) {
+      double fj[3];
+      ssp_node_force(c, sig, j, fj);
+      for (int q = 0; q < 3; ++q) {
+        out[3 * j + q] = fj[q];
+        if (cM != 0.0) out[3 * j + q] += cM * (p.rho * c[25 + j] * v[3 * j + q]);
+      }
+    }
+    if (cE != 0.0) {
+      double kv[24];
+      ssp_kstab(c, v, kv);
+      for (int i = 0; i < 24; ++i) out[i] += cE * kv[i];
+    }
+  }
+  for (int i = 0; i < 24; ++i) a.slots[i * P + e] = out[i];
+}
+
 // The operator (DIAG false) or its diagonal on the stored tangents into a.slots, element values
 // a * padded + e: one thread per point below kPointOperatorBelow elements, one per element above.
 template <bool DIAG>
 void launch_operator_slots(fx_batch* b, const PimyBatchArgs& a) {
   hipStream_t st = b->ctx->stream;
-  if (b->count > 0 && b->count < kPointOperatorBelow) {  // small: one thread per point
+  if (b->ssp) {
+    if (b->count > 0)
+      hipLaunchKernelGGL(pimy_ssp_operator_kernel<DIAG>, dim3((b->count + 127) / 128), dim3(128), 0, st, a);
+  } else if (b->count > 0 && b->count < kPointOperatorBelow) {  // small: one thread per point
     const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
     hipLaunchKernelGGL((pimy_point_operator_kernel<DIAG>), grid, dim3(kPointsPerBlock), 0, st, a);
   } else if (b->count > 0) {  // one element per thread
@@ -772,7 +1067,10 @@ int run_pimy(fx_batch* b, const PimyBatchArgs& a, double* y, const int32_t* done
   const dim3 grid((b->count + kElemsPerBlock - 1) / kElemsPerBlock);
   hipStream_t st = b->ctx->stream;
   if constexpr (MODE == 0) {
-    if (b->count > 0 && a.v) {  // the damping force: the fused return map, half blocks
+    if (b->ssp) {  // SSPbrick: one thread per element, its one point
+      if (b->count > 0)
+        hipLaunchKernelGGL(pimy_ssp_kernel<STORE>, dim3((b->count + 127) / 128), dim3(128), 0, st, a);
+    } else if (b->count > 0 && a.v) {  // the damping force: the fused return map, half blocks
       const dim3 g2((b->count + kElemsPerBlock / 2 - 1) / (kElemsPerBlock / 2));
       hipLaunchKernelGGL((pimy_batch_kernel<0, STORE, false, true>), g2, dim3(kPointsPerBlock / 2), 0, st, a);
     } else if (b->count > 0 && b->cres) {  // compacted: predict, correct, replay

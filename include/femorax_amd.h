@@ -420,6 +420,11 @@ typedef struct fx_pimy_batch_desc {
  * kept by the last fx_store_tangents (store_tangents, internal_force.cpp:532-543; D_eff = cK D_t +
  * c0 D_0 + cc D_c with the factors of element_operator.hpp:45-56), fx_explicit_cd commits after
  * every step. No FP32 operator. */
+/* With FX_BATCH_SSPBRICK: SSPbrick<PressureIndependMultiYield> (ssp_brick.hpp:1056-1156), one material
+ * point per element (the element's point 0 of the eight-point history layout), Kstab from the
+ * material's initial tangent (element_constants, :881), the lumped mass, the initial stiffness
+ * taken as the tangent in the operator's factors; no compacted return map, FP32 operator or
+ * stable step (FX_EUNSUP). */
 int fx_pimy_batch_create(fx_ctx* ctx, const fx_pimy_batch_desc* desc, uint32_t flags, fx_batch** out);
 /* updateMaterialStage for every parameter set (0 elastic, 1 plastic). No-op on elastic batches. */
 int fx_set_material_stage(fx_batch* b, int32_t stage);
