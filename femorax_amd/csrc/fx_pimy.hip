@@ -218,11 +218,13 @@ __device__ __forceinline__ double gauss_N(int gp, int k) {
 // each point's stress and tangent from the compacted pass instead of updating; DAMP adds the
 // Rayleigh damping force at the velocities a.v, std_brick.hpp:409-464). One thread per point.
 // A point leaves in LDS its node forces (and damping terms) and its dvol, 25 (49) values: the
-// element's first lane forms the body terms itself. Without DAMP the registers are held to 256,
-// two waves per SIMD (the return map's history loads then overlap another wave's arithmetic); the
-// damping terms need more, one wave. DAMP blocks hold half the points.
+// element's first lane forms the body terms itself. The registers are held to 256, two waves per
+// SIMD (the return map's history loads then overlap another wave's arithmetic): the element's
+// geometry is formed again after the return map instead of held through it, the tangent kept as
+// its parts (coefficient, deviator, G, K) and the damping's D, D_0 and D_c formed entry by entry.
+// DAMP blocks hold half the points.
 template <int MODE, bool STORE, bool REPLAY = false, bool DAMP = false>
-__global__ __launch_bounds__(kPointsPerBlock, DAMP ? 1 : 2) void pimy_batch_kernel(PimyBatchArgs a) {
+__global__ __launch_bounds__(kPointsPerBlock, 2) void pimy_batch_kernel(PimyBatchArgs a) {
   static_assert(MODE == 0, "the operator and its diagonal run in pimy_operator_kernel");
   constexpr int PTS = DAMP ? kPointsPerBlock / 2 : kPointsPerBlock, W = DAMP ? 6 : 3;
   // per point: 8 nodes x (3 force terms [, 3 damping terms]), then dvol
@@ -231,31 +233,42 @@ __global__ __launch_bounds__(kPointsPerBlock, DAMP ? 1 : 2) void pimy_batch_kern
   const int e = blockIdx.x * (PTS / 8) + t / 8, gp = t % 8;
   const bool live = e < a.count;
   if (live) {
-    double X[8], Y[8], Z[8];
-    int eq[24];
-    element_geometry(a, e, X, Y, Z, eq);
     const PimyParams& p = a.params[a.param[e]];
-    Shape<double> s;
-    gauss_shape(X, Y, Z, gp, s);
     const int64_t pt = static_cast<int64_t>(e) * 8 + gp;
     double* mine = part[t];
     if constexpr (MODE == 0) {
-      double sig[6], D[36];
+      // the point's tangent as its parts (entries formed where they are used, the same values)
+      double sig[6];
+      PimyTangentParts Dt{};
       if constexpr (REPLAY) {
         for (int c = 0; c < 6; ++c) sig[c] = a.cres[c * a.npts + pt];
         if constexpr (STORE)
           for (int w = 0; w < 21; ++w) a.tangent[tangent_at(a, e, gp, w)] = a.cres[(6 + w) * a.npts + pt];
       } else {
-        double u[24], eps[6];
-        for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
-        point_strain(s, u, eps);
-        pimy_update(p, a.C + pt, a.T + pt, a.npts, eps, sig, D, a.touched + pt);
+        // the strain in a scope of its own: the geometry is formed again after the return map
+        // rather than held through it (the same operations, so the same values)
+        double eps[6];
+        {
+          double X[8], Y[8], Z[8], u[24];
+          int eq[24];
+          element_geometry(a, e, X, Y, Z, eq);
+          Shape<double> s;
+          gauss_shape(X, Y, Z, gp, s);
+          for (int k = 0; k < 24; ++k) u[k] = eq[k] >= 0 ? a.x[eq[k]] : 0.0;
+          point_strain(s, u, eps);
+        }
+        pimy_update_parts(p, a.C + pt, a.T + pt, a.npts, eps, sig, Dt, a.touched + pt);
         if constexpr (STORE) {
           int w = 0;
           for (int i = 0; i < 6; ++i)
-            for (int j = i; j < 6; ++j, ++w) a.tangent[tangent_at(a, e, gp, w)] = D[6 * i + j];
+            for (int j = i; j < 6; ++j, ++w) a.tangent[tangent_at(a, e, gp, w)] = Dt.entry(p.load_stage, i, j);
         }
       }
+      double X[8], Y[8], Z[8];
+      int eq[24];
+      element_geometry(a, e, X, Y, Z, eq);
+      Shape<double> s;
+      gauss_shape(X, Y, Z, gp, s);
       for (int j = 0; j < 8; ++j) {
         double rj[3];
         node_force(s, sig, j, rj);
@@ -269,16 +282,17 @@ __global__ __launch_bounds__(kPointsPerBlock, DAMP ? 1 : 2) void pimy_batch_kern
         double vel[24], dsig[6] = {0, 0, 0, 0, 0, 0}, vgp[3] = {0, 0, 0};
         for (int k = 0; k < 24; ++k) vel[k] = eq[k] >= 0 ? a.v[eq[k]] : 0.0;
         if (stiff) {
-          double rate[6], D0[36], Dc[36];
-          if (p.betaKinit != 0.0) initial_D(p, D0);
-          if (p.betaKcomm != 0.0) pimy_committed_tangent(p, a.C + pt, a.npts, Dc);
+          // D_0 and D_c entry by entry (the same values as the matrices, without holding them)
+          double rate[6];
+          PimyTangentParts Dc{};
+          if (p.betaKcomm != 0.0) Dc = pimy_committed_tangent_parts(p, a.C + pt, a.npts);
           point_strain(s, vel, rate);
           for (int r = 0; r < 6; ++r) {
             double sum = 0.0;
             for (int c = 0; c < 6; ++c) {
-              double Drc = p.betaK * D[6 * r + c];
-              if (p.betaKinit != 0.0) Drc += p.betaKinit * D0[6 * r + c];
-              if (p.betaKcomm != 0.0) Drc += p.betaKcomm * Dc[6 * r + c];
+              double Drc = p.betaK * Dt.entry(p.load_stage, r, c);
+              if (p.betaKinit != 0.0) Drc += p.betaKinit * initial_entry(p, r, c);
+              if (p.betaKcomm != 0.0) Drc += p.betaKcomm * Dc.entry(p.load_stage, r, c);
               sum += Drc * rate[c];
             }
             dsig[r] = sum;

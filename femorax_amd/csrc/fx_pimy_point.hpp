@@ -441,10 +441,11 @@ struct Point {
     }
   }
 
-  // the 6 x 6 tangent, row-major
-  __device__ void tangent(double (&D)[36]) const {
-    double coeff = 0.;
-    V6 devia = zero6();
+  // the tangent's two parts: coeff and the deviator from the active surface's centre (getTangent
+  // :860-886), zero when elastic
+  __device__ void tangent_parts(double& coeff, V6& devia) const {
+    coeff = 0.;
+    devia = zero6();
     if (p.load_stage == 1 && active > 0) {
       const V6 cc = center(active);
       devia = trial.dev;
@@ -452,14 +453,25 @@ struct Point {
       const double sz = size(active), pm = modulus(active);
       coeff = 6. * G * G / (2. * G + pm) / sz / sz;
     }
+  }
+
+  // entry (i, j) of the tangent from its parts (the operations of tangent below)
+  __device__ static double tangent_entry(int load_stage, double coeff, const V6& devia, double G, double K, int i,
+                                         int j) {
+    double v = load_stage == 1 ? -coeff * devia.v[i] * devia.v[j] : 0.0;
+    if (i == j) v += G;
+    if (i < 3 && j < 3 && i == j) v += G;
+    if (i < 3 && j < 3) v += (K - 2. * G / 3.);
+    return v;
+  }
+
+  // the 6 x 6 tangent, row-major
+  __device__ void tangent(double (&D)[36]) const {
+    double coeff;
+    V6 devia;
+    tangent_parts(coeff, devia);
     for (int i = 0; i < 6; ++i)
-      for (int j = 0; j < 6; ++j) {
-        double v = p.load_stage == 1 ? -coeff * devia.v[i] * devia.v[j] : 0.0;
-        if (i == j) v += G;
-        if (i < 3 && j < 3 && i == j) v += G;
-        if (i < 3 && j < 3) v += (K - 2. * G / 3.);
-        D[6 * i + j] = v;
-      }
+      for (int j = 0; j < 6; ++j) D[6 * i + j] = tangent_entry(p.load_stage, coeff, devia, G, K, i, j);
   }
 
   // getStress (:455-509). With stop_at_correction (the compacted path's predictor) it returns
@@ -539,6 +551,14 @@ struct Point {
     finish(sig, D);
   }
 
+  // the same with the tangent as its parts (coeff, devia, G, K; entries by tangent_entry)
+  __device__ void update_parts(const double (&eps)[6], double (&sig)[6], double& coeff, V6& devia) {
+    get_stress(eps, false);
+    tangent_parts(coeff, devia);
+    double D[36];  // finish's tangent, unused (its stores fold away)
+    finish(sig, D);
+  }
+
   // The compacted path's predictor (pressure_independ_multi_yield.hpp:247-255): a point that
   // needs no stress correction is finished (false); otherwise true, and the corrector redoes
   // the whole update (the same function, so the bits are those of update).
@@ -562,6 +582,12 @@ __device__ inline void pimy_update(const PimyParams& p, const double* C, double*
   if (touched && pt.moved > copy) *touched = pt.moved;
 }
 
+// pimy_update with the tangent as its parts (PimyTangentParts below, load_stage p.load_stage)
+struct PimyTangentParts;
+__device__ inline void pimy_update_parts(const PimyParams& p, const double* C, double* T, int64_t stride,
+                                         const double (&eps)[6], double (&sig)[6], PimyTangentParts& parts,
+                                         int32_t* touched = nullptr);
+
 // committed_tangent (:162-168): getTangent with the committed stress, active surface, surfaces
 // and moduli (the surfaces are read from C; nothing is written).
 __device__ inline void pimy_committed_tangent(const PimyParams& p, const double* C, int64_t stride,
@@ -572,6 +598,39 @@ __device__ inline void pimy_committed_tangent(const PimyParams& p, const double*
   pt.G = pt.c(kPG);
   pt.K = pt.c(kPK);
   pt.tangent(D);
+}
+
+// The committed tangent's parts (entries by pimy::Point::tangent_entry with p.load_stage, G, K):
+// the damping force forms sigma_d entry by entry without holding the 6 x 6 matrix.
+struct PimyTangentParts {
+  double coeff, G, K;
+  pimy::V6 devia;
+  __device__ double entry(int load_stage, int i, int j) const {
+    return pimy::Point::tangent_entry(load_stage, coeff, devia, G, K, i, j);
+  }
+};
+__device__ inline void pimy_update_parts(const PimyParams& p, const double* C, double* T, int64_t stride,
+                                         const double (&eps)[6], double (&sig)[6], PimyTangentParts& parts,
+                                         int32_t* touched) {
+  const int copy = touched ? *touched : -1;
+  pimy::Point pt(p, C, T, stride, copy);
+  pt.update_parts(eps, sig, parts.coeff, parts.devia);
+  parts.G = pt.G;
+  parts.K = pt.K;
+  if (touched && pt.moved > copy) *touched = pt.moved;
+}
+
+__device__ inline PimyTangentParts pimy_committed_tangent_parts(const PimyParams& p, const double* C, int64_t stride) {
+  pimy::Point pt(p, C, const_cast<double*>(C), stride);
+  pt.trial = pt.load_t2(kPStress);
+  pt.active = static_cast<int>(pt.c(kPActive));
+  pt.G = pt.c(kPG);
+  pt.K = pt.c(kPK);
+  PimyTangentParts t;
+  pt.tangent_parts(t.coeff, t.devia);
+  t.G = pt.G;
+  t.K = pt.K;
+  return t;
 }
 
 __device__ inline bool pimy_predict(const PimyParams& p, const double* C, double* T, int64_t stride,

@@ -6,6 +6,8 @@
                                                              state (pimy_batch_kernel) on the SIDE^3 reference column
     python femorax_amd/profile_kernels.py pimy_time SIDE REPS   wall time per force call (every kernel of the call) on
                                                              the same column, elastic and yielding, fused and compacted
+                                                             (pimy_time_damped: with the column's Rayleigh damping at
+                                                             a velocity field, fused)
     python femorax_amd/profile_kernels.py general_time N REPS   wall time per apply (hex8_block_kernel and the
                                                              shared-node sum) on the distorted N^3 box, FP64 and FP32
     python femorax_amd/profile_kernels.py cg N REPS        CG + Jacobi to rtol 1e-8 on the C2 N^3 box (nodal load), the
@@ -79,30 +81,32 @@ def _pimy_materials(model, keep):
     return mats
 
 
-def pimy_time(side, reps):
+def pimy_time(side, reps, damped=False):
     import time
     model = S.reference_column(side)
     mesh = core.Mesh(model)
     ctx = core.Context(0)
     keep = []
     mats = _pimy_materials(model, keep)
+    rays = [p["rayleigh"] for p in model["params"]] if damped else None
     z = np.linspace(0.0, 1.0, mesh.neq)
-    for compact in (False, True):
-        pb = core.Batch.pimy(ctx, mesh, mats, compact=compact)
+    dv = core.DeviceArray.from_host(ctx, 1e-2 * np.cos(30.0 * z)) if damped else None
+    for compact in ((False,) if damped else (False, True)):
+        pb = core.Batch.pimy(ctx, mesh, mats, compact=compact, rayleigh=rays)
         pb.set_stage(1)
         du = core.DeviceArray.from_host(ctx, np.zeros(mesh.neq))
         df = core.DeviceArray(ctx, mesh.neq)
         for name, amp in (("elastic", 1e-7), ("yielding", 2e-3)):
             du.upload(amp * np.sin(40.0 * z))
-            pb.force(du, df)
+            pb.force(du, df, dv)
             ctx.sync()
             t0 = time.perf_counter()
             for _ in range(reps):
-                pb.force(du, df)
+                pb.force(du, df, dv)
             ctx.sync()
             us = (time.perf_counter() - t0) / reps * 1e6
-            print(f"pimy force, {mesh.nelem} elements, {'compacted' if compact else 'fused'}, {name}: {us:.1f} us",
-                  flush=True)
+            print(f"pimy force, {mesh.nelem} elements, {'compacted' if compact else 'fused'}"
+                  f"{', damped' if damped else ''}, {name}: {us:.1f} us", flush=True)
         pb.close()
     ctx.close()
 
@@ -191,4 +195,5 @@ if __name__ == "__main__":
     size = int(sys.argv[2]) if len(sys.argv) > 2 else (40 if what.startswith("pimy") else 128)
     reps = int(sys.argv[3]) if len(sys.argv) > 3 else 10
     {"general": general, "pimy": pimy, "ssp": ssp, "rect": rect, "pimy_time": pimy_time,
-     "general_time": general_time, "cg": cg, "cg_eager": lambda n, r: cg(n, r, graphs=False)}[what](size, reps)
+     "general_time": general_time, "cg": cg, "cg_eager": lambda n, r: cg(n, r, graphs=False),
+     "pimy_time_damped": lambda n, r: pimy_time(n, r, damped=True)}[what](size, reps)
