@@ -429,8 +429,7 @@ def bench_c5(core, S, duration=60.0, rate=200.0, burst=2000, plain_columns=2, pl
             "config": {"rate_per_s": rate, "duration_s": duration, "columns": "70%, N~U{20..400}, 500 steps",
                        "blocks": "30%, n~U{8..48}, rtol 1e-8", "hbm_budget_GB": 200, "block_workers": 2,
                        "pool": "one stream, 4 iterations per admission round",
-                       "blocks_served": "admission rounds: every waiting block request (one per size) in one fx_cg_many "
-                                        "call per free worker",
+                       "blocks_served": "one fx_cg per block request on a free worker",
                        "precision": "columns: FP32 inner PCG in FP64 refinement (FP64 return map and state); blocks: FP64 CG "
                                     "(latency-bound at these sizes, where the FP32 inner solves measured slower)"}}
 

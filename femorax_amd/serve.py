@@ -11,12 +11,16 @@ while earlier requests are still on the device.
   freed at once.
 - Static elastic blocks (n^3, CG + Jacobi to rtol 1e-8) go to worker threads. Each worker
   has its own context, so its own HIP stream: block solves overlap with the column iterations
-  and with each other on the device. A free worker takes every block request waiting (one per
-  block size) as one admission round and solves them together (fx_cg_many: each solve's next
-  iterations launched in turn, one synchronisation per poll round for all of them; each solve's
-  bits are those of its own fx_cg). Measured on C5 (20 s at 200/s): block p50 / p99 10.3 / 35.9
-  ms against 15.6 / 44.9 ms one solve per request, columns 740 against 907 ms p50 (fewer host
-  synchronisations compete with the pool), burst capacity 619 against 540 requests/s. A worker keeps the compiled batch of each block size it
+  and with each other on the device. A free worker takes the next block request and solves it
+  with fx_cg. The alternative admission rounds (block_rounds=True: a free worker takes every
+  block request waiting, one per block size, and solves them together with fx_cg_many, each
+  solve's next iterations launched in turn and one synchronisation per poll round for all of
+  them; each solve's bits are those of its own fx_cg) measured worse on C5 at 200 requests/s,
+  in one process on one box: blocks p50 / p99 16.6 / 59.5 ms in rounds against 10.2 / 23.7 ms
+  one solve per request over 20 s (15.7 / 43.9 ms over 60 s), columns p50 908 against 740 ms.
+  A round lasts as long as its largest block, and every block of it waits for that end. Three
+  workers wait less (block wait p99 7 against 22 ms over 60 s) at the same service times. A
+  worker keeps the compiled batch of each block size it
   has served resident, within the block share of the budget, evicting the least recently
   used; the block sizes the service is set up for (block_sizes) are made resident before the
   first request. Blocks solve with FP64 CG (fx_cg) by default: up to 48^3 (346k DoF) a solve is
@@ -238,12 +242,12 @@ class Server:
     threads, each its own stream). hbm_budget: bytes for all resident state; column_share of
     it sizes the column pool, the rest is split between the block workers. Defaults measured on
     C5 (Poisson 200/s): the pool's solve sizes on its one stream (forked side streams share the
-    process's hardware queues with the block solves and slow both), two block workers, four
-    pool iterations between admissions."""
+    process's hardware queues with the block solves and slow both), two block workers solving
+    one block request at a time, four pool iterations between admissions."""
 
     def __init__(self, device=0, hbm_budget=200e9, column_share=0.5, block_workers=2, mixed=True, dt=0.005,
                  max_columns=4096, iterations_per_call=4, keep_results=False, max_record_steps=500,
-                 one_stream=True, block_mixed=False, block_sizes=(), block_rounds=True, block_graphs=False):
+                 one_stream=True, block_mixed=False, block_sizes=(), block_rounds=False, block_graphs=False):
         self.ctx = core.Context(device)
         self.mixed, self.iters, self.keep = mixed, iterations_per_call, keep_results
         pool_elems = int(column_share * hbm_budget / COLUMN_BYTES_PER_ELEMENT)

@@ -16,7 +16,8 @@ pytestmark = pytest.mark.gpu
 DT = 0.005
 
 
-def test_each_result_is_the_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("rounds", [False, True], ids=["per_request", "rounds"])
+def test_each_result_is_the_same_alone_and_in_the_mix(rounds):
     reqs = serve.mixed_requests(duration=0.6, rate=60.0, nsteps=20, column_sizes=(10, 90), block_sizes=(4, 10),
                                 seed=4242, nrecords=3)
     kinds = {r.kind for r in reqs}
@@ -24,7 +25,7 @@ def test_each_result_is_the_same_alone_and_in_the_mix():
     # about 300 element slots for columns: later columns wait for room
     budget = 2 * 300 * serve.COLUMN_BYTES_PER_ELEMENT
     srv = serve.Server(hbm_budget=budget, column_share=0.5, block_workers=2, mixed=True, dt=DT, max_columns=8,
-                       keep_results=True, max_record_steps=20)
+                       keep_results=True, max_record_steps=20, block_rounds=rounds)
     try:
         recs = srv.run(reqs)
     finally:
