@@ -17,7 +17,7 @@ _LIB = None
 
 FX_OK = 0
 FX_EINVAL = -1
-ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP"}
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "EZERO", -5: "EUNSUP", -6: "EFULL", -7: "EIO"}
 FX_BATCH_FP32 = 1
 FX_BATCH_GENERAL = 2
 FX_BATCH_XCD_SLABS = 8
@@ -82,7 +82,7 @@ class CdDesc(C.Structure):
     _fields_ = [("dt", C.c_double), ("nsteps", C.c_int32), ("dash", C.c_void_p), ("nload", C.c_int32),
                 ("load_eq", C.c_void_p), ("load_hist", C.c_void_p), ("scheme", C.c_int32), ("ground_dir", C.c_int32),
                 ("ground_accel", C.c_void_p), ("ground_series", C.c_void_p), ("ground_factor", C.c_double),
-                ("time", C.c_double)]
+                ("time", C.c_double), ("recorder", C.c_void_p)]
 
 
 class TimeSeries(C.Structure):
@@ -98,7 +98,8 @@ class NewmarkDesc(C.Structure):
                 ("ground_series", C.c_void_p), ("ground_factor", C.c_double), ("time", C.c_double),
                 ("mixed", C.c_int32), ("inner_rtol", C.c_double), ("analysis", C.c_int32), ("algorithm", C.c_int32),
                 ("modified_initial", C.c_int32), ("factor_once", C.c_int32), ("load_increment", C.c_double),
-                ("load_min_increment", C.c_double), ("load_max_increment", C.c_double), ("load_num_iter", C.c_int32)]
+                ("load_min_increment", C.c_double), ("load_max_increment", C.c_double), ("load_num_iter", C.c_int32),
+                ("recorder", C.c_void_p)]
 
 
 class NewmarkResult(C.Structure):
@@ -121,6 +122,7 @@ class PimyMaterial(C.Structure):
 
 PIMY_NSTATE = 190
 FX_EFULL = -6
+FX_EIO = -7
 FX_COLUMN_MAX_ELEMS = 512
 FX_COLUMN_MIXED = 1
 FX_COLUMN_NO_GRAPH = 2
@@ -261,6 +263,13 @@ SIGNATURES = [
     ("fx_column_fetch", C.c_int, [vp, i32, vp, vp, vp, vp]),
     ("fx_column_state", C.c_int, [vp, i32, vp]),
     ("fx_column_remove", C.c_int, [vp, i32]),
+    ("fx_recorder_create", C.c_int, [vp, C.POINTER(vp)]),
+    ("fx_recorder_add_node", C.c_int, [vp, C.c_char_p, i32, i32, dbl, i32, i64, vp, C.POINTER(i32)]),
+    ("fx_recorder_activate", C.c_int, [vp, i32]),
+    ("fx_recorder_record", C.c_int, [vp, dbl, i64, vp, vp, vp]),
+    ("fx_recorder_finish", C.c_int, [vp]),
+    ("fx_recorder_records", i64, [vp]),
+    ("fx_recorder_destroy", None, [vp]),
 ]
 
 
@@ -452,6 +461,12 @@ class Mesh:
             np.add.at(b, q[m], nodal[m, d])
         return b
 
+    def recorder_columns(self, nodes, dofs=(1, 2, 3)):
+        """The equations a Node recorder of model nodes `nodes` and DOFs `dofs` (1-based) reads,
+        node-major (-1 for a fixed DOF; a tied DOF reads its retained equation)."""
+        eqs = self.node_eq.reshape(-1, 3)[self.node_rank[np.asarray(nodes, np.int64)]]
+        return np.ascontiguousarray(eqs[:, [d - 1 for d in dofs]].ravel(), np.int32)
+
     def to_nodes(self, x):
         """Equation values -> [nnode, 3] in model order (fixed DOFs 0)."""
         eqs = self.node_eq.reshape(-1, 3)[self.node_rank]
@@ -563,7 +578,7 @@ class Batch:
                 cg_max_it=10000, ground_dir=-1, ground_accel=None, max_retries=0, num_sublevels=0, num_substeps=2,
                 series=None, ground_factor=1.0, time=0.0, mixed=False, inner_rtol=1e-3, static=False, algorithm=0,
                 modified_initial=False, factor_once=False, load_increment=None, load_min_increment=0.0,
-                load_max_increment=0.0, load_num_iter=1):
+                load_max_increment=0.0, load_num_iter=1, recorder=None):
         """nsteps Newmark + Newton steps on device arrays u, v, a, du (in place); returns the
         fx_newmark_result as a dict. test: 0 NormUnbalance, 1 NormDispIncr, 2 EnergyIncr.
         series: the excitation's series (below), evaluated at every attempt's end time (needed to
@@ -586,7 +601,7 @@ class Batch:
                         None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time,
                         int(bool(mixed)), inner_rtol, int(bool(static)), algorithm, int(bool(modified_initial)),
                         int(bool(factor_once)), dt if load_increment is None else load_increment, load_min_increment,
-                        load_max_increment, load_num_iter)
+                        load_max_increment, load_num_iter, None if recorder is None else recorder.h)
         r = NewmarkResult()
         _check(lib().fx_newmark(self.h, C.byref(d), u.ptr, v.ptr, a.ptr, du.ptr, C.byref(r)), self.ctx)
         return {f: getattr(r, f) for f, _ in NewmarkResult._fields_}
@@ -700,10 +715,11 @@ class Batch:
         return dt.value
 
     def explicit_cd(self, dt, nsteps, u, um, v, a, dash=None, load_eq=None, load_hist=None, scheme=0, ground_dir=-1,
-                    ground_accel=None, series=None, ground_factor=1.0, time=0.0):
+                    ground_accel=None, series=None, ground_factor=1.0, time=0.0, recorder=None):
         """nsteps explicit steps on device arrays u, um, v, a (in place): scheme 0 CentralDifference
         (um = U(n-1)), 1 ExplicitDifference (um = the element velocity); a uniform excitation in
-        ground_dir from ground_accel (per step) or series ({"values", "dt"} or {"values", "time"})."""
+        ground_dir from ground_accel (per step) or series ({"values", "dt"} or {"values", "time"});
+        recorder: a Recorder fed after every step."""
         le = np.ascontiguousarray(load_eq if load_eq is not None else np.zeros(0), np.int32)
         lh = np.ascontiguousarray(load_hist if load_hist is not None else np.zeros(0), np.float64)
         ga = None if ground_accel is None else np.ascontiguousarray(ground_accel, np.float64)
@@ -718,8 +734,52 @@ class Batch:
                             None if tv is None else tv.ctypes.data, None)
         d = CdDesc(dt, nsteps, dash.ptr if dash is not None else None, len(le), le.ctypes.data if len(le) else None,
                    lh.ctypes.data if len(lh) else None, scheme, ground_dir, None if ga is None else ga.ctypes.data,
-                   None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time)
+                   None if ts is None else C.cast(C.pointer(ts), C.c_void_p), ground_factor, time,
+                   None if recorder is None else recorder.h)
         _check(lib().fx_explicit_cd(self.h, C.byref(d), u.ptr, um.ptr, v.ptr, a.ptr), self.ctx)
+
+
+class Recorder:
+    """Node recorders (fx_recorder_*): OpenSees `recorder Node` text files written from device
+    vectors by a writer thread, the values taken off the device by asynchronous copies."""
+
+    RESPONSES = {"disp": 0, "vel": 1, "accel": 2}
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = vp()
+        _check(lib().fx_recorder_create(ctx.h, C.byref(h)), ctx)
+        self.h = h
+
+    def add_node(self, file, response, eqs, time=True, dT=0.0, precision=6):
+        """A Node recorder of equations `eqs` (Mesh.recorder_columns); returns its index."""
+        e = np.ascontiguousarray(eqs, np.int32)
+        idx = C.c_int32()
+        _check(lib().fx_recorder_add_node(self.h, os.fsencode(file), self.RESPONSES[response], int(bool(time)),
+                                          float(dT), int(precision), len(e), e.ctypes.data if len(e) else None,
+                                          C.byref(idx)), self.ctx)
+        return idx.value
+
+    def activate(self, index):
+        _check(lib().fx_recorder_activate(self.h, index), self.ctx)
+
+    def record(self, time, u=None, v=None, a=None):
+        """Records every due recorder from device arrays u, v, a (one DeviceArray at least)."""
+        n = next(x.n for x in (u, v, a) if x is not None)
+        _check(lib().fx_recorder_record(self.h, float(time), n, *(None if x is None else x.ptr for x in (u, v, a))),
+               self.ctx)
+
+    def finish(self):
+        _check(lib().fx_recorder_finish(self.h), self.ctx)
+
+    @property
+    def records(self):
+        return lib().fx_recorder_records(self.h)
+
+    def close(self):
+        if self.h:
+            lib().fx_recorder_destroy(self.h)
+            self.h = None
 
 
 def copy(ctx, src, dst):

@@ -251,7 +251,7 @@ void fx_batch_destroy(fx_batch* b) {
                   b->state_c, b->state_t, b->tangent, b->tangent32, b->slots, b->gather_start, b->gather_list, b->long_eqs, b->pimy_touched, b->dyn, b->gm,
                   b->cres, b->clist, b->ccount, b->nodal_m, b->nodal_am, b->nodal_m32, b->nodal_am32, b->eq_dir_dev,
                   b->words_ref, b->shared_word_ref, b->tied_word_ref, b->orphans_ref, b->ssp_elem, b->ssp_box,
-                  b->kx, b->ky, b->kz, b->cg_state, b->cg_red};
+                  b->kx, b->ky, b->kz, b->cg_state, b->cg_red, b->internal_of_dev};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   ctx->used -= std::min<uint64_t>(ctx->used, b->device_bytes);

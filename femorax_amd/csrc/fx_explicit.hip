@@ -230,6 +230,11 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   g.dash = dash;
   g.dir = b->eq_dir_dev;
   g.gdir = d.ground_dir;
+  // node recorders after every step (explicit_analysis.cpp:724-731): u and the recorded v, a
+  const bool rec = recorder_active(d.recorder);
+  const int32_t* rmap = nullptr;
+  if (rec)
+    if (int rc = batch_internal_map(b, &rmap)) return rc;
   if (ed) {
     double* VH = Um;
     for (int32_t k = 0; k < d.nsteps; ++k) {
@@ -255,6 +260,8 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
       g.a = Acc;
       hipLaunchKernelGGL(ed_correct_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
       if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state
+      if (rec)
+        if (int rc = recorder_record(d.recorder, time, n, U, V, Acc, rmap)) return rc;
     }
   }
   for (int32_t k = 0; !ed && k < d.nsteps; ++k) {
@@ -279,12 +286,14 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
     g.um = Um;
     g.v = V;
     g.a = Acc;
-    if (k >= d.nsteps - 2)
+    if (rec || k >= d.nsteps - 2)  // a record needs this step's v, a (and so the step before's v)
       hipLaunchKernelGGL(cd_update_kernel<true>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
     else
       hipLaunchKernelGGL(cd_update_kernel<false>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
     std::swap(U, Um);  // U(n+1) sits in the old U(n-1) buffer
     if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state: the trial history becomes committed
+    if (rec)
+      if (int rc = recorder_record(d.recorder, time, n, U, V, Acc, rmap)) return rc;
   }
   FX_HIP(ctx, hipGetLastError());
   // back to the caller's buffers: u = U(n+1), um = U(n)

@@ -173,6 +173,11 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     hipLaunchKernelGGL(mul_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, st, n, b->lumped_m, SCR, MR);
     FX_HIP(ctx, hipGetLastError());
   }
+  // node recorders after every commit (implicit_stage.cpp:371-381)
+  const bool rec = recorder_active(d.recorder);
+  const int32_t* rmap = nullptr;
+  if (rec)
+    if (int rc = batch_internal_map(b, &rmap)) return rc;
   const double gamma = d.gamma, beta = d.beta;
   const bool is_static = d.analysis == 1;
   const bool linear = d.algorithm == 1, modified = d.algorithm == 2;
@@ -324,6 +329,8 @@ int newmark_run(fx_batch* b, const fx_newmark_desc& d, double* u, double* v, dou
     out.max_newton_iterations = std::max(out.max_newton_iterations, iterations);
     ++out.commits;
     if (b->pimy) std::swap(b->state_c, b->state_t);  // commit_state
+    if (rec)  // static stages record the committed velocity, which they keep
+      if (int rc = recorder_record(d.recorder, out.time, n, U, V, A, rmap)) return rc;
     return FX_OK;
   };
   // recover (:385-402): retries at the same size, then substeps one level down

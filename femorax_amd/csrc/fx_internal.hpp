@@ -311,6 +311,7 @@ struct fx_batch {
   double* lumped_am = nullptr;
   double* body = nullptr;
   std::vector<int32_t> internal_of;   // host: reference equation -> internal (lattice batches)
+  int32_t* internal_of_dev = nullptr; // its device copy, made for the first recorded stage
   // host copies for the stable time step of general batches (lattice batches need none)
   std::vector<int32_t> host_node, host_param;
   std::vector<double> host_coords;

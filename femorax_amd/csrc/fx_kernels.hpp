@@ -116,6 +116,13 @@ int pimy_diagonal64(fx_batch* b, const fx_coeffs& k, double* d);
 // Internal force and the CentralDifference loop (fx_explicit.hip), solver numbering.
 int force64(fx_batch* b, const double* u, const double* v, double* f, double* scratch);
 int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, double* a);
+// Node recorders (fx_recorder.hip): whether r records anything; the device map from the model
+// numbering to the batch's solver numbering (null when they are the same); a record of the
+// solver-numbered vectors u, v, a of n equations at `time` (nothing when r is null).
+bool recorder_active(const fx_recorder* r);
+int batch_internal_map(fx_batch* b, const int32_t** map);
+int recorder_record(fx_recorder* r, double time, int64_t n, const double* u, const double* v, const double* a,
+                    const int32_t* map);
 
 // z = x * y elementwise (fx_implicit.hip).
 int multiply64(fx_ctx* ctx, int32_t n, const double* x, const double* y, double* z);

@@ -36,10 +36,12 @@ extern "C" {
 #define FX_EZERO -4    /* Jacobi: zero diagonal entries (jacobi.cpp:19-22) */
 #define FX_EUNSUP -5   /* unsupported element/material/option */
 #define FX_EFULL -6    /* a column pool has no room for the column now (retry after columns finish) */
+#define FX_EIO -7      /* a recorder file could not be opened or written */
 
 typedef struct fx_ctx fx_ctx;
 typedef struct fx_mesh fx_mesh;
 typedef struct fx_batch fx_batch;
+typedef struct fx_recorder fx_recorder;
 
 /* ---- context: one device, one stream, an HBM budget (exec/device_memory.cpp:26-38) ------- */
 /* hbm_budget_bytes = 0: 90% of free memory at creation (explicit_analysis.cpp:446-454). */
@@ -357,6 +359,8 @@ typedef struct fx_cd_desc {
   const fx_time_series* ground_series; /* the excitation's series, evaluated at those times, or NULL */
   double ground_factor;      /* UniformExcitation factor applied to ground_series */
   double time;               /* the stage's time before the first step */
+  fx_recorder* recorder;     /* NULL, or records u, the recorded v and a after every step
+                                (explicit_analysis.cpp:724-731); equations in the model numbering */
 } fx_cd_desc;
 /* nsteps explicit steps with the batch's lumped mass (the nodal masses included), the material
  * history committed after every step (Domain::commit), and the ground load -m a_g of a uniform
@@ -477,6 +481,8 @@ typedef struct fx_newmark_desc {
                                  given (the increment stays fixed) */
   int32_t load_num_iter;      /* LoadControl numIter: with min and max given, dlambda *= numIter / (the last
                                  attempt's iterations), clamped to [min, max] (:262-269, 277-283) */
+  fx_recorder* recorder;      /* NULL, or records u, v, a after every commit, substeps included
+                                 (implicit_stage.cpp:371-381); equations in the model numbering */
 } fx_newmark_desc;
 typedef struct fx_newmark_result {
   int32_t steps;              /* converged steps of size dt (each may have been substepped) */
@@ -595,6 +601,34 @@ int fx_column_fetch(fx_column_pool* p, int32_t id, double* u, double* v, double*
 int fx_column_state(fx_column_pool* p, int32_t id, double* state);
 /* Frees the column's slots (a running column is stopped). */
 int fx_column_remove(fx_column_pool* p, int32_t id);
+
+/* Node recorders (exec/recorders.hpp:5-20; recorders.cpp:193-240, 272-298): OpenSees'
+ * `recorder Node -file f [-time] [-dT dT] [-precision p] -node ... -dof ... disp|vel|accel`, fed
+ * from device vectors. A recorder is due after a commit at `time` when dT == 0 or
+ * time - next >= -dT * 1e-5 (then next = time + dT); a record is one line: the time (echo_time),
+ * then each column's value, every number printed as %.{precision}g, separated by one space. A
+ * column is an equation of the model numbering, or -1 for a fixed DOF (printed 0). At a record
+ * one gather kernel per due recorder copies its columns into a device staging array, an
+ * asynchronous copy on the context's stream takes them to pinned host memory (two buffers: the
+ * host waits for the older copy only when both are in use), and a writer thread formats and
+ * writes the files. */
+int fx_recorder_create(fx_ctx* ctx, fx_recorder** out);
+/* Adds a Node recorder (response 0 disp, 1 vel, 2 accel; ncolumns equations, node-major with the
+ * DOFs of each node); *index names it. Every recorder is added before the first activation. */
+int fx_recorder_add_node(fx_recorder* r, const char* file, int32_t response, int32_t echo_time, double dT,
+                         int32_t precision, int64_t ncolumns, const int32_t* eq, int32_t* index);
+/* Starts recording (the reference's `recorder` process step, recorders.cpp:181-193): its file is
+ * created or truncated by the writer. */
+int fx_recorder_activate(fx_recorder* r, int32_t index);
+/* After a committed step at `time`: records every active recorder that is due, from device
+ * vectors of n equations in the model numbering (a vector nobody due records may be NULL).
+ * Asynchronous: the stream is not synchronised unless both staging buffers are in use. */
+int fx_recorder_record(fx_recorder* r, double time, int64_t n, const double* u, const double* v, const double* a);
+/* Writes every pending record and closes the files; FX_EIO names a file that failed. Records
+ * after it are refused. */
+int fx_recorder_finish(fx_recorder* r);
+int64_t fx_recorder_records(const fx_recorder* r);  /* lines written so far */
+void fx_recorder_destroy(fx_recorder* r);           /* finishes first */
 
 #ifdef __cplusplus
 }

@@ -13,6 +13,9 @@ reads the files the hot path needs and stores them unchanged in value (float64 r
   system_pimy.npz  system/<case>/opensees_{disp,vel,accel}.txt for the PressureIndependMultiYield
                    soil-column cases (implicit stdBrick; SSPbrick with explicit shaking): OpenSees'
                    node recorders, one row per shaking step: time, then nodes x DOFs
+  recorder_text.json  the recorders' text itself: the time tokens of
+                   system/column25_ssp_explicit/opensees_disp.txt (-dT 0.01 at dt 2.5e-4) and the
+                   first three lines of system/column4_pimy_std_implicit/opensees_disp.txt
 
 Each data file becomes three arrays: "<name>/data" (rows x columns), "<name>/columns" and
 "<name>/header" (the '# key value...' lines as a JSON object). Run:
@@ -114,6 +117,19 @@ def pack_system(ref, out):
     print(out, len(SYSTEM_CASES), "cases")
 
 
+def pack_recorder_text(ref, out):
+    sysdir = os.path.join(ref, "tests/reference/data/system")
+    with open(os.path.join(sysdir, "column25_ssp_explicit", "opensees_disp.txt")) as f:
+        times = [line.split(" ", 1)[0] for line in f]
+    with open(os.path.join(sysdir, "column4_pimy_std_implicit", "opensees_disp.txt")) as f:
+        head = [f.readline() for _ in range(3)]
+    doc = {"column25_ssp_explicit_times": {"dt": 2.5e-4, "dT": 0.01, "precision": 17, "times": times},
+           "column4_pimy_std_implicit_disp_head": {"precision": 17, "lines": head}}
+    with open(os.path.join(HERE, out), "w") as f:
+        json.dump(doc, f, indent=1)
+    print(out, len(times), "times", len(head), "lines")
+
+
 def main():
     ref = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
     data = os.path.join(ref, "tests/reference/data")
@@ -129,6 +145,7 @@ def main():
          "pimy.npz")
     pack_constraints(ref, "constraints.npz")
     pack_system(ref, "system_pimy.npz")
+    pack_recorder_text(ref, "recorder_text.json")
 
 
 if __name__ == "__main__":
