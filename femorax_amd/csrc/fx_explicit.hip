@@ -231,7 +231,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
   g.dir = b->eq_dir_dev;
   g.gdir = d.ground_dir;
   // node recorders after every step (explicit_analysis.cpp:724-731): u and the recorded v, a
-  const bool rec = recorder_active(d.recorder);
+  const bool rec = recorder_active(d.recorder), rec_rates = recorder_reads_rates(d.recorder);
   const int32_t* rmap = nullptr;
   if (rec)
     if (int rc = batch_internal_map(b, &rmap)) return rc;
@@ -286,7 +286,9 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
     g.um = Um;
     g.v = V;
     g.a = Acc;
-    if (rec || k >= d.nsteps - 2)  // a record needs this step's v, a (and so the step before's v)
+    // a record of v or a needs this step's (and so the step before's v): the full update every
+    // step, as the reference's cd_solve always does
+    if (rec_rates || k >= d.nsteps - 2)
       hipLaunchKernelGGL(cd_update_kernel<true>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);
     else
       hipLaunchKernelGGL(cd_update_kernel<false>, dim3(blocks_for(n)), dim3(kThreads), 0, ctx->stream, g);

@@ -120,6 +120,7 @@ int cd_run(fx_batch* b, const fx_cd_desc& d, double* u, double* um, double* v, d
 // numbering to the batch's solver numbering (null when they are the same); a record of the
 // solver-numbered vectors u, v, a of n equations at `time` (nothing when r is null).
 bool recorder_active(const fx_recorder* r);
+bool recorder_reads_rates(const fx_recorder* r);  // an active recorder of velocities or accelerations
 int batch_internal_map(fx_batch* b, const int32_t** map);
 int recorder_record(fx_recorder* r, double time, int64_t n, const double* u, const double* v, const double* a,
                     const int32_t* map);

@@ -10,9 +10,9 @@
 //   - a column is a node DOF: its equation's value, or 0 for a fixed DOF.
 // Here: at a record, one gather kernel per due recorder copies its columns into a device
 // staging array and only those values go to pinned host memory by an asynchronous copy on the
-// context's stream, into one of two buffers (the host waits for the older copy only when both
-// are in use). A writer thread formats and writes the files, so the stream never waits for the
-// disk.
+// context's stream, into one of a ring of buffers (the host waits for a copy only when every
+// buffer holds a record in flight). A writer thread formats and writes the files, so the stream
+// never waits for the disk.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,17 +36,33 @@ int rec_fail(fx_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
-// stage[off + j] = src[map ? map[eq[j]] : eq[j]] for a free DOF, 0 for a fixed one (eq < 0)
-__global__ __launch_bounds__(256) void recorder_gather_kernel(int64_t ncol, const int32_t* __restrict__ eq,
-                                                              const int32_t* __restrict__ map,
-                                                              const double* __restrict__ src,
+// The due recorders of one record (up to kGatherSegs per launch): recorder s copies its columns
+// into stage[off[s] ...]: src[s][map ? map[eq[j]] : eq[j]] for a free DOF, 0 for a fixed one.
+constexpr int kGatherSegs = 8;
+struct GatherSegs {
+  const int32_t* eq[kGatherSegs];
+  const double* src[kGatherSegs];
+  int64_t off[kGatherSegs];
+  int64_t cols[kGatherSegs];
+};
+
+__global__ __launch_bounds__(256) void recorder_gather_kernel(GatherSegs g, const int32_t* __restrict__ map,
                                                               double* __restrict__ stage) {
-  const int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j >= ncol) return;
-  int32_t q = eq[j];
-  if (q >= 0 && map) q = map[q];
-  stage[j] = q >= 0 ? src[q] : 0.0;
+  const int s = blockIdx.y;
+  const int32_t* __restrict__ eq = g.eq[s];
+  const double* __restrict__ src = g.src[s];
+  double* __restrict__ dst = stage + g.off[s];
+  for (int64_t j = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; j < g.cols[s];
+       j += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    int32_t q = eq[j];
+    if (q >= 0 && map) q = map[q];
+    dst[j] = q >= 0 ? src[q] : 0.0;
+  }
 }
+
+// Records in flight: one pinned buffer and event each, as many as fit 64 MiB (2 to 32).
+constexpr int kMaxSlots = 32;
+constexpr size_t kPinnedBudget = size_t(64) << 20;
 
 }  // namespace
 
@@ -79,8 +95,9 @@ struct fx_recorder {
   bool started = false, finished = false, any_active = false;
   int64_t total = 0;
   double* stage = nullptr;
-  double* pinned[2] = {nullptr, nullptr};
-  hipEvent_t copied[2] = {nullptr, nullptr};
+  int nslots = 0;
+  double* pinned[kMaxSlots] = {};
+  hipEvent_t copied[kMaxSlots] = {};
   int next_slot = 0;
   std::deque<Pending> pending;
   std::atomic<int64_t> records{0};
@@ -178,8 +195,9 @@ struct fx_recorder {
       total += r.columns;
     }
     const size_t sz = sizeof(double) * static_cast<size_t>(std::max<int64_t>(total, 1));
+    nslots = static_cast<int>(std::min<size_t>(kMaxSlots, std::max<size_t>(2, kPinnedBudget / sz)));
     FX_HIP(ctx, hipMalloc(reinterpret_cast<void**>(&stage), sz));
-    for (int s = 0; s < 2; ++s) {
+    for (int s = 0; s < nslots; ++s) {
       FX_HIP(ctx, hipHostMalloc(reinterpret_cast<void**>(&pinned[s]), sz, hipHostMallocDefault));
       FX_HIP(ctx, hipEventCreateWithFlags(&copied[s], hipEventDisableTiming));
     }
@@ -212,22 +230,50 @@ struct fx_recorder {
     }
     if (!started)
       if (int rc = start()) return rc;
-    if (pending.size() >= 2)
+    // hand the records whose copies have landed to the writer; wait only when every slot is in use
+    while (!pending.empty()) {
+      const hipError_t q = hipEventQuery(copied[pending.front().slot]);
+      if (q == hipErrorNotReady) break;
+      FX_HIP(ctx, q);
+      if (int rc = flush_oldest()) return rc;
+    }
+    if (static_cast<int>(pending.size()) >= nslots)
       if (int rc = flush_oldest()) return rc;
     const int slot = next_slot;
-    next_slot ^= 1;
+    next_slot = (next_slot + 1) % nslots;
     hipStream_t s = ctx->stream;
+    // one gather launch per kGatherSegs due recorders, one copy of the span they fill
+    int64_t lo = total, hi = 0, maxcols = 0;
+    GatherSegs g{};
+    int nseg = 0;
+    auto launch = [&]() -> int {
+      if (nseg == 0) return FX_OK;
+      const int64_t blocks = std::min<int64_t>((maxcols + 255) / 256, 1024);
+      hipLaunchKernelGGL(recorder_gather_kernel, dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(nseg)),
+                         dim3(256), 0, s, g, map, stage);
+      FX_HIP(ctx, hipGetLastError());
+      nseg = 0;
+      maxcols = 0;
+      return FX_OK;
+    };
     for (const auto& [i, t] : due) {
       const Node& r = recs[static_cast<size_t>(i)];
       if (r.columns == 0) continue;
-      const double* src = r.response == 0 ? u : r.response == 1 ? v : a;
-      const unsigned grid = static_cast<unsigned>((r.columns + 255) / 256);
-      hipLaunchKernelGGL(recorder_gather_kernel, dim3(grid), dim3(256), 0, s, r.columns, r.eq_dev, map, src,
-                         stage + r.offset);
-      FX_HIP(ctx, hipGetLastError());
-      FX_HIP(ctx, hipMemcpyAsync(pinned[slot] + r.offset, stage + r.offset, sizeof(double) * r.columns,
-                                 hipMemcpyDeviceToHost, s));
+      g.eq[nseg] = r.eq_dev;
+      g.src[nseg] = r.response == 0 ? u : r.response == 1 ? v : a;
+      g.off[nseg] = r.offset;
+      g.cols[nseg] = r.columns;
+      ++nseg;
+      maxcols = std::max(maxcols, r.columns);
+      lo = std::min(lo, r.offset);
+      hi = std::max(hi, r.offset + r.columns);
+      if (nseg == kGatherSegs)
+        if (int rc = launch()) return rc;
     }
+    if (int rc = launch()) return rc;
+    if (hi > lo)
+      FX_HIP(ctx, hipMemcpyAsync(pinned[slot] + lo, stage + lo, sizeof(double) * static_cast<size_t>(hi - lo),
+                                 hipMemcpyDeviceToHost, s));
     FX_HIP(ctx, hipEventRecord(copied[slot], s));
     Pending p;
     p.slot = slot;
@@ -255,6 +301,13 @@ struct fx_recorder {
 
 namespace fx {
 bool recorder_active(const fx_recorder* r) { return r && r->any_active && !r->finished; }
+
+bool recorder_reads_rates(const fx_recorder* r) {
+  if (!recorder_active(r)) return false;
+  for (const auto& nd : r->recs)
+    if (nd.active && nd.response != 0) return true;
+  return false;
+}
 
 int batch_internal_map(fx_batch* b, const int32_t** map) {
   *map = nullptr;
@@ -353,7 +406,7 @@ void fx_recorder_destroy(fx_recorder* r) {
   for (auto& nd : r->recs)
     if (nd.eq_dev) (void)hipFree(nd.eq_dev);
   if (r->stage) (void)hipFree(r->stage);
-  for (int s = 0; s < 2; ++s) {
+  for (int s = 0; s < kMaxSlots; ++s) {
     if (r->pinned[s]) (void)hipHostFree(r->pinned[s]);
     if (r->copied[s]) (void)hipEventDestroy(r->copied[s]);
   }

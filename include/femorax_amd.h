@@ -608,10 +608,10 @@ int fx_column_remove(fx_column_pool* p, int32_t id);
  * time - next >= -dT * 1e-5 (then next = time + dT); a record is one line: the time (echo_time),
  * then each column's value, every number printed as %.{precision}g, separated by one space. A
  * column is an equation of the model numbering, or -1 for a fixed DOF (printed 0). At a record
- * one gather kernel per due recorder copies its columns into a device staging array, an
- * asynchronous copy on the context's stream takes them to pinned host memory (two buffers: the
- * host waits for the older copy only when both are in use), and a writer thread formats and
- * writes the files. */
+ * one gather launch copies the due recorders' columns into a device staging array, one
+ * asynchronous copy on the context's stream takes them to pinned host memory (a ring of 2 to 32
+ * buffers within 64 MiB: the host waits for a copy only when every buffer holds a record in
+ * flight), and a writer thread formats and writes the files. */
 int fx_recorder_create(fx_ctx* ctx, fx_recorder** out);
 /* Adds a Node recorder (response 0 disp, 1 vel, 2 accel; ncolumns equations, node-major with the
  * DOFs of each node); *index names it. Every recorder is added before the first activation. */
@@ -622,7 +622,7 @@ int fx_recorder_add_node(fx_recorder* r, const char* file, int32_t response, int
 int fx_recorder_activate(fx_recorder* r, int32_t index);
 /* After a committed step at `time`: records every active recorder that is due, from device
  * vectors of n equations in the model numbering (a vector nobody due records may be NULL).
- * Asynchronous: the stream is not synchronised unless both staging buffers are in use. */
+ * Asynchronous: the host waits only when every pinned buffer holds a record in flight. */
 int fx_recorder_record(fx_recorder* r, double time, int64_t n, const double* u, const double* v, const double* a);
 /* Writes every pending record and closes the files; FX_EIO names a file that failed. Records
  * after it are refused. */
