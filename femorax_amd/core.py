@@ -326,6 +326,15 @@ class Context:
         """CG iterations from captured hipGraphs, or launched one by one (default)."""
         _check(lib().fx_ctx_set_option(self.h, 1, int(bool(enable))), self)
 
+    def set_stream(self, handle):
+        """Queue the context's work on the caller's HIP stream (an integer hipStream_t, e.g.
+        torch.cuda.Stream().cuda_stream), or on its own again with 0."""
+        _check(lib().fx_ctx_set_option(self.h, 2, int(handle)), self)
+
+    def set_stream_ordered(self, enable):
+        """The drop-in operator calls return once queued (results ordered on the stream)."""
+        _check(lib().fx_ctx_set_option(self.h, 3, int(bool(enable))), self)
+
     def memory(self):
         f, t = C.c_uint64(), C.c_uint64()
         _check(lib().fx_device_memory(self.h, C.byref(f), C.byref(t)), self)

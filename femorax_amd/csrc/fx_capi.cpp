@@ -100,6 +100,8 @@ int fx_ctx_create(int device, uint64_t hbm_budget_bytes, fx_ctx** out) {
     delete c;
     return fail(nullptr, FX_EDEVICE, "hipStreamCreate failed");
   }
+  c->own_stream = c->stream;
+
   size_t fr = 0, tot = 0;
   (void)hipMemGetInfo(&fr, &tot);
   c->budget = hbm_budget_bytes ? hbm_budget_bytes : static_cast<uint64_t>(fr / 10 * 9);
@@ -130,7 +132,8 @@ void fx_ctx_destroy(fx_ctx* ctx) {
   if (ctx->red) (void)hipFree(ctx->red);
   if (ctx->scalars) (void)hipFree(ctx->scalars);
   if (ctx->pimy_params) (void)hipFree(ctx->pimy_params);
-  (void)hipStreamDestroy(ctx->stream);
+  (void)hipStreamSynchronize(ctx->own_stream);
+  (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
 }
 
@@ -140,6 +143,17 @@ int fx_ctx_set_option(fx_ctx* ctx, int option, int64_t value) {
     case FX_OPT_GRAPHS:
       ctx->use_graphs = value != 0;
       if (!ctx->use_graphs) fx::release_cg_graphs(ctx, nullptr);
+      return FX_OK;
+    case FX_OPT_STREAM: {
+      // the work queued so far completes first; graphs captured on the old stream are dropped
+      FX_HIP(ctx, hipSetDevice(ctx->device));
+      FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+      fx::release_cg_graphs(ctx, nullptr);
+      ctx->stream = value ? reinterpret_cast<hipStream_t>(static_cast<intptr_t>(value)) : ctx->own_stream;
+      return FX_OK;
+    }
+    case FX_OPT_STREAM_ORDERED:
+      ctx->stream_ordered = value != 0;
       return FX_OK;
     default:
       return fail(ctx, FX_EINVAL, "fx_ctx_set_option: unknown option");
@@ -909,7 +923,7 @@ int fx_run_operator(fx_batch* b, int diagonal, const fx_coeffs* k, int out, cons
     rc = fx::add_elements64(b, diagonal != 0, *k, x, y);
   }
   if (rc) return rc;
-  FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (!ctx->stream_ordered) FX_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return FX_OK;
 }
 

@@ -141,6 +141,8 @@ struct fx_ctx {
   void* host_pinned = nullptr;
   void* pimy_params = nullptr;  // material-point driver: up to kMaxPointMaterials parameter sets
   bool use_graphs = false;      // CG iterations replayed from captured graphs (FX_OPT_GRAPHS)
+  hipStream_t own_stream = nullptr;  // the stream the context created (stream is the caller's with FX_OPT_STREAM)
+  bool stream_ordered = false;  // the drop-in operator calls return without synchronising (FX_OPT_STREAM_ORDERED)
   std::vector<fx_cg_graph> cg_graphs;
 };
 

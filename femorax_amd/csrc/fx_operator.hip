@@ -758,7 +758,7 @@ int apply_ref(fx_batch* b, const fx_coeffs& k, const T* x, T* y) {
     hipLaunchKernelGGL(zero_list_kernel<T>, dim3((b->norphans + 255) / 256), dim3(256), 0, b->ctx->stream, b->norphans,
                        b->orphans_ref, y);
   FX_HIP(b->ctx, hipGetLastError());
-  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  if (!b->ctx->stream_ordered) FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return FX_OK;
 }
 
@@ -773,7 +773,7 @@ int external_call(fx_batch* b, const T* x, T* y, T* xi, T* yi, Op op) {
   } else {
     if (int rc = op(x, y)) return rc;
   }
-  FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
+  if (!b->ctx->stream_ordered) FX_HIP(b->ctx, hipStreamSynchronize(b->ctx->stream));
   return FX_OK;
 }
 }  // namespace

@@ -56,6 +56,17 @@ int fx_sync(fx_ctx* ctx);
  *   no other thread to use the null stream meanwhile (serving loops with upload threads keep it
  *   off). */
 #define FX_OPT_GRAPHS 1
+/* FX_OPT_STREAM: the context's work goes to the caller's HIP stream (value: the hipStream_t, 0 to
+ *   return to the context's own), e.g. the Kokkos HIP execution space's stream, so that the
+ *   library's kernels and the caller's are ordered as the reference's Kokkos launches are. The
+ *   work queued so far completes first. The caller keeps the stream alive and at the default
+ *   priority.
+ * FX_OPT_STREAM_ORDERED (default 0): 1 makes the drop-in operator calls (fx_operator_apply,
+ *   fx_operator_apply_f32, fx_operator_diagonal, fx_run_operator) return once their kernels are
+ *   queued, as Kokkos launches do; their results are ordered on the context's stream (fx_sync, or
+ *   later work on that stream, sees them). Default: they return with the results in place. */
+#define FX_OPT_STREAM 2
+#define FX_OPT_STREAM_ORDERED 3
 int fx_ctx_set_option(fx_ctx* ctx, int option, int64_t value);
 /* Free and total device memory (src/native/device_memory_hip.cpp:6-11). */
 int fx_device_memory(fx_ctx* ctx, uint64_t* free_bytes, uint64_t* total_bytes);

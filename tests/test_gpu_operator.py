@@ -429,3 +429,42 @@ def test_general_path_staged_and_gathered_blocks(ctx):
     du, dv, df = core.DeviceArray.from_host(ctx, u), core.DeviceArray.from_host(ctx, v), core.DeviceArray(ctx, pm.neq)
     batch.force(du, df, dv)
     assert rel_err(df.get(), om.force(op, u, v)) <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["c2_layered16", "distorted6"])
+def test_caller_stream_and_stream_ordered_dropin(ctx, name):
+    """FX_OPT_STREAM (the context's work on a stream the caller made) and FX_OPT_STREAM_ORDERED
+    (the drop-in calls return once queued): the drop-in apply and a CG solve give the bits of the
+    blocking calls on the context's own stream."""
+    import ctypes as C
+
+    core.lib()  # the library and the HIP runtime it links are loaded first
+    hip = C.CDLL("libamdhip64.so.7")
+    stream = C.c_void_p()
+    assert hip.hipStreamCreate(C.byref(stream)) == 0
+    model = CASES[name]()
+    pm, batch, _, _ = setup(ctx, model)
+    n = pm.neq
+    x = core.DeviceArray.from_host(ctx, np.sin(0.37 * np.arange(n)))
+    b = core.DeviceArray.from_host(ctx, np.cos(0.11 * np.arange(n)))
+    y0, y1, s0, s1 = (core.DeviceArray(ctx, n) for _ in range(4))
+    try:
+        batch.apply(x, y0)
+        s0.zero()
+        r0 = batch.cg(b, s0, rtol=1e-10)
+        ctx.set_stream(stream.value)
+        ctx.set_stream_ordered(True)
+        for _ in range(3):
+            batch.apply(x, y1)
+        assert hip.hipStreamSynchronize(stream) == 0
+        np.testing.assert_array_equal(y1.get(), y0.get())
+        s1.zero()
+        r1 = batch.cg(b, s1, rtol=1e-10)
+        assert hip.hipStreamSynchronize(stream) == 0
+        assert r1["iterations"] == r0["iterations"] and r0["converged"]
+        np.testing.assert_array_equal(s1.get(), s0.get())
+    finally:
+        ctx.set_stream_ordered(False)
+        ctx.set_stream(0)
+        batch.close()
+        hip.hipStreamDestroy(stream)
