@@ -170,6 +170,9 @@ def bench_refcol(core, S, ctx, side, reps):
             pb.force(dup, dfp)
         ctx.sync()
         res[f"pimy_force_{name}_us"] = round((time.perf_counter() - t0) / 5 * 1e6, 1)
+    res["pimy_force_note"] = ("host wall time per blocking fx_force call (5 calls after one untimed): the return-map "
+                              "kernel, the slot gather and the synchronisation; the kernels alone are in "
+                              "profiles/pimy_force_column40_trace_and_counters.md")
     active = pb.pimy_state(False)[:, 26]
     res["pimy_yielding_points_frac"] = round(float(np.mean(active > 0)), 3)
     if side == 40:
