@@ -170,9 +170,11 @@ def bench_refcol(core, S, ctx, side, reps):
             pb.force(dup, dfp)
         ctx.sync()
         res[f"pimy_force_{name}_us"] = round((time.perf_counter() - t0) / 5 * 1e6, 1)
-    res["pimy_force_note"] = ("host wall time per blocking fx_force call (5 calls after one untimed): the return-map "
-                              "kernel, the slot gather and the synchronisation; the kernels alone are in "
-                              "profiles/pimy_force_column40_trace_and_counters.md")
+    res["pimy_force_note"] = ("host wall time per blocking fx_force call (5 calls after one untimed) on this batch "
+                              "(two Rayleigh regions, FP32 tangent copy, material stage 1, no velocities): the "
+                              "return-map kernel, the two gathers and the synchronisation; a kernel trace of this "
+                              "leg measured the return-map kernel at 262-268 us (elastic) and 1,236-1,261 us "
+                              "(yielding), the gathers at about 10 us each")
     active = pb.pimy_state(False)[:, 26]
     res["pimy_yielding_points_frac"] = round(float(np.mean(active > 0)), 3)
     if side == 40:
