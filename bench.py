@@ -173,8 +173,9 @@ def bench_refcol(core, S, ctx, side, reps):
     res["pimy_force_note"] = ("host wall time per blocking fx_force call (5 calls after one untimed) on this batch "
                               "(two Rayleigh regions, FP32 tangent copy, material stage 1, no velocities): the "
                               "return-map kernel, the two gathers and the synchronisation; a kernel trace of this "
-                              "leg measured the return-map kernel at 262-268 us (elastic) and 1,236-1,261 us "
-                              "(yielding), the gathers at about 10 us each")
+                              "leg measured the return-map kernel at 262-268 us (elastic: after the tangent store at a "
+                              "yielding displacement, whose centre-copy bound the later updates keep) and 1,236-1,261 "
+                              "us (yielding), the gathers at about 10 us each")
     active = pb.pimy_state(False)[:, 26]
     res["pimy_yielding_points_frac"] = round(float(np.mean(active > 0)), 3)
     if side == 40:
